@@ -1,0 +1,126 @@
+/*
+ * fm_oracle.h -- FP64 CPU oracle for the FlexMesh quasi-static Newton hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the HIP path is
+ * compared against and the CPU baseline it is timed beside.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product library (atlas-flexmesh_amd/) never links or calls it.
+ *
+ * Conventions (same as the reference, FEM/fem_types.hpp and
+ * FEM/adaptive_fem_engine.hpp):
+ *   - DOF numbering is node*3 + d.
+ *   - Node coordinates are passed SoA: X[0..nn) = x, X[nn..2nn) = y, X[2nn..3nn) = z.
+ *   - Connectivity is SoA: conn[a*ne + e] = node a of element e (a = 0..3).
+ *   - 3x3 matrices are row-major, F[i*3+j] = F_ij.
+ *   - Element stiffness K_e[(a*3+i)*12 + (b*3+k)], as in the reference K_elem.
+ */
+#ifndef FM_ORACLE_H
+#define FM_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* threads used by the OpenMP loops (assembly, SpMV, dots); 1 = the exact serial restatement */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
+/* a2: structured box, node id k(nx+1)(ny+1) + j(nx+1) + i, 6 Kuhn tets per hex */
+void orc_box_mesh(int nx, int ny, int nz, double Lx, double Ly, double Lz, double* X, uint32_t* conn);
+
+/* a4 / a5 / a8 */
+void orc_inv3(const double A[9], double Ainv[9]);
+void orc_pk1_refform(const double F[9], double mu, double lam, double P[9]);
+void orc_pk1(const double F[9], double mu, double lam, double P[9]);
+double orc_energy(const double F[9], double mu, double lam);
+
+/* a9: element residual and exact tangent. Xe/ue are node-major [a*3+d]. returns 0, or 1 if skipped (vol<1e-30) */
+int orc_element(const double Xe[12], const double ue[12], double mu, double lam, double fe[12], double Ke[144]);
+
+/* a10: intended CSR rule. row_ptr has 3nn+1 entries; returns nnz */
+uint64_t orc_pattern_rows(uint32_t nn, uint32_t ne, const uint32_t* conn, uint32_t* row_ptr);
+void orc_pattern_cols(uint32_t nn, uint32_t ne, const uint32_t* conn, const uint32_t* row_ptr, uint32_t* col_idx);
+
+/* a13: greedy first-fit element coloring in element order; returns n_colors */
+uint32_t orc_color(uint32_t nn, uint32_t ne, const uint32_t* conn, uint32_t* colors);
+
+/* a11 / a12: colored-order assembly of f_int (and K when K != NULL) */
+void orc_assemble(uint32_t nn, uint32_t ne, const double* X, const uint32_t* conn,
+                  const double* mu, const double* lam, const uint32_t* colors, uint32_t n_colors,
+                  const double* u, const uint32_t* row_ptr, const uint32_t* col_idx,
+                  double* K, double* f_int);
+
+/* Dirichlet elimination: rows and columns of constrained DOFs zeroed, unit diagonal */
+void orc_eliminate(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, double* K,
+                   const uint8_t* is_dir);
+
+/* a15 */
+void orc_spmv(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const double* val,
+              const double* x, double* y);
+
+/* a14: natural-order ILU(0). LU holds L (strict lower, unit diagonal implied) and U (strict upper)
+   at A's positions, and u_ii at the diagonal positions; diag[i] = u_ii as well. */
+void orc_ilu0(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const double* val,
+              double* LU, double* diag);
+void orc_ilu0_apply(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const double* LU,
+                    const double* diag, const double* x, double* y);
+
+/* a16: returns iterations (max_iter = not converged, 0 = trivially converged) */
+int orc_pcg(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const double* val,
+            const double* b, double* x, double tol, int max_iter, double* relres_out);
+
+/* a17 / a18 -------------------------------------------------------------- */
+typedef struct {
+    uint32_t n_nodes, n_elems;
+    const double* X;           /* 3*nn SoA */
+    const uint32_t* conn;      /* 4*ne SoA */
+    const double* mu;          /* ne */
+    const double* lam;         /* ne */
+    uint32_t n_dir;
+    const uint32_t* dir_dofs;  /* n_dir */
+    const double* dir_vals;    /* value at load factor 1 */
+    const double* f_ext;       /* 3*nn nodal force at load factor 1, or NULL */
+    uint32_t n_steps;
+    double newton_rtol;
+    int max_newton;
+    double pcg_tol;
+    int pcg_max;
+    int ls_max;                /* Armijo backtracks (reference: 10) */
+    double armijo_c;           /* reference: 1e-4 */
+    int lift;                  /* 1: predictor solve with the prescribed increment */
+    const uint32_t* colors;    /* ne, from orc_color */
+    uint32_t n_colors;
+    const uint32_t* row_ptr;   /* 3nn+1 */
+    const uint32_t* col_idx;   /* nnz */
+} orc_problem;
+
+typedef struct {
+    int converged;
+    int newton_iters;
+    int pcg_iters;
+    int assemblies;
+    int residuals;
+    double r0;
+    double rel_res;
+    double t_assemble, t_residual, t_factor, t_pcg, t_total; /* seconds */
+    int factorizations;
+} orc_step_stats;
+
+double orc_armijo(const orc_problem* P, double lf, const double* u, const double* du, double R_norm,
+                  double* u_out, double* R_norm_out);
+int orc_newton_step(const orc_problem* P, uint32_t step, double* u, orc_step_stats* st);
+
+/* a20: FP64 rotation part */
+int orc_polar(const double F[9], double R[9]);
+void orc_rot_log(const double R[9], double w[3]);
+void orc_rot_exp(const double w[3], double R[9]);
+
+/* a22: adjoint at a converged state of the final load step (load factor 1) */
+int orc_adjoint(const orc_problem* P, const double* u, const double* dLdu, double tol, int max_iter,
+                double* dL_dmu, double* dL_dlam, double* dL_dload, double* lam_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,131 @@
+// ref_driver.cpp -- C entry points over the compiled reference, used only to
+// write golden vectors (tests/golden/make_golden.py) and to cross-check the
+// oracle in this container.  TEST INFRASTRUCTURE ONLY.
+//
+// Built by oracle/build_ref.sh from the reference sources where they lie;
+// the output goes to oracle/_ref/ and is never committed.  Every function
+// below calls the reference's own code unchanged.
+#include "fem/adaptive_fem_engine.hpp"
+#include "fem/constitutive_models.hpp"
+
+#include <cstring>
+
+using namespace atlas::fem;
+
+namespace {
+MeshTopology mesh_from(uint32_t nn, const double* X, uint32_t ne, const uint32_t* conn) {
+    MeshTopology m;
+    m.nodes.resize(nn);
+    for (uint32_t n = 0; n < nn; ++n) {
+        m.nodes[n].x = X[n];
+        m.nodes[n].y = X[nn + n];
+        m.nodes[n].z = X[2 * nn + n];
+        m.nodes[n].id = n;
+    }
+    m.elements.resize(ne);
+    m.states.resize(ne);
+    for (uint32_t e = 0; e < ne; ++e) {
+        for (int a = 0; a < 4; ++a) m.elements[e].nodes[a] = conn[a * (size_t)ne + e];
+        m.elements[e].id = e;
+        m.elements[e].volume = 1.0;
+    }
+    m.n_dofs = 3 * nn;
+    return m;
+}
+
+SparseCSR csr_from(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const double* val) {
+    SparseCSR A;
+    A.n_rows = n;
+    A.row_ptr.assign(row_ptr, row_ptr + n + 1);
+    A.col_idx.assign(col_idx, col_idx + row_ptr[n]);
+    A.val.assign(val, val + row_ptr[n]);
+    return A;
+}
+}  // namespace
+
+extern "C" {
+
+// NeoHookeanMaterial::compute_pk1_stress (FEM/constitutive_models.hpp:61-113)
+void ref_pk1(const double F[9], double E, double nu, double P[9]) {
+    NeoHookeanParameters p;
+    p.E = E;
+    p.nu = nu;
+    NeoHookeanMaterial m(p);
+    m.compute_pk1_stress(F, P);
+}
+
+// generate_unit_cube_mesh (FEM/adaptive_fem_engine.hpp:2219-2294)
+uint32_t ref_cube_mesh(int N, double* X, uint32_t* conn) {
+    MeshTopology m = generate_unit_cube_mesh(N);
+    const uint32_t nn = (uint32_t)m.n_nodes(), ne = (uint32_t)m.n_elements();
+    for (uint32_t n = 0; n < nn; ++n) {
+        X[n] = m.nodes[n].x;
+        X[nn + n] = m.nodes[n].y;
+        X[2 * nn + n] = m.nodes[n].z;
+    }
+    for (uint32_t e = 0; e < ne; ++e)
+        for (int a = 0; a < 4; ++a) conn[a * (size_t)ne + e] = m.elements[e].nodes[a];
+    return ne;
+}
+
+// build_csr_sparsity_pattern_rigorous (:1069-1120); returns nnz, fills row_ptr (n_dofs+1)
+// and col_idx (nnz, may be NULL)
+uint32_t ref_csr(uint32_t nn, const double* X, uint32_t ne, const uint32_t* conn, uint32_t* row_ptr,
+                 uint32_t* col_idx) {
+    MeshTopology m = mesh_from(nn, X, ne, conn);
+    SparseCSR K;
+    build_csr_sparsity_pattern_rigorous(m, K);
+    std::memcpy(row_ptr, K.row_ptr.data(), K.row_ptr.size() * sizeof(uint32_t));
+    if (col_idx) std::memcpy(col_idx, K.col_idx.data(), K.col_idx.size() * sizeof(uint32_t));
+    return K.row_ptr.back();
+}
+
+// MeshColoring::compute (:1549-1602)
+uint32_t ref_color(uint32_t nn, const double* X, uint32_t ne, const uint32_t* conn, uint32_t* colors) {
+    MeshTopology m = mesh_from(nn, X, ne, conn);
+    MeshColoring c = MeshColoring::compute(m);
+    std::memcpy(colors, c.element_colors.data(), ne * sizeof(uint32_t));
+    return c.n_colors;
+}
+
+// ILU0Preconditioner::factorize (:305-470)
+void ref_ilu0(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const double* val, double* L_val,
+              double* U_val, double* diag) {
+    SparseCSR A = csr_from(n, row_ptr, col_idx, val);
+    auto f = ILU0Preconditioner::factorize(A);
+    std::memcpy(L_val, f.L_val.data(), f.L_val.size() * sizeof(double));
+    std::memcpy(U_val, f.U_val.data(), f.U_val.size() * sizeof(double));
+    std::memcpy(diag, f.diag.data(), n * sizeof(double));
+}
+
+// ILU0Preconditioner::solve (:473-504) on a fresh factorization
+void ref_ilu0_solve(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const double* val, const double* x,
+                    double* y) {
+    SparseCSR A = csr_from(n, row_ptr, col_idx, val);
+    auto f = ILU0Preconditioner::factorize(A);
+    ILU0Preconditioner::solve(f, x, y, n);
+}
+
+// pcg_solve_ilu0 (:1283-1341)
+int ref_pcg(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const double* val, const double* b,
+            double* x, double tol, int max_iter) {
+    SparseCSR A = csr_from(n, row_ptr, col_idx, val);
+    return pcg_solve_ilu0(A, b, x, n, tol, max_iter);
+}
+
+// exponential_map / matrix_logarithm (CORE/lie_operator.cpp:94-275), column-major 3x3
+void ref_expm(const double A[9], double out[9]) {
+    atlas::Matrix3x3 M;
+    for (int i = 0; i < 9; ++i) M.data[i] = A[i];
+    atlas::Matrix3x3 R = atlas::exponential_map(M);
+    for (int i = 0; i < 9; ++i) out[i] = R.data[i];
+}
+
+void ref_logm(const double A[9], double out[9]) {
+    atlas::Matrix3x3 M;
+    for (int i = 0; i < 9; ++i) M.data[i] = A[i];
+    atlas::Matrix3x3 R = atlas::matrix_logarithm(M);
+    for (int i = 0; i < 9; ++i) out[i] = R.data[i];
+}
+
+}  // extern "C"
