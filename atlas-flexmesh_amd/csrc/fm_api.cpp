@@ -1,0 +1,1291 @@
+// fm_api.cpp -- C ABI and host drivers (PCG, Newton, adjoint) of the MI355X
+// quasi-static Newton path.  Host code orchestrates device-resident data; the
+// only host round trips inside a solve are the scalar checks (convergence,
+// line search), read through pinned memory.
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <utility>
+
+#include "fm_internal.hpp"
+
+using namespace fm;
+
+struct fm_ctx : public fm::Ctx {};
+
+namespace {
+
+constexpr int kBatch = 8;  // PCG iterations enqueued between host checks
+
+#define FM_HIP(call)                                                            \
+    do {                                                                        \
+        hipError_t e_ = (call);                                                 \
+        if (e_ != hipSuccess) {                                                 \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);       \
+            return FM_EHIP;                                                     \
+        }                                                                       \
+    } while (0)
+
+#define FM_TRY(expr)          \
+    do {                      \
+        int rc_ = (expr);     \
+        if (rc_ != FM_OK) return rc_; \
+    } while (0)
+
+int fail(fm_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// ---- profiling: HIP events around launches of one kernel class --------------
+void prof_flush(fm_ctx* ctx) {
+    if (ctx->ev_pending.empty()) return;
+    (void)hipEventSynchronize(ctx->ev_pending.back().second.second);
+    for (auto& pe : ctx->ev_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) ctx->ms[pe.first] += ms;
+        ctx->launches[pe.first] += 1;
+        ctx->ev_pool.push_back(pe.second.first);
+        ctx->ev_pool.push_back(pe.second.second);
+    }
+    ctx->ev_pending.clear();
+}
+
+struct ProfScope {
+    fm_ctx* ctx;
+    int k;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(fm_ctx* c, int kind) : ctx(c), k(kind) {
+        if (!ctx->prof) return;
+        if (ctx->ev_pool.size() < 2) {
+            if (ctx->ev_pending.size() >= 1024) prof_flush(ctx);
+            if (ctx->ev_pool.size() < 2) {
+                hipEvent_t e1, e2;
+                if (hipEventCreate(&e1) != hipSuccess || hipEventCreate(&e2) != hipSuccess) return;
+                ctx->ev_pool.push_back(e1);
+                ctx->ev_pool.push_back(e2);
+            }
+        }
+        a = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        b = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        (void)hipEventRecord(a, ctx->stream);
+    }
+    ~ProfScope() {
+        if (!a) return;
+        (void)hipEventRecord(b, ctx->stream);
+        ctx->ev_pending.push_back({k, {a, b}});
+    }
+};
+
+int check_launch(fm_ctx* ctx, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ctx, FM_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+    return FM_OK;
+}
+
+int sync(fm_ctx* ctx) {
+    FM_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->prof) prof_flush(ctx);
+    return FM_OK;
+}
+
+int check_device_error(fm_ctx* ctx) {
+    int h = 0;
+    FM_HIP(hipMemcpyAsync(&h, ctx->d_err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    FM_HIP(hipStreamSynchronize(ctx->stream));
+    if (h) {
+        FM_HIP(hipMemsetAsync(ctx->d_err.p, 0, sizeof(int), ctx->stream));
+        return fail(ctx, FM_EDEVICE, "a device-side wait exceeded its spin limit (dependence schedule broken)");
+    }
+    return FM_OK;
+}
+
+// ---- set-up ---------------------------------------------------------------------
+int ensure_pattern(fm_ctx* ctx);
+
+int ensure_colors(fm_ctx* ctx) {
+    if (ctx->have_colors) return FM_OK;
+    if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
+    FM_TRY(ensure_pattern(ctx));
+    const uint32_t ne = ctx->ne;
+    ctx->hcolors.assign(ne, 0);
+    const uint32_t nc = greedy_colors(ctx->nn, ne, ctx->hconn.data(), ctx->hcolors.data());
+    if (nc == 0 && ne > 0) return fail(ctx, FM_EINVAL, "mesh needs more than 256 colours");
+    ctx->n_colors = nc;
+    ctx->color_ptr.assign(nc + 1, 0);
+    for (uint32_t e = 0; e < ne; ++e) ctx->color_ptr[ctx->hcolors[e] + 1]++;
+    for (uint32_t c = 0; c < nc; ++c) ctx->color_ptr[c + 1] += ctx->color_ptr[c];
+    ctx->hperm.resize(ne);
+    {
+        std::vector<uint32_t> fill(ctx->color_ptr.begin(), ctx->color_ptr.end() - 1);
+        for (uint32_t e = 0; e < ne; ++e) ctx->hperm[fill[ctx->hcolors[e]]++] = e;
+    }
+    // colour-ordered SoA element arrays and the node-pair rank table
+    std::vector<uint32_t> connc((size_t)4 * ne);
+    std::vector<uint8_t> rank((size_t)16 * ne);
+    const auto& np = ctx->nbr_ptr;
+    const auto& nb = ctx->nbr;
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < (int64_t)ne; ++t) {
+        const uint32_t e = ctx->hperm[t];
+        uint32_t nd[4];
+        for (int a = 0; a < 4; ++a) nd[a] = connc[(size_t)a * ne + t] = ctx->hconn[(size_t)a * ne + e];
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t* lo = nb.data() + np[nd[a]];
+                const uint32_t* hi = nb.data() + np[nd[a] + 1];
+                rank[(size_t)(a * 4 + b) * ne + t] = (uint8_t)(std::lower_bound(lo, hi, nd[b]) - lo);
+            }
+    }
+    FM_HIP(ctx->conn_c.alloc((size_t)4 * ne));
+    FM_HIP(ctx->eperm.alloc(ne));
+    FM_HIP(ctx->erank_c.alloc((size_t)16 * ne));
+    FM_HIP(hipMemcpy(ctx->conn_c.p, connc.data(), connc.size() * 4, hipMemcpyHostToDevice));
+    FM_HIP(hipMemcpy(ctx->eperm.p, ctx->hperm.data(), (size_t)ne * 4, hipMemcpyHostToDevice));
+    FM_HIP(hipMemcpy(ctx->erank_c.p, rank.data(), rank.size(), hipMemcpyHostToDevice));
+    ctx->have_colors = true;
+    return FM_OK;
+}
+
+int upload_material(fm_ctx* ctx) {
+    const uint32_t ne = ctx->ne;
+    std::vector<double> mc(ne), lc(ne);
+    for (uint32_t t = 0; t < ne; ++t) {
+        mc[t] = ctx->hmu[ctx->hperm[t]];
+        lc[t] = ctx->hlam[ctx->hperm[t]];
+    }
+    FM_HIP(ctx->mu_c.alloc(ne));
+    FM_HIP(ctx->lam_c.alloc(ne));
+    FM_HIP(hipMemcpy(ctx->mu_c.p, mc.data(), (size_t)ne * 8, hipMemcpyHostToDevice));
+    FM_HIP(hipMemcpy(ctx->lam_c.p, lc.data(), (size_t)ne * 8, hipMemcpyHostToDevice));
+    ctx->have_material = true;
+    return FM_OK;
+}
+
+int ensure_pattern(fm_ctx* ctx) {
+    if (ctx->have_pattern) return FM_OK;
+    if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
+    node_adjacency(ctx->nn, ctx->ne, ctx->hconn.data(), ctx->nbr_ptr, ctx->nbr);
+    for (uint32_t n = 0; n < ctx->nn; ++n)
+        if (ctx->nbr_ptr[n + 1] - ctx->nbr_ptr[n] > 85)
+            return fail(ctx, FM_EINVAL, "a node has more than 85 neighbours (row length above 256)");
+    Matrix& M = ctx->tangent;
+    const uint32_t n = 3 * ctx->nn;
+    M.n = n;
+    M.h_row_ptr.assign(n + 1, 0);
+    uint64_t acc = 0;
+    for (uint32_t v = 0; v < ctx->nn; ++v) {
+        const uint32_t deg = ctx->nbr_ptr[v + 1] - ctx->nbr_ptr[v];
+        for (int d = 0; d < 3; ++d) {
+            acc += 3ull * deg;
+            if (acc > 0xFFFFFFFFull) return fail(ctx, FM_EINVAL, "nnz exceeds uint32 CSR indexing");
+            M.h_row_ptr[3 * v + d + 1] = (uint32_t)acc;
+        }
+    }
+    M.nnz = acc;
+    FM_HIP(M.row_ptr.alloc(n + 1));
+    FM_HIP(M.col_idx.alloc(M.nnz));
+    FM_HIP(hipMemcpy(M.row_ptr.p, M.h_row_ptr.data(), (size_t)(n + 1) * 4, hipMemcpyHostToDevice));
+    {
+        DevBuf<uint32_t> dptr, dnbr;
+        FM_HIP(dptr.alloc(ctx->nn + 1));
+        FM_HIP(dnbr.alloc(ctx->nbr.size()));
+        FM_HIP(hipMemcpy(dptr.p, ctx->nbr_ptr.data(), (size_t)(ctx->nn + 1) * 4, hipMemcpyHostToDevice));
+        FM_HIP(hipMemcpy(dnbr.p, ctx->nbr.data(), ctx->nbr.size() * 4, hipMemcpyHostToDevice));
+        launch_expand_pattern(ctx->stream, ctx->nn, dptr.p, dnbr.p, M.row_ptr.p, M.col_idx.p);
+        FM_TRY(check_launch(ctx, "expand_pattern"));
+        FM_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    // level schedule: build_schedule reads rows 3v only, so a compact view
+    // holding row 3v's columns (and empty rows 3v+1, 3v+2) is enough
+    {
+        std::vector<uint32_t> rp3(n + 1), ci3;
+        ci3.reserve(3 * ctx->nbr.size());
+        for (uint32_t v = 0; v < ctx->nn; ++v) {
+            rp3[3 * v] = (uint32_t)ci3.size();
+            for (uint32_t q = ctx->nbr_ptr[v]; q < ctx->nbr_ptr[v + 1]; ++q)
+                for (int e = 0; e < 3; ++e) ci3.push_back(3 * ctx->nbr[q] + e);
+            rp3[3 * v + 1] = rp3[3 * v + 2] = (uint32_t)ci3.size();
+        }
+        rp3[n] = (uint32_t)ci3.size();
+        std::string err;
+        if (!build_schedule(rp3, ci3, 3, M.sched, ctx->stream, err)) return fail(ctx, FM_EINVAL, err);
+    }
+    M.val_valid[0] = M.val_valid[1] = false;
+    M.lu_valid[0] = M.lu_valid[1] = false;
+    M.ready = true;
+    ctx->have_pattern = true;
+    return FM_OK;
+}
+
+ElemArgs elem_args(fm_ctx* ctx, const double* u) {
+    ElemArgs a;
+    a.X = ctx->X.p;
+    a.nn = ctx->nn;
+    a.conn = ctx->conn_c.p;
+    a.ne = ctx->ne;
+    a.mu = ctx->mu_c.p;
+    a.lam = ctx->lam_c.p;
+    a.erank = ctx->erank_c.p;
+    a.u = u;
+    return a;
+}
+
+int ensure_vectors(fm_ctx* ctx) {
+    const size_t n = 3ull * ctx->nn;
+    for (auto* b : {&ctx->u, &ctx->f, &ctx->R, &ctx->du, &ctx->ut, &ctx->dD, &ctx->lift, &ctx->hv}) FM_HIP(b->alloc(n));
+    FM_HIP(ctx->isdir.alloc(n));
+    return FM_OK;
+}
+
+int ensure_ready(fm_ctx* ctx) {
+    if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh (fm_set_mesh)");
+    FM_TRY(ensure_pattern(ctx));
+    FM_TRY(ensure_colors(ctx));
+    if (!ctx->have_material) return fail(ctx, FM_ESTATE, "no material (fm_set_material)");
+    FM_TRY(ensure_vectors(ctx));
+    return FM_OK;
+}
+
+int ensure_partials(fm_ctx* ctx, size_t n) {
+    if (ctx->partials.n < n) FM_HIP(ctx->partials.alloc(std::max<size_t>(n, 1024)));
+    return FM_OK;
+}
+
+// f = f_int(u) (and the tangent with BC masking/lift when with_K)
+int assemble(fm_ctx* ctx, const double* u, bool with_K, int prec, bool mask, const double* dD, int prof_kind) {
+    Matrix& M = ctx->tangent;
+    hipStream_t s = ctx->stream;
+    launch_fill_f64(s, ctx->f.p, ctx->f.n, 0.0);
+    void* K = nullptr;
+    if (with_K) {
+        if (prec == PREC64) {
+            FM_HIP(M.val64.alloc(M.nnz));
+            launch_fill_f64(s, M.val64.p, M.nnz, 0.0);
+            K = M.val64.p;
+        } else {
+            FM_HIP(M.val32.alloc(M.nnz));
+            launch_fill_f32(s, M.val32.p, M.nnz, 0.f);
+            K = M.val32.p;
+        }
+        if (dD) launch_fill_f64(s, ctx->lift.p, ctx->lift.n, 0.0);
+    }
+    ElemArgs a = elem_args(ctx, u);
+    const uint8_t* isdir = (mask && !ctx->dir_dofs.empty()) ? ctx->isdir.p : nullptr;
+    for (uint32_t c = 0; c < ctx->n_colors; ++c) {
+        ProfScope ps(ctx, prof_kind);
+        launch_assemble_color(s, prec, with_K, a, ctx->color_ptr[c], ctx->color_ptr[c + 1], ctx->f.p, K,
+                              M.row_ptr.p, isdir, dD, ctx->lift.p);
+    }
+    FM_TRY(check_launch(ctx, "assemble"));
+    if (with_K) {
+        if (isdir)
+            launch_set_dir_diag(s, prec, (uint32_t)ctx->dir_dofs.size(), ctx->d_dir_dofs.p, M.row_ptr.p,
+                                M.sched.nlow.p, K);
+        M.val_valid[prec] = true;
+        M.val_valid[1 - prec] = false;
+        M.lu_valid[0] = M.lu_valid[1] = false;
+        ctx->tangent_elim = isdir != nullptr;
+    }
+    return FM_OK;
+}
+
+// R = load f_ext - f_int(u) (constrained rows zeroed); returns ||R|| in *norm
+int residual(fm_ctx* ctx, const double* u, double load, double* norm) {
+    FM_TRY(assemble(ctx, u, false, PREC64, false, nullptr, FM_K_RESIDUAL));
+    const uint32_t n = 3 * ctx->nn;
+    FM_TRY(ensure_partials(ctx, n / 256 + 1));
+    const uint32_t np = launch_residual_finish(ctx->stream, n, load, ctx->have_fext ? ctx->f_ext.p : nullptr,
+                                               ctx->f.p, ctx->dir_dofs.empty() ? nullptr : ctx->isdir.p, ctx->R.p,
+                                               ctx->partials.p);
+    launch_reduce(ctx->stream, ctx->partials.p, np, ctx->st.p, 0, 0);
+    FM_HIP(hipMemcpyAsync(ctx->h_st, ctx->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
+    FM_TRY(sync(ctx));
+    *norm = std::sqrt(ctx->h_st->slot[0]);
+    return FM_OK;
+}
+
+// ---- PCG ------------------------------------------------------------------------
+template <typename T>
+struct PcgVecs {
+    T *b, *x, *r, *p, *Ap, *z, *y;
+};
+
+int pcg_vectors(fm_ctx* ctx, int prec, uint32_t n) {
+    if (prec == PREC64) {
+        for (auto* v : {&ctx->b64, &ctx->x64, &ctx->r64, &ctx->p64, &ctx->Ap64, &ctx->z64, &ctx->y64})
+            FM_HIP(v->alloc(n));
+    } else {
+        for (auto* v : {&ctx->b32, &ctx->x32, &ctx->r32, &ctx->p32, &ctx->Ap32, &ctx->z32, &ctx->y32})
+            FM_HIP(v->alloc(n));
+    }
+    return FM_OK;
+}
+
+void* vb(fm_ctx* c, int prec, int which) {
+    if (prec == PREC64) {
+        DevBuf<double>* v[] = {&c->b64, &c->x64, &c->r64, &c->p64, &c->Ap64, &c->z64, &c->y64};
+        return v[which]->p;
+    }
+    DevBuf<float>* v[] = {&c->b32, &c->x32, &c->r32, &c->p32, &c->Ap32, &c->z32, &c->y32};
+    return v[which]->p;
+}
+enum { VB = 0, VX, VR, VP, VAP, VZ, VY };
+
+int factorize(fm_ctx* ctx, Matrix& A, int prec) {
+    if (A.lu_valid[prec]) return FM_OK;
+    if (!A.val_valid[prec]) return fail(ctx, FM_ESTATE, "matrix values not available in this precision");
+    const void* val;
+    void* lu;
+    if (prec == PREC64) {
+        FM_HIP(A.lu64.alloc(A.nnz));
+        val = A.val64.p;
+        lu = A.lu64.p;
+    } else {
+        FM_HIP(A.lu32.alloc(A.nnz));
+        val = A.val32.p;
+        lu = A.lu32.p;
+    }
+    // ticket[2] of the state is the factorization queue
+    PcgState init{};
+    *ctx->h_st = init;
+    FM_HIP(hipMemcpyAsync(ctx->st.p, ctx->h_st, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
+    {
+        ProfScope ps(ctx, FM_K_ILU_FACTOR);
+        launch_ilu_factor(ctx->stream, prec, A, val, lu, ctx->st.p, ctx->d_err.p);
+    }
+    FM_TRY(check_launch(ctx, "ilu_factor"));
+    FM_TRY(check_device_error(ctx));
+    A.lu_valid[prec] = true;
+    return FM_OK;
+}
+
+// M^{-1} applied to vector r -> y (vectors of the PCG work set)
+int precond_apply(fm_ctx* ctx, Matrix& A, int prec, const void* r, void* z, void* y, double* partials) {
+    const void* lu = prec == PREC64 ? (const void*)A.lu64.p : (const void*)A.lu32.p;
+    launch_fill_sentinel(ctx->stream, prec, z, y, A.n);
+    {
+        ProfScope ps(ctx, FM_K_ILU_FWD);
+        launch_ilu_fwd(ctx->stream, prec, A, lu, r, nullptr, nullptr, nullptr, nullptr, z, ctx->st.p, false,
+                       ctx->d_err.p);
+    }
+    uint32_t np;
+    {
+        ProfScope ps(ctx, FM_K_ILU_BWD);
+        np = launch_ilu_bwd(ctx->stream, prec, A, lu, z, r, y, partials, ctx->st.p, false, ctx->d_err.p);
+    }
+    return (int)np;
+}
+
+struct PcgResult {
+    int iters;
+    double relres;
+    int status;
+};
+
+// pcg_solve_ilu0 (FEM/adaptive_fem_engine.hpp:1283-1341) on device vectors
+// b -> x of the work set (x holds the initial guess).
+int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult* res) {
+    hipStream_t s = ctx->stream;
+    const uint32_t n = A.n;
+    FM_TRY(factorize(ctx, A, prec));
+    const uint32_t nbw = A.sched.q_bwd + 64;
+    FM_TRY(ensure_partials(ctx, std::max<size_t>({(size_t)spmv_blocks(A), (size_t)n / 256 + 1, (size_t)nbw})));
+    void* b = vb(ctx, prec, VB);
+    void* x = vb(ctx, prec, VX);
+    void* r = vb(ctx, prec, VR);
+    void* p = vb(ctx, prec, VP);
+    void* Ap = vb(ctx, prec, VAP);
+    void* z = vb(ctx, prec, VZ);
+    void* y = vb(ctx, prec, VY);
+    PcgState init{};
+    init.tol = tol;
+    init.max_it = max_it;
+    *ctx->h_st = init;
+    FM_HIP(hipMemcpyAsync(ctx->st.p, ctx->h_st, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
+    // r = b - A x; ||r||^2 and b.b
+    {
+        ProfScope ps(ctx, FM_K_SPMV);
+        launch_spmv(s, prec, A, x, Ap, false, nullptr, nullptr, nullptr, nullptr);
+    }
+    uint32_t np = launch_pcg_init_r(s, prec, n, b, Ap, r, ctx->partials.p);
+    launch_reduce(s, ctx->partials.p, np, ctx->st.p, 0, 0);
+    // b.b through the same kernel with a zero "Ap" (z is scratch here)
+    if (prec == PREC64)
+        launch_fill_f64(s, (double*)Ap, n, 0.0);
+    else
+        launch_fill_f32(s, (float*)Ap, n, 0.f);
+    np = launch_pcg_init_r(s, prec, n, b, Ap, z, ctx->partials.p);
+    launch_reduce(s, ctx->partials.p, np, ctx->st.p, 0, 1);
+    FM_HIP(hipMemcpyAsync(ctx->h_st, ctx->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    FM_TRY(sync(ctx));
+    const double rr = ctx->h_st->slot[0], bb = ctx->h_st->slot[1];
+    if (std::sqrt(rr) < tol * 1e-3) {
+        res->iters = 0;
+        res->relres = std::sqrt(rr);
+        res->status = 1;
+        return FM_OK;
+    }
+    ctx->h_st->bb = bb;
+    ctx->h_st->tol = tol;
+    ctx->h_st->max_it = max_it;
+    FM_HIP(hipMemcpyAsync(ctx->st.p, ctx->h_st, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    // z = M^{-1} r; rz; p = z
+    np = (uint32_t)precond_apply(ctx, A, prec, r, z, y, ctx->partials.p);
+    launch_reduce(s, ctx->partials.p, np, ctx->st.p, 3, 0);
+    launch_copy(s, prec, n, y, p);
+    FM_TRY(check_launch(ctx, "pcg init"));
+    const void* lu = prec == PREC64 ? (const void*)A.lu64.p : (const void*)A.lu32.p;
+    int done = 0;
+    int enq = 0;
+    while (!done) {
+        for (int k = 0; k < kBatch && enq < max_it; ++k, ++enq) {
+            {
+                ProfScope ps(ctx, FM_K_SPMV);
+                launch_spmv(s, prec, A, p, Ap, true, ctx->partials.p, z, y, ctx->st.p);
+            }
+            launch_reduce(s, ctx->partials.p, spmv_blocks(A), ctx->st.p, 1, 0);
+            {
+                ProfScope ps(ctx, FM_K_ILU_FWD);
+                launch_ilu_fwd(s, prec, A, lu, nullptr, r, x, p, Ap, z, ctx->st.p, true, ctx->d_err.p);
+            }
+            uint32_t nbb;
+            {
+                ProfScope ps(ctx, FM_K_ILU_BWD);
+                nbb = launch_ilu_bwd(s, prec, A, lu, z, r, y, ctx->partials.p, ctx->st.p, true, ctx->d_err.p);
+            }
+            launch_reduce(s, ctx->partials.p, nbb, ctx->st.p, 2, 0);
+            launch_pcg_p_update(s, prec, n, y, p, ctx->st.p);
+        }
+        FM_TRY(check_launch(ctx, "pcg iteration"));
+        FM_HIP(hipMemcpyAsync(ctx->h_st, ctx->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+        FM_TRY(sync(ctx));
+        done = ctx->h_st->done || enq >= max_it;
+    }
+    FM_TRY(check_device_error(ctx));
+    const PcgState& h = *ctx->h_st;
+    res->status = h.status;
+    res->relres = h.relres;
+    res->iters = (h.status == 1) ? h.iters : max_it;
+    return FM_OK;
+}
+
+// Solve K du = rhs (FP64 device vector) with the tangent; du FP64.
+int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double tol, int max_it, int* iters) {
+    Matrix& A = ctx->tangent;
+    ctx->cur = &A;
+    const uint32_t n = A.n;
+    FM_TRY(pcg_vectors(ctx, prec, n));
+    PcgResult pr{};
+    if (prec == PREC64) {
+        FM_HIP(hipMemcpyAsync(ctx->b64.p, rhs, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        launch_fill_f64(ctx->stream, ctx->x64.p, n, 0.0);
+        FM_TRY(pcg_run(ctx, A, PREC64, tol, max_it, &pr));
+        FM_HIP(hipMemcpyAsync(du, ctx->x64.p, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+        // FP32 inner solve on the normalised right-hand side
+        FM_TRY(ensure_partials(ctx, n / 256 + 1));
+        const uint32_t np = launch_dot64(ctx->stream, n, rhs, rhs, ctx->partials.p);
+        launch_reduce(ctx->stream, ctx->partials.p, np, ctx->st.p, 0, 2);
+        FM_HIP(hipMemcpyAsync(ctx->h_st, ctx->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
+        FM_TRY(sync(ctx));
+        const double nr = std::sqrt(ctx->h_st->slot[2]);
+        if (nr == 0.0) {
+            launch_fill_f64(ctx->stream, du, n, 0.0);
+            *iters = 0;
+            return FM_OK;
+        }
+        launch_f64_to_f32(ctx->stream, n, rhs, 1.0 / nr, ctx->b32.p);
+        launch_fill_f32(ctx->stream, ctx->x32.p, n, 0.f);
+        FM_TRY(pcg_run(ctx, A, PREC32, tol, max_it, &pr));
+        launch_f32_to_f64(ctx->stream, n, ctx->x32.p, nr, du);
+    }
+    *iters = pr.iters;
+    return FM_OK;
+}
+
+void fnv1a_update(uint64_t& h, const void* data, size_t n) {
+    const uint8_t* p = (const uint8_t*)data;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+}
+
+// counter-based generator for the synthetic inputs (splitmix64 finaliser)
+inline uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+inline double uniform01(uint64_t seed, uint64_t stream, uint64_t i) {
+    return (double)(mix64(mix64(seed ^ (stream * 0xD1B54A32D192ED03ull)) + i) >> 11) * 0x1.0p-53;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* fm_version(void) { return "flexmesh-mi355x 0.1 (gfx950)"; }
+
+int fm_create(int device, fm_ctx** out) {
+    if (!out) return FM_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return FM_EHIP;
+    if (device < 0 || device >= ndev) return FM_EINVAL;
+    fm_ctx* ctx = new fm_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        ctx->st.alloc(1) != hipSuccess || ctx->d_err.alloc(1) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_st, sizeof(PcgState), hipHostMallocDefault) != hipSuccess) {
+        delete ctx;
+        return FM_EHIP;
+    }
+    int zero = 0;
+    (void)hipMemcpy(ctx->d_err.p, &zero, sizeof(int), hipMemcpyHostToDevice);
+    *out = ctx;
+    return FM_OK;
+}
+
+int fm_destroy(fm_ctx* ctx) {
+    if (!ctx) return FM_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    prof_flush(ctx);
+    for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+    if (ctx->h_st) (void)hipHostFree(ctx->h_st);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return FM_OK;
+}
+
+const char* fm_last_error(const fm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int fm_mesh_box(int nx, int ny, int nz, double Lx, double Ly, double Lz, double* X, uint32_t* conn) {
+    if (nx < 1 || ny < 1 || nz < 1 || !X || !conn) return FM_EINVAL;
+    // Kuhn split (all six tets share the 0-6 diagonal of the hex, corner
+    // numbering of the reference hex_off table, FEM/adaptive_fem_engine.hpp:2245-2248);
+    // vertex order gives positive orientation on the undistorted grid
+    static const int kKuhn[6][4] = {{0, 1, 2, 6}, {0, 5, 1, 6}, {0, 2, 3, 6}, {0, 3, 7, 6}, {0, 4, 5, 6}, {0, 7, 4, 6}};
+    static const int kOff[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    const uint64_t NX = nx + 1, NY = ny + 1, NZ = nz + 1;
+    const uint64_t nn = NX * NY * NZ, ne = 6ull * nx * ny * nz;
+    if (nn > 0xFFFFFFFFull || ne > 0xFFFFFFFFull) return FM_EINVAL;
+    const double hx = Lx / nx, hy = Ly / ny, hz = Lz / nz;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < (int64_t)NZ; ++k)
+        for (uint64_t j = 0; j < NY; ++j)
+            for (uint64_t i = 0; i < NX; ++i) {
+                const uint64_t v = k * NX * NY + j * NX + i;
+                X[v] = i * hx;
+                X[nn + v] = j * hy;
+                X[2 * nn + v] = k * hz;
+            }
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const uint64_t e0 = 6ull * (((uint64_t)k * ny + j) * nx + i);
+                uint32_t hex[8];
+                for (int c = 0; c < 8; ++c)
+                    hex[c] = (uint32_t)((k + kOff[c][2]) * NX * NY + (j + kOff[c][1]) * NX + (i + kOff[c][0]));
+                for (int t = 0; t < 6; ++t)
+                    for (int a = 0; a < 4; ++a) conn[a * ne + e0 + t] = hex[kKuhn[t][a]];
+            }
+    return FM_OK;
+}
+
+int fm_mesh_jitter(uint32_t n_nodes, double* X, const uint8_t* is_interior, double amp, uint64_t seed) {
+    if (!X || !is_interior) return FM_EINVAL;
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < (int64_t)n_nodes; ++v) {
+        if (!is_interior[v]) continue;
+        for (int d = 0; d < 3; ++d) X[(size_t)d * n_nodes + v] += amp * (2.0 * uniform01(seed, 1 + d, v) - 1.0);
+    }
+    return FM_OK;
+}
+
+int fm_field_lognormal(uint32_t n, double median, double sigma, uint64_t seed, double* out) {
+    if (!out) return FM_EINVAL;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < (int64_t)n; ++e) {
+        const double u1 = 1.0 - uniform01(seed, 11, e);  // (0, 1]
+        const double u2 = uniform01(seed, 12, e);
+        const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+        out[e] = median * std::exp(sigma * z);
+    }
+    return FM_OK;
+}
+
+int fm_set_mesh(fm_ctx* ctx, uint32_t n_nodes, const double* X, uint32_t n_elems, const uint32_t* conn) {
+    if (!ctx || !X || !conn || !n_nodes || !n_elems) return fail(ctx, FM_EINVAL, "fm_set_mesh: bad arguments");
+    if (3ull * n_nodes > 0xFFFFFFFFull) return fail(ctx, FM_EINVAL, "too many nodes");
+    for (size_t k = 0; k < 4ull * n_elems; ++k)
+        if (conn[k] >= n_nodes) return fail(ctx, FM_EINVAL, "connectivity refers to a node out of range");
+    FM_HIP(hipSetDevice(ctx->device));
+    ctx->nn = n_nodes;
+    ctx->ne = n_elems;
+    ctx->hX.assign(X, X + 3ull * n_nodes);
+    ctx->hconn.assign(conn, conn + 4ull * n_elems);
+    ctx->have_pattern = ctx->have_colors = ctx->have_material = false;
+    ctx->hmu.clear();
+    ctx->hlam.clear();
+    ctx->u_valid = false;
+    ctx->dir_dofs.clear();
+    ctx->dir_vals.clear();
+    ctx->have_fext = false;
+    FM_HIP(ctx->X.alloc(3ull * n_nodes));
+    FM_HIP(hipMemcpy(ctx->X.p, X, 3ull * n_nodes * 8, hipMemcpyHostToDevice));
+    FM_TRY(ensure_vectors(ctx));
+    launch_fill_u8(ctx->stream, ctx->isdir.p, ctx->isdir.n, 0);
+    FM_HIP(hipStreamSynchronize(ctx->stream));
+    return FM_OK;
+}
+
+int fm_set_material(fm_ctx* ctx, const double* mu_e, const double* lam_e) {
+    if (!ctx || !mu_e || !lam_e) return fail(ctx, FM_EINVAL, "fm_set_material: bad arguments");
+    if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
+    ctx->hmu.assign(mu_e, mu_e + ctx->ne);
+    ctx->hlam.assign(lam_e, lam_e + ctx->ne);
+    FM_TRY(ensure_colors(ctx));
+    return upload_material(ctx);
+}
+
+int fm_set_bcs(fm_ctx* ctx, uint32_t n_dir, const uint32_t* dofs, const double* vals) {
+    if (!ctx || (n_dir && !dofs)) return fail(ctx, FM_EINVAL, "fm_set_bcs: bad arguments");
+    if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
+    const uint32_t n = 3 * ctx->nn;
+    std::vector<uint8_t> mask(n, 0);
+    for (uint32_t k = 0; k < n_dir; ++k) {
+        if (dofs[k] >= n) return fail(ctx, FM_EINVAL, "Dirichlet DOF out of range");
+        if (mask[dofs[k]]) return fail(ctx, FM_EINVAL, "duplicate Dirichlet DOF");
+        mask[dofs[k]] = 1;
+    }
+    ctx->dir_dofs.assign(dofs, dofs + n_dir);
+    ctx->dir_vals.assign(n_dir, 0.0);
+    if (vals) std::copy(vals, vals + n_dir, ctx->dir_vals.begin());
+    FM_HIP(hipMemcpy(ctx->isdir.p, mask.data(), n, hipMemcpyHostToDevice));
+    FM_HIP(ctx->d_dir_dofs.alloc(std::max<uint32_t>(n_dir, 1)));
+    FM_HIP(ctx->d_dir_vals.alloc(std::max<uint32_t>(n_dir, 1)));
+    if (n_dir) {
+        FM_HIP(hipMemcpy(ctx->d_dir_dofs.p, dofs, n_dir * 4ull, hipMemcpyHostToDevice));
+        FM_HIP(hipMemcpy(ctx->d_dir_vals.p, ctx->dir_vals.data(), n_dir * 8ull, hipMemcpyHostToDevice));
+    }
+    return FM_OK;
+}
+
+int fm_set_load(fm_ctx* ctx, const double* f_ext) {
+    if (!ctx) return FM_EINVAL;
+    if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
+    if (!f_ext) {
+        ctx->have_fext = false;
+        return FM_OK;
+    }
+    FM_HIP(ctx->f_ext.alloc(3ull * ctx->nn));
+    FM_HIP(hipMemcpy(ctx->f_ext.p, f_ext, 3ull * ctx->nn * 8, hipMemcpyHostToDevice));
+    ctx->have_fext = true;
+    return FM_OK;
+}
+
+int fm_build_pattern(fm_ctx* ctx, uint32_t* row_ptr_out, uint32_t* col_idx_out, uint64_t* nnz_out) {
+    if (!ctx) return FM_EINVAL;
+    FM_TRY(ensure_pattern(ctx));
+    Matrix& M = ctx->tangent;
+    if (nnz_out) *nnz_out = M.nnz;
+    if (row_ptr_out) std::copy(M.h_row_ptr.begin(), M.h_row_ptr.end(), row_ptr_out);
+    if (col_idx_out) FM_HIP(hipMemcpy(col_idx_out, M.col_idx.p, M.nnz * 4, hipMemcpyDeviceToHost));
+    return FM_OK;
+}
+
+int fm_color(fm_ctx* ctx, uint32_t* colors_out, uint32_t* n_colors_out) {
+    if (!ctx) return FM_EINVAL;
+    FM_TRY(ensure_colors(ctx));
+    if (colors_out) std::copy(ctx->hcolors.begin(), ctx->hcolors.end(), colors_out);
+    if (n_colors_out) *n_colors_out = ctx->n_colors;
+    return FM_OK;
+}
+
+int fm_assemble(fm_ctx* ctx, const double* u, double* f_int_out, double* K_vals_out, int prec) {
+    if (!ctx || (prec != FM_FP64 && prec != FM_MIXED)) return fail(ctx, FM_EINVAL, "fm_assemble: bad arguments");
+    FM_TRY(ensure_ready(ctx));
+    const size_t n = 3ull * ctx->nn;
+    const double* du = nullptr;
+    if (u) {
+        FM_HIP(hipMemcpy(ctx->hv.p, u, n * 8, hipMemcpyHostToDevice));
+        du = ctx->hv.p;
+    }
+    const int p = prec == FM_FP64 ? PREC64 : PREC32;
+    FM_TRY(assemble(ctx, du, true, p, false, nullptr, FM_K_ASSEMBLE));
+    FM_TRY(sync(ctx));
+    if (f_int_out) FM_HIP(hipMemcpy(f_int_out, ctx->f.p, n * 8, hipMemcpyDeviceToHost));
+    if (K_vals_out) {
+        Matrix& M = ctx->tangent;
+        if (p == PREC64) {
+            FM_HIP(hipMemcpy(K_vals_out, M.val64.p, M.nnz * 8, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> tmp(M.nnz);
+            FM_HIP(hipMemcpy(tmp.data(), M.val32.p, M.nnz * 4, hipMemcpyDeviceToHost));
+            std::copy(tmp.begin(), tmp.end(), K_vals_out);
+        }
+    }
+    ctx->cur = &ctx->tangent;
+    return FM_OK;
+}
+
+int fm_residual(fm_ctx* ctx, const double* u, double load, double* R_out, double* norm_out) {
+    if (!ctx) return FM_EINVAL;
+    FM_TRY(ensure_ready(ctx));
+    const size_t n = 3ull * ctx->nn;
+    const double* du = nullptr;
+    if (u) {
+        FM_HIP(hipMemcpy(ctx->hv.p, u, n * 8, hipMemcpyHostToDevice));
+        du = ctx->hv.p;
+    }
+    double nr = 0.0;
+    FM_TRY(residual(ctx, du, load, &nr));
+    if (R_out) FM_HIP(hipMemcpy(R_out, ctx->R.p, n * 8, hipMemcpyDeviceToHost));
+    if (norm_out) *norm_out = nr;
+    return FM_OK;
+}
+
+int fm_set_matrix(fm_ctx* ctx, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const double* vals) {
+    if (!ctx || !n || !row_ptr || !col_idx || !vals) return fail(ctx, FM_EINVAL, "fm_set_matrix: bad arguments");
+    FM_HIP(hipSetDevice(ctx->device));
+    Matrix& M = ctx->ext;
+    const uint64_t nnz = row_ptr[n];
+    M.n = n;
+    M.nnz = nnz;
+    M.h_row_ptr.assign(row_ptr, row_ptr + n + 1);
+    M.h_col_idx.resize(nnz);
+    M.perm.resize(nnz);
+    std::vector<double> sv(nnz);
+    for (uint32_t r = 0; r < n; ++r) {
+        if (row_ptr[r + 1] < row_ptr[r]) return fail(ctx, FM_EINVAL, "row_ptr not monotone");
+        const uint32_t b = row_ptr[r], e = row_ptr[r + 1];
+        std::iota(M.perm.begin() + b, M.perm.begin() + e, b);
+        std::sort(M.perm.begin() + b, M.perm.begin() + e, [&](uint32_t i, uint32_t j) { return col_idx[i] < col_idx[j]; });
+        for (uint32_t k = b; k < e; ++k) {
+            M.h_col_idx[k] = col_idx[M.perm[k]];
+            if (M.h_col_idx[k] >= n) return fail(ctx, FM_EINVAL, "column index out of range");
+            if (k > b && M.h_col_idx[k] == M.h_col_idx[k - 1]) return fail(ctx, FM_EINVAL, "duplicate column");
+            sv[k] = vals[M.perm[k]];
+        }
+    }
+    const int B = detect_block(M.h_row_ptr, M.h_col_idx);
+    std::string err;
+    if (!build_schedule(M.h_row_ptr, M.h_col_idx, B, M.sched, ctx->stream, err)) return fail(ctx, FM_EINVAL, err);
+    FM_HIP(M.row_ptr.alloc(n + 1));
+    FM_HIP(M.col_idx.alloc(nnz));
+    FM_HIP(M.val64.alloc(nnz));
+    FM_HIP(M.val32.alloc(nnz));
+    FM_HIP(hipMemcpy(M.row_ptr.p, M.h_row_ptr.data(), (n + 1) * 4ull, hipMemcpyHostToDevice));
+    FM_HIP(hipMemcpy(M.col_idx.p, M.h_col_idx.data(), nnz * 4, hipMemcpyHostToDevice));
+    FM_HIP(hipMemcpy(M.val64.p, sv.data(), nnz * 8, hipMemcpyHostToDevice));
+    std::vector<float> sf(sv.begin(), sv.end());
+    FM_HIP(hipMemcpy(M.val32.p, sf.data(), nnz * 4, hipMemcpyHostToDevice));
+    M.val_valid[0] = M.val_valid[1] = true;
+    M.lu_valid[0] = M.lu_valid[1] = false;
+    M.ready = true;
+    ctx->cur = &M;
+    return FM_OK;
+}
+
+int fm_use_tangent(fm_ctx* ctx, int eliminate) {
+    if (!ctx) return FM_EINVAL;
+    FM_TRY(ensure_pattern(ctx));
+    Matrix& M = ctx->tangent;
+    if (!M.val_valid[0] && !M.val_valid[1]) return fail(ctx, FM_ESTATE, "no assembled tangent (fm_assemble)");
+    if (eliminate && !ctx->tangent_elim && !ctx->dir_dofs.empty()) {
+        for (int p = 0; p < 2; ++p)
+            if (M.val_valid[p])
+                launch_eliminate(ctx->stream, p, M.n, M.row_ptr.p, M.col_idx.p, ctx->isdir.p,
+                                 p == 0 ? (void*)M.val64.p : (void*)M.val32.p);
+        M.lu_valid[0] = M.lu_valid[1] = false;
+        ctx->tangent_elim = true;
+        FM_TRY(sync(ctx));
+    }
+    ctx->cur = &M;
+    return FM_OK;
+}
+
+static int matrix_prec(fm_ctx* ctx, int prec, Matrix** A, int* p) {
+    if (!ctx->cur || !ctx->cur->ready) return fail(ctx, FM_ESTATE, "no matrix (fm_set_matrix / fm_use_tangent)");
+    if (prec != FM_FP64 && prec != FM_MIXED) return fail(ctx, FM_EINVAL, "bad precision");
+    *A = ctx->cur;
+    *p = prec == FM_FP64 ? PREC64 : PREC32;
+    if (!(*A)->val_valid[*p]) {
+        // derive the other precision from the one present
+        Matrix& M = **A;
+        if (*p == PREC32 && M.val_valid[0]) {
+            FM_HIP(M.val32.alloc(M.nnz));
+            std::vector<double> t(M.nnz);
+            FM_HIP(hipMemcpy(t.data(), M.val64.p, M.nnz * 8, hipMemcpyDeviceToHost));
+            std::vector<float> f(t.begin(), t.end());
+            FM_HIP(hipMemcpy(M.val32.p, f.data(), M.nnz * 4, hipMemcpyHostToDevice));
+            M.val_valid[1] = true;
+        } else {
+            return fail(ctx, FM_ESTATE, "matrix values not available in FP64 (assembled in mixed precision)");
+        }
+    }
+    return FM_OK;
+}
+
+static void scatter_perm(const Matrix& M, const double* sorted, double* out) {
+    if (M.perm.empty()) {
+        std::copy(sorted, sorted + M.nnz, out);
+        return;
+    }
+    for (uint64_t k = 0; k < M.nnz; ++k) out[M.perm[k]] = sorted[k];
+}
+
+int fm_spmv(fm_ctx* ctx, int prec, const double* x, double* y) {
+    if (!ctx || !x || !y) return fail(ctx, FM_EINVAL, "fm_spmv: bad arguments");
+    Matrix* A;
+    int p;
+    FM_TRY(matrix_prec(ctx, prec, &A, &p));
+    FM_TRY(pcg_vectors(ctx, p, A->n));
+    if (p == PREC64) {
+        FM_HIP(hipMemcpy(ctx->p64.p, x, A->n * 8ull, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> t(x, x + A->n);
+        FM_HIP(hipMemcpy(ctx->p32.p, t.data(), A->n * 4ull, hipMemcpyHostToDevice));
+    }
+    {
+        ProfScope ps(ctx, FM_K_SPMV);
+        launch_spmv(ctx->stream, p, *A, vb(ctx, p, VP), vb(ctx, p, VAP), false, nullptr, nullptr, nullptr, nullptr);
+    }
+    FM_TRY(check_launch(ctx, "spmv"));
+    FM_TRY(sync(ctx));
+    if (p == PREC64) {
+        FM_HIP(hipMemcpy(y, ctx->Ap64.p, A->n * 8ull, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> t(A->n);
+        FM_HIP(hipMemcpy(t.data(), ctx->Ap32.p, A->n * 4ull, hipMemcpyDeviceToHost));
+        std::copy(t.begin(), t.end(), y);
+    }
+    return FM_OK;
+}
+
+int fm_ilu0_factor(fm_ctx* ctx, int prec, double* LU_out) {
+    if (!ctx) return FM_EINVAL;
+    Matrix* A;
+    int p;
+    FM_TRY(matrix_prec(ctx, prec, &A, &p));
+    A->lu_valid[p] = false;
+    FM_TRY(factorize(ctx, *A, p));
+    FM_TRY(sync(ctx));
+    if (LU_out) {
+        std::vector<double> t(A->nnz);
+        if (p == PREC64) {
+            FM_HIP(hipMemcpy(t.data(), A->lu64.p, A->nnz * 8, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> f(A->nnz);
+            FM_HIP(hipMemcpy(f.data(), A->lu32.p, A->nnz * 4, hipMemcpyDeviceToHost));
+            std::copy(f.begin(), f.end(), t.begin());
+        }
+        scatter_perm(*A, t.data(), LU_out);
+    }
+    return FM_OK;
+}
+
+int fm_ilu0_apply(fm_ctx* ctx, int prec, const double* r, double* z) {
+    if (!ctx || !r || !z) return fail(ctx, FM_EINVAL, "fm_ilu0_apply: bad arguments");
+    Matrix* A;
+    int p;
+    FM_TRY(matrix_prec(ctx, prec, &A, &p));
+    FM_TRY(factorize(ctx, *A, p));
+    FM_TRY(pcg_vectors(ctx, p, A->n));
+    if (p == PREC64) {
+        FM_HIP(hipMemcpy(ctx->r64.p, r, A->n * 8ull, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> t(r, r + A->n);
+        FM_HIP(hipMemcpy(ctx->r32.p, t.data(), A->n * 4ull, hipMemcpyHostToDevice));
+    }
+    PcgState init{};
+    *ctx->h_st = init;
+    FM_HIP(hipMemcpyAsync(ctx->st.p, ctx->h_st, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
+    FM_TRY(ensure_partials(ctx, A->sched.q_bwd + 64));
+    precond_apply(ctx, *A, p, vb(ctx, p, VR), vb(ctx, p, VZ), vb(ctx, p, VY), nullptr);
+    FM_TRY(check_launch(ctx, "ilu apply"));
+    FM_TRY(sync(ctx));
+    FM_TRY(check_device_error(ctx));
+    if (p == PREC64) {
+        FM_HIP(hipMemcpy(z, ctx->y64.p, A->n * 8ull, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> t(A->n);
+        FM_HIP(hipMemcpy(t.data(), ctx->y32.p, A->n * 4ull, hipMemcpyDeviceToHost));
+        std::copy(t.begin(), t.end(), z);
+    }
+    return FM_OK;
+}
+
+int fm_pcg(fm_ctx* ctx, int prec, const double* b, double* x, double tol, int max_it, int* iters, double* relres) {
+    if (!ctx || !b || !x || max_it < 0) return fail(ctx, FM_EINVAL, "fm_pcg: bad arguments");
+    Matrix* A;
+    int p;
+    FM_TRY(matrix_prec(ctx, prec, &A, &p));
+    A->lu_valid[p] = false;  // pcg_solve_ilu0 factorizes on every call
+    FM_TRY(pcg_vectors(ctx, p, A->n));
+    const size_t n = A->n;
+    if (p == PREC64) {
+        FM_HIP(hipMemcpy(ctx->b64.p, b, n * 8, hipMemcpyHostToDevice));
+        FM_HIP(hipMemcpy(ctx->x64.p, x, n * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tb(b, b + n), tx(x, x + n);
+        FM_HIP(hipMemcpy(ctx->b32.p, tb.data(), n * 4, hipMemcpyHostToDevice));
+        FM_HIP(hipMemcpy(ctx->x32.p, tx.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    PcgResult pr{};
+    FM_TRY(pcg_run(ctx, *A, p, tol, max_it, &pr));
+    if (p == PREC64) {
+        FM_HIP(hipMemcpy(x, ctx->x64.p, n * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> t(n);
+        FM_HIP(hipMemcpy(t.data(), ctx->x32.p, n * 4, hipMemcpyDeviceToHost));
+        std::copy(t.begin(), t.end(), x);
+    }
+    if (iters) *iters = pr.iters;
+    if (relres) *relres = pr.relres;
+    return FM_OK;
+}
+
+void fm_newton_cfg_default(fm_newton_cfg* c) {
+    if (!c) return;
+    c->n_steps = 1;
+    c->prec = FM_FP64;
+    c->newton_rtol = 1e-8;
+    c->max_newton = 30;
+    c->pcg_tol = 1e-10;  // newton_tol * 0.01 (FEM/adaptive_fem_engine.hpp:2577-2578)
+    c->pcg_max = 300;
+    c->inner_tol = 1e-4;
+    c->ls_max = 10;
+    c->armijo_c = 1e-4;
+    c->lift = 1;
+}
+
+int fm_set_u(fm_ctx* ctx, const double* u) {
+    if (!ctx || !u) return FM_EINVAL;
+    FM_TRY(ensure_vectors(ctx));
+    FM_HIP(hipMemcpy(ctx->u.p, u, 3ull * ctx->nn * 8, hipMemcpyHostToDevice));
+    ctx->u_valid = true;
+    return FM_OK;
+}
+
+int fm_get_u(fm_ctx* ctx, double* u) {
+    if (!ctx || !u) return FM_EINVAL;
+    if (!ctx->u_valid) return fail(ctx, FM_ESTATE, "no device state");
+    FM_HIP(hipMemcpy(u, ctx->u.p, 3ull * ctx->nn * 8, hipMemcpyDeviceToHost));
+    return FM_OK;
+}
+
+// One load step of the Newton loop (same algorithm as the FP64 CPU oracle's
+// orc_newton_step): prescribed values lf*dir_vals, R_0 at the previous state
+// with them set, optional predictor solve with the prescribed increment,
+// then Newton + Armijo until ||R|| <= rtol ||R_0||.
+int fm_newton_step(fm_ctx* ctx, uint32_t step, double* u, const fm_newton_cfg* cfg_in, fm_newton_stats* stats) {
+    if (!ctx) return FM_EINVAL;
+    fm_newton_cfg cfg;
+    if (cfg_in)
+        cfg = *cfg_in;
+    else
+        fm_newton_cfg_default(&cfg);
+    if (cfg.n_steps == 0 || step == 0 || step > cfg.n_steps) return fail(ctx, FM_EINVAL, "bad load step");
+    if (cfg.prec != FM_FP64 && cfg.prec != FM_MIXED) return fail(ctx, FM_EINVAL, "bad precision");
+    FM_TRY(ensure_ready(ctx));
+    const double t0 = now_s();
+    hipStream_t s = ctx->stream;
+    const uint32_t n = 3 * ctx->nn;
+    if (u) {
+        FM_HIP(hipMemcpyAsync(ctx->u.p, u, n * 8ull, hipMemcpyHostToDevice, s));
+        ctx->u_valid = true;
+    }
+    if (!ctx->u_valid) {
+        launch_fill_f64(s, ctx->u.p, n, 0.0);
+        ctx->u_valid = true;
+    }
+    const int kprec = cfg.prec == FM_FP64 ? PREC64 : PREC32;
+    const double lf = (double)step / (double)cfg.n_steps;
+    fm_newton_stats S{};
+    const uint32_t ndir = (uint32_t)ctx->dir_dofs.size();
+    bool any_inc = false;
+    for (double v : ctx->dir_vals) any_inc |= v != 0.0;
+    launch_fill_f64(s, ctx->dD.p, n, 0.0);
+    launch_dir_increment(s, ndir, ctx->d_dir_dofs.p, ctx->d_dir_vals.p, lf, ctx->u.p, ctx->dD.p);
+    launch_axpy_to(s, n, ctx->u.p, 1.0, ctx->dD.p, ctx->ut.p);
+    FM_TRY(residual(ctx, ctx->ut.p, lf, &S.r0));
+    S.residuals++;
+    if (cfg.lift && any_inc) {
+        FM_TRY(assemble(ctx, ctx->u.p, true, kprec, true, ctx->dD.p, FM_K_ASSEMBLE));
+        S.assemblies++;
+        FM_TRY(ensure_partials(ctx, n / 256 + 1));
+        launch_residual_finish(s, n, lf, ctx->have_fext ? ctx->f_ext.p : nullptr, ctx->f.p, ctx->isdir.p, ctx->R.p,
+                               ctx->partials.p);
+        launch_sub_masked(s, n, ctx->R.p, ctx->lift.p, ctx->isdir.p, ctx->hv.p);
+        int its = 0;
+        const double tol = kprec == PREC64 ? cfg.pcg_tol : cfg.inner_tol;
+        FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, tol, cfg.pcg_max, &its));
+        S.pcg_iters += its;
+        S.factorizations++;
+        launch_apply_predictor(s, n, ctx->du.p, ctx->dD.p, ctx->isdir.p, ctx->u.p);
+    } else {
+        launch_axpy_to(s, n, ctx->u.p, 1.0, ctx->dD.p, ctx->u.p);
+    }
+    double rn = 0.0;
+    FM_TRY(residual(ctx, ctx->u.p, lf, &rn));
+    S.residuals++;
+    const double target = cfg.newton_rtol * S.r0;
+    int it = 0;
+    while (!(rn <= target) && it < cfg.max_newton) {
+        FM_TRY(assemble(ctx, ctx->u.p, true, kprec, true, nullptr, FM_K_ASSEMBLE));
+        S.assemblies++;
+        int its = 0;
+        const double tol = kprec == PREC64 ? cfg.pcg_tol : cfg.inner_tol;
+        // R of the current u is in ctx->R (last residual evaluation)
+        FM_HIP(hipMemcpyAsync(ctx->hv.p, ctx->R.p, n * 8ull, hipMemcpyDeviceToDevice, s));
+        FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, tol, cfg.pcg_max, &its));
+        S.pcg_iters += its;
+        S.factorizations++;
+        // armijo_line_search_true (FEM/adaptive_fem_engine.hpp:933-985)
+        double alpha = 1.0, rt = 0.0;
+        bool accepted = false;
+        for (int bt = 0; bt < cfg.ls_max; ++bt) {
+            launch_axpy_to(s, n, ctx->u.p, alpha, ctx->du.p, ctx->ut.p);
+            FM_TRY(residual(ctx, ctx->ut.p, lf, &rt));
+            S.residuals++;
+            if (rt <= (1.0 - cfg.armijo_c * alpha) * rn + 1e-14) {
+                accepted = true;
+                break;
+            }
+            alpha *= 0.5;
+        }
+        if (!accepted) {
+            launch_axpy_to(s, n, ctx->u.p, alpha, ctx->du.p, ctx->ut.p);
+            FM_TRY(residual(ctx, ctx->ut.p, lf, &rt));
+            S.residuals++;
+        }
+        std::swap(ctx->u.p, ctx->ut.p);
+        rn = rt;
+        ++it;
+    }
+    FM_TRY(sync(ctx));
+    S.newton_iters = it;
+    S.converged = rn <= target;
+    S.rel_res = S.r0 > 0 ? rn / S.r0 : 0.0;
+    S.seconds = now_s() - t0;
+    if (stats) *stats = S;
+    if (u) FM_HIP(hipMemcpy(u, ctx->u.p, n * 8ull, hipMemcpyDeviceToHost));
+    return S.converged ? FM_OK : fail(ctx, FM_ENOCONV, "Newton did not reach the relative tolerance");
+}
+
+int fm_adjoint(fm_ctx* ctx, const double* dLdu, double tol, int max_it, double* dL_dmu, double* dL_dlam,
+               double* dL_dload, int* iters) {
+    if (!ctx || !dLdu) return fail(ctx, FM_EINVAL, "fm_adjoint: bad arguments");
+    FM_TRY(ensure_ready(ctx));
+    if (!ctx->u_valid) return fail(ctx, FM_ESTATE, "no converged state (fm_newton_step / fm_set_u)");
+    hipStream_t s = ctx->stream;
+    const uint32_t n = 3 * ctx->nn;
+    const uint32_t ndir = (uint32_t)ctx->dir_dofs.size();
+    // uD: Dirichlet values at load factor 1 (zero elsewhere), through dD
+    launch_fill_f64(s, ctx->hv.p, n, 0.0);
+    launch_fill_f64(s, ctx->dD.p, n, 0.0);
+    launch_dir_increment(s, ndir, ctx->d_dir_dofs.p, ctx->d_dir_vals.p, 1.0, ctx->hv.p, ctx->dD.p);
+    FM_TRY(assemble(ctx, ctx->u.p, true, PREC64, true, ctx->dD.p, FM_K_ASSEMBLE));
+    std::vector<double> g(dLdu, dLdu + n);
+    for (uint32_t d : ctx->dir_dofs) g[d] = 0.0;
+    FM_HIP(hipMemcpyAsync(ctx->R.p, g.data(), n * 8ull, hipMemcpyHostToDevice, s));
+    FM_TRY(sync(ctx));
+    int its = 0;
+    FM_TRY(linear_solve(ctx, PREC64, ctx->R.p, ctx->du.p, tol, max_it, &its));
+    if (iters) *iters = its;
+    double* dmu = nullptr;
+    double* dlam = nullptr;
+    DevBuf<double> bm, bl;
+    if (dL_dmu) {
+        FM_HIP(bm.alloc(ctx->ne));
+        dmu = bm.p;
+    }
+    if (dL_dlam) {
+        FM_HIP(bl.alloc(ctx->ne));
+        dlam = bl.p;
+    }
+    if (dmu || dlam) {
+        launch_adjoint_elem(s, elem_args(ctx, ctx->u.p), ctx->eperm.p, ctx->du.p, dmu, dlam);
+        FM_TRY(check_launch(ctx, "adjoint"));
+    }
+    FM_TRY(sync(ctx));
+    if (dL_dmu) FM_HIP(hipMemcpy(dL_dmu, bm.p, ctx->ne * 8ull, hipMemcpyDeviceToHost));
+    if (dL_dlam) FM_HIP(hipMemcpy(dL_dlam, bl.p, ctx->ne * 8ull, hipMemcpyDeviceToHost));
+    if (dL_dload) {
+        std::vector<double> lam(n), lift(n), fe(n, 0.0);
+        FM_HIP(hipMemcpy(lam.data(), ctx->du.p, n * 8ull, hipMemcpyDeviceToHost));
+        FM_HIP(hipMemcpy(lift.data(), ctx->lift.p, n * 8ull, hipMemcpyDeviceToHost));
+        if (ctx->have_fext) FM_HIP(hipMemcpy(fe.data(), ctx->f_ext.p, n * 8ull, hipMemcpyDeviceToHost));
+        std::vector<uint8_t> isd(n, 0);
+        std::vector<double> uD(n, 0.0);
+        for (uint32_t k = 0; k < ndir; ++k) {
+            isd[ctx->dir_dofs[k]] = 1;
+            uD[ctx->dir_dofs[k]] = ctx->dir_vals[k];
+        }
+        double acc = 0.0;
+        for (uint32_t i = 0; i < n; ++i) acc += isd[i] ? dLdu[i] * uD[i] : lam[i] * (fe[i] - lift[i]);
+        *dL_dload = acc;
+    }
+    return FM_OK;
+}
+
+int fm_rot_logexp_batch(fm_ctx* ctx, uint32_t n, const double* F, double alpha, double* R, double* w, double* Q) {
+    if (!ctx || !F) return fail(ctx, FM_EINVAL, "fm_rot_logexp_batch: bad arguments");
+    if (!n) return FM_OK;
+    DevBuf<double> dF, dR, dw, dQ;
+    FM_HIP(dF.alloc(9ull * n));
+    FM_HIP(hipMemcpy(dF.p, F, 9ull * n * 8, hipMemcpyHostToDevice));
+    if (R) FM_HIP(dR.alloc(9ull * n));
+    if (w) FM_HIP(dw.alloc(3ull * n));
+    if (Q) FM_HIP(dQ.alloc(9ull * n));
+    launch_rot_batch(ctx->stream, n, dF.p, alpha, dR.p, dw.p, dQ.p);
+    FM_TRY(check_launch(ctx, "rot_batch"));
+    FM_TRY(sync(ctx));
+    if (R) FM_HIP(hipMemcpy(R, dR.p, 9ull * n * 8, hipMemcpyDeviceToHost));
+    if (w) FM_HIP(hipMemcpy(w, dw.p, 3ull * n * 8, hipMemcpyDeviceToHost));
+    if (Q) FM_HIP(hipMemcpy(Q, dQ.p, 9ull * n * 8, hipMemcpyDeviceToHost));
+    return FM_OK;
+}
+
+int fm_element_rotations(fm_ctx* ctx, double* w) {
+    if (!ctx || !w) return FM_EINVAL;
+    FM_TRY(ensure_ready(ctx));
+    if (!ctx->u_valid) return fail(ctx, FM_ESTATE, "no device state");
+    DevBuf<double> dw;
+    FM_HIP(dw.alloc(3ull * ctx->ne));
+    launch_elem_rot(ctx->stream, elem_args(ctx, ctx->u.p), ctx->eperm.p, dw.p);
+    FM_TRY(check_launch(ctx, "elem_rot"));
+    FM_TRY(sync(ctx));
+    FM_HIP(hipMemcpy(w, dw.p, 3ull * ctx->ne * 8, hipMemcpyDeviceToHost));
+    return FM_OK;
+}
+
+int fm_checkpoint_save(fm_ctx* ctx, const char* path, uint32_t load_step) {
+    if (!ctx || !path) return FM_EINVAL;
+    if (!ctx->u_valid) return fail(ctx, FM_ESTATE, "no device state");
+    const uint32_t nn = ctx->nn, ne = ctx->ne, nd = 3 * nn;
+    std::vector<double> u(nd);
+    FM_HIP(hipMemcpy(u.data(), ctx->u.p, nd * 8ull, hipMemcpyDeviceToHost));
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return fail(ctx, FM_EINVAL, std::string("cannot open ") + path);
+    uint64_t h = 1469598103934665603ull;
+    auto put = [&](const void* p, size_t bytes) {
+        fnv1a_update(h, p, bytes);
+        return std::fwrite(p, 1, bytes, fp) == bytes;
+    };
+    const char magic[8] = {'F', 'M', 'C', 'K', 'P', 'T', '0', '1'};
+    const uint32_t hdr[6] = {1u /*version*/, load_step, 0u /*adapt_iter*/, nn, ne, nd};
+    bool ok = put(magic, 8) && put(hdr, sizeof hdr);
+    // nodes: x, y, z, u, v, w (the reference's 6 x f64 node record)
+    for (uint32_t v = 0; v < nn && ok; ++v) {
+        const double rec[6] = {ctx->hX[v], ctx->hX[nn + v], ctx->hX[2ull * nn + v], u[3ull * v], u[3ull * v + 1],
+                               u[3ull * v + 2]};
+        ok = put(rec, sizeof rec);
+    }
+    // elements: 4 x u32 nodes, mu_e, lam_e
+    for (uint32_t e = 0; e < ne && ok; ++e) {
+        const uint32_t c[4] = {ctx->hconn[e], ctx->hconn[(size_t)ne + e], ctx->hconn[2ull * ne + e],
+                               ctx->hconn[3ull * ne + e]};
+        const double m[2] = {ctx->hmu.empty() ? 0.0 : ctx->hmu[e], ctx->hlam.empty() ? 0.0 : ctx->hlam[e]};
+        ok = put(c, sizeof c) && put(m, sizeof m);
+    }
+    ok = ok && put(u.data(), nd * 8ull);
+    const uint64_t hh = h;
+    ok = ok && std::fwrite(&hh, 1, 8, fp) == 8;
+    ok = (std::fclose(fp) == 0) && ok;
+    return ok ? FM_OK : fail(ctx, FM_EINVAL, "checkpoint write failed");
+}
+
+int fm_checkpoint_load(fm_ctx* ctx, const char* path, uint32_t* load_step) {
+    if (!ctx || !path) return FM_EINVAL;
+    if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return fail(ctx, FM_EINVAL, std::string("cannot open ") + path);
+    std::fseek(fp, 0, SEEK_END);
+    const long size = std::ftell(fp);
+    std::fseek(fp, 0, SEEK_SET);
+    std::vector<uint8_t> buf(size > 0 ? (size_t)size : 0);
+    const bool rd = size > 0 && std::fread(buf.data(), 1, buf.size(), fp) == buf.size();
+    std::fclose(fp);
+    const uint32_t nn = ctx->nn, ne = ctx->ne, nd = 3 * nn;
+    const size_t expect = 8 + 24 + (size_t)nn * 48 + (size_t)ne * 32 + (size_t)nd * 8 + 8;
+    if (!rd || buf.size() != expect) return fail(ctx, FM_EINVAL, "checkpoint truncated or of another mesh size");
+    if (std::memcmp(buf.data(), "FMCKPT01", 8) != 0) return fail(ctx, FM_EINVAL, "not a checkpoint file");
+    uint64_t h = 1469598103934665603ull;
+    fnv1a_update(h, buf.data(), buf.size() - 8);
+    uint64_t stored;
+    std::memcpy(&stored, buf.data() + buf.size() - 8, 8);
+    if (h != stored) return fail(ctx, FM_EINVAL, "checkpoint checksum mismatch");
+    uint32_t hdr[6];
+    std::memcpy(hdr, buf.data() + 8, sizeof hdr);
+    if (hdr[0] != 1 || hdr[3] != nn || hdr[4] != ne || hdr[5] != nd)
+        return fail(ctx, FM_EINVAL, "checkpoint version or sizes differ from the mesh");
+    std::vector<double> u(nd);
+    std::memcpy(u.data(), buf.data() + buf.size() - 8 - (size_t)nd * 8, (size_t)nd * 8);
+    if (load_step) *load_step = hdr[1];
+    return fm_set_u(ctx, u.data());
+}
+
+int fm_profile_enable(fm_ctx* ctx, int on) {
+    if (!ctx) return FM_EINVAL;
+    (void)hipStreamSynchronize(ctx->stream);
+    prof_flush(ctx);
+    ctx->prof = on != 0;
+    return FM_OK;
+}
+
+int fm_profile_reset(fm_ctx* ctx) {
+    if (!ctx) return FM_EINVAL;
+    (void)hipStreamSynchronize(ctx->stream);
+    prof_flush(ctx);
+    for (int k = 0; k < FM_K_COUNT; ++k) {
+        ctx->ms[k] = 0;
+        ctx->launches[k] = 0;
+    }
+    return FM_OK;
+}
+
+int fm_profile_get(fm_ctx* ctx, double* ms, uint64_t* launches) {
+    if (!ctx) return FM_EINVAL;
+    (void)hipStreamSynchronize(ctx->stream);
+    prof_flush(ctx);
+    for (int k = 0; k < FM_K_COUNT; ++k) {
+        if (ms) ms[k] = ctx->ms[k];
+        if (launches) launches[k] = ctx->launches[k];
+    }
+    return FM_OK;
+}
+
+int fm_info(fm_ctx* ctx, uint64_t* out5) {
+    if (!ctx || !out5) return FM_EINVAL;
+    const Matrix* A = ctx->cur ? ctx->cur : &ctx->tangent;
+    out5[0] = A->n;
+    out5[1] = A->nnz;
+    out5[2] = ctx->ne;
+    out5[3] = ctx->nn;
+    out5[4] = ctx->n_colors;
+    return FM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,227 @@
+// fm_internal.hpp -- context state and kernel launchers of the MI355X Newton path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "flexmesh/fm_api.h"
+
+namespace fm {
+
+// ---------------------------------------------------------------------------
+// device buffer (owned by the context)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    hipError_t alloc(size_t count) {
+        if (count == n && p) return hipSuccess;
+        release();
+        if (count == 0) return hipSuccess;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Scalars of one PCG solve, kept on the device so that no host round trip is
+// needed between iterations.  Kernels read `done` and return at once when set.
+struct PcgState {
+    double rz, pAp, alpha, beta, bb, rz_new, relres, tol;
+    double slot[4];
+    int done;     // 1 once converged, broken down or out of iterations
+    int status;   // 0 running, 1 converged, 2 breakdown (|pAp| < 1e-60), 3 max_it
+    int iters;
+    int max_it;
+    uint32_t ticket[4];  // work-queue counters of the triangular-solve kernels
+};
+
+// Level schedule of the natural-order ILU(0) dependence graph on groups of B
+// rows ("nodes": B = 3 for the 3-DOF mesh pattern, B = 1 otherwise).
+struct Schedule {
+    int B = 1;
+    uint32_t n_nodes = 0;
+    int slots = 1;           // ceil(max row length / 64)
+    int lpn = 16;            // lanes per node in the triangular solves
+    uint32_t levels_fwd = 0, levels_bwd = 0;
+    // level-ordered node queues; each level is padded with 0xFFFFFFFF entries
+    // to a multiple of the nodes one wave takes, so that a wave never holds
+    // two nodes of which one depends on the other
+    uint32_t q_fwd = 0, q_bwd = 0;
+    DevBuf<uint32_t> fwd_order, bwd_order;
+    DevBuf<uint16_t> nlow;   // number of lower column blocks per node
+};
+
+struct Matrix {
+    uint32_t n = 0;
+    uint64_t nnz = 0;
+    std::vector<uint32_t> h_row_ptr, h_col_idx;
+    DevBuf<uint32_t> row_ptr, col_idx;
+    DevBuf<double> val64;
+    DevBuf<float> val32;
+    DevBuf<double> lu64;
+    DevBuf<float> lu32;
+    bool lu_valid[2] = {false, false};
+    bool val_valid[2] = {false, false};
+    Schedule sched;
+    std::vector<uint32_t> perm;  // sorted position k -> caller's storage position (external matrices)
+    bool ready = false;
+};
+
+enum { PREC64 = 0, PREC32 = 1 };
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // mesh (host copies are kept for the host-side set-up passes)
+    uint32_t nn = 0, ne = 0;
+    std::vector<double> hX;
+    std::vector<uint32_t> hconn;
+    std::vector<double> hmu, hlam;
+    DevBuf<double> X;  // 3*nn SoA
+
+    // colouring and colour-ordered element arrays
+    bool have_colors = false;
+    uint32_t n_colors = 0;
+    std::vector<uint32_t> hcolors, color_ptr, hperm;
+    DevBuf<uint32_t> conn_c;  // 4*ne SoA, colour order
+    DevBuf<uint32_t> eperm;   // colour-order index -> element id
+    DevBuf<double> mu_c, lam_c;
+    DevBuf<uint8_t> erank_c;  // 16*ne SoA: rank of node b in nbr(node a)
+    bool have_material = false;
+
+    // pattern (node adjacency + tangent matrix)
+    bool have_pattern = false;
+    std::vector<uint32_t> nbr_ptr, nbr;
+    Matrix tangent;
+    Matrix ext;
+    Matrix* cur = nullptr;
+    bool tangent_elim = false;
+
+    // boundary conditions and load
+    std::vector<uint32_t> dir_dofs;
+    std::vector<double> dir_vals;
+    DevBuf<uint8_t> isdir;
+    DevBuf<uint32_t> d_dir_dofs;
+    DevBuf<double> d_dir_vals;
+    DevBuf<double> f_ext;
+    bool have_fext = false;
+
+    // FP64 state and work vectors
+    DevBuf<double> u, f, R, du, ut, dD, lift, hv;
+    bool u_valid = false;
+    // PCG work vectors
+    DevBuf<double> b64, x64, r64, p64, Ap64, z64, y64;
+    DevBuf<float> b32, x32, r32, p32, Ap32, z32, y32;
+
+    // scalars and reductions
+    DevBuf<double> partials;
+    DevBuf<PcgState> st;
+    PcgState* h_st = nullptr;  // pinned
+    DevBuf<int> d_err;
+
+    // profiling
+    bool prof = false;
+    double ms[FM_K_COUNT] = {0};
+    uint64_t launches[FM_K_COUNT] = {0};
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+};
+
+// ---------------------------------------------------------------------------
+// host passes (fm_setup.cpp)
+// ---------------------------------------------------------------------------
+void node_adjacency(uint32_t nn, uint32_t ne, const uint32_t* conn, std::vector<uint32_t>& ptr,
+                    std::vector<uint32_t>& nbr);
+uint32_t greedy_colors(uint32_t nn, uint32_t ne, const uint32_t* conn, uint32_t* colors);
+// level schedule over a sorted CSR grouped in B rows; returns false if the
+// matrix lacks a stored diagonal or exceeds the supported row length
+bool build_schedule(const std::vector<uint32_t>& rp, const std::vector<uint32_t>& ci, int B, Schedule& S,
+                    hipStream_t stream, std::string& err);
+int detect_block(const std::vector<uint32_t>& rp, const std::vector<uint32_t>& ci);
+
+// ---------------------------------------------------------------------------
+// kernel launchers (fm_kernels.hip); all enqueue on `s`
+// ---------------------------------------------------------------------------
+struct ElemArgs {
+    const double* X;
+    uint32_t nn;
+    const uint32_t* conn;  // colour order, stride ne
+    uint32_t ne;
+    const double* mu;
+    const double* lam;
+    const uint8_t* erank;
+    const double* u;       // 3nn or nullptr
+};
+
+void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v);
+void launch_fill_f64(hipStream_t s, double* p, size_t n, double v);
+void launch_fill_f32(hipStream_t s, float* p, size_t n, float v);
+void launch_expand_pattern(hipStream_t s, uint32_t nn, const uint32_t* nbr_ptr, const uint32_t* nbr,
+                           const uint32_t* row_ptr, uint32_t* col_idx);
+void launch_assemble_color(hipStream_t s, int prec, bool with_K, const ElemArgs& a, uint32_t begin, uint32_t end,
+                           double* f, void* K, const uint32_t* row_ptr, const uint8_t* isdir, const double* dD,
+                           double* lift);
+void launch_set_dir_diag(hipStream_t s, int prec, uint32_t n_dir, const uint32_t* dofs, const uint32_t* row_ptr,
+                         const uint16_t* nlow, void* K);
+void launch_eliminate(hipStream_t s, int prec, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
+                      const uint8_t* isdir, void* K);
+// R = load f_ext - f (constrained rows zeroed); block partials of R.R into partials; returns #partials
+uint32_t launch_residual_finish(hipStream_t s, uint32_t n, double load, const double* f_ext, const double* f,
+                                const uint8_t* isdir, double* R, double* partials);
+// partials of a.b; returns #partials
+uint32_t launch_dot64(hipStream_t s, uint32_t n, const double* a, const double* b, double* partials);
+// reduce partials into st: op 0 = store in slot[k]; 1 = pAp/alpha; 2 = rz_new/convergence; 3 = init rz
+void launch_reduce(hipStream_t s, const double* partials, uint32_t np, PcgState* st, int op, int k);
+void launch_spmv(hipStream_t s, int prec, const Matrix& A, const void* x, void* y, bool dot, double* partials,
+                 void* sent0, void* sent1, const PcgState* st);
+uint32_t spmv_blocks(const Matrix& A);
+void launch_ilu_factor(hipStream_t s, int prec, const Matrix& A, const void* val, void* lu, PcgState* st,
+                       int* err);
+void launch_ilu_fwd(hipStream_t s, int prec, const Matrix& A, const void* lu, const void* rhs, void* r, void* x,
+                    const void* p, const void* Ap, void* z, PcgState* st, bool upd, int* err);
+uint32_t launch_ilu_bwd(hipStream_t s, int prec, const Matrix& A, const void* lu, const void* z, const void* r,
+                        void* y, double* partials, PcgState* st, bool check_done, int* err);
+void launch_fill_sentinel(hipStream_t s, int prec, void* a, void* b, size_t n);
+void launch_pcg_p_update(hipStream_t s, int prec, uint32_t n, const void* y, void* p, const PcgState* st);
+// r = b - Ap; x untouched; partials of r.r
+uint32_t launch_pcg_init_r(hipStream_t s, int prec, uint32_t n, const void* b, const void* Ap, void* r,
+                           double* partials);
+void launch_copy(hipStream_t s, int prec, uint32_t n, const void* src, void* dst);
+// dst32 = (float)(src64 * scale)
+void launch_f64_to_f32(hipStream_t s, uint32_t n, const double* src, double scale, float* dst);
+// dst64 = (double)src32 * scale
+void launch_f32_to_f64(hipStream_t s, uint32_t n, const float* src, double scale, double* dst);
+// y = a + alpha * b (FP64)
+void launch_axpy_to(hipStream_t s, uint32_t n, const double* a, double alpha, const double* b, double* y);
+// u[dofs[k]] = scale * vals[k] (values), and dD[dofs[k]] = scale*vals[k] - u_old[dofs[k]]
+void launch_dir_increment(hipStream_t s, uint32_t n_dir, const uint32_t* dofs, const double* vals, double scale,
+                          const double* u, double* dD);
+// rhs = R - lift (free rows), 0 on constrained rows
+void launch_sub_masked(hipStream_t s, uint32_t n, const double* R, const double* lift, const uint8_t* isdir,
+                       double* out);
+// u += du on free rows, u += dD on constrained rows
+void launch_apply_predictor(hipStream_t s, uint32_t n, const double* du, const double* dD, const uint8_t* isdir,
+                            double* u);
+void launch_adjoint_elem(hipStream_t s, const ElemArgs& a, const uint32_t* eperm, const double* lam_vec,
+                         double* dmu, double* dlam);
+void launch_rot_batch(hipStream_t s, uint32_t n, const double* F, double alpha, double* R, double* w, double* Q);
+void launch_elem_rot(hipStream_t s, const ElemArgs& a, const uint32_t* eperm, double* w);
+
+}  // namespace fm

@@ -1,0 +1,1239 @@
+// fm_kernels.hip -- hand-written gfx950 kernels of the quasi-static Newton path.
+//
+// All of this is HBM- or latency-bound integer/FP gather-scatter work: no
+// dense contraction exists, so MFMA is not used.  Kernels are written for
+// 64-wide wavefronts; grid-wide kernels remap blockIdx so that each of the
+// eight XCDs (own L2 each) works on one contiguous slice of rows / elements.
+#include "fm_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace fm {
+namespace {
+
+constexpr uint32_t kSpinLimit = 1u << 20;  // bounded wait (~1 s); then the kernel flags an error and drains
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;   // padding entry of a level-ordered queue
+
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nb) {
+    // workgroups are handed to XCDs round-robin; give XCD x a contiguous chunk
+    constexpr uint32_t NX = 8;
+    if (nb < NX) return bid;
+    const uint32_t q = nb / NX, r = nb % NX;
+    const uint32_t x = bid % NX, loc = bid / NX;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + loc;
+}
+
+// ---------------------------------------------------------------------------
+// ready-by-value publication for the sync-free solves: every output slot is
+// preset to an all-ones NaN pattern; a producer stores the value with an
+// agent-scope atomic store, a consumer spins (agent-scope atomic load) until
+// the slot is no longer the sentinel.  A computed all-ones pattern is replaced
+// by the canonical quiet NaN so it cannot be mistaken for "not ready".
+// ---------------------------------------------------------------------------
+template <typename T>
+struct SB;
+template <>
+struct SB<float> {
+    using U = uint32_t;
+    static constexpr U S = 0xFFFFFFFFu, Q = 0x7FC00000u;
+};
+template <>
+struct SB<double> {
+    using U = unsigned long long;
+    static constexpr U S = 0xFFFFFFFFFFFFFFFFull, Q = 0x7FF8000000000000ull;
+};
+
+template <typename T>
+__device__ __forceinline__ void publish(T* p, T v) {
+    using U = typename SB<T>::U;
+    U b = __builtin_bit_cast(U, v);
+    if (b == SB<T>::S) b = SB<T>::Q;
+    __hip_atomic_store(reinterpret_cast<U*>(p), b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__device__ __forceinline__ T await_val(const T* p, int* err) {
+    using U = typename SB<T>::U;
+    U* q = const_cast<U*>(reinterpret_cast<const U*>(p));
+    U b = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t spins = 0;
+    while (b == SB<T>::S) {
+        if (++spins > kSpinLimit) {
+            atomicOr(err, 1);
+            break;
+        }
+        // once any wave has hit the limit every other wait gives up too
+        if ((spins & 255u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        __builtin_amdgcn_s_sleep(1);
+        b = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return __builtin_bit_cast(T, b);
+}
+
+template <typename T>
+__device__ __forceinline__ T sentinel_value() {
+    return __builtin_bit_cast(T, SB<T>::S);
+}
+
+// fixed-order block sum (blockDim.x a multiple of 64); result valid in thread 0
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double s_part[16];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) s_part[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += s_part[i];
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// element evaluation (FP64): geometry, kinematics, force and tangent factors
+// ---------------------------------------------------------------------------
+struct Elem {
+    uint32_t node[4];
+    double g[4][3];   // shape-function gradients
+    double a[4][3];   // F^{-T} g
+    double Fg[4][3];  // F g
+    double V, lnJ, mu, lam;
+    bool clamped, ok;
+};
+
+__device__ __forceinline__ double det3(const double* A) {
+    return A[0] * (A[4] * A[8] - A[7] * A[5]) - A[3] * (A[1] * A[8] - A[7] * A[2]) +
+           A[6] * (A[1] * A[5] - A[4] * A[2]);
+}
+
+// F, ln J with the clamp J >= 0.01, F^{-T} (with the C^{-1} guard of the
+// reference inverse carried to F^{-T} = F C^{-1} when |det F| < 1e-15)
+__device__ __forceinline__ void kinematics(const double* F, double* FiT, double& lnJ, bool& clamped) {
+    const double det = det3(F);
+    const double J = fabs(det);
+    if (J < 1e-15) {
+        for (int k = 0; k < 9; ++k) FiT[k] = 1e30 * F[k];
+    } else {
+        const double id = 1.0 / det;
+        FiT[0] = (F[4] * F[8] - F[5] * F[7]) * id;
+        FiT[1] = (F[5] * F[6] - F[3] * F[8]) * id;
+        FiT[2] = (F[3] * F[7] - F[4] * F[6]) * id;
+        FiT[3] = (F[2] * F[7] - F[1] * F[8]) * id;
+        FiT[4] = (F[0] * F[8] - F[2] * F[6]) * id;
+        FiT[5] = (F[1] * F[6] - F[0] * F[7]) * id;
+        FiT[6] = (F[1] * F[5] - F[2] * F[4]) * id;
+        FiT[7] = (F[2] * F[3] - F[0] * F[5]) * id;
+        FiT[8] = (F[0] * F[4] - F[1] * F[3]) * id;
+    }
+    clamped = J < 0.01;
+    lnJ = log(fmax(J, 0.01));
+}
+
+__device__ __forceinline__ void elem_eval(const ElemArgs& A, uint32_t t, Elem& E, double* Fout = nullptr) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) E.node[q] = A.conn[(size_t)q * A.ne + t];
+    double Xe[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) Xe[q][d] = A.X[(size_t)d * A.nn + E.node[q]];
+    double J[9];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        J[d * 3 + 0] = Xe[1][d] - Xe[0][d];
+        J[d * 3 + 1] = Xe[2][d] - Xe[0][d];
+        J[d * 3 + 2] = Xe[3][d] - Xe[0][d];
+    }
+    const double det = det3(J);
+    E.V = fabs(det) / 6.0;
+    E.ok = E.V >= 1e-30;
+    E.mu = A.mu[t];
+    E.lam = A.lam[t];
+    if (!E.ok) return;
+    const double id = 1.0 / det;
+    // J^{-1} rows (cofactor form of matrix_3x3_inv); g_q = J^{-T} dN_xi,q
+    double Ji[9];
+    Ji[0] = (J[4] * J[8] - J[7] * J[5]) * id;
+    Ji[1] = -(J[1] * J[8] - J[7] * J[2]) * id;
+    Ji[2] = (J[1] * J[5] - J[4] * J[2]) * id;
+    Ji[3] = -(J[3] * J[8] - J[6] * J[5]) * id;
+    Ji[4] = (J[0] * J[8] - J[6] * J[2]) * id;
+    Ji[5] = -(J[0] * J[5] - J[3] * J[2]) * id;
+    Ji[6] = (J[3] * J[7] - J[6] * J[4]) * id;
+    Ji[7] = -(J[0] * J[7] - J[6] * J[1]) * id;
+    Ji[8] = (J[0] * J[4] - J[3] * J[1]) * id;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        E.g[1][i] = Ji[0 * 3 + i];
+        E.g[2][i] = Ji[1 * 3 + i];
+        E.g[3][i] = Ji[2 * 3 + i];
+        E.g[0][i] = -(Ji[0 * 3 + i] + Ji[1 * 3 + i] + Ji[2 * 3 + i]);
+    }
+    double F[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) F[i * 3 + j] = (i == j) ? 1.0 : 0.0;
+    if (A.u) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double* uq = A.u + 3 * (size_t)E.node[q];
+            const double u0 = uq[0], u1 = uq[1], u2 = uq[2];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                F[0 * 3 + j] += u0 * E.g[q][j];
+                F[1 * 3 + j] += u1 * E.g[q][j];
+                F[2 * 3 + j] += u2 * E.g[q][j];
+            }
+        }
+    }
+    if (Fout)
+        for (int k = 0; k < 9; ++k) Fout[k] = F[k];
+    double FiT[9];
+    kinematics(F, FiT, E.lnJ, E.clamped);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            E.a[q][i] = FiT[i * 3 + 0] * E.g[q][0] + FiT[i * 3 + 1] * E.g[q][1] + FiT[i * 3 + 2] * E.g[q][2];
+            E.Fg[q][i] = F[i * 3 + 0] * E.g[q][0] + F[i * 3 + 1] * E.g[q][1] + F[i * 3 + 2] * E.g[q][2];
+        }
+}
+
+// Colour-by-colour element kernel: one element per lane computes f_e
+// (FP64, scattered straight away) and the factors of its 12x12 tangent
+//   K_ab,ik = V [mu (g_a.g_b) d_ik + lam' a_a,i a_b,k + c a_a,k a_b,i],
+// which are staged in LDS; the wave then writes the 144 entries of each of
+// its elements with consecutive lanes on consecutive CSR positions.  No two
+// elements of one colour share a node, so no atomics are needed.
+template <typename T, bool WITH_K>
+__global__ __launch_bounds__(64) void assemble_color_kernel(ElemArgs A, uint32_t begin, uint32_t end,
+                                                            double* __restrict__ f, T* __restrict__ K,
+                                                            const uint32_t* __restrict__ row_ptr,
+                                                            const uint8_t* __restrict__ isdir,
+                                                            const double* __restrict__ dD,
+                                                            double* __restrict__ lift) {
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t t = begin + blk * 64 + threadIdx.x;
+    const int lane = threadIdx.x;
+    Elem E;
+    E.ok = false;
+    if (t < end) elem_eval(A, t, E);
+    if (E.ok) {
+        const double s = E.lam * E.lnJ - E.mu;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double* fq = f + 3 * (size_t)E.node[q];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) fq[i] += E.V * (E.mu * E.Fg[q][i] + s * E.a[q][i]);
+        }
+    }
+    if constexpr (WITH_K) {
+        __shared__ T sV[64], smu[64], slp[64], sc[64];
+        __shared__ T sg[64][12], sa[64][12];
+        __shared__ uint32_t srp[64][12];
+        __shared__ uint8_t srank[64][16];
+        __shared__ uint8_t sdir[64][12];
+        __shared__ int sok[64];
+        const double lp = E.clamped ? 0.0 : E.lam;
+        const double c = E.mu - E.lam * E.lnJ;
+        sok[lane] = E.ok ? 1 : 0;
+        if (E.ok) {
+            sV[lane] = (T)E.V;
+            smu[lane] = (T)E.mu;
+            slp[lane] = (T)lp;
+            sc[lane] = (T)c;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    sg[lane][q * 3 + i] = (T)E.g[q][i];
+                    sa[lane][q * 3 + i] = (T)E.a[q][i];
+                    srp[lane][q * 3 + i] = row_ptr[3 * E.node[q] + i];
+                    sdir[lane][q * 3 + i] = isdir ? isdir[3 * E.node[q] + i] : 0;
+                }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) srank[lane][k] = A.erank[(size_t)k * A.ne + t];
+            if (dD) {
+                // K_e dD_e with the un-eliminated element matrix (FP64): the
+                // right-hand-side correction of a prescribed increment
+                double dq[4][3];
+                bool any = false;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        dq[q][i] = dD[3 * (size_t)E.node[q] + i];
+                        any |= dq[q][i] != 0.0;
+                    }
+                if (any) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        double out[3] = {0, 0, 0};
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) {
+                            const double gg = E.g[q][0] * E.g[b][0] + E.g[q][1] * E.g[b][1] + E.g[q][2] * E.g[b][2];
+                            const double ad = E.a[b][0] * dq[b][0] + E.a[b][1] * dq[b][1] + E.a[b][2] * dq[b][2];
+                            const double qd = E.a[q][0] * dq[b][0] + E.a[q][1] * dq[b][1] + E.a[q][2] * dq[b][2];
+#pragma unroll
+                            for (int i = 0; i < 3; ++i) out[i] += E.mu * gg * dq[b][i] + lp * E.a[q][i] * ad + c * E.a[b][i] * qd;
+                        }
+                        double* lq = lift + 3 * (size_t)E.node[q];
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) lq[i] += E.V * out[i];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (int el = 0; el < 64; ++el) {
+            if (!sok[el]) continue;
+            const T V = sV[el], mu = smu[el], lpe = slp[el], ce = sc[el];
+            for (int j = lane; j < 144; j += 64) {
+                // j = ((q*3 + i)*4 + b)*3 + k
+                const int k = j % 3;
+                const int b = (j / 3) & 3;
+                const int qi = j / 12;
+                const int q = qi / 3, i = qi % 3;
+                if (sdir[el][qi] | sdir[el][b * 3 + k]) continue;
+                const T gg = sg[el][q * 3 + 0] * sg[el][b * 3 + 0] + sg[el][q * 3 + 1] * sg[el][b * 3 + 1] +
+                             sg[el][q * 3 + 2] * sg[el][b * 3 + 2];
+                T v = lpe * sa[el][q * 3 + i] * sa[el][b * 3 + k] + ce * sa[el][q * 3 + k] * sa[el][b * 3 + i];
+                if (i == k) v += mu * gg;
+                K[srp[el][qi] + 3u * srank[el][q * 4 + b] + k] += V * v;
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ void set_dir_diag_kernel(uint32_t n_dir, const uint32_t* __restrict__ dofs,
+                                    const uint32_t* __restrict__ row_ptr, const uint16_t* __restrict__ nlow,
+                                    T* __restrict__ K) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_dir) return;
+    const uint32_t r = dofs[k];
+    const uint32_t node = r / 3, d = r % 3;
+    K[row_ptr[r] + 3u * nlow[node] + d] = (T)1;
+}
+
+template <typename T>
+__global__ void eliminate_kernel(uint32_t n, const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col_idx,
+                                 const uint8_t* __restrict__ isdir, T* __restrict__ K) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const bool dr = isdir[r];
+    for (uint32_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k) {
+        const uint32_t c = col_idx[k];
+        if (dr || isdir[c]) K[k] = (c == r) ? (T)1 : (T)0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CSR SpMV: LPR lanes per row, fixed-order lane reduction; optional fused
+// x.y partial per block and sentinel preset of the solve outputs
+// ---------------------------------------------------------------------------
+template <typename T, int LPR, bool DOT>
+__global__ __launch_bounds__(256) void spmv_kernel(uint32_t n, const uint32_t* __restrict__ rp,
+                                                   const uint32_t* __restrict__ ci, const T* __restrict__ val,
+                                                   const T* __restrict__ x, T* __restrict__ y,
+                                                   double* __restrict__ partials, T* __restrict__ sent0,
+                                                   T* __restrict__ sent1, const PcgState* __restrict__ st) {
+    if (st && st->done) return;
+    constexpr int RPB = 256 / LPR;
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t row = blk * RPB + threadIdx.x / LPR;
+    const int l = threadIdx.x % LPR;
+    T sum = 0;
+    if (row < n) {
+        const uint32_t e = rp[row + 1];
+        for (uint32_t k = rp[row] + l; k < e; k += LPR)
+            sum += __builtin_nontemporal_load(val + k) * x[__builtin_nontemporal_load(ci + k)];
+    }
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+    double part = 0.0;
+    if (row < n && l == 0) {
+        y[row] = sum;
+        if (sent0) {
+            sent0[row] = sentinel_value<T>();
+            sent1[row] = sentinel_value<T>();
+        }
+        if (DOT) part = (double)sum * (double)x[row];
+    }
+    if (DOT) {
+        const double bs = block_sum(part);
+        if (threadIdx.x == 0) partials[blk] = bs;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// natural-order ILU(0), node-blocked (B rows per node) and sync-free:
+// nodes are taken from a level-ordered queue through a ticket counter, so a
+// node only ever waits on nodes that a running or finished wave owns.
+// ---------------------------------------------------------------------------
+template <typename T, int S>
+__device__ __forceinline__ T pick(const T (&a)[S], int sl) {
+    T v = a[0];
+#pragma unroll
+    for (int s = 1; s < S; ++s)
+        if (sl == s) v = a[s];
+    return v;
+}
+template <typename T, int S>
+__device__ __forceinline__ void put(T (&a)[S], int sl, T v) {
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        if (sl == s) a[s] = v;
+}
+
+// One wave factorizes the B rows of one node (IKJ order, which subtracts the
+// terms of every L and U entry in ascending pivot order, as the reference
+// factorization does), reading the finished U rows of its lower neighbours.
+template <typename T, int B, int S>
+__global__ __launch_bounds__(64) void ilu_factor_kernel(uint32_t n_nodes, const uint32_t* __restrict__ order,
+                                                        const uint32_t* __restrict__ rp,
+                                                        const uint32_t* __restrict__ ci,
+                                                        const uint16_t* __restrict__ nlow, const T* __restrict__ A,
+                                                        T* LU, uint32_t* ticket, int* err) {
+    __shared__ uint32_t s_t;
+    if (threadIdx.x == 0) s_t = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t t = s_t;
+    if (t >= n_nodes) return;
+    const int lane = threadIdx.x;
+    const uint32_t node = order[t];
+    if (node == kNoNode) return;
+    const uint32_t r0 = B * node;
+    uint32_t rs[B];
+#pragma unroll
+    for (int d = 0; d < B; ++d) rs[d] = rp[r0 + d];
+    const uint32_t len = rp[r0 + 1] - rs[0];
+    const uint32_t p = B * (uint32_t)nlow[node];
+    T a[B][S];
+    uint32_t col[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const uint32_t j = lane + 64 * s;
+        const bool v = j < len;
+        col[s] = v ? ci[rs[0] + j] : 0xFFFFFFFFu;
+#pragma unroll
+        for (int d = 0; d < B; ++d) a[d][s] = v ? A[rs[d] + j] : (T)0;
+    }
+    for (uint32_t q = 0; q < p; q += B) {
+        const uint32_t m = ci[rs[0] + q] / B;
+        const uint32_t rm = rp[B * m];
+        const uint32_t lenm = rp[B * m + 1] - rm;
+        const uint32_t pm = B * (uint32_t)nlow[m];
+        int pos[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            pos[s] = -1;
+            const uint32_t j = lane + 64 * s;
+            if (j < len && j > q) {
+                uint32_t lo = pm, hi = lenm;
+                const uint32_t c = col[s];
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (ci[rm + mid] < c)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                if (lo < lenm && ci[rm + lo] == c) pos[s] = (int)lo;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < B; ++e) {
+            const uint32_t k = B * m + e;
+            const uint32_t rk = rp[k];
+            const T ukk = await_val(&LU[rk + pm + e], err);
+            const uint32_t jj = q + e;
+            const int owner = jj & 63, sl = jj >> 6;
+            T l[B];
+#pragma unroll
+            for (int d = 0; d < B; ++d) l[d] = __shfl(pick(a[d], sl), owner) / ukk;
+            if (lane == owner)
+#pragma unroll
+                for (int d = 0; d < B; ++d) put(a[d], sl, l[d]);
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                if (pos[s] >= 0 && (uint32_t)(lane + 64 * s) > jj) {
+                    const T ukj = await_val(&LU[rk + pos[s]], err);
+#pragma unroll
+                    for (int d = 0; d < B; ++d) a[d][s] -= l[d] * ukj;
+                }
+            }
+        }
+    }
+    // diagonal block of the node itself
+#pragma unroll
+    for (int d = 0; d < B; ++d) {
+#pragma unroll
+        for (int e = 0; e < d; ++e) {
+            const uint32_t jj = p + e;
+            const int owner = jj & 63, sl = jj >> 6;
+            const T ue = __shfl(pick(a[e], sl), owner);
+            const T ld = __shfl(pick(a[d], sl), owner) / ue;
+            if (lane == owner) put(a[d], sl, ld);
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                if ((uint32_t)(lane + 64 * s) > jj && lane + 64 * s < (int)len) a[d][s] -= ld * a[e][s];
+        }
+        const uint32_t jd = p + d;
+        if (lane == (int)(jd & 63)) {
+            T u = pick(a[d], jd >> 6);
+            if (fabs((double)u) < 1e-12) u += (u >= (T)0) ? (T)1e-8 : (T)-1e-8;
+            put(a[d], jd >> 6, u);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const uint32_t j = lane + 64 * s;
+        if (j < len)
+#pragma unroll
+            for (int d = 0; d < B; ++d) publish(&LU[rs[d] + j], a[d][s]);
+    }
+}
+
+// forward sweep z = L^{-1} rhs; with UPD the PCG update r -= alpha Ap,
+// x += alpha p of the node's rows is fused in and the new r is the rhs
+template <typename T, int B, int LPN, bool UPD>
+__global__ __launch_bounds__(64) void ilu_fwd_kernel(uint32_t n_nodes, const uint32_t* __restrict__ order,
+                                                     const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+                                                     const uint16_t* __restrict__ nlow, const T* __restrict__ LU,
+                                                     const T* __restrict__ rhs, T* __restrict__ r, T* __restrict__ x,
+                                                     const T* __restrict__ p, const T* __restrict__ Ap, T* z,
+                                                     PcgState* st, int* err) {
+    if (UPD && st->done) return;
+    __shared__ uint32_t s_t;
+    if (threadIdx.x == 0) s_t = atomicAdd(&st->ticket[0], 1u);
+    __syncthreads();
+    constexpr int NPW = 64 / LPN;
+    const int lane = threadIdx.x, gl = lane % LPN;
+    const uint32_t idx = s_t * NPW + lane / LPN;
+    const uint32_t entry = idx < n_nodes ? order[idx] : kNoNode;
+    const bool valid = entry != kNoNode;
+    const uint32_t node = valid ? entry : 0;
+    const uint32_t r0 = B * node;
+    const uint32_t pl = valid ? B * (uint32_t)nlow[node] : 0;
+    T acc[B];
+#pragma unroll
+    for (int d = 0; d < B; ++d) acc[d] = 0;
+    if (valid) {
+        uint32_t rs[B];
+#pragma unroll
+        for (int d = 0; d < B; ++d) rs[d] = rp[r0 + d];
+        for (uint32_t j = gl; j < pl; j += LPN) {
+            const uint32_t c = ci[rs[0] + j];
+            const T zc = await_val(&z[c], err);
+#pragma unroll
+            for (int d = 0; d < B; ++d) acc[d] += LU[rs[d] + j] * zc;
+        }
+    }
+#pragma unroll
+    for (int o = LPN / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int d = 0; d < B; ++d) acc[d] += __shfl_xor(acc[d], o);
+    if (valid && gl == 0) {
+        T v[B];
+        T alpha = 0;
+        if (UPD) alpha = (T)st->alpha;
+#pragma unroll
+        for (int d = 0; d < B; ++d) {
+            T rhs_d;
+            if (UPD) {
+                const T rr = r[r0 + d] - alpha * Ap[r0 + d];
+                r[r0 + d] = rr;
+                x[r0 + d] += alpha * p[r0 + d];
+                rhs_d = rr;
+            } else {
+                rhs_d = rhs[r0 + d];
+            }
+            T s = rhs_d - acc[d];
+            const uint32_t rd = rp[r0 + d] + pl;
+#pragma unroll
+            for (int e = 0; e < d; ++e) s -= LU[rd + e] * v[e];
+            v[d] = s;
+        }
+#pragma unroll
+        for (int d = 0; d < B; ++d) publish(&z[r0 + d], v[d]);
+    }
+}
+
+// backward sweep y = U^{-1} z with the diagonal u_ii at the diagonal
+// positions; fused partial of r.y per wave (indexed by ticket, fixed order)
+template <typename T, int B, int LPN>
+__global__ __launch_bounds__(64) void ilu_bwd_kernel(uint32_t n_nodes, const uint32_t* __restrict__ order,
+                                                     const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+                                                     const uint16_t* __restrict__ nlow, const T* __restrict__ LU,
+                                                     const T* __restrict__ z, const T* __restrict__ r, T* y,
+                                                     double* __restrict__ partials, PcgState* st, int check_done,
+                                                     int* err) {
+    if (check_done && st->done) return;
+    __shared__ uint32_t s_t;
+    if (threadIdx.x == 0) s_t = atomicAdd(&st->ticket[1], 1u);
+    __syncthreads();
+    constexpr int NPW = 64 / LPN;
+    const int lane = threadIdx.x, gl = lane % LPN;
+    const uint32_t idx = s_t * NPW + lane / LPN;
+    const uint32_t entry = idx < n_nodes ? order[idx] : kNoNode;
+    const bool valid = entry != kNoNode;
+    const uint32_t node = valid ? entry : 0;
+    const uint32_t r0 = B * node;
+    const uint32_t pl = valid ? B * (uint32_t)nlow[node] : 0;
+    T acc[B];
+#pragma unroll
+    for (int d = 0; d < B; ++d) acc[d] = 0;
+    if (valid) {
+        uint32_t rs[B];
+#pragma unroll
+        for (int d = 0; d < B; ++d) rs[d] = rp[r0 + d];
+        const uint32_t len = rp[r0 + 1] - rs[0];
+        for (uint32_t j = pl + B + gl; j < len; j += LPN) {
+            const uint32_t c = ci[rs[0] + j];
+            const T yc = await_val(&y[c], err);
+#pragma unroll
+            for (int d = 0; d < B; ++d) acc[d] += LU[rs[d] + j] * yc;
+        }
+    }
+#pragma unroll
+    for (int o = LPN / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int d = 0; d < B; ++d) acc[d] += __shfl_xor(acc[d], o);
+    double part = 0.0;
+    if (valid && gl == 0) {
+        T v[B];
+#pragma unroll
+        for (int d = B - 1; d >= 0; --d) {
+            const uint32_t rd = rp[r0 + d] + pl;
+            T s = z[r0 + d] - acc[d];
+#pragma unroll
+            for (int e = d + 1; e < B; ++e) s -= LU[rd + e] * v[e];
+            v[d] = s / LU[rd + d];
+        }
+#pragma unroll
+        for (int d = 0; d < B; ++d) {
+            publish(&y[r0 + d], v[d]);
+            if (partials) part += (double)r[r0 + d] * (double)v[d];
+        }
+    }
+    if (partials) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+        if (lane == 0) partials[s_t] = part;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PCG scalar bookkeeping: one block, fixed-order sum of the partials
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void reduce_kernel(const double* __restrict__ partials, uint32_t np,
+                                                      PcgState* st, int op, int k) {
+    double s = 0.0;
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) s += partials[i];
+    s = block_sum(s);
+    if (threadIdx.x != 0) return;
+    switch (op) {
+        case 0:
+            st->slot[k] = s;
+            break;
+        case 1:  // p.Ap -> alpha (reference: break when |pAp| < 1e-60)
+            if (st->done) return;
+            st->pAp = s;
+            st->ticket[0] = 0;
+            st->ticket[1] = 0;
+            if (fabs(s) < 1e-60) {
+                st->done = 1;
+                st->status = 2;
+            } else {
+                st->alpha = st->rz / s;
+            }
+            break;
+        case 2: {  // r.z -> stopping rule sqrt(rz/bb) < tol, beta
+            if (st->done) return;
+            st->rz_new = s;
+            st->iters += 1;
+            const double rel = sqrt(s / fmax(st->bb, 1e-300));
+            st->relres = rel;
+            if (rel < st->tol) {
+                st->done = 1;
+                st->status = 1;
+            } else if (st->iters >= st->max_it) {
+                st->done = 1;
+                st->status = 3;
+            } else {
+                st->beta = s / st->rz;
+                st->rz = s;
+            }
+            break;
+        }
+        case 3:
+            st->rz = s;
+            st->relres = sqrt(s / fmax(st->bb, 1e-300));
+            break;
+    }
+}
+
+template <typename T>
+__global__ void pcg_p_update_kernel(uint32_t n, const T* __restrict__ y, T* __restrict__ p,
+                                    const PcgState* __restrict__ st) {
+    if (st->done) return;
+    const uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T beta = (T)st->beta;
+    p[i] = y[i] + beta * p[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void pcg_init_r_kernel(uint32_t n, const T* __restrict__ b, const T* __restrict__ Ap,
+                                                         T* __restrict__ r, double* __restrict__ partials) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double part = 0.0;
+    if (i < n) {
+        const T v = b[i] - Ap[i];
+        r[i] = v;
+        part = (double)v * (double)v;
+    }
+    const double s = block_sum(part);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ void copy_kernel(uint32_t n, const T* __restrict__ s, T* __restrict__ d) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+
+template <typename T>
+__global__ void fill_kernel(T* __restrict__ p, size_t n, T v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+template <typename T>
+__global__ void fill_sentinel_kernel(T* __restrict__ a, T* __restrict__ b, size_t n) {
+    const T s = sentinel_value<T>();
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        a[i] = s;
+        if (b) b[i] = s;
+    }
+}
+
+__global__ void expand_pattern_kernel(uint32_t nn, const uint32_t* __restrict__ nbr_ptr,
+                                      const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ row_ptr,
+                                      uint32_t* __restrict__ col_idx) {
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    const uint32_t b = nbr_ptr[n], e = nbr_ptr[n + 1];
+    for (int d = 0; d < 3; ++d) {
+        uint32_t k = row_ptr[3 * n + d];
+        for (uint32_t q = b; q < e; ++q) {
+            const uint32_t m = 3 * nbr[q];
+            col_idx[k++] = m;
+            col_idx[k++] = m + 1;
+            col_idx[k++] = m + 2;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void residual_finish_kernel(uint32_t n, double load,
+                                                              const double* __restrict__ f_ext,
+                                                              const double* __restrict__ f,
+                                                              const uint8_t* __restrict__ isdir,
+                                                              double* __restrict__ R, double* __restrict__ partials) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double part = 0.0;
+    if (i < n) {
+        const double v = (isdir && isdir[i]) ? 0.0 : (f_ext ? load * f_ext[i] : 0.0) - f[i];
+        R[i] = v;
+        part = v * v;
+    }
+    const double s = block_sum(part);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void dot64_kernel(uint32_t n, const double* __restrict__ a,
+                                                    const double* __restrict__ b, double* __restrict__ partials) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double part = (i < n) ? a[i] * b[i] : 0.0;
+    const double s = block_sum(part);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void f64_to_f32_kernel(uint32_t n, const double* __restrict__ s, double scale, float* __restrict__ d) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (float)(s[i] * scale);
+}
+
+__global__ void f32_to_f64_kernel(uint32_t n, const float* __restrict__ s, double scale, double* __restrict__ d) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (double)s[i] * scale;
+}
+
+__global__ void axpy_to_kernel(uint32_t n, const double* __restrict__ a, double alpha, const double* __restrict__ b,
+                               double* __restrict__ y) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a[i] + alpha * b[i];
+}
+
+__global__ void dir_increment_kernel(uint32_t n_dir, const uint32_t* __restrict__ dofs,
+                                     const double* __restrict__ vals, double scale, const double* __restrict__ u,
+                                     double* __restrict__ dD) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_dir) return;
+    const uint32_t d = dofs[k];
+    dD[d] = scale * (vals ? vals[k] : 0.0) - u[d];
+}
+
+__global__ void sub_masked_kernel(uint32_t n, const double* __restrict__ R, const double* __restrict__ lift,
+                                  const uint8_t* __restrict__ isdir, double* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = isdir[i] ? 0.0 : R[i] - lift[i];
+}
+
+__global__ void apply_predictor_kernel(uint32_t n, const double* __restrict__ du, const double* __restrict__ dD,
+                                       const uint8_t* __restrict__ isdir, double* __restrict__ u) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u[i] += isdir[i] ? dD[i] : du[i];
+}
+
+// dL/dmu_e = -lam_e . V (F g - F^{-T} g), dL/dlam_e = -lam_e . V lnJ F^{-T} g
+__global__ void adjoint_elem_kernel(ElemArgs A, const uint32_t* __restrict__ eperm, const double* __restrict__ lv,
+                                    double* __restrict__ dmu, double* __restrict__ dlam) {
+    const uint32_t t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= A.ne) return;
+    Elem E;
+    elem_eval(A, t, E);
+    double dm = 0.0, dl = 0.0;
+    if (E.ok) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const double l = lv[3 * (size_t)E.node[q] + i];
+                dm += l * E.V * (E.Fg[q][i] - E.a[q][i]);
+                dl += l * E.V * E.lnJ * E.a[q][i];
+            }
+    }
+    const uint32_t e = eperm[t];
+    if (dmu) dmu[e] = -dm;
+    if (dlam) dlam[e] = -dl;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 rotation part: polar factor by the scaled Newton iteration
+// X <- (zeta X + X^{-T}/zeta)/2 (Higham 1986), angle-axis logarithm with the
+// symmetric-part branch near pi, Rodrigues exponential with
+// (1 - cos t)/t^2 = (sin(t/2)/(t/2))^2 / 2 to avoid cancellation.
+// ---------------------------------------------------------------------------
+__device__ int polar_dev(const double* F, double* R) {
+    double X[9];
+    for (int k = 0; k < 9; ++k) X[k] = F[k];
+    for (int it = 0; it < 60; ++it) {
+        const double det = det3(X);
+        if (fabs(det) < 1e-300) return 1;
+        const double id = 1.0 / det;
+        double XiT[9];  // X^{-T} = cof(X) / det
+        XiT[0] = (X[4] * X[8] - X[5] * X[7]) * id;
+        XiT[1] = (X[5] * X[6] - X[3] * X[8]) * id;
+        XiT[2] = (X[3] * X[7] - X[4] * X[6]) * id;
+        XiT[3] = (X[2] * X[7] - X[1] * X[8]) * id;
+        XiT[4] = (X[0] * X[8] - X[2] * X[6]) * id;
+        XiT[5] = (X[1] * X[6] - X[0] * X[7]) * id;
+        XiT[6] = (X[1] * X[5] - X[2] * X[4]) * id;
+        XiT[7] = (X[2] * X[3] - X[0] * X[5]) * id;
+        XiT[8] = (X[0] * X[4] - X[1] * X[3]) * id;
+        double nx = 0.0, ni = 0.0;
+        for (int k = 0; k < 9; ++k) {
+            nx += X[k] * X[k];
+            ni += XiT[k] * XiT[k];
+        }
+        const double zeta = (it < 8) ? sqrt(sqrt(ni / nx)) : 1.0;
+        double diff = 0.0;
+        for (int k = 0; k < 9; ++k) {
+            const double xn = 0.5 * (zeta * X[k] + XiT[k] / zeta);
+            diff += (xn - X[k]) * (xn - X[k]);
+            X[k] = xn;
+        }
+        if (it >= 8 && diff < 1e-30) break;
+    }
+    for (int k = 0; k < 9; ++k) R[k] = X[k];
+    return det3(R) > 0 ? 0 : 2;
+}
+
+__device__ void rot_log_dev(const double* R, double* w) {
+    const double c = 0.5 * (R[0] + R[4] + R[8] - 1.0);
+    const double v0 = 0.5 * (R[7] - R[5]), v1 = 0.5 * (R[2] - R[6]), v2 = 0.5 * (R[3] - R[1]);
+    const double s = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    const double th = atan2(s, c);
+    if (th < 1e-4) {
+        const double t2 = th * th;
+        const double f = 1.0 + t2 / 6.0 + 7.0 * t2 * t2 / 360.0;
+        w[0] = f * v0;
+        w[1] = f * v1;
+        w[2] = f * v2;
+        return;
+    }
+    if (c > -0.9) {
+        const double f = th / s;
+        w[0] = f * v0;
+        w[1] = f * v1;
+        w[2] = f * v2;
+        return;
+    }
+    double B[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) B[i * 3 + j] = 0.5 * (R[i * 3 + j] + R[j * 3 + i]) - (i == j ? c : 0.0);
+    int m = 0;
+    if (B[4] > B[0]) m = 1;
+    if (B[8] > B[m * 4]) m = 2;
+    const double n0 = B[0 * 3 + m], n1 = B[1 * 3 + m], n2 = B[2 * 3 + m];
+    const double nrm = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    const double sgn = (n0 * v0 + n1 * v1 + n2 * v2) < 0 ? -1.0 : 1.0;
+    w[0] = sgn * th * n0 / nrm;
+    w[1] = sgn * th * n1 / nrm;
+    w[2] = sgn * th * n2 / nrm;
+}
+
+__device__ void rot_exp_dev(const double* w, double* R) {
+    const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    const double th = sqrt(th2);
+    double a, b;
+    if (th < 1e-6) {
+        a = 1.0 - th2 / 6.0;
+        b = 0.5 - th2 / 24.0;
+    } else {
+        a = sin(th) / th;
+        const double h = sin(0.5 * th) / (0.5 * th);
+        b = 0.5 * h * h;
+    }
+    const double K[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double k2 = 0.0;
+            for (int k = 0; k < 3; ++k) k2 += K[i * 3 + k] * K[k * 3 + j];
+            R[i * 3 + j] = (i == j ? 1.0 : 0.0) + a * K[i * 3 + j] + b * k2;
+        }
+}
+
+__global__ void rot_batch_kernel(uint32_t n, const double* __restrict__ F, double alpha, double* __restrict__ Rout,
+                                 double* __restrict__ wout, double* __restrict__ Qout) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double Fi[9], R[9], w[3];
+    for (int k = 0; k < 9; ++k) Fi[k] = F[9 * (size_t)i + k];
+    polar_dev(Fi, R);
+    rot_log_dev(R, w);
+    if (Rout)
+        for (int k = 0; k < 9; ++k) Rout[9 * (size_t)i + k] = R[k];
+    if (wout)
+        for (int k = 0; k < 3; ++k) wout[3 * (size_t)i + k] = w[k];
+    if (Qout) {
+        double aw[3] = {alpha * w[0], alpha * w[1], alpha * w[2]};
+        double Q[9];
+        rot_exp_dev(aw, Q);
+        for (int k = 0; k < 9; ++k) Qout[9 * (size_t)i + k] = Q[k];
+    }
+}
+
+__global__ void elem_rot_kernel(ElemArgs A, const uint32_t* __restrict__ eperm, double* __restrict__ wout) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.ne) return;
+    Elem E;
+    double F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    elem_eval(A, t, E, F);
+    double R[9], w[3] = {0, 0, 0};
+    if (E.ok && polar_dev(F, R) == 0) rot_log_dev(R, w);
+    const uint32_t e = eperm[t];
+    for (int k = 0; k < 3; ++k) wout[3 * (size_t)e + k] = w[k];
+}
+
+inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace
+
+// ===========================================================================
+// launchers
+// ===========================================================================
+void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v) {
+    if (n) fill_kernel<uint8_t><<<std::min<uint64_t>(cdiv(n, 256), 65535), 256, 0, s>>>(p, n, v);
+}
+void launch_fill_f64(hipStream_t s, double* p, size_t n, double v) {
+    if (n) fill_kernel<double><<<std::min<uint64_t>(cdiv(n, 256), 65535), 256, 0, s>>>(p, n, v);
+}
+void launch_fill_f32(hipStream_t s, float* p, size_t n, float v) {
+    if (n) fill_kernel<float><<<std::min<uint64_t>(cdiv(n, 256), 65535), 256, 0, s>>>(p, n, v);
+}
+
+void launch_expand_pattern(hipStream_t s, uint32_t nn, const uint32_t* nbr_ptr, const uint32_t* nbr,
+                           const uint32_t* row_ptr, uint32_t* col_idx) {
+    expand_pattern_kernel<<<cdiv(nn, 256), 256, 0, s>>>(nn, nbr_ptr, nbr, row_ptr, col_idx);
+}
+
+void launch_assemble_color(hipStream_t s, int prec, bool with_K, const ElemArgs& a, uint32_t begin, uint32_t end,
+                           double* f, void* K, const uint32_t* row_ptr, const uint8_t* isdir, const double* dD,
+                           double* lift) {
+    if (end <= begin) return;
+    const uint32_t nb = cdiv(end - begin, 64);
+    if (!with_K)
+        assemble_color_kernel<double, false><<<nb, 64, 0, s>>>(a, begin, end, f, nullptr, row_ptr, isdir, nullptr,
+                                                                nullptr);
+    else if (prec == PREC64)
+        assemble_color_kernel<double, true><<<nb, 64, 0, s>>>(a, begin, end, f, (double*)K, row_ptr, isdir, dD, lift);
+    else
+        assemble_color_kernel<float, true><<<nb, 64, 0, s>>>(a, begin, end, f, (float*)K, row_ptr, isdir, dD, lift);
+}
+
+void launch_set_dir_diag(hipStream_t s, int prec, uint32_t n_dir, const uint32_t* dofs, const uint32_t* row_ptr,
+                         const uint16_t* nlow, void* K) {
+    if (!n_dir) return;
+    if (prec == PREC64)
+        set_dir_diag_kernel<double><<<cdiv(n_dir, 256), 256, 0, s>>>(n_dir, dofs, row_ptr, nlow, (double*)K);
+    else
+        set_dir_diag_kernel<float><<<cdiv(n_dir, 256), 256, 0, s>>>(n_dir, dofs, row_ptr, nlow, (float*)K);
+}
+
+void launch_eliminate(hipStream_t s, int prec, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
+                      const uint8_t* isdir, void* K) {
+    if (prec == PREC64)
+        eliminate_kernel<double><<<cdiv(n, 256), 256, 0, s>>>(n, row_ptr, col_idx, isdir, (double*)K);
+    else
+        eliminate_kernel<float><<<cdiv(n, 256), 256, 0, s>>>(n, row_ptr, col_idx, isdir, (float*)K);
+}
+
+uint32_t launch_residual_finish(hipStream_t s, uint32_t n, double load, const double* f_ext, const double* f,
+                                const uint8_t* isdir, double* R, double* partials) {
+    const uint32_t nb = cdiv(n, 256);
+    residual_finish_kernel<<<nb, 256, 0, s>>>(n, load, f_ext, f, isdir, R, partials);
+    return nb;
+}
+
+uint32_t launch_dot64(hipStream_t s, uint32_t n, const double* a, const double* b, double* partials) {
+    const uint32_t nb = cdiv(n, 256);
+    dot64_kernel<<<nb, 256, 0, s>>>(n, a, b, partials);
+    return nb;
+}
+
+void launch_reduce(hipStream_t s, const double* partials, uint32_t np, PcgState* st, int op, int k) {
+    reduce_kernel<<<1, 1024, 0, s>>>(partials, np, st, op, k);
+}
+
+static int spmv_lpr(const Matrix& A) {
+    const double avg = A.n ? (double)A.nnz / A.n : 1.0;
+    if (avg <= 6) return 4;
+    if (avg <= 14) return 8;
+    if (avg <= 40) return 16;
+    if (avg <= 64) return 16;
+    return 32;
+}
+
+uint32_t spmv_blocks(const Matrix& A) { return cdiv(A.n, 256 / spmv_lpr(A)); }
+
+template <typename T>
+static void spmv_t(hipStream_t s, const Matrix& A, const T* val, const T* x, T* y, bool dot, double* partials,
+                   T* s0, T* s1, const PcgState* st) {
+    const int lpr = spmv_lpr(A);
+    const uint32_t nb = cdiv(A.n, 256 / lpr);
+#define FM_SPMV(L)                                                                                         \
+    if (dot)                                                                                               \
+        spmv_kernel<T, L, true><<<nb, 256, 0, s>>>(A.n, A.row_ptr.p, A.col_idx.p, val, x, y, partials, s0, \
+                                                   s1, st);                                                \
+    else                                                                                                   \
+        spmv_kernel<T, L, false><<<nb, 256, 0, s>>>(A.n, A.row_ptr.p, A.col_idx.p, val, x, y, partials, s0, s1, st);
+    switch (lpr) {
+        case 4:
+            FM_SPMV(4);
+            break;
+        case 8:
+            FM_SPMV(8);
+            break;
+        case 16:
+            FM_SPMV(16);
+            break;
+        default:
+            FM_SPMV(32);
+            break;
+    }
+#undef FM_SPMV
+}
+
+void launch_spmv(hipStream_t s, int prec, const Matrix& A, const void* x, void* y, bool dot, double* partials,
+                 void* sent0, void* sent1, const PcgState* st) {
+    if (prec == PREC64)
+        spmv_t<double>(s, A, A.val64.p, (const double*)x, (double*)y, dot, partials, (double*)sent0, (double*)sent1,
+                       st);
+    else
+        spmv_t<float>(s, A, A.val32.p, (const float*)x, (float*)y, dot, partials, (float*)sent0, (float*)sent1, st);
+}
+
+template <typename T, int B>
+static void factor_t(hipStream_t s, const Matrix& A, const T* val, T* lu, PcgState* st, int* err) {
+    const Schedule& S = A.sched;
+    fill_sentinel_kernel<T><<<std::min<uint64_t>(cdiv(A.nnz, 256), 65535), 256, 0, s>>>(lu, nullptr, A.nnz);
+    const uint32_t nb = S.q_fwd;
+    uint32_t* tk = &st->ticket[2];
+    switch (S.slots) {
+        case 1:
+            ilu_factor_kernel<T, B, 1><<<nb, 64, 0, s>>>(nb, S.fwd_order.p, A.row_ptr.p, A.col_idx.p, S.nlow.p, val,
+                                                         lu, tk, err);
+            break;
+        case 2:
+            ilu_factor_kernel<T, B, 2><<<nb, 64, 0, s>>>(nb, S.fwd_order.p, A.row_ptr.p, A.col_idx.p, S.nlow.p, val,
+                                                         lu, tk, err);
+            break;
+        default:
+            ilu_factor_kernel<T, B, 4><<<nb, 64, 0, s>>>(nb, S.fwd_order.p, A.row_ptr.p, A.col_idx.p, S.nlow.p, val,
+                                                         lu, tk, err);
+            break;
+    }
+}
+
+void launch_ilu_factor(hipStream_t s, int prec, const Matrix& A, const void* val, void* lu, PcgState* st, int* err) {
+    if (prec == PREC64) {
+        if (A.sched.B == 3)
+            factor_t<double, 3>(s, A, (const double*)val, (double*)lu, st, err);
+        else
+            factor_t<double, 1>(s, A, (const double*)val, (double*)lu, st, err);
+    } else {
+        if (A.sched.B == 3)
+            factor_t<float, 3>(s, A, (const float*)val, (float*)lu, st, err);
+        else
+            factor_t<float, 1>(s, A, (const float*)val, (float*)lu, st, err);
+    }
+}
+
+template <typename T, int B, int LPN>
+static void fwd_t(hipStream_t s, const Matrix& A, const T* lu, const T* rhs, T* r, T* x, const T* p, const T* Ap,
+                  T* z, PcgState* st, bool upd, int* err) {
+    const Schedule& S = A.sched;
+    const uint32_t nb = cdiv(S.q_fwd, 64 / LPN);
+    if (upd)
+        ilu_fwd_kernel<T, B, LPN, true><<<nb, 64, 0, s>>>(S.q_fwd, S.fwd_order.p, A.row_ptr.p, A.col_idx.p,
+                                                          S.nlow.p, lu, rhs, r, x, p, Ap, z, st, err);
+    else
+        ilu_fwd_kernel<T, B, LPN, false><<<nb, 64, 0, s>>>(S.q_fwd, S.fwd_order.p, A.row_ptr.p, A.col_idx.p,
+                                                           S.nlow.p, lu, rhs, r, x, p, Ap, z, st, err);
+}
+
+template <typename T>
+static void fwd_dispatch(hipStream_t s, const Matrix& A, const T* lu, const T* rhs, T* r, T* x, const T* p,
+                         const T* Ap, T* z, PcgState* st, bool upd, int* err) {
+    if (A.sched.B == 3) {
+        if (A.sched.lpn == 16)
+            fwd_t<T, 3, 16>(s, A, lu, rhs, r, x, p, Ap, z, st, upd, err);
+        else
+            fwd_t<T, 3, 4>(s, A, lu, rhs, r, x, p, Ap, z, st, upd, err);
+    } else {
+        if (A.sched.lpn == 16)
+            fwd_t<T, 1, 16>(s, A, lu, rhs, r, x, p, Ap, z, st, upd, err);
+        else
+            fwd_t<T, 1, 4>(s, A, lu, rhs, r, x, p, Ap, z, st, upd, err);
+    }
+}
+
+void launch_ilu_fwd(hipStream_t s, int prec, const Matrix& A, const void* lu, const void* rhs, void* r, void* x,
+                    const void* p, const void* Ap, void* z, PcgState* st, bool upd, int* err) {
+    if (prec == PREC64)
+        fwd_dispatch<double>(s, A, (const double*)lu, (const double*)rhs, (double*)r, (double*)x, (const double*)p,
+                             (const double*)Ap, (double*)z, st, upd, err);
+    else
+        fwd_dispatch<float>(s, A, (const float*)lu, (const float*)rhs, (float*)r, (float*)x, (const float*)p,
+                            (const float*)Ap, (float*)z, st, upd, err);
+}
+
+template <typename T, int B, int LPN>
+static uint32_t bwd_t(hipStream_t s, const Matrix& A, const T* lu, const T* z, const T* r, T* y, double* partials,
+                      PcgState* st, bool check_done, int* err) {
+    const Schedule& S = A.sched;
+    const uint32_t nb = cdiv(S.q_bwd, 64 / LPN);
+    ilu_bwd_kernel<T, B, LPN><<<nb, 64, 0, s>>>(S.q_bwd, S.bwd_order.p, A.row_ptr.p, A.col_idx.p, S.nlow.p, lu, z, r,
+                                               y, partials, st, check_done ? 1 : 0, err);
+    return nb;
+}
+
+template <typename T>
+static uint32_t bwd_dispatch(hipStream_t s, const Matrix& A, const T* lu, const T* z, const T* r, T* y,
+                             double* partials, PcgState* st, bool check_done, int* err) {
+    if (A.sched.B == 3)
+        return A.sched.lpn == 16 ? bwd_t<T, 3, 16>(s, A, lu, z, r, y, partials, st, check_done, err)
+                                 : bwd_t<T, 3, 4>(s, A, lu, z, r, y, partials, st, check_done, err);
+    return A.sched.lpn == 16 ? bwd_t<T, 1, 16>(s, A, lu, z, r, y, partials, st, check_done, err)
+                             : bwd_t<T, 1, 4>(s, A, lu, z, r, y, partials, st, check_done, err);
+}
+
+uint32_t launch_ilu_bwd(hipStream_t s, int prec, const Matrix& A, const void* lu, const void* z, const void* r,
+                        void* y, double* partials, PcgState* st, bool check_done, int* err) {
+    if (prec == PREC64)
+        return bwd_dispatch<double>(s, A, (const double*)lu, (const double*)z, (const double*)r, (double*)y, partials,
+                                    st, check_done, err);
+    return bwd_dispatch<float>(s, A, (const float*)lu, (const float*)z, (const float*)r, (float*)y, partials, st,
+                               check_done, err);
+}
+
+void launch_fill_sentinel(hipStream_t s, int prec, void* a, void* b, size_t n) {
+    const uint32_t nb = std::min<uint64_t>(cdiv(n, 256), 65535);
+    if (prec == PREC64)
+        fill_sentinel_kernel<double><<<nb, 256, 0, s>>>((double*)a, (double*)b, n);
+    else
+        fill_sentinel_kernel<float><<<nb, 256, 0, s>>>((float*)a, (float*)b, n);
+}
+
+void launch_pcg_p_update(hipStream_t s, int prec, uint32_t n, const void* y, void* p, const PcgState* st) {
+    if (prec == PREC64)
+        pcg_p_update_kernel<double><<<cdiv(n, 256), 256, 0, s>>>(n, (const double*)y, (double*)p, st);
+    else
+        pcg_p_update_kernel<float><<<cdiv(n, 256), 256, 0, s>>>(n, (const float*)y, (float*)p, st);
+}
+
+uint32_t launch_pcg_init_r(hipStream_t s, int prec, uint32_t n, const void* b, const void* Ap, void* r,
+                           double* partials) {
+    const uint32_t nb = cdiv(n, 256);
+    if (prec == PREC64)
+        pcg_init_r_kernel<double><<<nb, 256, 0, s>>>(n, (const double*)b, (const double*)Ap, (double*)r, partials);
+    else
+        pcg_init_r_kernel<float><<<nb, 256, 0, s>>>(n, (const float*)b, (const float*)Ap, (float*)r, partials);
+    return nb;
+}
+
+void launch_copy(hipStream_t s, int prec, uint32_t n, const void* src, void* dst) {
+    if (prec == PREC64)
+        copy_kernel<double><<<cdiv(n, 256), 256, 0, s>>>(n, (const double*)src, (double*)dst);
+    else
+        copy_kernel<float><<<cdiv(n, 256), 256, 0, s>>>(n, (const float*)src, (float*)dst);
+}
+
+void launch_f64_to_f32(hipStream_t s, uint32_t n, const double* src, double scale, float* dst) {
+    f64_to_f32_kernel<<<cdiv(n, 256), 256, 0, s>>>(n, src, scale, dst);
+}
+void launch_f32_to_f64(hipStream_t s, uint32_t n, const float* src, double scale, double* dst) {
+    f32_to_f64_kernel<<<cdiv(n, 256), 256, 0, s>>>(n, src, scale, dst);
+}
+void launch_axpy_to(hipStream_t s, uint32_t n, const double* a, double alpha, const double* b, double* y) {
+    axpy_to_kernel<<<cdiv(n, 256), 256, 0, s>>>(n, a, alpha, b, y);
+}
+void launch_dir_increment(hipStream_t s, uint32_t n_dir, const uint32_t* dofs, const double* vals, double scale,
+                          const double* u, double* dD) {
+    if (n_dir) dir_increment_kernel<<<cdiv(n_dir, 256), 256, 0, s>>>(n_dir, dofs, vals, scale, u, dD);
+}
+void launch_sub_masked(hipStream_t s, uint32_t n, const double* R, const double* lift, const uint8_t* isdir,
+                       double* out) {
+    sub_masked_kernel<<<cdiv(n, 256), 256, 0, s>>>(n, R, lift, isdir, out);
+}
+void launch_apply_predictor(hipStream_t s, uint32_t n, const double* du, const double* dD, const uint8_t* isdir,
+                            double* u) {
+    apply_predictor_kernel<<<cdiv(n, 256), 256, 0, s>>>(n, du, dD, isdir, u);
+}
+void launch_adjoint_elem(hipStream_t s, const ElemArgs& a, const uint32_t* eperm, const double* lam_vec,
+                         double* dmu, double* dlam) {
+    adjoint_elem_kernel<<<cdiv(a.ne, 128), 128, 0, s>>>(a, eperm, lam_vec, dmu, dlam);
+}
+void launch_rot_batch(hipStream_t s, uint32_t n, const double* F, double alpha, double* R, double* w, double* Q) {
+    if (n) rot_batch_kernel<<<cdiv(n, 128), 128, 0, s>>>(n, F, alpha, R, w, Q);
+}
+void launch_elem_rot(hipStream_t s, const ElemArgs& a, const uint32_t* eperm, double* w) {
+    elem_rot_kernel<<<cdiv(a.ne, 128), 128, 0, s>>>(a, eperm, w);
+}
+
+}  // namespace fm

@@ -1,0 +1,327 @@
+"""ctypes binding of libflexmesh.so (the C ABI declared in include/flexmesh/fm_api.h).
+
+The binding has no fallback: if the HIP library is missing or no GPU is
+visible, loading raises.  Nothing here computes on the host beyond argument
+marshalling.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libflexmesh.so")
+
+FM_OK, FM_EINVAL, FM_ESTATE, FM_EHIP, FM_ENOCONV, FM_EDEVICE = 0, -1, -2, -3, -4, -5
+FM_FP64, FM_MIXED = 0, 1
+KERNEL_CLASSES = ["spmv", "ilu_fwd", "ilu_bwd", "ilu_factor", "assemble", "residual", "pcg_other"]
+
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+class NewtonCfg(C.Structure):
+    _fields_ = [("n_steps", C.c_uint32), ("prec", C.c_int), ("newton_rtol", C.c_double), ("max_newton", C.c_int),
+                ("pcg_tol", C.c_double), ("pcg_max", C.c_int), ("inner_tol", C.c_double), ("ls_max", C.c_int),
+                ("armijo_c", C.c_double), ("lift", C.c_int)]
+
+
+class NewtonStats(C.Structure):
+    _fields_ = [("converged", C.c_int), ("newton_iters", C.c_int), ("pcg_iters", C.c_int), ("assemblies", C.c_int),
+                ("residuals", C.c_int), ("factorizations", C.c_int), ("r0", C.c_double), ("rel_res", C.c_double),
+                ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class FlexMeshError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"flexmesh error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libflexmesh.so once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() (or make -C atlas-flexmesh_amd/csrc)")
+    L = C.CDLL(LIB_PATH)
+    L.fm_version.restype = C.c_char_p
+    L.fm_last_error.restype = C.c_char_p
+    L.fm_last_error.argtypes = [_vp]
+    L.fm_create.argtypes = [C.c_int, C.POINTER(_vp)]
+    L.fm_destroy.argtypes = [_vp]
+    L.fm_mesh_box.argtypes = [C.c_int] * 3 + [C.c_double] * 3 + [_f64p, _u32p]
+    L.fm_mesh_jitter.argtypes = [C.c_uint32, _f64p, _u8p, C.c_double, C.c_uint64]
+    L.fm_field_lognormal.argtypes = [C.c_uint32, C.c_double, C.c_double, C.c_uint64, _f64p]
+    L.fm_set_mesh.argtypes = [_vp, C.c_uint32, _f64p, C.c_uint32, _u32p]
+    L.fm_set_material.argtypes = [_vp, _f64p, _f64p]
+    L.fm_set_bcs.argtypes = [_vp, C.c_uint32, _vp, _vp]
+    L.fm_set_load.argtypes = [_vp, _vp]
+    L.fm_build_pattern.argtypes = [_vp, _vp, _vp, C.POINTER(C.c_uint64)]
+    L.fm_color.argtypes = [_vp, _vp, C.POINTER(C.c_uint32)]
+    L.fm_assemble.argtypes = [_vp, _vp, _vp, _vp, C.c_int]
+    L.fm_residual.argtypes = [_vp, _vp, C.c_double, _vp, C.POINTER(C.c_double)]
+    L.fm_set_matrix.argtypes = [_vp, C.c_uint32, _u32p, _u32p, _f64p]
+    L.fm_use_tangent.argtypes = [_vp, C.c_int]
+    L.fm_spmv.argtypes = [_vp, C.c_int, _f64p, _f64p]
+    L.fm_ilu0_factor.argtypes = [_vp, C.c_int, _vp]
+    L.fm_ilu0_apply.argtypes = [_vp, C.c_int, _f64p, _f64p]
+    L.fm_pcg.argtypes = [_vp, C.c_int, _f64p, _f64p, C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.fm_newton_cfg_default.argtypes = [C.POINTER(NewtonCfg)]
+    L.fm_newton_cfg_default.restype = None
+    L.fm_newton_step.argtypes = [_vp, C.c_uint32, _vp, C.POINTER(NewtonCfg), C.POINTER(NewtonStats)]
+    L.fm_get_u.argtypes = [_vp, _f64p]
+    L.fm_set_u.argtypes = [_vp, _f64p]
+    L.fm_adjoint.argtypes = [_vp, _f64p, C.c_double, C.c_int, _vp, _vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.fm_rot_logexp_batch.argtypes = [_vp, C.c_uint32, _f64p, C.c_double, _vp, _vp, _vp]
+    L.fm_element_rotations.argtypes = [_vp, _f64p]
+    L.fm_checkpoint_save.argtypes = [_vp, C.c_char_p, C.c_uint32]
+    L.fm_checkpoint_load.argtypes = [_vp, C.c_char_p, C.POINTER(C.c_uint32)]
+    L.fm_profile_enable.argtypes = [_vp, C.c_int]
+    L.fm_profile_reset.argtypes = [_vp]
+    L.fm_profile_get.argtypes = [_vp, _f64p, np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")]
+    L.fm_info.argtypes = [_vp, np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def mesh_box(nx, ny, nz, Lx=1.0, Ly=1.0, Lz=1.0):
+    nn = (nx + 1) * (ny + 1) * (nz + 1)
+    ne = 6 * nx * ny * nz
+    X = np.zeros(3 * nn)
+    conn = np.zeros(4 * ne, dtype=np.uint32)
+    rc = lib().fm_mesh_box(nx, ny, nz, Lx, Ly, Lz, X, conn)
+    if rc != FM_OK:
+        raise FlexMeshError(rc, "fm_mesh_box")
+    return X, conn
+
+
+def mesh_jitter(X, interior, amp, seed):
+    X = np.array(X, dtype=np.float64)
+    rc = lib().fm_mesh_jitter(X.size // 3, X, np.ascontiguousarray(interior, dtype=np.uint8), amp, seed)
+    if rc != FM_OK:
+        raise FlexMeshError(rc, "fm_mesh_jitter")
+    return X
+
+
+def field_lognormal(n, median, sigma, seed):
+    out = np.zeros(n)
+    rc = lib().fm_field_lognormal(n, median, sigma, seed, out)
+    if rc != FM_OK:
+        raise FlexMeshError(rc, "fm_field_lognormal")
+    return out
+
+
+class Solver:
+    """One fm_ctx: device-resident mesh, pattern, tangent, factors and state."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        h = _vp()
+        rc = self.L.fm_create(device, C.byref(h))
+        if rc != FM_OK:
+            raise FlexMeshError(rc, "fm_create failed (no HIP device visible?)")
+        self.h = h
+        self.nn = self.ne = 0
+
+    def close(self):
+        if self.h:
+            self.L.fm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != FM_OK:
+            msg = self.L.fm_last_error(self.h).decode()
+            raise FlexMeshError(rc, f"{what}: {msg}")
+
+    # ---- inputs ---------------------------------------------------------------
+    def set_mesh(self, X, conn):
+        self.X = f64(X)
+        self.conn = u32(conn)
+        self.nn, self.ne = self.X.size // 3, self.conn.size // 4
+        self._chk(self.L.fm_set_mesh(self.h, self.nn, self.X, self.ne, self.conn), "fm_set_mesh")
+
+    def set_material(self, mu, lam):
+        self._chk(self.L.fm_set_material(self.h, f64(mu), f64(lam)), "fm_set_material")
+
+    def set_bcs(self, dofs, vals=None):
+        d = u32(dofs)
+        v = None if vals is None else f64(vals)
+        self._keep_bc = (d, v)
+        self._chk(self.L.fm_set_bcs(self.h, d.size, _ptr(d), _ptr(v)), "fm_set_bcs")
+
+    def set_load(self, f_ext):
+        f = None if f_ext is None else f64(f_ext)
+        self._chk(self.L.fm_set_load(self.h, _ptr(f)), "fm_set_load")
+
+    # ---- pattern / colouring ------------------------------------------------
+    def pattern(self):
+        nnz = C.c_uint64(0)
+        self._chk(self.L.fm_build_pattern(self.h, None, None, C.byref(nnz)), "fm_build_pattern")
+        rp = np.zeros(3 * self.nn + 1, dtype=np.uint32)
+        ci = np.zeros(nnz.value, dtype=np.uint32)
+        self._chk(self.L.fm_build_pattern(self.h, rp.ctypes.data, ci.ctypes.data, C.byref(nnz)), "fm_build_pattern")
+        return rp, ci
+
+    def colors(self):
+        c = np.zeros(self.ne, dtype=np.uint32)
+        nc = C.c_uint32(0)
+        self._chk(self.L.fm_color(self.h, c.ctypes.data, C.byref(nc)), "fm_color")
+        return c, int(nc.value)
+
+    # ---- assembly ---------------------------------------------------------------
+    def assemble(self, u=None, prec=FM_FP64, want_K=True):
+        f = np.zeros(3 * self.nn)
+        K = None
+        if want_K:
+            nnz = C.c_uint64(0)
+            self._chk(self.L.fm_build_pattern(self.h, None, None, C.byref(nnz)), "fm_build_pattern")
+            K = np.zeros(nnz.value)
+        uu = None if u is None else f64(u)
+        self._chk(self.L.fm_assemble(self.h, _ptr(uu), f.ctypes.data, _ptr(K), prec), "fm_assemble")
+        return f, K
+
+    def residual(self, u=None, load=1.0):
+        R = np.zeros(3 * self.nn)
+        nr = C.c_double(0.0)
+        uu = None if u is None else f64(u)
+        self._chk(self.L.fm_residual(self.h, _ptr(uu), load, R.ctypes.data, C.byref(nr)), "fm_residual")
+        return R, nr.value
+
+    # ---- sparse linear algebra -------------------------------------------------
+    def set_matrix(self, row_ptr, col_idx, vals):
+        rp, ci, v = u32(row_ptr), u32(col_idx), f64(vals)
+        self._mat = (rp, ci, v)
+        self._chk(self.L.fm_set_matrix(self.h, rp.size - 1, rp, ci, v), "fm_set_matrix")
+        self._nrows = rp.size - 1
+        self._nnz = int(rp[-1])
+
+    def use_tangent(self, eliminate=True):
+        self._chk(self.L.fm_use_tangent(self.h, int(eliminate)), "fm_use_tangent")
+        info = self.info()
+        self._nrows, self._nnz = int(info["n"]), int(info["nnz"])
+
+    def spmv(self, x, prec=FM_FP64):
+        y = np.zeros(self._nrows)
+        self._chk(self.L.fm_spmv(self.h, prec, f64(x), y), "fm_spmv")
+        return y
+
+    def ilu0_factor(self, prec=FM_FP64):
+        LU = np.zeros(self._nnz)
+        self._chk(self.L.fm_ilu0_factor(self.h, prec, LU.ctypes.data), "fm_ilu0_factor")
+        return LU
+
+    def ilu0_apply(self, r, prec=FM_FP64):
+        z = np.zeros(self._nrows)
+        self._chk(self.L.fm_ilu0_apply(self.h, prec, f64(r), z), "fm_ilu0_apply")
+        return z
+
+    def pcg(self, b, x0=None, tol=1e-10, max_it=1000, prec=FM_FP64):
+        x = np.zeros(self._nrows) if x0 is None else np.array(x0, dtype=np.float64)
+        its = C.c_int(0)
+        rel = C.c_double(0.0)
+        self._chk(self.L.fm_pcg(self.h, prec, f64(b), x, tol, max_it, C.byref(its), C.byref(rel)), "fm_pcg")
+        return x, its.value, rel.value
+
+    # ---- Newton ----------------------------------------------------------------
+    @staticmethod
+    def newton_cfg(**kw):
+        cfg = NewtonCfg()
+        lib().fm_newton_cfg_default(C.byref(cfg))
+        for k, v in kw.items():
+            setattr(cfg, k, v)
+        return cfg
+
+    def newton_step(self, step, cfg, u=None, check=True):
+        st = NewtonStats()
+        uu = None if u is None else u
+        rc = self.L.fm_newton_step(self.h, step, _ptr(uu), C.byref(cfg), C.byref(st))
+        if check:
+            self._chk(rc, "fm_newton_step")
+        return rc, st.as_dict()
+
+    def get_u(self):
+        u = np.zeros(3 * self.nn)
+        self._chk(self.L.fm_get_u(self.h, u), "fm_get_u")
+        return u
+
+    def set_u(self, u):
+        self._chk(self.L.fm_set_u(self.h, f64(u)), "fm_set_u")
+
+    # ---- adjoint / rotations / checkpoint --------------------------------------
+    def adjoint(self, dLdu, tol=1e-12, max_it=5000):
+        dmu = np.zeros(self.ne)
+        dlam = np.zeros(self.ne)
+        dl = C.c_double(0.0)
+        its = C.c_int(0)
+        self._chk(self.L.fm_adjoint(self.h, f64(dLdu), tol, max_it, dmu.ctypes.data, dlam.ctypes.data, C.byref(dl),
+                                    C.byref(its)), "fm_adjoint")
+        return dmu, dlam, dl.value, its.value
+
+    def rot_logexp(self, F, alpha=1.0):
+        F = f64(F).reshape(-1, 9)
+        n = F.shape[0]
+        R = np.zeros((n, 9))
+        w = np.zeros((n, 3))
+        Q = np.zeros((n, 9))
+        self._chk(self.L.fm_rot_logexp_batch(self.h, n, F.ravel(), alpha, R.ctypes.data, w.ctypes.data,
+                                             Q.ctypes.data), "fm_rot_logexp_batch")
+        return R.reshape(n, 3, 3), w, Q.reshape(n, 3, 3)
+
+    def element_rotations(self):
+        w = np.zeros(3 * self.ne)
+        self._chk(self.L.fm_element_rotations(self.h, w), "fm_element_rotations")
+        return w.reshape(self.ne, 3)
+
+    def checkpoint_save(self, path, load_step):
+        self._chk(self.L.fm_checkpoint_save(self.h, path.encode(), load_step), "fm_checkpoint_save")
+
+    def checkpoint_load(self, path):
+        s = C.c_uint32(0)
+        self._chk(self.L.fm_checkpoint_load(self.h, path.encode(), C.byref(s)), "fm_checkpoint_load")
+        return s.value
+
+    # ---- measurement ------------------------------------------------------------
+    def profile(self, on=True):
+        self._chk(self.L.fm_profile_enable(self.h, int(on)), "fm_profile_enable")
+
+    def profile_reset(self):
+        self._chk(self.L.fm_profile_reset(self.h), "fm_profile_reset")
+
+    def profile_get(self):
+        ms = np.zeros(7)
+        n = np.zeros(7, dtype=np.uint64)
+        self._chk(self.L.fm_profile_get(self.h, ms, n), "fm_profile_get")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KERNEL_CLASSES)}
+
+    def info(self):
+        o = np.zeros(5, dtype=np.uint64)
+        self._chk(self.L.fm_info(self.h, o), "fm_info")
+        return dict(zip(["n", "nnz", "ne", "nn", "ncolors"], (int(v) for v in o)))

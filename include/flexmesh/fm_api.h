@@ -1,0 +1,190 @@
+/*
+ * fm_api.h -- C ABI of the MI355X quasi-static Newton path (P1-tet
+ * Neo-Hookean residual and tangent, fixed-CSR assembly, ILU(0)-PCG, Newton).
+ *
+ * Plain pointers and sizes only; no exceptions cross this boundary.  Every call
+ * returns an int status (FM_OK = 0, < 0 on error) and fm_last_error() gives the
+ * message.  Host arrays are caller-owned and copied; device memory is owned by
+ * the context.  One HIP stream per context; every call is synchronous on
+ * return.  A context is not thread-safe; distinct contexts may be used
+ * concurrently.
+ *
+ * Conventions (those of the reference engine, FEM/fem_types.hpp):
+ *   DOF numbering node*3 + d; node coordinates SoA (x[nn], y[nn], z[nn]);
+ *   connectivity SoA (conn[a*ne + e], a = 0..3); CSR with uint32 row_ptr and
+ *   col_idx; FP64 values at the host boundary.
+ */
+#ifndef FLEXMESH_FM_API_H
+#define FLEXMESH_FM_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FM_OK 0
+#define FM_EINVAL (-1)      /* bad argument or size */
+#define FM_ESTATE (-2)      /* call out of order (e.g. no mesh) */
+#define FM_EHIP (-3)        /* HIP runtime error */
+#define FM_ENOCONV (-4)     /* solver did not converge */
+#define FM_EDEVICE (-5)     /* a device-side bound (spin limit) was hit */
+
+/* precision of the linear algebra: FM_FP64 = plain FP64 drop-in path;
+   FM_MIXED = FP32 assembly/SpMV/ILU(0)/PCG under an FP64 Newton residual */
+#define FM_FP64 0
+#define FM_MIXED 1
+
+typedef struct fm_ctx fm_ctx;
+
+/* ---- context ---------------------------------------------------------------- */
+int fm_create(int device, fm_ctx** out);
+int fm_destroy(fm_ctx* ctx);
+const char* fm_last_error(const fm_ctx* ctx);
+const char* fm_version(void);
+
+/* ---- synthetic inputs (host side, no context) ---------------------------------
+ * Structured box of nx*ny*nz hexes, 6 Kuhn tets each, node id
+ * k(nx+1)(ny+1) + j(nx+1) + i.  Replaces generate_unit_cube_mesh
+ * (FEM/adaptive_fem_engine.hpp:2219-2294), whose split is non-conforming.
+ * X: 3*(nx+1)(ny+1)(nz+1) doubles SoA; conn: 4*6*nx*ny*nz uint32 SoA. */
+int fm_mesh_box(int nx, int ny, int nz, double Lx, double Ly, double Lz, double* X, uint32_t* conn);
+/* Moves interior nodes by U(-amp, +amp) per coordinate (counter-based RNG keyed by seed). */
+int fm_mesh_jitter(uint32_t n_nodes, double* X, const uint8_t* is_interior, double amp, uint64_t seed);
+/* out[e] = median * exp(sigma * z_e), z_e standard normal (counter-based Box-Muller keyed by seed). */
+int fm_field_lognormal(uint32_t n, double median, double sigma, uint64_t seed, double* out);
+
+/* ---- mesh, material, boundary conditions ---------------------------------------
+ * Replace MeshTopology / ProblemDescriptor (FEM/fem_types.hpp:168-256); the
+ * material is per element (mu_e, lam_e) instead of one (E, nu) per problem
+ * (ProblemDescriptor.E/.nu, used at FEM/adaptive_fem_engine.hpp:1034, 1187). */
+int fm_set_mesh(fm_ctx* ctx, uint32_t n_nodes, const double* X, uint32_t n_elems, const uint32_t* conn);
+int fm_set_material(fm_ctx* ctx, const double* mu_e, const double* lam_e);
+/* Dirichlet DOFs and their values at load factor 1 (BoundaryConditions::dirichlet_bcs,
+   FEM/fem_types.hpp:258-264); vals may be NULL (all zero). */
+int fm_set_bcs(fm_ctx* ctx, uint32_t n_dir, const uint32_t* dofs, const double* vals);
+/* External nodal force at load factor 1 (3*nn doubles) or NULL (no force);
+   replaces apply_neumann_traction (FEM/adaptive_fem_engine.hpp:733-742). */
+int fm_set_load(fm_ctx* ctx, const double* f_ext);
+
+/* ---- pattern and colouring ----------------------------------------------------
+ * build_csr_sparsity_pattern_rigorous (FEM/adaptive_fem_engine.hpp:1069-1120)
+ * with the intended column rule; outputs may be NULL.  row_ptr: 3nn+1. */
+int fm_build_pattern(fm_ctx* ctx, uint32_t* row_ptr_out, uint32_t* col_idx_out, uint64_t* nnz_out);
+/* MeshColoring::compute (FEM/adaptive_fem_engine.hpp:1549-1602); colors_out may be NULL. */
+int fm_color(fm_ctx* ctx, uint32_t* colors_out, uint32_t* n_colors_out);
+
+/* ---- assembly ------------------------------------------------------------------
+ * assemble_global_threadlocal (FEM/adaptive_fem_engine.hpp:1126-1233):
+ * f_int(u) and, when K_vals_out != NULL or keep != 0, the tangent K on the
+ * pattern.  prec selects the K precision on the device (K_vals_out is FP64
+ * either way).  No boundary conditions are applied.  u may be NULL (zero). */
+int fm_assemble(fm_ctx* ctx, const double* u, double* f_int_out, double* K_vals_out, int prec);
+/* assemble_global_residual_only (:990-1059) and the Newton residual
+   R = load*f_ext - f_int(u) with constrained rows zeroed. R_out may be NULL;
+   norm_out (||R||_2) may be NULL. */
+int fm_residual(fm_ctx* ctx, const double* u, double load, double* R_out, double* norm_out);
+
+/* ---- sparse linear algebra -----------------------------------------------------
+ * A general CSR matrix (sorted or unsorted columns) for the solver entries
+ * below; replaces SparseCSR (FEM/fem_types.hpp:304-320).  Rows whose groups of
+ * three share one column set made of whole 3x3 blocks are solved node-blocked. */
+int fm_set_matrix(fm_ctx* ctx, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const double* vals);
+/* use the assembled tangent as the matrix (Dirichlet rows/columns eliminated
+   when eliminate != 0, FEM/nonlinear_solver.hpp:388-396) */
+int fm_use_tangent(fm_ctx* ctx, int eliminate);
+/* SparseCSR::matvec (FEM/fem_types.hpp:310-319) in the given precision */
+int fm_spmv(fm_ctx* ctx, int prec, const double* x, double* y);
+/* ILU0Preconditioner::factorize (FEM/adaptive_fem_engine.hpp:305-470): L and U at
+   A's positions, u_ii at the diagonal positions (LU_out may be NULL) */
+int fm_ilu0_factor(fm_ctx* ctx, int prec, double* LU_out);
+/* ILU0Preconditioner::solve (:473-504) with the last factorization */
+int fm_ilu0_apply(fm_ctx* ctx, int prec, const double* r, double* z);
+/* pcg_solve_ilu0 (:1283-1341): factorizes, then PCG from x (in/out) with the
+   stopping rule sqrt(r.z / b.b) < tol.  *iters follows the reference return
+   value (max_it = not converged, 0 = trivially converged). */
+int fm_pcg(fm_ctx* ctx, int prec, const double* b, double* x, double tol, int max_it, int* iters, double* relres);
+
+/* ---- Newton ---------------------------------------------------------------------
+ * One load step (1-based) of the Newton loop of AdaptiveFEMEngine::run
+ * (FEM/adaptive_fem_engine.hpp:2456-2592) behind the NonlinearNewtonSolver
+ * interface (FEM/nonlinear_solver.hpp:23-64, 87-238). */
+typedef struct {
+    uint32_t n_steps;     /* load steps to reach load factor 1 */
+    int prec;             /* FM_FP64 or FM_MIXED */
+    double newton_rtol;   /* ||R_k|| <= rtol ||R_0|| */
+    int max_newton;
+    double pcg_tol;       /* pcg_solve_ilu0 tol (FP64 path and the FP64 lift solve) */
+    int pcg_max;
+    double inner_tol;     /* FP32 PCG tol of the mixed path */
+    int ls_max;           /* Armijo backtracks (armijo_line_search_true, :933-985) */
+    double armijo_c;
+    int lift;             /* 1: predictor solve with the prescribed increment */
+} fm_newton_cfg;
+
+typedef struct {
+    int converged;
+    int newton_iters;
+    int pcg_iters;
+    int assemblies;
+    int residuals;
+    int factorizations;
+    double r0;
+    double rel_res;
+    double seconds;
+} fm_newton_stats;
+
+void fm_newton_cfg_default(fm_newton_cfg* cfg);
+/* u: 3nn doubles in/out (NULL: use and keep the context's device-resident state) */
+int fm_newton_step(fm_ctx* ctx, uint32_t step, double* u, const fm_newton_cfg* cfg, fm_newton_stats* stats);
+/* device-resident displacement (3nn doubles) */
+int fm_get_u(fm_ctx* ctx, double* u);
+int fm_set_u(fm_ctx* ctx, const double* u);
+
+/* ---- adjoint ----------------------------------------------------------------------
+ * No reference counterpart (new): at the converged final-load state u, solve
+ * K lambda = dL/du (FP64 PCG) and return dL/dmu_e, dL/dlam_e and dL/d(load
+ * factor of f_ext and of the Dirichlet values).  Outputs may be NULL. */
+int fm_adjoint(fm_ctx* ctx, const double* dLdu, double tol, int max_it, double* dL_dmu, double* dL_dlam,
+               double* dL_dload, int* iters);
+
+/* ---- rotation part (FP64) -----------------------------------------------------------
+ * Replaces sl3_retraction / exponential_map / matrix_logarithm
+ * (CORE/lie_operator.cpp:94-312) and transport_stress_frame_objective
+ * (FEM/state_transport.hpp:201-249) for the rotation part: R = polar(F),
+ * w = log R (angle-axis), Q = exp(alpha [w]x) (Rodrigues).  F, R, Q are n
+ * row-major 3x3; w is 3n.  R, w, Q may be NULL. */
+int fm_rot_logexp_batch(fm_ctx* ctx, uint32_t n, const double* F, double alpha, double* R, double* w, double* Q);
+/* per-element rotation vector of the polar factor of F at the device-resident u
+   (3*ne doubles, element order) */
+int fm_element_rotations(fm_ctx* ctx, double* w);
+
+/* ---- checkpoint ----------------------------------------------------------------------
+ * Versioned, checksummed load-step checkpoint (superset of CheckpointFile,
+ * FEM/adaptive_fem_engine.hpp:1347-1452): magic, version, counts, load step,
+ * u, FNV-1a checksum. */
+int fm_checkpoint_save(fm_ctx* ctx, const char* path, uint32_t load_step);
+int fm_checkpoint_load(fm_ctx* ctx, const char* path, uint32_t* load_step);
+
+/* ---- measurement ------------------------------------------------------------------------
+ * Per-kernel-class HIP-event timing on the context's stream. */
+#define FM_K_SPMV 0
+#define FM_K_ILU_FWD 1
+#define FM_K_ILU_BWD 2
+#define FM_K_ILU_FACTOR 3
+#define FM_K_ASSEMBLE 4
+#define FM_K_RESIDUAL 5
+#define FM_K_PCG_OTHER 6
+#define FM_K_COUNT 7
+int fm_profile_enable(fm_ctx* ctx, int on);
+int fm_profile_reset(fm_ctx* ctx);
+/* ms[k] total milliseconds and launches[k] count for kernel class k */
+int fm_profile_get(fm_ctx* ctx, double* ms, uint64_t* launches);
+/* sizes for roofline bookkeeping: n (rows), nnz, n_elems, n_nodes, n_colors */
+int fm_info(fm_ctx* ctx, uint64_t* out5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

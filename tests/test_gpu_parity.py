@@ -1,0 +1,311 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 CPU oracle.
+
+Integer outputs (CSR pattern, colouring) must be bit-exact.  Floating-point
+tolerances, relative to the largest magnitude of the compared quantity:
+  FP64 kernels            <= 1e-12 (summation order differs from the oracle)
+  FP32 tangent / SpMV     <= 2e-6
+  FP32 ILU(0) factors      <= 1e-3 (pivot-error growth over the factorization)
+  Newton displacement     <= 1e-9 (FP64 path), <= 1e-7 (mixed path, rtol 1e-8)
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def fm():
+    import flexmesh
+
+    flexmesh.lib()
+    return flexmesh
+
+
+@pytest.fixture
+def S(fm):
+    s = fm.Solver(0)
+    yield s
+    s.close()
+
+
+def small_problem(fm, k=3, dims=(4, 3, 5)):
+    cfg = fm.configs.make(k, dims=dims)
+    return cfg
+
+
+def oracle_problem(O, cfg, **kw):
+    return O.NewtonProblem(cfg["X"], cfg["conn"], cfg["mu"], cfg["lam"], cfg["dir_dofs"], cfg["dir_vals"],
+                           cfg["f_ext"], cfg["n_steps"], **kw)
+
+
+# ---------------------------------------------------------------------------- pattern / colouring
+@pytest.mark.parametrize("k,dims", [(1, (8, 8, 8)), (2, (6, 5, 7)), (5, (16, 3, 2))])
+def test_pattern_and_coloring_bit_exact(fm, S, O, k, dims):
+    cfg = fm.configs.make(k, dims=dims)
+    fm.configs.load(S, cfg)
+    rp, ci = S.pattern()
+    nn = cfg["X"].size // 3
+    rpo, cio = O.pattern(nn, cfg["conn"])
+    assert np.array_equal(rp, rpo) and np.array_equal(ci, cio)
+    col, nc = S.colors()
+    colo, nco = O.color(nn, cfg["conn"])
+    assert nc == nco and np.array_equal(col, colo)
+
+
+def test_coloring_matches_compiled_reference(fm, S):
+    g = golden("mesh_pattern_color.npz")
+    X, conn = fm.mesh_box(8, 8, 8)
+    S.set_mesh(X, conn)
+    col, nc = S.colors()
+    assert nc == int(g["ref_ncolors_kuhn_8"][0]) and np.array_equal(col, g["ref_colors_kuhn_8"])
+    rp, _ = S.pattern()
+    assert np.array_equal(rp, g["ref_rowptr_kuhn_8"])
+
+
+# ---------------------------------------------------------------------------- assembly
+@pytest.mark.parametrize("k", [2, 3])
+def test_assembly_matches_oracle(fm, S, O, k):
+    cfg = small_problem(fm, k)
+    fm.configs.load(S, cfg)
+    rng = np.random.default_rng(k)
+    nn = cfg["X"].size // 3
+    u = rng.uniform(-0.02, 0.02, 3 * nn)
+    rp, ci = O.pattern(nn, cfg["conn"])
+    colors, nc = O.color(nn, cfg["conn"])
+    Ko, fo = O.assemble(cfg["X"], cfg["conn"], cfg["mu"], cfg["lam"], colors, nc, u, rp, ci)
+    f64, K64 = S.assemble(u, fm.FM_FP64)
+    assert rel(f64, fo) < 1e-12 and rel(K64, Ko) < 1e-12
+    f32, K32 = S.assemble(u, fm.FM_MIXED)
+    assert rel(f32, fo) < 1e-12  # the force is FP64 in both paths
+    assert rel(K32, Ko) < 2e-6
+    R, nr = S.residual(u, 0.7)
+    Ro = -fo.copy()
+    Ro[cfg["dir_dofs"]] = 0.0
+    assert rel(R, Ro) < 1e-12 and abs(nr - np.linalg.norm(Ro)) < 1e-12 * np.linalg.norm(Ro)
+
+
+def test_assembly_is_bitwise_repeatable(fm, S):
+    cfg = small_problem(fm, 2, (5, 4, 4))
+    fm.configs.load(S, cfg)
+    u = np.random.default_rng(1).uniform(-0.02, 0.02, cfg["X"].size)
+    f1, K1 = S.assemble(u)
+    f2, K2 = S.assemble(u)
+    assert np.array_equal(f1, f2) and np.array_equal(K1, K2)
+
+
+# ---------------------------------------------------------------------------- SpMV / ILU(0) / PCG
+def poisson(g):
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import make_golden as MG
+
+    return MG.poisson(g, False)
+
+
+def elastic_matrix(O, fm, dims=(4, 4, 4), seed=3, eliminate=True):
+    cfg = fm.configs.make(2, dims=dims)
+    nn = cfg["X"].size // 3
+    rp, ci = O.pattern(nn, cfg["conn"])
+    colors, nc = O.color(nn, cfg["conn"])
+    u = np.random.default_rng(seed).uniform(-0.02, 0.02, 3 * nn)
+    K, _ = O.assemble(cfg["X"], cfg["conn"], cfg["mu"], cfg["lam"], colors, nc, u, rp, ci)
+    if eliminate:
+        isd = np.zeros(3 * nn, dtype=np.uint8)
+        isd[cfg["dir_dofs"]] = 1
+        O.eliminate(rp, ci, K, isd)
+    return rp, ci, K
+
+
+def sorted_csr(rp, ci, va):
+    perm = np.concatenate([rp[r] + np.argsort(ci[rp[r]:rp[r + 1]], kind="stable") for r in range(rp.size - 1)])
+    return ci[perm], va[perm], perm
+
+
+@pytest.mark.parametrize("which", ["poisson", "elastic"])
+def test_spmv_ilu0_match_oracle(fm, S, O, which):
+    if which == "poisson":
+        rp, ci, va, b, xs = poisson(24)
+    else:
+        rp, ci, va = elastic_matrix(O, fm)
+    S.set_matrix(rp, ci, va)
+    n = rp.size - 1
+    x = np.random.default_rng(2).standard_normal(n)
+    cs, vs, perm = sorted_csr(rp, ci, va)
+    yo = O.spmv(rp, cs, vs, x)
+    assert rel(S.spmv(x), yo) < 1e-13
+    assert rel(S.spmv(x, fm.FM_MIXED), yo) < 2e-6
+    LUo, diag = O.ilu0(rp, cs, vs)
+    LU = S.ilu0_factor()
+    LUo_user = np.empty_like(LUo)
+    LUo_user[perm] = LUo
+    assert rel(LU, LUo_user) < 1e-12
+    zo = O.ilu0_apply(rp, cs, LUo, diag, x)
+    assert rel(S.ilu0_apply(x), zo) < 1e-12
+    LU32 = S.ilu0_factor(fm.FM_MIXED)
+    assert rel(LU32, LUo_user) < 1e-3
+    assert rel(S.ilu0_apply(x, fm.FM_MIXED), zo) < 1e-3
+
+
+@pytest.mark.parametrize("gn", [16, 32, 48])
+def test_pcg_matches_reference_golden(fm, S, O, gn):
+    g = golden("pcg.npz")
+    rp, ci, va, b, xs = poisson(gn)
+    S.set_matrix(rp, ci, va)
+    x, its, relres = S.pcg(b, tol=1e-10, max_it=4000)
+    its_ref = int(g[f"poisson{gn}_refcols_iters"][0])
+    assert abs(its - its_ref) <= 1
+    assert rel(x, g[f"poisson{gn}_refcols_x"]) < 1e-9
+    assert relres < 1e-10
+
+
+def test_pcg_elastic_matches_oracle(fm, S, O):
+    rp, ci, va = elastic_matrix(O, fm, dims=(6, 5, 5))
+    b = np.random.default_rng(9).standard_normal(rp.size - 1)
+    S.set_matrix(rp, ci, va)
+    xo, itso, _ = O.pcg(rp, ci, va, b, tol=1e-10, max_iter=500)
+    x, its, relres = S.pcg(b, tol=1e-10, max_it=500)
+    assert abs(its - itso) <= 1
+    assert rel(x, xo) < 1e-9
+    x32, its32, _ = S.pcg(b, tol=1e-5, max_it=500, prec=fm.FM_MIXED)
+    assert rel(x32, xo) < 1e-3
+
+
+def test_pcg_edge_cases(fm, S):
+    rp = np.array([0, 1, 2, 3], np.uint32)
+    ci = np.array([0, 1, 2], np.uint32)
+    S.set_matrix(rp, ci, np.array([2.0, 4.0, 8.0]))
+    x, its, _ = S.pcg(np.zeros(3), tol=1e-10)  # b = 0: trivially converged
+    assert its == 0 and not x.any()
+    x, its, _ = S.pcg(np.array([2.0, 4.0, 8.0]), tol=1e-10)
+    assert its == 1 and np.allclose(x, 1.0, rtol=0, atol=1e-15)
+    # zero pivot: the reference perturbation u_ii += 1e-8
+    rp = np.array([0, 2, 4], np.uint32)
+    ci = np.array([0, 1, 0, 1], np.uint32)
+    S.set_matrix(rp, ci, np.ones(4))
+    LU = S.ilu0_factor()
+    assert LU[3] == 1e-8 and LU[0] == 1.0
+
+
+# ---------------------------------------------------------------------------- Newton
+def run_oracle(O, cfg, steps, **kw):
+    P = oracle_problem(O, cfg, newton_rtol=cfg["newton_rtol"], pcg_tol=cfg["pcg_tol"], **kw)
+    u = np.zeros(cfg["X"].size)
+    stats = []
+    for s in steps:
+        rc, st = P.step(s, u)
+        assert rc == 0
+        stats.append(st)
+    return u, stats
+
+
+def test_newton_config1_fp64_matches_oracle(fm, S, O):
+    cfg = fm.configs.make(1)
+    fm.configs.load(S, cfg)
+    uo, sto = run_oracle(O, cfg, range(1, 11))
+    c = fm.configs.newton_cfg(cfg)
+    u = np.zeros(cfg["X"].size)
+    for s in range(1, 11):
+        rc, st = S.newton_step(s, c, u)
+        assert st["converged"] and st["rel_res"] <= 1e-10
+        assert st["newton_iters"] == sto[s - 1]["newton_iters"]
+    assert rel(u, uo) < 1e-9
+
+
+def test_newton_mixed_matches_fp64_oracle(fm, S, O):
+    cfg = fm.configs.make(3, dims=(6, 6, 6))
+    cfg["n_steps"] = 4
+    fm.configs.load(S, cfg)
+    uo, _ = run_oracle(O, cfg, range(1, 5))
+    c = fm.configs.newton_cfg(cfg, inner_tol=1e-4)
+    for s in range(1, 5):
+        rc, st = S.newton_step(s, c)
+        assert st["converged"] and st["rel_res"] <= 1e-8
+    u = S.get_u()
+    assert rel(u, uo) < 1e-7
+    # the FP64 residual of the mixed solution, evaluated by the oracle
+    nn = cfg["X"].size // 3
+    rp, ci = O.pattern(nn, cfg["conn"])
+    colors, nc = O.color(nn, cfg["conn"])
+    _, f = O.assemble(cfg["X"], cfg["conn"], cfg["mu"], cfg["lam"], colors, nc, u, rp, ci, want_K=False)
+    f[cfg["dir_dofs"]] = 0.0
+    _, f0 = O.assemble(cfg["X"], cfg["conn"], cfg["mu"], cfg["lam"], colors, nc, uo, rp, ci, want_K=False)
+    assert np.linalg.norm(f) <= 1e-6 * np.abs(f0).max()
+
+
+def test_newton_traction_beam_fp64(fm, S, O):
+    # traction-driven (no Dirichlet increment, so no lift): the first two of 40 steps
+    cfg = fm.configs.make(5, dims=(12, 2, 2))
+    cfg["prec"] = fm.FM_FP64
+    fm.configs.load(S, cfg)
+    uo, sto = run_oracle(O, cfg, range(1, 3))
+    c = fm.configs.newton_cfg(cfg)
+    for s in range(1, 3):
+        rc, st = S.newton_step(s, c)
+        assert st["converged"] and st["newton_iters"] == sto[s - 1]["newton_iters"]
+    assert rel(S.get_u(), uo) < 1e-9
+
+
+# ---------------------------------------------------------------------------- adjoint / rotation / checkpoint
+def test_adjoint_matches_oracle_and_fd(fm, S, O):
+    cfg = fm.configs.make(4, dims=(3, 3, 3))
+    cfg["n_steps"] = 2
+    fm.configs.load(S, cfg)
+    c = fm.configs.newton_cfg(cfg, pcg_max=2000)
+    for s in (1, 2):
+        S.newton_step(s, c)
+    u = S.get_u()
+    nn = cfg["X"].size // 3
+    ustar = np.random.default_rng(4).uniform(-0.01, 0.01, 3 * nn)
+    top = cfg["loss_nodes"]
+    dLdu = np.zeros(3 * nn)
+    dLdu.reshape(nn, 3)[top] = (u - ustar).reshape(nn, 3)[top]
+    dmu, dlam, dload, its = S.adjoint(dLdu)
+    P = oracle_problem(O, cfg, newton_rtol=1e-13, pcg_tol=1e-13, pcg_max=2000)
+    uo = np.zeros(3 * nn)
+    for s in (1, 2):
+        P.step(s, uo)
+    dmo, dlo, dldo, _, _ = P.adjoint(uo, dLdu)
+    assert rel(dmu, dmo) < 1e-8 and rel(dlam, dlo) < 1e-8 and abs(dload - dldo) <= 1e-8 * abs(dldo)
+
+
+def test_rotations_match_oracle(fm, S, O):
+    rng = np.random.default_rng(5)
+    Fs = []
+    for ang in (1e-9, 1e-3, 0.4, 1.2, 2.5, np.pi - 1e-4):
+        n = rng.standard_normal(3)
+        n /= np.linalg.norm(n)
+        Rt = O.rot_exp(ang * n)
+        A = rng.standard_normal((3, 3))
+        Fs.append(Rt @ (np.eye(3) + 0.2 * A @ A.T))
+    R, w, Q = S.rot_logexp(np.array(Fs), alpha=0.5)
+    for i, F in enumerate(Fs):
+        Ro, _ = O.polar(F)
+        wo = O.rot_log(Ro)
+        assert np.abs(R[i] - Ro).max() < 1e-14
+        assert np.abs(w[i] - wo).max() < 1e-13
+        assert np.abs(Q[i] - O.rot_exp(0.5 * wo)).max() < 1e-14
+
+
+def test_checkpoint_roundtrip(fm, S, tmp_path):
+    cfg = fm.configs.make(1, dims=(3, 3, 3))
+    fm.configs.load(S, cfg)
+    u = np.random.default_rng(6).standard_normal(cfg["X"].size)
+    S.set_u(u)
+    p = str(tmp_path / "ck.bin")
+    S.checkpoint_save(p, 7)
+    S.set_u(np.zeros_like(u))
+    assert S.checkpoint_load(p) == 7
+    assert np.array_equal(S.get_u(), u)
+    data = bytearray(open(p, "rb").read())
+    data[100] ^= 1
+    open(p, "wb").write(bytes(data))
+    with pytest.raises(fm.FlexMeshError):
+        S.checkpoint_load(p)
