@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 namespace fm {
 namespace {
@@ -65,7 +66,7 @@ __device__ __forceinline__ T await_val(const T* p, int* err) {
         }
         // once any wave has hit the limit every other wait gives up too
         if ((spins & 255u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-        __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_s_sleep(2);
         b = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     return __builtin_bit_cast(T, b);
@@ -392,19 +393,10 @@ __device__ __forceinline__ void put(T (&a)[S], int sl, T v) {
 // terms of every L and U entry in ascending pivot order, as the reference
 // factorization does), reading the finished U rows of its lower neighbours.
 template <typename T, int B, int S>
-__global__ __launch_bounds__(64) void ilu_factor_kernel(uint32_t n_nodes, const uint32_t* __restrict__ order,
-                                                        const uint32_t* __restrict__ rp,
-                                                        const uint32_t* __restrict__ ci,
-                                                        const uint16_t* __restrict__ nlow, const T* __restrict__ A,
-                                                        T* LU, uint32_t* ticket, int* err) {
-    __shared__ uint32_t s_t;
-    if (threadIdx.x == 0) s_t = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t t = s_t;
-    if (t >= n_nodes) return;
+__device__ __forceinline__ void factor_node(uint32_t node, const uint32_t* __restrict__ rp,
+                                            const uint32_t* __restrict__ ci, const uint16_t* __restrict__ nlow,
+                                            const T* __restrict__ A, T* LU, int* err) {
     const int lane = threadIdx.x;
-    const uint32_t node = order[t];
-    if (node == kNoNode) return;
     const uint32_t r0 = B * node;
     uint32_t rs[B];
 #pragma unroll
@@ -497,37 +489,99 @@ __global__ __launch_bounds__(64) void ilu_factor_kernel(uint32_t n_nodes, const 
     }
 }
 
-// forward sweep z = L^{-1} rhs; with UPD the PCG update r -= alpha Ap,
-// x += alpha p of the node's rows is fused in and the new r is the rhs
+// persistent: a bounded grid of waves takes one node per ticket, in level order
+template <typename T, int B, int S>
+__global__ __launch_bounds__(64) void ilu_factor_kernel(uint32_t nq, const uint32_t* __restrict__ order,
+                                                        const uint32_t* __restrict__ rp,
+                                                        const uint32_t* __restrict__ ci,
+                                                        const uint16_t* __restrict__ nlow, const T* __restrict__ A,
+                                                        T* LU, uint32_t* ticket, int* err) {
+    uint32_t t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(ticket, 1u);
+    t = __shfl(t, 0);
+    while (t < nq) {
+        uint32_t tn = 0;
+        if (threadIdx.x == 0) tn = atomicAdd(ticket, 1u);
+        const uint32_t node = order[t];
+        if (node != kNoNode) factor_node<T, B, S>(node, rp, ci, nlow, A, LU, err);
+        t = __shfl(tn, 0);
+    }
+}
+
+// Triangular sweeps.  Both are persistent: a bounded grid of waves loops
+// over tickets of the level-ordered queue (the next ticket is claimed before
+// the current one is worked, to hide the atomic), so only waves near the
+// dependence frontier exist and spin.  Everything a node needs that does not
+// depend on other nodes (matrix values, columns, right-hand side, PCG vectors,
+// diagonal-block entries) is loaded before the first wait.
+constexpr int kMJ = 2;  // lower/upper entries per lane kept in registers (more fall back to a loop)
+
+// forward sweep z = L^{-1} rhs for the NPW nodes of ticket t; with UPD the PCG
+// update r -= alpha Ap, x += alpha p of the node's rows is fused in and the
+// new r is the right-hand side
 template <typename T, int B, int LPN, bool UPD>
-__global__ __launch_bounds__(64) void ilu_fwd_kernel(uint32_t n_nodes, const uint32_t* __restrict__ order,
-                                                     const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
-                                                     const uint16_t* __restrict__ nlow, const T* __restrict__ LU,
-                                                     const T* __restrict__ rhs, T* __restrict__ r, T* __restrict__ x,
-                                                     const T* __restrict__ p, const T* __restrict__ Ap, T* z,
-                                                     PcgState* st, int* err) {
-    if (UPD && st->done) return;
-    __shared__ uint32_t s_t;
-    if (threadIdx.x == 0) s_t = atomicAdd(&st->ticket[0], 1u);
-    __syncthreads();
+__device__ __forceinline__ void fwd_ticket(uint32_t t, uint32_t nq, const uint32_t* __restrict__ order,
+                                           const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+                                           const uint16_t* __restrict__ nlow, const T* __restrict__ LU,
+                                           const T* __restrict__ rhs, T* __restrict__ r, T* __restrict__ x,
+                                           const T* __restrict__ p, const T* __restrict__ Ap, T* z, T alpha,
+                                           int* err) {
     constexpr int NPW = 64 / LPN;
     const int lane = threadIdx.x, gl = lane % LPN;
-    const uint32_t idx = s_t * NPW + lane / LPN;
-    const uint32_t entry = idx < n_nodes ? order[idx] : kNoNode;
+    const uint32_t idx = t * NPW + lane / LPN;
+    const uint32_t entry = idx < nq ? order[idx] : kNoNode;
     const bool valid = entry != kNoNode;
     const uint32_t node = valid ? entry : 0;
     const uint32_t r0 = B * node;
-    const uint32_t pl = valid ? B * (uint32_t)nlow[node] : 0;
+    uint32_t rs[B];
+    uint32_t pl = 0;
+    uint32_t c[kMJ];
+    T l[kMJ][B];
+    T rv[B], apv[B], pv[B], xv[B], ld[B][B];
+    if (valid) {
+#pragma unroll
+        for (int d = 0; d < B; ++d) rs[d] = rp[r0 + d];
+        pl = B * (uint32_t)nlow[node];
+#pragma unroll
+        for (int m = 0; m < kMJ; ++m) {
+            const uint32_t j = gl + m * LPN;
+            if (j < pl) {
+                c[m] = ci[rs[0] + j];
+#pragma unroll
+                for (int d = 0; d < B; ++d) l[m][d] = LU[rs[d] + j];
+            }
+        }
+        if (gl == 0) {
+#pragma unroll
+            for (int d = 0; d < B; ++d) {
+                if (UPD) {
+                    rv[d] = r[r0 + d];
+                    apv[d] = Ap[r0 + d];
+                    pv[d] = p[r0 + d];
+                    xv[d] = x[r0 + d];
+                } else {
+                    rv[d] = rhs[r0 + d];
+                }
+#pragma unroll
+                for (int e = 0; e < d; ++e) ld[d][e] = LU[rs[d] + pl + e];
+            }
+        }
+    }
     T acc[B];
 #pragma unroll
     for (int d = 0; d < B; ++d) acc[d] = 0;
     if (valid) {
-        uint32_t rs[B];
 #pragma unroll
-        for (int d = 0; d < B; ++d) rs[d] = rp[r0 + d];
-        for (uint32_t j = gl; j < pl; j += LPN) {
-            const uint32_t c = ci[rs[0] + j];
-            const T zc = await_val(&z[c], err);
+        for (int m = 0; m < kMJ; ++m) {
+            const uint32_t j = gl + m * LPN;
+            if (j < pl) {
+                const T zc = await_val(&z[c[m]], err);
+#pragma unroll
+                for (int d = 0; d < B; ++d) acc[d] += l[m][d] * zc;
+            }
+        }
+        for (uint32_t j = gl + kMJ * LPN; j < pl; j += LPN) {
+            const T zc = await_val(&z[ci[rs[0] + j]], err);
 #pragma unroll
             for (int d = 0; d < B; ++d) acc[d] += LU[rs[d] + j] * zc;
         }
@@ -538,23 +592,17 @@ __global__ __launch_bounds__(64) void ilu_fwd_kernel(uint32_t n_nodes, const uin
         for (int d = 0; d < B; ++d) acc[d] += __shfl_xor(acc[d], o);
     if (valid && gl == 0) {
         T v[B];
-        T alpha = 0;
-        if (UPD) alpha = (T)st->alpha;
 #pragma unroll
         for (int d = 0; d < B; ++d) {
-            T rhs_d;
+            T rhs_d = rv[d];
             if (UPD) {
-                const T rr = r[r0 + d] - alpha * Ap[r0 + d];
-                r[r0 + d] = rr;
-                x[r0 + d] += alpha * p[r0 + d];
-                rhs_d = rr;
-            } else {
-                rhs_d = rhs[r0 + d];
+                rhs_d = rv[d] - alpha * apv[d];
+                r[r0 + d] = rhs_d;
+                x[r0 + d] = xv[d] + alpha * pv[d];
             }
             T s = rhs_d - acc[d];
-            const uint32_t rd = rp[r0 + d] + pl;
 #pragma unroll
-            for (int e = 0; e < d; ++e) s -= LU[rd + e] * v[e];
+            for (int e = 0; e < d; ++e) s -= ld[d][e] * v[e];
             v[d] = s;
         }
 #pragma unroll
@@ -562,38 +610,87 @@ __global__ __launch_bounds__(64) void ilu_fwd_kernel(uint32_t n_nodes, const uin
     }
 }
 
-// backward sweep y = U^{-1} z with the diagonal u_ii at the diagonal
-// positions; fused partial of r.y per wave (indexed by ticket, fixed order)
-template <typename T, int B, int LPN>
-__global__ __launch_bounds__(64) void ilu_bwd_kernel(uint32_t n_nodes, const uint32_t* __restrict__ order,
+template <typename T, int B, int LPN, bool UPD>
+__global__ __launch_bounds__(64) void ilu_fwd_kernel(uint32_t nq, const uint32_t* __restrict__ order,
                                                      const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
                                                      const uint16_t* __restrict__ nlow, const T* __restrict__ LU,
-                                                     const T* __restrict__ z, const T* __restrict__ r, T* y,
-                                                     double* __restrict__ partials, PcgState* st, int check_done,
-                                                     int* err) {
-    if (check_done && st->done) return;
-    __shared__ uint32_t s_t;
-    if (threadIdx.x == 0) s_t = atomicAdd(&st->ticket[1], 1u);
-    __syncthreads();
+                                                     const T* __restrict__ rhs, T* __restrict__ r, T* __restrict__ x,
+                                                     const T* __restrict__ p, const T* __restrict__ Ap, T* z,
+                                                     PcgState* st, int* err) {
+    if (UPD && st->done) return;
+    const T alpha = UPD ? (T)st->alpha : (T)0;
+    constexpr int NPW = 64 / LPN;
+    const uint32_t ntk = nq / NPW;
+    uint32_t t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(&st->ticket[0], 1u);
+    t = __shfl(t, 0);
+    while (t < ntk) {
+        uint32_t tn = 0;
+        if (threadIdx.x == 0) tn = atomicAdd(&st->ticket[0], 1u);
+        fwd_ticket<T, B, LPN, UPD>(t, nq, order, rp, ci, nlow, LU, rhs, r, x, p, Ap, z, alpha, err);
+        t = __shfl(tn, 0);
+    }
+}
+
+// backward sweep y = U^{-1} z with the diagonal u_ii at the diagonal
+// positions; fused partial of r.y per ticket (fixed order, deterministic)
+template <typename T, int B, int LPN>
+__device__ __forceinline__ double bwd_ticket(uint32_t t, uint32_t nq, const uint32_t* __restrict__ order,
+                                             const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+                                             const uint16_t* __restrict__ nlow, const T* __restrict__ LU,
+                                             const T* __restrict__ z, const T* __restrict__ r, T* y, int* err) {
     constexpr int NPW = 64 / LPN;
     const int lane = threadIdx.x, gl = lane % LPN;
-    const uint32_t idx = s_t * NPW + lane / LPN;
-    const uint32_t entry = idx < n_nodes ? order[idx] : kNoNode;
+    const uint32_t idx = t * NPW + lane / LPN;
+    const uint32_t entry = idx < nq ? order[idx] : kNoNode;
     const bool valid = entry != kNoNode;
     const uint32_t node = valid ? entry : 0;
     const uint32_t r0 = B * node;
-    const uint32_t pl = valid ? B * (uint32_t)nlow[node] : 0;
+    uint32_t rs[B];
+    uint32_t us = 0, len = 0, pl = 0;
+    uint32_t c[kMJ];
+    T l[kMJ][B];
+    T zv[B], rv[B], ud[B][B];
+    if (valid) {
+#pragma unroll
+        for (int d = 0; d < B; ++d) rs[d] = rp[r0 + d];
+        len = rp[r0 + 1] - rs[0];
+        pl = B * (uint32_t)nlow[node];
+        us = pl + B;
+#pragma unroll
+        for (int m = 0; m < kMJ; ++m) {
+            const uint32_t j = us + gl + m * LPN;
+            if (j < len) {
+                c[m] = ci[rs[0] + j];
+#pragma unroll
+                for (int d = 0; d < B; ++d) l[m][d] = LU[rs[d] + j];
+            }
+        }
+        if (gl == 0) {
+#pragma unroll
+            for (int d = 0; d < B; ++d) {
+                zv[d] = z[r0 + d];
+                rv[d] = r ? r[r0 + d] : (T)0;
+#pragma unroll
+                for (int e = d; e < B; ++e) ud[d][e] = LU[rs[d] + pl + e];
+            }
+        }
+    }
     T acc[B];
 #pragma unroll
     for (int d = 0; d < B; ++d) acc[d] = 0;
     if (valid) {
-        uint32_t rs[B];
 #pragma unroll
-        for (int d = 0; d < B; ++d) rs[d] = rp[r0 + d];
-        const uint32_t len = rp[r0 + 1] - rs[0];
-        for (uint32_t j = pl + B + gl; j < len; j += LPN) {
-            const uint32_t c = ci[rs[0] + j];
-            const T yc = await_val(&y[c], err);
+        for (int m = 0; m < kMJ; ++m) {
+            const uint32_t j = us + gl + m * LPN;
+            if (j < len) {
+                const T yc = await_val(&y[c[m]], err);
+#pragma unroll
+                for (int d = 0; d < B; ++d) acc[d] += l[m][d] * yc;
+            }
+        }
+        for (uint32_t j = us + gl + kMJ * LPN; j < len; j += LPN) {
+            const T yc = await_val(&y[ci[rs[0] + j]], err);
 #pragma unroll
             for (int d = 0; d < B; ++d) acc[d] += LU[rs[d] + j] * yc;
         }
@@ -607,22 +704,43 @@ __global__ __launch_bounds__(64) void ilu_bwd_kernel(uint32_t n_nodes, const uin
         T v[B];
 #pragma unroll
         for (int d = B - 1; d >= 0; --d) {
-            const uint32_t rd = rp[r0 + d] + pl;
-            T s = z[r0 + d] - acc[d];
+            T s = zv[d] - acc[d];
 #pragma unroll
-            for (int e = d + 1; e < B; ++e) s -= LU[rd + e] * v[e];
-            v[d] = s / LU[rd + d];
+            for (int e = d + 1; e < B; ++e) s -= ud[d][e] * v[e];
+            v[d] = s / ud[d][d];
         }
 #pragma unroll
         for (int d = 0; d < B; ++d) {
             publish(&y[r0 + d], v[d]);
-            if (partials) part += (double)r[r0 + d] * (double)v[d];
+            part += (double)rv[d] * (double)v[d];
         }
     }
-    if (partials) {
+    return part;
+}
+
+template <typename T, int B, int LPN>
+__global__ __launch_bounds__(64) void ilu_bwd_kernel(uint32_t nq, const uint32_t* __restrict__ order,
+                                                     const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+                                                     const uint16_t* __restrict__ nlow, const T* __restrict__ LU,
+                                                     const T* __restrict__ z, const T* __restrict__ r, T* y,
+                                                     double* __restrict__ partials, PcgState* st, int check_done,
+                                                     int* err) {
+    if (check_done && st->done) return;
+    constexpr int NPW = 64 / LPN;
+    const uint32_t ntk = nq / NPW;
+    uint32_t t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(&st->ticket[1], 1u);
+    t = __shfl(t, 0);
+    while (t < ntk) {
+        uint32_t tn = 0;
+        if (threadIdx.x == 0) tn = atomicAdd(&st->ticket[1], 1u);
+        double part = bwd_ticket<T, B, LPN>(t, nq, order, rp, ci, nlow, LU, z, partials ? r : nullptr, y, err);
+        if (partials) {
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
-        if (lane == 0) partials[s_t] = part;
+            for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+            if (threadIdx.x == 0) partials[t] = part;
+        }
+        t = __shfl(tn, 0);
     }
 }
 
@@ -952,6 +1070,23 @@ __global__ void elem_rot_kernel(ElemArgs A, const uint32_t* __restrict__ eperm, 
 
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// waves of the persistent kernels: few for the sweeps (only the frontier
+// should spin), more for the factorization (long per-node latency chains);
+// FM_SOLVE_WAVES / FM_FACTOR_WAVES override
+uint32_t env_waves(const char* name, uint32_t dflt) {
+    const char* e = std::getenv(name);
+    const long v = e ? std::atol(e) : 0;
+    return (uint32_t)(v > 0 ? v : dflt);
+}
+uint32_t solve_waves() {
+    static const uint32_t w = env_waves("FM_SOLVE_WAVES", 512);
+    return w;
+}
+uint32_t factor_waves() {
+    static const uint32_t w = env_waves("FM_FACTOR_WAVES", 4096);
+    return w;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1072,19 +1207,19 @@ template <typename T, int B>
 static void factor_t(hipStream_t s, const Matrix& A, const T* val, T* lu, PcgState* st, int* err) {
     const Schedule& S = A.sched;
     fill_sentinel_kernel<T><<<std::min<uint64_t>(cdiv(A.nnz, 256), 65535), 256, 0, s>>>(lu, nullptr, A.nnz);
-    const uint32_t nb = S.q_fwd;
+    const uint32_t nb = std::min(S.q_fwd, factor_waves());
     uint32_t* tk = &st->ticket[2];
     switch (S.slots) {
         case 1:
-            ilu_factor_kernel<T, B, 1><<<nb, 64, 0, s>>>(nb, S.fwd_order.p, A.row_ptr.p, A.col_idx.p, S.nlow.p, val,
+            ilu_factor_kernel<T, B, 1><<<nb, 64, 0, s>>>(S.q_fwd, S.fwd_order.p, A.row_ptr.p, A.col_idx.p, S.nlow.p, val,
                                                          lu, tk, err);
             break;
         case 2:
-            ilu_factor_kernel<T, B, 2><<<nb, 64, 0, s>>>(nb, S.fwd_order.p, A.row_ptr.p, A.col_idx.p, S.nlow.p, val,
+            ilu_factor_kernel<T, B, 2><<<nb, 64, 0, s>>>(S.q_fwd, S.fwd_order.p, A.row_ptr.p, A.col_idx.p, S.nlow.p, val,
                                                          lu, tk, err);
             break;
         default:
-            ilu_factor_kernel<T, B, 4><<<nb, 64, 0, s>>>(nb, S.fwd_order.p, A.row_ptr.p, A.col_idx.p, S.nlow.p, val,
+            ilu_factor_kernel<T, B, 4><<<nb, 64, 0, s>>>(S.q_fwd, S.fwd_order.p, A.row_ptr.p, A.col_idx.p, S.nlow.p, val,
                                                          lu, tk, err);
             break;
     }
@@ -1108,7 +1243,7 @@ template <typename T, int B, int LPN>
 static void fwd_t(hipStream_t s, const Matrix& A, const T* lu, const T* rhs, T* r, T* x, const T* p, const T* Ap,
                   T* z, PcgState* st, bool upd, int* err) {
     const Schedule& S = A.sched;
-    const uint32_t nb = cdiv(S.q_fwd, 64 / LPN);
+    const uint32_t nb = std::min(cdiv(S.q_fwd, 64 / LPN), solve_waves());
     if (upd)
         ilu_fwd_kernel<T, B, LPN, true><<<nb, 64, 0, s>>>(S.q_fwd, S.fwd_order.p, A.row_ptr.p, A.col_idx.p,
                                                           S.nlow.p, lu, rhs, r, x, p, Ap, z, st, err);
@@ -1147,10 +1282,11 @@ template <typename T, int B, int LPN>
 static uint32_t bwd_t(hipStream_t s, const Matrix& A, const T* lu, const T* z, const T* r, T* y, double* partials,
                       PcgState* st, bool check_done, int* err) {
     const Schedule& S = A.sched;
-    const uint32_t nb = cdiv(S.q_bwd, 64 / LPN);
+    const uint32_t ntk = cdiv(S.q_bwd, 64 / LPN);
+    const uint32_t nb = std::min(ntk, solve_waves());
     ilu_bwd_kernel<T, B, LPN><<<nb, 64, 0, s>>>(S.q_bwd, S.bwd_order.p, A.row_ptr.p, A.col_idx.p, S.nlow.p, lu, z, r,
                                                y, partials, st, check_done ? 1 : 0, err);
-    return nb;
+    return ntk;  // one partial per ticket
 }
 
 template <typename T>

@@ -1,0 +1,244 @@
+#!/usr/bin/env python3
+"""Benchmark of the quasi-static Newton path on one MI355X.
+
+Workload (default): configuration 3 -- a 128^3-hex cube as 12,582,912 Kuhn
+tets (2,146,689 nodes, 6,440,067 DOFs), interior jitter, lognormal mu_e,
+nu = 0.49, x=0 clamped, 25% uniaxial stretch of x=1 over 20 load steps.  A
+"step" is one load step solved by Newton to ||R|| <= 1e-8 ||R_0|| (FP64
+residual) with FP32 assembly, SpMV, ILU(0) and PCG (the mixed path).  Inputs
+are synthetic and already resident in HBM when the timed region starts.
+
+value = DOFs x (load steps converged) / wall seconds of the timed steps.
+The same steps are then re-run from the same state on the plain FP64 GPU path,
+and the FP64 CPU oracle is timed on a bounded sample on the host cores
+(assembly, residual, ILU(0) factorization, PCG iterations) and extrapolated
+with the operation counts of the FP64 GPU run.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "atlas-flexmesh_amd"))
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_bytes(info, kind, v):
+    """Algorithmic bytes per launch of kernel class `kind` (v = value bytes, 4-byte indices)."""
+    n, nnz, ne, nn = info["n"], info["nnz"], info["ne"], info["nn"]
+    i = 4
+    nlow = (nnz - n) // 2  # strictly lower entries (structurally symmetric pattern)
+    if kind == "spmv":
+        # values + column indices once, row pointers, x once, y once (+ fused p.Ap reads x again, cached)
+        return nnz * (v + i) + n * (i + 2 * v)
+    if kind == "ilu_fwd":
+        # L part values + columns, row pointers, rhs in, z out; fused PCG update reads r, Ap, p, x, writes r, x
+        return nlow * (v + i) + n * (i + 2 * v) + n * 6 * v
+    if kind == "ilu_bwd":
+        # U part (incl. diagonal) values + columns, row pointers, z and r in, y out
+        return (nnz - nlow) * (v + i) + n * (i + 3 * v)
+    if kind == "ilu_factor":
+        return nnz * (2 * v + i) + n * i
+    if kind == "assemble":
+        # per colour launch: the compulsory traffic of one assembly divided over the colour launches
+        return (ne * (16 + 2 * 8) + nn * 24 + n * 8 + nnz * v + n * 8) // max(info["ncolors"], 1)
+    if kind == "residual":
+        return (ne * (16 + 2 * 8) + nn * 24 + n * 8 + n * 8) // max(info["ncolors"], 1)
+    return 0
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return platform.processor() or "unknown"
+
+
+def physical_cores():
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True).stdout
+        cores = {l for l in out.splitlines() if l and not l.startswith("#")}
+        return max(1, len(cores))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(cfg, u_start, step, counts, S, budget_s):
+    """Time the FP64 CPU oracle on a bounded sample of the same load step and
+    extrapolate one load step with the FP64 GPU run's operation counts."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    cores = physical_cores()
+    O.set_threads(cores)
+    X, conn, mu, lam = cfg["X"], cfg["conn"], cfg["mu"], cfg["lam"]
+    rp, ci = S.pattern()  # bit-exact with the oracle's pattern (tested)
+    colors, nc = S.colors()
+    lf = step / cfg["n_steps"]
+    u = u_start.copy()
+    isd = np.zeros(u.size, dtype=np.uint8)
+    isd[cfg["dir_dofs"]] = 1
+    t = time.perf_counter()
+    K, f = O.assemble(X, conn, mu, lam, colors, nc, u, rp, ci)
+    O.eliminate(rp, ci, K, isd)
+    t_asm = time.perf_counter() - t
+    t = time.perf_counter()
+    _, f = O.assemble(X, conn, mu, lam, colors, nc, u, rp, ci, want_K=False)
+    t_res = time.perf_counter() - t
+    R = -f
+    R[isd == 1] = 0.0
+    t = time.perf_counter()
+    LU, diag = O.ilu0(rp, ci, K)
+    t_fac = time.perf_counter() - t
+    # k iterations of the oracle's pcg_solve_ilu0 (which factorizes first)
+    k = max(1, int(budget_s))
+    t = time.perf_counter()
+    O.pcg(rp, ci, K, R, tol=1e-30, max_iter=k)
+    t_it = max(time.perf_counter() - t - t_fac, 1e-9) / k
+    n_it = k
+    t_step = (counts["assemblies"] * t_asm + counts["residuals"] * t_res + counts["factorizations"] * t_fac +
+              counts["pcg_iters"] * t_it)
+    dofs = u.size
+    sample = (f"one FP64 assembly ({t_asm:.2f} s), one residual ({t_res:.2f} s), one natural-order ILU(0) "
+              f"factorization ({t_fac:.2f} s) and {n_it} PCG iterations ({t_it:.3f} s each) of load step {step} at "
+              f"{dofs} DOFs on {cores} cores ({cpu_model()}); one load step extrapolated with the FP64 GPU run's "
+              f"counts ({counts['assemblies']} assemblies, {counts['residuals']} residuals, "
+              f"{counts['factorizations']} factorizations, {counts['pcg_iters']} PCG iterations) = {t_step:.1f} s")
+    return {"value": dofs / t_step, "unit": "DOFs/s", "cores": cores, "kind": "port", "sample": sample,
+            "seconds_per_load_step": t_step, "t_assemble": t_asm, "t_residual": t_res, "t_factor": t_fac,
+            "t_pcg_iter": t_it}
+
+
+def run_steps(S, cfg, prec, first, count, inner_tol, profile):
+    import flexmesh as fm
+
+    c = fm.configs.newton_cfg(cfg, prec=prec, inner_tol=inner_tol)
+    tot = dict(newton_iters=0, pcg_iters=0, assemblies=0, residuals=0, factorizations=0, converged=0)
+    worst = 0.0
+    if profile:
+        S.profile(True)
+        S.profile_reset()
+    t0 = time.perf_counter()
+    for s in range(first, first + count):
+        rc, st = S.newton_step(s, c, check=False)
+        for k in tot:
+            tot[k] += st[k]
+        worst = max(worst, st["rel_res"])
+        log(f"  {'fp64' if prec == fm.FM_FP64 else 'mixed'} step {s}: newton {st['newton_iters']} pcg "
+            f"{st['pcg_iters']} rel_res {st['rel_res']:.2e} {st['seconds']:.2f} s")
+    wall = time.perf_counter() - t0
+    prof = S.profile_get() if profile else None
+    if profile:
+        S.profile(False)
+    return wall, tot, worst, prof
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=3)
+    ap.add_argument("--dims", type=str, default=None, help="override hex counts, e.g. 64,64,64")
+    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=5.0, help="PCG iterations in the CPU sample")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this path "
+              "is a design note, not a measured configuration", file=sys.stderr)
+        sys.exit(2)
+    import flexmesh as fm
+
+    dims = tuple(int(v) for v in args.dims.split(",")) if args.dims else None
+    t = time.perf_counter()
+    cfg = fm.configs.make(args.config, dims=dims)
+    log(f"config {args.config} dims {cfg['dims']}: {cfg['X'].size} DOFs, {cfg['conn'].size // 4} tets "
+        f"({time.perf_counter() - t:.1f} s)")
+    if args.warmup + args.steps > cfg["n_steps"]:
+        raise SystemExit("warmup + steps exceeds the load steps of the configuration")
+    S = fm.Solver(0)
+    t = time.perf_counter()
+    fm.configs.load(S, cfg)
+    info = S.info()
+    log(f"set-up (pattern, colouring, schedule, upload) {time.perf_counter() - t:.1f} s: {info}")
+    mixed = cfg["prec"] == fm.FM_MIXED
+    prec = fm.FM_MIXED if mixed else fm.FM_FP64
+    if args.warmup:
+        run_steps(S, cfg, prec, 1, args.warmup, args.inner_tol, False)
+    u0 = S.get_u() if args.warmup else np.zeros(cfg["X"].size)
+    first = args.warmup + 1
+    wall, tot, worst, prof = run_steps(S, cfg, prec, first, args.steps, args.inner_tol, True)
+    dofs = cfg["X"].size
+    value = dofs * tot["converged"] / wall
+    out = {
+        "metric": "Newton DOFs/s to 1e-8 rel. residual (FP32-mixed; FP64 GPU and CPU beside it)",
+        "value": value, "unit": "DOFs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32-mixed" if mixed else "f64", "data": "synthetic (seeded mesh jitter and lognormal fields)",
+        "config": {"workload": f"config {args.config}: {cfg['dims'][0]}x{cfg['dims'][1]}x{cfg['dims'][2]} hexes x6 "
+                               f"Kuhn tets, {dofs} DOFs, load steps {first}..{first + args.steps - 1} of "
+                               f"{cfg['n_steps']}", "dofs": dofs, "tets": info["ne"], "nnz": info["nnz"]},
+        "newton": {"converged_steps": tot["converged"], "newton_iters": tot["newton_iters"],
+                   "pcg_iters": tot["pcg_iters"], "newton_it_per_s": tot["newton_iters"] / wall,
+                   "worst_rel_res": worst},
+    }
+    # roofline of the dominant kernel class
+    v = 4 if mixed else 8
+    kern = {}
+    for k, (ms, nl) in prof.items():
+        if nl:
+            b = kernel_bytes(info, k, v)
+            kern[k] = {"ms_total": ms, "launches": nl, "avg_us": 1e3 * ms / nl, "bytes_per_launch": b,
+                       "GBps": b / (ms / nl * 1e-3) / 1e9 if b else None}
+    dom = max(kern, key=lambda k: kern[k]["ms_total"])
+    d = kern[dom]
+    out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": d["GBps"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                       "frac": d["GBps"] / PEAK_HBM_GBS, "traffic": None}
+    out["kernels"] = kern
+    if "spmv" in kern:
+        out["spmv_frac_of_peak"] = kern["spmv"]["GBps"] / PEAK_HBM_GBS
+    # FP64 GPU path on the same steps from the same state
+    if mixed and not args.no_fp64:
+        S.set_u(u0)
+        wall64, tot64, worst64, _ = run_steps(S, cfg, fm.FM_FP64, first, args.steps, args.inner_tol, False)
+        out["fp64_gpu"] = {"value": dofs * tot64["converged"] / wall64, "unit": "DOFs/s",
+                           "ms_per_step": 1e3 * wall64 / args.steps, "newton_iters": tot64["newton_iters"],
+                           "pcg_iters": tot64["pcg_iters"], "worst_rel_res": worst64}
+        out["mixed_over_fp64"] = value / out["fp64_gpu"]["value"]
+        u64 = S.get_u()
+        S.set_u(u0)
+        run_steps(S, cfg, prec, first, args.steps, args.inner_tol, False)
+        umix = S.get_u()
+        out["mixed_vs_fp64_u_rel_err"] = float(np.abs(umix - u64).max() / np.abs(u64).max())
+        counts = {k: tot64[k] / args.steps for k in ("assemblies", "residuals", "factorizations", "pcg_iters")}
+    else:
+        counts = {k: tot[k] / args.steps for k in ("assemblies", "residuals", "factorizations", "pcg_iters")}
+    if not args.no_cpu:
+        cb = cpu_baseline(cfg, u0, first, counts, S, args.cpu_budget)
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["gpu_over_cpu"] = value / cb["value"]
+    S.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
