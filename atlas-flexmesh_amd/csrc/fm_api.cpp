@@ -90,7 +90,8 @@ struct ProfScope {
 };
 
 int check_launch(fm_ctx* ctx, const char* what) {
-    hipError_t e = hipGetLastError();
+    hipError_t e = take_launch_error();
+    if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) return fail(ctx, FM_EHIP, std::string(what) + ": " + hipGetErrorString(e));
     return FM_OK;
 }

@@ -170,6 +170,8 @@ struct ElemArgs {
     const double* u;       // 3nn or nullptr
 };
 
+// error of the last failed cooperative launch (cleared when read)
+hipError_t take_launch_error();
 void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v);
 void launch_fill_f64(hipStream_t s, double* p, size_t n, double v);
 void launch_fill_f32(hipStream_t s, float* p, size_t n, float v);

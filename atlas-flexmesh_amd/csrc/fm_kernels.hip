@@ -348,9 +348,23 @@ __global__ __launch_bounds__(256) void spmv_kernel(uint32_t n, const uint32_t* _
     const int l = threadIdx.x % LPR;
     T sum = 0;
     if (row < n) {
+        // UF entries per lane are loaded before any x gather, so a wave keeps
+        // UF times more bytes in flight (the kernel is bound by memory-level
+        // parallelism, not by instruction count)
+        constexpr int UF = 4;
         const uint32_t e = rp[row + 1];
-        for (uint32_t k = rp[row] + l; k < e; k += LPR)
-            sum += __builtin_nontemporal_load(val + k) * x[__builtin_nontemporal_load(ci + k)];
+        for (uint32_t k0 = rp[row] + l; k0 < e; k0 += LPR * UF) {
+            T v[UF];
+            uint32_t c[UF];
+#pragma unroll
+            for (int u = 0; u < UF; ++u) {
+                const uint32_t k = k0 + u * LPR;
+                v[u] = k < e ? __builtin_nontemporal_load(val + k) : (T)0;
+                c[u] = k < e ? __builtin_nontemporal_load(ci + k) : row;
+            }
+#pragma unroll
+            for (int u = 0; u < UF; ++u) sum += v[u] * x[c[u]];
+        }
     }
 #pragma unroll
     for (int o = LPR / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
@@ -366,6 +380,70 @@ __global__ __launch_bounds__(256) void spmv_kernel(uint32_t n, const uint32_t* _
     if (DOT) {
         const double bs = block_sum(part);
         if (threadIdx.x == 0) partials[blk] = bs;
+    }
+}
+
+// CSR-stream SpMV: a block owns a tile of TR consecutive rows, streams the
+// tile's contiguous nonzeros with every load of a pass issued before any
+// gather (U per thread), stages the products in LDS and reduces one row per
+// thread.  Tiles whose nonzeros exceed the LDS stage fall back to lanes-per-row.
+constexpr int kTR = 64;          // rows per tile
+constexpr int kStage = 64 * kTR; // products staged per tile
+
+template <typename T, bool DOT>
+__global__ __launch_bounds__(256) void spmv_stream_kernel(uint32_t n, const uint32_t* __restrict__ rp,
+                                                          const uint32_t* __restrict__ ci, const T* __restrict__ val,
+                                                          const T* __restrict__ x, T* __restrict__ y,
+                                                          double* __restrict__ partials, T* __restrict__ sent0,
+                                                          T* __restrict__ sent1, const PcgState* __restrict__ st) {
+    if (st && st->done) return;
+    __shared__ uint32_t srp[kTR + 1];
+    __shared__ T sprod[kStage];
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t r0 = tile * kTR;
+    const uint32_t nr = min((uint32_t)kTR, n - r0);
+    const int tid = threadIdx.x;
+    if (tid <= (int)nr) srp[tid] = rp[r0 + tid];
+    __syncthreads();
+    const uint32_t base = srp[0];
+    const uint32_t cnt = srp[nr] - base;
+    T sum = 0;
+    if (cnt <= (uint32_t)kStage) {
+        constexpr int U = 8;
+        for (uint32_t k0 = tid; k0 < cnt; k0 += 256 * U) {
+            T v[U];
+            uint32_t c[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t k = k0 + u * 256;
+                v[u] = k < cnt ? __builtin_nontemporal_load(val + base + k) : (T)0;
+                c[u] = k < cnt ? __builtin_nontemporal_load(ci + base + k) : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t k = k0 + u * 256;
+                if (k < cnt) sprod[k] = v[u] * x[c[u]];
+            }
+        }
+        __syncthreads();
+        if (tid < (int)nr)
+            for (uint32_t k = srp[tid] - base; k < srp[tid + 1] - base; ++k) sum += sprod[k];
+    } else if (tid < (int)nr) {
+        for (uint32_t k = srp[tid]; k < srp[tid + 1]; ++k) sum += val[k] * x[ci[k]];
+    }
+    double part = 0.0;
+    if (tid < (int)nr) {
+        const uint32_t row = r0 + tid;
+        y[row] = sum;
+        if (sent0) {
+            sent0[row] = sentinel_value<T>();
+            sent1[row] = sentinel_value<T>();
+        }
+        if (DOT) part = (double)sum * (double)x[row];
+    }
+    if (DOT) {
+        const double bs = block_sum(part);
+        if (tid == 0) partials[tile] = bs;
     }
 }
 
@@ -489,29 +567,26 @@ __device__ __forceinline__ void factor_node(uint32_t node, const uint32_t* __res
     }
 }
 
-// persistent: a bounded grid of waves takes one node per ticket, in level order
+// persistent and cooperative, like the sweeps: wave w takes queue entries w, w+P, ...
 template <typename T, int B, int S>
 __global__ __launch_bounds__(64) void ilu_factor_kernel(uint32_t nq, const uint32_t* __restrict__ order,
                                                         const uint32_t* __restrict__ rp,
                                                         const uint32_t* __restrict__ ci,
                                                         const uint16_t* __restrict__ nlow, const T* __restrict__ A,
                                                         T* LU, uint32_t* ticket, int* err) {
-    uint32_t t = 0;
-    if (threadIdx.x == 0) t = atomicAdd(ticket, 1u);
-    t = __shfl(t, 0);
-    while (t < nq) {
-        uint32_t tn = 0;
-        if (threadIdx.x == 0) tn = atomicAdd(ticket, 1u);
+    for (uint32_t t = blockIdx.x; t < nq; t += gridDim.x) {
         const uint32_t node = order[t];
         if (node != kNoNode) factor_node<T, B, S>(node, rp, ci, nlow, A, LU, err);
-        t = __shfl(tn, 0);
     }
 }
 
-// Triangular sweeps.  Both are persistent: a bounded grid of waves loops
-// over tickets of the level-ordered queue (the next ticket is claimed before
-// the current one is worked, to hide the atomic), so only waves near the
-// dependence frontier exist and spin.  Everything a node needs that does not
+// Triangular sweeps.  Both are persistent and launched cooperatively (all
+// waves co-resident): wave w works queue chunks w, w+P, w+2P, ... in order.
+// The lowest unfinished chunk therefore always belongs to a running wave and
+// depends only on lower, finished chunks, so the sweep always progresses; and
+// the P waves in flight cover a window of consecutive chunks at the dependence
+// frontier.  (A shared ticket counter was the bottleneck: one atomic per chunk
+// on one address serializes at ~80 M/s.)  Everything a node needs that does not
 // depend on other nodes (matrix values, columns, right-hand side, PCG vectors,
 // diagonal-block entries) is loaded before the first wait.
 constexpr int kMJ = 2;  // lower/upper entries per lane kept in registers (more fall back to a loop)
@@ -621,15 +696,8 @@ __global__ __launch_bounds__(64) void ilu_fwd_kernel(uint32_t nq, const uint32_t
     const T alpha = UPD ? (T)st->alpha : (T)0;
     constexpr int NPW = 64 / LPN;
     const uint32_t ntk = nq / NPW;
-    uint32_t t = 0;
-    if (threadIdx.x == 0) t = atomicAdd(&st->ticket[0], 1u);
-    t = __shfl(t, 0);
-    while (t < ntk) {
-        uint32_t tn = 0;
-        if (threadIdx.x == 0) tn = atomicAdd(&st->ticket[0], 1u);
+    for (uint32_t t = blockIdx.x; t < ntk; t += gridDim.x)
         fwd_ticket<T, B, LPN, UPD>(t, nq, order, rp, ci, nlow, LU, rhs, r, x, p, Ap, z, alpha, err);
-        t = __shfl(tn, 0);
-    }
 }
 
 // backward sweep y = U^{-1} z with the diagonal u_ii at the diagonal
@@ -728,19 +796,13 @@ __global__ __launch_bounds__(64) void ilu_bwd_kernel(uint32_t nq, const uint32_t
     if (check_done && st->done) return;
     constexpr int NPW = 64 / LPN;
     const uint32_t ntk = nq / NPW;
-    uint32_t t = 0;
-    if (threadIdx.x == 0) t = atomicAdd(&st->ticket[1], 1u);
-    t = __shfl(t, 0);
-    while (t < ntk) {
-        uint32_t tn = 0;
-        if (threadIdx.x == 0) tn = atomicAdd(&st->ticket[1], 1u);
-        double part = bwd_ticket<T, B, LPN>(t, nq, order, rp, ci, nlow, LU, z, partials ? r : nullptr, y, err);
-        if (partials) {
+    double acc = 0.0;  // this wave's chunks, in increasing order: a fixed summation order
+    for (uint32_t t = blockIdx.x; t < ntk; t += gridDim.x)
+        acc += bwd_ticket<T, B, LPN>(t, nq, order, rp, ci, nlow, LU, z, partials ? r : nullptr, y, err);
+    if (partials) {
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
-            if (threadIdx.x == 0) partials[t] = part;
-        }
-        t = __shfl(tn, 0);
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        if (threadIdx.x == 0) partials[blockIdx.x] = acc;
     }
 }
 
@@ -749,8 +811,17 @@ __global__ __launch_bounds__(64) void ilu_bwd_kernel(uint32_t nq, const uint32_t
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void reduce_kernel(const double* __restrict__ partials, uint32_t np,
                                                       PcgState* st, int op, int k) {
-    double s = 0.0;
-    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) s += partials[i];
+    // eight independent accumulators per thread (loads in flight), combined
+    // in a fixed order: deterministic for a given partial count
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t stride = blockDim.x * 8;
+    for (uint32_t i0 = threadIdx.x; i0 < np; i0 += stride)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint32_t i = i0 + u * blockDim.x;
+            if (i < np) acc[u] += partials[i];
+        }
+    double s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     s = block_sum(s);
     if (threadIdx.x != 0) return;
     switch (op) {
@@ -1078,8 +1149,32 @@ uint32_t env_waves(const char* name, uint32_t dflt) {
     const long v = e ? std::atol(e) : 0;
     return (uint32_t)(v > 0 ? v : dflt);
 }
+hipError_t g_launch_err = hipSuccess;
+
+// cooperative launch of a persistent kernel with at most `want` one-wave
+// blocks, clamped to what the device can keep resident at once
+template <typename... Args>
+uint32_t launch_coop(void (*kern)(Args...), uint32_t want, hipStream_t s, Args... args) {
+    static thread_local int dev = -1, cus = 0;
+    int d = 0;
+    (void)hipGetDevice(&d);
+    if (d != dev) {
+        dev = d;
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d);
+    }
+    int occ = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(kern), 64, 0);
+    const uint32_t cap = (uint32_t)std::max(1, occ * cus);
+    const uint32_t grid = std::max(1u, std::min(want, cap));
+    void* argv[] = {reinterpret_cast<void*>(&args)...};
+    const hipError_t e =
+        hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3(grid), dim3(64), argv, 0, s);
+    if (e != hipSuccess) g_launch_err = e;
+    return grid;
+}
+
 uint32_t solve_waves() {
-    static const uint32_t w = env_waves("FM_SOLVE_WAVES", 512);
+    static const uint32_t w = env_waves("FM_SOLVE_WAVES", 2048);
     return w;
 }
 uint32_t factor_waves() {
@@ -1092,6 +1187,12 @@ uint32_t factor_waves() {
 // ===========================================================================
 // launchers
 // ===========================================================================
+hipError_t take_launch_error() {
+    const hipError_t e = g_launch_err;
+    g_launch_err = hipSuccess;
+    return e;
+}
+
 void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v) {
     if (n) fill_kernel<uint8_t><<<std::min<uint64_t>(cdiv(n, 256), 65535), 256, 0, s>>>(p, n, v);
 }
@@ -1156,6 +1257,8 @@ void launch_reduce(hipStream_t s, const double* partials, uint32_t np, PcgState*
 }
 
 static int spmv_lpr(const Matrix& A) {
+    static const int forced = (int)env_waves("FM_SPMV_LPR", 0);
+    if (forced == 4 || forced == 8 || forced == 16 || forced == 32) return forced;
     const double avg = A.n ? (double)A.nnz / A.n : 1.0;
     if (avg <= 6) return 4;
     if (avg <= 14) return 8;
@@ -1164,11 +1267,30 @@ static int spmv_lpr(const Matrix& A) {
     return 32;
 }
 
-uint32_t spmv_blocks(const Matrix& A) { return cdiv(A.n, 256 / spmv_lpr(A)); }
+static bool spmv_use_stream(const Matrix& A) {
+    static const int forced = (int)env_waves("FM_SPMV_STREAM", 2);  // 1 on, anything else: automatic
+    if (forced == 1) return true;
+    if (forced == 3) return false;
+    return A.n && (double)A.nnz / A.n <= 48.0;
+}
+
+uint32_t spmv_blocks(const Matrix& A) {
+    return spmv_use_stream(A) ? cdiv(A.n, kTR) : cdiv(A.n, 256 / spmv_lpr(A));
+}
 
 template <typename T>
 static void spmv_t(hipStream_t s, const Matrix& A, const T* val, const T* x, T* y, bool dot, double* partials,
                    T* s0, T* s1, const PcgState* st) {
+    if (spmv_use_stream(A)) {
+        const uint32_t nb = cdiv(A.n, kTR);
+        if (dot)
+            spmv_stream_kernel<T, true><<<nb, 256, 0, s>>>(A.n, A.row_ptr.p, A.col_idx.p, val, x, y, partials, s0, s1,
+                                                          st);
+        else
+            spmv_stream_kernel<T, false><<<nb, 256, 0, s>>>(A.n, A.row_ptr.p, A.col_idx.p, val, x, y, partials, s0,
+                                                           s1, st);
+        return;
+    }
     const int lpr = spmv_lpr(A);
     const uint32_t nb = cdiv(A.n, 256 / lpr);
 #define FM_SPMV(L)                                                                                         \
@@ -1208,19 +1330,22 @@ static void factor_t(hipStream_t s, const Matrix& A, const T* val, T* lu, PcgSta
     const Schedule& S = A.sched;
     fill_sentinel_kernel<T><<<std::min<uint64_t>(cdiv(A.nnz, 256), 65535), 256, 0, s>>>(lu, nullptr, A.nnz);
     const uint32_t nb = std::min(S.q_fwd, factor_waves());
-    uint32_t* tk = &st->ticket[2];
+    const uint32_t q = S.q_fwd;
     switch (S.slots) {
         case 1:
-            ilu_factor_kernel<T, B, 1><<<nb, 64, 0, s>>>(S.q_fwd, S.fwd_order.p, A.row_ptr.p, A.col_idx.p, S.nlow.p, val,
-                                                         lu, tk, err);
+            launch_coop(ilu_factor_kernel<T, B, 1>, nb, s, q, (const uint32_t*)S.fwd_order.p,
+                        (const uint32_t*)A.row_ptr.p, (const uint32_t*)A.col_idx.p, (const uint16_t*)S.nlow.p, val,
+                        lu, &st->ticket[2], err);
             break;
         case 2:
-            ilu_factor_kernel<T, B, 2><<<nb, 64, 0, s>>>(S.q_fwd, S.fwd_order.p, A.row_ptr.p, A.col_idx.p, S.nlow.p, val,
-                                                         lu, tk, err);
+            launch_coop(ilu_factor_kernel<T, B, 2>, nb, s, q, (const uint32_t*)S.fwd_order.p,
+                        (const uint32_t*)A.row_ptr.p, (const uint32_t*)A.col_idx.p, (const uint16_t*)S.nlow.p, val,
+                        lu, &st->ticket[2], err);
             break;
         default:
-            ilu_factor_kernel<T, B, 4><<<nb, 64, 0, s>>>(S.q_fwd, S.fwd_order.p, A.row_ptr.p, A.col_idx.p, S.nlow.p, val,
-                                                         lu, tk, err);
+            launch_coop(ilu_factor_kernel<T, B, 4>, nb, s, q, (const uint32_t*)S.fwd_order.p,
+                        (const uint32_t*)A.row_ptr.p, (const uint32_t*)A.col_idx.p, (const uint16_t*)S.nlow.p, val,
+                        lu, &st->ticket[2], err);
             break;
     }
 }
@@ -1244,12 +1369,15 @@ static void fwd_t(hipStream_t s, const Matrix& A, const T* lu, const T* rhs, T* 
                   T* z, PcgState* st, bool upd, int* err) {
     const Schedule& S = A.sched;
     const uint32_t nb = std::min(cdiv(S.q_fwd, 64 / LPN), solve_waves());
+    const uint32_t q = S.q_fwd;
     if (upd)
-        ilu_fwd_kernel<T, B, LPN, true><<<nb, 64, 0, s>>>(S.q_fwd, S.fwd_order.p, A.row_ptr.p, A.col_idx.p,
-                                                          S.nlow.p, lu, rhs, r, x, p, Ap, z, st, err);
+        launch_coop(ilu_fwd_kernel<T, B, LPN, true>, nb, s, q, (const uint32_t*)S.fwd_order.p,
+                    (const uint32_t*)A.row_ptr.p, (const uint32_t*)A.col_idx.p, (const uint16_t*)S.nlow.p, lu, rhs, r,
+                    x, p, Ap, z, st, err);
     else
-        ilu_fwd_kernel<T, B, LPN, false><<<nb, 64, 0, s>>>(S.q_fwd, S.fwd_order.p, A.row_ptr.p, A.col_idx.p,
-                                                           S.nlow.p, lu, rhs, r, x, p, Ap, z, st, err);
+        launch_coop(ilu_fwd_kernel<T, B, LPN, false>, nb, s, q, (const uint32_t*)S.fwd_order.p,
+                    (const uint32_t*)A.row_ptr.p, (const uint32_t*)A.col_idx.p, (const uint16_t*)S.nlow.p, lu, rhs, r,
+                    x, p, Ap, z, st, err);
 }
 
 template <typename T>
@@ -1284,9 +1412,9 @@ static uint32_t bwd_t(hipStream_t s, const Matrix& A, const T* lu, const T* z, c
     const Schedule& S = A.sched;
     const uint32_t ntk = cdiv(S.q_bwd, 64 / LPN);
     const uint32_t nb = std::min(ntk, solve_waves());
-    ilu_bwd_kernel<T, B, LPN><<<nb, 64, 0, s>>>(S.q_bwd, S.bwd_order.p, A.row_ptr.p, A.col_idx.p, S.nlow.p, lu, z, r,
-                                               y, partials, st, check_done ? 1 : 0, err);
-    return ntk;  // one partial per ticket
+    return launch_coop(ilu_bwd_kernel<T, B, LPN>, nb, s, S.q_bwd, (const uint32_t*)S.bwd_order.p,
+                       (const uint32_t*)A.row_ptr.p, (const uint32_t*)A.col_idx.p, (const uint16_t*)S.nlow.p, lu, z, r,
+                       y, partials, st, check_done ? 1 : 0, err);  // one partial per wave
 }
 
 template <typename T>

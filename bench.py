@@ -126,10 +126,10 @@ def cpu_baseline(cfg, u_start, step, counts, S, budget_s):
             "t_pcg_iter": t_it}
 
 
-def run_steps(S, cfg, prec, first, count, inner_tol, profile):
+def run_steps(S, cfg, prec, first, count, inner_tol, profile, pcg_max=300):
     import flexmesh as fm
 
-    c = fm.configs.newton_cfg(cfg, prec=prec, inner_tol=inner_tol)
+    c = fm.configs.newton_cfg(cfg, prec=prec, inner_tol=inner_tol, pcg_max=pcg_max)
     tot = dict(newton_iters=0, pcg_iters=0, assemblies=0, residuals=0, factorizations=0, converged=0)
     worst = 0.0
     if profile:
@@ -150,6 +150,32 @@ def run_steps(S, cfg, prec, first, count, inner_tol, profile):
     return wall, tot, worst, prof
 
 
+def micro(S, cfg, info, iters):
+    """Per-kernel timings on the tangent at u = 0 with the Dirichlet rows eliminated."""
+    import flexmesh as fm
+
+    out = {"config": cfg["k"], "dims": cfg["dims"], "info": info}
+    b = np.random.default_rng(0).standard_normal(cfg["X"].size)
+    b[cfg["dir_dofs"]] = 0.0
+    for name, prec, v in (("mixed", fm.FM_MIXED, 4), ("fp64", fm.FM_FP64, 8)):
+        S.profile(True)
+        S.profile_reset()
+        S.assemble(None, prec, want_K=False)
+        S.use_tangent(True)
+        t = time.perf_counter()
+        _, its, rel = S.pcg(b, tol=1e-30, max_it=iters, prec=prec)
+        wall = time.perf_counter() - t
+        prof = S.profile_get()
+        S.profile(False)
+        out[name] = {"pcg_iters": its, "ms_per_pcg_iter": 1e3 * wall / max(its, 1), "relres": rel,
+                     "kernels": {k: {"avg_us": 1e3 * ms / n, "launches": n,
+                                     "GBps": kernel_bytes(info, k, v) / (ms / n * 1e-3) / 1e9}
+                                 for k, (ms, n) in prof.items() if n}}
+        log(f"{name}: {out[name]['ms_per_pcg_iter']:.2f} ms/iter " +
+            " ".join(f"{k}={d['avg_us']:.0f}us/{d['GBps']:.0f}GB/s" for k, d in out[name]["kernels"].items()))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,9 +184,12 @@ def main():
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--dims", type=str, default=None, help="override hex counts, e.g. 64,64,64")
     ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--pcg-max", type=int, default=300, help="PCG iteration cap of the mixed path")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=5.0, help="PCG iterations in the CPU sample")
+    ap.add_argument("--micro", type=int, default=0,
+                    help="kernel timing only: one assembly, one factorization and this many PCG iterations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this path "
@@ -182,11 +211,14 @@ def main():
     log(f"set-up (pattern, colouring, schedule, upload) {time.perf_counter() - t:.1f} s: {info}")
     mixed = cfg["prec"] == fm.FM_MIXED
     prec = fm.FM_MIXED if mixed else fm.FM_FP64
+    if args.micro:
+        micro(S, cfg, info, args.micro)
+        return
     if args.warmup:
-        run_steps(S, cfg, prec, 1, args.warmup, args.inner_tol, False)
+        run_steps(S, cfg, prec, 1, args.warmup, args.inner_tol, False, args.pcg_max)
     u0 = S.get_u() if args.warmup else np.zeros(cfg["X"].size)
     first = args.warmup + 1
-    wall, tot, worst, prof = run_steps(S, cfg, prec, first, args.steps, args.inner_tol, True)
+    wall, tot, worst, prof = run_steps(S, cfg, prec, first, args.steps, args.inner_tol, True, args.pcg_max)
     dofs = cfg["X"].size
     value = dofs * tot["converged"] / wall
     out = {
@@ -226,7 +258,7 @@ def main():
         out["mixed_over_fp64"] = value / out["fp64_gpu"]["value"]
         u64 = S.get_u()
         S.set_u(u0)
-        run_steps(S, cfg, prec, first, args.steps, args.inner_tol, False)
+        run_steps(S, cfg, prec, first, args.steps, args.inner_tol, False, args.pcg_max)
         umix = S.get_u()
         out["mixed_vs_fp64_u_rel_err"] = float(np.abs(umix - u64).max() / np.abs(u64).max())
         counts = {k: tot64[k] / args.steps for k in ("assemblies", "residuals", "factorizations", "pcg_iters")}
