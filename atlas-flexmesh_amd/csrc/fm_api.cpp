@@ -975,10 +975,11 @@ void fm_newton_cfg_default(fm_newton_cfg* c) {
     c->max_newton = 30;
     c->pcg_tol = 1e-10;  // newton_tol * 0.01 (FEM/adaptive_fem_engine.hpp:2577-2578)
     c->pcg_max = 300;
-    c->inner_tol = 1e-4;
+    c->inner_tol = 1e-2;
     c->ls_max = 10;
     c->armijo_c = 1e-4;
     c->lift = 1;
+    c->inner_max = 1000;
 }
 
 int fm_set_u(fm_ctx* ctx, const double* u) {
@@ -1041,7 +1042,8 @@ int fm_newton_step(fm_ctx* ctx, uint32_t step, double* u, const fm_newton_cfg* c
         launch_sub_masked(s, n, ctx->R.p, ctx->lift.p, ctx->isdir.p, ctx->hv.p);
         int its = 0;
         const double tol = kprec == PREC64 ? cfg.pcg_tol : cfg.inner_tol;
-        FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, tol, cfg.pcg_max, &its));
+        const int cap = kprec == PREC64 ? cfg.pcg_max : cfg.inner_max;
+        FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, tol, cap, &its));
         S.pcg_iters += its;
         S.factorizations++;
         launch_apply_predictor(s, n, ctx->du.p, ctx->dD.p, ctx->isdir.p, ctx->u.p);
@@ -1058,9 +1060,10 @@ int fm_newton_step(fm_ctx* ctx, uint32_t step, double* u, const fm_newton_cfg* c
         S.assemblies++;
         int its = 0;
         const double tol = kprec == PREC64 ? cfg.pcg_tol : cfg.inner_tol;
+        const int cap = kprec == PREC64 ? cfg.pcg_max : cfg.inner_max;
         // R of the current u is in ctx->R (last residual evaluation)
         FM_HIP(hipMemcpyAsync(ctx->hv.p, ctx->R.p, n * 8ull, hipMemcpyDeviceToDevice, s));
-        FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, tol, cfg.pcg_max, &its));
+        FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, tol, cap, &its));
         S.pcg_iters += its;
         S.factorizations++;
         // armijo_line_search_true (FEM/adaptive_fem_engine.hpp:933-985)

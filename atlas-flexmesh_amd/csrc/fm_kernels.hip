@@ -13,6 +13,9 @@
 namespace fm {
 namespace {
 
+#ifndef FM_SPIN_SLEEP
+#define FM_SPIN_SLEEP 2
+#endif
 constexpr uint32_t kSpinLimit = 1u << 20;  // bounded wait (~1 s); then the kernel flags an error and drains
 constexpr uint32_t kNoNode = 0xFFFFFFFFu;   // padding entry of a level-ordered queue
 
@@ -66,10 +69,33 @@ __device__ __forceinline__ T await_val(const T* p, int* err) {
         }
         // once any wave has hit the limit every other wait gives up too
         if ((spins & 255u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-        __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_s_sleep(FM_SPIN_SLEEP);
         b = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     return __builtin_bit_cast(T, b);
+}
+
+// two waits whose loads are issued together (one round trip when both are ready)
+template <typename T>
+__device__ __forceinline__ void await2(const T* p0, const T* p1, bool two, T& v0, T& v1, int* err) {
+    using U = typename SB<T>::U;
+    U* q0 = const_cast<U*>(reinterpret_cast<const U*>(p0));
+    U* q1 = const_cast<U*>(reinterpret_cast<const U*>(p1));
+    U b0 = __hip_atomic_load(q0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    U b1 = two ? __hip_atomic_load(q1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (U)0;
+    uint32_t spins = 0;
+    while (b0 == SB<T>::S || b1 == SB<T>::S) {
+        if (++spins > kSpinLimit) {
+            atomicOr(err, 1);
+            break;
+        }
+        if ((spins & 255u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        __builtin_amdgcn_s_sleep(FM_SPIN_SLEEP);
+        if (b0 == SB<T>::S) b0 = __hip_atomic_load(q0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (b1 == SB<T>::S) b1 = __hip_atomic_load(q1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    v0 = __builtin_bit_cast(T, b0);
+    v1 = __builtin_bit_cast(T, b1);
 }
 
 template <typename T>
@@ -646,14 +672,15 @@ __device__ __forceinline__ void fwd_ticket(uint32_t t, uint32_t nq, const uint32
 #pragma unroll
     for (int d = 0; d < B; ++d) acc[d] = 0;
     if (valid) {
+        const bool h0 = (uint32_t)gl < pl, h1 = (uint32_t)gl + LPN < pl;
+        if (h0) {
+            T z0, z1;
+            await2(&z[c[0]], &z[h1 ? c[1] : c[0]], h1, z0, z1, err);
 #pragma unroll
-        for (int m = 0; m < kMJ; ++m) {
-            const uint32_t j = gl + m * LPN;
-            if (j < pl) {
-                const T zc = await_val(&z[c[m]], err);
+            for (int d = 0; d < B; ++d) acc[d] += l[0][d] * z0;
+            if (h1)
 #pragma unroll
-                for (int d = 0; d < B; ++d) acc[d] += l[m][d] * zc;
-            }
+                for (int d = 0; d < B; ++d) acc[d] += l[1][d] * z1;
         }
         for (uint32_t j = gl + kMJ * LPN; j < pl; j += LPN) {
             const T zc = await_val(&z[ci[rs[0] + j]], err);
@@ -748,14 +775,15 @@ __device__ __forceinline__ double bwd_ticket(uint32_t t, uint32_t nq, const uint
 #pragma unroll
     for (int d = 0; d < B; ++d) acc[d] = 0;
     if (valid) {
+        const bool h0 = us + gl < len, h1 = us + gl + LPN < len;
+        if (h0) {
+            T y0, y1;
+            await2(&y[c[0]], &y[h1 ? c[1] : c[0]], h1, y0, y1, err);
 #pragma unroll
-        for (int m = 0; m < kMJ; ++m) {
-            const uint32_t j = us + gl + m * LPN;
-            if (j < len) {
-                const T yc = await_val(&y[c[m]], err);
+            for (int d = 0; d < B; ++d) acc[d] += l[0][d] * y0;
+            if (h1)
 #pragma unroll
-                for (int d = 0; d < B; ++d) acc[d] += l[m][d] * yc;
-            }
+                for (int d = 0; d < B; ++d) acc[d] += l[1][d] * y1;
         }
         for (uint32_t j = us + gl + kMJ * LPN; j < len; j += LPN) {
             const T yc = await_val(&y[ci[rs[0] + j]], err);
@@ -1384,7 +1412,9 @@ template <typename T>
 static void fwd_dispatch(hipStream_t s, const Matrix& A, const T* lu, const T* rhs, T* r, T* x, const T* p,
                          const T* Ap, T* z, PcgState* st, bool upd, int* err) {
     if (A.sched.B == 3) {
-        if (A.sched.lpn == 16)
+        if (A.sched.lpn == 32)
+            fwd_t<T, 3, 32>(s, A, lu, rhs, r, x, p, Ap, z, st, upd, err);
+        else if (A.sched.lpn == 16)
             fwd_t<T, 3, 16>(s, A, lu, rhs, r, x, p, Ap, z, st, upd, err);
         else
             fwd_t<T, 3, 4>(s, A, lu, rhs, r, x, p, Ap, z, st, upd, err);
@@ -1421,8 +1451,9 @@ template <typename T>
 static uint32_t bwd_dispatch(hipStream_t s, const Matrix& A, const T* lu, const T* z, const T* r, T* y,
                              double* partials, PcgState* st, bool check_done, int* err) {
     if (A.sched.B == 3)
-        return A.sched.lpn == 16 ? bwd_t<T, 3, 16>(s, A, lu, z, r, y, partials, st, check_done, err)
-                                 : bwd_t<T, 3, 4>(s, A, lu, z, r, y, partials, st, check_done, err);
+        return A.sched.lpn == 32   ? bwd_t<T, 3, 32>(s, A, lu, z, r, y, partials, st, check_done, err)
+               : A.sched.lpn == 16 ? bwd_t<T, 3, 16>(s, A, lu, z, r, y, partials, st, check_done, err)
+                                   : bwd_t<T, 3, 4>(s, A, lu, z, r, y, partials, st, check_done, err);
     return A.sched.lpn == 16 ? bwd_t<T, 1, 16>(s, A, lu, z, r, y, partials, st, check_done, err)
                              : bwd_t<T, 1, 4>(s, A, lu, z, r, y, partials, st, check_done, err);
 }

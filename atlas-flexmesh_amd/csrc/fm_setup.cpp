@@ -4,6 +4,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "fm_internal.hpp"
@@ -158,6 +159,10 @@ bool build_schedule(const std::vector<uint32_t>& rp, const std::vector<uint32_t>
         LB = std::max(LB, l + 1);
     }
     S.lpn = (B == 3 || (nnode && (double)lower_total / nnode > 6.0)) ? 16 : 4;
+    if (B == 3) {
+        const char* e = std::getenv("FM_TRSV_LPN");  // lanes per node of the sweeps (16 or 32)
+        if (e && std::atoi(e) == 32) S.lpn = 32;
+    }
     const uint32_t npw = 64 / S.lpn;  // nodes per wave in the triangular solves
     auto queue = [npw](const std::vector<uint32_t>& level, uint32_t nlev, bool descending_ids) {
         std::vector<uint32_t> cnt(nlev + 1, 0);

@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libflexmesh.so")
+# FM_LIB_PATH selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("FM_LIB_PATH") or os.path.join(HERE, "libflexmesh.so")
 
 FM_OK, FM_EINVAL, FM_ESTATE, FM_EHIP, FM_ENOCONV, FM_EDEVICE = 0, -1, -2, -3, -4, -5
 FM_FP64, FM_MIXED = 0, 1
@@ -25,7 +26,7 @@ _vp = C.c_void_p
 class NewtonCfg(C.Structure):
     _fields_ = [("n_steps", C.c_uint32), ("prec", C.c_int), ("newton_rtol", C.c_double), ("max_newton", C.c_int),
                 ("pcg_tol", C.c_double), ("pcg_max", C.c_int), ("inner_tol", C.c_double), ("ls_max", C.c_int),
-                ("armijo_c", C.c_double), ("lift", C.c_int)]
+                ("armijo_c", C.c_double), ("lift", C.c_int), ("inner_max", C.c_int)]
 
 
 class NewtonStats(C.Structure):

@@ -126,10 +126,10 @@ def cpu_baseline(cfg, u_start, step, counts, S, budget_s):
             "t_pcg_iter": t_it}
 
 
-def run_steps(S, cfg, prec, first, count, inner_tol, profile, pcg_max=300):
+def run_steps(S, cfg, prec, first, count, inner_tol, profile, pcg_max=1000):
     import flexmesh as fm
 
-    c = fm.configs.newton_cfg(cfg, prec=prec, inner_tol=inner_tol, pcg_max=pcg_max)
+    c = fm.configs.newton_cfg(cfg, prec=prec, inner_tol=inner_tol, inner_max=pcg_max)
     tot = dict(newton_iters=0, pcg_iters=0, assemblies=0, residuals=0, factorizations=0, converged=0)
     worst = 0.0
     if profile:
@@ -183,8 +183,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--dims", type=str, default=None, help="override hex counts, e.g. 64,64,64")
-    ap.add_argument("--inner-tol", type=float, default=1e-4)
-    ap.add_argument("--pcg-max", type=int, default=300, help="PCG iteration cap of the mixed path")
+    ap.add_argument("--inner-tol", type=float, default=1e-2, help="FP32 PCG tolerance of the mixed path")
+    ap.add_argument("--pcg-max", type=int, default=1000, help="FP32 PCG iteration cap of the mixed path")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=5.0, help="PCG iterations in the CPU sample")

@@ -117,10 +117,11 @@ typedef struct {
     int max_newton;
     double pcg_tol;       /* pcg_solve_ilu0 tol (FP64 path and the FP64 lift solve) */
     int pcg_max;
-    double inner_tol;     /* FP32 PCG tol of the mixed path */
+    double inner_tol;     /* FP32 PCG tol of the mixed path (inexact-Newton forcing term) */
     int ls_max;           /* Armijo backtracks (armijo_line_search_true, :933-985) */
     double armijo_c;
     int lift;             /* 1: predictor solve with the prescribed increment */
+    int inner_max;        /* FP32 PCG iteration cap of the mixed path */
 } fm_newton_cfg;
 
 typedef struct {
