@@ -60,6 +60,21 @@ def kernel_bytes(info, kind, v):
     return 0
 
 
+def measured_traffic(kind, prec_name, dims):
+    """Calibrated HBM bytes per launch from the committed rocprofv3 PMC summary
+    of the full-size configuration 3 workload (None for other workloads)."""
+    if tuple(dims) != (128, 128, 128):
+        return None
+    path = os.path.join(ROOT, "profiles", "config3_128cube_pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 4 and f[0] == "TRAFFIC" and f[1] == kind and f[2] == prec_name:
+            return float(f[3])
+    return None
+
+
 def cpu_model():
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
@@ -244,7 +259,9 @@ def main():
     dom = max(kern, key=lambda k: kern[k]["ms_total"])
     d = kern[dom]
     out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": d["GBps"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                       "frac": d["GBps"] / PEAK_HBM_GBS, "traffic": None}
+                       "frac": d["GBps"] / PEAK_HBM_GBS,
+                       "traffic": measured_traffic(dom, "fp32" if mixed else "fp64", cfg["dims"]),
+                       "algorithmic_bytes": d["bytes_per_launch"]}
     out["kernels"] = kern
     if "spmv" in kern:
         out["spmv_frac_of_peak"] = kern["spmv"]["GBps"] / PEAK_HBM_GBS
