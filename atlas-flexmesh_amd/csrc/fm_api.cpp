@@ -1009,6 +1009,18 @@ int fm_newton_step(fm_ctx* ctx, uint32_t step, double* u, const fm_newton_cfg* c
     else
         fm_newton_cfg_default(&cfg);
     if (cfg.n_steps == 0 || step == 0 || step > cfg.n_steps) return fail(ctx, FM_EINVAL, "bad load step");
+    return fm_newton_solve_to(ctx, (double)step / (double)cfg.n_steps, u, &cfg, stats);
+}
+
+int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newton_cfg* cfg_in,
+                       fm_newton_stats* stats) {
+    if (!ctx) return FM_EINVAL;
+    fm_newton_cfg cfg;
+    if (cfg_in)
+        cfg = *cfg_in;
+    else
+        fm_newton_cfg_default(&cfg);
+    if (!(load_factor >= 0.0) || !std::isfinite(load_factor)) return fail(ctx, FM_EINVAL, "bad load factor");
     if (cfg.prec != FM_FP64 && cfg.prec != FM_MIXED) return fail(ctx, FM_EINVAL, "bad precision");
     FM_TRY(ensure_ready(ctx));
     const double t0 = now_s();
@@ -1023,7 +1035,7 @@ int fm_newton_step(fm_ctx* ctx, uint32_t step, double* u, const fm_newton_cfg* c
         ctx->u_valid = true;
     }
     const int kprec = cfg.prec == FM_FP64 ? PREC64 : PREC32;
-    const double lf = (double)step / (double)cfg.n_steps;
+    const double lf = load_factor;
     fm_newton_stats S{};
     const uint32_t ndir = (uint32_t)ctx->dir_dofs.size();
     bool any_inc = false;

@@ -115,7 +115,7 @@ typedef struct {
     int prec;             /* FM_FP64 or FM_MIXED */
     double newton_rtol;   /* ||R_k|| <= rtol ||R_0|| */
     int max_newton;
-    double pcg_tol;       /* pcg_solve_ilu0 tol (FP64 path and the FP64 lift solve) */
+    double pcg_tol;       /* pcg_solve_ilu0 tol of the FP64 path */
     int pcg_max;
     double inner_tol;     /* FP32 PCG tol of the mixed path (inexact-Newton forcing term) */
     int ls_max;           /* Armijo backtracks (armijo_line_search_true, :933-985) */
@@ -139,6 +139,10 @@ typedef struct {
 void fm_newton_cfg_default(fm_newton_cfg* cfg);
 /* u: 3nn doubles in/out (NULL: use and keep the context's device-resident state) */
 int fm_newton_step(fm_ctx* ctx, uint32_t step, double* u, const fm_newton_cfg* cfg, fm_newton_stats* stats);
+/* the same Newton solve to an arbitrary load factor (LoadStep::load_factor,
+   FEM/fem_types.hpp:266-271): prescribed values load_factor*dir_vals, external
+   force load_factor*f_ext */
+int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newton_cfg* cfg, fm_newton_stats* stats);
 /* device-resident displacement (3nn doubles) */
 int fm_get_u(fm_ctx* ctx, double* u);
 int fm_set_u(fm_ctx* ctx, const double* u);

@@ -1,0 +1,272 @@
+// fm_engine.hpp -- C++ drop-in for the engine's assembler and solver entry
+// points, running on one MI355X through the C ABI of fm_api.h.
+//
+// The functions keep the names, argument meaning and error behaviour of the
+// reference (namespace atlas::fem, FEM/adaptive_fem_engine.hpp and
+// FEM/nonlinear_solver.hpp) so host code can call atlas::fem::gpu::X beside
+// the CPU path atlas::fem::X.  They are templates over the mesh, problem and
+// CSR types and only use the members the reference declares:
+//   Mesh : nodes[i].x/.y/.z, elements[e].nodes[0..3], n_dofs         (FEM/fem_types.hpp:58-205)
+//   Prob : E, nu                                                      (FEM/fem_types.hpp:241-256)
+//   CSR  : n_rows, row_ptr, col_idx, val (std::vector)                (FEM/fem_types.hpp:304-320)
+// so the reference's own MeshTopology / ProblemDescriptor / SparseCSR work
+// unchanged.  Header-only; link with libflexmesh.so.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "flexmesh/fm_api.h"
+
+namespace atlas::fem::gpu {
+
+// One device context per thread (device 0 unless select_device() is called
+// first); the mesh is uploaded again only when its address or size changes.
+class Device {
+public:
+    static Device& get() {
+        thread_local Device d;
+        return d;
+    }
+    static void select_device(int dev) { get().device_ = dev; }
+
+    fm_ctx* ctx() {
+        if (!ctx_) {
+            const int rc = fm_create(device_, &ctx_);
+            if (rc != FM_OK) throw std::runtime_error("flexmesh: fm_create failed (" + std::to_string(rc) + ")");
+        }
+        return ctx_;
+    }
+
+    template <class Mesh>
+    void upload(const Mesh& mesh) {
+        const void* key = static_cast<const void*>(&mesh);
+        const size_t nn = mesh.nodes.size(), ne = mesh.elements.size();
+        if (key == mesh_key_ && nn == nn_ && ne == ne_) return;
+        std::vector<double> X(3 * nn);
+        for (size_t n = 0; n < nn; ++n) {
+            X[n] = mesh.nodes[n].x;
+            X[nn + n] = mesh.nodes[n].y;
+            X[2 * nn + n] = mesh.nodes[n].z;
+        }
+        std::vector<uint32_t> conn(4 * ne);
+        for (size_t e = 0; e < ne; ++e)
+            for (int a = 0; a < 4; ++a) conn[a * ne + e] = mesh.elements[e].nodes[a];
+        check(fm_set_mesh(ctx(), (uint32_t)nn, X.data(), (uint32_t)ne, conn.data()), "fm_set_mesh");
+        mesh_key_ = key;
+        nn_ = nn;
+        ne_ = ne;
+        E_ = nu_ = std::nan("");
+    }
+
+    // one (E, nu) per problem (ProblemDescriptor.E/.nu): the Lame parameters of
+    // NeoHookeanParameters::mu/lambda (FEM/constitutive_models.hpp:35-41)
+    void material(double E, double nu) {
+        if (E == E_ && nu == nu_) return;
+        const double mu = E / (2.0 * (1.0 + nu));
+        const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+        std::vector<double> m(ne_, mu), l(ne_, lam);
+        check(fm_set_material(ctx(), m.data(), l.data()), "fm_set_material");
+        E_ = E;
+        nu_ = nu;
+    }
+
+    void check(int rc, const char* what) {
+        if (rc != FM_OK)
+            throw std::runtime_error(std::string("flexmesh: ") + what + ": " + fm_last_error(ctx_));
+    }
+
+    ~Device() {
+        if (ctx_) fm_destroy(ctx_);
+    }
+
+private:
+    Device() = default;
+    fm_ctx* ctx_ = nullptr;
+    int device_ = 0;
+    const void* mesh_key_ = nullptr;
+    size_t nn_ = 0, ne_ = 0;
+    double E_ = std::nan(""), nu_ = std::nan("");
+};
+
+// build_csr_sparsity_pattern_rigorous (FEM/adaptive_fem_engine.hpp:1069-1120),
+// with the intended column rule (row 3n+d lists 3m..3m+2 for sorted m in nbr(n)).
+template <class Mesh, class CSR>
+void build_csr_sparsity_pattern_rigorous(const Mesh& mesh, CSR& K) {
+    Device& d = Device::get();
+    d.upload(mesh);
+    uint64_t nnz = 0;
+    d.check(fm_build_pattern(d.ctx(), nullptr, nullptr, &nnz), "fm_build_pattern");
+    K.n_rows = mesh.n_dofs;
+    K.row_ptr.assign(K.n_rows + 1, 0);
+    K.col_idx.assign(nnz, 0);
+    K.val.assign(nnz, 0.0);
+    d.check(fm_build_pattern(d.ctx(), K.row_ptr.data(), K.col_idx.data(), &nnz), "fm_build_pattern");
+}
+
+// assemble_global_threadlocal (FEM/adaptive_fem_engine.hpp:1126-1233): K.val
+// overwritten on K's (pre-built) pattern, f_int resized and overwritten.
+template <class Mesh, class Prob, class CSR>
+void assemble_global_threadlocal(const Mesh& mesh, const Prob& prob, const double* u, CSR& K,
+                                 std::vector<double>& f_int) {
+    Device& d = Device::get();
+    d.upload(mesh);
+    d.material(prob.E, prob.nu);
+    uint64_t nnz = 0;
+    d.check(fm_build_pattern(d.ctx(), nullptr, nullptr, &nnz), "fm_build_pattern");
+    if (K.col_idx.size() != nnz || K.row_ptr.size() != (size_t)mesh.n_dofs + 1)
+        throw std::runtime_error("flexmesh: K does not hold the pattern of build_csr_sparsity_pattern_rigorous");
+    K.val.resize(nnz);
+    f_int.assign(mesh.n_dofs, 0.0);
+    d.check(fm_assemble(d.ctx(), u, f_int.data(), K.val.data(), FM_FP64), "fm_assemble");
+}
+
+// assemble_global_residual_only (FEM/adaptive_fem_engine.hpp:990-1059)
+template <class Mesh, class Prob>
+void assemble_global_residual_only(const Mesh& mesh, const Prob& prob, const double* u,
+                                   std::vector<double>& f_int) noexcept {
+    try {
+        Device& d = Device::get();
+        d.upload(mesh);
+        d.material(prob.E, prob.nu);
+        std::vector<double> R(mesh.n_dofs);
+        // R = 0 * f_ext - f_int with no constraints set: f_int = -R
+        d.check(fm_set_bcs(d.ctx(), 0, nullptr, nullptr), "fm_set_bcs");
+        d.check(fm_residual(d.ctx(), u, 0.0, R.data(), nullptr), "fm_residual");
+        f_int.assign(mesh.n_dofs, 0.0);
+        for (size_t i = 0; i < R.size(); ++i) f_int[i] = -R[i];
+    } catch (...) {
+        f_int.assign(mesh.n_dofs, std::nan(""));
+    }
+}
+
+// pcg_solve_ilu0 (FEM/adaptive_fem_engine.hpp:1283-1341): factorizes A, runs
+// PCG from x with the stopping rule sqrt(r.z / b.b) < tol; returns the
+// iteration count (max_iter = not converged, 0 = trivially converged), -1 on a
+// device error (the reference is noexcept).
+template <class CSR>
+[[nodiscard]] int pcg_solve_ilu0(const CSR& A, const double* b, double* x, uint32_t N, double tol = 1e-10,
+                                 int max_iter = 1000) noexcept {
+    try {
+        Device& d = Device::get();
+        if (N != A.n_rows) return -1;
+        d.check(fm_set_matrix(d.ctx(), A.n_rows, A.row_ptr.data(), A.col_idx.data(), A.val.data()), "fm_set_matrix");
+        int iters = 0;
+        double rel = 0.0;
+        d.check(fm_pcg(d.ctx(), FM_FP64, b, x, tol, max_iter, &iters, &rel), "fm_pcg");
+        return iters;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// MeshColoring (FEM/adaptive_fem_engine.hpp:1549-1602)
+struct MeshColoring {
+    std::vector<uint32_t> element_colors;
+    uint32_t n_colors{1};
+
+    template <class Mesh>
+    static MeshColoring compute(const Mesh& mesh) {
+        Device& d = Device::get();
+        d.upload(mesh);
+        MeshColoring c;
+        c.element_colors.assign(mesh.elements.size(), 0);
+        d.check(fm_color(d.ctx(), c.element_colors.data(), &c.n_colors), "fm_color");
+        return c;
+    }
+};
+
+// NewtonSolverConfig / NewtonSolverResult (FEM/nonlinear_solver.hpp:23-64)
+struct NewtonSolverConfig {
+    int max_iterations = 50;
+    double tol_absolute = 1e-8;  // not used: the device loop stops on the relative test
+    double tol_relative = 1e-6;
+    int max_line_search = 10;
+    double line_search_alpha = 1.0;
+    double line_search_beta = 0.5;
+    bool reuse_tangent = false;
+    bool verbose = true;
+    double energy_convergence_tol = 1e-2;
+    // device extensions
+    int precision = FM_FP64;
+    double pcg_tol = 1e-10;
+    int pcg_max = 300;
+};
+
+struct NewtonSolverResult {
+    bool converged = false;
+    int num_iterations = 0;
+    double final_residual_norm = 0;
+    double final_residual_relative = 0;
+    double final_energy_change = 0;
+    std::vector<double> residual_history;
+    std::vector<double> step_size_history;
+    std::vector<double> energy_history;
+    int pcg_iterations = 0;
+};
+
+// NonlinearNewtonSolver (FEM/nonlinear_solver.hpp:68-238): constructed from
+// the mesh, a material with params.E/params.nu and BoundaryConditions
+// (dirichlet_bcs (node, dof, value), neumann_bcs (node, fx, fy, fz)); solve()
+// brings u to equilibrium at load.load_factor, with the Dirichlet values and
+// the Neumann forces scaled by it.
+template <class Mesh, class Material, class BCs>
+class NonlinearNewtonSolver {
+public:
+    NonlinearNewtonSolver(const Mesh& mesh, const Material& mat, const BCs& bcs)
+        : mesh_(mesh), material_(mat), bcs_(bcs) {}
+
+    template <class Solution, class LoadStep>
+    NewtonSolverResult solve(Solution& u, const LoadStep& load,
+                             const NewtonSolverConfig& config = NewtonSolverConfig()) const {
+        Device& d = Device::get();
+        d.upload(mesh_);
+        d.material(material_.params.E, material_.params.nu);
+        std::vector<uint32_t> dofs;
+        std::vector<double> vals;
+        for (const auto& bc : bcs_.dirichlet_bcs) {
+            dofs.push_back(3u * (uint32_t)std::get<0>(bc) + (uint32_t)std::get<1>(bc));
+            vals.push_back(std::get<2>(bc));
+        }
+        d.check(fm_set_bcs(d.ctx(), (uint32_t)dofs.size(), dofs.data(), vals.data()), "fm_set_bcs");
+        std::vector<double> f(mesh_.n_dofs, 0.0);
+        for (const auto& nb : bcs_.neumann_bcs) {
+            const int n = std::get<0>(nb);
+            f[3 * n] += std::get<1>(nb);
+            f[3 * n + 1] += std::get<2>(nb);
+            f[3 * n + 2] += std::get<3>(nb);
+        }
+        d.check(fm_set_load(d.ctx(), bcs_.neumann_bcs.empty() ? nullptr : f.data()), "fm_set_load");
+        fm_newton_cfg cfg;
+        fm_newton_cfg_default(&cfg);
+        cfg.prec = config.precision;
+        cfg.newton_rtol = config.tol_relative;
+        cfg.max_newton = config.max_iterations;
+        cfg.ls_max = config.max_line_search;
+        cfg.pcg_tol = config.pcg_tol;
+        cfg.pcg_max = config.pcg_max;
+        fm_newton_stats st{};
+        const int rc = fm_newton_solve_to(d.ctx(), load.load_factor, u.data(), &cfg, &st);
+        if (rc != FM_OK && rc != FM_ENOCONV) d.check(rc, "fm_newton_solve_to");
+        NewtonSolverResult r;
+        r.converged = st.converged != 0;
+        r.num_iterations = st.newton_iters;
+        r.final_residual_relative = st.rel_res;
+        r.final_residual_norm = st.rel_res * st.r0;
+        r.residual_history = {st.r0, r.final_residual_norm};
+        r.pcg_iterations = st.pcg_iters;
+        return r;
+    }
+
+private:
+    const Mesh& mesh_;
+    const Material& material_;
+    const BCs& bcs_;
+};
+
+}  // namespace atlas::fem::gpu
