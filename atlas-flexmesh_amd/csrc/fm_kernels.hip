@@ -593,7 +593,7 @@ __device__ __forceinline__ void factor_node(uint32_t node, const uint32_t* __res
     }
 }
 
-// persistent and cooperative, like the sweeps: wave w takes queue entries w, w+P, ...
+// persistent, like the sweeps: wave w takes queue entries w, w+P, ...
 template <typename T, int B, int S>
 __global__ __launch_bounds__(64) void ilu_factor_kernel(uint32_t nq, const uint32_t* __restrict__ order,
                                                         const uint32_t* __restrict__ rp,
@@ -606,8 +606,8 @@ __global__ __launch_bounds__(64) void ilu_factor_kernel(uint32_t nq, const uint3
     }
 }
 
-// Triangular sweeps.  Both are persistent and launched cooperatively (all
-// waves co-resident): wave w works queue chunks w, w+P, w+2P, ... in order.
+// Triangular sweeps.  Both are persistent with every wave resident from the
+// start (launch_persistent): wave w works queue chunks w, w+P, w+2P, ... in order.
 // The lowest unfinished chunk therefore always belongs to a running wave and
 // depends only on lower, finished chunks, so the sweep always progresses; and
 // the P waves in flight cover a window of consecutive chunks at the dependence
@@ -1179,10 +1179,15 @@ uint32_t env_waves(const char* name, uint32_t dflt) {
 }
 hipError_t g_launch_err = hipSuccess;
 
-// cooperative launch of a persistent kernel with at most `want` one-wave
-// blocks, clamped to what the device can keep resident at once
+// Launch of a persistent kernel: at most `want` one-wave blocks, clamped to
+// half of what the device keeps resident at once, so every block is running
+// from the start (the dependence waits rely on it).  A plain launch is used,
+// not hipLaunchCooperativeKernel: cooperative launches hold a device-wide
+// resource and fail when another process on the same GPU uses them.  Should
+// a block nevertheless not be resident, the bounded waits end the kernel with
+// FM_EDEVICE instead of hanging.
 template <typename... Args>
-uint32_t launch_coop(void (*kern)(Args...), uint32_t want, hipStream_t s, Args... args) {
+uint32_t launch_persistent(void (*kern)(Args...), uint32_t want, hipStream_t s, Args... args) {
     static thread_local int dev = -1, cus = 0;
     int d = 0;
     (void)hipGetDevice(&d);
@@ -1192,11 +1197,10 @@ uint32_t launch_coop(void (*kern)(Args...), uint32_t want, hipStream_t s, Args..
     }
     int occ = 0;
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(kern), 64, 0);
-    const uint32_t cap = (uint32_t)std::max(1, occ * cus);
+    const uint32_t cap = (uint32_t)std::max(1, occ * cus / 2);
     const uint32_t grid = std::max(1u, std::min(want, cap));
-    void* argv[] = {reinterpret_cast<void*>(&args)...};
-    const hipError_t e =
-        hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3(grid), dim3(64), argv, 0, s);
+    kern<<<grid, 64, 0, s>>>(args...);
+    const hipError_t e = hipGetLastError();
     if (e != hipSuccess) g_launch_err = e;
     return grid;
 }
@@ -1361,17 +1365,17 @@ static void factor_t(hipStream_t s, const Matrix& A, const T* val, T* lu, PcgSta
     const uint32_t q = S.q_fwd;
     switch (S.slots) {
         case 1:
-            launch_coop(ilu_factor_kernel<T, B, 1>, nb, s, q, (const uint32_t*)S.fwd_order.p,
+            launch_persistent(ilu_factor_kernel<T, B, 1>, nb, s, q, (const uint32_t*)S.fwd_order.p,
                         (const uint32_t*)A.row_ptr.p, (const uint32_t*)A.col_idx.p, (const uint16_t*)S.nlow.p, val,
                         lu, &st->ticket[2], err);
             break;
         case 2:
-            launch_coop(ilu_factor_kernel<T, B, 2>, nb, s, q, (const uint32_t*)S.fwd_order.p,
+            launch_persistent(ilu_factor_kernel<T, B, 2>, nb, s, q, (const uint32_t*)S.fwd_order.p,
                         (const uint32_t*)A.row_ptr.p, (const uint32_t*)A.col_idx.p, (const uint16_t*)S.nlow.p, val,
                         lu, &st->ticket[2], err);
             break;
         default:
-            launch_coop(ilu_factor_kernel<T, B, 4>, nb, s, q, (const uint32_t*)S.fwd_order.p,
+            launch_persistent(ilu_factor_kernel<T, B, 4>, nb, s, q, (const uint32_t*)S.fwd_order.p,
                         (const uint32_t*)A.row_ptr.p, (const uint32_t*)A.col_idx.p, (const uint16_t*)S.nlow.p, val,
                         lu, &st->ticket[2], err);
             break;
@@ -1399,11 +1403,11 @@ static void fwd_t(hipStream_t s, const Matrix& A, const T* lu, const T* rhs, T* 
     const uint32_t nb = std::min(cdiv(S.q_fwd, 64 / LPN), solve_waves());
     const uint32_t q = S.q_fwd;
     if (upd)
-        launch_coop(ilu_fwd_kernel<T, B, LPN, true>, nb, s, q, (const uint32_t*)S.fwd_order.p,
+        launch_persistent(ilu_fwd_kernel<T, B, LPN, true>, nb, s, q, (const uint32_t*)S.fwd_order.p,
                     (const uint32_t*)A.row_ptr.p, (const uint32_t*)A.col_idx.p, (const uint16_t*)S.nlow.p, lu, rhs, r,
                     x, p, Ap, z, st, err);
     else
-        launch_coop(ilu_fwd_kernel<T, B, LPN, false>, nb, s, q, (const uint32_t*)S.fwd_order.p,
+        launch_persistent(ilu_fwd_kernel<T, B, LPN, false>, nb, s, q, (const uint32_t*)S.fwd_order.p,
                     (const uint32_t*)A.row_ptr.p, (const uint32_t*)A.col_idx.p, (const uint16_t*)S.nlow.p, lu, rhs, r,
                     x, p, Ap, z, st, err);
 }
@@ -1442,7 +1446,7 @@ static uint32_t bwd_t(hipStream_t s, const Matrix& A, const T* lu, const T* z, c
     const Schedule& S = A.sched;
     const uint32_t ntk = cdiv(S.q_bwd, 64 / LPN);
     const uint32_t nb = std::min(ntk, solve_waves());
-    return launch_coop(ilu_bwd_kernel<T, B, LPN>, nb, s, S.q_bwd, (const uint32_t*)S.bwd_order.p,
+    return launch_persistent(ilu_bwd_kernel<T, B, LPN>, nb, s, S.q_bwd, (const uint32_t*)S.bwd_order.p,
                        (const uint32_t*)A.row_ptr.p, (const uint32_t*)A.col_idx.p, (const uint16_t*)S.nlow.p, lu, z, r,
                        y, partials, st, check_done ? 1 : 0, err);  // one partial per wave
 }

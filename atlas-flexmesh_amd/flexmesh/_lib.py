@@ -80,6 +80,7 @@ def lib():
     L.fm_newton_cfg_default.argtypes = [C.POINTER(NewtonCfg)]
     L.fm_newton_cfg_default.restype = None
     L.fm_newton_step.argtypes = [_vp, C.c_uint32, _vp, C.POINTER(NewtonCfg), C.POINTER(NewtonStats)]
+    L.fm_newton_solve_to.argtypes = [_vp, C.c_double, _vp, C.POINTER(NewtonCfg), C.POINTER(NewtonStats)]
     L.fm_get_u.argtypes = [_vp, _f64p]
     L.fm_set_u.argtypes = [_vp, _f64p]
     L.fm_adjoint.argtypes = [_vp, _f64p, C.c_double, C.c_int, _vp, _vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
@@ -266,6 +267,13 @@ class Solver:
         rc = self.L.fm_newton_step(self.h, step, _ptr(uu), C.byref(cfg), C.byref(st))
         if check:
             self._chk(rc, "fm_newton_step")
+        return rc, st.as_dict()
+
+    def newton_solve_to(self, load_factor, cfg, u=None, check=True):
+        st = NewtonStats()
+        rc = self.L.fm_newton_solve_to(self.h, load_factor, _ptr(u), C.byref(cfg), C.byref(st))
+        if check:
+            self._chk(rc, "fm_newton_solve_to")
         return rc, st.as_dict()
 
     def get_u(self):

@@ -608,22 +608,56 @@ double residual(const orc_problem* P, double lf, const double* u, const uint8_t*
 
 extern "C" {
 
+// Potential energy: sum over elements of V_e psi(F_e) (orc_energy) minus the
+// work lf f_ext.u; its gradient is -R, so a Newton direction with R.du > 0 is
+// a descent direction.  Elements in index order (serial) for a fixed sum.
+double orc_total_energy(const orc_problem* P, double lf, const double* u) {
+    const uint32_t nn = P->n_nodes, ne = P->n_elems;
+    double e_int = 0.0;
+    for (uint32_t e = 0; e < ne; ++e) {
+        double Xe[12], ue[12];
+        uint32_t nodes[4];
+        gather_elem(P->X, nn, P->conn, ne, e, u, Xe, ue, nodes);
+        ElemGeo G;
+        if (!elem_geometry(Xe, G)) continue;
+        ElemKin K;
+        elem_kinematics(G, ue, K);
+        e_int += G.vol * orc_energy(K.F, P->mu[e], P->lam[e]);
+    }
+    double w = 0.0;
+    if (P->f_ext)
+        for (size_t i = 0; i < 3ull * nn; ++i) w += P->f_ext[i] * u[i];
+    return e_int - lf * w;
+}
+
 // armijo_line_search_true (FEM/adaptive_fem_engine.hpp:933-985): alpha = 1,
 // 1/2, ... for at most ls_max trials; accept the first alpha with
 // ||R(u + alpha du)|| <= (1 - c alpha) ||R(u)|| + 1e-14; otherwise return the
-// last halved alpha.  u_out receives u + alpha du for the returned alpha.
+// last halved alpha.  With ls_mode = 1 a trial is also accepted when the
+// energy satisfies Pi(u + alpha du) <= Pi(u) - c alpha R.du (R.du > 0): the
+// residual norm may grow on the way into the quadratic basin (slender bodies
+// under traction) while the energy decreases.  u_out receives u + alpha du.
 double orc_armijo(const orc_problem* P, double lf, const double* u, const double* du, double R_norm, double* u_out,
                   double* R_norm_out) {
     const size_t n = 3ull * P->n_nodes;
     std::vector<uint8_t> is_dir(n, 0);
     for (uint32_t k = 0; k < P->n_dir; ++k) is_dir[P->dir_dofs[k]] = 1;
     std::vector<double> f(n), R(n);
+    double pi0 = 0.0, slope = 0.0;
+    if (P->ls_mode == 1) {
+        pi0 = orc_total_energy(P, lf, u);
+        residual(P, lf, u, is_dir.data(), f, R);
+        for (size_t i = 0; i < n; ++i) slope += R[i] * du[i];
+    }
     double alpha = 1.0;
     for (int bt = 0; bt < P->ls_max; ++bt) {
         for (size_t i = 0; i < n; ++i) u_out[i] = u[i] + alpha * du[i];
         const double Rt = residual(P, lf, u_out, is_dir.data(), f, R);
         if (R_norm_out) *R_norm_out = Rt;
         if (Rt <= (1.0 - P->armijo_c * alpha) * R_norm + 1e-14) return alpha;
+        if (P->ls_mode == 1 && slope > 0.0 &&
+            orc_total_energy(P, lf, u_out) <= pi0 - P->armijo_c * alpha * slope)
+            return alpha;
         alpha *= 0.5;
     }
     for (size_t i = 0; i < n; ++i) u_out[i] = u[i] + alpha * du[i];

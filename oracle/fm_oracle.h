@@ -93,6 +93,7 @@ typedef struct {
     uint32_t n_colors;
     const uint32_t* row_ptr;   /* 3nn+1 */
     const uint32_t* col_idx;   /* nnz */
+    int ls_mode;               /* 0: Armijo on ||R|| (reference); 1: also accept Armijo on the energy */
 } orc_problem;
 
 typedef struct {
@@ -107,6 +108,8 @@ typedef struct {
     int factorizations;
 } orc_step_stats;
 
+/* potential energy sum_e V_e psi(F_e) - lf f_ext.u */
+double orc_total_energy(const orc_problem* P, double lf, const double* u);
 double orc_armijo(const orc_problem* P, double lf, const double* u, const double* du, double R_norm,
                   double* u_out, double* R_norm_out);
 int orc_newton_step(const orc_problem* P, uint32_t step, double* u, orc_step_stats* st);

@@ -28,7 +28,7 @@ class Problem(C.Structure):
         ("pcg_tol", C.c_double), ("pcg_max", C.c_int),
         ("ls_max", C.c_int), ("armijo_c", C.c_double), ("lift", C.c_int),
         ("colors", C.c_void_p), ("n_colors", C.c_uint32),
-        ("row_ptr", C.c_void_p), ("col_idx", C.c_void_p),
+        ("row_ptr", C.c_void_p), ("col_idx", C.c_void_p), ("ls_mode", C.c_int),
     ]
 
 
@@ -87,6 +87,8 @@ def lib():
         L.orc_adjoint.argtypes = [C.POINTER(Problem), _f64p, _f64p, C.c_double, C.c_int, _f64p, _f64p,
                                   C.POINTER(C.c_double), C.c_void_p]
         L.orc_adjoint.restype = C.c_int
+        L.orc_total_energy.argtypes = [C.POINTER(Problem), C.c_double, _f64p]
+        L.orc_total_energy.restype = C.c_double
         _lib = L
     return _lib
 
@@ -210,7 +212,7 @@ class NewtonProblem:
 
     def __init__(self, X, conn, mu, lam, dir_dofs, dir_vals, f_ext, n_steps, newton_rtol=1e-8,
                  max_newton=30, pcg_tol=1e-10, pcg_max=300, ls_max=10, armijo_c=1e-4, lift=1,
-                 colors=None, n_colors=None, row_ptr=None, col_idx=None):
+                 colors=None, n_colors=None, row_ptr=None, col_idx=None, ls_mode=0):
         self.X = np.ascontiguousarray(X, dtype=np.float64)
         self.conn = np.ascontiguousarray(conn, dtype=np.uint32)
         self.mu = np.ascontiguousarray(mu, dtype=np.float64)
@@ -248,12 +250,16 @@ class NewtonProblem:
         p.n_colors = n_colors
         p.row_ptr = self.row_ptr.ctypes.data
         p.col_idx = self.col_idx.ctypes.data
+        p.ls_mode = ls_mode
         self.p = p
 
     def step(self, s, u):
         st = StepStats()
         rc = lib().orc_newton_step(C.byref(self.p), s, u, C.byref(st))
         return rc, st.as_dict()
+
+    def energy(self, lf, u):
+        return lib().orc_total_energy(C.byref(self.p), lf, np.ascontiguousarray(u, dtype=np.float64))
 
     def adjoint(self, u, dLdu, tol=1e-12, max_iter=5000):
         ne = self.conn.size // 4

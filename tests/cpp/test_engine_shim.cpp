@@ -38,7 +38,19 @@ static double rel(const std::vector<double>& a, const std::vector<double>& b) {
     return d / (m > 0 ? m : 1.0);
 }
 
+int run();
+
 int main() {
+    std::setvbuf(stdout, nullptr, _IONBF, 0);
+    try {
+        return run();
+    } catch (const std::exception& e) {
+        std::printf("exception: %s\n", e.what());
+        return 2;
+    }
+}
+
+int run() {
     namespace G = atlas::fem::gpu;
     const int N = 4;
     const uint32_t nn = (N + 1) * (N + 1) * (N + 1), ne = 6 * N * N * N;

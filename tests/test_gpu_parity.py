@@ -309,3 +309,28 @@ def test_checkpoint_roundtrip(fm, S, tmp_path):
     open(p, "wb").write(bytes(data))
     with pytest.raises(fm.FlexMeshError):
         S.checkpoint_load(p)
+
+
+def test_one_context_across_meshes_and_matrices(fm, O):
+    """One context: mesh A (pattern, colours, assembly), an external CSR
+    matrix through PCG, then mesh B through Newton -- the sequence a host
+    engine drives through the C++ shim."""
+    S = fm.Solver(0)
+    try:
+        a = fm.configs.make(2, dims=(3, 3, 3))
+        fm.configs.load(S, a)
+        S.pattern()
+        S.colors()
+        S.assemble(np.zeros(a["X"].size))
+        rp, ci, va, b, xs = poisson(16)
+        S.set_matrix(rp, ci, va)
+        x, its, _ = S.pcg(b, tol=1e-10, max_it=4000)
+        assert its == int(golden("pcg.npz")["poisson16_refcols_iters"][0])
+        c = fm.configs.make(1, dims=(3, 3, 3))
+        fm.configs.load(S, c)
+        rc, st = S.newton_step(1, fm.configs.newton_cfg(c))
+        assert st["converged"]
+        uo, _ = run_oracle(O, c, [1])
+        assert rel(S.get_u(), uo) < 1e-9
+    finally:
+        S.close()

@@ -1,0 +1,279 @@
+"""Validation runs of the benchmark configurations against the FP64 CPU oracle
+and against finite differences.  Test infrastructure (imports the oracle);
+prints one JSON object per run.
+
+  python tests/validation/run_configs.py cfg1
+  python tests/validation/run_configs.py cfg2 --steps 2
+  python tests/validation/run_configs.py cfg4 --dims 48 --fd-dirs 8
+  python tests/validation/run_configs.py cfg4 --dims 12 --fd-dirs 64 --oracle
+  python tests/validation/run_configs.py cfg5 --steps 40
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path[:0] = [os.path.join(ROOT, "atlas-flexmesh_amd"), os.path.join(ROOT, "oracle")]
+import flexmesh as fm  # noqa: E402
+import oracle as O  # noqa: E402
+
+
+def log(*a):
+    print("[validate]", *a, file=sys.stderr, flush=True)
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def oracle_problem(cfg, **kw):
+    return O.NewtonProblem(cfg["X"], cfg["conn"], cfg["mu"], cfg["lam"], cfg["dir_dofs"], cfg["dir_vals"],
+                           cfg["f_ext"], cfg["n_steps"], **kw)
+
+
+def oracle_residual(cfg, u, lf):
+    nn = cfg["X"].size // 3
+    rp, ci = O.pattern(nn, cfg["conn"])
+    colors, nc = O.color(nn, cfg["conn"])
+    _, f = O.assemble(cfg["X"], cfg["conn"], cfg["mu"], cfg["lam"], colors, nc, u, rp, ci, want_K=False)
+    R = (0.0 if cfg["f_ext"] is None else lf * cfg["f_ext"]) - f
+    R[cfg["dir_dofs"]] = 0.0
+    return R
+
+
+def run_cfg1(args):
+    cfg = fm.configs.make(1)
+    P = oracle_problem(cfg, newton_rtol=cfg["newton_rtol"], pcg_tol=cfg["pcg_tol"])
+    uo = np.zeros(cfg["X"].size)
+    t = time.perf_counter()
+    for s in range(1, cfg["n_steps"] + 1):
+        P.step(s, uo)
+    t_cpu = time.perf_counter() - t
+    out = {"run": "cfg1", "dofs": cfg["X"].size, "oracle_seconds_1thread": t_cpu}
+    for name, prec in (("fp64", fm.FM_FP64), ("mixed", fm.FM_MIXED)):
+        S = fm.Solver(0)
+        fm.configs.load(S, cfg)
+        c = fm.configs.newton_cfg(cfg, prec=prec)
+        its = []
+        for s in range(1, cfg["n_steps"] + 1):
+            _, st = S.newton_step(s, c)
+            its.append((st["newton_iters"], st["pcg_iters"], st["rel_res"]))
+        u = S.get_u()
+        R = oracle_residual(cfg, u, 1.0)
+        f0 = oracle_residual(cfg, np.zeros_like(u), 1.0)
+        out[name] = {"u_rel_err_vs_oracle": rel(u, uo), "fp64_residual_of_gpu_u": float(np.linalg.norm(R)),
+                     "newton_pcg_relres_per_step": its}
+        S.close()
+    print(json.dumps(out), flush=True)
+
+
+def run_cfg2(args):
+    cfg = fm.configs.make(2, dims=(args.dims,) * 3 if args.dims else None)
+    steps = range(1, args.steps + 1)
+    S = fm.Solver(0)
+    fm.configs.load(S, cfg)
+    c = fm.configs.newton_cfg(cfg)
+    t = time.perf_counter()
+    gst = [S.newton_step(s, c)[1] for s in steps]
+    t_gpu = time.perf_counter() - t
+    u = S.get_u()
+    S.close()
+    O.set_threads(min(16, os.cpu_count() or 1))
+    P = oracle_problem(cfg, newton_rtol=cfg["newton_rtol"], pcg_tol=cfg["pcg_tol"])
+    uo = np.zeros_like(u)
+    t = time.perf_counter()
+    ost = []
+    for s in steps:
+        rc, st = P.step(s, uo)
+        ost.append(st)
+        log(f"oracle step {s}: newton {st['newton_iters']} pcg {st['pcg_iters']} {st['t_total']:.1f} s")
+    t_cpu = time.perf_counter() - t
+    lf = args.steps / cfg["n_steps"]
+    out = {"run": "cfg2", "dims": cfg["dims"], "dofs": u.size, "steps": args.steps,
+           "u_rel_err_fp64_gpu_vs_oracle": rel(u, uo),
+           "residual_rel_gpu_u": float(np.linalg.norm(oracle_residual(cfg, u, lf)) / max(ost[-1]["r0"], 1e-300)),
+           "gpu_newton_pcg": [(s["newton_iters"], s["pcg_iters"]) for s in gst],
+           "oracle_newton_pcg": [(s["newton_iters"], s["pcg_iters"]) for s in ost],
+           "gpu_seconds": t_gpu, "oracle_seconds": t_cpu, "oracle_threads": O.get_threads() if hasattr(O, "get_threads") else None}
+    print(json.dumps(out), flush=True)
+
+
+def loss_and_grad(u, ustar, nodes, nn):
+    d = (u - ustar).reshape(nn, 3)[nodes]
+    g = np.zeros(3 * nn)
+    g.reshape(nn, 3)[nodes] = d
+    return 0.5 * float(np.sum(d * d)), g
+
+
+def run_cfg4(args):
+    dims = (args.dims,) * 3
+    cfg = fm.configs.make(4, dims=dims)
+    nn = cfg["X"].size // 3
+    ne = cfg["conn"].size // 4
+    top = cfg["loss_nodes"]
+    c = fm.configs.newton_cfg(cfg, pcg_max=args.pcg_max, max_newton=40)
+    S = fm.Solver(0)
+    fm.configs.load(S, cfg)
+    # target displacement from the seed-B fields
+    S.set_material(cfg["mu_target"], cfg["lam_target"])
+    t = time.perf_counter()
+    for s in range(1, cfg["n_steps"] + 1):
+        S.newton_step(s, c)
+    ustar = S.get_u()
+    # forward solve with the seed-A fields, checkpoint of step n-1
+    S.set_material(cfg["mu"], cfg["lam"])
+    S.set_u(np.zeros(3 * nn))
+    ck = os.path.join(args.out, f"cfg4_{args.dims}_step{cfg['n_steps'] - 1}.ckpt")
+    fstats = []
+    for s in range(1, cfg["n_steps"] + 1):
+        _, st = S.newton_step(s, c)
+        fstats.append((st["newton_iters"], st["pcg_iters"]))
+        if s == cfg["n_steps"] - 1:
+            S.checkpoint_save(ck, s)
+    t_fwd = time.perf_counter() - t
+    u = S.get_u()
+    L0, g = loss_and_grad(u, ustar, top, nn)
+    t = time.perf_counter()
+    dmu, dlam, dload, ad_its = S.adjoint(g, tol=1e-13, max_it=20000)
+    t_adj = time.perf_counter() - t
+
+    def loss_at(mu, lam, scale=1.0):
+        S.set_material(mu, lam)
+        if scale != 1.0:
+            S.set_bcs(cfg["dir_dofs"], scale * cfg["dir_vals"])
+        S.checkpoint_load(ck)
+        S.newton_step(cfg["n_steps"], c)
+        if scale != 1.0:
+            S.set_bcs(cfg["dir_dofs"], cfg["dir_vals"])
+        return loss_and_grad(S.get_u(), ustar, top, nn)[0]
+
+    rng = np.random.default_rng(args.seed)
+    h = 1e-6
+    fd = []
+    t = time.perf_counter()
+    for k in range(args.fd_dirs):
+        v, w = rng.standard_normal(ne), rng.standard_normal(ne)
+        # relative perturbation of each parameter keeps the step size meaningful
+        vm, vl = v * cfg["mu"], w * cfg["lam"]
+        Lp = loss_at(cfg["mu"] + h * vm, cfg["lam"] + h * vl)
+        Lm = loss_at(cfg["mu"] - h * vm, cfg["lam"] - h * vl)
+        d_fd = (Lp - Lm) / (2 * h)
+        d_ad = float(dmu @ vm + dlam @ vl)
+        fd.append((d_fd, d_ad, abs(d_fd - d_ad) / max(abs(d_ad), 1e-300)))
+        log(f"direction {k}: fd {d_fd:.10e} adjoint {d_ad:.10e} rel {fd[-1][2]:.2e}")
+    Lp = loss_at(cfg["mu"], cfg["lam"], 1 + h)
+    Lm = loss_at(cfg["mu"], cfg["lam"], 1 - h)
+    fd_load = (Lp - Lm) / (2 * h)
+    t_fd = time.perf_counter() - t
+    out = {"run": "cfg4", "dims": dims, "dofs": 3 * nn, "params": 2 * ne, "loss": L0, "forward_newton_pcg": fstats,
+           "forward_seconds": t_fwd, "adjoint_seconds": t_adj, "adjoint_pcg_iters": ad_its,
+           "fd_dirs": args.fd_dirs, "fd_max_rel_err": max(x[2] for x in fd) if fd else None,
+           "fd_median_rel_err": float(np.median([x[2] for x in fd])) if fd else None,
+           "dload_adjoint": dload, "dload_fd": fd_load, "dload_rel_err": abs(fd_load - dload) / max(abs(dload), 1e-300),
+           "fd_seconds": t_fd}
+    if args.oracle:
+        O.set_threads(min(16, os.cpu_count() or 1))
+        P = oracle_problem(cfg, newton_rtol=1e-13, pcg_tol=1e-13, pcg_max=20000, max_newton=40)
+        uo = np.zeros(3 * nn)
+        for s in range(1, cfg["n_steps"] + 1):
+            P.step(s, uo)
+        Lo, go = loss_and_grad(uo, ustar, top, nn)
+        dmo, dlo, dldo, _, _ = P.adjoint(uo, go, tol=1e-13, max_iter=20000)
+        out.update({"u_rel_err_vs_oracle": rel(u, uo), "dmu_rel_err_vs_oracle": rel(dmu, dmo),
+                    "dlam_rel_err_vs_oracle": rel(dlam, dlo), "dload_rel_err_vs_oracle": abs(dload - dldo) / abs(dldo)})
+    S.close()
+    print(json.dumps(out), flush=True)
+
+
+def run_cfg5(args):
+    cfg = fm.configs.make(5, dims=tuple(int(v) for v in args.beam.split(",")) if args.beam else None)
+    nn = cfg["X"].size // 3
+    ne = cfg["conn"].size // 4
+    nx = cfg["dims"][0]
+    gi, _, _ = fm.configs.grid_index(*cfg["dims"])
+    tip = np.where(gi == nx)[0]
+    S = fm.Solver(0)
+    fm.configs.load(S, cfg)
+    prec = fm.FM_MIXED if args.prec == "mixed" else fm.FM_FP64
+    c = fm.configs.newton_cfg(cfg, prec=prec, max_newton=60)
+    n = min(args.steps, cfg["n_steps"])
+    ck = os.path.join(args.out, "cfg5_prev.ckpt")
+    t = time.perf_counter()
+    hist = []
+    for s in range(1, n + 1):
+        if s == n:
+            S.checkpoint_save(ck, s - 1)
+        _, st = S.newton_step(s, c)
+        hist.append((st["newton_iters"], st["pcg_iters"], st["rel_res"]))
+        log(f"step {s}: newton {st['newton_iters']} pcg {st['pcg_iters']} rel {st['rel_res']:.1e} {st['seconds']:.1f}s")
+    wall = time.perf_counter() - t
+    u = S.get_u()
+    w = S.element_rotations()
+    conn = cfg["conn"].reshape(4, ne).T
+    tip_elems = np.where(np.isin(conn, tip).any(axis=1))[0]
+    ang = np.degrees(np.linalg.norm(w[tip_elems], axis=1))
+    defl = float(u.reshape(nn, 3)[tip, 2].mean())
+    out = {"run": "cfg5", "dims": cfg["dims"], "dofs": 3 * nn, "steps": n, "load_factor": n / cfg["n_steps"],
+           "traction": cfg["traction"], "prec": args.prec, "seconds": wall, "newton_pcg_relres": hist,
+           "tip_rotation_deg_mean": float(ang.mean()), "tip_rotation_deg_max": float(ang.max()),
+           "tip_deflection_z": defl}
+    if args.adjoint:
+        # gradient of the mean tip deflection w.r.t. the traction magnitude and mu_e
+        g = np.zeros(3 * nn)
+        g.reshape(nn, 3)[tip, 2] = 1.0 / tip.size
+        dmu, dlam, dload, its = S.adjoint(g, tol=1e-12, max_it=20000)
+        cfd = fm.configs.newton_cfg(cfg, prec=fm.FM_FP64, newton_rtol=1e-12, pcg_tol=1e-13, pcg_max=20000,
+                                    max_newton=60)
+        lf = n / cfg["n_steps"]
+
+        def defl_at(scale=1.0, mu=None):
+            S.set_load(scale * cfg["f_ext"])
+            if mu is not None:
+                S.set_material(mu, cfg["lam"])
+            S.checkpoint_load(ck)
+            S.newton_solve_to(lf, cfd)
+            S.set_load(cfg["f_ext"])
+            if mu is not None:
+                S.set_material(cfg["mu"], cfg["lam"])
+            return float(S.get_u().reshape(nn, 3)[tip, 2].mean())
+
+        base = defl_at()
+        h = 1e-6
+        # the adjoint's load derivative is d/d(load factor); the traction scales the same way
+        fd_t = (defl_at(1 + h) - defl_at(1 - h)) / (2 * h)
+        rng = np.random.default_rng(3)
+        v = rng.standard_normal(ne) * cfg["mu"]
+        fd_mu = (defl_at(mu=cfg["mu"] + h * v) - defl_at(mu=cfg["mu"] - h * v)) / (2 * h)
+        out.update({"adjoint_pcg_iters": its, "tip_deflection_fp64_tight": base,
+                    "d_defl_d_traction_scale_adjoint": dload * lf, "d_defl_d_traction_scale_fd": fd_t,
+                    "d_defl_d_mu_dir_adjoint": float(dmu @ v), "d_defl_d_mu_dir_fd": fd_mu})
+    S.close()
+    print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("run", choices=["cfg1", "cfg2", "cfg4", "cfg5"])
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--dims", type=int, default=None)
+    ap.add_argument("--beam", type=str, default=None)
+    ap.add_argument("--fd-dirs", type=int, default=8)
+    ap.add_argument("--pcg-max", type=int, default=5000)
+    ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--oracle", action="store_true")
+    ap.add_argument("--adjoint", action="store_true")
+    ap.add_argument("--prec", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--out", default=os.path.join(ROOT, "gpurun_out"))
+    args = ap.parse_args()
+    os.makedirs(args.out, exist_ok=True)
+    if args.run == "cfg4" and args.dims is None:
+        args.dims = 48
+    {"cfg1": run_cfg1, "cfg2": run_cfg2, "cfg4": run_cfg4, "cfg5": run_cfg5}[args.run](args)
+
+
+if __name__ == "__main__":
+    main()
