@@ -318,6 +318,30 @@ int residual(fm_ctx* ctx, const double* u, double load, double* norm) {
     return FM_OK;
 }
 
+// Potential energy sum_e V_e psi(F_e) - load f_ext.u (FP64, fixed-order
+// block reductions) and, when du != nullptr, the slope R.du with the R of the
+// last residual evaluation.
+int energy(fm_ctx* ctx, const double* u, double load, const double* du, double* pi, double* slope) {
+    hipStream_t s = ctx->stream;
+    const uint32_t n = 3 * ctx->nn;
+    FM_TRY(ensure_partials(ctx, std::max(n, ctx->ne) / 256 + 1));
+    uint32_t np = launch_energy_elem(s, elem_args(ctx, u), ctx->partials.p);
+    launch_reduce(s, ctx->partials.p, np, ctx->st.p, 0, 0);
+    if (ctx->have_fext) {
+        np = launch_dot64(s, n, ctx->f_ext.p, u, ctx->partials.p);
+        launch_reduce(s, ctx->partials.p, np, ctx->st.p, 0, 1);
+    }
+    if (du) {
+        np = launch_dot64(s, n, ctx->R.p, du, ctx->partials.p);
+        launch_reduce(s, ctx->partials.p, np, ctx->st.p, 0, 2);
+    }
+    FM_HIP(hipMemcpyAsync(ctx->h_st, ctx->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    FM_TRY(sync(ctx));
+    *pi = ctx->h_st->slot[0] - (ctx->have_fext ? load * ctx->h_st->slot[1] : 0.0);
+    if (slope) *slope = du ? ctx->h_st->slot[2] : 0.0;
+    return FM_OK;
+}
+
 // ---- PCG ------------------------------------------------------------------------
 template <typename T>
 struct PcgVecs {
@@ -980,6 +1004,7 @@ void fm_newton_cfg_default(fm_newton_cfg* c) {
     c->armijo_c = 1e-4;
     c->lift = 1;
     c->inner_max = 1000;
+    c->ls_mode = 0;
 }
 
 int fm_set_u(fm_ctx* ctx, const double* u) {
@@ -1078,9 +1103,13 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, tol, cap, &its));
         S.pcg_iters += its;
         S.factorizations++;
-        // armijo_line_search_true (FEM/adaptive_fem_engine.hpp:933-985)
-        double alpha = 1.0, rt = 0.0;
+        // armijo_line_search_true (FEM/adaptive_fem_engine.hpp:933-985); with
+        // ls_mode 1 a trial that fails the residual-norm test is also accepted
+        // when the energy decreases by c alpha R.du (the residual norm may grow
+        // on the way into the quadratic basin while the energy decreases)
+        double alpha = 1.0, rt = 0.0, pi0 = 0.0, slope = 0.0;
         bool accepted = false;
+        if (cfg.ls_mode == 1) FM_TRY(energy(ctx, ctx->u.p, lf, ctx->du.p, &pi0, &slope));
         for (int bt = 0; bt < cfg.ls_max; ++bt) {
             launch_axpy_to(s, n, ctx->u.p, alpha, ctx->du.p, ctx->ut.p);
             FM_TRY(residual(ctx, ctx->ut.p, lf, &rt));
@@ -1088,6 +1117,14 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
             if (rt <= (1.0 - cfg.armijo_c * alpha) * rn + 1e-14) {
                 accepted = true;
                 break;
+            }
+            if (cfg.ls_mode == 1 && slope > 0.0) {
+                double pit = 0.0;
+                FM_TRY(energy(ctx, ctx->ut.p, lf, nullptr, &pit, nullptr));
+                if (pit <= pi0 - cfg.armijo_c * alpha * slope) {
+                    accepted = true;
+                    break;
+                }
             }
             alpha *= 0.5;
         }

@@ -170,7 +170,7 @@ struct ElemArgs {
     const double* u;       // 3nn or nullptr
 };
 
-// error of the last failed cooperative launch (cleared when read)
+// error of the last failed persistent-kernel launch (cleared when read)
 hipError_t take_launch_error();
 void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v);
 void launch_fill_f64(hipStream_t s, double* p, size_t n, double v);
@@ -221,6 +221,8 @@ void launch_sub_masked(hipStream_t s, uint32_t n, const double* R, const double*
 // u += du on free rows, u += dD on constrained rows
 void launch_apply_predictor(hipStream_t s, uint32_t n, const double* du, const double* dD, const uint8_t* isdir,
                             double* u);
+// block partials of sum_e V_e psi(F_e); returns #partials
+uint32_t launch_energy_elem(hipStream_t s, const ElemArgs& a, double* partials);
 void launch_adjoint_elem(hipStream_t s, const ElemArgs& a, const uint32_t* eperm, const double* lam_vec,
                          double* dmu, double* dlam);
 void launch_rot_batch(hipStream_t s, uint32_t n, const double* F, double alpha, double* R, double* w, double* Q);

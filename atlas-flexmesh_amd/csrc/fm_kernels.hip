@@ -1039,6 +1039,27 @@ __global__ void adjoint_elem_kernel(ElemArgs A, const uint32_t* __restrict__ epe
     if (dlam) dlam[e] = -dl;
 }
 
+// Stored energy V_e psi(F_e), psi = mu/2 (tr F^T F - 3) - mu ln J + lam/2 ln^2 J
+// with the same ln max(J, 0.01) clamp as the stress; block partials of the sum.
+__global__ __launch_bounds__(256) void energy_elem_kernel(ElemArgs A, double* __restrict__ partials) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    double part = 0.0;
+    if (t < A.ne) {
+        Elem E;
+        double F[9];
+        elem_eval(A, t, E, F);
+        if (E.ok) {
+            double I1 = 0.0;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) I1 += F[k] * F[k];
+            const double lnJ = log(fmax(fabs(det3(F)), 0.01));
+            part = E.V * (0.5 * E.mu * (I1 - 3.0) - E.mu * lnJ + 0.5 * E.lam * lnJ * lnJ);
+        }
+    }
+    const double s = block_sum(part);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
 // ---------------------------------------------------------------------------
 // FP64 rotation part: polar factor by the scaled Newton iteration
 // X <- (zeta X + X^{-T}/zeta)/2 (Higham 1986), angle-axis logarithm with the
@@ -1527,6 +1548,11 @@ void launch_apply_predictor(hipStream_t s, uint32_t n, const double* du, const d
 void launch_adjoint_elem(hipStream_t s, const ElemArgs& a, const uint32_t* eperm, const double* lam_vec,
                          double* dmu, double* dlam) {
     adjoint_elem_kernel<<<cdiv(a.ne, 128), 128, 0, s>>>(a, eperm, lam_vec, dmu, dlam);
+}
+uint32_t launch_energy_elem(hipStream_t s, const ElemArgs& a, double* partials) {
+    const uint32_t nb = cdiv(a.ne, 256);
+    energy_elem_kernel<<<nb, 256, 0, s>>>(a, partials);
+    return nb;
 }
 void launch_rot_batch(hipStream_t s, uint32_t n, const double* F, double alpha, double* R, double* w, double* Q) {
     if (n) rot_batch_kernel<<<cdiv(n, 128), 128, 0, s>>>(n, F, alpha, R, w, Q);

@@ -26,7 +26,8 @@ _vp = C.c_void_p
 class NewtonCfg(C.Structure):
     _fields_ = [("n_steps", C.c_uint32), ("prec", C.c_int), ("newton_rtol", C.c_double), ("max_newton", C.c_int),
                 ("pcg_tol", C.c_double), ("pcg_max", C.c_int), ("inner_tol", C.c_double), ("ls_max", C.c_int),
-                ("armijo_c", C.c_double), ("lift", C.c_int), ("inner_max", C.c_int)]
+                ("armijo_c", C.c_double), ("lift", C.c_int), ("inner_max", C.c_int),
+                ("ls_mode", C.c_int)]
 
 
 class NewtonStats(C.Structure):

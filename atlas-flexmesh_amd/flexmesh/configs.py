@@ -98,7 +98,10 @@ def make(k, dims=None, L_box=None):
         # transverse (-z) dead traction; magnitude gives > 60 degrees of tip rotation
         cfg["traction"] = 0.0036
         cfg["f_ext"] = face_traction(X, conn, on, np.array([0.0, 0.0, -cfg["traction"]]))
-        cfg.update(newton_rtol=1e-8, pcg_tol=1e-10, prec=L.FM_MIXED)
+        # the slender beam's bending modes hold ILU(0)-PCG on a plateau of
+        # ~4 n_x iterations before it converges, at any tolerance
+        cfg.update(newton_rtol=1e-8, pcg_tol=1e-10, prec=L.FM_MIXED, ls_mode=1, max_newton=60,
+                   pcg_max=8 * nx + 400, inner_max=8 * nx + 400)
     else:
         raise ValueError(k)
     if stretch != 0.0:
@@ -123,6 +126,9 @@ def load(solver, cfg):
 def newton_cfg(cfg, **kw):
     c = L.Solver.newton_cfg(n_steps=cfg["n_steps"], prec=cfg["prec"], newton_rtol=cfg["newton_rtol"],
                             pcg_tol=cfg["pcg_tol"])
+    for k in ("ls_mode", "max_newton", "pcg_max", "inner_max"):
+        if k in cfg:
+            setattr(c, k, cfg[k])
     for k, v in kw.items():
         setattr(c, k, v)
     return c

@@ -122,6 +122,8 @@ typedef struct {
     double armijo_c;
     int lift;             /* 1: predictor solve with the prescribed increment */
     int inner_max;        /* FP32 PCG iteration cap of the mixed path */
+    int ls_mode;          /* 0: Armijo on ||R|| only (reference); 1: a trial is also accepted when the
+                             potential energy sum V psi - lf f_ext.u decreases by c alpha R.du */
 } fm_newton_cfg;
 
 typedef struct {

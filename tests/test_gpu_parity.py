@@ -44,7 +44,7 @@ def small_problem(fm, k=3, dims=(4, 3, 5)):
 
 def oracle_problem(O, cfg, **kw):
     return O.NewtonProblem(cfg["X"], cfg["conn"], cfg["mu"], cfg["lam"], cfg["dir_dofs"], cfg["dir_vals"],
-                           cfg["f_ext"], cfg["n_steps"], **kw)
+                           cfg["f_ext"], cfg["n_steps"], **{**{k: cfg[k] for k in ("ls_mode", "max_newton", "pcg_max") if k in cfg}, **kw})
 
 
 # ---------------------------------------------------------------------------- pattern / colouring
@@ -240,17 +240,36 @@ def test_newton_mixed_matches_fp64_oracle(fm, S, O):
     assert np.linalg.norm(f) <= 1e-6 * np.abs(f0).max()
 
 
-def test_newton_traction_beam_fp64(fm, S, O):
-    # traction-driven (no Dirichlet increment, so no lift): the first two of 40 steps
-    cfg = fm.configs.make(5, dims=(12, 2, 2))
+@pytest.mark.parametrize("ls_mode", [0, 1])
+def test_newton_traction_beam_fp64(fm, S, O, ls_mode):
+    # traction-driven (no Dirichlet increment, so no lift): the first two of 40
+    # steps, with the residual-norm rule (0) and the energy rule (1)
+    cfg = fm.configs.make(5, dims=(16, 2, 2))
     cfg["prec"] = fm.FM_FP64
+    cfg["ls_mode"] = ls_mode
     fm.configs.load(S, cfg)
     uo, sto = run_oracle(O, cfg, range(1, 3))
     c = fm.configs.newton_cfg(cfg)
     for s in range(1, 3):
         rc, st = S.newton_step(s, c)
         assert st["converged"] and st["newton_iters"] == sto[s - 1]["newton_iters"]
+    if ls_mode == 1:
+        assert max(t["newton_iters"] for t in sto) <= 5
     assert rel(S.get_u(), uo) < 1e-9
+
+
+def test_newton_traction_beam_mixed_energy_rule(fm, S, O):
+    # the beam's own setting: mixed precision with the energy rule
+    cfg = fm.configs.make(5, dims=(32, 4, 4))
+    assert cfg["ls_mode"] == 1 and cfg["prec"] == fm.FM_MIXED
+    fm.configs.load(S, cfg)
+    c = fm.configs.newton_cfg(cfg)
+    for s in range(1, 3):
+        rc, st = S.newton_step(s, c)
+        assert st["converged"] and st["newton_iters"] <= 8
+    cfg["prec"] = fm.FM_FP64
+    uo, _ = run_oracle(O, cfg, range(1, 3), pcg_max=3000)
+    assert rel(S.get_u(), uo) < 1e-6
 
 
 # ---------------------------------------------------------------------------- adjoint / rotation / checkpoint

@@ -460,3 +460,66 @@ def test_adjoint_matches_central_fd(O):
         assert abs(fd - ad) <= 1e-5 * abs(ad) + 1e-12
     fd = (loss(solve(mu0, lam0, 1 + h)[1]) - loss(solve(mu0, lam0, 1 - h)[1])) / (2 * h)
     assert abs(fd - dload) <= 1e-5 * abs(dload) + 1e-12
+
+
+def _slender_beam(O, nx=16, load=0.0036):
+    # 8 x 1 x 1 cantilever, clamped at x = 0, transverse nodal load on the x = 8 face
+    X, conn = O.box_mesh(nx, 2, 2, 8.0, 1.0, 1.0)
+    nn, ne = X.size // 3, conn.size // 4
+    x = X[:nn]
+    clamp = np.where(x == 0.0)[0]
+    dofs = np.sort(np.concatenate([3 * clamp, 3 * clamp + 1, 3 * clamp + 2])).astype(np.uint32)
+    f_ext = np.zeros(3 * nn)
+    tip = np.where(x == 8.0)[0]
+    f_ext[3 * tip + 2] = -load / tip.size
+    E, nu = 1.0, 0.45
+    mu, lam = E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+    return X, conn, np.full(ne, mu), np.full(ne, lam), dofs, np.zeros(dofs.size), f_ext
+
+
+def test_total_energy_gradient_is_minus_residual(O):
+    X, conn, mu, lam, dofs, vals, f_ext = _slender_beam(O, nx=6)
+    nn = X.size // 3
+    P = O.NewtonProblem(X, conn, mu, lam, dofs, vals, f_ext, 1)
+    assert P.energy(1.0, np.zeros(3 * nn)) == 0.0  # psi(I) = 0 and no work at u = 0
+    rng = np.random.default_rng(7)
+    u = 0.05 * rng.standard_normal(3 * nn)
+    u[dofs] = 0.0
+    rp, ci = O.pattern(nn, conn)
+    col, nc = O.color(nn, conn)
+    _, f = O.assemble(X, conn, mu, lam, col, nc, u, rp, ci, want_K=False)
+    lf = 0.7
+    R = lf * f_ext - f
+    R[dofs] = 0.0
+    v = rng.standard_normal(3 * nn)
+    v[dofs] = 0.0
+    h = 1e-5
+    fd = (P.energy(lf, u + h * v) - P.energy(lf, u - h * v)) / (2 * h)
+    assert abs(fd + R @ v) <= 1e-7 * np.abs(R).sum() * np.abs(v).max()
+    # a plausible slip (forgetting the work term) fails the same check
+    Pn = O.NewtonProblem(X, conn, mu, lam, dofs, vals, None, 1)
+    fdn = (Pn.energy(lf, u + h * v) - Pn.energy(lf, u - h * v)) / (2 * h)
+    assert abs(fdn + R @ v) > 1e3 * abs(fd + R @ v)
+
+
+def test_energy_line_search_converges_where_residual_rule_stalls(O):
+    # Slender cantilever under a transverse load: the full Newton step raises
+    # ||R|| by ~100x before quadratic convergence, so the residual-norm Armijo
+    # rule backtracks into many short steps; the energy rule accepts it.
+    X, conn, mu, lam, dofs, vals, f_ext = _slender_beam(O)
+    nn = X.size // 3
+    out = {}
+    for mode in (0, 1):
+        P = O.NewtonProblem(X, conn, mu, lam, dofs, vals, f_ext, 40, newton_rtol=1e-8, pcg_tol=1e-10,
+                            pcg_max=3000, max_newton=60, ls_mode=mode)
+        u = np.zeros(3 * nn)
+        its = []
+        for s in (1, 2):
+            rc, st = P.step(s, u)
+            assert rc == 0
+            its.append(st["newton_iters"])
+        out[mode] = (u, its, P.energy(2 / 40, u))
+    assert max(out[1][1]) <= 5 and min(out[0][1]) > max(out[1][1])
+    u0, u1 = out[0][0], out[1][0]
+    assert np.abs(u1 - u0).max() <= 1e-6 * np.abs(u0).max()
+    assert abs(out[1][2] - out[0][2]) <= 1e-9 * abs(out[0][2])
