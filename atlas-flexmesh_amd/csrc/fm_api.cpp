@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <utility>
@@ -114,6 +115,21 @@ int check_device_error(fm_ctx* ctx) {
 }
 
 // ---- set-up ---------------------------------------------------------------------
+// chain-bundle sweeps for 3-row node-blocked matrices that fit them
+// (FM_SWEEP=level keeps the level-scheduled sweeps)
+int chain_setup(fm_ctx* ctx, Matrix& M, const std::vector<uint32_t>& rp, const std::vector<uint32_t>& ci) {
+    M.chain.ok = false;
+    static const bool level_only = [] {
+        const char* e = std::getenv("FM_SWEEP");
+        return e && std::string(e) == "level";
+    }();
+    if (level_only || M.sched.B != 3) return FM_OK;
+    std::string why, err;
+    if (!build_chain_schedule(rp, ci, 4, M.chain, ctx->stream, why, err) && !err.empty())
+        return fail(ctx, FM_EDEVICE, err);
+    return FM_OK;
+}
+
 int ensure_pattern(fm_ctx* ctx);
 
 int ensure_colors(fm_ctx* ctx) {
@@ -223,6 +239,7 @@ int ensure_pattern(fm_ctx* ctx) {
         rp3[n] = (uint32_t)ci3.size();
         std::string err;
         if (!build_schedule(rp3, ci3, 3, M.sched, ctx->stream, err)) return fail(ctx, FM_EINVAL, err);
+        FM_TRY(chain_setup(ctx, M, rp3, ci3));
     }
     M.val_valid[0] = M.val_valid[1] = false;
     M.lu_valid[0] = M.lu_valid[1] = false;
@@ -392,6 +409,11 @@ int factorize(fm_ctx* ctx, Matrix& A, int prec) {
         launch_ilu_factor(ctx->stream, prec, A, val, lu, ctx->st.p, ctx->d_err.p);
     }
     FM_TRY(check_launch(ctx, "ilu_factor"));
+    if (A.chain.ok) {
+        ProfScope ps(ctx, FM_K_ILU_FACTOR);
+        if (chain_pack(ctx->stream, prec, A) != FM_OK) return fail(ctx, FM_EDEVICE, "allocation of the packed factor failed");
+        FM_TRY(check_launch(ctx, "chain_pack"));
+    }
     FM_TRY(check_device_error(ctx));
     A.lu_valid[prec] = true;
     return FM_OK;
@@ -403,13 +425,18 @@ int precond_apply(fm_ctx* ctx, Matrix& A, int prec, const void* r, void* z, void
     launch_fill_sentinel(ctx->stream, prec, z, y, A.n);
     {
         ProfScope ps(ctx, FM_K_ILU_FWD);
-        launch_ilu_fwd(ctx->stream, prec, A, lu, r, nullptr, nullptr, nullptr, nullptr, z, ctx->st.p, false,
-                       ctx->d_err.p);
+        if (A.chain.ok)
+            launch_chain_fwd(ctx->stream, prec, A, r, nullptr, nullptr, nullptr, nullptr, z, ctx->st.p, false, false,
+                             ctx->d_err.p);
+        else
+            launch_ilu_fwd(ctx->stream, prec, A, lu, r, nullptr, nullptr, nullptr, nullptr, z, ctx->st.p, false,
+                           ctx->d_err.p);
     }
     uint32_t np;
     {
         ProfScope ps(ctx, FM_K_ILU_BWD);
-        np = launch_ilu_bwd(ctx->stream, prec, A, lu, z, r, y, partials, ctx->st.p, false, ctx->d_err.p);
+        np = A.chain.ok ? launch_chain_bwd(ctx->stream, prec, A, z, r, y, partials, ctx->st.p, false, ctx->d_err.p)
+                        : launch_ilu_bwd(ctx->stream, prec, A, lu, z, r, y, partials, ctx->st.p, false, ctx->d_err.p);
     }
     return (int)np;
 }
@@ -483,13 +510,27 @@ int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult*
             }
             launch_reduce(s, ctx->partials.p, spmv_blocks(A), ctx->st.p, 1, 0);
             {
-                ProfScope ps(ctx, FM_K_ILU_FWD);
-                launch_ilu_fwd(s, prec, A, lu, nullptr, r, x, p, Ap, z, ctx->st.p, true, ctx->d_err.p);
+                if (A.chain.ok) {
+                    // the update runs as its own streaming kernel: fused, its
+                    // stores would sit on the sweep's serial step path
+                    {
+                        ProfScope pu(ctx, FM_K_PCG_OTHER);
+                        launch_pcg_xr_update(s, prec, n, p, Ap, x, r, ctx->st.p);
+                    }
+                    ProfScope ps(ctx, FM_K_ILU_FWD);
+                    launch_chain_fwd(s, prec, A, r, nullptr, nullptr, nullptr, nullptr, z, ctx->st.p, false, true,
+                                     ctx->d_err.p);
+                } else {
+                    ProfScope ps(ctx, FM_K_ILU_FWD);
+                    launch_ilu_fwd(s, prec, A, lu, nullptr, r, x, p, Ap, z, ctx->st.p, true, ctx->d_err.p);
+                }
             }
             uint32_t nbb;
             {
                 ProfScope ps(ctx, FM_K_ILU_BWD);
-                nbb = launch_ilu_bwd(s, prec, A, lu, z, r, y, ctx->partials.p, ctx->st.p, true, ctx->d_err.p);
+                nbb = A.chain.ok
+                          ? launch_chain_bwd(s, prec, A, z, r, y, ctx->partials.p, ctx->st.p, true, ctx->d_err.p)
+                          : launch_ilu_bwd(s, prec, A, lu, z, r, y, ctx->partials.p, ctx->st.p, true, ctx->d_err.p);
             }
             launch_reduce(s, ctx->partials.p, nbb, ctx->st.p, 2, 0);
             launch_pcg_p_update(s, prec, n, y, p, ctx->st.p);
@@ -816,6 +857,7 @@ int fm_set_matrix(fm_ctx* ctx, uint32_t n, const uint32_t* row_ptr, const uint32
     const int B = detect_block(M.h_row_ptr, M.h_col_idx);
     std::string err;
     if (!build_schedule(M.h_row_ptr, M.h_col_idx, B, M.sched, ctx->stream, err)) return fail(ctx, FM_EINVAL, err);
+    FM_TRY(chain_setup(ctx, M, M.h_row_ptr, M.h_col_idx));
     FM_HIP(M.row_ptr.alloc(n + 1));
     FM_HIP(M.col_idx.alloc(nnz));
     FM_HIP(M.val64.alloc(nnz));
@@ -1338,6 +1380,22 @@ int fm_info(fm_ctx* ctx, uint64_t* out5) {
     out5[2] = ctx->ne;
     out5[3] = ctx->nn;
     out5[4] = ctx->n_colors;
+    return FM_OK;
+}
+
+int fm_sweep_info(fm_ctx* ctx, uint64_t* o) {
+    if (!ctx || !o) return FM_EINVAL;
+    const Matrix* A = ctx->cur ? ctx->cur : &ctx->tangent;
+    const ChainSchedule& C = A->chain;
+    o[0] = C.ok ? 1 : 0;
+    o[1] = A->sched.levels_fwd;
+    o[2] = A->sched.levels_bwd;
+    o[3] = C.ok ? C.chains : 0;
+    o[4] = C.ok ? C.fwd.nb : 0;
+    o[5] = C.ok ? C.fwd.nq : 0;
+    o[6] = C.ok ? (uint64_t)C.lpn : 0;
+    o[7] = C.ok ? C.gj : 0;
+    o[8] = C.ok ? C.gk : 0;
     return FM_OK;
 }
 

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "flexmesh/fm_api.h"
+#include "fm_plan.hpp"
 
 namespace fm {
 
@@ -66,6 +67,31 @@ struct Schedule {
     DevBuf<uint16_t> nlow;   // number of lower column blocks per node
 };
 
+// Chain-bundle schedule of the triangular sweeps of a 3-row node-blocked
+// matrix.  A chain is a maximal run of consecutive nodes in which each node
+// depends on its predecessor (the x-lines of a box mesh in natural order); a
+// bundle is 64/lpn chains that one wave advances together, one node per chain
+// per step, in a skewed wavefront: dependencies inside the bundle pass through
+// an LDS ring, dependencies on other bundles through the ready-by-value
+// protocol.  Bundles are processed in a topological order of the bundle graph.
+struct ChainSweep {
+    uint32_t nb = 0;                 // bundles, in processing order
+    uint32_t nq = 0;                 // wave-steps over all bundles
+    DevBuf<uint32_t> bstart, bsteps; // per bundle: first wave-step, number of steps
+    DevBuf<uint32_t> far;            // nq * 64 * 4: per (wave-step, lane) node of the lane's quad
+                                     // (kDepNone: idle), dependency words 0 and 1, zero
+    DevBuf<float> pk32;              // packed factor values of this sweep
+    DevBuf<double> pk64;
+};
+
+struct ChainSchedule {
+    bool ok = false;
+    int lpn = 4;                     // lanes per node (the sweep kernels use one quad per node)
+    uint32_t chains = 0, stride = 0, gj = 0, gk = 0;
+    ChainSweep fwd, bwd;
+    DevBuf<double> sink;             // 4 values per lane and bundle (>= per block): idle-lane stores and polls
+};
+
 struct Matrix {
     uint32_t n = 0;
     uint64_t nnz = 0;
@@ -78,6 +104,7 @@ struct Matrix {
     bool lu_valid[2] = {false, false};
     bool val_valid[2] = {false, false};
     Schedule sched;
+    ChainSchedule chain;
     std::vector<uint32_t> perm;  // sorted position k -> caller's storage position (external matrices)
     bool ready = false;
 };
@@ -155,6 +182,11 @@ uint32_t greedy_colors(uint32_t nn, uint32_t ne, const uint32_t* conn, uint32_t*
 bool build_schedule(const std::vector<uint32_t>& rp, const std::vector<uint32_t>& ci, int B, Schedule& S,
                     hipStream_t stream, std::string& err);
 int detect_block(const std::vector<uint32_t>& rp, const std::vector<uint32_t>& ci);
+// chain-bundle schedule (plan_chain_schedule, then upload); returns false
+// with the reason in `why` when the matrix does not fit it (the
+// level-scheduled sweeps are used then), and also on an upload failure (err set)
+bool build_chain_schedule(const std::vector<uint32_t>& rp, const std::vector<uint32_t>& ci, int lpn,
+                          ChainSchedule& C, hipStream_t stream, std::string& why, std::string& err);
 
 // ---------------------------------------------------------------------------
 // kernel launchers (fm_kernels.hip); all enqueue on `s`
@@ -172,6 +204,10 @@ struct ElemArgs {
 
 // error of the last failed persistent-kernel launch (cleared when read)
 hipError_t take_launch_error();
+void record_launch_error(hipError_t e);
+// grid of a persistent one-wave-block kernel: min(want, half of what the
+// device keeps resident at once), so every block runs from the start
+uint32_t persistent_grid(const void* kern, uint32_t want);
 void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v);
 void launch_fill_f64(hipStream_t s, double* p, size_t n, double v);
 void launch_fill_f32(hipStream_t s, float* p, size_t n, float v);
@@ -200,8 +236,19 @@ void launch_ilu_fwd(hipStream_t s, int prec, const Matrix& A, const void* lu, co
                     const void* p, const void* Ap, void* z, PcgState* st, bool upd, int* err);
 uint32_t launch_ilu_bwd(hipStream_t s, int prec, const Matrix& A, const void* lu, const void* z, const void* r,
                         void* y, double* partials, PcgState* st, bool check_done, int* err);
+// chain-bundle sweeps (fm_sweep.hip): repack of the factor after each
+// factorization, then forward (z = L^{-1} rhs, or with the fused PCG update
+// when upd) and backward (y = U^{-1} z with the fused r.y partials; returns
+// the number of partials)
+int chain_pack(hipStream_t s, int prec, Matrix& A);
+void launch_chain_fwd(hipStream_t s, int prec, const Matrix& A, const void* rhs, void* r, void* x, const void* p,
+                      const void* Ap, void* z, PcgState* st, bool upd, bool check_done, int* err);
+uint32_t launch_chain_bwd(hipStream_t s, int prec, const Matrix& A, const void* z, const void* r, void* y,
+                          double* partials, PcgState* st, bool check_done, int* err);
 void launch_fill_sentinel(hipStream_t s, int prec, void* a, void* b, size_t n);
 void launch_pcg_p_update(hipStream_t s, int prec, uint32_t n, const void* y, void* p, const PcgState* st);
+void launch_pcg_xr_update(hipStream_t s, int prec, uint32_t n, const void* p, const void* Ap, void* x, void* r,
+                          const PcgState* st);
 // r = b - Ap; x untouched; partials of r.r
 uint32_t launch_pcg_init_r(hipStream_t s, int prec, uint32_t n, const void* b, const void* Ap, void* r,
                            double* partials);

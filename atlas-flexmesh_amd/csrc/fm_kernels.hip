@@ -4,6 +4,7 @@
 // dense contraction exists, so MFMA is not used.  Kernels are written for
 // 64-wide wavefronts; grid-wide kernels remap blockIdx so that each of the
 // eight XCDs (own L2 each) works on one contiguous slice of rows / elements.
+#include "fm_device.hpp"
 #include "fm_internal.hpp"
 
 #include <algorithm>
@@ -13,10 +14,6 @@
 namespace fm {
 namespace {
 
-#ifndef FM_SPIN_SLEEP
-#define FM_SPIN_SLEEP 2
-#endif
-constexpr uint32_t kSpinLimit = 1u << 20;  // bounded wait (~1 s); then the kernel flags an error and drains
 constexpr uint32_t kNoNode = 0xFFFFFFFFu;   // padding entry of a level-ordered queue
 
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nb) {
@@ -26,81 +23,6 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nb) {
     const uint32_t q = nb / NX, r = nb % NX;
     const uint32_t x = bid % NX, loc = bid / NX;
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + loc;
-}
-
-// ---------------------------------------------------------------------------
-// ready-by-value publication for the sync-free solves: every output slot is
-// preset to an all-ones NaN pattern; a producer stores the value with an
-// agent-scope atomic store, a consumer spins (agent-scope atomic load) until
-// the slot is no longer the sentinel.  A computed all-ones pattern is replaced
-// by the canonical quiet NaN so it cannot be mistaken for "not ready".
-// ---------------------------------------------------------------------------
-template <typename T>
-struct SB;
-template <>
-struct SB<float> {
-    using U = uint32_t;
-    static constexpr U S = 0xFFFFFFFFu, Q = 0x7FC00000u;
-};
-template <>
-struct SB<double> {
-    using U = unsigned long long;
-    static constexpr U S = 0xFFFFFFFFFFFFFFFFull, Q = 0x7FF8000000000000ull;
-};
-
-template <typename T>
-__device__ __forceinline__ void publish(T* p, T v) {
-    using U = typename SB<T>::U;
-    U b = __builtin_bit_cast(U, v);
-    if (b == SB<T>::S) b = SB<T>::Q;
-    __hip_atomic_store(reinterpret_cast<U*>(p), b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-template <typename T>
-__device__ __forceinline__ T await_val(const T* p, int* err) {
-    using U = typename SB<T>::U;
-    U* q = const_cast<U*>(reinterpret_cast<const U*>(p));
-    U b = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t spins = 0;
-    while (b == SB<T>::S) {
-        if (++spins > kSpinLimit) {
-            atomicOr(err, 1);
-            break;
-        }
-        // once any wave has hit the limit every other wait gives up too
-        if ((spins & 255u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-        __builtin_amdgcn_s_sleep(FM_SPIN_SLEEP);
-        b = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return __builtin_bit_cast(T, b);
-}
-
-// two waits whose loads are issued together (one round trip when both are ready)
-template <typename T>
-__device__ __forceinline__ void await2(const T* p0, const T* p1, bool two, T& v0, T& v1, int* err) {
-    using U = typename SB<T>::U;
-    U* q0 = const_cast<U*>(reinterpret_cast<const U*>(p0));
-    U* q1 = const_cast<U*>(reinterpret_cast<const U*>(p1));
-    U b0 = __hip_atomic_load(q0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    U b1 = two ? __hip_atomic_load(q1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (U)0;
-    uint32_t spins = 0;
-    while (b0 == SB<T>::S || b1 == SB<T>::S) {
-        if (++spins > kSpinLimit) {
-            atomicOr(err, 1);
-            break;
-        }
-        if ((spins & 255u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-        __builtin_amdgcn_s_sleep(FM_SPIN_SLEEP);
-        if (b0 == SB<T>::S) b0 = __hip_atomic_load(q0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (b1 == SB<T>::S) b1 = __hip_atomic_load(q1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    v0 = __builtin_bit_cast(T, b0);
-    v1 = __builtin_bit_cast(T, b1);
-}
-
-template <typename T>
-__device__ __forceinline__ T sentinel_value() {
-    return __builtin_bit_cast(T, SB<T>::S);
 }
 
 // fixed-order block sum (blockDim.x a multiple of 64); result valid in thread 0
@@ -903,6 +825,18 @@ __global__ void pcg_p_update_kernel(uint32_t n, const T* __restrict__ y, T* __re
     p[i] = y[i] + beta * p[i];
 }
 
+// x += alpha p, r -= alpha Ap (the update the level-scheduled forward sweep fuses)
+template <typename T>
+__global__ void pcg_xr_update_kernel(uint32_t n, const T* __restrict__ p, const T* __restrict__ Ap,
+                                     T* __restrict__ x, T* __restrict__ r, const PcgState* __restrict__ st) {
+    if (st->done) return;
+    const uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T alpha = (T)st->alpha;
+    x[i] = x[i] + alpha * p[i];
+    r[i] = r[i] - alpha * Ap[i];
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void pcg_init_r_kernel(uint32_t n, const T* __restrict__ b, const T* __restrict__ Ap,
                                                          T* __restrict__ r, double* __restrict__ partials) {
@@ -1209,20 +1143,9 @@ hipError_t g_launch_err = hipSuccess;
 // FM_EDEVICE instead of hanging.
 template <typename... Args>
 uint32_t launch_persistent(void (*kern)(Args...), uint32_t want, hipStream_t s, Args... args) {
-    static thread_local int dev = -1, cus = 0;
-    int d = 0;
-    (void)hipGetDevice(&d);
-    if (d != dev) {
-        dev = d;
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d);
-    }
-    int occ = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(kern), 64, 0);
-    const uint32_t cap = (uint32_t)std::max(1, occ * cus / 2);
-    const uint32_t grid = std::max(1u, std::min(want, cap));
+    const uint32_t grid = persistent_grid(reinterpret_cast<const void*>(kern), want);
     kern<<<grid, 64, 0, s>>>(args...);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) g_launch_err = e;
+    record_launch_error(hipGetLastError());
     return grid;
 }
 
@@ -1244,6 +1167,22 @@ hipError_t take_launch_error() {
     const hipError_t e = g_launch_err;
     g_launch_err = hipSuccess;
     return e;
+}
+void record_launch_error(hipError_t e) {
+    if (e != hipSuccess) g_launch_err = e;
+}
+uint32_t persistent_grid(const void* kern, uint32_t want) {
+    static thread_local int dev = -1, cus = 0;
+    int d = 0;
+    (void)hipGetDevice(&d);
+    if (d != dev) {
+        dev = d;
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d);
+    }
+    int occ = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64, 0);
+    const uint32_t cap = (uint32_t)std::max(1, occ * cus / 2);
+    return std::max(1u, std::min(want, cap));
 }
 
 void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v) {
@@ -1505,6 +1444,16 @@ void launch_pcg_p_update(hipStream_t s, int prec, uint32_t n, const void* y, voi
         pcg_p_update_kernel<double><<<cdiv(n, 256), 256, 0, s>>>(n, (const double*)y, (double*)p, st);
     else
         pcg_p_update_kernel<float><<<cdiv(n, 256), 256, 0, s>>>(n, (const float*)y, (float*)p, st);
+}
+
+void launch_pcg_xr_update(hipStream_t s, int prec, uint32_t n, const void* p, const void* Ap, void* x, void* r,
+                          const PcgState* st) {
+    if (prec == PREC64)
+        pcg_xr_update_kernel<double><<<cdiv(n, 256), 256, 0, s>>>(n, (const double*)p, (const double*)Ap,
+                                                                  (double*)x, (double*)r, st);
+    else
+        pcg_xr_update_kernel<float><<<cdiv(n, 256), 256, 0, s>>>(n, (const float*)p, (const float*)Ap, (float*)x,
+                                                                 (float*)r, st);
 }
 
 uint32_t launch_pcg_init_r(hipStream_t s, int prec, uint32_t n, const void* b, const void* Ap, void* r,

@@ -1,0 +1,42 @@
+// fm_plan.hpp -- host-side plan of the chain-bundle triangular sweeps (no HIP
+// types, so host-only programs can build and inspect a plan).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fm {
+
+constexpr int kChainMaxBlk = 8;               // dependency node blocks per node and direction
+constexpr int kChainRing = 8;                 // LDS ring depth in steps
+constexpr uint32_t kDepNone = 0xFFFFFFFFu;    // no dependency in this lane / idle lane group
+constexpr uint32_t kDepLocal = 0x80000000u;   // | LDS slot: dependency inside the bundle
+
+// One sweep direction.  Wave-step q of bundle b is bstart[b] + s for step s <
+// bsteps[b]; bundles are listed in processing order.
+//   node[q * (64/lpn) + g]            node of lane group g at wave-step q
+//   dep[(q * bpl + j) * 64 + lane]    dependency word of block j of the lane:
+//       kDepNone, kDepLocal | slot (values in LDS slot (step % ring) * 64/lpn + group),
+//       or the node id whose published values are awaited
+// Lane gl of a group holds the node's dependency blocks gl, gl + lpn, ...
+// (lower blocks in column order for the forward sweep, upper blocks for the
+// backward sweep).
+struct ChainPlanDir {
+    uint32_t nb = 0, nq = 0;
+    std::vector<uint32_t> bstart, bsteps, node, dep;
+};
+
+struct ChainPlan {
+    int lpn = 4;
+    uint32_t chains = 0, stride = 0, gj = 0, gk = 0;
+    ChainPlanDir fwd, bwd;
+};
+
+// Plan for a 3-row node-blocked CSR with sorted columns (only rows 3v are
+// read, so a compact view holding the node columns in row 3v suffices).
+// Returns false with the reason in `why` when the matrix does not fit.
+bool plan_chain_schedule(const std::vector<uint32_t>& rp, const std::vector<uint32_t>& ci, int lpn, ChainPlan& P,
+                         std::string& why);
+
+}  // namespace fm

@@ -203,4 +203,274 @@ bool build_schedule(const std::vector<uint32_t>& rp, const std::vector<uint32_t>
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Chain-bundle schedule of the sweeps (B = 3).  Chains: node v continues the
+// chain of v - 1 when v - 1 is one of its lower neighbours.  Chains are laid
+// out on a grid of `stride` columns (the most frequent chain-to-chain
+// dependence distance above one: the chains of one plane of a box mesh) and
+// grouped into gj x gk bundles, gj * gk = 64 / lpn.  Within a bundle the
+// step of a node is one more than the latest step of its chain predecessor
+// and of its dependencies inside the bundle; bundles are ordered by their
+// level in the bundle graph.  Every property the kernels rely on is checked
+// here; a matrix that fails one keeps the level-scheduled sweeps.
+// ---------------------------------------------------------------------------
+namespace {
+
+// dependency node blocks of v in one direction: lower blocks 0..nlow-1 or
+// upper blocks nlow+1..nblk-1, in column order
+inline void dep_blocks(const std::vector<uint32_t>& rp, const std::vector<uint32_t>& ci, uint32_t v, uint32_t nl,
+                       bool bwd, uint32_t* out, uint32_t& cnt) {
+    const uint32_t b = rp[3 * v], nblk = (rp[3 * v + 1] - b) / 3;
+    cnt = 0;
+    if (!bwd)
+        for (uint32_t k = 0; k < nl; ++k) out[cnt++] = ci[b + 3 * k] / 3;
+    else
+        for (uint32_t k = nl + 1; k < nblk; ++k) out[cnt++] = ci[b + 3 * k] / 3;
+}
+
+bool chain_direction(const std::vector<uint32_t>& rp, const std::vector<uint32_t>& ci,
+                     const std::vector<uint16_t>& nlow, const std::vector<uint32_t>& cid,
+                     const std::vector<uint32_t>& bnd, const std::vector<uint32_t>& grp, uint32_t nbund, int lpn,
+                     bool bwd, ChainPlanDir& H, std::string& why) {
+    const uint32_t nn = (uint32_t)nlow.size();
+    const int npw = 64 / lpn, bpl = kChainMaxBlk / lpn;
+    // bundle graph: edge bundle(m) -> bundle(v) for a dependency m of v
+    std::vector<uint64_t> edges;
+    uint32_t deps[kChainMaxBlk + 1], cnt = 0;
+    for (uint32_t v = 0; v < nn; ++v) {
+        dep_blocks(rp, ci, v, nlow[v], bwd, deps, cnt);
+        for (uint32_t k = 0; k < cnt; ++k)
+            if (bnd[deps[k]] != bnd[v]) edges.push_back((uint64_t)bnd[deps[k]] << 32 | bnd[v]);
+    }
+    std::sort(edges.begin(), edges.end());
+    edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
+    std::vector<uint32_t> indeg(nbund, 0), eptr(nbund + 1, 0), lev(nbund, 0);
+    for (uint64_t e : edges) {
+        eptr[(e >> 32) + 1]++;
+        indeg[(uint32_t)e]++;
+    }
+    for (uint32_t b = 0; b < nbund; ++b) eptr[b + 1] += eptr[b];
+    std::vector<uint32_t> stack;
+    for (uint32_t b = 0; b < nbund; ++b)
+        if (!indeg[b]) stack.push_back(b);
+    uint32_t seen = 0;
+    while (!stack.empty()) {
+        const uint32_t b = stack.back();
+        stack.pop_back();
+        ++seen;
+        for (uint32_t k = eptr[b]; k < eptr[b + 1]; ++k) {
+            const uint32_t t = (uint32_t)edges[k];
+            lev[t] = std::max(lev[t], lev[b] + 1);
+            if (--indeg[t] == 0) stack.push_back(t);
+        }
+    }
+    if (seen != nbund) {
+        why = "bundle graph has a cycle";
+        return false;
+    }
+    std::vector<uint32_t> order(nbund), pos(nbund);
+    for (uint32_t b = 0; b < nbund; ++b) order[b] = b;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return lev[a] != lev[b] ? lev[a] < lev[b] : a < b;
+    });
+    for (uint32_t i = 0; i < nbund; ++i) pos[order[i]] = i;
+    // steps, in the direction's node order (a topological order of the node DAG)
+    std::vector<uint32_t> st(nn, 0), nsteps(nbund, 0);
+    for (uint32_t t = 0; t < nn; ++t) {
+        const uint32_t v = bwd ? nn - 1 - t : t;
+        uint32_t sv = 0;
+        const uint32_t pred = bwd ? v + 1 : v - 1;
+        if ((bwd ? v + 1 < nn : v > 0) && cid[pred] == cid[v]) sv = st[pred] + 1;
+        dep_blocks(rp, ci, v, nlow[v], bwd, deps, cnt);
+        for (uint32_t k = 0; k < cnt; ++k)
+            if (bnd[deps[k]] == bnd[v]) sv = std::max(sv, st[deps[k]] + 1);
+        st[v] = sv;
+        nsteps[bnd[v]] = std::max(nsteps[bnd[v]], sv + 1);
+    }
+    H.bstart.assign(nbund, 0);
+    H.bsteps.assign(nbund, 0);
+    uint64_t q = 0;
+    for (uint32_t i = 0; i < nbund; ++i) {
+        H.bstart[i] = (uint32_t)q;
+        H.bsteps[i] = nsteps[order[i]];
+        q += nsteps[order[i]];
+    }
+    if (q >= (1ull << 31) / 64) {
+        why = "schedule too large";
+        return false;
+    }
+    H.nq = (uint32_t)q;
+    H.node.assign((size_t)q * npw, kDepNone);
+    H.dep.assign((size_t)q * bpl * 64, kDepNone);
+    for (uint32_t v = 0; v < nn; ++v) {
+        const uint32_t b = bnd[v], g = grp[v];
+        const uint64_t qs = (uint64_t)H.bstart[pos[b]] + st[v];
+        uint32_t& slot = H.node[qs * npw + g];
+        if (slot != kDepNone) {
+            why = "two nodes of one chain group share a step";
+            return false;
+        }
+        slot = v;
+        dep_blocks(rp, ci, v, nlow[v], bwd, deps, cnt);
+        if (cnt > (uint32_t)kChainMaxBlk) {
+            why = "too many dependency blocks";
+            return false;
+        }
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const uint32_t m = deps[k];
+            uint32_t w;
+            if (bnd[m] == b) {
+                if (st[m] >= st[v]) {
+                    why = "dependency inside a bundle is not earlier";
+                    return false;
+                }
+                w = st[v] - st[m] < (uint32_t)kChainRing
+                        ? kDepLocal | ((st[m] % kChainRing) * (uint32_t)npw + grp[m])
+                        : m;  // far back in this bundle: already published
+            } else {
+                if (pos[bnd[m]] >= pos[b]) {
+                    why = "dependency on a later bundle";
+                    return false;
+                }
+                w = m;
+            }
+            const int lane = (int)g * lpn + (int)(k % lpn), j = (int)(k / lpn);
+            H.dep[(qs * bpl + j) * 64 + lane] = w;
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+bool plan_chain_schedule(const std::vector<uint32_t>& rp, const std::vector<uint32_t>& ci, int lpn, ChainPlan& P,
+                         std::string& why) {
+    if (lpn != 4 && lpn != 8) {
+        why = "lanes per node must be 4 or 8";
+        return false;
+    }
+    const uint32_t n = (uint32_t)(rp.size() - 1);
+    if (n % 3 != 0 || n == 0 || n / 3 >= kDepLocal) {
+        why = "not a 3-row node-blocked matrix";
+        return false;
+    }
+    const uint32_t nn = n / 3;
+    std::vector<uint16_t> nlow(nn);
+    for (uint32_t v = 0; v < nn; ++v) {
+        const uint32_t b = rp[3 * v], e = rp[3 * v + 1];
+        uint32_t k = b;
+        while (k < e && ci[k] < 3 * v) ++k;
+        if (k == e || ci[k] != 3 * v || (k - b) % 3 != 0) {
+            why = "row without a stored diagonal block";
+            return false;
+        }
+        const uint32_t nl = (k - b) / 3, nu = (e - b) / 3 - nl - 1;
+        if (nl > (uint32_t)kChainMaxBlk || nu > (uint32_t)kChainMaxBlk) {
+            why = "more than 8 dependency blocks in a row";
+            return false;
+        }
+        nlow[v] = (uint16_t)nl;
+    }
+    // chains; the backward sweep walks the same chains in reverse, which
+    // needs v + 1 to be an upper neighbour of v exactly when v is a lower
+    // neighbour of v + 1 (true for a structurally symmetric pattern)
+    std::vector<uint32_t> cid(nn);
+    uint32_t nch = 0;
+    for (uint32_t v = 0; v < nn; ++v) {
+        const bool cont = v > 0 && nlow[v] > 0 && ci[rp[3 * v] + 3 * (nlow[v] - 1)] / 3 == v - 1;
+        cid[v] = cont ? cid[v - 1] : nch++;
+        if (v > 0) {
+            const uint32_t b = rp[3 * (v - 1)], nblk = (rp[3 * (v - 1) + 1] - b) / 3;
+            const bool up = (uint32_t)nlow[v - 1] + 1 < nblk && ci[b + 3 * (nlow[v - 1] + 1)] / 3 == v;
+            if (up != cont) {
+                why = "pattern is not structurally symmetric";
+                return false;
+            }
+        }
+    }
+    // chain grid stride: most frequent chain distance above one
+    std::vector<uint32_t> hist(nch + 1, 0);
+    for (uint32_t v = 0; v < nn; ++v)
+        for (uint32_t k = 0; k < nlow[v]; ++k) {
+            const uint32_t d = cid[v] - cid[ci[rp[3 * v] + 3 * k] / 3];
+            if (d >= 2) hist[d]++;
+        }
+    uint32_t stride = nch, best = 0;
+    for (uint32_t d = 2; d <= nch; ++d)
+        if (hist[d] > best) {
+            best = hist[d];
+            stride = d;
+        }
+    const uint32_t nj = stride, nk = (nch + stride - 1) / stride;
+    const uint32_t npw = 64 / (uint32_t)lpn;
+    uint32_t gj = npw, gk = 1, cost = UINT32_MAX;
+    for (uint32_t a = npw; a >= 1; a /= 2) {
+        const uint32_t c = (nj + a - 1) / a + (nk + npw / a - 1) / (npw / a);
+        if (c < cost) {
+            cost = c;
+            gj = a;
+            gk = npw / a;
+        }
+    }
+    const uint32_t nbj = (nj + gj - 1) / gj, nbk = (nk + gk - 1) / gk, nbund = nbj * nbk;
+    std::vector<uint32_t> bnd(nn), grp(nn);
+    for (uint32_t v = 0; v < nn; ++v) {
+        const uint32_t c = cid[v], j = c % stride, k = c / stride;
+        bnd[v] = (j / gj) + nbj * (k / gk);
+        grp[v] = (j % gj) + gj * (k % gk);
+    }
+    P.fwd.nb = P.bwd.nb = nbund;
+    if (!chain_direction(rp, ci, nlow, cid, bnd, grp, nbund, lpn, false, P.fwd, why) ||
+        !chain_direction(rp, ci, nlow, cid, bnd, grp, nbund, lpn, true, P.bwd, why))
+        return false;
+    P.lpn = lpn;
+    P.chains = nch;
+    P.stride = stride;
+    P.gj = gj;
+    P.gk = gk;
+    return true;
+}
+
+bool build_chain_schedule(const std::vector<uint32_t>& rp, const std::vector<uint32_t>& ci, int lpn,
+                          ChainSchedule& C, hipStream_t stream, std::string& why, std::string& err) {
+    C.ok = false;
+    ChainPlan P;
+    if (!plan_chain_schedule(rp, ci, lpn, P, why)) return false;
+    const uint32_t nbund = P.fwd.nb;
+    // device form: one 16-byte record per (wave-step, lane), read in one load
+    const int npw = 64 / P.lpn, bpl = kChainMaxBlk / P.lpn;
+    std::vector<uint32_t> far;
+    auto upload = [&](const ChainPlanDir& H, ChainSweep& W) {
+        W.nb = nbund;
+        W.nq = H.nq;
+        far.assign((size_t)H.nq * 64 * 4, 0);
+        for (size_t q = 0; q < H.nq; ++q)
+            for (int lane = 0; lane < 64; ++lane) {
+                uint32_t* f = &far[(q * 64 + lane) * 4];
+                f[0] = H.node[q * npw + lane / P.lpn];
+                for (int j = 0; j < bpl; ++j) f[1 + j] = H.dep[(q * bpl + j) * 64 + lane];
+            }
+        if (W.bstart.alloc(nbund) != hipSuccess || W.bsteps.alloc(nbund) != hipSuccess ||
+            W.far.alloc(far.size()) != hipSuccess)
+            return false;
+        return hipMemcpyAsync(W.bstart.p, H.bstart.data(), nbund * 4ull, hipMemcpyHostToDevice, stream) ==
+                   hipSuccess &&
+               hipMemcpyAsync(W.bsteps.p, H.bsteps.data(), nbund * 4ull, hipMemcpyHostToDevice, stream) ==
+                   hipSuccess &&
+               hipMemcpy(W.far.p, far.data(), far.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (P.lpn != 4 || !upload(P.fwd, C.fwd) || !upload(P.bwd, C.bwd) ||
+        C.sink.alloc((size_t)nbund * 64 * 4) != hipSuccess || hipStreamSynchronize(stream) != hipSuccess) {
+        err = "upload of the chain schedule failed";
+        return false;
+    }
+    C.lpn = P.lpn;
+    C.chains = P.chains;
+    C.stride = P.stride;
+    C.gj = P.gj;
+    C.gk = P.gk;
+    C.ok = true;
+    return true;
+}
+
 }  // namespace fm

@@ -93,6 +93,7 @@ def lib():
     L.fm_profile_reset.argtypes = [_vp]
     L.fm_profile_get.argtypes = [_vp, _f64p, np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")]
     L.fm_info.argtypes = [_vp, np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")]
+    L.fm_sweep_info.argtypes = [_vp, np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")]
     _lib = L
     return L
 
@@ -335,3 +336,11 @@ class Solver:
         o = np.zeros(5, dtype=np.uint64)
         self._chk(self.L.fm_info(self.h, o), "fm_info")
         return dict(zip(["n", "nnz", "ne", "nn", "ncolors"], (int(v) for v in o)))
+
+    def sweep_info(self):
+        o = np.zeros(9, dtype=np.uint64)
+        self._chk(self.L.fm_sweep_info(self.h, o), "fm_sweep_info")
+        d = dict(zip(["chain", "levels_fwd", "levels_bwd", "chains", "bundles", "wave_steps", "lanes_per_node",
+                      "gj", "gk"], (int(v) for v in o)))
+        d["sweep"] = "chain" if d.pop("chain") else "level"
+        return d

@@ -44,10 +44,18 @@ def kernel_bytes(info, kind, v):
     if kind == "spmv":
         # values + column indices once, row pointers, x once, y once (+ fused p.Ap reads x again, cached)
         return nnz * (v + i) + n * (i + 2 * v)
+    chain = info.get("sweep") == "chain"
     if kind == "ilu_fwd":
+        if chain:
+            # L part values, one dependency word per 3x3 block, rhs in, z out
+            # (the PCG update runs in its own kernel on this path)
+            return nlow * v + (nlow // 9) * i + n * 2 * v
         # L part values + columns, row pointers, rhs in, z out; fused PCG update reads r, Ap, p, x, writes r, x
         return nlow * (v + i) + n * (i + 2 * v) + n * 6 * v
     if kind == "ilu_bwd":
+        if chain:
+            # U part values (incl. the diagonal blocks), one word per off-diagonal block, z and r in, y out
+            return (nnz - nlow) * v + (nlow // 9) * i + n * 3 * v
         # U part (incl. diagonal) values + columns, row pointers, z and r in, y out
         return (nnz - nlow) * (v + i) + n * (i + 3 * v)
     if kind == "ilu_factor":
@@ -223,7 +231,9 @@ def main():
     t = time.perf_counter()
     fm.configs.load(S, cfg)
     info = S.info()
-    log(f"set-up (pattern, colouring, schedule, upload) {time.perf_counter() - t:.1f} s: {info}")
+    sw = S.sweep_info()
+    info["sweep"] = sw["sweep"]
+    log(f"set-up (pattern, colouring, schedule, upload) {time.perf_counter() - t:.1f} s: {info} sweeps {sw}")
     mixed = cfg["prec"] == fm.FM_MIXED
     prec = fm.FM_MIXED if mixed else fm.FM_FP64
     if args.micro:

@@ -8,6 +8,7 @@ tolerances, relative to the largest magnitude of the compared quantity:
   Newton displacement     <= 1e-9 (FP64 path), <= 1e-7 (mixed path, rtol 1e-8)
 """
 import os
+import subprocess
 import sys
 
 import numpy as np
@@ -136,6 +137,7 @@ def test_spmv_ilu0_match_oracle(fm, S, O, which):
     else:
         rp, ci, va = elastic_matrix(O, fm)
     S.set_matrix(rp, ci, va)
+    assert S.sweep_info()["sweep"] == ("level" if which == "poisson" else "chain")
     n = rp.size - 1
     x = np.random.default_rng(2).standard_normal(n)
     cs, vs, perm = sorted_csr(rp, ci, va)
@@ -152,6 +154,41 @@ def test_spmv_ilu0_match_oracle(fm, S, O, which):
     LU32 = S.ilu0_factor(fm.FM_MIXED)
     assert rel(LU32, LUo_user) < 1e-3
     assert rel(S.ilu0_apply(x, fm.FM_MIXED), zo) < 1e-3
+
+
+@pytest.mark.parametrize("dims", [(9, 7, 6), (24, 3, 3), (5, 11, 13)])
+def test_chain_sweeps_match_oracle(fm, S, O, dims):
+    # several bundles per direction, partial bundles at the edges, a slender
+    # beam; clamped (eliminated) tangents, as the Newton solve factorizes them
+    rp, ci, va = elastic_matrix(O, fm, dims=dims)
+    S.set_matrix(rp, ci, va)
+    info = S.sweep_info()
+    assert info["sweep"] == "chain" and info["chains"] == (dims[1] + 1) * (dims[2] + 1)
+    n = rp.size - 1
+    LUo, diag = O.ilu0(rp, ci, va)
+    # the FP32 sweeps against the oracle apply of the GPU's own FP32 factor:
+    # what is left is the FP32 arithmetic of the two triangular solves
+    LU32 = S.ilu0_factor(fm.FM_MIXED)
+    dpos = np.array([rp[i] + np.searchsorted(ci[rp[i]:rp[i + 1]], i) for i in range(n)])
+    for seed in range(3):
+        x = np.random.default_rng(seed).standard_normal(n)
+        zo = O.ilu0_apply(rp, ci, LUo, diag, x)
+        assert rel(S.ilu0_apply(x), zo) < 1e-12
+        x32 = x.astype(np.float32).astype(np.float64)
+        z32o = O.ilu0_apply(rp, ci, LU32, LU32[dpos], x32)
+        z32 = S.ilu0_apply(x, fm.FM_MIXED)
+        print(f"seed {seed}: fp32 sweeps vs own-factor oracle {rel(z32, z32o):.2e}, vs fp64 {rel(z32, zo):.2e}")
+        assert rel(z32, z32o) < 1e-4
+
+
+def test_level_sweeps_still_match_oracle(fm):
+    # the level-scheduled sweeps (FM_SWEEP=level) on a 3-row node-blocked matrix
+    here = os.path.dirname(os.path.abspath(__file__))
+    env = dict(os.environ, FM_SWEEP="level")
+    p = subprocess.run([sys.executable, os.path.join(here, "sweep_check.py"), "level"], env=env,
+                       capture_output=True, text=True, timeout=300)
+    print(p.stdout, p.stderr)
+    assert p.returncode == 0 and "SWEEP OK level" in p.stdout
 
 
 @pytest.mark.parametrize("gn", [16, 32, 48])
