@@ -130,6 +130,26 @@ int chain_setup(fm_ctx* ctx, Matrix& M, const std::vector<uint32_t>& rp, const s
     return FM_OK;
 }
 
+// column-node lists of a 3-row node-blocked matrix for the node-blocked SpMV
+// (rows 3v hold the node's columns; rp/ci may be the compact view)
+int node_cols_setup(fm_ctx* ctx, Matrix& M, const std::vector<uint32_t>& rp, const std::vector<uint32_t>& ci) {
+    M.node_spmv = false;
+    if (M.sched.B != 3) return FM_OK;
+    const uint32_t nn = (uint32_t)((rp.size() - 1) / 3);
+    std::vector<uint32_t> nptr(nn + 1, 0), ncol;
+    ncol.reserve(rp[3 * nn] / 3);
+    for (uint32_t v = 0; v < nn; ++v) {
+        for (uint32_t k = rp[3 * v]; k < rp[3 * v + 1]; k += 3) ncol.push_back(ci[k] / 3);
+        nptr[v + 1] = (uint32_t)ncol.size();
+    }
+    FM_HIP(M.nptr.alloc(nn + 1));
+    FM_HIP(M.ncol.alloc(ncol.size()));
+    FM_HIP(hipMemcpy(M.nptr.p, nptr.data(), (nn + 1) * 4ull, hipMemcpyHostToDevice));
+    FM_HIP(hipMemcpy(M.ncol.p, ncol.data(), ncol.size() * 4, hipMemcpyHostToDevice));
+    M.node_spmv = true;
+    return FM_OK;
+}
+
 int ensure_pattern(fm_ctx* ctx);
 
 int ensure_colors(fm_ctx* ctx) {
@@ -240,6 +260,7 @@ int ensure_pattern(fm_ctx* ctx) {
         std::string err;
         if (!build_schedule(rp3, ci3, 3, M.sched, ctx->stream, err)) return fail(ctx, FM_EINVAL, err);
         FM_TRY(chain_setup(ctx, M, rp3, ci3));
+        FM_TRY(node_cols_setup(ctx, M, rp3, ci3));
     }
     M.val_valid[0] = M.val_valid[1] = false;
     M.lu_valid[0] = M.lu_valid[1] = false;
@@ -858,6 +879,7 @@ int fm_set_matrix(fm_ctx* ctx, uint32_t n, const uint32_t* row_ptr, const uint32
     std::string err;
     if (!build_schedule(M.h_row_ptr, M.h_col_idx, B, M.sched, ctx->stream, err)) return fail(ctx, FM_EINVAL, err);
     FM_TRY(chain_setup(ctx, M, M.h_row_ptr, M.h_col_idx));
+    FM_TRY(node_cols_setup(ctx, M, M.h_row_ptr, M.h_col_idx));
     FM_HIP(M.row_ptr.alloc(n + 1));
     FM_HIP(M.col_idx.alloc(nnz));
     FM_HIP(M.val64.alloc(nnz));
@@ -1396,6 +1418,7 @@ int fm_sweep_info(fm_ctx* ctx, uint64_t* o) {
     o[6] = C.ok ? (uint64_t)C.lpn : 0;
     o[7] = C.ok ? C.gj : 0;
     o[8] = C.ok ? C.gk : 0;
+    o[9] = A->node_spmv ? 1 : 0;
     return FM_OK;
 }
 

@@ -105,6 +105,10 @@ struct Matrix {
     bool val_valid[2] = {false, false};
     Schedule sched;
     ChainSchedule chain;
+    // column nodes of each node row block (3-row node-blocked matrices): the
+    // node-blocked SpMV reads one index per 3x3 block
+    bool node_spmv = false;
+    DevBuf<uint32_t> nptr, ncol;
     std::vector<uint32_t> perm;  // sorted position k -> caller's storage position (external matrices)
     bool ready = false;
 };

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <string>
 
 namespace fm {
 namespace {
@@ -392,6 +393,71 @@ __global__ __launch_bounds__(256) void spmv_stream_kernel(uint32_t n, const uint
     if (DOT) {
         const double bs = block_sum(part);
         if (tid == 0) partials[tile] = bs;
+    }
+}
+
+// Node-blocked SpMV of a 3-row node-blocked matrix: one column-node index
+// per 3x3 block (nptr/ncol) instead of one column index per entry, which is
+// a third of the CSR bytes at 4-byte values.  16 lanes per node: lane k takes
+// blocks k, k + 16, ...; each block is a 3-value gather of x and the three
+// contiguous 3-value segments of the node's rows.  Fixed-order reduction.
+constexpr int kNodeLanes = 16;
+
+template <typename T, bool DOT>
+__global__ __launch_bounds__(256) void spmv_node_kernel(uint32_t nn, const uint32_t* __restrict__ rp,
+                                                        const uint32_t* __restrict__ nptr,
+                                                        const uint32_t* __restrict__ ncol,
+                                                        const T* __restrict__ val, const T* __restrict__ x,
+                                                        T* __restrict__ y, double* __restrict__ partials,
+                                                        T* __restrict__ sent0, T* __restrict__ sent1,
+                                                        const PcgState* __restrict__ st) {
+    if (st && st->done) return;
+    constexpr int NPB = 256 / kNodeLanes;
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t node = blk * NPB + threadIdx.x / kNodeLanes;
+    const int k = threadIdx.x % kNodeLanes;
+    T s0 = 0, s1 = 0, s2 = 0;
+    if (node < nn) {
+        const uint32_t b0 = nptr[node], nb = nptr[node + 1] - b0;
+        const uint32_t r0 = rp[3 * node], r1 = rp[3 * node + 1], r2 = rp[3 * node + 2];
+        for (uint32_t kk = k; kk < nb; kk += kNodeLanes) {
+            const T* a0 = val + r0 + 3 * kk;
+            const T* a1 = val + r1 + 3 * kk;
+            const T* a2 = val + r2 + 3 * kk;
+            T a[9];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                a[c] = __builtin_nontemporal_load(a0 + c);
+                a[3 + c] = __builtin_nontemporal_load(a1 + c);
+                a[6 + c] = __builtin_nontemporal_load(a2 + c);
+            }
+            const T* xm = x + 3 * (size_t)__builtin_nontemporal_load(ncol + b0 + kk);
+            const T x0 = xm[0], x1 = xm[1], x2 = xm[2];
+            s0 += a[0] * x0 + a[1] * x1 + a[2] * x2;
+            s1 += a[3] * x0 + a[4] * x1 + a[5] * x2;
+            s2 += a[6] * x0 + a[7] * x1 + a[8] * x2;
+        }
+    }
+#pragma unroll
+    for (int o = kNodeLanes / 2; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o);
+        s1 += __shfl_xor(s1, o);
+        s2 += __shfl_xor(s2, o);
+    }
+    double part = 0.0;
+    if (node < nn && k < 3) {
+        const uint32_t row = 3 * node + k;
+        const T sum = k == 0 ? s0 : (k == 1 ? s1 : s2);
+        y[row] = sum;
+        if (sent0) {
+            sent0[row] = sentinel_value<T>();
+            sent1[row] = sentinel_value<T>();
+        }
+        if (DOT) part = (double)sum * (double)x[row];
+    }
+    if (DOT) {
+        const double bs = block_sum(part);
+        if (threadIdx.x == 0) partials[blk] = bs;
     }
 }
 
@@ -1266,13 +1332,32 @@ static bool spmv_use_stream(const Matrix& A) {
     return A.n && (double)A.nnz / A.n <= 48.0;
 }
 
+static bool spmv_use_node(const Matrix& A) {
+    static const bool csr_only = [] {
+        const char* e = std::getenv("FM_SPMV");
+        return e && std::string(e) == "csr";
+    }();
+    return A.node_spmv && !csr_only;
+}
+
 uint32_t spmv_blocks(const Matrix& A) {
+    if (spmv_use_node(A)) return cdiv(A.n / 3, 256 / kNodeLanes);
     return spmv_use_stream(A) ? cdiv(A.n, kTR) : cdiv(A.n, 256 / spmv_lpr(A));
 }
 
 template <typename T>
 static void spmv_t(hipStream_t s, const Matrix& A, const T* val, const T* x, T* y, bool dot, double* partials,
                    T* s0, T* s1, const PcgState* st) {
+    if (spmv_use_node(A)) {
+        const uint32_t nn = A.n / 3, nb = cdiv(nn, 256 / kNodeLanes);
+        if (dot)
+            spmv_node_kernel<T, true><<<nb, 256, 0, s>>>(nn, A.row_ptr.p, A.nptr.p, A.ncol.p, val, x, y, partials, s0,
+                                                         s1, st);
+        else
+            spmv_node_kernel<T, false><<<nb, 256, 0, s>>>(nn, A.row_ptr.p, A.nptr.p, A.ncol.p, val, x, y, partials,
+                                                          s0, s1, st);
+        return;
+    }
     if (spmv_use_stream(A)) {
         const uint32_t nb = cdiv(A.n, kTR);
         if (dot)

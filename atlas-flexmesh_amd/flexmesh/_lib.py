@@ -338,9 +338,10 @@ class Solver:
         return dict(zip(["n", "nnz", "ne", "nn", "ncolors"], (int(v) for v in o)))
 
     def sweep_info(self):
-        o = np.zeros(9, dtype=np.uint64)
+        o = np.zeros(10, dtype=np.uint64)
         self._chk(self.L.fm_sweep_info(self.h, o), "fm_sweep_info")
         d = dict(zip(["chain", "levels_fwd", "levels_bwd", "chains", "bundles", "wave_steps", "lanes_per_node",
-                      "gj", "gk"], (int(v) for v in o)))
+                      "gj", "gk", "node_spmv"], (int(v) for v in o)))
         d["sweep"] = "chain" if d.pop("chain") else "level"
+        d["spmv"] = "node" if d.pop("node_spmv") else "csr"
         return d

@@ -42,6 +42,10 @@ def kernel_bytes(info, kind, v):
     i = 4
     nlow = (nnz - n) // 2  # strictly lower entries (structurally symmetric pattern)
     if kind == "spmv":
+        if info.get("spmv") == "node":
+            # values once, one column-node index per 3x3 block, 3 row pointers and a
+            # block pointer per node, x once, y once
+            return nnz * v + (nnz // 9) * i + (n // 3) * 4 * i + n * 2 * v
         # values + column indices once, row pointers, x once, y once (+ fused p.Ap reads x again, cached)
         return nnz * (v + i) + n * (i + 2 * v)
     chain = info.get("sweep") == "chain"
@@ -232,7 +236,7 @@ def main():
     fm.configs.load(S, cfg)
     info = S.info()
     sw = S.sweep_info()
-    info["sweep"] = sw["sweep"]
+    info["sweep"], info["spmv"] = sw["sweep"], sw["spmv"]
     log(f"set-up (pattern, colouring, schedule, upload) {time.perf_counter() - t:.1f} s: {info} sweeps {sw}")
     mixed = cfg["prec"] == fm.FM_MIXED
     prec = fm.FM_MIXED if mixed else fm.FM_FP64

@@ -190,10 +190,11 @@ int fm_profile_reset(fm_ctx* ctx);
 int fm_profile_get(fm_ctx* ctx, double* ms, uint64_t* launches);
 /* sizes for roofline bookkeeping: n (rows), nnz, n_elems, n_nodes, n_colors */
 int fm_info(fm_ctx* ctx, uint64_t* out5);
-/* schedule of the ILU(0) sweeps of the current matrix: kind (0 level-scheduled,
+/* kernels chosen for the current matrix: ILU(0) sweep kind (0 level-scheduled,
    1 chain bundles), forward levels, backward levels, chains, bundles, forward
-   wave-steps, lanes per node, bundle extent gj, bundle extent gk */
-int fm_sweep_info(fm_ctx* ctx, uint64_t* out9);
+   wave-steps, lanes per node, bundle extent gj, bundle extent gk, SpMV kind
+   (0 CSR, 1 node-blocked) */
+int fm_sweep_info(fm_ctx* ctx, uint64_t* out10);
 
 #ifdef __cplusplus
 }

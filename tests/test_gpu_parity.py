@@ -137,7 +137,9 @@ def test_spmv_ilu0_match_oracle(fm, S, O, which):
     else:
         rp, ci, va = elastic_matrix(O, fm)
     S.set_matrix(rp, ci, va)
-    assert S.sweep_info()["sweep"] == ("level" if which == "poisson" else "chain")
+    si = S.sweep_info()
+    assert si["sweep"] == ("level" if which == "poisson" else "chain")
+    assert si["spmv"] == ("csr" if which == "poisson" else "node")
     n = rp.size - 1
     x = np.random.default_rng(2).standard_normal(n)
     cs, vs, perm = sorted_csr(rp, ci, va)
