@@ -569,13 +569,155 @@ int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult*
     return FM_OK;
 }
 
+// FP32 values derived on the device from the FP64 ones
+int values_to_fp32(fm_ctx* ctx, Matrix& M) {
+    FM_HIP(M.val32.alloc(M.nnz));
+    launch_f64_to_f32(ctx->stream, (uint32_t)M.nnz, M.val64.p, 1.0, M.val32.p);
+    FM_TRY(check_launch(ctx, "f64_to_f32"));
+    M.val_valid[PREC32] = true;
+    return FM_OK;
+}
+
+// pcg_solve_ilu0 with FP64 vectors, SpMV and scalars, and the FP32 ILU(0)
+// factor of the FP32-rounded matrix as preconditioner: z = M32^{-1} (float)r,
+// r.z in FP64.  On an ill-conditioned matrix the FP32 application is a
+// slightly variable preconditioner (its relative error grows with the
+// condition number), so beta takes the flexible (Polak-Ribiere) form
+// r_new.(z_new - z_old) / r_old.z_old, which equals the standard one for a
+// fixed preconditioner.  On b64 -> x64 of the work set (x64: initial guess).
+int pcg_run_pc(fm_ctx* ctx, Matrix& A, double tol, int max_it, PcgResult* res) {
+    hipStream_t s = ctx->stream;
+    const uint32_t n = A.n;
+    if (!A.val_valid[PREC32]) FM_TRY(values_to_fp32(ctx, A));
+    FM_TRY(factorize(ctx, A, PREC32));
+    FM_TRY(pcg_vectors(ctx, PREC64, n));
+    FM_TRY(pcg_vectors(ctx, PREC32, n));
+    const uint32_t nbw = A.sched.q_bwd + 64;
+    // partials, then the r.z_old partials of the flexible beta
+    const size_t npart = std::max<size_t>({(size_t)spmv_blocks(A), (size_t)n / 256 + 1, (size_t)nbw});
+    FM_TRY(ensure_partials(ctx, npart + n / 256 + 1));
+    double* old_part = ctx->partials.p + npart;
+    double *b = ctx->b64.p, *x = ctx->x64.p, *r = ctx->r64.p, *p = ctx->p64.p, *Ap = ctx->Ap64.p, *y = ctx->y64.p;
+    float *r32 = ctx->r32.p, *z32 = ctx->z32.p, *y32 = ctx->y32.p;
+    const float* lu = A.lu32.p;
+    PcgState init{};
+    init.tol = tol;
+    init.max_it = max_it;
+    *ctx->h_st = init;
+    FM_HIP(hipMemcpyAsync(ctx->st.p, ctx->h_st, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    {
+        ProfScope ps(ctx, FM_K_SPMV);
+        launch_spmv(s, PREC64, A, x, Ap, false, nullptr, nullptr, nullptr, nullptr);
+    }
+    uint32_t np = launch_pcg_init_r(s, PREC64, n, b, Ap, r, ctx->partials.p);
+    launch_reduce(s, ctx->partials.p, np, ctx->st.p, 0, 0);
+    launch_fill_f64(s, Ap, n, 0.0);
+    np = launch_pcg_init_r(s, PREC64, n, b, Ap, y, ctx->partials.p);
+    launch_reduce(s, ctx->partials.p, np, ctx->st.p, 0, 1);
+    FM_HIP(hipMemcpyAsync(ctx->h_st, ctx->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    FM_TRY(sync(ctx));
+    const double rr = ctx->h_st->slot[0], bb = ctx->h_st->slot[1];
+    if (std::sqrt(rr) < tol * 1e-3) {
+        res->iters = 0;
+        res->relres = std::sqrt(rr);
+        res->status = 1;
+        return FM_OK;
+    }
+    ctx->h_st->bb = bb;
+    ctx->h_st->tol = tol;
+    ctx->h_st->max_it = max_it;
+    FM_HIP(hipMemcpyAsync(ctx->st.p, ctx->h_st, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    // y = M^{-1} r in FP32, cast back with the FP64 partials of r.y
+    auto precond = [&](bool check_done) -> uint32_t {
+        launch_f64_to_f32(s, n, r, 1.0, r32);
+        launch_fill_sentinel(s, PREC32, z32, y32, n);
+        {
+            ProfScope ps(ctx, FM_K_ILU_FWD);
+            if (A.chain.ok)
+                launch_chain_fwd(s, PREC32, A, r32, nullptr, nullptr, nullptr, nullptr, z32, ctx->st.p, false,
+                                 check_done, ctx->d_err.p);
+            else
+                launch_ilu_fwd(s, PREC32, A, lu, r32, nullptr, nullptr, nullptr, nullptr, z32, ctx->st.p, false,
+                               ctx->d_err.p);
+        }
+        {
+            ProfScope ps(ctx, FM_K_ILU_BWD);
+            if (A.chain.ok)
+                launch_chain_bwd(s, PREC32, A, z32, nullptr, y32, nullptr, ctx->st.p, check_done, ctx->d_err.p);
+            else
+                launch_ilu_bwd(s, PREC32, A, lu, z32, nullptr, y32, nullptr, ctx->st.p, check_done, ctx->d_err.p);
+        }
+        ProfScope pu(ctx, FM_K_PCG_OTHER);
+        return launch_f32_to_f64_dot(s, n, y32, r, y, ctx->partials.p, check_done ? old_part : nullptr,
+                                     check_done ? ctx->st.p : nullptr);
+    };
+    np = precond(false);
+    launch_reduce(s, ctx->partials.p, np, ctx->st.p, 3, 0);
+    launch_copy(s, PREC64, n, y, p);
+    FM_TRY(check_launch(ctx, "pcg init"));
+    int done = 0, enq = 0;
+    while (!done) {
+        for (int k = 0; k < kBatch && enq < max_it; ++k, ++enq) {
+            {
+                ProfScope ps(ctx, FM_K_SPMV);
+                launch_spmv(s, PREC64, A, p, Ap, true, ctx->partials.p, nullptr, nullptr, ctx->st.p);
+            }
+            launch_reduce(s, ctx->partials.p, spmv_blocks(A), ctx->st.p, 1, 0);
+            {
+                ProfScope pu(ctx, FM_K_PCG_OTHER);
+                launch_pcg_xr_update(s, PREC64, n, p, Ap, x, r, ctx->st.p);
+            }
+            np = precond(true);
+            launch_reduce(s, old_part, np, ctx->st.p, 0, 3);
+            launch_reduce(s, ctx->partials.p, np, ctx->st.p, 4, 3);
+            launch_pcg_p_update(s, PREC64, n, y, p, ctx->st.p);
+        }
+        FM_TRY(check_launch(ctx, "pcg iteration"));
+        FM_HIP(hipMemcpyAsync(ctx->h_st, ctx->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+        FM_TRY(sync(ctx));
+        done = ctx->h_st->done || enq >= max_it;
+    }
+    FM_TRY(check_device_error(ctx));
+    const PcgState& h = *ctx->h_st;
+    res->status = h.status;
+    res->relres = h.relres;
+    res->iters = (h.status == 1) ? h.iters : max_it;
+    return FM_OK;
+}
+
 // Solve K du = rhs (FP64 device vector) with the tangent; du FP64.
 int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double tol, int max_it, int* iters) {
     Matrix& A = ctx->tangent;
     ctx->cur = &A;
     const uint32_t n = A.n;
-    FM_TRY(pcg_vectors(ctx, prec, n));
     PcgResult pr{};
+    double nr = 1.0;
+    if (prec != PREC64) {
+        // the inner solves work on the normalised right-hand side: the
+        // reference's absolute early exit (||r|| < tol 1e-3) would otherwise
+        // end a loose forcing-term solve at once on a small load-step residual
+        FM_TRY(ensure_partials(ctx, n / 256 + 1));
+        const uint32_t np = launch_dot64(ctx->stream, n, rhs, rhs, ctx->partials.p);
+        launch_reduce(ctx->stream, ctx->partials.p, np, ctx->st.p, 0, 2);
+        FM_HIP(hipMemcpyAsync(ctx->h_st, ctx->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
+        FM_TRY(sync(ctx));
+        nr = std::sqrt(ctx->h_st->slot[2]);
+        if (nr == 0.0) {
+            launch_fill_f64(ctx->stream, du, n, 0.0);
+            *iters = 0;
+            return FM_OK;
+        }
+    }
+    if (prec == PRECPC) {
+        FM_TRY(pcg_vectors(ctx, PREC64, n));
+        launch_scale(ctx->stream, n, rhs, 1.0 / nr, ctx->b64.p);
+        launch_fill_f64(ctx->stream, ctx->x64.p, n, 0.0);
+        FM_TRY(pcg_run_pc(ctx, A, tol, max_it, &pr));
+        launch_scale(ctx->stream, n, ctx->x64.p, nr, du);
+        *iters = pr.iters;
+        return FM_OK;
+    }
+    FM_TRY(pcg_vectors(ctx, prec, n));
     if (prec == PREC64) {
         FM_HIP(hipMemcpyAsync(ctx->b64.p, rhs, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));
         launch_fill_f64(ctx->stream, ctx->x64.p, n, 0.0);
@@ -583,17 +725,6 @@ int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double to
         FM_HIP(hipMemcpyAsync(du, ctx->x64.p, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));
     } else {
         // FP32 inner solve on the normalised right-hand side
-        FM_TRY(ensure_partials(ctx, n / 256 + 1));
-        const uint32_t np = launch_dot64(ctx->stream, n, rhs, rhs, ctx->partials.p);
-        launch_reduce(ctx->stream, ctx->partials.p, np, ctx->st.p, 0, 2);
-        FM_HIP(hipMemcpyAsync(ctx->h_st, ctx->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
-        FM_TRY(sync(ctx));
-        const double nr = std::sqrt(ctx->h_st->slot[2]);
-        if (nr == 0.0) {
-            launch_fill_f64(ctx->stream, du, n, 0.0);
-            *iters = 0;
-            return FM_OK;
-        }
         launch_f64_to_f32(ctx->stream, n, rhs, 1.0 / nr, ctx->b32.p);
         launch_fill_f32(ctx->stream, ctx->x32.p, n, 0.f);
         FM_TRY(pcg_run(ctx, A, PREC32, tol, max_it, &pr));
@@ -923,12 +1054,7 @@ static int matrix_prec(fm_ctx* ctx, int prec, Matrix** A, int* p) {
         // derive the other precision from the one present
         Matrix& M = **A;
         if (*p == PREC32 && M.val_valid[0]) {
-            FM_HIP(M.val32.alloc(M.nnz));
-            std::vector<double> t(M.nnz);
-            FM_HIP(hipMemcpy(t.data(), M.val64.p, M.nnz * 8, hipMemcpyDeviceToHost));
-            std::vector<float> f(t.begin(), t.end());
-            FM_HIP(hipMemcpy(M.val32.p, f.data(), M.nnz * 4, hipMemcpyHostToDevice));
-            M.val_valid[1] = true;
+            FM_TRY(values_to_fp32(ctx, M));
         } else {
             return fail(ctx, FM_ESTATE, "matrix values not available in FP64 (assembled in mixed precision)");
         }
@@ -1029,6 +1155,23 @@ int fm_pcg(fm_ctx* ctx, int prec, const double* b, double* x, double tol, int ma
     if (!ctx || !b || !x || max_it < 0) return fail(ctx, FM_EINVAL, "fm_pcg: bad arguments");
     Matrix* A;
     int p;
+    if (prec == FM_MIXED_PC) {
+        if (!ctx->cur || !ctx->cur->ready) return fail(ctx, FM_ESTATE, "no matrix (fm_set_matrix / fm_use_tangent)");
+        A = ctx->cur;
+        if (!A->val_valid[PREC64]) return fail(ctx, FM_ESTATE, "FP64 matrix values needed for FM_MIXED_PC");
+        FM_TRY(values_to_fp32(ctx, *A));
+        A->lu_valid[PREC32] = false;
+        FM_TRY(pcg_vectors(ctx, PREC64, A->n));
+        const size_t n = A->n;
+        FM_HIP(hipMemcpy(ctx->b64.p, b, n * 8, hipMemcpyHostToDevice));
+        FM_HIP(hipMemcpy(ctx->x64.p, x, n * 8, hipMemcpyHostToDevice));
+        PcgResult pr{};
+        FM_TRY(pcg_run_pc(ctx, *A, tol, max_it, &pr));
+        FM_HIP(hipMemcpy(x, ctx->x64.p, n * 8, hipMemcpyDeviceToHost));
+        if (iters) *iters = pr.iters;
+        if (relres) *relres = pr.relres;
+        return FM_OK;
+    }
     FM_TRY(matrix_prec(ctx, prec, &A, &p));
     A->lu_valid[p] = false;  // pcg_solve_ilu0 factorizes on every call
     FM_TRY(pcg_vectors(ctx, p, A->n));
@@ -1110,7 +1253,8 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     else
         fm_newton_cfg_default(&cfg);
     if (!(load_factor >= 0.0) || !std::isfinite(load_factor)) return fail(ctx, FM_EINVAL, "bad load factor");
-    if (cfg.prec != FM_FP64 && cfg.prec != FM_MIXED) return fail(ctx, FM_EINVAL, "bad precision");
+    if (cfg.prec != FM_FP64 && cfg.prec != FM_MIXED && cfg.prec != FM_MIXED_PC)
+        return fail(ctx, FM_EINVAL, "bad precision");
     FM_TRY(ensure_ready(ctx));
     const double t0 = now_s();
     hipStream_t s = ctx->stream;
@@ -1123,7 +1267,9 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         launch_fill_f64(s, ctx->u.p, n, 0.0);
         ctx->u_valid = true;
     }
-    const int kprec = cfg.prec == FM_FP64 ? PREC64 : PREC32;
+    // linear-solve mode and the precision the tangent is assembled in
+    const int kprec = cfg.prec == FM_FP64 ? PREC64 : (cfg.prec == FM_MIXED ? PREC32 : PRECPC);
+    const int aprec = kprec == PREC32 ? PREC32 : PREC64;
     const double lf = load_factor;
     fm_newton_stats S{};
     const uint32_t ndir = (uint32_t)ctx->dir_dofs.size();
@@ -1135,7 +1281,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     FM_TRY(residual(ctx, ctx->ut.p, lf, &S.r0));
     S.residuals++;
     if (cfg.lift && any_inc) {
-        FM_TRY(assemble(ctx, ctx->u.p, true, kprec, true, ctx->dD.p, FM_K_ASSEMBLE));
+        FM_TRY(assemble(ctx, ctx->u.p, true, aprec, true, ctx->dD.p, FM_K_ASSEMBLE));
         S.assemblies++;
         FM_TRY(ensure_partials(ctx, n / 256 + 1));
         launch_residual_finish(s, n, lf, ctx->have_fext ? ctx->f_ext.p : nullptr, ctx->f.p, ctx->isdir.p, ctx->R.p,
@@ -1157,7 +1303,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     const double target = cfg.newton_rtol * S.r0;
     int it = 0;
     while (!(rn <= target) && it < cfg.max_newton) {
-        FM_TRY(assemble(ctx, ctx->u.p, true, kprec, true, nullptr, FM_K_ASSEMBLE));
+        FM_TRY(assemble(ctx, ctx->u.p, true, aprec, true, nullptr, FM_K_ASSEMBLE));
         S.assemblies++;
         int its = 0;
         const double tol = kprec == PREC64 ? cfg.pcg_tol : cfg.inner_tol;

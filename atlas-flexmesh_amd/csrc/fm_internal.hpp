@@ -113,7 +113,7 @@ struct Matrix {
     bool ready = false;
 };
 
-enum { PREC64 = 0, PREC32 = 1 };
+enum { PREC64 = 0, PREC32 = 1, PRECPC = 2 };  // PRECPC: FP64 Krylov, FP32 ILU(0) preconditioner
 
 struct Ctx {
     int device = 0;
@@ -229,7 +229,8 @@ uint32_t launch_residual_finish(hipStream_t s, uint32_t n, double load, const do
                                 const uint8_t* isdir, double* R, double* partials);
 // partials of a.b; returns #partials
 uint32_t launch_dot64(hipStream_t s, uint32_t n, const double* a, const double* b, double* partials);
-// reduce partials into st: op 0 = store in slot[k]; 1 = pAp/alpha; 2 = rz_new/convergence; 3 = init rz
+// reduce partials into st: op 0 = store in slot[k]; 1 = pAp/alpha; 2 = rz_new/convergence; 3 = init rz;
+// 4 = as 2 with the flexible beta (the r.z_old term stored in slot[k] by op 0)
 void launch_reduce(hipStream_t s, const double* partials, uint32_t np, PcgState* st, int op, int k);
 void launch_spmv(hipStream_t s, int prec, const Matrix& A, const void* x, void* y, bool dot, double* partials,
                  void* sent0, void* sent1, const PcgState* st);
@@ -251,6 +252,10 @@ uint32_t launch_chain_bwd(hipStream_t s, int prec, const Matrix& A, const void* 
                           double* partials, PcgState* st, bool check_done, int* err);
 void launch_fill_sentinel(hipStream_t s, int prec, void* a, void* b, size_t n);
 void launch_pcg_p_update(hipStream_t s, int prec, uint32_t n, const void* y, void* p, const PcgState* st);
+// dst = (double)src and block partials of r.dst (and of r.dst_before when
+// old_partials != null); returns #partials
+uint32_t launch_f32_to_f64_dot(hipStream_t s, uint32_t n, const float* src, const double* r, double* dst,
+                               double* partials, double* old_partials, const PcgState* st);
 void launch_pcg_xr_update(hipStream_t s, int prec, uint32_t n, const void* p, const void* Ap, void* x, void* r,
                           const PcgState* st);
 // r = b - Ap; x untouched; partials of r.r
@@ -262,6 +267,8 @@ void launch_f64_to_f32(hipStream_t s, uint32_t n, const double* src, double scal
 // dst64 = (double)src32 * scale
 void launch_f32_to_f64(hipStream_t s, uint32_t n, const float* src, double scale, double* dst);
 // y = a + alpha * b (FP64)
+// y = alpha a
+void launch_scale(hipStream_t s, uint32_t n, const double* a, double alpha, double* y);
 void launch_axpy_to(hipStream_t s, uint32_t n, const double* a, double alpha, const double* b, double* y);
 // u[dofs[k]] = scale * vals[k] (values), and dD[dofs[k]] = scale*vals[k] - u_old[dofs[k]]
 void launch_dir_increment(hipStream_t s, uint32_t n_dir, const uint32_t* dofs, const double* vals, double scale,

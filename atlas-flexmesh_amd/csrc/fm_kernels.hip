@@ -878,6 +878,24 @@ __global__ __launch_bounds__(1024) void reduce_kernel(const double* __restrict__
             st->rz = s;
             st->relres = sqrt(s / fmax(st->bb, 1e-300));
             break;
+        case 4: {  // as 2, with the flexible beta r_new.(z_new - z_old) / r_old.z_old (z_old term in slot[k])
+            if (st->done) return;
+            st->rz_new = s;
+            st->iters += 1;
+            const double rel = sqrt(s / fmax(st->bb, 1e-300));
+            st->relres = rel;
+            if (rel < st->tol) {
+                st->done = 1;
+                st->status = 1;
+            } else if (st->iters >= st->max_it) {
+                st->done = 1;
+                st->status = 3;
+            } else {
+                st->beta = (s - st->slot[k]) / st->rz;
+                st->rz = s;
+            }
+            break;
+        }
     }
 }
 
@@ -987,6 +1005,37 @@ __global__ void f64_to_f32_kernel(uint32_t n, const double* __restrict__ s, doub
 __global__ void f32_to_f64_kernel(uint32_t n, const float* __restrict__ s, double scale, double* __restrict__ d) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) d[i] = (double)s[i] * scale;
+}
+
+// d = (double)s with block partials of r.d (new d) and, when old != null,
+// of r.d with the d it overwrites (the FP64 r.z of a PCG whose
+// preconditioner runs in FP32, and the r.z_old of its flexible beta)
+__global__ __launch_bounds__(256) void f32_to_f64_dot_kernel(uint32_t n, const float* __restrict__ s,
+                                                             const double* __restrict__ r, double* __restrict__ d,
+                                                             double* __restrict__ partials,
+                                                             double* __restrict__ old_partials,
+                                                             const PcgState* __restrict__ st) {
+    if (st && st->done) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double part = 0.0, part_old = 0.0;
+    if (i < n) {
+        const double v = (double)s[i];
+        if (old_partials) part_old = r[i] * d[i];
+        d[i] = v;
+        part = r[i] * v;
+    }
+    const double bs = block_sum(part);
+    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+    if (old_partials) {
+        __syncthreads();  // block_sum's shared scratch is reused
+        const double bo = block_sum(part_old);
+        if (threadIdx.x == 0) old_partials[blockIdx.x] = bo;
+    }
+}
+
+__global__ void scale_kernel(uint32_t n, const double* __restrict__ a, double alpha, double* __restrict__ y) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = alpha * a[i];
 }
 
 __global__ void axpy_to_kernel(uint32_t n, const double* __restrict__ a, double alpha, const double* __restrict__ b,
@@ -1198,7 +1247,7 @@ uint32_t env_waves(const char* name, uint32_t dflt) {
     const long v = e ? std::atol(e) : 0;
     return (uint32_t)(v > 0 ? v : dflt);
 }
-hipError_t g_launch_err = hipSuccess;
+thread_local hipError_t g_launch_err = hipSuccess;  // per calling thread, like its contexts
 
 // Launch of a persistent kernel: at most `want` one-wave blocks, clamped to
 // half of what the device keeps resident at once, so every block is running
@@ -1561,8 +1610,17 @@ void launch_copy(hipStream_t s, int prec, uint32_t n, const void* src, void* dst
 void launch_f64_to_f32(hipStream_t s, uint32_t n, const double* src, double scale, float* dst) {
     f64_to_f32_kernel<<<cdiv(n, 256), 256, 0, s>>>(n, src, scale, dst);
 }
+uint32_t launch_f32_to_f64_dot(hipStream_t s, uint32_t n, const float* src, const double* r, double* dst,
+                               double* partials, double* old_partials, const PcgState* st) {
+    const uint32_t nb = cdiv(n, 256);
+    f32_to_f64_dot_kernel<<<nb, 256, 0, s>>>(n, src, r, dst, partials, old_partials, st);
+    return nb;
+}
 void launch_f32_to_f64(hipStream_t s, uint32_t n, const float* src, double scale, double* dst) {
     f32_to_f64_kernel<<<cdiv(n, 256), 256, 0, s>>>(n, src, scale, dst);
+}
+void launch_scale(hipStream_t s, uint32_t n, const double* a, double alpha, double* y) {
+    scale_kernel<<<cdiv(n, 256), 256, 0, s>>>(n, a, alpha, y);
 }
 void launch_axpy_to(hipStream_t s, uint32_t n, const double* a, double alpha, const double* b, double* y) {
     axpy_to_kernel<<<cdiv(n, 256), 256, 0, s>>>(n, a, alpha, b, y);

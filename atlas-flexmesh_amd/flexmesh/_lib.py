@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("FM_LIB_PATH") or os.path.join(HERE, "libflexmesh.so")
 
 FM_OK, FM_EINVAL, FM_ESTATE, FM_EHIP, FM_ENOCONV, FM_EDEVICE = 0, -1, -2, -3, -4, -5
-FM_FP64, FM_MIXED = 0, 1
+FM_FP64, FM_MIXED, FM_MIXED_PC = 0, 1, 2
 KERNEL_CLASSES = ["spmv", "ilu_fwd", "ilu_bwd", "ilu_factor", "assemble", "residual", "pcg_other"]
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")

@@ -99,8 +99,11 @@ def make(k, dims=None, L_box=None):
         cfg["traction"] = 0.0036
         cfg["f_ext"] = face_traction(X, conn, on, np.array([0.0, 0.0, -cfg["traction"]]))
         # the slender beam's bending modes hold ILU(0)-PCG on a plateau of
-        # ~4 n_x iterations before it converges, at any tolerance
-        cfg.update(newton_rtol=1e-8, pcg_tol=1e-10, prec=L.FM_MIXED, ls_mode=1, max_newton=60,
+        # ~4 n_x iterations before it converges, at any tolerance; its tangent
+        # (condition number ~1e6-1e7) is too ill-conditioned for FP32 search
+        # directions, so the Krylov iteration runs in FP64 with the FP32
+        # ILU(0) factor as preconditioner
+        cfg.update(newton_rtol=1e-8, pcg_tol=1e-10, prec=L.FM_MIXED_PC, ls_mode=1, max_newton=60,
                    pcg_max=8 * nx + 400, inner_max=8 * nx + 400)
     else:
         raise ValueError(k)

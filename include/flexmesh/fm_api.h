@@ -32,9 +32,13 @@ extern "C" {
 #define FM_EDEVICE (-5)     /* a device-side bound (spin limit) was hit */
 
 /* precision of the linear algebra: FM_FP64 = plain FP64 drop-in path;
-   FM_MIXED = FP32 assembly/SpMV/ILU(0)/PCG under an FP64 Newton residual */
+   FM_MIXED = FP32 assembly/SpMV/ILU(0)/PCG under an FP64 Newton residual;
+   FM_MIXED_PC = FP64 tangent, SpMV and PCG vectors with the FP32 ILU(0)
+   factor of the FP32-rounded tangent as preconditioner (for tangents whose
+   condition number makes FP32 directions inaccurate: slender bodies) */
 #define FM_FP64 0
 #define FM_MIXED 1
+#define FM_MIXED_PC 2
 
 typedef struct fm_ctx fm_ctx;
 

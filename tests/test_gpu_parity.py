@@ -217,6 +217,33 @@ def test_pcg_elastic_matches_oracle(fm, S, O):
     assert rel(x32, xo) < 1e-3
 
 
+@pytest.mark.parametrize("dims", [(6, 5, 5), (20, 3, 3)])
+def test_pcg_fp32_preconditioner_matches_oracle(fm, S, O, dims):
+    # FP64 Krylov with the FP32 factor: the solution is the FP64 one.  The
+    # iteration count follows the FP32 factor, which on an ill-conditioned
+    # tangent (the slender 20x3x3 beam) is a weaker preconditioner than the
+    # FP64 one, so it is only bounded here
+    rp, ci, va = elastic_matrix(O, fm, dims=dims)
+    b = np.random.default_rng(9).standard_normal(rp.size - 1)
+    S.set_matrix(rp, ci, va)
+    xo, itso, _ = O.pcg(rp, ci, va, b, tol=1e-10, max_iter=2000)
+    x, its, relres = S.pcg(b, tol=1e-10, max_it=2000, prec=fm.FM_MIXED_PC)
+    print(f"{dims}: FP32-preconditioned PCG {its} iterations, FP64 oracle {itso}")
+    assert relres < 1e-10 and its <= 3 * itso
+    assert rel(x, xo) < 1e-8
+
+
+def test_newton_fp32_preconditioner_matches_fp64_oracle(fm, S, O):
+    cfg = small_problem(fm, k=3, dims=(6, 6, 6))
+    fm.configs.load(S, cfg)
+    uo, sto = run_oracle(O, cfg, range(1, 3))
+    c = fm.configs.newton_cfg(cfg, prec=fm.FM_MIXED_PC)
+    for s in range(1, 3):
+        rc, st = S.newton_step(s, c)
+        assert st["converged"]
+    assert rel(S.get_u(), uo) < 1e-7
+
+
 def test_pcg_edge_cases(fm, S):
     rp = np.array([0, 1, 2, 3], np.uint32)
     ci = np.array([0, 1, 2], np.uint32)
@@ -298,9 +325,10 @@ def test_newton_traction_beam_fp64(fm, S, O, ls_mode):
 
 
 def test_newton_traction_beam_mixed_energy_rule(fm, S, O):
-    # the beam's own setting: mixed precision with the energy rule
+    # the beam's own setting: FP64 Krylov with the FP32 ILU(0) preconditioner
+    # and the energy rule
     cfg = fm.configs.make(5, dims=(32, 4, 4))
-    assert cfg["ls_mode"] == 1 and cfg["prec"] == fm.FM_MIXED
+    assert cfg["ls_mode"] == 1 and cfg["prec"] == fm.FM_MIXED_PC
     fm.configs.load(S, cfg)
     c = fm.configs.newton_cfg(cfg)
     for s in range(1, 3):

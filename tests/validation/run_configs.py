@@ -198,8 +198,11 @@ def run_cfg5(args):
     tip = np.where(gi == nx)[0]
     S = fm.Solver(0)
     fm.configs.load(S, cfg)
-    prec = fm.FM_MIXED if args.prec == "mixed" else fm.FM_FP64
+    prec = {"mixed": fm.FM_MIXED, "fp64": fm.FM_FP64, "mixed_pc": fm.FM_MIXED_PC}[args.prec] if args.prec \
+        else cfg["prec"]
     c = fm.configs.newton_cfg(cfg, prec=prec, max_newton=60)
+    if args.inner_tol is not None:
+        c.inner_tol = args.inner_tol
     n = min(args.steps, cfg["n_steps"])
     ck = os.path.join(args.out, "cfg5_prev.ckpt")
     t = time.perf_counter()
@@ -218,7 +221,7 @@ def run_cfg5(args):
     ang = np.degrees(np.linalg.norm(w[tip_elems], axis=1))
     defl = float(u.reshape(nn, 3)[tip, 2].mean())
     out = {"run": "cfg5", "dims": cfg["dims"], "dofs": 3 * nn, "steps": n, "load_factor": n / cfg["n_steps"],
-           "traction": cfg["traction"], "prec": args.prec, "seconds": wall, "newton_pcg_relres": hist,
+           "traction": cfg["traction"], "prec": int(prec), "seconds": wall, "newton_pcg_relres": hist,
            "tip_rotation_deg_mean": float(ang.mean()), "tip_rotation_deg_max": float(ang.max()),
            "tip_deflection_z": defl}
     if args.adjoint:
@@ -279,7 +282,9 @@ def main():
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--oracle", action="store_true")
     ap.add_argument("--adjoint", action="store_true")
-    ap.add_argument("--prec", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--prec", choices=["mixed", "fp64", "mixed_pc"], default=None,
+                    help="precision of the linear solves (default: the configuration's)")
+    ap.add_argument("--inner-tol", type=float, default=None, help="FP32 PCG forcing term of the mixed path")
     ap.add_argument("--out", default=os.path.join(ROOT, "gpurun_out"))
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
