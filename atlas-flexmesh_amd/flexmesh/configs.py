@@ -96,7 +96,7 @@ def make(k, dims=None, L_box=None):
         on = np.zeros(nn, dtype=bool)
         on[x1] = True
         # transverse (-z) dead traction; magnitude gives > 60 degrees of tip rotation
-        cfg["traction"] = 0.0036
+        cfg["traction"] = 0.0048
         cfg["f_ext"] = face_traction(X, conn, on, np.array([0.0, 0.0, -cfg["traction"]]))
         # the slender beam's bending modes hold ILU(0)-PCG on a plateau of
         # ~4 n_x iterations before it converges, at any tolerance; its tangent
