@@ -4,7 +4,9 @@ prints one JSON object per run.
 
   python tests/validation/run_configs.py cfg1
   python tests/validation/run_configs.py cfg2 --steps 2
-  python tests/validation/run_configs.py cfg4 --dims 48 --fd-dirs 8
+  python tests/validation/run_configs.py cfg2 --steps 2 --save-gpu u.npz     (GPU half)
+  python tests/validation/run_configs.py cfg2 --steps 2 --gpu-result u.npz   (oracle half, CPU)
+  python tests/validation/run_configs.py cfg4 --dims 48 --fd-dirs 64 --out /tmp
   python tests/validation/run_configs.py cfg4 --dims 12 --fd-dirs 64 --oracle
   python tests/validation/run_configs.py cfg5 --steps 40
 """
@@ -74,14 +76,26 @@ def run_cfg1(args):
 def run_cfg2(args):
     cfg = fm.configs.make(2, dims=(args.dims,) * 3 if args.dims else None)
     steps = range(1, args.steps + 1)
-    S = fm.Solver(0)
-    fm.configs.load(S, cfg)
-    c = fm.configs.newton_cfg(cfg)
-    t = time.perf_counter()
-    gst = [S.newton_step(s, c)[1] for s in steps]
-    t_gpu = time.perf_counter() - t
-    u = S.get_u()
-    S.close()
+    if args.gpu_result:
+        # the GPU half ran elsewhere (--save-gpu): compare its u here
+        z = np.load(args.gpu_result)
+        u, t_gpu = z["u"], float(z["seconds"])
+        gst = [dict(newton_iters=int(a), pcg_iters=int(b)) for a, b in z["newton_pcg"]]
+    else:
+        S = fm.Solver(0)
+        fm.configs.load(S, cfg)
+        c = fm.configs.newton_cfg(cfg)
+        t = time.perf_counter()
+        gst = [S.newton_step(s, c)[1] for s in steps]
+        t_gpu = time.perf_counter() - t
+        u = S.get_u()
+        S.close()
+        if args.save_gpu:
+            np.savez(args.save_gpu, u=u, seconds=t_gpu,
+                     newton_pcg=np.array([(s["newton_iters"], s["pcg_iters"]) for s in gst]))
+            print(json.dumps({"run": "cfg2-gpu", "dofs": u.size, "steps": args.steps, "gpu_seconds": t_gpu,
+                              "gpu_newton_pcg": [(s["newton_iters"], s["pcg_iters"]) for s in gst]}), flush=True)
+            return
     O.set_threads(min(16, os.cpu_count() or 1))
     P = oracle_problem(cfg, newton_rtol=cfg["newton_rtol"], pcg_tol=cfg["pcg_tol"])
     uo = np.zeros_like(u)
@@ -141,12 +155,15 @@ def run_cfg4(args):
     dmu, dlam, dload, ad_its = S.adjoint(g, tol=1e-13, max_it=20000)
     t_adj = time.perf_counter() - t
 
+    fd_rel = []
+
     def loss_at(mu, lam, scale=1.0):
         S.set_material(mu, lam)
         if scale != 1.0:
             S.set_bcs(cfg["dir_dofs"], scale * cfg["dir_vals"])
         S.checkpoint_load(ck)
-        S.newton_step(cfg["n_steps"], c)
+        _, st = S.newton_step(cfg["n_steps"], c, check=False)
+        fd_rel.append(st["rel_res"])
         if scale != 1.0:
             S.set_bcs(cfg["dir_dofs"], cfg["dir_vals"])
         return loss_and_grad(S.get_u(), ustar, top, nn)[0]
@@ -174,7 +191,7 @@ def run_cfg4(args):
            "fd_dirs": args.fd_dirs, "fd_max_rel_err": max(x[2] for x in fd) if fd else None,
            "fd_median_rel_err": float(np.median([x[2] for x in fd])) if fd else None,
            "dload_adjoint": dload, "dload_fd": fd_load, "dload_rel_err": abs(fd_load - dload) / max(abs(dload), 1e-300),
-           "fd_seconds": t_fd}
+           "fd_seconds": t_fd, "fd_solves_max_rel_res": max(fd_rel) if fd_rel else None}
     if args.oracle:
         O.set_threads(min(16, os.cpu_count() or 1))
         P = oracle_problem(cfg, newton_rtol=1e-13, pcg_tol=1e-13, pcg_max=20000, max_newton=40)
@@ -289,6 +306,8 @@ def main():
                     help="precision of the linear solves (default: the configuration's)")
     ap.add_argument("--inner-tol", type=float, default=None, help="FP32 PCG forcing term of the mixed path")
     ap.add_argument("--out", default=os.path.join(ROOT, "gpurun_out"))
+    ap.add_argument("--save-gpu", default=None, help="cfg2: run the GPU half only and save u here (.npz)")
+    ap.add_argument("--gpu-result", default=None, help="cfg2: compare a saved GPU result with the oracle")
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
     if args.run == "cfg4" and args.dims is None:
