@@ -10,6 +10,9 @@ namespace fm {
 
 constexpr int kChainMaxBlk = 8;               // dependency node blocks per node and direction
 constexpr int kChainRing = 8;                 // LDS ring depth in steps
+// largest published vector (bytes) the sweep kernels address: their buffer
+// offsets are 32-bit, and offsets from 2^31 up mark lanes with no access
+constexpr uint32_t kChainMaxBytes = 0x80000000u;
 constexpr uint32_t kDepNone = 0xFFFFFFFFu;    // no dependency in this lane / idle lane group
 constexpr uint32_t kDepLocal = 0x80000000u;   // | LDS slot: dependency inside the bundle
 

@@ -434,6 +434,12 @@ bool plan_chain_schedule(const std::vector<uint32_t>& rp, const std::vector<uint
 bool build_chain_schedule(const std::vector<uint32_t>& rp, const std::vector<uint32_t>& ci, int lpn,
                           ChainSchedule& C, hipStream_t stream, std::string& why, std::string& err) {
     C.ok = false;
+    // the sweeps address the published vector with 32-bit buffer offsets
+    const size_t nn = rp.empty() ? 0 : (rp.size() - 1) / 3;
+    if (nn * 3 * sizeof(double) >= (size_t)kChainMaxBytes) {
+        why = "vector too large for 32-bit buffer offsets";
+        return false;
+    }
     ChainPlan P;
     if (!plan_chain_schedule(rp, ci, lpn, P, why)) return false;
     const uint32_t nbund = P.fwd.nb;

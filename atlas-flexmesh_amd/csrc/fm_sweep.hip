@@ -26,9 +26,11 @@
 // The step body has no branch around a global load or store: the vector
 // memory counter is in-order, and where a load or store may be skipped the
 // compiler has to wait for every outstanding load, which would drain the
-// prefetch each step.  Loads therefore always issue (clamped steps, dummy
-// addresses for absent dependencies), and lanes with nothing to store write
-// to their own sink slot and a spare LDS slot.  The only branch left is the
+// prefetch each step.  Loads therefore always issue (clamped steps), and the
+// polls and publications of the result vector use buffer instructions whose
+// offset is moved past the buffer for lanes with nothing to do: such a load
+// returns zeros and such a store is dropped, without a memory access.  Lanes
+// with nothing to store write a spare LDS slot.  The only branch left is the
 // rare, wave-uniform re-poll of a dependency that was not yet published when
 // it was polled early.
 #include "fm_device.hpp"
@@ -124,21 +126,69 @@ struct SweepArgs {
     // MODE 2 (backward): in[0] = z, in[1] = r (for the r.y partials)
     const T* in[4];
     T* out;   // published result: z (forward) or y (backward)
+    uint32_t out_bytes;  // its size (< kChainMaxBytes)
     T* r;     // MODE 1: r <- r - alpha Ap (written by quad lane 0)
     T* x;     // MODE 1: x <- x + alpha p (written by quad lane 3)
-    T* sink;  // 4 values per lane and block: stores and dummy polls of lanes with nothing to do
+    T* sink;  // 4 values per lane and block: MODE 1 stores of lanes with nothing to do
     double* partials;
     PcgState* st;
     int check_done;
     int* err;
 };
 
+// Polls and publications go through a raw buffer resource over `out`.  A
+// lane whose offset is kOob (past the buffer) loads zeros and its store is
+// dropped, without a memory access: every lane issues the same instructions
+// (no branch, so no waits at joins) and idle lanes add no traffic.  The
+// cache policy is that of agent-scope atomics (sc1): the value is read from
+// and written to the device-coherent level, never a stale CU or XCD copy.
+constexpr uint32_t kOob = kChainMaxBytes;
+constexpr int kSc1 = 16;               // buffer instruction cache-policy bits: sc1
+constexpr int kRsrcWord3 = 0x00020000;  // raw buffer, 32-bit elements, no swizzle
+
 template <typename T>
-__device__ __forceinline__ T poll_once(const T* p) {
-    using U = typename SB<T>::U;
-    U* q = const_cast<U*>(reinterpret_cast<const U*>(p));
-    return __builtin_bit_cast(T, __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t out_rsrc(const SweepArgs<T>& A) {
+    return __builtin_amdgcn_make_buffer_rsrc(A.out, (short)0, (int)A.out_bytes, kRsrcWord3);
 }
+
+// LLVM's raw buffer intrinsics, declared directly: the clang builtins for
+// loads wider than one dword return the first dword splatted in this
+// toolchain
+typedef float fm_f3 __attribute__((ext_vector_type(3)));
+typedef double fm_d2 __attribute__((ext_vector_type(2)));
+__device__ fm_f3 buf_load_f3(__amdgpu_buffer_rsrc_t, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v3f32");
+__device__ fm_d2 buf_load_d2(__amdgpu_buffer_rsrc_t, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f64");
+__device__ double buf_load_d1(__amdgpu_buffer_rsrc_t, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.f64");
+__device__ void buf_store_u32(uint32_t, __amdgpu_buffer_rsrc_t, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.store.i32");
+__device__ void buf_store_u64(unsigned long long, __amdgpu_buffer_rsrc_t, int, int,
+                              int) __asm("llvm.amdgcn.raw.ptr.buffer.store.i64");
+
+// the three components published for one node (byte offset `off`)
+__device__ __forceinline__ void poll3(__amdgpu_buffer_rsrc_t r, uint32_t off, float (&v)[3]) {
+    const fm_f3 w = buf_load_f3(r, (int)off, 0, kSc1);
+    v[0] = w.x;
+    v[1] = w.y;
+    v[2] = w.z;
+}
+__device__ __forceinline__ void poll3(__amdgpu_buffer_rsrc_t r, uint32_t off, double (&v)[3]) {
+    const fm_d2 a = buf_load_d2(r, (int)off, 0, kSc1);
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = buf_load_d1(r, (int)(off + 16), 0, kSc1);
+}
+
+// ready-by-value publication (the all-ones pattern is reserved for "not ready")
+__device__ __forceinline__ void publish_at(__amdgpu_buffer_rsrc_t r, uint32_t off, float v) {
+    uint32_t b = __builtin_bit_cast(uint32_t, v);
+    if (b == SB<float>::S) b = SB<float>::Q;
+    buf_store_u32(b, r, (int)off, 0, kSc1);
+}
+__device__ __forceinline__ void publish_at(__amdgpu_buffer_rsrc_t r, uint32_t off, double v) {
+    unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    if (b == SB<double>::S) b = SB<double>::Q;
+    buf_store_u64(b, r, (int)off, 0, kSc1);
+}
+
 template <typename T>
 __device__ __forceinline__ bool is_sentinel(T v) {
     return __builtin_bit_cast(typename SB<T>::U, v) == SB<T>::S;
@@ -157,9 +207,9 @@ __device__ __forceinline__ T* sink_of(const SweepArgs<T>& A, int lane) {
     return A.sink + ((size_t)blockIdx.x * 64 + lane) * 4;
 }
 
-// every load issues: absent dependencies poll the lane's own sink slot (each
-// block has its own, so no line is shared between CUs), idle quads read their
-// zero payload and the first elements of the input vectors
+// every load issues: absent dependencies poll past the buffer (no access),
+// idle quads read their zero payload and the first elements of the input
+// vectors
 template <typename T, int MODE>
 __device__ __forceinline__ void near_load(const SweepArgs<T>& A, uint32_t q, int lane, const Far& F, Near<T>& N) {
     using S = Shape<T>;
@@ -170,12 +220,11 @@ __device__ __forceinline__ void near_load(const SweepArgs<T>& A, uint32_t q, int
     const V* src = reinterpret_cast<const V*>(A.pk) + (size_t)q * S::NC * 64 + lane;
 #pragma unroll
     for (int c = 0; c < S::NC; ++c) unpack(src[c * 64], &N.v[c * S::W]);
+    const __amdgpu_buffer_rsrc_t ro = out_rsrc(A);
 #pragma unroll
     for (int j = 0; j < kBPL; ++j) {
-        const uint32_t dw = F.dw[j];
-        const T* e = (dw & kDepLocal) ? sink_of(A, lane) : A.out + 3 * (size_t)dw;  // kDepNone has the bit set
-#pragma unroll
-        for (int c = 0; c < 3; ++c) N.ext[j][c] = poll_once(e + c);
+        const uint32_t dw = F.dw[j];  // kDepNone has the local bit set
+        poll3(ro, (dw & kDepLocal) ? kOob : dw * (3 * (uint32_t)sizeof(T)), N.ext[j]);
     }
     const int gl = lane & (kLPN - 1);
     const int which = MODE == 1 ? gl : (MODE == 2 && gl == 1 ? 1 : 0);
@@ -192,18 +241,33 @@ __device__ __forceinline__ void await_late(const SweepArgs<T>& A, const Near<T>&
     const bool act = N.nd != kDepNone;
     uint32_t spins = 0;
     for (;;) {
-        bool pending = false;
+        // the re-polls of both dependencies issue unconditionally (lanes
+        // with nothing pending load past the buffer) and are waited for
+        // together: one round trip per attempt, where loads under a
+        // per-value branch would each end in a full wait
+        __asm__ volatile("" ::: "memory");  // every attempt loads again
+        T z[kBPL][3];
+        bool need[kBPL][3];
 #pragma unroll
         for (int j = 0; j < kBPL; ++j) {
             const uint32_t dw = N.dw[j];
             const bool ext = act && dw != kDepNone && !(dw & kDepLocal);
+            bool any = false;
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-                if (ext && is_sentinel(y[j][c])) {
-                    y[j][c] = poll_once(A.out + 3 * (size_t)dw + c);
-                    pending |= is_sentinel(y[j][c]);
-                }
+            for (int c = 0; c < 3; ++c) {
+                need[j][c] = ext && is_sentinel(y[j][c]);
+                any |= need[j][c];
+            }
+            poll3(out_rsrc(A), any ? dw * (3 * (uint32_t)sizeof(T)) : kOob, z[j]);
         }
+        bool pending = false;
+#pragma unroll
+        for (int j = 0; j < kBPL; ++j)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                y[j][c] = need[j][c] ? z[j][c] : y[j][c];
+                pending |= need[j][c] && is_sentinel(z[j][c]);
+            }
         if (!__any(pending)) return;
         if (++spins > kSpinLimit) {
             if (pending) atomicOr(A.err, 1);
@@ -283,14 +347,14 @@ __device__ __forceinline__ void process(const SweepArgs<T>& A, const Near<T>& N,
     }
     const bool wr = act && gl < 3;
     const T vd = gl == 0 ? v0 : (gl == 1 ? v1 : v2);
-    T* const sk = sink_of(A, lane);
-    publish(wr ? A.out + 3 * (size_t)N.nd + gl : sk, vd);
+    publish_at(out_rsrc(A), wr ? (3 * N.nd + gl) * (uint32_t)sizeof(T) : kOob, vd);
     ring[wr ? ((s % kChainRing) * kNPW + g) * kRingStride + gl : kChainRing * kNPW * kRingStride + lane] = vd;
     if (MODE == 1) {
         // lane 0 writes r - alpha Ap, lane 3 writes x + alpha p (p from lane 2)
         T pv[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) pv[d] = quad<kBc2>(N.vec[d]);
+        T* const sk = sink_of(A, lane);
         T* dst = !act ? sk : (gl == 0 ? A.r + 3 * (size_t)N.nd : (gl == 3 ? A.x + 3 * (size_t)N.nd : sk));
 #pragma unroll
         for (int d = 0; d < 3; ++d) dst[d] = gl == 0 ? b[d] : N.vec[d] + alpha * pv[d];
@@ -501,6 +565,7 @@ static void chain_fwd_t(hipStream_t s, const Matrix& A, const void* rhs, void* r
         a.in[0] = (const T*)rhs;
     }
     a.out = (T*)z;
+    a.out_bytes = (uint32_t)((size_t)A.n * sizeof(T));
     a.r = (T*)r;
     a.x = (T*)x;
     a.st = st;
@@ -529,6 +594,7 @@ static uint32_t chain_bwd_t(hipStream_t s, const Matrix& A, const void* z, const
     a.in[0] = (const T*)z;
     a.in[1] = partials ? (const T*)r : nullptr;
     a.out = (T*)y;
+    a.out_bytes = (uint32_t)((size_t)A.n * sizeof(T));
     a.partials = partials;
     a.st = st;
     a.check_done = check_done ? 1 : 0;
