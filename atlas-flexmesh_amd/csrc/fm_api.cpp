@@ -447,8 +447,7 @@ int precond_apply(fm_ctx* ctx, Matrix& A, int prec, const void* r, void* z, void
     {
         ProfScope ps(ctx, FM_K_ILU_FWD);
         if (A.chain.ok)
-            launch_chain_fwd(ctx->stream, prec, A, r, nullptr, nullptr, nullptr, nullptr, z, ctx->st.p, false, false,
-                             ctx->d_err.p);
+            launch_chain_fwd(ctx->stream, prec, A, r, z, ctx->st.p, false, ctx->d_err.p);
         else
             launch_ilu_fwd(ctx->stream, prec, A, lu, r, nullptr, nullptr, nullptr, nullptr, z, ctx->st.p, false,
                            ctx->d_err.p);
@@ -539,8 +538,7 @@ int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult*
                         launch_pcg_xr_update(s, prec, n, p, Ap, x, r, ctx->st.p);
                     }
                     ProfScope ps(ctx, FM_K_ILU_FWD);
-                    launch_chain_fwd(s, prec, A, r, nullptr, nullptr, nullptr, nullptr, z, ctx->st.p, false, true,
-                                     ctx->d_err.p);
+                    launch_chain_fwd(s, prec, A, r, z, ctx->st.p, true, ctx->d_err.p);
                 } else {
                     ProfScope ps(ctx, FM_K_ILU_FWD);
                     launch_ilu_fwd(s, prec, A, lu, nullptr, r, x, p, Ap, z, ctx->st.p, true, ctx->d_err.p);
@@ -634,8 +632,7 @@ int pcg_run_pc(fm_ctx* ctx, Matrix& A, double tol, int max_it, PcgResult* res) {
         {
             ProfScope ps(ctx, FM_K_ILU_FWD);
             if (A.chain.ok)
-                launch_chain_fwd(s, PREC32, A, r32, nullptr, nullptr, nullptr, nullptr, z32, ctx->st.p, false,
-                                 check_done, ctx->d_err.p);
+                launch_chain_fwd(s, PREC32, A, r32, z32, ctx->st.p, check_done, ctx->d_err.p);
             else
                 launch_ilu_fwd(s, PREC32, A, lu, r32, nullptr, nullptr, nullptr, nullptr, z32, ctx->st.p, false,
                                ctx->d_err.p);

@@ -89,7 +89,6 @@ struct ChainSchedule {
     int lpn = 4;                     // lanes per node (the sweep kernels use one quad per node)
     uint32_t chains = 0, stride = 0, gj = 0, gk = 0;
     ChainSweep fwd, bwd;
-    DevBuf<double> sink;             // 4 values per lane and bundle (>= per block): idle-lane stores and polls
 };
 
 struct Matrix {
@@ -246,8 +245,8 @@ uint32_t launch_ilu_bwd(hipStream_t s, int prec, const Matrix& A, const void* lu
 // when upd) and backward (y = U^{-1} z with the fused r.y partials; returns
 // the number of partials)
 int chain_pack(hipStream_t s, int prec, Matrix& A);
-void launch_chain_fwd(hipStream_t s, int prec, const Matrix& A, const void* rhs, void* r, void* x, const void* p,
-                      const void* Ap, void* z, PcgState* st, bool upd, bool check_done, int* err);
+void launch_chain_fwd(hipStream_t s, int prec, const Matrix& A, const void* rhs, void* z, PcgState* st,
+                      bool check_done, int* err);
 uint32_t launch_chain_bwd(hipStream_t s, int prec, const Matrix& A, const void* z, const void* r, void* y,
                           double* partials, PcgState* st, bool check_done, int* err);
 void launch_fill_sentinel(hipStream_t s, int prec, void* a, void* b, size_t n);

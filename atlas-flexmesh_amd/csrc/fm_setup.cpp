@@ -453,8 +453,15 @@ bool build_chain_schedule(const std::vector<uint32_t>& rp, const std::vector<uin
         for (size_t q = 0; q < H.nq; ++q)
             for (int lane = 0; lane < 64; ++lane) {
                 uint32_t* f = &far[(q * 64 + lane) * 4];
-                f[0] = H.node[q * npw + lane / P.lpn];
-                for (int j = 0; j < bpl; ++j) f[1 + j] = H.dep[(q * bpl + j) * 64 + lane];
+                const uint32_t nd = H.node[q * npw + lane / P.lpn];
+                f[0] = nd == kDepNone ? kOffNone : 12u * nd;
+                f[3] = 0;
+                for (int j = 0; j < bpl; ++j) {
+                    const uint32_t dw = H.dep[(q * bpl + j) * 64 + lane];
+                    const bool local = dw != kDepNone && (dw & kDepLocal);
+                    f[1 + j] = (dw == kDepNone || local) ? kOffNone : 12u * dw;
+                    f[3] |= (local ? (dw & ~kDepLocal) : kZeroSlot) << (16 * j);
+                }
             }
         if (W.bstart.alloc(nbund) != hipSuccess || W.bsteps.alloc(nbund) != hipSuccess ||
             W.far.alloc(far.size()) != hipSuccess)
@@ -466,7 +473,7 @@ bool build_chain_schedule(const std::vector<uint32_t>& rp, const std::vector<uin
                hipMemcpy(W.far.p, far.data(), far.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
     };
     if (P.lpn != 4 || !upload(P.fwd, C.fwd) || !upload(P.bwd, C.bwd) ||
-        C.sink.alloc((size_t)nbund * 64 * 4) != hipSuccess || hipStreamSynchronize(stream) != hipSuccess) {
+        hipStreamSynchronize(stream) != hipSuccess) {
         err = "upload of the chain schedule failed";
         return false;
     }

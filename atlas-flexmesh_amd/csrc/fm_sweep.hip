@@ -14,6 +14,15 @@
 // quad and broadcast inside it with DPP quad permutes, which stay in the
 // VALU (no LDS round trip) on the step's critical path.
 //
+// One wave issues about one instruction per four cycles, and a bundle's steps
+// are serial, so the step's instruction count sets the sweep's pace.  The
+// step therefore decides nothing per lane: the host encodes every lane's
+// step as byte offsets and LDS slots (fm_plan.hpp), a dependency value is
+// always (LDS slot) + (polled value), where a non-local block reads a slot
+// that holds zeros and a non-external block polls past the end of the buffer
+// (which loads zeros without a memory access), and the payload is addressed
+// through per-bundle buffer descriptors with the step in a scalar offset.
+//
 // The factor values are repacked after each factorization into a step-major
 // layout: the payload of one lane in one step is NC 16-byte chunks, stored so
 // that each chunk index of a wave-step is one coalesced 1 KiB access.
@@ -26,13 +35,11 @@
 // The step body has no branch around a global load or store: the vector
 // memory counter is in-order, and where a load or store may be skipped the
 // compiler has to wait for every outstanding load, which would drain the
-// prefetch each step.  Loads therefore always issue (clamped steps), and the
-// polls and publications of the result vector use buffer instructions whose
-// offset is moved past the buffer for lanes with nothing to do: such a load
-// returns zeros and such a store is dropped, without a memory access.  Lanes
-// with nothing to store write a spare LDS slot.  The only branch left is the
-// rare, wave-uniform re-poll of a dependency that was not yet published when
-// it was polled early.
+// prefetch each step.  Loads therefore always issue (clamped steps), lanes
+// with nothing to poll or publish use an offset past the buffer (a dropped
+// store), and lanes with nothing to store write a spare LDS slot.  The only
+// branch left is the rare, wave-uniform re-poll of a dependency that was not
+// yet published when it was polled early.
 #include "fm_device.hpp"
 #include "fm_internal.hpp"
 
@@ -46,30 +53,19 @@ constexpr int kLPN = 4;                    // lanes per node: one quad
 constexpr int kNPW = 64 / kLPN;            // chains per wave
 constexpr int kBPL = kChainMaxBlk / kLPN;  // dependency blocks per lane
 constexpr int kRingStride = 4;             // values per LDS ring slot (3 used)
+static_assert(kZeroSlot == (uint32_t)(kChainRing * kNPW), "the zero slot follows the ring");
+static_assert(kBPL == 2, "the step record holds two dependency blocks per lane");
 
 template <typename T>
 struct Chunk;
 template <>
 struct Chunk<float> {
-    using V = float4;
     static constexpr int W = 4;
 };
 template <>
 struct Chunk<double> {
-    using V = double2;
     static constexpr int W = 2;
 };
-
-__device__ __forceinline__ void unpack(const float4& t, float* o) {
-    o[0] = t.x;
-    o[1] = t.y;
-    o[2] = t.z;
-    o[3] = t.w;
-}
-__device__ __forceinline__ void unpack(const double2& t, double* o) {
-    o[0] = t.x;
-    o[1] = t.y;
-}
 
 template <typename T>
 struct Shape {
@@ -97,93 +93,70 @@ __device__ __forceinline__ double quad(double v) {
     return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
 }
 
-// step data loaded 2D steps ahead: node of the lane's quad, dependency words
-struct Far {
-    uint32_t nd;
-    uint32_t dw[kBPL];
-};
-
-// step data loaded D steps ahead
-template <typename T>
-struct Near {
-    using S = Shape<T>;
-    uint32_t nd;
-    uint32_t dw[kBPL];
-    T v[S::NV];      // dependency blocks (row-major 3x3 each), then two diagonal-block values
-    T ext[kBPL][3];  // external dependency values polled early (may still be the sentinel)
-    T vec[3];        // the node's three components of input vector gl (see SweepArgs)
-};
-
-template <typename T>
-struct SweepArgs {
-    uint32_t nb;
-    const uint32_t* bstart;
-    const uint32_t* bsteps;
-    const uint4* far;  // per (wave-step, lane): node of the quad, dependency words 0 and 1
-    const T* pk;
-    // input vector read by quad lane gl: MODE 0 (forward, rhs): in[0] = rhs;
-    // MODE 1 (forward with the fused PCG update): in = r, Ap, p, x;
-    // MODE 2 (backward): in[0] = z, in[1] = r (for the r.y partials)
-    const T* in[4];
-    T* out;   // published result: z (forward) or y (backward)
-    uint32_t out_bytes;  // its size (< kChainMaxBytes)
-    T* r;     // MODE 1: r <- r - alpha Ap (written by quad lane 0)
-    T* x;     // MODE 1: x <- x + alpha p (written by quad lane 3)
-    T* sink;  // 4 values per lane and block: MODE 1 stores of lanes with nothing to do
-    double* partials;
-    PcgState* st;
-    int check_done;
-    int* err;
-};
-
-// Polls and publications go through a raw buffer resource over `out`.  A
-// lane whose offset is kOob (past the buffer) loads zeros and its store is
-// dropped, without a memory access: every lane issues the same instructions
-// (no branch, so no waits at joins) and idle lanes add no traffic.  The
-// cache policy is that of agent-scope atomics (sc1): the value is read from
-// and written to the device-coherent level, never a stale CU or XCD copy.
-constexpr uint32_t kOob = kChainMaxBytes;
-constexpr int kSc1 = 16;               // buffer instruction cache-policy bits: sc1
-constexpr int kRsrcWord3 = 0x00020000;  // raw buffer, 32-bit elements, no swizzle
-
-template <typename T>
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t out_rsrc(const SweepArgs<T>& A) {
-    return __builtin_amdgcn_make_buffer_rsrc(A.out, (short)0, (int)A.out_bytes, kRsrcWord3);
-}
-
+// ---------------------------------------------------------------------------
+// buffer accesses
+// ---------------------------------------------------------------------------
 // LLVM's raw buffer intrinsics, declared directly: the clang builtins for
 // loads wider than one dword return the first dword splatted in this
-// toolchain
+// toolchain.
 typedef float fm_f3 __attribute__((ext_vector_type(3)));
+typedef float fm_f4 __attribute__((ext_vector_type(4)));
 typedef double fm_d2 __attribute__((ext_vector_type(2)));
-__device__ fm_f3 buf_load_f3(__amdgpu_buffer_rsrc_t, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v3f32");
-__device__ fm_d2 buf_load_d2(__amdgpu_buffer_rsrc_t, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f64");
-__device__ double buf_load_d1(__amdgpu_buffer_rsrc_t, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.f64");
-__device__ void buf_store_u32(uint32_t, __amdgpu_buffer_rsrc_t, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.store.i32");
-__device__ void buf_store_u64(unsigned long long, __amdgpu_buffer_rsrc_t, int, int,
-                              int) __asm("llvm.amdgcn.raw.ptr.buffer.store.i64");
+typedef unsigned int fm_u4 __attribute__((ext_vector_type(4)));
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ fm_f3 buf_load_f3(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v3f32");
+__device__ fm_f4 buf_load_f4(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v4f32");
+__device__ fm_d2 buf_load_d2(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f64");
+__device__ double buf_load_d1(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.f64");
+__device__ fm_u4 buf_load_u4(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v4i32");
+__device__ void buf_store_u32(uint32_t, Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.store.i32");
+__device__ void buf_store_u64(unsigned long long, Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.store.i64");
 
-// the three components published for one node (byte offset `off`)
-__device__ __forceinline__ void poll3(__amdgpu_buffer_rsrc_t r, uint32_t off, float (&v)[3]) {
-    const fm_f3 w = buf_load_f3(r, (int)off, 0, kSc1);
+// A lane whose offset is past the buffer (kOob, or kOffNone of the step
+// record) loads zeros and its store is dropped, without a memory access.
+// Polls and publications use the cache policy of agent-scope atomics (sc1):
+// the value is read from and written to the device-coherent level, never a
+// stale CU or XCD copy.
+constexpr uint32_t kOob = kChainMaxBytes;
+constexpr int kSc1 = 16;                // buffer instruction cache-policy bits: sc1
+constexpr int kRsrcWord3 = 0x00020000;  // raw buffer, 32-bit elements, no swizzle
+#ifndef FM_PK_AUX
+#define FM_PK_AUX 2  // cache policy of the streamed step records and payload: nt (no reuse)
+#endif
+
+__device__ __forceinline__ Rsrc make_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, kRsrcWord3);
+}
+
+// step-record offsets count FP32 bytes
+template <typename T>
+__device__ __forceinline__ uint32_t scaled(uint32_t off) {
+    return sizeof(T) == 4 ? off : off << 1;
+}
+
+// the three components of one node (byte offset `off`)
+template <int AUX>
+__device__ __forceinline__ void load3(Rsrc r, uint32_t off, float (&v)[3]) {
+    const fm_f3 w = buf_load_f3(r, (int)off, 0, AUX);
     v[0] = w.x;
     v[1] = w.y;
     v[2] = w.z;
 }
-__device__ __forceinline__ void poll3(__amdgpu_buffer_rsrc_t r, uint32_t off, double (&v)[3]) {
-    const fm_d2 a = buf_load_d2(r, (int)off, 0, kSc1);
+template <int AUX>
+__device__ __forceinline__ void load3(Rsrc r, uint32_t off, double (&v)[3]) {
+    const fm_d2 a = buf_load_d2(r, (int)off, 0, AUX);
     v[0] = a.x;
     v[1] = a.y;
-    v[2] = buf_load_d1(r, (int)(off + 16), 0, kSc1);
+    v[2] = buf_load_d1(r, (int)(off + 16), 0, AUX);
 }
 
 // ready-by-value publication (the all-ones pattern is reserved for "not ready")
-__device__ __forceinline__ void publish_at(__amdgpu_buffer_rsrc_t r, uint32_t off, float v) {
+__device__ __forceinline__ void publish_at(Rsrc r, uint32_t off, float v) {
     uint32_t b = __builtin_bit_cast(uint32_t, v);
     if (b == SB<float>::S) b = SB<float>::Q;
     buf_store_u32(b, r, (int)off, 0, kSc1);
 }
-__device__ __forceinline__ void publish_at(__amdgpu_buffer_rsrc_t r, uint32_t off, double v) {
+__device__ __forceinline__ void publish_at(Rsrc r, uint32_t off, double v) {
     unsigned long long b = __builtin_bit_cast(unsigned long long, v);
     if (b == SB<double>::S) b = SB<double>::Q;
     buf_store_u64(b, r, (int)off, 0, kSc1);
@@ -193,79 +166,146 @@ template <typename T>
 __device__ __forceinline__ bool is_sentinel(T v) {
     return __builtin_bit_cast(typename SB<T>::U, v) == SB<T>::S;
 }
-
-template <typename T>
-__device__ __forceinline__ void far_load(const SweepArgs<T>& A, uint32_t q, int lane, Far& F) {
-    const uint4 w = A.far[(size_t)q * 64 + lane];
-    F.nd = w.x;
-    F.dw[0] = w.y;
-    F.dw[1] = w.z;
+// high word of a value: all ones for the sentinel (and for no value a sweep
+// publishes, except a negative NaN with that exact high word, which only
+// costs an exact re-check)
+__device__ __forceinline__ uint32_t hiword(float v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ uint32_t hiword(double v) {
+    return (uint32_t)(__builtin_bit_cast(unsigned long long, v) >> 32);
 }
 
+// ---------------------------------------------------------------------------
+// kernel
+// ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ T* sink_of(const SweepArgs<T>& A, int lane) {
-    return A.sink + ((size_t)blockIdx.x * 64 + lane) * 4;
-}
+struct SweepArgs {
+    uint32_t nb;
+    const uint32_t* bstart;
+    const uint32_t* bsteps;
+    const uint4* far;  // step records (fm_plan.hpp), 64 per wave-step
+    const T* pk;
+    const T* in0;  // forward: rhs; backward: z
+    const T* in1;  // backward: r for the r.y partials (else in0)
+    T* out;        // published result: z (forward) or y (backward)
+    uint32_t out_bytes;  // size of in0, in1 and out (< kChainMaxBytes)
+    double* partials;
+    PcgState* st;
+    int check_done;
+    int* err;
+};
 
-// every load issues: absent dependencies poll past the buffer (no access),
-// idle quads read their zero payload and the first elements of the input
-// vectors
-template <typename T, int MODE>
-__device__ __forceinline__ void near_load(const SweepArgs<T>& A, uint32_t q, int lane, const Far& F, Near<T>& N) {
+// step record (2D steps ahead)
+struct Far {
+    uint32_t pub;        // offset of the quad's node triple
+    uint32_t ext[kBPL];  // offsets of the external dependency nodes
+    uint32_t lds;        // ring slots of the dependency blocks
+};
+
+// step data (D steps ahead)
+template <typename T>
+struct Near {
     using S = Shape<T>;
-    using V = typename Chunk<T>::V;
-    N.nd = F.nd;
+    Far f;
+    T v[S::NV];  // dependency blocks (row-major 3x3 each), then two diagonal-block values
+    T vec[3];    // the node's components of in0
+    T vec1[3];   // backward: of in1
+};
+
+// external dependency values of a step, polled kPollAhead steps early (may
+// still be the sentinel)
+template <typename T>
+struct Ext {
+    T v[kBPL][3];
+};
+// The payload is prefetched D steps ahead (enough loads in flight to cover
+// the memory latency), the external dependencies are polled only kPollAhead
+// steps ahead: a poll issued earlier finds the producer's value missing more
+// often, and each miss costs a re-poll round trip.
+constexpr int kPollAhead = 2;
+
+// descriptors of one bundle's step records and payload
+struct BundleRsrc {
+    Rsrc far, pk;
+};
+
+template <typename T>
+__device__ __forceinline__ BundleRsrc bundle_rsrc(const SweepArgs<T>& A, uint32_t q0, uint32_t ns) {
+    using S = Shape<T>;
+    BundleRsrc R;
+    R.far = make_rsrc(reinterpret_cast<const char*>(A.far) + (size_t)q0 * 1024, ns * 1024u);
+    R.pk = make_rsrc(reinterpret_cast<const char*>(A.pk) + (size_t)q0 * S::NC * 1024, ns * S::NC * 1024u);
+    return R;
+}
+
+__device__ __forceinline__ void far_load(const BundleRsrc& R, uint32_t s, int lane, Far& F) {
+    const fm_u4 w = buf_load_u4(R.far, lane * 16, (int)(s * 1024), FM_PK_AUX);
+    F.pub = w.x;
+    F.ext[0] = w.y;
+    F.ext[1] = w.z;
+    F.lds = w.w;
+}
+
+// every load issues: idle quads read their zero payload, absent dependencies
+// and idle quads' vectors load past the buffers (zeros, no access)
+template <typename T, bool BWD>
+__device__ __forceinline__ void near_load(const SweepArgs<T>& A, const BundleRsrc& R, uint32_t s, int lane,
+                                          const Far& F, Near<T>& N) {
+    using S = Shape<T>;
+    N.f = F;
 #pragma unroll
-    for (int j = 0; j < kBPL; ++j) N.dw[j] = F.dw[j];
-    const V* src = reinterpret_cast<const V*>(A.pk) + (size_t)q * S::NC * 64 + lane;
-#pragma unroll
-    for (int c = 0; c < S::NC; ++c) unpack(src[c * 64], &N.v[c * S::W]);
-    const __amdgpu_buffer_rsrc_t ro = out_rsrc(A);
-#pragma unroll
-    for (int j = 0; j < kBPL; ++j) {
-        const uint32_t dw = F.dw[j];  // kDepNone has the local bit set
-        poll3(ro, (dw & kDepLocal) ? kOob : dw * (3 * (uint32_t)sizeof(T)), N.ext[j]);
+    for (int c = 0; c < S::NC; ++c) {
+        const int voff = lane * 16 + c * 1024, soff = (int)(s * (S::NC * 1024));
+        if constexpr (sizeof(T) == 4) {
+            const fm_f4 w = buf_load_f4(R.pk, voff, soff, FM_PK_AUX);
+            N.v[c * 4 + 0] = w.x;
+            N.v[c * 4 + 1] = w.y;
+            N.v[c * 4 + 2] = w.z;
+            N.v[c * 4 + 3] = w.w;
+        } else {
+            const fm_d2 w = buf_load_d2(R.pk, voff, soff, FM_PK_AUX);
+            N.v[c * 2 + 0] = w.x;
+            N.v[c * 2 + 1] = w.y;
+        }
     }
-    const int gl = lane & (kLPN - 1);
-    const int which = MODE == 1 ? gl : (MODE == 2 && gl == 1 ? 1 : 0);
-    const T* v = A.in[which] + (F.nd == kDepNone ? 0 : 3 * (size_t)F.nd);
+    load3<0>(make_rsrc(A.in0, A.out_bytes), scaled<T>(F.pub), N.vec);
+    if (BWD) load3<0>(make_rsrc(A.in1, A.out_bytes), scaled<T>(F.pub), N.vec1);
+}
+
+template <typename T>
+__device__ __forceinline__ void poll_ext(const SweepArgs<T>& A, const Far& F, Ext<T>& X) {
+    const Rsrc ro = make_rsrc(A.out, A.out_bytes);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) N.vec[c] = v[c];
+    for (int j = 0; j < kBPL; ++j) load3<kSc1>(ro, scaled<T>(F.ext[j]), X.v[j]);
 }
 
 // Re-poll of the external dependency values that were still the sentinel
-// when polled early: all of them in flight together, so a stall costs one
-// round trip rather than one per value.  Bounded like await_val.
+// when polled early.  The re-polls of both blocks issue unconditionally
+// (lanes with nothing pending load past the buffer) and are waited for
+// together: one round trip per attempt, where loads under a per-value branch
+// would each end in a full wait.  Bounded like await_val.
 template <typename T>
-__device__ __forceinline__ void await_late(const SweepArgs<T>& A, const Near<T>& N, T (&y)[kBPL][3]) {
-    const bool act = N.nd != kDepNone;
+__device__ __forceinline__ void await_late(const SweepArgs<T>& A, const Far& f, T (&e)[kBPL][3]) {
     uint32_t spins = 0;
     for (;;) {
-        // the re-polls of both dependencies issue unconditionally (lanes
-        // with nothing pending load past the buffer) and are waited for
-        // together: one round trip per attempt, where loads under a
-        // per-value branch would each end in a full wait
         __asm__ volatile("" ::: "memory");  // every attempt loads again
         T z[kBPL][3];
         bool need[kBPL][3];
 #pragma unroll
         for (int j = 0; j < kBPL; ++j) {
-            const uint32_t dw = N.dw[j];
-            const bool ext = act && dw != kDepNone && !(dw & kDepLocal);
             bool any = false;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                need[j][c] = ext && is_sentinel(y[j][c]);
+                need[j][c] = is_sentinel(e[j][c]);  // values of absent blocks are zeros
                 any |= need[j][c];
             }
-            poll3(out_rsrc(A), any ? dw * (3 * (uint32_t)sizeof(T)) : kOob, z[j]);
+            load3<kSc1>(make_rsrc(A.out, A.out_bytes), any ? scaled<T>(f.ext[j]) : kOob, z[j]);
         }
         bool pending = false;
 #pragma unroll
         for (int j = 0; j < kBPL; ++j)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                y[j][c] = need[j][c] ? z[j][c] : y[j][c];
+                e[j][c] = need[j][c] ? z[j][c] : e[j][c];
                 pending |= need[j][c] && is_sentinel(z[j][c]);
             }
         if (!__any(pending)) return;
@@ -280,27 +320,29 @@ __device__ __forceinline__ void await_late(const SweepArgs<T>& A, const Near<T>&
 
 // One step of one quad: its node's row block against the dependency values,
 // the 3x3 diagonal-block solve, publication of the three results.
-template <typename T, int MODE>
-__device__ __forceinline__ void process(const SweepArgs<T>& A, const Near<T>& N, uint32_t s, int lane, T* ring,
-                                        T alpha, double& dot) {
+template <typename T, bool BWD>
+__device__ __forceinline__ void process(const SweepArgs<T>& A, const Near<T>& N, const Ext<T>& X, uint32_t s,
+                                        int lane, T* ring, double& dot) {
     using S = Shape<T>;
     const int g = lane / kLPN, gl = lane & (kLPN - 1);
-    const bool act = N.nd != kDepNone;
-    T y[kBPL][3];
-    bool late = false;
+    T e[kBPL][3];
+    uint32_t hi = 0;
 #pragma unroll
-    for (int j = 0; j < kBPL; ++j) {
-        const uint32_t dw = N.dw[j];
-        const bool none = !act || dw == kDepNone, local = !none && (dw & kDepLocal);
-        const T* src = ring + (size_t)(local ? (dw & ~kDepLocal) : 0) * kRingStride;
+    for (int j = 0; j < kBPL; ++j)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const T l = src[c], e = N.ext[j][c];
-            late |= !none && !local && is_sentinel(e);
-            y[j][c] = none ? (T)0 : (local ? l : e);
+            e[j][c] = X.v[j][c];
+            hi = max(hi, hiword(e[j][c]));
         }
+    // a dependency was not yet published when polled early
+    if (__any(hi == 0xFFFFFFFFu)) await_late<T>(A, N.f, e);
+    T y[kBPL][3];
+#pragma unroll
+    for (int j = 0; j < kBPL; ++j) {
+        const T* src = ring + ((N.f.lds >> (16 * j)) & 0xFFFFu) * kRingStride;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) y[j][c] = src[c] + e[j][c];
     }
-    if (__any(late)) await_late<T>(A, N, y);  // a dependency was not yet published when polled early
     T part[3] = {(T)0, (T)0, (T)0};
 #pragma unroll
     for (int j = 0; j < kBPL; ++j)
@@ -313,16 +355,13 @@ __device__ __forceinline__ void process(const SweepArgs<T>& A, const Near<T>& N,
         part[d] += quad<kXor1>(part[d]);
         part[d] += quad<kXor2>(part[d]);
     }
-    // right-hand side: quad lane 0 holds rhs / r / z, lane 1 Ap (MODE 1)
+    // right-hand side: quad lane 0's view of in0
     T b[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        b[d] = quad<kBc0>(N.vec[d]);
-        if (MODE == 1) b[d] -= alpha * quad<kBc1>(N.vec[d]);
-    }
+    for (int d = 0; d < 3; ++d) b[d] = quad<kBc0>(N.vec[d]);
     const T d0 = N.v[S::DG], d1 = N.v[S::DG + 1];
     T v0, v1, v2;
-    if (MODE != 2) {
+    if (!BWD) {
         // unit lower diagonal block: l10, l20 in quad lane 0, l21 in lane 1
         const T l10 = quad<kBc0>(d0), l20 = quad<kBc0>(d1), l21 = quad<kBc1>(d0);
         v0 = b[0] - part[0];
@@ -345,44 +384,45 @@ __device__ __forceinline__ void process(const SweepArgs<T>& A, const Near<T>& N,
         t0 -= u02 * v2;
         v0 = t0 * r00;
     }
-    const bool wr = act && gl < 3;
+    // lanes 0..2 publish components 0..2 (an idle quad's offset is past the
+    // buffer); lane 3 stores into its spare LDS slot
+    const bool wr = gl < 3;
     const T vd = gl == 0 ? v0 : (gl == 1 ? v1 : v2);
-    publish_at(out_rsrc(A), wr ? (3 * N.nd + gl) * (uint32_t)sizeof(T) : kOob, vd);
-    ring[wr ? ((s % kChainRing) * kNPW + g) * kRingStride + gl : kChainRing * kNPW * kRingStride + lane] = vd;
-    if (MODE == 1) {
-        // lane 0 writes r - alpha Ap, lane 3 writes x + alpha p (p from lane 2)
-        T pv[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) pv[d] = quad<kBc2>(N.vec[d]);
-        T* const sk = sink_of(A, lane);
-        T* dst = !act ? sk : (gl == 0 ? A.r + 3 * (size_t)N.nd : (gl == 3 ? A.x + 3 * (size_t)N.nd : sk));
-#pragma unroll
-        for (int d = 0; d < 3; ++d) dst[d] = gl == 0 ? b[d] : N.vec[d] + alpha * pv[d];
-    }
-    if (MODE == 2 && act && gl == 1)
-        dot += (double)N.vec[0] * (double)v0 + (double)N.vec[1] * (double)v1 + (double)N.vec[2] * (double)v2;
+    publish_at(make_rsrc(A.out, A.out_bytes), wr ? scaled<T>(N.f.pub) + gl * (uint32_t)sizeof(T) : kOob, vd);
+    ring[wr ? ((s % kChainRing) * kNPW + g) * kRingStride + gl : (kZeroSlot + 1) * kRingStride + lane] = vd;
+    // r.y partials (idle quads contribute zeros)
+    if (BWD && gl == 1)
+        dot += (double)N.vec1[0] * (double)v0 + (double)N.vec1[1] * (double)v1 + (double)N.vec1[2] * (double)v2;
 }
 
 // Persistent: block b of the grid takes bundles b, b + P, ... in processing
 // order, which is a topological order of the bundle graph, so the earliest
 // unfinished bundle only ever waits on finished ones.
-template <typename T, int D, int MODE>
+template <typename T, int D, bool BWD>
 __global__ __launch_bounds__(64) void chain_sweep_kernel(SweepArgs<T> A) {
     if (A.check_done && A.st->done) return;
-    const T alpha = MODE == 1 ? (T)A.st->alpha : (T)0;
-    // the ring, then one spare slot per lane for lanes with nothing to store
-    __shared__ T ring[kChainRing * kNPW * kRingStride + 64];
+    // the ring, the zero slot, then one spare slot per lane for lanes with nothing to store
+    __shared__ T ring[(kZeroSlot + 1) * kRingStride + 64];
     const int lane = threadIdx.x;
+    if (lane < kRingStride) ring[kZeroSlot * kRingStride + lane] = (T)0;
     double dot = 0.0;
     for (uint32_t b = blockIdx.x; b < A.nb; b += gridDim.x) {
-        const uint32_t q0 = A.bstart[b], ns = A.bsteps[b], ql = q0 + ns - 1;
+        // wave-uniform: the descriptors built from them must live in scalar registers
+        const uint32_t q0 = __builtin_amdgcn_readfirstlane(A.bstart[b]);
+        const uint32_t ns = __builtin_amdgcn_readfirstlane(A.bsteps[b]), sl = ns - 1;
+        const BundleRsrc R = bundle_rsrc(A, q0, ns);
         // steps past the end re-read the last step (never processed)
+        constexpr int DE = kPollAhead;
+        static_assert(DE <= D && (2 * D) % DE == 0, "poll distance must divide the unrolled block");
         Far F[2 * D];
         Near<T> N[D];
+        Ext<T> X[DE];
 #pragma unroll
-        for (int u = 0; u < 2 * D; ++u) far_load(A, min(q0 + u, ql), lane, F[u]);
+        for (int u = 0; u < 2 * D; ++u) far_load(R, min((uint32_t)u, sl), lane, F[u]);
 #pragma unroll
-        for (int u = 0; u < D; ++u) near_load<T, MODE>(A, min(q0 + u, ql), lane, F[u], N[u]);
+        for (int u = 0; u < DE; ++u) poll_ext(A, F[u], X[u]);
+#pragma unroll
+        for (int u = 0; u < D; ++u) near_load<T, BWD>(A, R, min((uint32_t)u, sl), lane, F[u], N[u]);
         // 2D steps per block, unguarded: every block issues the same memory
         // operations, so the waits the compiler derives inside the loop see
         // loads issued a full block earlier.  The first block is peeled (its
@@ -392,9 +432,10 @@ __global__ __launch_bounds__(64) void chain_sweep_kernel(SweepArgs<T> A) {
 #pragma unroll
             for (int u = 0; u < 2 * D; ++u) {
                 const uint32_t s = s0 + u;
-                process<T, MODE>(A, N[u % D], s, lane, ring, alpha, dot);
-                near_load<T, MODE>(A, min(q0 + s + D, ql), lane, F[(u + D) % (2 * D)], N[u % D]);
-                far_load(A, min(q0 + s + 2 * D, ql), lane, F[u]);
+                process<T, BWD>(A, N[u % D], X[u % DE], s, lane, ring, dot);
+                poll_ext(A, F[(u + DE) % (2 * D)], X[u % DE]);
+                near_load<T, BWD>(A, R, min(s + D, sl), lane, F[(u + D) % (2 * D)], N[u % D]);
+                far_load(R, min(s + 2 * D, sl), lane, F[u]);
             }
         };
         uint32_t s0 = 0;
@@ -406,13 +447,14 @@ __global__ __launch_bounds__(64) void chain_sweep_kernel(SweepArgs<T> A) {
         for (int u = 0; u < 2 * D; ++u) {
             const uint32_t s = s0 + u;
             if (s < ns) {
-                process<T, MODE>(A, N[u % D], s, lane, ring, alpha, dot);
-                near_load<T, MODE>(A, min(q0 + s + D, ql), lane, F[(u + D) % (2 * D)], N[u % D]);
-                far_load(A, min(q0 + s + 2 * D, ql), lane, F[u]);
+                process<T, BWD>(A, N[u % D], X[u % DE], s, lane, ring, dot);
+                poll_ext(A, F[(u + DE) % (2 * D)], X[u % DE]);
+                near_load<T, BWD>(A, R, min(s + D, sl), lane, F[(u + D) % (2 * D)], N[u % D]);
+                far_load(R, min(s + 2 * D, sl), lane, F[u]);
             }
         }
     }
-    if (MODE == 2 && A.partials) {
+    if (BWD && A.partials) {
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o);
         if (lane == 0) A.partials[blockIdx.x] = dot;
@@ -434,11 +476,12 @@ __global__ __launch_bounds__(256) void chain_pack_kernel(uint32_t nq, const uint
     if (t >= (uint64_t)nq * 64) return;
     const uint32_t q = (uint32_t)(t >> 6);
     const int lane = (int)(t & 63), gl = lane & (kLPN - 1);
-    const uint32_t v = far[t].x;
+    const uint32_t x = far[t].x;
     T val[S::NV];
 #pragma unroll
     for (int k = 0; k < S::NV; ++k) val[k] = (T)0;
-    if (v != kDepNone) {
+    if (x != kOffNone) {
+        const uint32_t v = x / 12u;
         uint32_t rs[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) rs[d] = rp[3 * v + d];
@@ -475,23 +518,18 @@ __global__ __launch_bounds__(256) void chain_pack_kernel(uint32_t nq, const uint
             }
         }
     }
-    using V = typename Chunk<T>::V;
-    V* dst = reinterpret_cast<V*>(pk) + (size_t)q * S::NC * 64 + lane;
+    T* dst = pk + ((size_t)q * S::NC * 64 + lane) * S::W;
 #pragma unroll
-    for (int c = 0; c < S::NC; ++c) {
-        V w;
-        T* wp = reinterpret_cast<T*>(&w);
+    for (int c = 0; c < S::NC; ++c)
 #pragma unroll
-        for (int k = 0; k < S::W; ++k) wp[k] = val[c * S::W + k];
-        dst[c * 64] = w;
-    }
+        for (int k = 0; k < S::W; ++k) dst[(size_t)c * 64 * S::W + k] = val[c * S::W + k];
 }
 
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 // prefetch depth in steps
 #ifndef FM_CHAIN_D32
-#define FM_CHAIN_D32 2
+#define FM_CHAIN_D32 3
 #endif
 #ifndef FM_CHAIN_D64
 #define FM_CHAIN_D64 2
@@ -501,19 +539,16 @@ constexpr int depth() {
     return sizeof(T) == 4 ? FM_CHAIN_D32 : FM_CHAIN_D64;
 }
 
-template <typename T, int MODE>
-uint32_t launch_sweep(hipStream_t s, const ChainSchedule& C, const ChainSweep& W, const T* pk, SweepArgs<T> a) {
-    a.sink = reinterpret_cast<T*>(C.sink.p);
-    for (int k = 1; k < 4; ++k)
-        if (!a.in[k]) a.in[k] = a.in[0];  // always read (value unused)
-    if (!a.r) a.r = a.sink;
-    if (!a.x) a.x = a.sink;
+template <typename T, bool BWD>
+uint32_t launch_sweep(hipStream_t s, const Matrix& A, const ChainSweep& W, const T* pk, SweepArgs<T> a) {
+    if (!a.in1) a.in1 = a.in0;  // always read (value unused)
+    a.out_bytes = (uint32_t)((size_t)A.n * sizeof(T));
     a.nb = W.nb;
     a.bstart = W.bstart.p;
     a.bsteps = W.bsteps.p;
     a.far = reinterpret_cast<const uint4*>(W.far.p);
     a.pk = pk;
-    auto kern = chain_sweep_kernel<T, depth<T>(), MODE>;
+    auto kern = chain_sweep_kernel<T, depth<T>(), BWD>;
     const uint32_t grid = persistent_grid(reinterpret_cast<const void*>(kern), W.nb);
     kern<<<grid, 64, 0, s>>>(a);
     record_launch_error(hipGetLastError());
@@ -551,62 +586,43 @@ int chain_pack(hipStream_t s, int prec, Matrix& A) {
     return FM_OK;
 }
 
-template <typename T>
-static void chain_fwd_t(hipStream_t s, const Matrix& A, const void* rhs, void* r, void* x, const void* p,
-                        const void* Ap, void* z, PcgState* st, bool upd, bool check_done, int* err) {
+void launch_chain_fwd(hipStream_t s, int prec, const Matrix& A, const void* rhs, void* z, PcgState* st,
+                      bool check_done, int* err) {
     const ChainSchedule& C = A.chain;
-    SweepArgs<T> a{};
-    if (upd) {
-        a.in[0] = (const T*)r;
-        a.in[1] = (const T*)Ap;
-        a.in[2] = (const T*)p;
-        a.in[3] = (const T*)x;
-    } else {
-        a.in[0] = (const T*)rhs;
-    }
-    a.out = (T*)z;
-    a.out_bytes = (uint32_t)((size_t)A.n * sizeof(T));
-    a.r = (T*)r;
-    a.x = (T*)x;
-    a.st = st;
-    a.check_done = (upd || check_done) ? 1 : 0;
-    a.err = err;
-    const T* pk = sizeof(T) == 8 ? (const T*)C.fwd.pk64.p : (const T*)C.fwd.pk32.p;
-    if (upd)
-        launch_sweep<T, 1>(s, C, C.fwd, pk, a);
-    else
-        launch_sweep<T, 0>(s, C, C.fwd, pk, a);
-}
-
-void launch_chain_fwd(hipStream_t s, int prec, const Matrix& A, const void* rhs, void* r, void* x, const void* p,
-                      const void* Ap, void* z, PcgState* st, bool upd, bool check_done, int* err) {
+    auto run = [&](auto zero) {
+        using T = decltype(zero);
+        SweepArgs<T> a{};
+        a.in0 = (const T*)rhs;
+        a.out = (T*)z;
+        a.st = st;
+        a.check_done = check_done ? 1 : 0;
+        a.err = err;
+        const T* pk = sizeof(T) == 8 ? (const T*)C.fwd.pk64.p : (const T*)C.fwd.pk32.p;
+        launch_sweep<T, false>(s, A, C.fwd, pk, a);
+    };
     if (prec == PREC64)
-        chain_fwd_t<double>(s, A, rhs, r, x, p, Ap, z, st, upd, check_done, err);
+        run(0.0);
     else
-        chain_fwd_t<float>(s, A, rhs, r, x, p, Ap, z, st, upd, check_done, err);
-}
-
-template <typename T>
-static uint32_t chain_bwd_t(hipStream_t s, const Matrix& A, const void* z, const void* r, void* y, double* partials,
-                            PcgState* st, bool check_done, int* err) {
-    const ChainSchedule& C = A.chain;
-    SweepArgs<T> a{};
-    a.in[0] = (const T*)z;
-    a.in[1] = partials ? (const T*)r : nullptr;
-    a.out = (T*)y;
-    a.out_bytes = (uint32_t)((size_t)A.n * sizeof(T));
-    a.partials = partials;
-    a.st = st;
-    a.check_done = check_done ? 1 : 0;
-    a.err = err;
-    const T* pk = sizeof(T) == 8 ? (const T*)C.bwd.pk64.p : (const T*)C.bwd.pk32.p;
-    return launch_sweep<T, 2>(s, C, C.bwd, pk, a);
+        run(0.0f);
 }
 
 uint32_t launch_chain_bwd(hipStream_t s, int prec, const Matrix& A, const void* z, const void* r, void* y,
                           double* partials, PcgState* st, bool check_done, int* err) {
-    if (prec == PREC64) return chain_bwd_t<double>(s, A, z, r, y, partials, st, check_done, err);
-    return chain_bwd_t<float>(s, A, z, r, y, partials, st, check_done, err);
+    const ChainSchedule& C = A.chain;
+    auto run = [&](auto zero) -> uint32_t {
+        using T = decltype(zero);
+        SweepArgs<T> a{};
+        a.in0 = (const T*)z;
+        a.in1 = partials ? (const T*)r : nullptr;
+        a.out = (T*)y;
+        a.partials = partials;
+        a.st = st;
+        a.check_done = check_done ? 1 : 0;
+        a.err = err;
+        const T* pk = sizeof(T) == 8 ? (const T*)C.bwd.pk64.p : (const T*)C.bwd.pk32.p;
+        return launch_sweep<T, true>(s, A, C.bwd, pk, a);
+    };
+    return prec == PREC64 ? run(0.0) : run(0.0f);
 }
 
 }  // namespace fm
