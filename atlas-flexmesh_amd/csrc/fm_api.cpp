@@ -407,37 +407,73 @@ void* vb(fm_ctx* c, int prec, int which) {
 }
 enum { VB = 0, VX, VR, VP, VAP, VZ, VY };
 
-int factorize(fm_ctx* ctx, Matrix& A, int prec) {
-    if (A.lu_valid[prec]) return FM_OK;
-    if (!A.val_valid[prec]) return fail(ctx, FM_ESTATE, "matrix values not available in this precision");
-    const void* val;
-    void* lu;
-    if (prec == PREC64) {
-        FM_HIP(A.lu64.alloc(A.nnz));
-        val = A.val64.p;
-        lu = A.lu64.p;
-    } else {
-        FM_HIP(A.lu32.alloc(A.nnz));
-        val = A.val32.p;
-        lu = A.lu32.p;
+int pack_factor(fm_ctx* ctx, Matrix& A, int prec);
+
+int factorize(fm_ctx* ctx, Matrix& A, int prec, bool pack = true) {
+    const bool have = A.lu_valid[prec] && !(prec == PREC32 && A.lu32_rounded);
+    if (!have) {
+        if (!A.val_valid[prec]) return fail(ctx, FM_ESTATE, "matrix values not available in this precision");
+        const void* val;
+        void* lu;
+        if (prec == PREC64) {
+            FM_HIP(A.lu64.alloc(A.nnz));
+            val = A.val64.p;
+            lu = A.lu64.p;
+        } else {
+            FM_HIP(A.lu32.alloc(A.nnz));
+            val = A.val32.p;
+            lu = A.lu32.p;
+        }
+        // ticket[2] of the state is the factorization queue
+        PcgState init{};
+        *ctx->h_st = init;
+        FM_HIP(hipMemcpyAsync(ctx->st.p, ctx->h_st, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
+        {
+            ProfScope ps(ctx, FM_K_ILU_FACTOR);
+            launch_ilu_factor(ctx->stream, prec, A, val, lu, ctx->st.p, ctx->d_err.p);
+        }
+        FM_TRY(check_launch(ctx, "ilu_factor"));
+        A.pk_valid[prec] = false;
+        if (pack) FM_TRY(pack_factor(ctx, A, prec));
+        FM_TRY(check_device_error(ctx));  // a pivot wait that hit its bound
+        A.lu_valid[prec] = true;
+        if (prec == PREC32) A.lu32_rounded = false;
     }
-    // ticket[2] of the state is the factorization queue
-    PcgState init{};
-    *ctx->h_st = init;
-    FM_HIP(hipMemcpyAsync(ctx->st.p, ctx->h_st, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
+    if (pack) FM_TRY(pack_factor(ctx, A, prec));
+    return FM_OK;
+}
+
+// sweep payload of the current factor in `prec`
+int pack_factor(fm_ctx* ctx, Matrix& A, int prec) {
+    if (!A.chain.ok || A.pk_valid[prec]) return FM_OK;
     {
         ProfScope ps(ctx, FM_K_ILU_FACTOR);
-        launch_ilu_factor(ctx->stream, prec, A, val, lu, ctx->st.p, ctx->d_err.p);
-    }
-    FM_TRY(check_launch(ctx, "ilu_factor"));
-    if (A.chain.ok) {
-        ProfScope ps(ctx, FM_K_ILU_FACTOR);
         if (chain_pack(ctx->stream, prec, A) != FM_OK) return fail(ctx, FM_EDEVICE, "allocation of the packed factor failed");
-        FM_TRY(check_launch(ctx, "chain_pack"));
     }
-    FM_TRY(check_device_error(ctx));
-    A.lu_valid[prec] = true;
+    FM_TRY(check_launch(ctx, "chain_pack"));
+    A.pk_valid[prec] = true;
     return FM_OK;
+}
+
+// Preconditioner of FM_MIXED_PC: the FP64 ILU(0) factor rounded to FP32 and
+// applied in FP32.  A factorization carried out in FP32 loses digits in the
+// Schur-complement updates of an ill-conditioned tangent (a slender beam),
+// and the preconditioned iteration count then depends on the rounding of
+// every other kernel; the rounded FP64 factor keeps it at the FP64 count.
+int factorize_rounded(fm_ctx* ctx, Matrix& A) {
+    if (!(A.lu_valid[PREC32] && A.lu32_rounded)) {
+        FM_TRY(factorize(ctx, A, PREC64, false));
+        FM_HIP(A.lu32.alloc(A.nnz));
+        {
+            ProfScope ps(ctx, FM_K_ILU_FACTOR);
+            launch_f64_to_f32(ctx->stream, (uint32_t)A.nnz, A.lu64.p, 1.0, A.lu32.p);
+        }
+        FM_TRY(check_launch(ctx, "factor rounding"));
+        A.lu_valid[PREC32] = true;
+        A.lu32_rounded = true;
+        A.pk_valid[PREC32] = false;
+    }
+    return pack_factor(ctx, A, PREC32);
 }
 
 // M^{-1} applied to vector r -> y (vectors of the PCG work set)
@@ -474,7 +510,7 @@ int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult*
     const uint32_t n = A.n;
     FM_TRY(factorize(ctx, A, prec));
     const uint32_t nbw = A.sched.q_bwd + 64;
-    FM_TRY(ensure_partials(ctx, std::max<size_t>({(size_t)spmv_blocks(A), (size_t)n / 256 + 1, (size_t)nbw})));
+    FM_TRY(ensure_partials(ctx, std::max<size_t>({(size_t)spmv_blocks(A, prec), (size_t)n / 256 + 1, (size_t)nbw})));
     void* b = vb(ctx, prec, VB);
     void* x = vb(ctx, prec, VX);
     void* r = vb(ctx, prec, VR);
@@ -528,7 +564,7 @@ int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult*
                 ProfScope ps(ctx, FM_K_SPMV);
                 launch_spmv(s, prec, A, p, Ap, true, ctx->partials.p, z, y, ctx->st.p);
             }
-            launch_reduce(s, ctx->partials.p, spmv_blocks(A), ctx->st.p, 1, 0);
+            launch_reduce(s, ctx->partials.p, spmv_blocks(A, prec), ctx->st.p, 1, 0);
             {
                 if (A.chain.ok) {
                     // the update runs as its own streaming kernel: fused, its
@@ -586,13 +622,12 @@ int values_to_fp32(fm_ctx* ctx, Matrix& M) {
 int pcg_run_pc(fm_ctx* ctx, Matrix& A, double tol, int max_it, PcgResult* res) {
     hipStream_t s = ctx->stream;
     const uint32_t n = A.n;
-    if (!A.val_valid[PREC32]) FM_TRY(values_to_fp32(ctx, A));
-    FM_TRY(factorize(ctx, A, PREC32));
+    FM_TRY(factorize_rounded(ctx, A));
     FM_TRY(pcg_vectors(ctx, PREC64, n));
     FM_TRY(pcg_vectors(ctx, PREC32, n));
     const uint32_t nbw = A.sched.q_bwd + 64;
     // partials, then the r.z_old partials of the flexible beta
-    const size_t npart = std::max<size_t>({(size_t)spmv_blocks(A), (size_t)n / 256 + 1, (size_t)nbw});
+    const size_t npart = std::max<size_t>({(size_t)spmv_blocks(A, PREC64), (size_t)n / 256 + 1, (size_t)nbw});
     FM_TRY(ensure_partials(ctx, npart + n / 256 + 1));
     double* old_part = ctx->partials.p + npart;
     double *b = ctx->b64.p, *x = ctx->x64.p, *r = ctx->r64.p, *p = ctx->p64.p, *Ap = ctx->Ap64.p, *y = ctx->y64.p;
@@ -659,7 +694,7 @@ int pcg_run_pc(fm_ctx* ctx, Matrix& A, double tol, int max_it, PcgResult* res) {
                 ProfScope ps(ctx, FM_K_SPMV);
                 launch_spmv(s, PREC64, A, p, Ap, true, ctx->partials.p, nullptr, nullptr, ctx->st.p);
             }
-            launch_reduce(s, ctx->partials.p, spmv_blocks(A), ctx->st.p, 1, 0);
+            launch_reduce(s, ctx->partials.p, spmv_blocks(A, PREC64), ctx->st.p, 1, 0);
             {
                 ProfScope pu(ctx, FM_K_PCG_OTHER);
                 launch_pcg_xr_update(s, PREC64, n, p, Ap, x, r, ctx->st.p);
@@ -1156,8 +1191,7 @@ int fm_pcg(fm_ctx* ctx, int prec, const double* b, double* x, double tol, int ma
         if (!ctx->cur || !ctx->cur->ready) return fail(ctx, FM_ESTATE, "no matrix (fm_set_matrix / fm_use_tangent)");
         A = ctx->cur;
         if (!A->val_valid[PREC64]) return fail(ctx, FM_ESTATE, "FP64 matrix values needed for FM_MIXED_PC");
-        FM_TRY(values_to_fp32(ctx, *A));
-        A->lu_valid[PREC32] = false;
+        A->lu_valid[PREC64] = A->lu_valid[PREC32] = false;  // pcg_solve_ilu0 factorizes on every call
         FM_TRY(pcg_vectors(ctx, PREC64, A->n));
         const size_t n = A->n;
         FM_HIP(hipMemcpy(ctx->b64.p, b, n * 8, hipMemcpyHostToDevice));

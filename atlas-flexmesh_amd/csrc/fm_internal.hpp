@@ -102,6 +102,8 @@ struct Matrix {
     DevBuf<float> lu32;
     bool lu_valid[2] = {false, false};
     bool val_valid[2] = {false, false};
+    bool pk_valid[2] = {false, false};  // sweep payload packed from the current factor
+    bool lu32_rounded = false;          // lu32 is the FP64 factor rounded (FM_MIXED_PC), not an FP32 factorization
     Schedule sched;
     ChainSchedule chain;
     // column nodes of each node row block (3-row node-blocked matrices): the
@@ -233,7 +235,7 @@ uint32_t launch_dot64(hipStream_t s, uint32_t n, const double* a, const double* 
 void launch_reduce(hipStream_t s, const double* partials, uint32_t np, PcgState* st, int op, int k);
 void launch_spmv(hipStream_t s, int prec, const Matrix& A, const void* x, void* y, bool dot, double* partials,
                  void* sent0, void* sent1, const PcgState* st);
-uint32_t spmv_blocks(const Matrix& A);
+uint32_t spmv_blocks(const Matrix& A, int prec);  // partials written by launch_spmv with dot
 void launch_ilu_factor(hipStream_t s, int prec, const Matrix& A, const void* val, void* lu, PcgState* st,
                        int* err);
 void launch_ilu_fwd(hipStream_t s, int prec, const Matrix& A, const void* lu, const void* rhs, void* r, void* x,

@@ -400,8 +400,45 @@ __global__ __launch_bounds__(256) void spmv_stream_kernel(uint32_t n, const uint
 // per 3x3 block (nptr/ncol) instead of one column index per entry, which is
 // a third of the CSR bytes at 4-byte values.  16 lanes per node: lane k takes
 // blocks k, k + 16, ...; each block is a 3-value gather of x and the three
-// contiguous 3-value segments of the node's rows.  Fixed-order reduction.
+// contiguous 3-value segments of the node's rows, which start at
+// row_ptr[0] + 9 * nptr[node] (the three rows of a node have equal length).
+// Each 16-lane group takes nodes_per_group nodes whose loads are all issued
+// before any is used, so every wave has that many times the bytes in flight
+// per round trip; lanes past a node's last block re-read a valid block with
+// weight zero.  Fixed-order reduction (DPP within the 16-lane row).
 constexpr int kNodeLanes = 16;
+#ifndef FM_NPG32
+#define FM_NPG32 2
+#endif
+#ifndef FM_NPG64
+#define FM_NPG64 1
+#endif
+template <typename T>
+constexpr int nodes_per_group() {
+    return sizeof(T) == 4 ? FM_NPG32 : FM_NPG64;  // FP64: two nodes per group measured slower
+}
+
+template <int CTRL>
+__device__ __forceinline__ float row_dpp(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double row_dpp(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
+}
+// sum over the 16 lanes of a DPP row, in every lane: rotations by 8 and 4,
+// then quad permutes (xor 2, xor 1)
+template <typename T>
+__device__ __forceinline__ T row16_sum(T v) {
+    v += row_dpp<0x128>(v);  // row_ror:8
+    v += row_dpp<0x124>(v);  // row_ror:4
+    v += row_dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += row_dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+    return v;
+}
 
 template <typename T, bool DOT>
 __global__ __launch_bounds__(256) void spmv_node_kernel(uint32_t nn, const uint32_t* __restrict__ rp,
@@ -412,48 +449,117 @@ __global__ __launch_bounds__(256) void spmv_node_kernel(uint32_t nn, const uint3
                                                         T* __restrict__ sent0, T* __restrict__ sent1,
                                                         const PcgState* __restrict__ st) {
     if (st && st->done) return;
-    constexpr int NPB = 256 / kNodeLanes;
+    constexpr int NPB = 256 / kNodeLanes, G = nodes_per_group<T>();
     const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t node = blk * NPB + threadIdx.x / kNodeLanes;
     const int k = threadIdx.x % kNodeLanes;
-    T s0 = 0, s1 = 0, s2 = 0;
-    if (node < nn) {
-        const uint32_t b0 = nptr[node], nb = nptr[node + 1] - b0;
-        const uint32_t r0 = rp[3 * node], r1 = rp[3 * node + 1], r2 = rp[3 * node + 2];
-        for (uint32_t kk = k; kk < nb; kk += kNodeLanes) {
-            const T* a0 = val + r0 + 3 * kk;
-            const T* a1 = val + r1 + 3 * kk;
-            const T* a2 = val + r2 + 3 * kk;
-            T a[9];
+    if constexpr (G == 1) {
+        // one node per group: lanes past the node's last block skip the loop
+        const uint32_t node = blk * NPB + threadIdx.x / kNodeLanes;
+        T s0 = 0, s1 = 0, s2 = 0;
+        if (node < nn) {
+            const uint32_t b0 = nptr[node], nb = nptr[node + 1] - b0;
+            const uint32_t r0 = rp[3 * node], r1 = rp[3 * node + 1], r2 = rp[3 * node + 2];
+            for (uint32_t kk = k; kk < nb; kk += kNodeLanes) {
+                const T* a0 = val + r0 + 3 * kk;
+                const T* a1 = val + r1 + 3 * kk;
+                const T* a2 = val + r2 + 3 * kk;
+                T a[9];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                a[c] = __builtin_nontemporal_load(a0 + c);
-                a[3 + c] = __builtin_nontemporal_load(a1 + c);
-                a[6 + c] = __builtin_nontemporal_load(a2 + c);
+                for (int c = 0; c < 3; ++c) {
+                    a[c] = __builtin_nontemporal_load(a0 + c);
+                    a[3 + c] = __builtin_nontemporal_load(a1 + c);
+                    a[6 + c] = __builtin_nontemporal_load(a2 + c);
+                }
+                const T* xm = x + 3 * (size_t)__builtin_nontemporal_load(ncol + b0 + kk);
+                const T x0 = xm[0], x1 = xm[1], x2 = xm[2];
+                s0 += a[0] * x0 + a[1] * x1 + a[2] * x2;
+                s1 += a[3] * x0 + a[4] * x1 + a[5] * x2;
+                s2 += a[6] * x0 + a[7] * x1 + a[8] * x2;
             }
-            const T* xm = x + 3 * (size_t)__builtin_nontemporal_load(ncol + b0 + kk);
-            const T x0 = xm[0], x1 = xm[1], x2 = xm[2];
-            s0 += a[0] * x0 + a[1] * x1 + a[2] * x2;
-            s1 += a[3] * x0 + a[4] * x1 + a[5] * x2;
-            s2 += a[6] * x0 + a[7] * x1 + a[8] * x2;
         }
-    }
 #pragma unroll
-    for (int o = kNodeLanes / 2; o > 0; o >>= 1) {
-        s0 += __shfl_xor(s0, o);
-        s1 += __shfl_xor(s1, o);
-        s2 += __shfl_xor(s2, o);
-    }
-    double part = 0.0;
-    if (node < nn && k < 3) {
-        const uint32_t row = 3 * node + k;
-        const T sum = k == 0 ? s0 : (k == 1 ? s1 : s2);
-        y[row] = sum;
-        if (sent0) {
-            sent0[row] = sentinel_value<T>();
-            sent1[row] = sentinel_value<T>();
+        for (int o = kNodeLanes / 2; o > 0; o >>= 1) {
+            s0 += __shfl_xor(s0, o);
+            s1 += __shfl_xor(s1, o);
+            s2 += __shfl_xor(s2, o);
         }
-        if (DOT) part = (double)sum * (double)x[row];
+        double part = 0.0;
+        if (node < nn && k < 3) {
+            const uint32_t row = 3 * node + k;
+            const T sum = k == 0 ? s0 : (k == 1 ? s1 : s2);
+            y[row] = sum;
+            if (sent0) {
+                sent0[row] = sentinel_value<T>();
+                sent1[row] = sentinel_value<T>();
+            }
+            if (DOT) part = (double)sum * (double)x[row];
+        }
+        if (DOT) {
+            const double bs = block_sum(part);
+            if (threadIdx.x == 0) partials[blk] = bs;
+        }
+        return;
+    }
+    const uint32_t rbase = rp[0];
+    uint32_t node[G], b0[G], nb[G];
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+        node[i] = (blk * G + i) * NPB + threadIdx.x / kNodeLanes;
+        const uint32_t v = min(node[i], nn - 1);
+        b0[i] = nptr[v];
+        nb[i] = node[i] < nn ? nptr[v + 1] - b0[i] : 0;
+    }
+    T s[G][3];
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+        // block k of the node (a valid block when k is past the end; weight zero)
+        const uint32_t kk = min((uint32_t)k, max(nb[i], 1u) - 1);
+        const T* a0 = val + rbase + 9 * (size_t)b0[i] + 3 * kk;
+        const T* a1 = a0 + 3 * nb[i];
+        const T* a2 = a1 + 3 * nb[i];
+        T a[9];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            a[c] = __builtin_nontemporal_load(a0 + c);
+            a[3 + c] = __builtin_nontemporal_load(a1 + c);
+            a[6 + c] = __builtin_nontemporal_load(a2 + c);
+        }
+        const T* xm = x + 3 * (size_t)__builtin_nontemporal_load(ncol + b0[i] + kk);
+        // a weight rather than a select keeps the gather unconditional (a
+        // load under a condition becomes a branch with a full wait)
+        const T w = (uint32_t)k < nb[i] ? (T)1 : (T)0;
+        const T x0 = xm[0], x1 = xm[1], x2 = xm[2];
+        s[i][0] = (a[0] * x0 + a[1] * x1 + a[2] * x2) * w;
+        s[i][1] = (a[3] * x0 + a[4] * x1 + a[5] * x2) * w;
+        s[i][2] = (a[6] * x0 + a[7] * x1 + a[8] * x2) * w;
+    }
+    // nodes with more than 16 blocks
+#pragma unroll
+    for (int i = 0; i < G; ++i)
+        for (uint32_t kk = k + kNodeLanes; kk < nb[i]; kk += kNodeLanes) {
+            const T* a0 = val + rbase + 9 * (size_t)b0[i] + 3 * kk;
+            const T* a1 = a0 + 3 * nb[i];
+            const T* a2 = a1 + 3 * nb[i];
+            const T* xm = x + 3 * (size_t)ncol[b0[i] + kk];
+            const T x0 = xm[0], x1 = xm[1], x2 = xm[2];
+            s[i][0] += a0[0] * x0 + a0[1] * x1 + a0[2] * x2;
+            s[i][1] += a1[0] * x0 + a1[1] * x1 + a1[2] * x2;
+            s[i][2] += a2[0] * x0 + a2[1] * x1 + a2[2] * x2;
+        }
+    double part = 0.0;
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+        const T t0 = row16_sum(s[i][0]), t1 = row16_sum(s[i][1]), t2 = row16_sum(s[i][2]);
+        if (node[i] < nn && k < 3) {
+            const uint32_t row = 3 * node[i] + k;
+            const T sum = k == 0 ? t0 : (k == 1 ? t1 : t2);
+            y[row] = sum;
+            if (sent0) {
+                sent0[row] = sentinel_value<T>();
+                sent1[row] = sentinel_value<T>();
+            }
+            if (DOT) part += (double)sum * (double)x[row];
+        }
     }
     if (DOT) {
         const double bs = block_sum(part);
@@ -1389,8 +1495,9 @@ static bool spmv_use_node(const Matrix& A) {
     return A.node_spmv && !csr_only;
 }
 
-uint32_t spmv_blocks(const Matrix& A) {
-    if (spmv_use_node(A)) return cdiv(A.n / 3, 256 / kNodeLanes);
+uint32_t spmv_blocks(const Matrix& A, int prec) {
+    if (spmv_use_node(A))
+        return cdiv(A.n / 3, 256 / kNodeLanes * (prec == PREC64 ? nodes_per_group<double>() : nodes_per_group<float>()));
     return spmv_use_stream(A) ? cdiv(A.n, kTR) : cdiv(A.n, 256 / spmv_lpr(A));
 }
 
@@ -1398,7 +1505,7 @@ template <typename T>
 static void spmv_t(hipStream_t s, const Matrix& A, const T* val, const T* x, T* y, bool dot, double* partials,
                    T* s0, T* s1, const PcgState* st) {
     if (spmv_use_node(A)) {
-        const uint32_t nn = A.n / 3, nb = cdiv(nn, 256 / kNodeLanes);
+        const uint32_t nn = A.n / 3, nb = cdiv(nn, 256 / kNodeLanes * nodes_per_group<T>());
         if (dot)
             spmv_node_kernel<T, true><<<nb, 256, 0, s>>>(nn, A.row_ptr.p, A.nptr.p, A.ncol.p, val, x, y, partials, s0,
                                                          s1, st);
