@@ -33,9 +33,9 @@ extern "C" {
 
 /* precision of the linear algebra: FM_FP64 = plain FP64 drop-in path;
    FM_MIXED = FP32 assembly/SpMV/ILU(0)/PCG under an FP64 Newton residual;
-   FM_MIXED_PC = FP64 tangent, SpMV and PCG vectors with the FP32 ILU(0)
-   factor of the FP32-rounded tangent as preconditioner (for tangents whose
-   condition number makes FP32 directions inaccurate: slender bodies) */
+   FM_MIXED_PC = FP64 tangent, SpMV and PCG vectors with the FP64 ILU(0)
+   factor rounded to FP32 and applied in FP32 as preconditioner (for tangents
+   whose condition number makes FP32 directions inaccurate: slender bodies) */
 #define FM_FP64 0
 #define FM_MIXED 1
 #define FM_MIXED_PC 2
