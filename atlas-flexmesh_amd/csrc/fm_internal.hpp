@@ -41,6 +41,8 @@ struct DevBuf {
 
 // Scalars of one PCG solve, kept on the device so that no host round trip is
 // needed between iterations.  Kernels read `done` and return at once when set.
+constexpr int kReduceBlocks = 64;  // most blocks of one reduce (each sums a contiguous chunk of partials)
+
 struct PcgState {
     double rz, pAp, alpha, beta, bb, rz_new, relres, tol;
     double slot[4];
@@ -49,6 +51,8 @@ struct PcgState {
     int iters;
     int max_it;
     uint32_t ticket[4];  // work-queue counters of the triangular-solve kernels
+    uint32_t red_count;  // blocks of a multi-block reduce that have stored their sum (0 between reduces)
+    double red_stage[kReduceBlocks];
 };
 
 // Level schedule of the natural-order ILU(0) dependence graph on groups of B

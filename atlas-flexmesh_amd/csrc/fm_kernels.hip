@@ -440,6 +440,16 @@ __device__ __forceinline__ T row16_sum(T v) {
     return v;
 }
 
+#ifndef FM_SPMV_NT
+#define FM_SPMV_NT 1
+#endif
+// matrix values and column nodes are streamed once per product
+template <typename T>
+__device__ __forceinline__ T spmv_stream_load(const T* p) {
+    if constexpr (FM_SPMV_NT) return __builtin_nontemporal_load(p);
+    return *p;
+}
+
 template <typename T, bool DOT>
 __global__ __launch_bounds__(256) void spmv_node_kernel(uint32_t nn, const uint32_t* __restrict__ rp,
                                                         const uint32_t* __restrict__ nptr,
@@ -520,11 +530,11 @@ __global__ __launch_bounds__(256) void spmv_node_kernel(uint32_t nn, const uint3
         T a[9];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            a[c] = __builtin_nontemporal_load(a0 + c);
-            a[3 + c] = __builtin_nontemporal_load(a1 + c);
-            a[6 + c] = __builtin_nontemporal_load(a2 + c);
+            a[c] = spmv_stream_load(a0 + c);
+            a[3 + c] = spmv_stream_load(a1 + c);
+            a[6 + c] = spmv_stream_load(a2 + c);
         }
-        const T* xm = x + 3 * (size_t)__builtin_nontemporal_load(ncol + b0[i] + kk);
+        const T* xm = x + 3 * (size_t)spmv_stream_load(ncol + b0[i] + kk);
         // a weight rather than a select keeps the gather unconditional (a
         // load under a condition becomes a branch with a full wait)
         const T w = (uint32_t)k < nb[i] ? (T)1 : (T)0;
@@ -933,18 +943,40 @@ __global__ __launch_bounds__(64) void ilu_bwd_kernel(uint32_t nq, const uint32_t
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void reduce_kernel(const double* __restrict__ partials, uint32_t np,
                                                       PcgState* st, int op, int k) {
-    // eight independent accumulators per thread (loads in flight), combined
-    // in a fixed order: deterministic for a given partial count
+    // Block b sums the contiguous chunk b of the partials with eight
+    // independent accumulators per thread (loads in flight).  With several
+    // blocks, each stores its sum, and the last block to finish (counted
+    // with an atomic) adds the block sums in block order.  The result is
+    // deterministic for a given partial count and grid.
+    const uint32_t chunk = (np + gridDim.x - 1) / gridDim.x;
+    const uint32_t lo = blockIdx.x * chunk, hi = min(np, lo + chunk);
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const uint32_t stride = blockDim.x * 8;
-    for (uint32_t i0 = threadIdx.x; i0 < np; i0 += stride)
+    for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += stride)
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const uint32_t i = i0 + u * blockDim.x;
-            if (i < np) acc[u] += partials[i];
+            if (i < hi) acc[u] += partials[i];
         }
     double s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     s = block_sum(s);
+    if (gridDim.x > 1) {
+        __shared__ int s_last;
+        if (threadIdx.x == 0) {
+            st->red_stage[blockIdx.x] = s;
+            __threadfence();
+            s_last = atomicAdd(&st->red_count, 1u) == gridDim.x - 1;
+        }
+        __syncthreads();
+        if (!s_last) return;
+        __threadfence();
+        const double v = threadIdx.x < gridDim.x
+                             ? __hip_atomic_load(&st->red_stage[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                             : 0.0;
+        __syncthreads();  // block_sum's shared slots are reused
+        s = block_sum(v);
+        if (threadIdx.x == 0) st->red_count = 0;
+    }
     if (threadIdx.x != 0) return;
     switch (op) {
         case 0:
@@ -1466,7 +1498,9 @@ uint32_t launch_dot64(hipStream_t s, uint32_t n, const double* a, const double* 
 }
 
 void launch_reduce(hipStream_t s, const double* partials, uint32_t np, PcgState* st, int op, int k) {
-    reduce_kernel<<<1, 1024, 0, s>>>(partials, np, st, op, k);
+    // one block per 8192 partials (one round of loads per thread), at most kReduceBlocks
+    const uint32_t nb = std::max(1u, std::min<uint32_t>(kReduceBlocks, (np + 8191) / 8192));
+    reduce_kernel<<<nb, 1024, 0, s>>>(partials, np, st, op, k);
 }
 
 static int spmv_lpr(const Matrix& A) {
