@@ -8,7 +8,9 @@
 
 namespace fm {
 
-constexpr int kChainMaxBlk = 8;               // dependency node blocks per node and direction
+constexpr int kChainMaxBlk = 8;               // block slots per node (two per lane of a quad)
+constexpr int kChainDepBlk = kChainMaxBlk - 1;  // dependency blocks per node and direction: the last
+                                                // slot carries the node's diagonal block
 constexpr int kChainRing = 8;                 // LDS ring depth in steps
 // largest published vector (bytes) the sweep kernels address: their buffer
 // offsets are 32-bit, and offsets from 2^31 up mark lanes with no access

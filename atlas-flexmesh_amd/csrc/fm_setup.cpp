@@ -312,7 +312,7 @@ bool chain_direction(const std::vector<uint32_t>& rp, const std::vector<uint32_t
         }
         slot = v;
         dep_blocks(rp, ci, v, nlow[v], bwd, deps, cnt);
-        if (cnt > (uint32_t)kChainMaxBlk) {
+        if (cnt > (uint32_t)kChainDepBlk) {
             why = "too many dependency blocks";
             return false;
         }
@@ -365,8 +365,8 @@ bool plan_chain_schedule(const std::vector<uint32_t>& rp, const std::vector<uint
             return false;
         }
         const uint32_t nl = (k - b) / 3, nu = (e - b) / 3 - nl - 1;
-        if (nl > (uint32_t)kChainMaxBlk || nu > (uint32_t)kChainMaxBlk) {
-            why = "more than 8 dependency blocks in a row";
+        if (nl > (uint32_t)kChainDepBlk || nu > (uint32_t)kChainDepBlk) {
+            why = "more than 7 dependency blocks in a row";
             return false;
         }
         nlow[v] = (uint16_t)nl;

@@ -10,9 +10,11 @@
 // costs no wait.
 //
 // A node belongs to one hardware quad (4 lanes).  Lane gl holds dependency
-// blocks gl and gl + 4 (3x3 each); the three row sums are reduced across the
-// quad and broadcast inside it with DPP quad permutes, which stay in the
-// VALU (no LDS round trip) on the step's critical path.
+// blocks gl and gl + 4 (3x3 each); the slot of block 7 (lane 3) carries the
+// node's diagonal block instead, so a node has at most 7 dependency blocks
+// per direction.  The three row sums are reduced across the quad and the
+// diagonal block is broadcast inside it with DPP quad permutes, which stay
+// in the VALU (no LDS round trip) on the step's critical path.
 //
 // One wave issues about one instruction per four cycles, and a bundle's steps
 // are serial, so the step's instruction count sets the sweep's pace.  The
@@ -24,8 +26,9 @@
 // through per-bundle buffer descriptors with the step in a scalar offset.
 //
 // The factor values are repacked after each factorization into a step-major
-// layout: the payload of one lane in one step is NC 16-byte chunks, stored so
-// that each chunk index of a wave-step is one coalesced 1 KiB access.
+// layout: the payload of one lane in one step is 18 values in 16-byte chunks
+// (FP32: four and a half), stored so that each chunk index of a wave-step is
+// one coalesced access of 64 lanes.
 //
 // The workgroup is a single wave, so no barrier is needed between steps: LDS
 // instructions of one wave execute in order, and the ring accesses use
@@ -56,30 +59,24 @@ constexpr int kRingStride = 4;             // values per LDS ring slot (3 used)
 static_assert(kZeroSlot == (uint32_t)(kChainRing * kNPW), "the zero slot follows the ring");
 static_assert(kBPL == 2, "the step record holds two dependency blocks per lane");
 
-template <typename T>
-struct Chunk;
-template <>
-struct Chunk<float> {
-    static constexpr int W = 4;
-};
-template <>
-struct Chunk<double> {
-    static constexpr int W = 2;
-};
-
+// payload of one lane and step: two 3x3 blocks; lane 3's second block slot
+// holds the diagonal block (forward: l10, l20, l21; backward: 1/u00, u01,
+// u02, 1/u11, u12, 1/u22)
 template <typename T>
 struct Shape {
-    static constexpr int W = Chunk<T>::W;
-    static constexpr int DG = kBPL * 9;                // two diagonal-block slots follow the blocks
-    static constexpr int NV = (DG + 2 + W - 1) / W * W;
-    static constexpr int NC = NV / W;                  // 16-byte chunks per lane and step
+    static constexpr int NV = kBPL * 9;
+    static constexpr int DG = 9;                         // diagonal values in lane 3, from this index
+    static constexpr int W = 16 / (int)sizeof(T);        // values per 16-byte chunk
+    static constexpr int NC = NV / W;                    // full chunks per lane and step
+    static constexpr int TAIL = NV - NC * W;             // values in the last, partial chunk
+    static constexpr uint32_t STEP = 64u * NV * sizeof(T);  // bytes per wave-step
 };
 
 // DPP quad permutes: xor 1, xor 2, broadcast of quad lane k
 constexpr int qp(int a, int b, int c, int d) { return a | (b << 2) | (c << 4) | (d << 6); }
 constexpr int kXor1 = qp(1, 0, 3, 2);
 constexpr int kXor2 = qp(2, 3, 0, 1);
-constexpr int kBc0 = qp(0, 0, 0, 0), kBc1 = qp(1, 1, 1, 1), kBc2 = qp(2, 2, 2, 2);
+constexpr int kBc0 = qp(0, 0, 0, 0), kBc3 = qp(3, 3, 3, 3);
 
 template <int CTRL>
 __device__ __forceinline__ float quad(float v) {
@@ -99,11 +96,13 @@ __device__ __forceinline__ double quad(double v) {
 // LLVM's raw buffer intrinsics, declared directly: the clang builtins for
 // loads wider than one dword return the first dword splatted in this
 // toolchain.
+typedef float fm_f2 __attribute__((ext_vector_type(2)));
 typedef float fm_f3 __attribute__((ext_vector_type(3)));
 typedef float fm_f4 __attribute__((ext_vector_type(4)));
 typedef double fm_d2 __attribute__((ext_vector_type(2)));
 typedef unsigned int fm_u4 __attribute__((ext_vector_type(4)));
 typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ fm_f2 buf_load_f2(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f32");
 __device__ fm_f3 buf_load_f3(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v3f32");
 __device__ fm_f4 buf_load_f4(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v4f32");
 __device__ fm_d2 buf_load_d2(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f64");
@@ -233,7 +232,7 @@ __device__ __forceinline__ BundleRsrc bundle_rsrc(const SweepArgs<T>& A, uint32_
     using S = Shape<T>;
     BundleRsrc R;
     R.far = make_rsrc(reinterpret_cast<const char*>(A.far) + (size_t)q0 * 1024, ns * 1024u);
-    R.pk = make_rsrc(reinterpret_cast<const char*>(A.pk) + (size_t)q0 * S::NC * 1024, ns * S::NC * 1024u);
+    R.pk = make_rsrc(reinterpret_cast<const char*>(A.pk) + (size_t)q0 * S::STEP, ns * S::STEP);
     return R;
 }
 
@@ -252,9 +251,10 @@ __device__ __forceinline__ void near_load(const SweepArgs<T>& A, const BundleRsr
                                           const Far& F, Near<T>& N) {
     using S = Shape<T>;
     N.f = F;
+    const int soff = (int)(s * S::STEP);
 #pragma unroll
     for (int c = 0; c < S::NC; ++c) {
-        const int voff = lane * 16 + c * 1024, soff = (int)(s * (S::NC * 1024));
+        const int voff = lane * 16 + c * 1024;
         if constexpr (sizeof(T) == 4) {
             const fm_f4 w = buf_load_f4(R.pk, voff, soff, FM_PK_AUX);
             N.v[c * 4 + 0] = w.x;
@@ -266,6 +266,12 @@ __device__ __forceinline__ void near_load(const SweepArgs<T>& A, const BundleRsr
             N.v[c * 2 + 0] = w.x;
             N.v[c * 2 + 1] = w.y;
         }
+    }
+    static_assert(S::TAIL == 0 || (S::TAIL == 2 && sizeof(T) == 4), "payload tail layout");
+    if constexpr (S::TAIL == 2 && sizeof(T) == 4) {
+        const fm_f2 w = buf_load_f2(R.pk, lane * 8 + S::NC * 1024, soff, FM_PK_AUX);
+        N.v[S::NC * 4 + 0] = w.x;
+        N.v[S::NC * 4 + 1] = w.y;
     }
     load3<0>(make_rsrc(A.in0, A.out_bytes), scaled<T>(F.pub), N.vec);
     if (BWD) load3<0>(make_rsrc(A.in1, A.out_bytes), scaled<T>(F.pub), N.vec1);
@@ -359,11 +365,10 @@ __device__ __forceinline__ void process(const SweepArgs<T>& A, const Near<T>& N,
     T b[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) b[d] = quad<kBc0>(N.vec[d]);
-    const T d0 = N.v[S::DG], d1 = N.v[S::DG + 1];
     T v0, v1, v2;
     if (!BWD) {
-        // unit lower diagonal block: l10, l20 in quad lane 0, l21 in lane 1
-        const T l10 = quad<kBc0>(d0), l20 = quad<kBc0>(d1), l21 = quad<kBc1>(d0);
+        // unit lower diagonal block (l10, l20, l21 in quad lane 3)
+        const T l10 = quad<kBc3>(N.v[S::DG]), l20 = quad<kBc3>(N.v[S::DG + 1]), l21 = quad<kBc3>(N.v[S::DG + 2]);
         v0 = b[0] - part[0];
         v1 = b[1] - part[1];
         v1 -= l10 * v0;
@@ -371,10 +376,9 @@ __device__ __forceinline__ void process(const SweepArgs<T>& A, const Near<T>& N,
         v2 -= l20 * v0;
         v2 -= l21 * v1;
     } else {
-        // upper diagonal block with reciprocal pivots: (1/u00, u01) lane 0,
-        // (u02, 1/u11) lane 1, (u12, 1/u22) lane 2
-        const T r00 = quad<kBc0>(d0), u01 = quad<kBc0>(d1), u02 = quad<kBc1>(d0), r11 = quad<kBc1>(d1),
-                u12 = quad<kBc2>(d0), r22 = quad<kBc2>(d1);
+        // upper diagonal block with reciprocal pivots (quad lane 3)
+        const T r00 = quad<kBc3>(N.v[S::DG]), u01 = quad<kBc3>(N.v[S::DG + 1]), u02 = quad<kBc3>(N.v[S::DG + 2]),
+                r11 = quad<kBc3>(N.v[S::DG + 3]), u12 = quad<kBc3>(N.v[S::DG + 4]), r22 = quad<kBc3>(N.v[S::DG + 5]);
         v2 = (b[2] - part[2]) * r22;
         T t1 = b[1] - part[1];
         t1 -= u12 * v2;
@@ -463,9 +467,10 @@ __global__ __launch_bounds__(64) void chain_sweep_kernel(SweepArgs<T> A) {
 
 // Repack of the CSR factor into the step-major payload of one sweep: one
 // thread per (wave-step, lane).  Forward: lower blocks k (node-block position
-// k of the row), unit lower diagonal block (l10, l20 | l21).  Backward: upper
+// k of the row), unit lower diagonal block (l10, l20, l21).  Backward: upper
 // blocks k (position nlow + 1 + k), upper diagonal block with reciprocal
-// pivots, rounded from FP64 (1/u00, u01 | u02, 1/u11 | u12, 1/u22).
+// pivots, rounded from FP64 (1/u00, u01, u02, 1/u11, u12, 1/u22).  The
+// diagonal block goes to lane 3's second block slot (block 7 never exists).
 template <typename T, bool BWD>
 __global__ __launch_bounds__(256) void chain_pack_kernel(uint32_t nq, const uint4* __restrict__ far,
                                                          const uint32_t* __restrict__ rp,
@@ -489,7 +494,7 @@ __global__ __launch_bounds__(256) void chain_pack_kernel(uint32_t nq, const uint
         const uint32_t nup = nblk - nl - 1;
 #pragma unroll
         for (int j = 0; j < kBPL; ++j) {
-            const uint32_t k = gl + j * kLPN;
+            const uint32_t k = gl + j * kLPN;  // block 7 never exists: at most kChainDepBlk
             if (k >= (BWD ? nup : nl)) continue;
             const uint32_t pos = BWD ? nl + 1 + k : k;
 #pragma unroll
@@ -498,31 +503,30 @@ __global__ __launch_bounds__(256) void chain_pack_kernel(uint32_t nq, const uint
                 for (int c = 0; c < 3; ++c) val[j * 9 + d * 3 + c] = LU[rs[d] + 3 * pos + c];
         }
         const uint32_t o = 3 * nl;
-        if (!BWD) {
-            if (gl == 0) {
+        if (gl == 3) {
+            if (!BWD) {
                 val[S::DG] = LU[rs[1] + o + 0];
                 val[S::DG + 1] = LU[rs[2] + o + 0];
-            } else if (gl == 1) {
-                val[S::DG] = LU[rs[2] + o + 1];
-            }
-        } else {
-            if (gl == 0) {
+                val[S::DG + 2] = LU[rs[2] + o + 1];
+            } else {
                 val[S::DG] = (T)(1.0 / (double)LU[rs[0] + o + 0]);
                 val[S::DG + 1] = LU[rs[0] + o + 1];
-            } else if (gl == 1) {
-                val[S::DG] = LU[rs[0] + o + 2];
-                val[S::DG + 1] = (T)(1.0 / (double)LU[rs[1] + o + 1]);
-            } else if (gl == 2) {
-                val[S::DG] = LU[rs[1] + o + 2];
-                val[S::DG + 1] = (T)(1.0 / (double)LU[rs[2] + o + 2]);
+                val[S::DG + 2] = LU[rs[0] + o + 2];
+                val[S::DG + 3] = (T)(1.0 / (double)LU[rs[1] + o + 1]);
+                val[S::DG + 4] = LU[rs[1] + o + 2];
+                val[S::DG + 5] = (T)(1.0 / (double)LU[rs[2] + o + 2]);
             }
         }
     }
-    T* dst = pk + ((size_t)q * S::NC * 64 + lane) * S::W;
+    char* step = reinterpret_cast<char*>(pk) + (size_t)q * S::STEP;
 #pragma unroll
     for (int c = 0; c < S::NC; ++c)
 #pragma unroll
-        for (int k = 0; k < S::W; ++k) dst[(size_t)c * 64 * S::W + k] = val[c * S::W + k];
+        for (int k = 0; k < S::W; ++k)
+            reinterpret_cast<T*>(step + c * 1024 + lane * 16)[k] = val[c * S::W + k];
+#pragma unroll
+    for (int k = 0; k < S::TAIL; ++k)
+        reinterpret_cast<T*>(step + S::NC * 1024 + lane * S::TAIL * sizeof(T))[k] = val[S::NC * S::W + k];
 }
 
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -571,13 +575,13 @@ void pack_t(hipStream_t s, const Matrix& A, const T* lu, T* pf, T* pb) {
 int chain_pack(hipStream_t s, int prec, Matrix& A) {
     ChainSchedule& C = A.chain;
     if (prec == PREC64) {
-        const size_t nv = (size_t)Shape<double>::NV * 64;
+        const size_t nv = Shape<double>::STEP / sizeof(double);
         if (C.fwd.pk64.alloc((size_t)C.fwd.nq * nv) != hipSuccess ||
             C.bwd.pk64.alloc((size_t)C.bwd.nq * nv) != hipSuccess)
             return FM_EDEVICE;
         pack_t<double>(s, A, A.lu64.p, C.fwd.pk64.p, C.bwd.pk64.p);
     } else {
-        const size_t nv = (size_t)Shape<float>::NV * 64;
+        const size_t nv = Shape<float>::STEP / sizeof(float);
         if (C.fwd.pk32.alloc((size_t)C.fwd.nq * nv) != hipSuccess ||
             C.bwd.pk32.alloc((size_t)C.bwd.nq * nv) != hipSuccess)
             return FM_EDEVICE;
