@@ -82,7 +82,7 @@ struct ChainSweep {
     uint32_t nb = 0;                 // bundles, in processing order
     uint32_t nq = 0;                 // wave-steps over all bundles
     DevBuf<uint32_t> bstart, bsteps; // per bundle: first wave-step, number of steps
-    DevBuf<uint32_t> far;            // nq * 64 * 4: per (wave-step, lane) node of the lane's quad
+    DevBuf<uint32_t> far;            // nq * kRecWords: step records (fm_plan.hpp)
                                      // (kDepNone: idle), dependency words 0 and 1, zero
     DevBuf<float> pk32;              // packed factor values of this sweep
     DevBuf<double> pk64;

@@ -18,16 +18,17 @@ constexpr uint32_t kChainMaxBytes = 0x80000000u;
 constexpr uint32_t kDepNone = 0xFFFFFFFFu;    // no dependency in this lane / idle lane group
 constexpr uint32_t kDepLocal = 0x80000000u;   // | LDS slot: dependency inside the bundle
 
-// Device form of one wave-step, one 16-byte record per lane (lpn 4):
-//   x    byte offset of the quad's node triple in an FP32 vector (12 * node),
-//        kOffNone for an idle quad
-//   y, z byte offsets (FP32) of the nodes of the lane's dependency blocks 0
-//        and 1 when they are external, else kOffNone
-//   w    LDS ring slots of blocks 0 and 1 (low and high 16 bits) when they
-//        are local, else kZeroSlot (a slot that holds zeros)
+// Device form of one wave-step (lpn 4), kRecWords 32-bit words:
+//   word 2 * lane + j  dependency block j of the lane: the byte offset of an
+//                      external dependency's node triple in an FP32 vector
+//                      (12 * node, below 2^31), or kDepLocal | LDS ring slot
+//                      (kZeroSlot, a slot that holds zeros, when absent)
+//   word 128 + g       byte offset of quad g's node triple (12 * node),
+//                      kOffNone for an idle quad
 // FP64 kernels double the offsets; kOffNone stays past every buffer either way.
 constexpr uint32_t kOffNone = 0xFFFFFFF0u;
 constexpr uint32_t kZeroSlot = kChainRing * 16;  // the slot after the ring (kChainRing steps x 16 quads)
+constexpr uint32_t kRecWords = 128 + 16;
 
 // One sweep direction.  Wave-step q of bundle b is bstart[b] + s for step s <
 // bsteps[b]; bundles are listed in processing order.

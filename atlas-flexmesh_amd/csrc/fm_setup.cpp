@@ -449,20 +449,19 @@ bool build_chain_schedule(const std::vector<uint32_t>& rp, const std::vector<uin
     auto upload = [&](const ChainPlanDir& H, ChainSweep& W) {
         W.nb = nbund;
         W.nq = H.nq;
-        far.assign((size_t)H.nq * 64 * 4, 0);
-        for (size_t q = 0; q < H.nq; ++q)
-            for (int lane = 0; lane < 64; ++lane) {
-                uint32_t* f = &far[(q * 64 + lane) * 4];
-                const uint32_t nd = H.node[q * npw + lane / P.lpn];
-                f[0] = nd == kDepNone ? kOffNone : 12u * nd;
-                f[3] = 0;
+        far.assign((size_t)H.nq * kRecWords, 0);
+        for (size_t q = 0; q < H.nq; ++q) {
+            uint32_t* f = &far[q * kRecWords];
+            for (int g = 0; g < npw; ++g) {
+                const uint32_t nd = H.node[q * npw + g];
+                f[128 + g] = nd == kDepNone ? kOffNone : 12u * nd;
+            }
+            for (int lane = 0; lane < 64; ++lane)
                 for (int j = 0; j < bpl; ++j) {
                     const uint32_t dw = H.dep[(q * bpl + j) * 64 + lane];
-                    const bool local = dw != kDepNone && (dw & kDepLocal);
-                    f[1 + j] = (dw == kDepNone || local) ? kOffNone : 12u * dw;
-                    f[3] |= (local ? (dw & ~kDepLocal) : kZeroSlot) << (16 * j);
+                    f[2 * lane + j] = dw == kDepNone ? kDepLocal | kZeroSlot : (dw & kDepLocal) ? dw : 12u * dw;
                 }
-            }
+        }
         if (W.bstart.alloc(nbund) != hipSuccess || W.bsteps.alloc(nbund) != hipSuccess ||
             W.far.alloc(far.size()) != hipSuccess)
             return false;

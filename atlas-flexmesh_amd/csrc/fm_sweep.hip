@@ -100,14 +100,15 @@ typedef float fm_f2 __attribute__((ext_vector_type(2)));
 typedef float fm_f3 __attribute__((ext_vector_type(3)));
 typedef float fm_f4 __attribute__((ext_vector_type(4)));
 typedef double fm_d2 __attribute__((ext_vector_type(2)));
-typedef unsigned int fm_u4 __attribute__((ext_vector_type(4)));
+typedef unsigned int fm_u2 __attribute__((ext_vector_type(2)));
 typedef __amdgpu_buffer_rsrc_t Rsrc;
 __device__ fm_f2 buf_load_f2(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f32");
 __device__ fm_f3 buf_load_f3(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v3f32");
 __device__ fm_f4 buf_load_f4(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v4f32");
 __device__ fm_d2 buf_load_d2(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f64");
 __device__ double buf_load_d1(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.f64");
-__device__ fm_u4 buf_load_u4(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v4i32");
+__device__ fm_u2 buf_load_u2(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2i32");
+__device__ uint32_t buf_load_u1(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.i32");
 __device__ void buf_store_u32(uint32_t, Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.store.i32");
 __device__ void buf_store_u64(unsigned long long, Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.store.i64");
 
@@ -181,7 +182,7 @@ struct SweepArgs {
     uint32_t nb;
     const uint32_t* bstart;
     const uint32_t* bsteps;
-    const uint4* far;  // step records (fm_plan.hpp), 64 per wave-step
+    const uint32_t* far;  // step records (fm_plan.hpp), kRecWords words per wave-step
     const T* pk;
     const T* in0;  // forward: rhs; backward: z
     const T* in1;  // backward: r for the r.y partials (else in0)
@@ -193,12 +194,15 @@ struct SweepArgs {
     int* err;
 };
 
-// step record (2D steps ahead)
+// step record of a lane (2D steps ahead)
 struct Far {
     uint32_t pub;        // offset of the quad's node triple
-    uint32_t ext[kBPL];  // offsets of the external dependency nodes
-    uint32_t lds;        // ring slots of the dependency blocks
+    uint32_t dep[kBPL];  // dependency blocks: external node offset or kDepLocal | ring slot
 };
+// an external dependency's offset (kOffNone for a local or absent block)
+__device__ __forceinline__ uint32_t ext_off(uint32_t w) { return (int32_t)w >= 0 ? w : kOffNone; }
+// a local dependency's ring slot (kZeroSlot for an external or absent block)
+__device__ __forceinline__ uint32_t ring_slot(uint32_t w) { return (int32_t)w >= 0 ? kZeroSlot : (w & 0xFFFFu); }
 
 // step data (D steps ahead)
 template <typename T>
@@ -231,17 +235,16 @@ template <typename T>
 __device__ __forceinline__ BundleRsrc bundle_rsrc(const SweepArgs<T>& A, uint32_t q0, uint32_t ns) {
     using S = Shape<T>;
     BundleRsrc R;
-    R.far = make_rsrc(reinterpret_cast<const char*>(A.far) + (size_t)q0 * 1024, ns * 1024u);
+    R.far = make_rsrc(A.far + (size_t)q0 * kRecWords, ns * kRecWords * 4u);
     R.pk = make_rsrc(reinterpret_cast<const char*>(A.pk) + (size_t)q0 * S::STEP, ns * S::STEP);
     return R;
 }
 
 __device__ __forceinline__ void far_load(const BundleRsrc& R, uint32_t s, int lane, Far& F) {
-    const fm_u4 w = buf_load_u4(R.far, lane * 16, (int)(s * 1024), FM_PK_AUX);
-    F.pub = w.x;
-    F.ext[0] = w.y;
-    F.ext[1] = w.z;
-    F.lds = w.w;
+    const fm_u2 w = buf_load_u2(R.far, lane * 8, (int)(s * kRecWords * 4), FM_PK_AUX);
+    F.dep[0] = w.x;
+    F.dep[1] = w.y;
+    F.pub = buf_load_u1(R.far, 512 + (lane >> 2) * 4, (int)(s * kRecWords * 4), FM_PK_AUX);
 }
 
 // every load issues: idle quads read their zero payload, absent dependencies
@@ -281,7 +284,7 @@ template <typename T>
 __device__ __forceinline__ void poll_ext(const SweepArgs<T>& A, const Far& F, Ext<T>& X) {
     const Rsrc ro = make_rsrc(A.out, A.out_bytes);
 #pragma unroll
-    for (int j = 0; j < kBPL; ++j) load3<kSc1>(ro, scaled<T>(F.ext[j]), X.v[j]);
+    for (int j = 0; j < kBPL; ++j) load3<kSc1>(ro, scaled<T>(ext_off(F.dep[j])), X.v[j]);
 }
 
 // Re-poll of the external dependency values that were still the sentinel
@@ -304,7 +307,7 @@ __device__ __forceinline__ void await_late(const SweepArgs<T>& A, const Far& f, 
                 need[j][c] = is_sentinel(e[j][c]);  // values of absent blocks are zeros
                 any |= need[j][c];
             }
-            load3<kSc1>(make_rsrc(A.out, A.out_bytes), any ? scaled<T>(f.ext[j]) : kOob, z[j]);
+            load3<kSc1>(make_rsrc(A.out, A.out_bytes), any ? scaled<T>(ext_off(f.dep[j])) : kOob, z[j]);
         }
         bool pending = false;
 #pragma unroll
@@ -345,7 +348,7 @@ __device__ __forceinline__ void process(const SweepArgs<T>& A, const Near<T>& N,
     T y[kBPL][3];
 #pragma unroll
     for (int j = 0; j < kBPL; ++j) {
-        const T* src = ring + ((N.f.lds >> (16 * j)) & 0xFFFFu) * kRingStride;
+        const T* src = ring + ring_slot(N.f.dep[j]) * kRingStride;
 #pragma unroll
         for (int c = 0; c < 3; ++c) y[j][c] = src[c] + e[j][c];
     }
@@ -472,7 +475,7 @@ __global__ __launch_bounds__(64) void chain_sweep_kernel(SweepArgs<T> A) {
 // pivots, rounded from FP64 (1/u00, u01, u02, 1/u11, u12, 1/u22).  The
 // diagonal block goes to lane 3's second block slot (block 7 never exists).
 template <typename T, bool BWD>
-__global__ __launch_bounds__(256) void chain_pack_kernel(uint32_t nq, const uint4* __restrict__ far,
+__global__ __launch_bounds__(256) void chain_pack_kernel(uint32_t nq, const uint32_t* __restrict__ far,
                                                          const uint32_t* __restrict__ rp,
                                                          const uint16_t* __restrict__ nlow, const T* __restrict__ LU,
                                                          T* __restrict__ pk) {
@@ -481,7 +484,7 @@ __global__ __launch_bounds__(256) void chain_pack_kernel(uint32_t nq, const uint
     if (t >= (uint64_t)nq * 64) return;
     const uint32_t q = (uint32_t)(t >> 6);
     const int lane = (int)(t & 63), gl = lane & (kLPN - 1);
-    const uint32_t x = far[t].x;
+    const uint32_t x = far[(size_t)q * kRecWords + 128 + lane / kLPN];
     T val[S::NV];
 #pragma unroll
     for (int k = 0; k < S::NV; ++k) val[k] = (T)0;
@@ -550,7 +553,7 @@ uint32_t launch_sweep(hipStream_t s, const Matrix& A, const ChainSweep& W, const
     a.nb = W.nb;
     a.bstart = W.bstart.p;
     a.bsteps = W.bsteps.p;
-    a.far = reinterpret_cast<const uint4*>(W.far.p);
+    a.far = W.far.p;
     a.pk = pk;
     auto kern = chain_sweep_kernel<T, depth<T>(), BWD>;
     const uint32_t grid = persistent_grid(reinterpret_cast<const void*>(kern), W.nb);
@@ -564,9 +567,9 @@ void pack_t(hipStream_t s, const Matrix& A, const T* lu, T* pf, T* pb) {
     const ChainSchedule& C = A.chain;
     const uint16_t* nlow = A.sched.nlow.p;
     chain_pack_kernel<T, false><<<cdiv((uint64_t)C.fwd.nq * 64, 256), 256, 0, s>>>(
-        C.fwd.nq, reinterpret_cast<const uint4*>(C.fwd.far.p), A.row_ptr.p, nlow, lu, pf);
+        C.fwd.nq, C.fwd.far.p, A.row_ptr.p, nlow, lu, pf);
     chain_pack_kernel<T, true><<<cdiv((uint64_t)C.bwd.nq * 64, 256), 256, 0, s>>>(
-        C.bwd.nq, reinterpret_cast<const uint4*>(C.bwd.far.p), A.row_ptr.p, nlow, lu, pb);
+        C.bwd.nq, C.bwd.far.p, A.row_ptr.p, nlow, lu, pb);
     record_launch_error(hipGetLastError());
 }
 
