@@ -402,11 +402,17 @@ __device__ __forceinline__ void process(const SweepArgs<T>& A, const Near<T>& N,
         dot += (double)N.vec1[0] * (double)v0 + (double)N.vec1[1] * (double)v1 + (double)N.vec1[2] * (double)v2;
 }
 
+// minimum waves per SIMD asked of the register allocator (1: no limit; the
+// FP64 sweeps would spill at 3)
+#ifndef FM_SWEEP_MINW
+#define FM_SWEEP_MINW 1
+#endif
+
 // Persistent: block b of the grid takes bundles b, b + P, ... in processing
 // order, which is a topological order of the bundle graph, so the earliest
 // unfinished bundle only ever waits on finished ones.
 template <typename T, int D, bool BWD>
-__global__ __launch_bounds__(64) void chain_sweep_kernel(SweepArgs<T> A) {
+__global__ __launch_bounds__(64, FM_SWEEP_MINW) void chain_sweep_kernel(SweepArgs<T> A) {
     if (A.check_done && A.st->done) return;
     // the ring, the zero slot, then one spare slot per lane for lanes with nothing to store
     __shared__ T ring[(kZeroSlot + 1) * kRingStride + 64];
