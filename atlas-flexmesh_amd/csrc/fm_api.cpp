@@ -505,6 +505,30 @@ struct PcgResult {
 
 // pcg_solve_ilu0 (FEM/adaptive_fem_engine.hpp:1283-1341) on device vectors
 // b -> x of the work set (x holds the initial guess).
+// PCG iterations in batches of kBatch with one batch of lookahead: batch b+1
+// is enqueued before the host waits for the state copied after batch b, so
+// the device does not idle at a host check.  Iterations enqueued past
+// convergence return at once (every kernel of the loop tests st->done).  On
+// return the final state is in h_st[0].
+template <typename F>
+int pcg_batches(fm_ctx* ctx, int max_it, F&& enqueue_one) {
+    hipStream_t s = ctx->stream;
+    int enq = 0;
+    for (int b = 0;; ++b) {
+        for (int k = 0; k < kBatch && enq < max_it; ++k, ++enq) enqueue_one();
+        FM_TRY(check_launch(ctx, "pcg iteration"));
+        FM_HIP(hipMemcpyAsync(ctx->h_st + 1 + (b & 1), ctx->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+        FM_HIP(hipEventRecord(ctx->batch_ev[b & 1], s));
+        if (b > 0) {
+            FM_HIP(hipEventSynchronize(ctx->batch_ev[(b - 1) & 1]));
+            if (ctx->h_st[1 + ((b - 1) & 1)].done) break;
+        }
+        if (enq >= max_it) break;
+    }
+    FM_HIP(hipMemcpyAsync(ctx->h_st, ctx->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    return sync(ctx);
+}
+
 int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult* res) {
     hipStream_t s = ctx->stream;
     const uint32_t n = A.n;
@@ -556,10 +580,7 @@ int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult*
     launch_copy(s, prec, n, y, p);
     FM_TRY(check_launch(ctx, "pcg init"));
     const void* lu = prec == PREC64 ? (const void*)A.lu64.p : (const void*)A.lu32.p;
-    int done = 0;
-    int enq = 0;
-    while (!done) {
-        for (int k = 0; k < kBatch && enq < max_it; ++k, ++enq) {
+    FM_TRY(pcg_batches(ctx, max_it, [&] {
             {
                 ProfScope ps(ctx, FM_K_SPMV);
                 launch_spmv(s, prec, A, p, Ap, true, ctx->partials.p, z, y, ctx->st.p);
@@ -589,12 +610,7 @@ int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult*
             }
             launch_reduce(s, ctx->partials.p, nbb, ctx->st.p, 2, 0);
             launch_pcg_p_update(s, prec, n, y, p, ctx->st.p);
-        }
-        FM_TRY(check_launch(ctx, "pcg iteration"));
-        FM_HIP(hipMemcpyAsync(ctx->h_st, ctx->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-        FM_TRY(sync(ctx));
-        done = ctx->h_st->done || enq >= max_it;
-    }
+    }));
     FM_TRY(check_device_error(ctx));
     const PcgState& h = *ctx->h_st;
     res->status = h.status;
@@ -687,9 +703,7 @@ int pcg_run_pc(fm_ctx* ctx, Matrix& A, double tol, int max_it, PcgResult* res) {
     launch_reduce(s, ctx->partials.p, np, ctx->st.p, 3, 0);
     launch_copy(s, PREC64, n, y, p);
     FM_TRY(check_launch(ctx, "pcg init"));
-    int done = 0, enq = 0;
-    while (!done) {
-        for (int k = 0; k < kBatch && enq < max_it; ++k, ++enq) {
+    FM_TRY(pcg_batches(ctx, max_it, [&] {
             {
                 ProfScope ps(ctx, FM_K_SPMV);
                 launch_spmv(s, PREC64, A, p, Ap, true, ctx->partials.p, nullptr, nullptr, ctx->st.p);
@@ -703,12 +717,7 @@ int pcg_run_pc(fm_ctx* ctx, Matrix& A, double tol, int max_it, PcgResult* res) {
             launch_reduce(s, old_part, np, ctx->st.p, 0, 3);
             launch_reduce(s, ctx->partials.p, np, ctx->st.p, 4, 3);
             launch_pcg_p_update(s, PREC64, n, y, p, ctx->st.p);
-        }
-        FM_TRY(check_launch(ctx, "pcg iteration"));
-        FM_HIP(hipMemcpyAsync(ctx->h_st, ctx->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-        FM_TRY(sync(ctx));
-        done = ctx->h_st->done || enq >= max_it;
-    }
+    }));
     FM_TRY(check_device_error(ctx));
     const PcgState& h = *ctx->h_st;
     res->status = h.status;
@@ -804,7 +813,9 @@ int fm_create(int device, fm_ctx** out) {
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         ctx->st.alloc(1) != hipSuccess || ctx->d_err.alloc(1) != hipSuccess ||
-        hipHostMalloc((void**)&ctx->h_st, sizeof(PcgState), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&ctx->h_st, 3 * sizeof(PcgState), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->batch_ev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->batch_ev[1], hipEventDisableTiming) != hipSuccess) {
         delete ctx;
         return FM_EHIP;
     }
@@ -820,6 +831,8 @@ int fm_destroy(fm_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     prof_flush(ctx);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+    for (auto e : ctx->batch_ev)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->h_st) (void)hipHostFree(ctx->h_st);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

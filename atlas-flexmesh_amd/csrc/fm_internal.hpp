@@ -169,7 +169,8 @@ struct Ctx {
     // scalars and reductions
     DevBuf<double> partials;
     DevBuf<PcgState> st;
-    PcgState* h_st = nullptr;  // pinned
+    PcgState* h_st = nullptr;  // pinned, 3 entries: [0] general use, [1..2] the PCG batch checks
+    hipEvent_t batch_ev[2] = {nullptr, nullptr};
     DevBuf<int> d_err;
 
     // profiling

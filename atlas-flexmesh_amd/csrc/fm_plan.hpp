@@ -53,6 +53,10 @@ struct ChainPlan {
 // Plan for a 3-row node-blocked CSR with sorted columns (only rows 3v are
 // read, so a compact view holding the node columns in row 3v suffices).
 // Returns false with the reason in `why` when the matrix does not fit.
+// Device form of one direction (lpn 4): kRecWords words per wave-step, as
+// described above.
+void encode_chain_records(const ChainPlanDir& H, int lpn, std::vector<uint32_t>& far);
+
 bool plan_chain_schedule(const std::vector<uint32_t>& rp, const std::vector<uint32_t>& ci, int lpn, ChainPlan& P,
                          std::string& why);
 

@@ -431,6 +431,23 @@ bool plan_chain_schedule(const std::vector<uint32_t>& rp, const std::vector<uint
     return true;
 }
 
+void encode_chain_records(const ChainPlanDir& H, int lpn, std::vector<uint32_t>& far) {
+    const int npw = 64 / lpn, bpl = kChainMaxBlk / lpn;
+    far.assign((size_t)H.nq * kRecWords, 0);
+    for (size_t q = 0; q < H.nq; ++q) {
+        uint32_t* f = &far[q * kRecWords];
+        for (int g = 0; g < npw; ++g) {
+            const uint32_t nd = H.node[q * npw + g];
+            f[128 + g] = nd == kDepNone ? kOffNone : 12u * nd;
+        }
+        for (int lane = 0; lane < 64; ++lane)
+            for (int j = 0; j < bpl; ++j) {
+                const uint32_t dw = H.dep[(q * bpl + j) * 64 + lane];
+                f[2 * lane + j] = dw == kDepNone ? kDepLocal | kZeroSlot : (dw & kDepLocal) ? dw : 12u * dw;
+            }
+    }
+}
+
 bool build_chain_schedule(const std::vector<uint32_t>& rp, const std::vector<uint32_t>& ci, int lpn,
                           ChainSchedule& C, hipStream_t stream, std::string& why, std::string& err) {
     C.ok = false;
@@ -443,25 +460,11 @@ bool build_chain_schedule(const std::vector<uint32_t>& rp, const std::vector<uin
     ChainPlan P;
     if (!plan_chain_schedule(rp, ci, lpn, P, why)) return false;
     const uint32_t nbund = P.fwd.nb;
-    // device form: one 16-byte record per (wave-step, lane), read in one load
-    const int npw = 64 / P.lpn, bpl = kChainMaxBlk / P.lpn;
-    std::vector<uint32_t> far;
+    std::vector<uint32_t> far;  // device form of the plan (encode_chain_records)
     auto upload = [&](const ChainPlanDir& H, ChainSweep& W) {
         W.nb = nbund;
         W.nq = H.nq;
-        far.assign((size_t)H.nq * kRecWords, 0);
-        for (size_t q = 0; q < H.nq; ++q) {
-            uint32_t* f = &far[q * kRecWords];
-            for (int g = 0; g < npw; ++g) {
-                const uint32_t nd = H.node[q * npw + g];
-                f[128 + g] = nd == kDepNone ? kOffNone : 12u * nd;
-            }
-            for (int lane = 0; lane < 64; ++lane)
-                for (int j = 0; j < bpl; ++j) {
-                    const uint32_t dw = H.dep[(q * bpl + j) * 64 + lane];
-                    f[2 * lane + j] = dw == kDepNone ? kDepLocal | kZeroSlot : (dw & kDepLocal) ? dw : 12u * dw;
-                }
-        }
+        encode_chain_records(H, P.lpn, far);
         if (W.bstart.alloc(nbund) != hipSuccess || W.bsteps.alloc(nbund) != hipSuccess ||
             W.far.alloc(far.size()) != hipSuccess)
             return false;

@@ -2,7 +2,8 @@
 // of assembled tangents, replays the plan with the data flow of the sweep
 // kernels (packed dependency blocks, the LDS ring with its overwrite timing,
 // published values) and compares with the oracle's sequential ILU(0) apply.
-// Every dependency word is checked against the CSR column it stands for.
+// Every dependency word is checked against the CSR column it stands for, and
+// the device form of each plan (the kernels' step records) decodes back to it.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -161,6 +162,40 @@ static Csr box_tangent(int nx, int ny, int nz, double Lx, uint32_t seed, bool pe
     return A;
 }
 
+// the device form of a plan direction (encode_chain_records) decodes back to
+// the plan's nodes and dependency words exactly
+static bool check_records(const ChainPlanDir& W, int lpn) {
+    const int npw = 64 / lpn, bpl = fm::kChainMaxBlk / lpn;
+    std::vector<uint32_t> far;
+    fm::encode_chain_records(W, lpn, far);
+    CHECK(far.size() == (size_t)W.nq * fm::kRecWords, "record size %zu", far.size());
+    for (size_t q = 0; q < W.nq; ++q) {
+        const uint32_t* f = &far[q * fm::kRecWords];
+        for (int g = 0; g < npw; ++g) {
+            const uint32_t x = f[128 + g];
+            CHECK(x == fm::kOffNone || x % 12 == 0, "step %zu quad %d: offset %u", q, g, x);
+            const uint32_t nd = x == fm::kOffNone ? fm::kDepNone : x / 12;
+            CHECK(nd == W.node[q * npw + g], "step %zu quad %d: node %u, plan %u", q, g, nd, W.node[q * npw + g]);
+        }
+        for (int lane = 0; lane < 64; ++lane)
+            for (int j = 0; j < bpl; ++j) {
+                const uint32_t w = f[2 * lane + j];
+                uint32_t dw;
+                if ((int32_t)w >= 0) {
+                    CHECK(w % 12 == 0, "step %zu lane %d: external offset %u", q, lane, w);
+                    dw = w / 12;
+                } else {
+                    const uint32_t slot = w & 0xFFFFu;
+                    CHECK(slot <= fm::kZeroSlot, "step %zu lane %d: slot %u", q, lane, slot);
+                    dw = slot == fm::kZeroSlot ? fm::kDepNone : (fm::kDepLocal | slot);
+                }
+                const uint32_t want = W.dep[(q * bpl + j) * 64 + lane];
+                CHECK(dw == want, "step %zu lane %d block %d: word %08x, plan %08x", q, lane, j, dw, want);
+            }
+    }
+    return true;
+}
+
 static bool run_case(const char* name, int nx, int ny, int nz, double Lx, bool permute, int lpn, bool expect_plan) {
     const Csr A = box_tangent(nx, ny, nz, Lx, 17u + nx, permute);
     ChainPlan P;
@@ -177,6 +212,7 @@ static bool run_case(const char* name, int nx, int ny, int nz, double Lx, bool p
     std::normal_distribution<double> N(0.0, 1.0);
     for (auto& v : x) v = N(rng);
     if (!replay(A, P, P.fwd, false, x, z) || !replay(A, P, P.bwd, true, z, y)) return false;
+    if (lpn == 4 && (!check_records(P.fwd, lpn) || !check_records(P.bwd, lpn))) return false;
     orc_ilu0_apply(n, A.rp.data(), A.ci.data(), A.LU.data(), A.diag.data(), x.data(), yref.data());
     double num = 0, den = 0;
     for (uint32_t i = 0; i < n; ++i) {
