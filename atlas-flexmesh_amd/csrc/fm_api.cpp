@@ -192,6 +192,30 @@ int ensure_colors(fm_ctx* ctx) {
     FM_HIP(hipMemcpy(ctx->conn_c.p, connc.data(), connc.size() * 4, hipMemcpyHostToDevice));
     FM_HIP(hipMemcpy(ctx->eperm.p, ctx->hperm.data(), (size_t)ne * 4, hipMemcpyHostToDevice));
     FM_HIP(hipMemcpy(ctx->erank_c.p, rank.data(), rank.size(), hipMemcpyHostToDevice));
+    // node -> incident elements, in colour order (the order of the colour launches)
+    {
+        const uint32_t nn = ctx->nn;
+        std::vector<uint32_t> ptr(nn + 1, 0), ent((size_t)4 * ne);
+        for (size_t k = 0; k < (size_t)4 * ne; ++k) ptr[connc[k] + 1]++;
+        for (uint32_t v = 0; v < nn; ++v) ptr[v + 1] += ptr[v];
+        std::vector<uint32_t> fill(ptr.begin(), ptr.end() - 1);
+        for (uint32_t t = 0; t < ne; ++t)
+            for (uint32_t a = 0; a < 4; ++a) ent[fill[connc[(size_t)a * ne + t]]++] = (t << 2) | a;
+        bool ok = ne < (1u << 30);
+        for (uint32_t v = 0; v < nn && ok; ++v)
+            ok = ptr[v + 1] - ptr[v] <= kAsmMaxElems && np[v + 1] - np[v] <= kAsmMaxBlk;
+        static const bool color_only = [] {
+            const char* e = std::getenv("FM_ASM");
+            return e && std::string(e) == "color";
+        }();
+        ctx->node_asm = ok && !color_only;
+        if (ctx->node_asm) {
+            FM_HIP(ctx->n2e_ptr.alloc(nn + 1));
+            FM_HIP(ctx->n2e.alloc(ent.size()));
+            FM_HIP(hipMemcpy(ctx->n2e_ptr.p, ptr.data(), (size_t)(nn + 1) * 4, hipMemcpyHostToDevice));
+            FM_HIP(hipMemcpy(ctx->n2e.p, ent.data(), ent.size() * 4, hipMemcpyHostToDevice));
+        }
+    }
     ctx->have_colors = true;
     return FM_OK;
 }
@@ -307,6 +331,32 @@ int ensure_partials(fm_ctx* ctx, size_t n) {
 int assemble(fm_ctx* ctx, const double* u, bool with_K, int prec, bool mask, const double* dD, int prof_kind) {
     Matrix& M = ctx->tangent;
     hipStream_t s = ctx->stream;
+    const uint8_t* isdir = (mask && !ctx->dir_dofs.empty()) ? ctx->isdir.p : nullptr;
+    if (with_K && ctx->node_asm) {
+        // node-gather tangent assembly: every entry of f, lift and K is written once
+        void* K;
+        if (prec == PREC64) {
+            FM_HIP(M.val64.alloc(M.nnz));
+            K = M.val64.p;
+        } else {
+            FM_HIP(M.val32.alloc(M.nnz));
+            K = M.val32.p;
+        }
+        {
+            ProfScope ps(ctx, prof_kind);
+            launch_assemble_node(s, prec, elem_args(ctx, u), ctx->n2e_ptr.p, ctx->n2e.p, ctx->f.p, K, M.row_ptr.p,
+                                 M.col_idx.p, isdir, dD, ctx->lift.p);
+        }
+        FM_TRY(check_launch(ctx, "assemble"));
+        if (isdir)
+            launch_set_dir_diag(s, prec, (uint32_t)ctx->dir_dofs.size(), ctx->d_dir_dofs.p, M.row_ptr.p,
+                                M.sched.nlow.p, K);
+        M.val_valid[prec] = true;
+        M.val_valid[1 - prec] = false;
+        M.lu_valid[0] = M.lu_valid[1] = false;
+        ctx->tangent_elim = isdir != nullptr;
+        return FM_OK;
+    }
     launch_fill_f64(s, ctx->f.p, ctx->f.n, 0.0);
     void* K = nullptr;
     if (with_K) {
@@ -322,7 +372,6 @@ int assemble(fm_ctx* ctx, const double* u, bool with_K, int prec, bool mask, con
         if (dD) launch_fill_f64(s, ctx->lift.p, ctx->lift.n, 0.0);
     }
     ElemArgs a = elem_args(ctx, u);
-    const uint8_t* isdir = (mask && !ctx->dir_dofs.empty()) ? ctx->isdir.p : nullptr;
     for (uint32_t c = 0; c < ctx->n_colors; ++c) {
         ProfScope ps(ctx, prof_kind);
         launch_assemble_color(s, prec, with_K, a, ctx->color_ptr[c], ctx->color_ptr[c + 1], ctx->f.p, K,

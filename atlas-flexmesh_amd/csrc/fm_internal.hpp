@@ -140,6 +140,10 @@ struct Ctx {
     DevBuf<uint32_t> eperm;   // colour-order index -> element id
     DevBuf<double> mu_c, lam_c;
     DevBuf<uint8_t> erank_c;  // 16*ne SoA: rank of node b in nbr(node a)
+    // node -> incident elements in colour order, (t << 2) | local index (the
+    // node-gather assembly; used when every node is within its limits)
+    DevBuf<uint32_t> n2e_ptr, n2e;
+    bool node_asm = false;
     bool have_material = false;
 
     // pattern (node adjacency + tangent matrix)
@@ -226,6 +230,11 @@ void launch_expand_pattern(hipStream_t s, uint32_t nn, const uint32_t* nbr_ptr, 
 void launch_assemble_color(hipStream_t s, int prec, bool with_K, const ElemArgs& a, uint32_t begin, uint32_t end,
                            double* f, void* K, const uint32_t* row_ptr, const uint8_t* isdir, const double* dD,
                            double* lift);
+// node-gather tangent assembly limits: incident elements and column blocks per node
+constexpr uint32_t kAsmMaxElems = 32, kAsmMaxBlk = 32;
+void launch_assemble_node(hipStream_t s, int prec, const ElemArgs& a, const uint32_t* n2e_ptr, const uint32_t* n2e,
+                          double* f, void* K, const uint32_t* row_ptr, const uint32_t* col_idx, const uint8_t* isdir,
+                          const double* dD, double* lift);
 void launch_set_dir_diag(hipStream_t s, int prec, uint32_t n_dir, const uint32_t* dofs, const uint32_t* row_ptr,
                          const uint16_t* nlow, void* K);
 void launch_eliminate(hipStream_t s, int prec, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,

@@ -257,6 +257,118 @@ __global__ __launch_bounds__(64) void assemble_color_kernel(ElemArgs A, uint32_t
     }
 }
 
+// Node-gather tangent assembly: two nodes per wave, one per half.  Lane l
+// of a half evaluates its node's l-th incident element (colour order) and
+// the contributions to the node's three rows: the force, the lift term and
+// the 3x12 tangent rows
+//   K_qb,ik = V [mu (g_q.g_b) d_ik + lam' a_q,i a_b,k + c a_q,k a_b,i]
+// (the arithmetic of the colour kernel).  The half then adds them into an
+// LDS copy of the node's CSR rows in colour order, the order in which the
+// colour launches add them, and writes the rows once, contiguous: no fill
+// of K or f and no read-back of K entries.
+template <typename T>
+__global__ __launch_bounds__(64) void assemble_node_kernel(ElemArgs A, const uint32_t* __restrict__ n2e_ptr,
+                                                           const uint32_t* __restrict__ n2e, double* __restrict__ f,
+                                                           T* __restrict__ K, const uint32_t* __restrict__ row_ptr,
+                                                           const uint32_t* __restrict__ col_idx,
+                                                           const uint8_t* __restrict__ isdir,
+                                                           const double* __restrict__ dD, double* __restrict__ lift) {
+    constexpr uint32_t H = kAsmMaxElems;  // lanes per node (a half wave)
+    static_assert(2 * H == 64, "two nodes per wave");
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int h = threadIdx.x / H, hl = threadIdx.x % H;
+    const uint32_t v = 2 * blk + h;
+    const bool have = v < A.nn;
+    const uint32_t e0 = have ? n2e_ptr[v] : 0, nev = have ? n2e_ptr[v + 1] - e0 : 0;  // <= H
+    __shared__ T sK[2][H][36];
+    __shared__ uint8_t srk[2][H][4];
+    __shared__ double sf[2][H][3], sl[2][H][3];
+    __shared__ T srow[2][3 * kAsmMaxBlk * 3];
+    Elem E;
+    E.ok = false;
+    uint32_t t = 0;
+    int q = 0;
+    if ((uint32_t)hl < nev) {
+        const uint32_t w = n2e[e0 + hl];
+        t = w >> 2;
+        q = (int)(w & 3u);
+        elem_eval(A, t, E);
+        const double s = E.lam * E.lnJ - E.mu;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) sf[h][hl][i] = E.ok ? E.V * (E.mu * E.Fg[q][i] + s * E.a[q][i]) : 0.0;
+        const double lp = E.clamped ? 0.0 : E.lam;
+        const double c = E.mu - E.lam * E.lnJ;
+        const T V = (T)E.V, mu = (T)E.mu, lpe = (T)lp, ce = (T)c;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            srk[h][hl][b] = A.erank[(size_t)(q * 4 + b) * A.ne + t];
+            const T gg = (T)E.g[q][0] * (T)E.g[b][0] + (T)E.g[q][1] * (T)E.g[b][1] + (T)E.g[q][2] * (T)E.g[b][2];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    T val = lpe * (T)E.a[q][i] * (T)E.a[b][k] + ce * (T)E.a[q][k] * (T)E.a[b][i];
+                    if (i == k) val += mu * gg;
+                    sK[h][hl][(i * 4 + b) * 3 + k] = E.ok ? V * val : (T)0;
+                }
+        }
+        double out[3] = {0, 0, 0};
+        if (dD && E.ok) {
+            // K_e dD_e with the un-eliminated element matrix (FP64), row block q
+            double dq[4][3];
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) dq[b][i] = dD[3 * (size_t)E.node[b] + i];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const double gg = E.g[q][0] * E.g[b][0] + E.g[q][1] * E.g[b][1] + E.g[q][2] * E.g[b][2];
+                const double ad = E.a[b][0] * dq[b][0] + E.a[b][1] * dq[b][1] + E.a[b][2] * dq[b][2];
+                const double qd = E.a[q][0] * dq[b][0] + E.a[q][1] * dq[b][1] + E.a[q][2] * dq[b][2];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) out[i] += E.mu * gg * dq[b][i] + lp * E.a[q][i] * ad + c * E.a[b][i] * qd;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) sl[h][hl][i] = E.V * out[i];
+    }
+    const uint32_t r0 = have ? row_ptr[3 * v] : 0, nbv = have ? (row_ptr[3 * v + 1] - r0) / 3 : 0;  // <= kAsmMaxBlk
+    const uint32_t nrow = 3 * nbv, nent = 3 * nrow;
+    for (uint32_t j = hl; j < nent; j += H) srow[h][j] = (T)0;
+    __syncthreads();
+    // force and lift: sums over the elements in colour order
+    if (hl < 3 && have) {
+        double sum = 0.0;
+        for (uint32_t l = 0; l < nev; ++l) sum += sf[h][l][hl];
+        f[3 * (size_t)v + hl] = sum;
+    } else if (dD && hl >= 3 && hl < 6 && have) {
+        double sum = 0.0;
+        for (uint32_t l = 0; l < nev; ++l) sum += sl[h][l][hl - 3];
+        lift[3 * (size_t)v + hl - 3] = sum;
+    }
+    // entry (i, b, k) of element l goes to row i, block rank_b, component k;
+    // lane hl takes entries hl and hl + 32 (< 36) of each element, element by
+    // element, so each row entry receives its contributions in colour order
+    // (the wave's LDS accesses take effect in program order)
+    {
+        const int ea = hl, eb = hl + (int)H;
+        const int ia = ea / 12, ba = (ea / 3) & 3, ka = ea % 3;
+        const int ib = eb / 12, bb = (eb / 3) & 3, kb = eb % 3;
+        for (uint32_t l = 0; l < nev; ++l) {
+            srow[h][ia * nrow + 3u * srk[h][l][ba] + ka] += sK[h][l][ea];
+            if (eb < 36) srow[h][ib * nrow + 3u * srk[h][l][bb] + kb] += sK[h][l][eb];
+        }
+    }
+    __syncthreads();
+    for (uint32_t j = hl; j < nent; j += H) {
+        const uint32_t i = j / nrow, rem = j - i * nrow;
+        const uint32_t pos = row_ptr[3 * v + i] + rem;
+        T val = srow[h][j];
+        if (isdir && (isdir[3 * v + i] || isdir[col_idx[pos]])) val = (T)0;
+        K[pos] = val;
+    }
+}
+
 template <typename T>
 __global__ void set_dir_diag_kernel(uint32_t n_dir, const uint32_t* __restrict__ dofs,
                                     const uint32_t* __restrict__ row_ptr, const uint16_t* __restrict__ nlow,
@@ -1465,6 +1577,17 @@ void launch_assemble_color(hipStream_t s, int prec, bool with_K, const ElemArgs&
         assemble_color_kernel<double, true><<<nb, 64, 0, s>>>(a, begin, end, f, (double*)K, row_ptr, isdir, dD, lift);
     else
         assemble_color_kernel<float, true><<<nb, 64, 0, s>>>(a, begin, end, f, (float*)K, row_ptr, isdir, dD, lift);
+}
+
+void launch_assemble_node(hipStream_t s, int prec, const ElemArgs& a, const uint32_t* n2e_ptr, const uint32_t* n2e,
+                          double* f, void* K, const uint32_t* row_ptr, const uint32_t* col_idx, const uint8_t* isdir,
+                          const double* dD, double* lift) {
+    if (!a.nn) return;
+    const uint32_t nb = cdiv(a.nn, 2);
+    if (prec == PREC64)
+        assemble_node_kernel<double><<<nb, 64, 0, s>>>(a, n2e_ptr, n2e, f, (double*)K, row_ptr, col_idx, isdir, dD, lift);
+    else
+        assemble_node_kernel<float><<<nb, 64, 0, s>>>(a, n2e_ptr, n2e, f, (float*)K, row_ptr, col_idx, isdir, dD, lift);
 }
 
 void launch_set_dir_diag(hipStream_t s, int prec, uint32_t n_dir, const uint32_t* dofs, const uint32_t* row_ptr,
