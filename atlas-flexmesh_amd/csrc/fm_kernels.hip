@@ -809,16 +809,221 @@ __device__ __forceinline__ void factor_node(uint32_t node, const uint32_t* __res
     }
 }
 
+// value of lane l (a compile-time constant) in every lane, through a scalar register
+__device__ __forceinline__ float lane_bcast(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double lane_bcast(double v, int l) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)b, l),
+                   hi = __builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// B = 3: the same IKJ arithmetic as factor_node with the U rows of all lower
+// neighbours staged in LDS first.  Their loads are issued together and
+// re-polled until none is the sentinel, and the column positions are found by
+// binary search in LDS, so a node costs a few memory round trips instead of
+// one per search step and per U entry.  Returns false (nothing written) when
+// the node exceeds the staging limits; the caller then runs factor_node.
+constexpr uint32_t kFacLow = 12;          // lower neighbours
+constexpr uint32_t kFacUp = 14;           // blocks of a neighbour's row from its diagonal block on
+constexpr uint32_t kFacColPad = 16;       // column list padded to a power of two for the search
+constexpr uint32_t kFacSeg = 3 * kFacUp;  // staged entries per row
+constexpr uint32_t kFacQ = 128;           // LDS stride of one neighbour: its 3 rows, padded to two wave loads
+static_assert(3 * kFacSeg <= kFacQ, "a neighbour's rows fit two wave loads");
+static_assert(kFacUp < kFacColPad && kFacLow * kFacColPad % 64 == 0, "padded column lists");
+template <typename T, int S>
+__device__ __forceinline__ bool factor_node3_staged(uint32_t node, const uint32_t* __restrict__ rp,
+                                                    const uint32_t* __restrict__ ci,
+                                                    const uint16_t* __restrict__ nlow, const T* __restrict__ A,
+                                                    T* LU, int* err) {
+    constexpr int B = 3;
+    using U = typename SB<T>::U;
+    constexpr int NIT = (int)(2 * kFacLow);  // item it: neighbour it / 2, half it % 2 of its padded stride
+    static_assert(NIT <= 32, "pending mask holds one bit per item");
+    __shared__ uint32_t s_col[kFacLow][kFacColPad];  // column node of each staged block; ~0 past the last
+    __shared__ T s_u[kFacLow * kFacQ];           // [q][e * kFacSeg + j]: row 3 m_q + e, entry j after the diagonal offset
+    const int lane = threadIdx.x;
+    const uint32_t r0 = B * node;
+    uint32_t rs[B];
+#pragma unroll
+    for (int d = 0; d < B; ++d) rs[d] = rp[r0 + d];
+    const uint32_t len = rp[r0 + 1] - rs[0];
+    const uint32_t nl = nlow[node], p = B * nl;
+    if (len > 64u * S || nl > kFacLow) return false;
+    // lane q < nl: lower neighbour m_q, its row starts, diagonal offset and staged blocks
+    uint32_t rk0 = 0, rk1 = 0, rk2 = 0, pm = 0, nup = 0;
+    if ((uint32_t)lane < nl) {
+        const uint32_t m = ci[rs[0] + B * lane] / B;
+        rk0 = rp[B * m];
+        rk1 = rp[B * m + 1];
+        rk2 = rp[B * m + 2];
+        pm = B * (uint32_t)nlow[m];
+        nup = (rk1 - rk0 - pm) / B;
+    }
+    if (__any((uint32_t)lane < nl && nup > kFacUp)) return false;
+    T a[B][S];
+    uint32_t col[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const uint32_t j = lane + 64 * s;
+        const bool v = j < len;
+        col[s] = v ? ci[rs[0] + j] : 0xFFFFFFFFu;
+#pragma unroll
+        for (int d = 0; d < B; ++d) a[d][s] = v ? A[rs[d] + j] : (T)0;
+    }
+    // column nodes of the staged blocks: loaded before the values, so waiting
+    // for them does not wait for the values (loads complete in order)
+#pragma unroll
+    for (uint32_t k = 0; k < kFacLow * kFacColPad / 64; ++k) {
+        const uint32_t c = lane + 64 * k, q = c / kFacColPad, b = c % kFacColPad;
+        const uint32_t row = __shfl(rk0, (int)q), off = __shfl(pm, (int)q), nq = __shfl(nup, (int)q);
+        (&s_col[0][0])[c] = (q < nl && b < nq) ? ci[row + off + B * b] / B : 0xFFFFFFFFu;
+    }
+    // global address of item it of this lane (neighbour constant per item: scalar reads)
+    auto item_ptr = [&](int it, bool& ok) -> U* {
+        const int q = it >> 1;
+        const uint32_t idx = lane + 64u * (it & 1), e = idx / kFacSeg, j = idx % kFacSeg;
+        const uint32_t x0 = __builtin_amdgcn_readlane(rk0, q), x1 = __builtin_amdgcn_readlane(rk1, q),
+                       x2 = __builtin_amdgcn_readlane(rk2, q);
+        const uint32_t off = __builtin_amdgcn_readlane(pm, q), nq = __builtin_amdgcn_readlane(nup, q);
+        ok = (uint32_t)q < nl && e < B && j < B * nq;
+        return reinterpret_cast<U*>(LU + (e == 0 ? x0 : e == 1 ? x1 : x2) + off + j);
+    };
+    U val[NIT];
+    uint32_t pending = 0;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        bool ok;
+        U* ptr = item_ptr(it, ok);
+        val[it] = SB<T>::S;
+        if (ok) {
+            val[it] = __hip_atomic_load(ptr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            pending |= 1u << it;
+        }
+    }
+    __syncthreads();
+    // positions of this lane's columns in every neighbour's staged rows, while the values are in flight
+    int pos[kFacLow][S];
+#pragma unroll
+    for (int q = 0; q < (int)kFacLow; ++q) {
+        const uint32_t nq = __builtin_amdgcn_readlane(nup, q);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            pos[q][s] = -1;
+            const uint32_t j = lane + 64 * s;
+            // branch-free lower bound over the padded list: the searches of all
+            // neighbours are independent and overlap their LDS latencies
+            const uint32_t cn = col[s] / B;
+            uint32_t lo = 0;
+#pragma unroll
+            for (uint32_t step = kFacColPad / 2; step >= 1; step >>= 1)
+                lo += s_col[q][lo + step - 1] < cn ? step : 0;
+            const bool hit = lo < nq && s_col[q][lo] == cn;
+            if ((uint32_t)q < nl && j < len && j > (uint32_t)(B * q) && hit) pos[q][s] = (int)(B * lo + col[s] % B);
+        }
+    }
+    // IKJ over the lower neighbours in ascending order, as factor_node; each
+    // neighbour is awaited just before its turn, so the elimination by early
+    // neighbours overlaps the wait for late ones (bounded like await_val)
+    uint32_t spins = 0;
+#pragma unroll
+    for (int q = 0; q < (int)kFacLow; ++q) {
+        if ((uint32_t)q >= nl) continue;
+        constexpr uint32_t none = 0;
+        const uint32_t mine_q = 3u << (2 * q);
+        for (;;) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int it = 2 * q + h;
+                if (((pending >> it) & 1u) && val[it] != SB<T>::S) {
+                    s_u[q * kFacQ + lane + 64u * h] = __builtin_bit_cast(T, val[it]);
+                    pending &= ~(1u << it);
+                }
+            }
+            if (!__any((pending & mine_q) != none)) break;
+            if (++spins > kSpinLimit) {
+                if (pending & mine_q) atomicOr(err, 1);
+                break;
+            }
+            if ((spins & 255u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+            __builtin_amdgcn_s_sleep(FM_SPIN_SLEEP);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int it = 2 * q + h;
+                bool ok;
+                U* ptr = item_ptr(it, ok);
+                if ((pending >> it) & 1u) val[it] = __hip_atomic_load(ptr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        __syncthreads();
+        const T* uq = s_u + q * kFacQ;
+#pragma unroll
+        for (int e = 0; e < B; ++e) {
+            const T ukk = uq[e * kFacSeg + e];
+            const int jj = B * q + e, owner = jj & 63, sl = jj >> 6;
+            T l[B];
+#pragma unroll
+            for (int d = 0; d < B; ++d) l[d] = lane_bcast(pick(a[d], sl), owner) / ukk;
+            if (lane == owner)
+#pragma unroll
+                for (int d = 0; d < B; ++d) put(a[d], sl, l[d]);
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                if (pos[q][s] >= 0 && lane + 64 * s > jj) {
+                    const T ukj = uq[e * kFacSeg + pos[q][s]];
+#pragma unroll
+                    for (int d = 0; d < B; ++d) a[d][s] -= l[d] * ukj;
+                }
+            }
+        }
+    }
+    // diagonal block of the node itself
+#pragma unroll
+    for (int d = 0; d < B; ++d) {
+#pragma unroll
+        for (int e = 0; e < d; ++e) {
+            const uint32_t jj = p + e;
+            const int owner = jj & 63, sl = jj >> 6;
+            const T ue = __shfl(pick(a[e], sl), owner);
+            const T ld = __shfl(pick(a[d], sl), owner) / ue;
+            if (lane == owner) put(a[d], sl, ld);
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                if ((uint32_t)(lane + 64 * s) > jj && lane + 64 * s < (int)len) a[d][s] -= ld * a[e][s];
+        }
+        const uint32_t jd = p + d;
+        if (lane == (int)(jd & 63)) {
+            T u = pick(a[d], jd >> 6);
+            if (fabs((double)u) < 1e-12) u += (u >= (T)0) ? (T)1e-8 : (T)-1e-8;
+            put(a[d], jd >> 6, u);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const uint32_t j = lane + 64 * s;
+        if (j < len)
+#pragma unroll
+            for (int d = 0; d < B; ++d) publish(&LU[rs[d] + j], a[d][s]);
+    }
+    __syncthreads();  // the next node of this wave overwrites the stage
+    return true;
+}
+
 // persistent, like the sweeps: wave w takes queue entries w, w+P, ...
 template <typename T, int B, int S>
 __global__ __launch_bounds__(64) void ilu_factor_kernel(uint32_t nq, const uint32_t* __restrict__ order,
                                                         const uint32_t* __restrict__ rp,
                                                         const uint32_t* __restrict__ ci,
                                                         const uint16_t* __restrict__ nlow, const T* __restrict__ A,
-                                                        T* LU, uint32_t* ticket, int* err) {
+                                                        T* LU, uint32_t* ticket, int staged, int* err) {
     for (uint32_t t = blockIdx.x; t < nq; t += gridDim.x) {
         const uint32_t node = order[t];
-        if (node != kNoNode) factor_node<T, B, S>(node, rp, ci, nlow, A, LU, err);
+        if (node == kNoNode) continue;
+        if constexpr (B == 3 && S <= 2)
+            if (staged && factor_node3_staged<T, S>(node, rp, ci, nlow, A, LU, err)) continue;
+        factor_node<T, B, S>(node, rp, ci, nlow, A, LU, err);
     }
 }
 
@@ -1721,21 +1926,24 @@ static void factor_t(hipStream_t s, const Matrix& A, const T* val, T* lu, PcgSta
     fill_sentinel_kernel<T><<<std::min<uint64_t>(cdiv(A.nnz, 256), 65535), 256, 0, s>>>(lu, nullptr, A.nnz);
     const uint32_t nb = std::min(S.q_fwd, factor_waves());
     const uint32_t q = S.q_fwd;
+    // FM_FACTOR_STAGED=0 runs every node through the unstaged code (both give the same bits)
+    const char* ev = std::getenv("FM_FACTOR_STAGED");
+    const int staged = !(ev && ev[0] == '0');
     switch (S.slots) {
         case 1:
             launch_persistent(ilu_factor_kernel<T, B, 1>, nb, s, q, (const uint32_t*)S.fwd_order.p,
                         (const uint32_t*)A.row_ptr.p, (const uint32_t*)A.col_idx.p, (const uint16_t*)S.nlow.p, val,
-                        lu, &st->ticket[2], err);
+                        lu, &st->ticket[2], staged, err);
             break;
         case 2:
             launch_persistent(ilu_factor_kernel<T, B, 2>, nb, s, q, (const uint32_t*)S.fwd_order.p,
                         (const uint32_t*)A.row_ptr.p, (const uint32_t*)A.col_idx.p, (const uint16_t*)S.nlow.p, val,
-                        lu, &st->ticket[2], err);
+                        lu, &st->ticket[2], staged, err);
             break;
         default:
             launch_persistent(ilu_factor_kernel<T, B, 4>, nb, s, q, (const uint32_t*)S.fwd_order.p,
                         (const uint32_t*)A.row_ptr.p, (const uint32_t*)A.col_idx.p, (const uint16_t*)S.nlow.p, val,
-                        lu, &st->ticket[2], err);
+                        lu, &st->ticket[2], staged, err);
             break;
     }
 }

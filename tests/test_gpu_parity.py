@@ -158,6 +158,67 @@ def test_spmv_ilu0_match_oracle(fm, S, O, which):
     assert rel(S.ilu0_apply(x, fm.FM_MIXED), zo) < 1e-3
 
 
+def dense_coupled_blocks(nn=40, seed=5):
+    """Node-blocked (3x3) SPD matrix: a block chain plus node 0 coupled to nodes
+    1..20 (21 upper blocks) and node nn-1 coupled to nodes 0..19 (more than 12
+    lower neighbours), so the factorization mixes nodes inside and outside the
+    limits of the LDS-staged path."""
+    rng = np.random.default_rng(seed)
+    nb = [set([i]) for i in range(nn)]
+    for i in range(nn - 1):
+        nb[i].add(i + 1)
+        nb[i + 1].add(i)
+    for j in range(1, 21):
+        nb[0].add(j)
+        nb[j].add(0)
+    for j in range(0, 20):
+        nb[nn - 1].add(j)
+        nb[j].add(nn - 1)
+    blocks = {}
+    for i in range(nn):
+        for j in nb[i]:
+            if j > i:
+                blocks[(i, j)] = rng.uniform(-1, 1, (3, 3))
+                blocks[(j, i)] = blocks[(i, j)].T
+    for i in range(nn):
+        g = rng.uniform(-1, 1, (3, 3))
+        blocks[(i, i)] = g @ g.T + 3.0 * (len(nb[i]) + 1) * np.eye(3)
+    rp, ci, va = [0], [], []
+    for i in range(nn):
+        cols = sorted(nb[i])
+        for d in range(3):
+            for j in cols:
+                ci.extend(3 * j + e for e in range(3))
+                va.extend(blocks[(i, j)][d])
+            rp.append(len(ci))
+    return np.array(rp, np.uint32), np.array(ci, np.uint32), np.array(va)
+
+
+@pytest.mark.parametrize("which", ["elastic", "dense"])
+def test_factor_staged_matches_unstaged(fm, S, O, which, monkeypatch):
+    """The LDS-staged 3x3-node factorization performs the unstaged code's
+    arithmetic in the same order: identical bits in FP64 and FP32, and the
+    oracle factor to the FP64 tolerance.  The dense case has nodes beyond the
+    staging limits (21 upper blocks; 20 lower neighbours), which the same
+    launch factorizes unstaged."""
+    rp, ci, va = elastic_matrix(O, fm, dims=(5, 4, 4)) if which == "elastic" else dense_coupled_blocks()
+    S.set_matrix(rp, ci, va)
+    assert S.sweep_info()["spmv"] == "node"
+    cs, vs, perm = sorted_csr(rp, ci, va)
+    LUo, _ = O.ilu0(rp, cs, vs)
+    LUo_user = np.empty_like(LUo)
+    LUo_user[perm] = LUo
+    got = {}
+    for staged in ("1", "0"):
+        monkeypatch.setenv("FM_FACTOR_STAGED", staged)
+        got[staged] = (S.ilu0_factor(), S.ilu0_factor(fm.FM_MIXED))
+    monkeypatch.delenv("FM_FACTOR_STAGED")
+    assert rel(got["1"][0], LUo_user) < 1e-12
+    assert rel(got["1"][1], LUo_user) < 1e-3
+    for k in range(2):
+        assert np.array_equal(got["1"][k].view(np.uint64), got["0"][k].view(np.uint64))
+
+
 @pytest.mark.parametrize("dims", [(9, 7, 6), (24, 3, 3), (5, 11, 13)])
 def test_chain_sweeps_match_oracle(fm, S, O, dims):
     # several bundles per direction, partial bundles at the edges, a slender
