@@ -820,12 +820,14 @@ __device__ __forceinline__ double lane_bcast(double v, int l) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// B = 3: the same IKJ arithmetic as factor_node with the U rows of all lower
-// neighbours staged in LDS first.  Their loads are issued together and
-// re-polled until none is the sentinel, and the column positions are found by
-// binary search in LDS, so a node costs a few memory round trips instead of
-// one per search step and per U entry.  Returns false (nothing written) when
-// the node exceeds the staging limits; the caller then runs factor_node.
+// B = 3: the same IKJ arithmetic as factor_node with the U rows of the lower
+// neighbours staged in LDS.  The loads of all neighbours are issued together;
+// each neighbour's values are re-polled (if still the sentinel) just before
+// its elimination step, and the column positions come from a search of the
+// neighbours' column lists in LDS.  A node costs a few memory round trips
+// instead of one per search step and per U entry.  Returns false (nothing
+// written) when the node exceeds the staging limits; the caller then runs
+// factor_node.
 constexpr uint32_t kFacLow = 12;          // lower neighbours
 constexpr uint32_t kFacUp = 14;           // blocks of a neighbour's row from its diagonal block on
 constexpr uint32_t kFacColPad = 16;       // column list padded to a power of two for the search
