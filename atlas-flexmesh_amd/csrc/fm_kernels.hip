@@ -828,12 +828,12 @@ __device__ __forceinline__ double lane_bcast(double v, int l) {
 // instead of one per search step and per U entry.  Returns false (nothing
 // written) when the node exceeds the staging limits; the caller then runs
 // factor_node.
-constexpr uint32_t kFacLow = 12;          // lower neighbours
-constexpr uint32_t kFacUp = 14;           // blocks of a neighbour's row from its diagonal block on
+constexpr uint32_t kFacLow = 8;           // lower neighbours (at most 7 in a Kuhn-split hex mesh)
+constexpr uint32_t kFacUp = 10;           // blocks of a neighbour's row from its diagonal block on (at most 8 there)
 constexpr uint32_t kFacColPad = 16;       // column list padded to a power of two for the search
 constexpr uint32_t kFacSeg = 3 * kFacUp;  // staged entries per row
-constexpr uint32_t kFacQ = 128;           // LDS stride of one neighbour: its 3 rows, padded to two wave loads
-static_assert(3 * kFacSeg <= kFacQ, "a neighbour's rows fit two wave loads");
+constexpr uint32_t kFacQ = 96;            // LDS stride of one neighbour: its 3 rows (two wave loads)
+static_assert(3 * kFacSeg <= kFacQ && kFacQ <= 128, "a neighbour's rows fit two wave loads");
 static_assert(kFacUp < kFacColPad && kFacLow * kFacColPad % 64 == 0, "padded column lists");
 template <typename T, int S>
 __device__ __forceinline__ bool factor_node3_staged(uint32_t node, const uint32_t* __restrict__ rp,

@@ -160,7 +160,7 @@ def test_spmv_ilu0_match_oracle(fm, S, O, which):
 
 def dense_coupled_blocks(nn=40, seed=5):
     """Node-blocked (3x3) SPD matrix: a block chain plus node 0 coupled to nodes
-    1..20 (21 upper blocks) and node nn-1 coupled to nodes 0..19 (more than 12
+    1..20 (21 upper blocks) and node nn-1 coupled to nodes 0..19 (more than 8
     lower neighbours), so the factorization mixes nodes inside and outside the
     limits of the LDS-staged path."""
     rng = np.random.default_rng(seed)
