@@ -623,37 +623,53 @@ __global__ __launch_bounds__(256) void spmv_node_kernel(uint32_t nn, const uint3
         return;
     }
     const uint32_t rbase = rp[0];
+    // three dependent rounds per wave: (1) the block ranges of its nodes and
+    // the own-row x of the dot, (2) the column nodes, then the values, which
+    // stay in flight while (3) the x gathers wait only for the column nodes
+    // (loads complete in order).  Every load is unconditional, at a clamped
+    // index: a load under a condition becomes a branch with a full wait.
     uint32_t node[G], b0[G], nb[G];
+    T xown[G];
 #pragma unroll
     for (int i = 0; i < G; ++i) {
         node[i] = (blk * G + i) * NPB + threadIdx.x / kNodeLanes;
         const uint32_t v = min(node[i], nn - 1);
         b0[i] = nptr[v];
-        nb[i] = node[i] < nn ? nptr[v + 1] - b0[i] : 0;
+        nb[i] = nptr[v + 1];
+        if (DOT) xown[i] = x[3 * v + min(k, 2)];
     }
+#pragma unroll
+    for (int i = 0; i < G; ++i) nb[i] = node[i] < nn ? nb[i] - b0[i] : 0;
+    // block k of each node (a valid block when k is past the end; weight zero)
+    uint32_t kk[G], cn[G];
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+        kk[i] = min((uint32_t)k, max(nb[i], 1u) - 1);
+        cn[i] = spmv_stream_load(ncol + b0[i] + kk[i]);
+    }
+    T a[G][9];
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+        const T* a0 = val + rbase + 9 * (size_t)b0[i] + 3 * kk[i];
+        const T* a1 = a0 + 3 * nb[i];
+        const T* a2 = a1 + 3 * nb[i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            a[i][c] = spmv_stream_load(a0 + c);
+            a[i][3 + c] = spmv_stream_load(a1 + c);
+            a[i][6 + c] = spmv_stream_load(a2 + c);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);  // all value loads issue before the wait for the column nodes
     T s[G][3];
 #pragma unroll
     for (int i = 0; i < G; ++i) {
-        // block k of the node (a valid block when k is past the end; weight zero)
-        const uint32_t kk = min((uint32_t)k, max(nb[i], 1u) - 1);
-        const T* a0 = val + rbase + 9 * (size_t)b0[i] + 3 * kk;
-        const T* a1 = a0 + 3 * nb[i];
-        const T* a2 = a1 + 3 * nb[i];
-        T a[9];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            a[c] = spmv_stream_load(a0 + c);
-            a[3 + c] = spmv_stream_load(a1 + c);
-            a[6 + c] = spmv_stream_load(a2 + c);
-        }
-        const T* xm = x + 3 * (size_t)spmv_stream_load(ncol + b0[i] + kk);
-        // a weight rather than a select keeps the gather unconditional (a
-        // load under a condition becomes a branch with a full wait)
+        const T* xm = x + 3 * (size_t)cn[i];
         const T w = (uint32_t)k < nb[i] ? (T)1 : (T)0;
         const T x0 = xm[0], x1 = xm[1], x2 = xm[2];
-        s[i][0] = (a[0] * x0 + a[1] * x1 + a[2] * x2) * w;
-        s[i][1] = (a[3] * x0 + a[4] * x1 + a[5] * x2) * w;
-        s[i][2] = (a[6] * x0 + a[7] * x1 + a[8] * x2) * w;
+        s[i][0] = (a[i][0] * x0 + a[i][1] * x1 + a[i][2] * x2) * w;
+        s[i][1] = (a[i][3] * x0 + a[i][4] * x1 + a[i][5] * x2) * w;
+        s[i][2] = (a[i][6] * x0 + a[i][7] * x1 + a[i][8] * x2) * w;
     }
     // nodes with more than 16 blocks
 #pragma unroll
@@ -680,7 +696,7 @@ __global__ __launch_bounds__(256) void spmv_node_kernel(uint32_t nn, const uint3
                 sent0[row] = sentinel_value<T>();
                 sent1[row] = sentinel_value<T>();
             }
-            if (DOT) part += (double)sum * (double)x[row];
+            if (DOT) part += (double)sum * (double)xown[i];
         }
     }
     if (DOT) {
