@@ -104,7 +104,7 @@ def make(k, dims=None, L_box=None):
         # directions, so the Krylov iteration runs in FP64 with the FP32
         # ILU(0) factor as preconditioner
         cfg.update(newton_rtol=1e-8, pcg_tol=1e-10, prec=L.FM_MIXED_PC, ls_mode=1, max_newton=60,
-                   pcg_max=8 * nx + 400, inner_max=8 * nx + 400)
+                   pcg_max=8 * nx + 400, inner_max=8 * nx + 400, inner_tol=1e-2)
     else:
         raise ValueError(k)
     if stretch != 0.0:
@@ -129,7 +129,7 @@ def load(solver, cfg):
 def newton_cfg(cfg, **kw):
     c = L.Solver.newton_cfg(n_steps=cfg["n_steps"], prec=cfg["prec"], newton_rtol=cfg["newton_rtol"],
                             pcg_tol=cfg["pcg_tol"])
-    for k in ("ls_mode", "max_newton", "pcg_max", "inner_max"):
+    for k in ("ls_mode", "max_newton", "pcg_max", "inner_max", "inner_tol"):
         if k in cfg:
             setattr(c, k, cfg[k])
     for k, v in kw.items():

@@ -210,7 +210,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--dims", type=str, default=None, help="override hex counts, e.g. 64,64,64")
-    ap.add_argument("--inner-tol", type=float, default=1e-2, help="FP32 PCG tolerance of the mixed path")
+    ap.add_argument("--inner-tol", type=float, default=3e-3, help="FP32 PCG tolerance of the mixed path")
     ap.add_argument("--pcg-max", type=int, default=1000, help="FP32 PCG iteration cap of the mixed path")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
