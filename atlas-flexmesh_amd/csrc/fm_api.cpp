@@ -191,6 +191,12 @@ int ensure_colors(fm_ctx* ctx) {
     FM_HIP(ctx->erank_c.alloc((size_t)16 * ne));
     FM_HIP(hipMemcpy(ctx->conn_c.p, connc.data(), connc.size() * 4, hipMemcpyHostToDevice));
     FM_HIP(hipMemcpy(ctx->eperm.p, ctx->hperm.data(), (size_t)ne * 4, hipMemcpyHostToDevice));
+    {
+        std::vector<uint32_t> iperm(ne);
+        for (uint32_t t = 0; t < ne; ++t) iperm[ctx->hperm[t]] = t;
+        FM_HIP(ctx->eiperm.alloc(ne));
+        FM_HIP(hipMemcpy(ctx->eiperm.p, iperm.data(), (size_t)ne * 4, hipMemcpyHostToDevice));
+    }
     FM_HIP(hipMemcpy(ctx->erank_c.p, rank.data(), rank.size(), hipMemcpyHostToDevice));
     // node -> incident elements, in colour order (the order of the colour launches)
     {
@@ -1534,10 +1540,43 @@ int fm_element_rotations(fm_ctx* ctx, double* w) {
     if (!ctx->u_valid) return fail(ctx, FM_ESTATE, "no device state");
     DevBuf<double> dw;
     FM_HIP(dw.alloc(3ull * ctx->ne));
-    launch_elem_rot(ctx->stream, elem_args(ctx, ctx->u.p), ctx->eperm.p, dw.p);
+    launch_elem_rot(ctx->stream, elem_args(ctx, ctx->u.p), ctx->eiperm.p, dw.p);
     FM_TRY(check_launch(ctx, "elem_rot"));
     FM_TRY(sync(ctx));
     FM_HIP(hipMemcpy(w, dw.p, 3ull * ctx->ne * 8, hipMemcpyDeviceToHost));
+    return FM_OK;
+}
+
+int fm_element_fields(fm_ctx* ctx, const double* u, const fm_element_fields_out* out) {
+    if (!ctx || !out) return fail(ctx, FM_EINVAL, "fm_element_fields: bad arguments");
+    FM_TRY(ensure_ready(ctx));
+    const uint32_t n = 3 * ctx->nn, ne = ctx->ne;
+    const double* ud = ctx->u.p;
+    DevBuf<double> du;
+    if (u) {
+        FM_HIP(du.alloc(n));
+        FM_HIP(hipMemcpyAsync(du.p, u, n * 8ull, hipMemcpyHostToDevice, ctx->stream));
+        ud = du.p;
+    } else if (!ctx->u_valid) {
+        return fail(ctx, FM_ESTATE, "no device state (pass u)");
+    }
+    DevBuf<double> dout;
+    FM_HIP(dout.alloc((size_t)kFieldCount * ne));
+    {
+        ProfScope ps(ctx, FM_K_FIELDS);
+        launch_element_fields(ctx->stream, elem_args(ctx, ud), ctx->eiperm.p, dout.p);
+    }
+    FM_TRY(check_launch(ctx, "element_fields"));
+    FM_TRY(sync(ctx));
+    // rows of the device result: det F, sigma (6), von Mises, pressure, s1, psi, stretch, F (9)
+    const struct {
+        double* dst;
+        int row, rows;
+    } parts[] = {{out->det_F, 0, 1},      {out->cauchy, 1, 6}, {out->von_mises, 7, 1},  {out->pressure, 8, 1},
+                 {out->max_principal, 9, 1}, {out->energy, 10, 1}, {out->max_stretch, 11, 1}, {out->F, 12, 9}};
+    for (const auto& pt : parts)
+        if (pt.dst)
+            FM_HIP(hipMemcpy(pt.dst, dout.p + (size_t)pt.row * ne, (size_t)pt.rows * ne * 8, hipMemcpyDeviceToHost));
     return FM_OK;
 }
 

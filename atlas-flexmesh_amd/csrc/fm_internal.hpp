@@ -138,6 +138,7 @@ struct Ctx {
     std::vector<uint32_t> hcolors, color_ptr, hperm;
     DevBuf<uint32_t> conn_c;  // 4*ne SoA, colour order
     DevBuf<uint32_t> eperm;   // colour-order index -> element id
+    DevBuf<uint32_t> eiperm;  // element id -> colour-order index
     DevBuf<double> mu_c, lam_c;
     DevBuf<uint8_t> erank_c;  // 16*ne SoA: rank of node b in nbr(node a)
     // node -> incident elements in colour order, (t << 2) | local index (the
@@ -299,6 +300,11 @@ uint32_t launch_energy_elem(hipStream_t s, const ElemArgs& a, double* partials);
 void launch_adjoint_elem(hipStream_t s, const ElemArgs& a, const uint32_t* eperm, const double* lam_vec,
                          double* dmu, double* dlam);
 void launch_rot_batch(hipStream_t s, uint32_t n, const double* F, double alpha, double* R, double* w, double* Q);
-void launch_elem_rot(hipStream_t s, const ElemArgs& a, const uint32_t* eperm, double* w);
+void launch_elem_rot(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, double* w);
+// per-element fields, kFieldCount x ne SoA in element order: det F, Cauchy
+// stress (6), von Mises, pressure, largest principal stress, strain energy,
+// maximum stretch, F (9)
+constexpr int kFieldCount = 21;
+void launch_element_fields(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, double* out);
 
 }  // namespace fm

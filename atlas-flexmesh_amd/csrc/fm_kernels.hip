@@ -1698,16 +1698,124 @@ __global__ void rot_batch_kernel(uint32_t n, const double* __restrict__ F, doubl
     }
 }
 
-__global__ void elem_rot_kernel(ElemArgs A, const uint32_t* __restrict__ eperm, double* __restrict__ wout) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= A.ne) return;
+// thread e takes element e (its colour-order index from eiperm): the node
+// gathers of neighbouring elements share cache lines and the stores are contiguous
+__global__ void elem_rot_kernel(ElemArgs A, const uint32_t* __restrict__ eiperm, double* __restrict__ wout) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= A.ne) return;
     Elem E;
     double F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    elem_eval(A, t, E, F);
+    elem_eval(A, eiperm[e], E, F);
     double R[9], w[3] = {0, 0, 0};
     if (E.ok && polar_dev(F, R) == 0) rot_log_dev(R, w);
-    const uint32_t e = eperm[t];
     for (int k = 0; k < 3; ++k) wout[3 * (size_t)e + k] = w[k];
+}
+
+// ---------------------------------------------------------------------------
+// Element fields of a displacement (FP64), the consumer of the converged
+// state: StressExtractor::extract_element_fields (FEM/stress_extraction.hpp:
+// 187-240) over the element state of update_element_states_rigorous
+// (FEM/adaptive_fem_engine.hpp:2716-2794).  F is the exact P1 gradient, kept
+// as computed (no projection onto det F = 1) with J = det F evaluated.
+// ---------------------------------------------------------------------------
+// largest eigenvalue of a symmetric 3x3 matrix (upper triangle a00 a01 a02 a11
+// a12 a22) by cyclic Jacobi rotations (Golub & Van Loan, Alg. 8.5.3) until the
+// off-diagonal part is below 1e-18 of the matrix norm: accurate to rounding
+// also for repeated eigenvalues, where the trigonometric closed form loses
+// half the digits
+__device__ double sym3_max_eig(double a00, double a01, double a02, double a11, double a12, double a22) {
+    double A[3][3] = {{a00, a01, a02}, {a01, a11, a12}, {a02, a12, a22}};
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        const double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+        const double dg = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+        if (off <= 1e-36 * (dg + 2.0 * off)) break;
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int q = p + 1; q < 3; ++q) {
+                if (A[p][q] == 0.0) continue;
+                const double th = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const double akp = A[k][p], akq = A[k][q];
+                    A[k][p] = c * akp - sn * akq;
+                    A[k][q] = sn * akp + c * akq;
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const double apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = c * apk - sn * aqk;
+                    A[q][k] = sn * apk + c * aqk;
+                }
+            }
+    }
+    return fmax(A[0][0], fmax(A[1][1], A[2][2]));
+}
+
+// out (kFieldCount x ne, SoA, element order): det F, Cauchy stress (xx, yy,
+// zz, yz, xz, xy; sigma = P F^T / max(|det F|, 0.01), FEM/constitutive_models.hpp:
+// 116-148), von Mises (FEM/stress_extraction.hpp:143-157), pressure tr/3,
+// largest principal stress, strain energy psi, maximum stretch
+// sqrt(max(eig_max(F^T F), 1)) (:160-176), F row-major.  Degenerate elements
+// (volume < 1e-30) give zeros.  Thread e takes element e (colour-order index
+// from eiperm), as elem_rot_kernel.
+__global__ __launch_bounds__(128) void element_fields_kernel(ElemArgs A, const uint32_t* __restrict__ eiperm,
+                                                             double* __restrict__ out) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= A.ne) return;
+    Elem E;
+    double F[9];
+    elem_eval(A, eiperm[e], E, F);
+    double f[kFieldCount];
+#pragma unroll
+    for (int k = 0; k < kFieldCount; ++k) f[k] = 0.0;
+    if (E.ok) {
+        double FiT[9], lnJ;
+        bool cl;
+        kinematics(F, FiT, lnJ, cl);
+        double P[9];
+        const double sP = E.lam * lnJ - E.mu;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) P[k] = E.mu * F[k] + sP * FiT[k];
+        const double det = det3(F);
+        const double inv_J = 1.0 / fmax(fabs(det), 0.01);
+        double sg[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                sg[i * 3 + j] = inv_J * (P[i * 3 + 0] * F[j * 3 + 0] + P[i * 3 + 1] * F[j * 3 + 1] +
+                                         P[i * 3 + 2] * F[j * 3 + 2]);
+        const double pr = (sg[0] + sg[4] + sg[8]) / 3.0;
+        const double d0 = sg[0] - pr, d4 = sg[4] - pr, d8 = sg[8] - pr;
+        double I1 = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) I1 += F[k] * F[k];
+        double Cm[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                Cm[i * 3 + j] = F[0 * 3 + i] * F[0 * 3 + j] + F[1 * 3 + i] * F[1 * 3 + j] + F[2 * 3 + i] * F[2 * 3 + j];
+        f[0] = det;
+        f[1] = sg[0];
+        f[2] = sg[4];
+        f[3] = sg[8];
+        f[4] = sg[5];
+        f[5] = sg[2];
+        f[6] = sg[1];
+        f[7] = sqrt(1.5 * (d0 * d0 + d4 * d4 + d8 * d8 + 2.0 * (sg[1] * sg[1] + sg[2] * sg[2] + sg[5] * sg[5])));
+        f[8] = pr;
+        f[9] = sym3_max_eig(sg[0], sg[1], sg[2], sg[4], sg[5], sg[8]);
+        f[10] = 0.5 * E.mu * (I1 - 3.0) - E.mu * lnJ + 0.5 * E.lam * lnJ * lnJ;
+        f[11] = sqrt(fmax(sym3_max_eig(Cm[0], Cm[1], Cm[2], Cm[4], Cm[5], Cm[8]), 1.0));
+#pragma unroll
+        for (int k = 0; k < 9; ++k) f[12 + k] = F[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kFieldCount; ++k) out[(size_t)k * A.ne + e] = f[k];
 }
 
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -2139,8 +2247,11 @@ uint32_t launch_energy_elem(hipStream_t s, const ElemArgs& a, double* partials) 
 void launch_rot_batch(hipStream_t s, uint32_t n, const double* F, double alpha, double* R, double* w, double* Q) {
     if (n) rot_batch_kernel<<<cdiv(n, 128), 128, 0, s>>>(n, F, alpha, R, w, Q);
 }
-void launch_elem_rot(hipStream_t s, const ElemArgs& a, const uint32_t* eperm, double* w) {
-    elem_rot_kernel<<<cdiv(a.ne, 128), 128, 0, s>>>(a, eperm, w);
+void launch_element_fields(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, double* out) {
+    if (a.ne) element_fields_kernel<<<cdiv(a.ne, 128), 128, 0, s>>>(a, eiperm, out);
+}
+void launch_elem_rot(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, double* w) {
+    elem_rot_kernel<<<cdiv(a.ne, 128), 128, 0, s>>>(a, eiperm, w);
 }
 
 }  // namespace fm

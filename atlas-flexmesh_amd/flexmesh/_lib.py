@@ -15,7 +15,7 @@ LIB_PATH = os.environ.get("FM_LIB_PATH") or os.path.join(HERE, "libflexmesh.so")
 
 FM_OK, FM_EINVAL, FM_ESTATE, FM_EHIP, FM_ENOCONV, FM_EDEVICE = 0, -1, -2, -3, -4, -5
 FM_FP64, FM_MIXED, FM_MIXED_PC = 0, 1, 2
-KERNEL_CLASSES = ["spmv", "ilu_fwd", "ilu_bwd", "ilu_factor", "assemble", "residual", "pcg_other"]
+KERNEL_CLASSES = ["spmv", "ilu_fwd", "ilu_bwd", "ilu_factor", "assemble", "residual", "pcg_other", "fields"]
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -28,6 +28,13 @@ class NewtonCfg(C.Structure):
                 ("pcg_tol", C.c_double), ("pcg_max", C.c_int), ("inner_tol", C.c_double), ("ls_max", C.c_int),
                 ("armijo_c", C.c_double), ("lift", C.c_int), ("inner_max", C.c_int),
                 ("ls_mode", C.c_int)]
+
+
+FIELD_NAMES = ("det_F", "cauchy", "von_mises", "pressure", "max_principal", "energy", "max_stretch", "F")
+
+
+class ElementFieldsOut(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in FIELD_NAMES]
 
 
 class NewtonStats(C.Structure):
@@ -87,6 +94,7 @@ def lib():
     L.fm_adjoint.argtypes = [_vp, _f64p, C.c_double, C.c_int, _vp, _vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.fm_rot_logexp_batch.argtypes = [_vp, C.c_uint32, _f64p, C.c_double, _vp, _vp, _vp]
     L.fm_element_rotations.argtypes = [_vp, _f64p]
+    L.fm_element_fields.argtypes = [_vp, _vp, C.POINTER(ElementFieldsOut)]
     L.fm_checkpoint_save.argtypes = [_vp, C.c_char_p, C.c_uint32]
     L.fm_checkpoint_load.argtypes = [_vp, C.c_char_p, C.POINTER(C.c_uint32)]
     L.fm_profile_enable.argtypes = [_vp, C.c_int]
@@ -311,6 +319,18 @@ class Solver:
         self._chk(self.L.fm_element_rotations(self.h, w), "fm_element_rotations")
         return w.reshape(self.ne, 3)
 
+    def element_fields(self, u=None, fields=FIELD_NAMES):
+        """Per-element fields of u (None: the device state), element order:
+        cauchy is 6 x ne (xx, yy, zz, yz, xz, xy), F is 9 x ne (F_ij row-major)."""
+        uu = None if u is None else f64(u)
+        out = {}
+        o = ElementFieldsOut()
+        for k in fields:
+            out[k] = np.zeros({"cauchy": (6, self.ne), "F": (9, self.ne)}.get(k, (self.ne,)))
+            setattr(o, k, out[k].ctypes.data)
+        self._chk(self.L.fm_element_fields(self.h, _ptr(uu), C.byref(o)), "fm_element_fields")
+        return out
+
     def checkpoint_save(self, path, load_step):
         self._chk(self.L.fm_checkpoint_save(self.h, path.encode(), load_step), "fm_checkpoint_save")
 
@@ -327,8 +347,8 @@ class Solver:
         self._chk(self.L.fm_profile_reset(self.h), "fm_profile_reset")
 
     def profile_get(self):
-        ms = np.zeros(7)
-        n = np.zeros(7, dtype=np.uint64)
+        ms = np.zeros(len(KERNEL_CLASSES))
+        n = np.zeros(len(KERNEL_CLASSES), dtype=np.uint64)
         self._chk(self.L.fm_profile_get(self.h, ms, n), "fm_profile_get")
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KERNEL_CLASSES)}
 

@@ -36,8 +36,10 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def kernel_bytes(info, kind, v):
-    """Algorithmic bytes per launch of kernel class `kind` (v = value bytes, 4-byte indices)."""
+def kernel_bytes(info, kind, v, launches_per_unit=None):
+    """Algorithmic bytes per launch of kernel class `kind` (v = value bytes, 4-byte
+    indices).  An assembly or residual is one node-gather launch or one launch per
+    colour; launches_per_unit says which (default: one per colour)."""
     n, nnz, ne, nn = info["n"], info["nnz"], info["ne"], info["nn"]
     i = 4
     nlow = (nnz - n) // 2  # strictly lower entries (structurally symmetric pattern)
@@ -64,11 +66,16 @@ def kernel_bytes(info, kind, v):
         return (nnz - nlow) * (v + i) + n * (i + 3 * v)
     if kind == "ilu_factor":
         return nnz * (2 * v + i) + n * i
+    per = launches_per_unit or max(info["ncolors"], 1)
     if kind == "assemble":
-        # per colour launch: the compulsory traffic of one assembly divided over the colour launches
-        return (ne * (16 + 2 * 8) + nn * 24 + n * 8 + nnz * v + n * 8) // max(info["ncolors"], 1)
+        # the compulsory traffic of one assembly divided over its launches
+        return int((ne * (16 + 2 * 8) + nn * 24 + n * 8 + nnz * v + n * 8) / per)
     if kind == "residual":
-        return (ne * (16 + 2 * 8) + nn * 24 + n * 8 + n * 8) // max(info["ncolors"], 1)
+        return int((ne * (16 + 2 * 8) + nn * 24 + n * 8 + n * 8) / per)
+    if kind == "fields":
+        # connectivity and the two material values per element, X and u once per
+        # node, 21 FP64 outputs per element
+        return ne * (16 + 2 * 8 + 21 * 8) + nn * 2 * 24
     return 0
 
 
@@ -177,6 +184,22 @@ def run_steps(S, cfg, prec, first, count, inner_tol, profile, pcg_max=1000):
     return wall, tot, worst, prof
 
 
+def element_fields_timing(S, info, reps=5):
+    """The per-element field pass over the converged state (after the timed
+    region, not part of `value`): HIP-event time of its kernel and algorithmic
+    bandwidth."""
+    S.profile(True)
+    S.profile_reset()
+    for _ in range(reps):
+        S.element_fields(None, fields=())
+    ms, n = S.profile_get()["fields"]
+    S.profile(False)
+    us = 1e3 * ms / max(n, 1)
+    b = kernel_bytes(info, "fields", 8)
+    return {"avg_us": us, "launches": n, "bytes_per_launch": b, "GBps": b / (us * 1e-6) / 1e9,
+            "frac_of_hbm_peak": b / (us * 1e-6) / 1e9 / PEAK_HBM_GBS}
+
+
 def micro(S, cfg, info, iters):
     """Per-kernel timings on the tangent at u = 0 with the Dirichlet rows eliminated."""
     import flexmesh as fm
@@ -196,7 +219,8 @@ def micro(S, cfg, info, iters):
         S.profile(False)
         out[name] = {"pcg_iters": its, "ms_per_pcg_iter": 1e3 * wall / max(its, 1), "relres": rel,
                      "kernels": {k: {"avg_us": 1e3 * ms / n, "launches": n,
-                                     "GBps": kernel_bytes(info, k, v) / (ms / n * 1e-3) / 1e9}
+                                     "GBps": kernel_bytes(info, k, v, n if k == "assemble" else None)
+                                     / (ms / n * 1e-3) / 1e9}
                                  for k, (ms, n) in prof.items() if n}}
         log(f"{name}: {out[name]['ms_per_pcg_iter']:.2f} ms/iter " +
             " ".join(f"{k}={d['avg_us']:.0f}us/{d['GBps']:.0f}GB/s" for k, d in out[name]["kernels"].items()))
@@ -265,9 +289,10 @@ def main():
     # roofline of the dominant kernel class
     v = 4 if mixed else 8
     kern = {}
+    units = {"assemble": tot["assemblies"], "residual": tot["residuals"]}
     for k, (ms, nl) in prof.items():
         if nl:
-            b = kernel_bytes(info, k, v)
+            b = kernel_bytes(info, k, v, nl / units[k] if units.get(k) else None)
             kern[k] = {"ms_total": ms, "launches": nl, "avg_us": 1e3 * ms / nl, "bytes_per_launch": b,
                        "GBps": b / (ms / nl * 1e-3) / 1e9 if b else None}
     dom = max(kern, key=lambda k: kern[k]["ms_total"])
@@ -279,6 +304,7 @@ def main():
     out["kernels"] = kern
     if "spmv" in kern:
         out["spmv_frac_of_peak"] = kern["spmv"]["GBps"] / PEAK_HBM_GBS
+    out["element_fields"] = element_fields_timing(S, info)
     # FP64 GPU path on the same steps from the same state
     if mixed and not args.no_fp64:
         S.set_u(u0)

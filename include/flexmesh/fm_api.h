@@ -171,6 +171,29 @@ int fm_rot_logexp_batch(fm_ctx* ctx, uint32_t n, const double* F, double alpha, 
    (3*ne doubles, element order) */
 int fm_element_rotations(fm_ctx* ctx, double* w);
 
+/* ---- element fields (FP64) -----------------------------------------------------------
+ * Replaces StressExtractor::extract_element_fields (FEM/stress_extraction.hpp:187-240)
+ * and the element-state update update_element_states_rigorous
+ * (FEM/adaptive_fem_engine.hpp:2716-2794) for the displacement u (3*nn, NULL: the
+ * device-resident state).  Per element, in element order: det F (signed),
+ * Cauchy stress sigma = P F^T / max(|det F|, 0.01) as six SoA rows
+ * (xx, yy, zz, yz, xz, xy; 6*ne), von Mises sqrt(3/2 dev:dev), pressure tr/3, the
+ * largest principal stress, the strain-energy density, the maximum stretch
+ * sqrt(max(eig_max(F^T F), 1)), and F as nine SoA rows (F_ij, row-major; 9*ne).
+ * F is the exact P1 gradient, not projected onto det F = 1.  Degenerate elements
+ * (volume < 1e-30) give zeros.  Any output may be NULL. */
+typedef struct fm_element_fields_out {
+    double* det_F;
+    double* cauchy;
+    double* von_mises;
+    double* pressure;
+    double* max_principal;
+    double* energy;
+    double* max_stretch;
+    double* F;
+} fm_element_fields_out;
+int fm_element_fields(fm_ctx* ctx, const double* u, const fm_element_fields_out* out);
+
 /* ---- checkpoint ----------------------------------------------------------------------
  * Versioned, checksummed load-step checkpoint (superset of CheckpointFile,
  * FEM/adaptive_fem_engine.hpp:1347-1452): magic, version, counts, load step,
@@ -187,7 +210,8 @@ int fm_checkpoint_load(fm_ctx* ctx, const char* path, uint32_t* load_step);
 #define FM_K_ASSEMBLE 4
 #define FM_K_RESIDUAL 5
 #define FM_K_PCG_OTHER 6
-#define FM_K_COUNT 7
+#define FM_K_FIELDS 7
+#define FM_K_COUNT 8
 int fm_profile_enable(fm_ctx* ctx, int on);
 int fm_profile_reset(fm_ctx* ctx);
 /* ms[k] total milliseconds and launches[k] count for kernel class k */

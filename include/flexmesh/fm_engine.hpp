@@ -181,6 +181,49 @@ struct MeshColoring {
     }
 };
 
+// StressExtractor (FEM/stress_extraction.hpp:31-241) over fm_element_fields:
+// extract_element_fields fills the reference's ElementFieldData in element
+// order from each element's exact P1 gradient F (the reference's
+// compute_deformation_gradient_at_element uses constant shape gradients that
+// ignore the geometry).  The strain energy is the energy whose derivative is
+// the stress (lam/2 ln^2 J); the error indicators come from the caller's
+// estimator, as in the reference (:205-206, :237-240).
+template <class Material>
+class StressExtractor {
+public:
+    struct ElementFieldData {
+        std::vector<double> von_mises, pressure, max_principal_stress, strain_energy, max_stretch, det_F,
+            error_indicators;
+    };
+    explicit StressExtractor(const Material& mat) : material(mat) {}
+    Material material;
+
+    template <class Mesh, class Solution, class Estimator>
+    ElementFieldData extract_element_fields(const Mesh& mesh, const Solution& u, const Estimator& estimator) const {
+        Device& d = Device::get();
+        d.upload(mesh);
+        d.material(material.params.E, material.params.nu);
+        if (u.size() != 3 * mesh.nodes.size()) throw std::runtime_error("flexmesh: u is not 3 * n_nodes long");
+        const size_t ne = mesh.elements.size();
+        ElementFieldData data;
+        for (auto* v : {&data.von_mises, &data.pressure, &data.max_principal_stress, &data.strain_energy,
+                        &data.max_stretch, &data.det_F, &data.error_indicators})
+            v->assign(ne, 0.0);
+        fm_element_fields_out o{};
+        o.det_F = data.det_F.data();
+        o.von_mises = data.von_mises.data();
+        o.pressure = data.pressure.data();
+        o.max_principal = data.max_principal_stress.data();
+        o.energy = data.strain_energy.data();
+        o.max_stretch = data.max_stretch.data();
+        d.check(fm_element_fields(d.ctx(), u.data(), &o), "fm_element_fields");
+        std::vector<double> elem_errors;
+        estimator.estimate(elem_errors);
+        for (size_t e = 0; e < ne && e < elem_errors.size(); ++e) data.error_indicators[e] = elem_errors[e];
+        return data;
+    }
+};
+
 // NewtonSolverConfig / NewtonSolverResult (FEM/nonlinear_solver.hpp:23-64)
 struct NewtonSolverConfig {
     int max_iterations = 50;

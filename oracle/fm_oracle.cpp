@@ -915,4 +915,127 @@ int orc_adjoint(const orc_problem* P, const double* u, const double* dLdu, doubl
     return its;
 }
 
+// ---------------------------------------------------------------------------
+// Element fields of a displacement: the downstream consumer of the converged
+// state.  F follows update_element_states_rigorous
+// (FEM/adaptive_fem_engine.hpp:2716-2774) with its defect D9 corrected: F is
+// stored as computed (no projection onto det F = 1, which is wrong for a
+// compressible law) and J = det F is evaluated, not passed uninitialised.
+// StressExtractor::compute_deformation_gradient_at_element
+// (FEM/stress_extraction.hpp:39-91) uses constant shape gradients that ignore
+// the geometry; the exact P1 gradient of the element kernels is used instead.
+// ---------------------------------------------------------------------------
+
+// Eigenvalues of a symmetric 3x3 matrix (row-major; the upper triangle is
+// read), descending, by cyclic Jacobi rotations (Golub & Van Loan, Alg. 8.5.3)
+// until the off-diagonal part is below 1e-18 of the matrix norm (the diagonal
+// then holds the eigenvalues to rounding) or 60 sweeps.
+void orc_sym_eig3(const double A_in[9], double w[3]) {
+    double A[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) A[i][j] = A_in[std::min(i, j) * 3 + std::max(i, j)];
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        const double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+        const double dg = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+        if (off <= 1e-36 * (dg + 2.0 * off)) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                if (A[p][q] == 0.0) continue;
+                const double theta = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < 3; ++k) {  // A <- A J (columns p, q)
+                    const double akp = A[k][p], akq = A[k][q];
+                    A[k][p] = c * akp - sn * akq;
+                    A[k][q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < 3; ++k) {  // A <- J^T A (rows p, q)
+                    const double apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = c * apk - sn * aqk;
+                    A[q][k] = sn * apk + c * aqk;
+                }
+            }
+    }
+    w[0] = A[0][0];
+    w[1] = A[1][1];
+    w[2] = A[2][2];
+    std::sort(w, w + 3, [](double a, double b) { return a > b; });
+}
+
+// NeoHookeanMaterial::compute_cauchy_stress (FEM/constitutive_models.hpp:116-148):
+// sigma = P F^T / max(|det F|, 0.01), with P the stress law of orc_pk1.
+void orc_cauchy(const double F[9], double mu, double lam, double sigma[9]) {
+    double P[9];
+    orc_pk1(F, mu, lam, P);
+    const double inv_J = 1.0 / std::max(std::fabs(det3(F)), 0.01);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double t = 0.0;
+            for (int k = 0; k < 3; ++k) t += P[i * 3 + k] * F[j * 3 + k];
+            sigma[i * 3 + j] = inv_J * t;
+        }
+}
+
+// StressExtractor::compute_von_mises (FEM/stress_extraction.hpp:143-157):
+// sqrt(3/2 dev:dev) with the off-diagonal terms of the upper triangle doubled.
+double orc_von_mises(const double s[9]) {
+    const double p = (s[0] + s[4] + s[8]) / 3.0;
+    const double d0 = s[0] - p, d4 = s[4] - p, d8 = s[8] - p;
+    const double n2 = d0 * d0 + d4 * d4 + d8 * d8 + 2.0 * (s[1] * s[1] + s[2] * s[2] + s[5] * s[5]);
+    return std::sqrt(1.5 * n2);
+}
+
+// StressExtractor::extract_element_fields (FEM/stress_extraction.hpp:187-240)
+// per element: det F (:229-231), Cauchy stress as (xx, yy, zz, yz, xz, xy)
+// from the upper triangle, von Mises, pressure tr(sigma)/3 (:226), largest
+// principal stress (:234-236), strain energy (orc_energy), maximum stretch
+// sqrt(max(eig_max(F^T F), 1)) (:160-176), and F.  The reference's power
+// iteration (:98-140) converges to the eigenvalue of largest magnitude from a
+// fixed start vector; the exact algebraically largest eigenvalue is used.
+// Degenerate elements (volume < 1e-30, skipped by :2750) give zeros.  Outputs
+// are per element in element order; any pointer may be null.
+void orc_element_fields(uint32_t nn, const double* X, uint32_t ne, const uint32_t* conn, const double* mu,
+                        const double* lam, const double* u, double* det_F, double* cauchy, double* vm,
+                        double* pressure, double* s1, double* psi, double* stretch, double* Fout) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int64_t ei = 0; ei < (int64_t)ne; ++ei) {
+        const uint32_t e = (uint32_t)ei;
+        double Xe[12], ue[12];
+        uint32_t nodes[4];
+        gather_elem(X, nn, conn, ne, e, u, Xe, ue, nodes);
+        ElemGeo G;
+        double F[9] = {0}, sig[9] = {0}, f[8] = {0};
+        if (elem_geometry(Xe, G)) {
+            ElemKin K;
+            elem_kinematics(G, ue, K);
+            std::memcpy(F, K.F, sizeof F);
+            orc_cauchy(F, mu[e], lam[e], sig);
+            double w[3], Cm[9] = {0};
+            f[0] = det3(F);
+            f[1] = orc_von_mises(sig);
+            f[2] = (sig[0] + sig[4] + sig[8]) / 3.0;
+            orc_sym_eig3(sig, w);
+            f[3] = w[0];
+            f[4] = orc_energy(F, mu[e], lam[e]);
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    for (int k = 0; k < 3; ++k) Cm[i * 3 + j] += F[k * 3 + i] * F[k * 3 + j];
+            orc_sym_eig3(Cm, w);
+            f[5] = std::sqrt(std::max(w[0], 1.0));
+        }
+        if (det_F) det_F[e] = f[0];
+        if (vm) vm[e] = f[1];
+        if (pressure) pressure[e] = f[2];
+        if (s1) s1[e] = f[3];
+        if (psi) psi[e] = f[4];
+        if (stretch) stretch[e] = f[5];
+        if (cauchy) {
+            const double c6[6] = {sig[0], sig[4], sig[8], sig[5], sig[2], sig[1]};
+            for (int k = 0; k < 6; ++k) cauchy[(size_t)k * ne + e] = c6[k];
+        }
+        if (Fout)
+            for (int k = 0; k < 9; ++k) Fout[(size_t)k * ne + e] = F[k];
+    }
+}
+
 }  // extern "C"

@@ -119,6 +119,15 @@ int orc_polar(const double F[9], double R[9]);
 void orc_rot_log(const double R[9], double w[3]);
 void orc_rot_exp(const double w[3], double R[9]);
 
+/* element fields of a displacement (det F, Cauchy stress, von Mises, pressure,
+ * largest principal stress, strain energy, maximum stretch, F) */
+void orc_sym_eig3(const double A[9], double w[3]);
+void orc_cauchy(const double F[9], double mu, double lam, double sigma[9]);
+double orc_von_mises(const double sigma[9]);
+void orc_element_fields(uint32_t nn, const double* X, uint32_t ne, const uint32_t* conn, const double* mu,
+                        const double* lam, const double* u, double* det_F, double* cauchy, double* vm,
+                        double* pressure, double* s1, double* psi, double* stretch, double* F);
+
 /* a22: adjoint at a converged state of the final load step (load factor 1) */
 int orc_adjoint(const orc_problem* P, const double* u, const double* dLdu, double tol, int max_iter,
                 double* dL_dmu, double* dL_dlam, double* dL_dload, double* lam_out);

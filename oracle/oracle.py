@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
 _u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 
 
@@ -80,6 +81,11 @@ def lib():
         L.orc_pcg.restype = C.c_int
         L.orc_newton_step.argtypes = [C.POINTER(Problem), C.c_uint32, _f64p, C.POINTER(StepStats)]
         L.orc_newton_step.restype = C.c_int
+        L.orc_sym_eig3.argtypes = [_f64p, _f64p]
+        L.orc_cauchy.argtypes = [_f64p, C.c_double, C.c_double, _f64p]
+        L.orc_von_mises.argtypes = [_f64p]
+        L.orc_von_mises.restype = C.c_double
+        L.orc_element_fields.argtypes = [C.c_uint32, _f64p, C.c_uint32, _u32p, _f64p, _f64p, _vp] + [_vp] * 8
         L.orc_polar.argtypes = [_f64p, _f64p]
         L.orc_polar.restype = C.c_int
         L.orc_rot_log.argtypes = [_f64p, _f64p]
@@ -187,6 +193,40 @@ def pcg(row_ptr, col_idx, val, b, tol=1e-10, max_iter=1000, x0=None):
     its = lib().orc_pcg(n, row_ptr, col_idx, val, np.ascontiguousarray(b, dtype=np.float64), x, tol, max_iter,
                         C.byref(rel))
     return x, its, rel.value
+
+
+def sym_eig3(A):
+    w = np.zeros(3)
+    lib().orc_sym_eig3(np.ascontiguousarray(A, dtype=np.float64).ravel(), w)
+    return w
+
+
+def cauchy(F, mu, lam):
+    s = np.zeros(9)
+    lib().orc_cauchy(np.ascontiguousarray(F, dtype=np.float64).ravel(), mu, lam, s)
+    return s.reshape(3, 3)
+
+
+def von_mises(sigma):
+    return lib().orc_von_mises(np.ascontiguousarray(sigma, dtype=np.float64).ravel())
+
+
+FIELD_NAMES = ("det_F", "cauchy", "von_mises", "pressure", "max_principal", "energy", "max_stretch", "F")
+
+
+def element_fields(X, conn, mu, lam, u):
+    """Per-element fields in element order: cauchy is 6 x ne (xx, yy, zz, yz, xz, xy), F is 9 x ne."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    conn = np.ascontiguousarray(conn, dtype=np.uint32)
+    nn, ne = X.size // 3, conn.size // 4
+    u = None if u is None else np.ascontiguousarray(u, dtype=np.float64)
+    out = {k: np.zeros(ne) for k in FIELD_NAMES}
+    out["cauchy"] = np.zeros((6, ne))
+    out["F"] = np.zeros((9, ne))
+    lib().orc_element_fields(nn, X, ne, conn, np.ascontiguousarray(mu, dtype=np.float64),
+                             np.ascontiguousarray(lam, dtype=np.float64), None if u is None else u.ctypes.data,
+                             *[out[k].ctypes.data for k in FIELD_NAMES])
+    return out
 
 
 def polar(F):

@@ -7,6 +7,7 @@
 // below calls the reference's own code unchanged.
 #include "fem/adaptive_fem_engine.hpp"
 #include "fem/constitutive_models.hpp"
+#include "fem/stress_extraction.hpp"
 
 #include <cstring>
 
@@ -52,6 +53,22 @@ void ref_pk1(const double F[9], double E, double nu, double P[9]) {
     p.nu = nu;
     NeoHookeanMaterial m(p);
     m.compute_pk1_stress(F, P);
+}
+
+// NeoHookeanMaterial::compute_cauchy_stress (FEM/constitutive_models.hpp:116-148)
+void ref_cauchy(const double F[9], double E, double nu, double sigma[9]) {
+    NeoHookeanParameters p;
+    p.E = E;
+    p.nu = nu;
+    NeoHookeanMaterial m(p);
+    m.compute_cauchy_stress(F, sigma);
+}
+
+// StressExtractor::compute_von_mises (FEM/stress_extraction.hpp:143-157)
+double ref_von_mises(const double sigma[9]) {
+    NeoHookeanParameters p;
+    StressExtractor x{NeoHookeanMaterial(p)};
+    return x.compute_von_mises(sigma);
 }
 
 // generate_unit_cube_mesh (FEM/adaptive_fem_engine.hpp:2219-2294)

@@ -22,6 +22,9 @@ struct BoundaryConditions {
     std::vector<std::tuple<int, double, double, double>> neumann_bcs;
 };
 struct LoadStep { double load_factor{1.0}; };
+struct ZZErrorEstimator {
+    void estimate(std::vector<double>& e) const { e.assign(5, 0.25); }
+};
 
 static int failures = 0;
 static void check(const char* name, bool ok, double v) {
@@ -174,6 +177,21 @@ int run() {
     }
     check("NonlinearNewtonSolver converged (two load steps)", r1.converged && r2.converged, r2.final_residual_relative);
     check("NonlinearNewtonSolver vs closed-form uniaxial", err < 1e-10, err);
+
+    // StressExtractor at the converged uniaxial state against the oracle
+    G::StressExtractor<NeoHookeanMaterial> sx(mat);
+    ZZErrorEstimator zz;
+    auto fd = sx.extract_element_fields(cube, uc, zz);
+    const uint32_t nec = (uint32_t)cube.elements.size();
+    std::vector<double> vm(nec), pr(nec), s1(nec), psi(nec), st(nec), dj(nec), ml(nec, mu), ll(nec, lam);
+    orc_element_fields(nn, Xc.data(), nec, conn.data(), ml.data(), ll.data(), uc.data(), dj.data(), nullptr,
+                       vm.data(), pr.data(), s1.data(), psi.data(), st.data(), nullptr);
+    const double fe = std::fmax(std::fmax(rel(fd.von_mises, vm), rel(fd.pressure, pr)),
+                                std::fmax(std::fmax(rel(fd.max_principal_stress, s1), rel(fd.strain_energy, psi)),
+                                          std::fmax(rel(fd.max_stretch, st), rel(fd.det_F, dj))));
+    check("StressExtractor fields vs oracle", fe < 1e-12, fe);
+    check("StressExtractor error indicators from the estimator",
+          fd.error_indicators.size() == nec && fd.error_indicators[4] == 0.25 && fd.error_indicators[5] == 0.0, 0.0);
     std::printf("%s\n", failures ? "FAILED" : "ALL OK");
     return failures ? 1 : 0;
 }

@@ -99,6 +99,37 @@ def elastic_matrix(N, seed):
     return rp, ci, K
 
 
+def stress_fixture(L):
+    """compute_cauchy_stress (FEM/constitutive_models.hpp:116-148) and
+    compute_von_mises (FEM/stress_extraction.hpp:143-157) of the compiled
+    reference, on the deformation gradients of the pk1 fixture plus a rotation
+    and a pure shear."""
+    L.ref_cauchy.argtypes = [f64p, C.c_double, C.c_double, f64p]
+    L.ref_von_mises.argtypes = [f64p]
+    L.ref_von_mises.restype = C.c_double
+    rng = np.random.default_rng(20240612)
+    Fs = [np.array([[1.1, 0.05, 0.0], [-0.02, 0.95, 0.03], [0.01, 0.0, 1.05]]), np.eye(3)]
+    for _ in range(24):
+        Fs.append(np.eye(3) + rng.uniform(-0.3, 0.3, (3, 3)))
+    Fs.append(np.diag([0.05, 0.1, 0.9]))  # J < 0.01: clamps of ln J and of 1/J
+    c, s_ = np.cos(0.7), np.sin(0.7)
+    Fs.append(np.array([[c, -s_, 0.0], [s_, c, 0.0], [0.0, 0.0, 1.0]]))  # rotation: sigma = 0
+    Fs.append(np.array([[1.0, 0.2, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]]))  # simple shear
+    mats = [(1.0, 0.3), (1.0, 0.49), (210e9, 0.3)]
+    Fa, Ea, nua, Sa, Va = [], [], [], [], []
+    for F in Fs:
+        for E, nu in mats:
+            S = np.zeros(9)
+            L.ref_cauchy(np.ascontiguousarray(F.ravel()), E, nu, S)
+            Fa.append(F.ravel())
+            Ea.append(E)
+            nua.append(nu)
+            Sa.append(S)
+            Va.append(L.ref_von_mises(np.ascontiguousarray(S)))
+    np.savez(os.path.join(HERE, "stress.npz"), F=np.array(Fa), E=np.array(Ea), nu=np.array(nua),
+             sigma=np.array(Sa), von_mises=np.array(Va))
+
+
 def main():
     L = load_ref()
     out = {}
@@ -207,6 +238,7 @@ def main():
     L.ref_expm(np.ascontiguousarray(xi_ref.T.ravel()), E)
     np.savez(os.path.join(HERE, "lie.npz"), w=np.array(ws), expm=np.array(Es), logm=np.array(Ls),
              xi_ref=xi_ref.ravel(), expm_xi_ref=E.reshape(3, 3).T.ravel())
+    stress_fixture(L)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

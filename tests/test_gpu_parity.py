@@ -444,6 +444,36 @@ def test_rotations_match_oracle(fm, S, O):
         assert np.abs(Q[i] - O.rot_exp(0.5 * wo)).max() < 1e-14
 
 
+def test_element_fields_match_oracle(fm, S, O):
+    """Per-element fields (det F, Cauchy stress, von Mises, pressure, largest
+    principal stress, energy, maximum stretch, F) against the FP64 oracle, for
+    a given u and for the device state; a rigid rotation gives zero stress."""
+    cfg = fm.configs.make(2, dims=(5, 4, 3))
+    fm.configs.load(S, cfg)
+    rng = np.random.default_rng(8)
+    u = rng.uniform(-0.04, 0.04, cfg["X"].size)
+    fo = O.element_fields(cfg["X"], cfg["conn"], cfg["mu"], cfg["lam"], u)
+    fg = S.element_fields(u)
+    for k in fo:
+        assert rel(fg[k], fo[k]) < 1e-12, k
+    S.set_u(u)
+    fs = S.element_fields(None, fields=("von_mises", "F"))
+    assert set(fs) == {"von_mises", "F"}
+    assert rel(fs["von_mises"], fo["von_mises"]) < 1e-12 and rel(fs["F"], fo["F"]) < 1e-12
+    # rigid rotation u = (Q - I) X: F = Q, det F = 1, stress and energy zero, stretch one
+    th = 0.9
+    Q = np.array([[np.cos(th), -np.sin(th), 0.0], [np.sin(th), np.cos(th), 0.0], [0.0, 0.0, 1.0]])
+    X = cfg["X"].reshape(3, -1)
+    ur = ((Q - np.eye(3)) @ X).T.ravel()
+    fr = S.element_fields(ur)
+    mu_max = cfg["mu"].max()
+    assert np.abs(fr["det_F"] - 1.0).max() < 1e-13
+    assert np.abs(fr["cauchy"]).max() < 1e-13 * mu_max and np.abs(fr["von_mises"]).max() < 1e-12 * mu_max
+    assert np.abs(fr["energy"]).max() < 1e-13 * mu_max
+    assert np.abs(fr["max_stretch"] - 1.0).max() < 1e-13
+    assert np.abs(fr["F"] - Q.ravel()[:, None]).max() < 1e-13
+
+
 def test_checkpoint_roundtrip(fm, S, tmp_path):
     cfg = fm.configs.make(1, dims=(3, 3, 3))
     fm.configs.load(S, cfg)

@@ -523,3 +523,99 @@ def test_energy_line_search_converges_where_residual_rule_stalls(O):
     u0, u1 = out[0][0], out[1][0]
     assert np.abs(u1 - u0).max() <= 1e-6 * np.abs(u0).max()
     assert abs(out[1][2] - out[0][2]) <= 1e-9 * abs(out[0][2])
+
+
+# ----------------------------------------------------------------------------- element fields (next row 3)
+def test_sym_eig3_matches_lapack(O):
+    rng = np.random.default_rng(7)
+    mats = [np.eye(3) * 2.0, np.diag([1.0, 1.0, 3.0]), np.diag([3.0, 3.0, 1.0]), np.zeros((3, 3)),
+            np.outer([1.0, 2.0, -1.0], [1.0, 2.0, -1.0])]
+    for _ in range(40):
+        A = rng.standard_normal((3, 3)) * 10.0 ** rng.uniform(-3, 3)
+        mats.append(A + A.T)
+    for _ in range(10):  # nearly repeated eigenvalues
+        Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        mats.append(Q @ np.diag([1.0, 1.0 + 1e-9, -2.0]) @ Q.T)
+    for A in mats:
+        w = O.sym_eig3(A)
+        ref = np.sort(np.linalg.eigvalsh(A))[::-1]
+        assert np.abs(w - ref).max() <= 1e-14 * max(np.abs(ref).max(), 1e-300) + 1e-300
+    # only the upper triangle is read
+    A = np.array([[1.0, 2.0, 3.0], [99.0, 4.0, 5.0], [99.0, 99.0, 6.0]])
+    S = np.triu(A) + np.triu(A, 1).T
+    assert np.abs(O.sym_eig3(A) - np.sort(np.linalg.eigvalsh(S))[::-1]).max() < 1e-13
+
+
+def test_cauchy_and_von_mises_match_reference_golden(O):
+    g = golden("stress.npz")
+    for F, E, nu, S, vm in zip(g["F"], g["E"], g["nu"], g["sigma"], g["von_mises"]):
+        mu, lam = lame(E, nu)
+        scale = max(np.abs(S).max(), mu)
+        assert np.abs(O.cauchy(F, mu, lam).ravel() - S).max() <= 1e-13 * scale
+        assert abs(O.von_mises(S) - vm) <= 1e-14 * max(vm, scale)
+
+
+def test_cauchy_closed_forms(O):
+    mu, lam = 0.7, 2.1
+    # uniaxial-type diagonal F: sigma_ii = (mu (a_i^2 - 1) + lam ln J) / J, off-diagonals zero
+    for a in ([1.2, 0.9, 0.9], [0.8, 1.1, 1.05], [1.0, 1.0, 1.0]):
+        F = np.diag(a)
+        J = np.prod(a)
+        s = O.cauchy(F, mu, lam)
+        ref = np.diag([(mu * (ai * ai - 1.0) + lam * np.log(J)) / J for ai in a])
+        assert np.abs(s - ref).max() < 1e-15
+    rng = np.random.default_rng(3)
+    for _ in range(10):
+        F = np.eye(3) + rng.uniform(-0.25, 0.25, (3, 3))
+        J = np.linalg.det(F)
+        # sigma = (mu (F F^T - I) + lam ln J I) / J, symmetric
+        ref = (mu * (F @ F.T - np.eye(3)) + lam * np.log(J) * np.eye(3)) / J
+        s = O.cauchy(F, mu, lam)
+        assert np.abs(s - ref).max() < 1e-14
+        # objectivity: sigma(QF) = Q sigma(F) Q^T
+        Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        if np.linalg.det(Q) < 0:
+            Q[:, 0] *= -1
+        assert np.abs(O.cauchy(Q @ F, mu, lam) - Q @ s @ Q.T).max() < 1e-14
+        # von Mises is invariant too, and zero for a pure pressure
+        assert abs(O.von_mises(Q @ s @ Q.T) - O.von_mises(s)) < 1e-13
+    assert np.abs(O.cauchy(Q, mu, lam)).max() < 1e-14 * mu  # rotation: rounding only
+    assert O.von_mises(np.eye(3) * 5.0) == 0.0
+    # pure shear tau: sqrt(3) tau; uniaxial s: |s|
+    tau = 0.3
+    assert abs(O.von_mises(np.array([[0, tau, 0], [tau, 0, 0], [0, 0, 0]])) - np.sqrt(3) * tau) < 1e-15
+    assert abs(O.von_mises(np.diag([-2.0, 0.0, 0.0])) - 2.0) < 1e-15
+
+
+def test_element_fields_homogeneous_deformation(O):
+    """u = G X gives F = I + G in every element: each field has a closed form."""
+    import flexmesh as fm
+
+    cfg = fm.configs.make(2, dims=(3, 2, 2))  # jittered interior, lognormal mu
+    X = cfg["X"].reshape(3, -1)
+    nn, ne = X.shape[1], cfg["conn"].size // 4
+    G = np.array([[0.1, 0.02, -0.03], [0.01, -0.05, 0.04], [0.0, 0.03, 0.08]])
+    u = (G @ X).T.ravel()
+    f = O.element_fields(cfg["X"], cfg["conn"], cfg["mu"], cfg["lam"], u)
+    F = np.eye(3) + G
+    J = np.linalg.det(F)
+    C = F.T @ F
+    assert np.abs(f["F"] - F.ravel()[:, None]).max() < 1e-14
+    assert np.abs(f["det_F"] - J).max() < 1e-14
+    assert np.abs(f["max_stretch"] - np.sqrt(max(np.linalg.eigvalsh(C).max(), 1.0))).max() < 1e-14
+    for e in range(0, ne, 7):
+        mu, lam = cfg["mu"][e], cfg["lam"][e]
+        s = (mu * (F @ F.T - np.eye(3)) + lam * np.log(J) * np.eye(3)) / J
+        c6 = f["cauchy"][:, e]
+        assert np.abs(c6 - [s[0, 0], s[1, 1], s[2, 2], s[1, 2], s[0, 2], s[0, 1]]).max() < 1e-14
+        assert abs(f["pressure"][e] - np.trace(s) / 3) < 1e-14
+        assert abs(f["max_principal"][e] - np.linalg.eigvalsh(s).max()) < 1e-14
+        dev = s - np.trace(s) / 3 * np.eye(3)
+        assert abs(f["von_mises"][e] - np.sqrt(1.5 * np.sum(dev * dev))) < 1e-14
+        lnJ = np.log(J)
+        assert abs(f["energy"][e] - (0.5 * mu * (np.sum(F * F) - 3) - mu * lnJ + 0.5 * lam * lnJ ** 2)) < 1e-14
+    # zero displacement: F = I, every stress-like field zero, stretch one
+    z = O.element_fields(cfg["X"], cfg["conn"], cfg["mu"], cfg["lam"], None)
+    assert np.abs(z["cauchy"]).max() == 0.0 and np.abs(z["energy"]).max() == 0.0
+    assert np.all(z["max_stretch"] == 1.0) and np.all(z["det_F"] == 1.0)
+    assert nn == cfg["X"].size // 3
