@@ -207,7 +207,9 @@ int ensure_colors(fm_ctx* ctx) {
         std::vector<uint32_t> fill(ptr.begin(), ptr.end() - 1);
         for (uint32_t t = 0; t < ne; ++t)
             for (uint32_t a = 0; a < 4; ++a) ent[fill[connc[(size_t)a * ne + t]]++] = (t << 2) | a;
-        bool ok = ne < (1u << 30);
+        // the lists are also the patches of the ZZ estimator (entries need t < 2^30)
+        ctx->have_n2e = ne < (1u << 30);
+        bool ok = ctx->have_n2e;
         for (uint32_t v = 0; v < nn && ok; ++v)
             ok = ptr[v + 1] - ptr[v] <= kAsmMaxElems && np[v + 1] - np[v] <= kAsmMaxBlk;
         static const bool color_only = [] {
@@ -215,7 +217,7 @@ int ensure_colors(fm_ctx* ctx) {
             return e && std::string(e) == "color";
         }();
         ctx->node_asm = ok && !color_only;
-        if (ctx->node_asm) {
+        if (ctx->have_n2e) {
             FM_HIP(ctx->n2e_ptr.alloc(nn + 1));
             FM_HIP(ctx->n2e.alloc(ent.size()));
             FM_HIP(hipMemcpy(ctx->n2e_ptr.p, ptr.data(), (size_t)(nn + 1) * 4, hipMemcpyHostToDevice));
@@ -305,8 +307,9 @@ ElemArgs elem_args(fm_ctx* ctx, const double* u) {
     a.nn = ctx->nn;
     a.conn = ctx->conn_c.p;
     a.ne = ctx->ne;
-    a.mu = ctx->mu_c.p;
-    a.lam = ctx->lam_c.p;
+    // no material yet (the ZZ estimate needs none): the kernels read zeros
+    a.mu = ctx->have_material ? ctx->mu_c.p : nullptr;
+    a.lam = ctx->have_material ? ctx->lam_c.p : nullptr;
     a.erank = ctx->erank_c.p;
     a.u = u;
     return a;
@@ -319,12 +322,18 @@ int ensure_vectors(fm_ctx* ctx) {
     return FM_OK;
 }
 
-int ensure_ready(fm_ctx* ctx) {
+// mesh-derived state: pattern, colours, node -> element lists, work vectors
+int ensure_mesh(fm_ctx* ctx) {
     if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh (fm_set_mesh)");
     FM_TRY(ensure_pattern(ctx));
     FM_TRY(ensure_colors(ctx));
-    if (!ctx->have_material) return fail(ctx, FM_ESTATE, "no material (fm_set_material)");
     FM_TRY(ensure_vectors(ctx));
+    return FM_OK;
+}
+
+int ensure_ready(fm_ctx* ctx) {
+    FM_TRY(ensure_mesh(ctx));
+    if (!ctx->have_material) return fail(ctx, FM_ESTATE, "no material (fm_set_material)");
     return FM_OK;
 }
 
@@ -1577,6 +1586,46 @@ int fm_element_fields(fm_ctx* ctx, const double* u, const fm_element_fields_out*
     for (const auto& pt : parts)
         if (pt.dst)
             FM_HIP(hipMemcpy(pt.dst, dout.p + (size_t)pt.row * ne, (size_t)pt.rows * ne * 8, hipMemcpyDeviceToHost));
+    return FM_OK;
+}
+
+int fm_zz_estimate(fm_ctx* ctx, const double* u, double* elem_errors, double* node_avg, double* recovered,
+                   double* global_estimate) {
+    if (!ctx) return FM_EINVAL;
+    FM_TRY(ensure_mesh(ctx));
+    if (!ctx->have_n2e) return fail(ctx, FM_EINVAL, "fm_zz_estimate: more than 2^30 elements");
+    const uint32_t nn = ctx->nn, ne = ctx->ne;
+    const double* ud = ctx->u.p;
+    DevBuf<double> du;
+    if (u) {
+        FM_HIP(du.alloc(3ull * nn));
+        FM_HIP(hipMemcpyAsync(du.p, u, 3ull * nn * 8, hipMemcpyHostToDevice, ctx->stream));
+        ud = du.p;
+    } else if (!ctx->u_valid) {
+        return fail(ctx, FM_ESTATE, "no device state (pass u)");
+    }
+    DevBuf<double> zz, raw, rec, err;
+    FM_HIP(zz.alloc((size_t)kZzRows * ne));
+    FM_HIP(raw.alloc(3ull * nn));
+    FM_HIP(rec.alloc(3ull * nn));
+    FM_HIP(err.alloc(ne));
+    FM_TRY(ensure_partials(ctx, ne / 256 + 1));
+    uint32_t np;
+    {
+        ProfScope ps(ctx, FM_K_FIELDS);
+        np = launch_zz(ctx->stream, elem_args(ctx, ud), ctx->eiperm.p, ctx->n2e_ptr.p, ctx->n2e.p, zz.p, raw.p,
+                       rec.p, err.p, ctx->partials.p);
+    }
+    FM_TRY(check_launch(ctx, "zz_estimate"));
+    std::vector<double> part(np);
+    FM_HIP(hipMemcpyAsync(part.data(), ctx->partials.p, (size_t)np * 8, hipMemcpyDeviceToHost, ctx->stream));
+    FM_TRY(sync(ctx));
+    double gsq = 0.0;
+    for (double p : part) gsq += p;
+    if (global_estimate) *global_estimate = std::sqrt(gsq);
+    if (elem_errors) FM_HIP(hipMemcpy(elem_errors, err.p, (size_t)ne * 8, hipMemcpyDeviceToHost));
+    if (node_avg) FM_HIP(hipMemcpy(node_avg, raw.p, 3ull * nn * 8, hipMemcpyDeviceToHost));
+    if (recovered) FM_HIP(hipMemcpy(recovered, rec.p, 3ull * nn * 8, hipMemcpyDeviceToHost));
     return FM_OK;
 }
 

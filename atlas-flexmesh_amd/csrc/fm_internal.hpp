@@ -145,6 +145,7 @@ struct Ctx {
     // node-gather assembly; used when every node is within its limits)
     DevBuf<uint32_t> n2e_ptr, n2e;
     bool node_asm = false;
+    bool have_n2e = false;  // node -> incident element lists (colour order) on the device
     bool have_material = false;
 
     // pattern (node adjacency + tangent matrix)
@@ -306,5 +307,10 @@ void launch_elem_rot(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, d
 // maximum stretch, F (9)
 constexpr int kFieldCount = 21;
 void launch_element_fields(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, double* out);
+// ZZ estimate: zz is kZzRows x ne scratch (colour order), raw and rec 3 nn,
+// err ne (element order); returns the number of partials of sum V sq / 4
+constexpr int kZzRows = 7;
+uint32_t launch_zz(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, const uint32_t* n2e_ptr,
+                   const uint32_t* n2e, double* zz, double* raw, double* rec, double* err, double* partials);
 
 }  // namespace fm

@@ -98,8 +98,8 @@ __device__ __forceinline__ void elem_eval(const ElemArgs& A, uint32_t t, Elem& E
     const double det = det3(J);
     E.V = fabs(det) / 6.0;
     E.ok = E.V >= 1e-30;
-    E.mu = A.mu[t];
-    E.lam = A.lam[t];
+    E.mu = A.mu ? A.mu[t] : 0.0;
+    E.lam = A.lam ? A.lam[t] : 0.0;
     if (!E.ok) return;
     const double id = 1.0 / det;
     // J^{-1} rows (cofactor form of matrix_3x3_inv); g_q = J^{-T} dN_xi,q
@@ -1818,6 +1818,150 @@ __global__ __launch_bounds__(128) void element_fields_kernel(ElemArgs A, const u
     for (int k = 0; k < kFieldCount; ++k) out[(size_t)k * A.ne + e] = f[k];
 }
 
+// ---------------------------------------------------------------------------
+// ZZ recovered-gradient error estimate (ZZErrorEstimator::estimate_with_bounds,
+// FEM/error_estimator.hpp:128-232) in three passes: element samples, node
+// patches, element indicators.
+// ---------------------------------------------------------------------------
+// pass 1, colour order t: raw sample g = (F11 - 1, F22 - 1, F33 - 1), the
+// reference volume |det J|/6 and the centroid; zz is kZzRows x ne SoA in
+// colour order, the order of the node -> element lists (within a colour,
+// element ids ascend, so the patches of neighbouring nodes read nearby
+// entries of every colour)
+__global__ __launch_bounds__(128) void zz_elem_kernel(ElemArgs A, double* __restrict__ zz) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.ne) return;
+    Elem E;
+    double F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    elem_eval(A, t, E, F);
+    double c[3] = {0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) c[d] += 0.25 * A.X[(size_t)d * A.nn + E.node[q]];
+    const double v[kZzRows] = {E.ok ? F[0] - 1.0 : 0.0, E.ok ? F[4] - 1.0 : 0.0, E.ok ? F[8] - 1.0 : 0.0,
+                               E.V, c[0], c[1], c[2]};
+#pragma unroll
+    for (int k = 0; k < kZzRows; ++k) zz[(size_t)k * A.ne + t] = v[k];
+}
+
+// pass 2, one node per lane: volume-weighted average of the incident samples
+// (:148-164) and the SPR fit [1, dx, dy, dz] over at most 32 of them with
+// offsets from the node, so the constant term is the value at the node
+// (:30-99); normal equations eliminated without pivoting as :64-82, and the
+// patch mean where they are singular to 1e-10 of their diagonal
+__global__ __launch_bounds__(128) void zz_node_kernel(uint32_t nn, uint32_t ne, const double* __restrict__ X,
+                                                      const uint32_t* __restrict__ n2e_ptr,
+                                                      const uint32_t* __restrict__ n2e,
+                                                      const double* __restrict__ zz, double* __restrict__ raw,
+                                                      double* __restrict__ rec) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nn) return;
+    const uint32_t p0 = n2e_ptr[v], p1 = n2e_ptr[v + 1];
+    const double x0 = X[v], x1 = X[(size_t)nn + v], x2 = X[2 * (size_t)nn + v];
+    double s[3] = {0, 0, 0}, sv = 0.0, mean[3] = {0, 0, 0};
+    double M[4][7];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int k = 0; k < 7; ++k) M[j][k] = 0.0;
+    // entries in chunks of four whose loads are all issued before any is used
+    // (a plain loop waits a full round trip per entry); past the end, a chunk
+    // re-reads the node's last entry with weight zero
+    constexpr int CH = 4;
+    for (uint32_t p = p0; p < p1; p += CH) {
+        uint32_t t[CH];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) t[k] = n2e[min(p + k, p1 - 1)] >> 2;
+        double z[CH][kZzRows];
+#pragma unroll
+        for (int k = 0; k < CH; ++k)
+#pragma unroll
+            for (int r = 0; r < kZzRows; ++r) z[k][r] = zz[(size_t)r * ne + t[k]];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            if (p + k >= p1) break;
+            const double vol = z[k][3];
+            s[0] += z[k][0] * vol;
+            s[1] += z[k][1] * vol;
+            s[2] += z[k][2] * vol;
+            sv += vol;
+            if (p + k - p0 < 32) {
+                const double b[4] = {1.0, z[k][4] - x0, z[k][5] - x1, z[k][6] - x2};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) M[j][q] += b[j] * b[q];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) M[j][4 + c] += b[j] * z[k][c];
+                }
+            }
+        }
+    }
+    const double m = (double)min(p1 - p0, 32u);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) mean[c] = m > 0 ? M[0][4 + c] / m : 0.0;
+    double diag[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) diag[j] = M[j][j];
+    bool singular = m == 0;
+#pragma unroll
+    for (int pv = 0; pv < 4; ++pv) {
+        singular = singular || !(fabs(M[pv][pv]) > 1e-10 * diag[pv]);
+        const double ip = singular ? 0.0 : 1.0 / M[pv][pv];
+#pragma unroll
+        for (int rr = pv + 1; rr < 4; ++rr) {
+            const double f = M[rr][pv] * ip;
+#pragma unroll
+            for (int c = pv; c < 7; ++c) M[rr][c] -= f * M[pv][c];
+        }
+    }
+    double sol[4][3];
+#pragma unroll
+    for (int pv = 3; pv >= 0; --pv)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double rhs = M[pv][4 + c];
+#pragma unroll
+            for (int k = pv + 1; k < 4; ++k) rhs -= M[pv][k] * sol[k][c];
+            sol[pv][c] = singular ? 0.0 : rhs / M[pv][pv];
+        }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        raw[3 * (size_t)v + c] = sv > 1e-30 ? s[c] / sv : s[c];
+        rec[3 * (size_t)v + c] = singular ? mean[c] : sol[0][c];
+    }
+}
+
+// pass 3, element order e: indicator sqrt(sum |rec - raw|^2 / 4) cbrt(V) and
+// the block partial of V sum / 4 (:198-214)
+__global__ __launch_bounds__(256) void zz_err_kernel(uint32_t ne, const uint32_t* __restrict__ conn_c,
+                                                     const uint32_t* __restrict__ eiperm,
+                                                     const double* __restrict__ zz, const double* __restrict__ raw,
+                                                     const double* __restrict__ rec, double* __restrict__ err,
+                                                     double* __restrict__ partials) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    double part = 0.0;
+    if (e < ne) {
+        const uint32_t t = eiperm[e];
+        double sq = 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const uint32_t v = conn_c[(size_t)a * ne + t];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double d = rec[3 * (size_t)v + c] - raw[3 * (size_t)v + c];
+                sq += d * d;
+            }
+        }
+        const double vol = zz[3 * (size_t)ne + t];
+        err[e] = sqrt(sq / 4.0) * cbrt(vol);
+        part = vol * sq / 4.0;
+    }
+    const double bs = block_sum(part);
+    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 // waves of the persistent kernels: few for the sweeps (only the frontier
@@ -2246,6 +2390,15 @@ uint32_t launch_energy_elem(hipStream_t s, const ElemArgs& a, double* partials) 
 }
 void launch_rot_batch(hipStream_t s, uint32_t n, const double* F, double alpha, double* R, double* w, double* Q) {
     if (n) rot_batch_kernel<<<cdiv(n, 128), 128, 0, s>>>(n, F, alpha, R, w, Q);
+}
+uint32_t launch_zz(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, const uint32_t* n2e_ptr,
+                   const uint32_t* n2e, double* zz, double* raw, double* rec, double* err, double* partials) {
+    if (!a.ne) return 0;
+    zz_elem_kernel<<<cdiv(a.ne, 128), 128, 0, s>>>(a, zz);
+    zz_node_kernel<<<cdiv(a.nn, 128), 128, 0, s>>>(a.nn, a.ne, a.X, n2e_ptr, n2e, zz, raw, rec);
+    const uint32_t nb = cdiv(a.ne, 256);
+    zz_err_kernel<<<nb, 256, 0, s>>>(a.ne, a.conn, eiperm, zz, raw, rec, err, partials);
+    return nb;
 }
 void launch_element_fields(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, double* out) {
     if (a.ne) element_fields_kernel<<<cdiv(a.ne, 128), 128, 0, s>>>(a, eiperm, out);

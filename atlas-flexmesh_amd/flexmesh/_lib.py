@@ -95,6 +95,7 @@ def lib():
     L.fm_rot_logexp_batch.argtypes = [_vp, C.c_uint32, _f64p, C.c_double, _vp, _vp, _vp]
     L.fm_element_rotations.argtypes = [_vp, _f64p]
     L.fm_element_fields.argtypes = [_vp, _vp, C.POINTER(ElementFieldsOut)]
+    L.fm_zz_estimate.argtypes = [_vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_double)]
     L.fm_checkpoint_save.argtypes = [_vp, C.c_char_p, C.c_uint32]
     L.fm_checkpoint_load.argtypes = [_vp, C.c_char_p, C.POINTER(C.c_uint32)]
     L.fm_profile_enable.argtypes = [_vp, C.c_int]
@@ -330,6 +331,16 @@ class Solver:
             setattr(o, k, out[k].ctypes.data)
         self._chk(self.L.fm_element_fields(self.h, _ptr(uu), C.byref(o)), "fm_element_fields")
         return out
+
+    def zz_estimate(self, u=None):
+        """ZZ estimate of u (None: the device state): (global, element indicators,
+        node averages 3*nn, recovered values 3*nn)."""
+        uu = None if u is None else f64(u)
+        err, raw, rec = np.zeros(self.ne), np.zeros(3 * self.nn), np.zeros(3 * self.nn)
+        g = C.c_double(0.0)
+        self._chk(self.L.fm_zz_estimate(self.h, _ptr(uu), err.ctypes.data, raw.ctypes.data, rec.ctypes.data,
+                                        C.byref(g)), "fm_zz_estimate")
+        return g.value, err, raw, rec
 
     def checkpoint_save(self, path, load_step):
         self._chk(self.L.fm_checkpoint_save(self.h, path.encode(), load_step), "fm_checkpoint_save")

@@ -72,6 +72,11 @@ def kernel_bytes(info, kind, v, launches_per_unit=None):
         return int((ne * (16 + 2 * 8) + nn * 24 + n * 8 + nnz * v + n * 8) / per)
     if kind == "residual":
         return int((ne * (16 + 2 * 8) + nn * 24 + n * 8 + n * 8) / per)
+    if kind == "zz":
+        # samples: connectivity, material, X and u once, 7 values written and read
+        # back, the node -> element lists; node averages and fits written and read
+        # back; indicators written
+        return ne * (16 + 16 + 2 * 7 * 8 + 16 + 4 + 8) + nn * (24 + 24 + 2 * 48)
     if kind == "fields":
         # connectivity and the two material values per element, X and u once per
         # node, 21 FP64 outputs per element
@@ -184,18 +189,21 @@ def run_steps(S, cfg, prec, first, count, inner_tol, profile, pcg_max=1000):
     return wall, tot, worst, prof
 
 
-def element_fields_timing(S, info, reps=5):
-    """The per-element field pass over the converged state (after the timed
-    region, not part of `value`): HIP-event time of its kernel and algorithmic
-    bandwidth."""
+def element_fields_timing(S, info, kind="fields", reps=5):
+    """A post-processing pass over the converged state (after the timed region,
+    not part of `value`): the element fields or the ZZ estimate (three kernels),
+    HIP-event time per call and algorithmic bandwidth."""
     S.profile(True)
     S.profile_reset()
     for _ in range(reps):
-        S.element_fields(None, fields=())
+        if kind == "fields":
+            S.element_fields(None, fields=())
+        else:
+            S.zz_estimate()
     ms, n = S.profile_get()["fields"]
     S.profile(False)
     us = 1e3 * ms / max(n, 1)
-    b = kernel_bytes(info, "fields", 8)
+    b = kernel_bytes(info, kind, 8)
     return {"avg_us": us, "launches": n, "bytes_per_launch": b, "GBps": b / (us * 1e-6) / 1e9,
             "frac_of_hbm_peak": b / (us * 1e-6) / 1e9 / PEAK_HBM_GBS}
 
@@ -305,6 +313,7 @@ def main():
     if "spmv" in kern:
         out["spmv_frac_of_peak"] = kern["spmv"]["GBps"] / PEAK_HBM_GBS
     out["element_fields"] = element_fields_timing(S, info)
+    out["zz_estimate"] = element_fields_timing(S, info, "zz")
     # FP64 GPU path on the same steps from the same state
     if mixed and not args.no_fp64:
         S.set_u(u0)

@@ -194,6 +194,19 @@ typedef struct fm_element_fields_out {
 } fm_element_fields_out;
 int fm_element_fields(fm_ctx* ctx, const double* u, const fm_element_fields_out* out);
 
+/* ---- ZZ recovered-gradient error estimate (FP64) --------------------------------------
+ * Replaces ZZErrorEstimator::estimate_with_bounds / estimate
+ * (FEM/error_estimator.hpp:128-232) for the displacement u (3*nn, NULL: the
+ * device-resident state): raw element samples g = (F11 - 1, F22 - 1, F33 - 1),
+ * their volume-weighted node averages (node_avg, 3*nn), SPR linear fits over
+ * each node's patch of at most 32 incident elements evaluated at the node
+ * (recovered, 3*nn), element indicators sqrt(sum |recovered - avg|^2 / 4) cbrt(V_e)
+ * (elem_errors, ne, element order) and the global estimate sqrt(sum V_e sum / 4).
+ * A patch whose fit is undetermined (fewer than four non-coplanar centroids)
+ * takes the patch mean.  Any output may be NULL. */
+int fm_zz_estimate(fm_ctx* ctx, const double* u, double* elem_errors, double* node_avg, double* recovered,
+                   double* global_estimate);
+
 /* ---- checkpoint ----------------------------------------------------------------------
  * Versioned, checksummed load-step checkpoint (superset of CheckpointFile,
  * FEM/adaptive_fem_engine.hpp:1347-1452): magic, version, counts, load step,

@@ -224,6 +224,47 @@ public:
     }
 };
 
+// ZZErrorEstimator (FEM/error_estimator.hpp:106-239) over fm_zz_estimate.  The
+// reference reads the element states of the mesh; the GPU twin takes the
+// displacement they come from (the states' F is recomputed exactly, without
+// the det F = 1 projection).  estimate() returns the global estimate and fills
+// the element indicators; estimate_with_bounds() adds the reference's bounds.
+template <class Mesh>
+class ZZErrorEstimator {
+public:
+    static constexpr double C_EFFICIENCY_LINEAR_ELEMENTS = 1.0;
+    static constexpr double C_RELIABILITY_LINEAR_ELEMENTS = 2.0;
+    struct EstimateResult {
+        double global_estimate{0.0};
+        double lower_bound{0.0};
+        double upper_bound{0.0};
+        double effective_mean_ratio{0.0};
+        bool meets_reliability{false};
+        std::vector<double> elem_errors;
+    };
+    ZZErrorEstimator(const Mesh& mesh, const std::vector<double>& u) : mesh_(mesh), u_(u) {}
+
+    EstimateResult estimate_with_bounds(std::vector<double>& elem_errors) const {
+        Device& d = Device::get();
+        d.upload(mesh_);
+        if (u_.size() != 3 * mesh_.nodes.size()) throw std::runtime_error("flexmesh: u is not 3 * n_nodes long");
+        elem_errors.assign(mesh_.elements.size(), 0.0);
+        EstimateResult r;
+        d.check(fm_zz_estimate(d.ctx(), u_.data(), elem_errors.data(), nullptr, nullptr, &r.global_estimate),
+                "fm_zz_estimate");
+        r.elem_errors = elem_errors;
+        r.lower_bound = r.global_estimate / C_RELIABILITY_LINEAR_ELEMENTS;
+        r.upper_bound = r.global_estimate * C_EFFICIENCY_LINEAR_ELEMENTS;
+        r.meets_reliability = true;
+        return r;
+    }
+    double estimate(std::vector<double>& elem_errors) const { return estimate_with_bounds(elem_errors).global_estimate; }
+
+private:
+    const Mesh& mesh_;
+    const std::vector<double>& u_;
+};
+
 // NewtonSolverConfig / NewtonSolverResult (FEM/nonlinear_solver.hpp:23-64)
 struct NewtonSolverConfig {
     int max_iterations = 50;

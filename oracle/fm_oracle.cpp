@@ -1038,4 +1038,146 @@ void orc_element_fields(uint32_t nn, const double* X, uint32_t ne, const uint32_
     }
 }
 
+// ---------------------------------------------------------------------------
+// ZZErrorEstimator::estimate_with_bounds (FEM/error_estimator.hpp:128-232).
+// Raw element "gradient" g_e = (F11 - 1, F22 - 1, F33 - 1) of the element
+// state (:141-146), volume-weighted node averages (:148-164), an SPR linear fit
+// [1, x, y, z] over each node's patch of at most 32 incident elements in
+// element order (:30-99, :166-196), element indicators
+// sqrt(sum over the 4 nodes of |recovered - average|^2 / 4) cbrt(V_e) and the
+// global sqrt(sum V_e sum/4) (:198-214).  Corrections: F is the exact,
+// unprojected gradient; the fit uses coordinates relative to the
+// node, so its constant term is the recovered value at the node (the
+// reference fits in global coordinates and takes the value at the origin,
+// contrary to its own comment at :94-96); a patch whose normal equations are
+// singular to 1e-10 of their diagonal (fewer than four non-coplanar
+// centroids, e.g. at corners, where the reference's absolute 1e-30 pivot test
+// divides by rounding noise) takes the constant fit, the patch mean.
+// ---------------------------------------------------------------------------
+namespace {
+// SPR fit of one patch: m samples (centroid offsets d, raw gradients r);
+// returns the value of the fit at offset 0
+void spr_fit(int m, const double (*d)[3], const double (*r)[3], double g[3]) {
+    if (m == 0) {
+        g[0] = g[1] = g[2] = 0.0;
+        return;
+    }
+    double M[4][7] = {};
+    for (int e = 0; e < m; ++e) {
+        const double b[4] = {1.0, d[e][0], d[e][1], d[e][2]};
+        for (int j = 0; j < 4; ++j) {
+            for (int k = 0; k < 4; ++k) M[j][k] += b[j] * b[k];
+            for (int c = 0; c < 3; ++c) M[j][4 + c] += b[j] * r[e][c];
+        }
+    }
+    double diag[4];
+    for (int j = 0; j < 4; ++j) diag[j] = M[j][j];
+    bool singular = false;
+    // elimination without pivoting, as :64-82 (the normal matrix is SPD)
+    for (int p = 0; p < 4 && !singular; ++p) {
+        if (!(std::fabs(M[p][p]) > 1e-10 * diag[p])) {
+            singular = true;
+            break;
+        }
+        const double ip = 1.0 / M[p][p];
+        for (int rr = p + 1; rr < 4; ++rr) {
+            const double f = M[rr][p] * ip;
+            for (int c = p; c < 7; ++c) M[rr][c] -= f * M[p][c];
+        }
+    }
+    if (singular) {
+        for (int c = 0; c < 3; ++c) {
+            double s = 0.0;
+            for (int e = 0; e < m; ++e) s += r[e][c];
+            g[c] = s / m;
+        }
+        return;
+    }
+    double sol[4][3];
+    for (int p = 3; p >= 0; --p)
+        for (int c = 0; c < 3; ++c) {
+            double rhs = M[p][4 + c];
+            for (int k = p + 1; k < 4; ++k) rhs -= M[p][k] * sol[k][c];
+            sol[p][c] = rhs / M[p][p];
+        }
+    for (int c = 0; c < 3; ++c) g[c] = sol[0][c];
+}
+}  // namespace
+
+double orc_zz_estimate(uint32_t nn, const double* X, uint32_t ne, const uint32_t* conn, const double* u,
+                       double* elem_err, double* raw_node, double* recovered) {
+    std::vector<double> g(3 * (size_t)ne, 0.0), vol(ne, 0.0), cen(3 * (size_t)ne, 0.0);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int64_t ei = 0; ei < (int64_t)ne; ++ei) {
+        const uint32_t e = (uint32_t)ei;
+        double Xe[12], ue[12];
+        uint32_t nodes[4];
+        gather_elem(X, nn, conn, ne, e, u, Xe, ue, nodes);
+        for (int a = 0; a < 4; ++a)
+            for (int c = 0; c < 3; ++c) cen[3 * (size_t)e + c] += 0.25 * Xe[a * 3 + c];
+        ElemGeo G;
+        const bool ok = elem_geometry(Xe, G);
+        double J[9];
+        for (int c = 0; c < 3; ++c) {
+            J[c * 3 + 0] = Xe[3 + c] - Xe[c];
+            J[c * 3 + 1] = Xe[6 + c] - Xe[c];
+            J[c * 3 + 2] = Xe[9 + c] - Xe[c];
+        }
+        vol[e] = std::fabs(det3(J)) / 6.0;
+        if (ok) {
+            ElemKin K;
+            elem_kinematics(G, ue, K);
+            for (int c = 0; c < 3; ++c) g[3 * (size_t)e + c] = K.F[c * 4] - 1.0;
+        }
+    }
+    // node -> elements in element order (generate_unit_cube_mesh :2279-2291)
+    std::vector<uint32_t> ptr(nn + 1, 0), data(4 * (size_t)ne);
+    for (uint32_t e = 0; e < ne; ++e)
+        for (int a = 0; a < 4; ++a) ptr[conn[(size_t)a * ne + e] + 1]++;
+    for (uint32_t v = 0; v < nn; ++v) ptr[v + 1] += ptr[v];
+    {
+        std::vector<uint32_t> fill(ptr.begin(), ptr.end() - 1);
+        for (uint32_t e = 0; e < ne; ++e)
+            for (int a = 0; a < 4; ++a) data[fill[conn[(size_t)a * ne + e]]++] = e;
+    }
+    std::vector<double> raw(3 * (size_t)nn, 0.0), rec(3 * (size_t)nn, 0.0);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int64_t vi = 0; vi < (int64_t)nn; ++vi) {
+        const uint32_t v = (uint32_t)vi;
+        double s[3] = {0, 0, 0}, sv = 0.0;
+        for (uint32_t p = ptr[v]; p < ptr[v + 1]; ++p) {
+            const uint32_t e = data[p];
+            for (int c = 0; c < 3; ++c) s[c] += g[3 * (size_t)e + c] * vol[e];
+            sv += vol[e];
+        }
+        for (int c = 0; c < 3; ++c) raw[3 * (size_t)v + c] = sv > 1e-30 ? s[c] / sv : s[c];
+        double d[32][3], r[32][3];
+        int m = 0;
+        for (uint32_t p = ptr[v]; p < ptr[v + 1] && m < 32; ++p, ++m) {
+            const uint32_t e = data[p];
+            for (int c = 0; c < 3; ++c) {
+                d[m][c] = cen[3 * (size_t)e + c] - X[(size_t)c * nn + v];
+                r[m][c] = g[3 * (size_t)e + c];
+            }
+        }
+        spr_fit(m, d, r, &rec[3 * (size_t)v]);
+    }
+    double gsq = 0.0;
+    for (uint32_t e = 0; e < ne; ++e) {
+        double sq = 0.0;
+        for (int a = 0; a < 4; ++a) {
+            const uint32_t v = conn[(size_t)a * ne + e];
+            for (int c = 0; c < 3; ++c) {
+                const double df = rec[3 * (size_t)v + c] - raw[3 * (size_t)v + c];
+                sq += df * df;
+            }
+        }
+        if (elem_err) elem_err[e] = std::sqrt(sq / 4.0) * std::cbrt(vol[e]);
+        gsq += vol[e] * sq / 4.0;
+    }
+    if (raw_node) std::memcpy(raw_node, raw.data(), raw.size() * sizeof(double));
+    if (recovered) std::memcpy(recovered, rec.data(), rec.size() * sizeof(double));
+    return std::sqrt(gsq);
+}
+
 }  // extern "C"

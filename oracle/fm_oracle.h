@@ -128,6 +128,11 @@ void orc_element_fields(uint32_t nn, const double* X, uint32_t ne, const uint32_
                         const double* lam, const double* u, double* det_F, double* cauchy, double* vm,
                         double* pressure, double* s1, double* psi, double* stretch, double* F);
 
+/* ZZ recovered-gradient error estimate: element indicators, node averages and
+ * SPR-recovered values (3*nn, node-major), global estimate returned */
+double orc_zz_estimate(uint32_t nn, const double* X, uint32_t ne, const uint32_t* conn, const double* u,
+                       double* elem_err, double* raw_node, double* recovered);
+
 /* a22: adjoint at a converged state of the final load step (load factor 1) */
 int orc_adjoint(const orc_problem* P, const double* u, const double* dLdu, double tol, int max_iter,
                 double* dL_dmu, double* dL_dlam, double* dL_dload, double* lam_out);

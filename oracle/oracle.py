@@ -86,6 +86,8 @@ def lib():
         L.orc_von_mises.argtypes = [_f64p]
         L.orc_von_mises.restype = C.c_double
         L.orc_element_fields.argtypes = [C.c_uint32, _f64p, C.c_uint32, _u32p, _f64p, _f64p, _vp] + [_vp] * 8
+        L.orc_zz_estimate.argtypes = [C.c_uint32, _f64p, C.c_uint32, _u32p, _vp, _vp, _vp, _vp]
+        L.orc_zz_estimate.restype = C.c_double
         L.orc_polar.argtypes = [_f64p, _f64p]
         L.orc_polar.restype = C.c_int
         L.orc_rot_log.argtypes = [_f64p, _f64p]
@@ -227,6 +229,18 @@ def element_fields(X, conn, mu, lam, u):
                              np.ascontiguousarray(lam, dtype=np.float64), None if u is None else u.ctypes.data,
                              *[out[k].ctypes.data for k in FIELD_NAMES])
     return out
+
+
+def zz_estimate(X, conn, u):
+    """ZZ estimate: (global, element indicators, node averages 3*nn, recovered 3*nn)."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    conn = np.ascontiguousarray(conn, dtype=np.uint32)
+    nn, ne = X.size // 3, conn.size // 4
+    u = None if u is None else np.ascontiguousarray(u, dtype=np.float64)
+    err, raw, rec = np.zeros(ne), np.zeros(3 * nn), np.zeros(3 * nn)
+    g = lib().orc_zz_estimate(nn, X, ne, conn, None if u is None else u.ctypes.data, err.ctypes.data,
+                              raw.ctypes.data, rec.ctypes.data)
+    return g, err, raw, rec
 
 
 def polar(F):

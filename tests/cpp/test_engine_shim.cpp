@@ -190,6 +190,20 @@ int run() {
                                 std::fmax(std::fmax(rel(fd.max_principal_stress, s1), rel(fd.strain_energy, psi)),
                                           std::fmax(rel(fd.max_stretch, st), rel(fd.det_F, dj))));
     check("StressExtractor fields vs oracle", fe < 1e-12, fe);
+    std::vector<double> u2(uc);  // a non-uniform field, so the indicators are not rounding noise
+    for (size_t i = 0; i < u2.size(); ++i) u2[i] += 1e-3 * std::sin(0.37 * (double)i);
+    G::ZZErrorEstimator<MeshTopology> zzg(cube, u2);
+    std::vector<double> ezz;
+    auto zr = zzg.estimate_with_bounds(ezz);
+    std::vector<double> ezo(nec);
+    const double gzo = orc_zz_estimate(nn, Xc.data(), nec, conn.data(), u2.data(), ezo.data(), nullptr, nullptr);
+    double ezmax = 0.0;
+    for (double v : ezo) ezmax = std::fmax(ezmax, v);
+    double ezd = 0.0;
+    for (uint32_t e = 0; e < nec; ++e) ezd = std::fmax(ezd, std::fabs(ezz[e] - ezo[e]));
+    check("ZZErrorEstimator vs oracle", std::fabs(zr.global_estimate - gzo) <= 1e-9 * gzo + 1e-300 &&
+                                            ezd <= 1e-9 * ezmax + 1e-300 && zr.lower_bound == zr.global_estimate / 2,
+          ezd);
     check("StressExtractor error indicators from the estimator",
           fd.error_indicators.size() == nec && fd.error_indicators[4] == 0.25 && fd.error_indicators[5] == 0.0, 0.0);
     std::printf("%s\n", failures ? "FAILED" : "ALL OK");

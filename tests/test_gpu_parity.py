@@ -474,6 +474,36 @@ def test_element_fields_match_oracle(fm, S, O):
     assert np.abs(fr["F"] - Q.ravel()[:, None]).max() < 1e-13
 
 
+def test_zz_estimate_matches_oracle(fm, S, O):
+    """ZZ recovered-gradient estimate: node averages, SPR-recovered values,
+    element indicators and the global estimate against the FP64 oracle (the
+    patches are summed in another order: rounding-level differences, amplified
+    by the patch normal equations); the device state; and a context with a
+    mesh but no material."""
+    cfg = fm.configs.make(2, dims=(5, 4, 3))
+    fm.configs.load(S, cfg)
+    Xn = cfg["X"].reshape(3, -1)
+    rng = np.random.default_rng(9)
+    u = np.stack([0.05 * Xn[0] ** 2, 0.03 * Xn[0] * Xn[1], -0.02 * Xn[2] ** 2]).T.ravel()
+    u += rng.uniform(-1e-3, 1e-3, u.size)
+    go, eo, ro, co = O.zz_estimate(cfg["X"], cfg["conn"], u)
+    gg, eg, rg, cg = S.zz_estimate(u)
+    scale = np.abs(ro).max()
+    assert np.abs(rg - ro).max() < 1e-13 * scale
+    assert np.abs(cg - co).max() < 1e-10 * scale
+    assert np.abs(eg - eo).max() < 1e-9 * eo.max() and abs(gg - go) < 1e-9 * go
+    S.set_u(u)
+    gs, es, _, _ = S.zz_estimate()
+    assert gs == gg and np.array_equal(es, eg)
+    T = fm.Solver(0)
+    try:
+        T.set_mesh(cfg["X"], cfg["conn"])
+        gt, et, _, _ = T.zz_estimate(u)
+        assert gt == gg and np.array_equal(et, eg)
+    finally:
+        T.close()
+
+
 def test_checkpoint_roundtrip(fm, S, tmp_path):
     cfg = fm.configs.make(1, dims=(3, 3, 3))
     fm.configs.load(S, cfg)

@@ -619,3 +619,79 @@ def test_element_fields_homogeneous_deformation(O):
     assert np.abs(z["cauchy"]).max() == 0.0 and np.abs(z["energy"]).max() == 0.0
     assert np.all(z["max_stretch"] == 1.0) and np.all(z["det_F"] == 1.0)
     assert nn == cfg["X"].size // 3
+
+
+# ----------------------------------------------------------------------------- ZZ estimator (next row 4)
+def zz_numpy(X, conn, u):
+    """Independent restatement with numpy: F per element from the P1 gradients,
+    volume-weighted node averages, least-squares patch fits (lstsq) in node-relative
+    coordinates, constant fit where the linear fit is rank-deficient."""
+    X = X.reshape(3, -1)
+    nn, ne = X.shape[1], conn.size // 4
+    cn = conn.reshape(4, ne)
+    uu = u.reshape(-1, 3)
+    g = np.zeros((ne, 3))
+    vol = np.zeros(ne)
+    cen = np.zeros((ne, 3))
+    for e in range(ne):
+        P = X[:, cn[:, e]].T
+        D = (P[1:] - P[0]).T
+        vol[e] = abs(np.linalg.det(D)) / 6
+        cen[e] = P.mean(axis=0)
+        U = uu[cn[:, e]]
+        Gu = (U[1:] - U[0]).T @ np.linalg.inv(D)
+        g[e] = np.diag(Gu)
+    raw = np.zeros((nn, 3))
+    rec = np.zeros((nn, 3))
+    for v in range(nn):
+        es = np.where((cn == v).any(axis=0))[0]
+        raw[v] = (g[es] * vol[es, None]).sum(0) / vol[es].sum()
+        A = np.hstack([np.ones((es.size, 1)), cen[es] - X[:, v]])
+        if np.linalg.matrix_rank(A, tol=1e-6 * np.abs(A).max()) < 4:
+            rec[v] = g[es].mean(0)
+        else:
+            rec[v] = np.linalg.lstsq(A, g[es], rcond=None)[0][0]
+    sq = ((rec[cn] - raw[cn]) ** 2).sum(axis=(0, 2))
+    return np.sqrt((vol * sq / 4).sum()), np.sqrt(sq / 4) * np.cbrt(vol), raw.ravel(), rec.ravel()
+
+
+def test_zz_estimate_matches_numpy_restatement(O):
+    import flexmesh as fm
+
+    cfg = fm.configs.make(2, dims=(3, 3, 2))
+    rng = np.random.default_rng(4)
+    Xn = cfg["X"].reshape(3, -1)
+    # a smooth nonlinear field plus noise: the recovered and raw values differ
+    u = np.stack([0.05 * Xn[0] ** 2, 0.03 * Xn[0] * Xn[1], -0.02 * Xn[2] ** 2]).T.ravel()
+    u += rng.uniform(-1e-3, 1e-3, u.size)
+    go, eo, ro, co = O.zz_estimate(cfg["X"], cfg["conn"], u)
+    gn, en, rn, cnum = zz_numpy(cfg["X"], cfg["conn"], u)
+    scale = np.abs(ro).max()
+    assert np.abs(ro - rn).max() < 1e-13 * scale
+    assert np.abs(co - cnum).max() < 1e-10 * scale
+    assert np.abs(eo - en).max() < 1e-10 * en.max() and abs(go - gn) < 1e-10 * gn
+    assert gn > 0
+
+
+def test_zz_estimate_vanishes_for_homogeneous_and_recovers_linear_fields(O):
+    import flexmesh as fm
+
+    cfg = fm.configs.make(2, dims=(3, 2, 2))
+    Xn = cfg["X"].reshape(3, -1)
+    G = np.array([[0.1, 0.02, -0.03], [0.01, -0.05, 0.04], [0.0, 0.03, 0.08]])
+    g, e, raw, rec = O.zz_estimate(cfg["X"], cfg["conn"], (G @ Xn).T.ravel())
+    assert abs(g) < 1e-14 and np.abs(e).max() < 1e-14
+    assert np.abs(rec.reshape(-1, 3) - np.diag(G)).max() < 1e-13
+    assert np.abs(raw.reshape(-1, 3) - np.diag(G)).max() < 1e-14
+    # u_x = a x^2 / 2 on an undistorted grid: F11 - 1 is linear in the centroid x, so
+    # the SPR fit at an interior node returns a x_node exactly (an evaluation at the
+    # origin instead of the node would return ~0)
+    cfg0 = fm.configs.make(1, dims=(4, 4, 4))
+    X0 = cfg0["X"].reshape(3, -1)
+    a = 0.2
+    u = np.zeros((X0.shape[1], 3))
+    u[:, 0] = 0.5 * a * X0[0] ** 2
+    _, _, _, rec = O.zz_estimate(cfg0["X"], cfg0["conn"], u.ravel())
+    interior = np.all((X0 > 1e-9) & (X0 < 1 - 1e-9), axis=0)
+    assert interior.sum() == 27
+    assert np.abs(rec.reshape(-1, 3)[interior, 0] - a * X0[0, interior]).max() < 1e-13
