@@ -963,6 +963,7 @@ int fm_field_lognormal(uint32_t n, double median, double sigma, uint64_t seed, d
 
 int fm_set_mesh(fm_ctx* ctx, uint32_t n_nodes, const double* X, uint32_t n_elems, const uint32_t* conn) {
     if (!ctx || !X || !conn || !n_nodes || !n_elems) return fail(ctx, FM_EINVAL, "fm_set_mesh: bad arguments");
+    ctx->sec_n = 0;
     if (3ull * n_nodes > 0xFFFFFFFFull) return fail(ctx, FM_EINVAL, "too many nodes");
     for (size_t k = 0; k < 4ull * n_elems; ++k)
         if (conn[k] >= n_nodes) return fail(ctx, FM_EINVAL, "connectivity refers to a node out of range");
@@ -988,6 +989,7 @@ int fm_set_mesh(fm_ctx* ctx, uint32_t n_nodes, const double* X, uint32_t n_elems
 
 int fm_set_material(fm_ctx* ctx, const double* mu_e, const double* lam_e) {
     if (!ctx || !mu_e || !lam_e) return fail(ctx, FM_EINVAL, "fm_set_material: bad arguments");
+    ctx->sec_n = 0;
     if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
     ctx->hmu.assign(mu_e, mu_e + ctx->ne);
     ctx->hlam.assign(lam_e, lam_e + ctx->ne);
@@ -997,6 +999,7 @@ int fm_set_material(fm_ctx* ctx, const double* mu_e, const double* lam_e) {
 
 int fm_set_bcs(fm_ctx* ctx, uint32_t n_dir, const uint32_t* dofs, const double* vals) {
     if (!ctx || (n_dir && !dofs)) return fail(ctx, FM_EINVAL, "fm_set_bcs: bad arguments");
+    ctx->sec_n = 0;
     if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
     const uint32_t n = 3 * ctx->nn;
     std::vector<uint8_t> mask(n, 0);
@@ -1020,6 +1023,7 @@ int fm_set_bcs(fm_ctx* ctx, uint32_t n_dir, const uint32_t* dofs, const double* 
 
 int fm_set_load(fm_ctx* ctx, const double* f_ext) {
     if (!ctx) return FM_EINVAL;
+    ctx->sec_n = 0;
     if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
     if (!f_ext) {
         ctx->have_fext = false;
@@ -1328,6 +1332,7 @@ void fm_newton_cfg_default(fm_newton_cfg* c) {
 
 int fm_set_u(fm_ctx* ctx, const double* u) {
     if (!ctx || !u) return FM_EINVAL;
+    ctx->sec_n = 0;
     FM_TRY(ensure_vectors(ctx));
     FM_HIP(hipMemcpy(ctx->u.p, u, 3ull * ctx->nn * 8, hipMemcpyHostToDevice));
     ctx->u_valid = true;
@@ -1374,10 +1379,14 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     if (u) {
         FM_HIP(hipMemcpyAsync(ctx->u.p, u, n * 8ull, hipMemcpyHostToDevice, s));
         ctx->u_valid = true;
+        ctx->sec_n = 0;  // a state from outside: no known history
     }
     if (!ctx->u_valid) {
+        // u = 0 is the equilibrium at load factor 0 (no load, no prescribed values)
         launch_fill_f64(s, ctx->u.p, n, 0.0);
         ctx->u_valid = true;
+        ctx->sec_n = 1;
+        ctx->sec_lf1 = 0.0;
     }
     // linear-solve mode and the precision the tangent is assembled in
     const int kprec = cfg.prec == FM_FP64 ? PREC64 : (cfg.prec == FM_MIXED ? PREC32 : PRECPC);
@@ -1392,7 +1401,24 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     launch_axpy_to(s, n, ctx->u.p, 1.0, ctx->dD.p, ctx->ut.p);
     FM_TRY(residual(ctx, ctx->ut.p, lf, &S.r0));
     S.residuals++;
-    if (cfg.lift && any_inc) {
+    // lift 2: secant predictor from the two previous converged states of this
+    // load path, u + (lf - lf1)/(lf1 - lf0) (u - u0), with the prescribed values
+    // set exactly (no linear solve); the predictor solve below where there is
+    // no such history
+    const bool secant = cfg.lift == 2 && ctx->sec_n == 2 && ctx->sec_lf1 != ctx->sec_lf0;
+    if (cfg.lift == 2) {
+        FM_HIP(ctx->sec_start.alloc(n));
+        FM_HIP(ctx->sec_u0.alloc(n));
+        FM_HIP(hipMemcpyAsync(ctx->sec_start.p, ctx->u.p, n * 8ull, hipMemcpyDeviceToDevice, s));
+    }
+    if (secant) {
+        const double theta = (lf - ctx->sec_lf1) / (ctx->sec_lf1 - ctx->sec_lf0);
+        launch_axpy_to(s, n, ctx->u.p, -1.0, ctx->sec_u0.p, ctx->du.p);
+        launch_axpy_to(s, n, ctx->u.p, theta, ctx->du.p, ctx->ut.p);
+        launch_fill_f64(s, ctx->dD.p, n, 0.0);
+        launch_dir_increment(s, ndir, ctx->d_dir_dofs.p, ctx->d_dir_vals.p, lf, ctx->ut.p, ctx->dD.p);
+        launch_axpy_to(s, n, ctx->ut.p, 1.0, ctx->dD.p, ctx->u.p);
+    } else if (cfg.lift && any_inc) {
         FM_TRY(assemble(ctx, ctx->u.p, true, aprec, true, ctx->dD.p, FM_K_ASSEMBLE));
         S.assemblies++;
         FM_TRY(ensure_partials(ctx, n / 256 + 1));
@@ -1462,6 +1488,16 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     FM_TRY(sync(ctx));
     S.newton_iters = it;
     S.converged = rn <= target;
+    if (cfg.lift == 2 && S.converged) {
+        // the start state becomes u0, this step's converged state the current one
+        std::swap(ctx->sec_u0.p, ctx->sec_start.p);
+        std::swap(ctx->sec_u0.n, ctx->sec_start.n);
+        ctx->sec_n = ctx->sec_n >= 1 ? 2 : 1;
+        ctx->sec_lf0 = ctx->sec_lf1;
+        ctx->sec_lf1 = lf;
+    } else {
+        ctx->sec_n = 0;
+    }
     S.rel_res = S.r0 > 0 ? rn / S.r0 : 0.0;
     S.seconds = now_s() - t0;
     if (stats) *stats = S;
@@ -1667,6 +1703,7 @@ int fm_checkpoint_save(fm_ctx* ctx, const char* path, uint32_t load_step) {
 
 int fm_checkpoint_load(fm_ctx* ctx, const char* path, uint32_t* load_step) {
     if (!ctx || !path) return FM_EINVAL;
+    ctx->sec_n = 0;
     if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
     FILE* fp = std::fopen(path, "rb");
     if (!fp) return fail(ctx, FM_EINVAL, std::string("cannot open ") + path);

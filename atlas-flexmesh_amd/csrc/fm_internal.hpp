@@ -146,6 +146,12 @@ struct Ctx {
     DevBuf<uint32_t> n2e_ptr, n2e;
     bool node_asm = false;
     bool have_n2e = false;  // node -> incident element lists (colour order) on the device
+    // converged states for the secant predictor (lift = 2): the device state u
+    // at load factor sec_lf1 (sec_n >= 1) and the one before, sec_u0 at
+    // sec_lf0 (sec_n == 2); cleared by anything that changes the problem or u
+    DevBuf<double> sec_u0, sec_start;
+    double sec_lf0 = 0.0, sec_lf1 = 0.0;
+    int sec_n = 0;
     bool have_material = false;
 
     // pattern (node adjacency + tangent matrix)

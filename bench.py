@@ -165,11 +165,12 @@ def cpu_baseline(cfg, u_start, step, counts, S, budget_s):
             "t_pcg_iter": t_it}
 
 
-def run_steps(S, cfg, prec, first, count, inner_tol, profile, pcg_max=1000):
+def run_steps(S, cfg, prec, first, count, inner_tol, profile, pcg_max=1000, lift=1):
     import flexmesh as fm
 
-    c = fm.configs.newton_cfg(cfg, prec=prec, inner_tol=inner_tol, inner_max=pcg_max)
+    c = fm.configs.newton_cfg(cfg, prec=prec, inner_tol=inner_tol, inner_max=pcg_max, lift=lift)
     tot = dict(newton_iters=0, pcg_iters=0, assemblies=0, residuals=0, factorizations=0, converged=0)
+    tot["step_seconds"] = []
     worst = 0.0
     if profile:
         S.profile(True)
@@ -178,7 +179,9 @@ def run_steps(S, cfg, prec, first, count, inner_tol, profile, pcg_max=1000):
     for s in range(first, first + count):
         rc, st = S.newton_step(s, c, check=False)
         for k in tot:
-            tot[k] += st[k]
+            if k != "step_seconds":
+                tot[k] += st[k]
+        tot["step_seconds"].append(st["seconds"])
         worst = max(worst, st["rel_res"])
         log(f"  {'fp64' if prec == fm.FM_FP64 else 'mixed'} step {s}: newton {st['newton_iters']} pcg "
             f"{st['pcg_iters']} rel_res {st['rel_res']:.2e} {st['seconds']:.2f} s")
@@ -238,13 +241,16 @@ def micro(S, cfg, info, iters):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=19, help="timed load steps (after the warm-up steps)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--dims", type=str, default=None, help="override hex counts, e.g. 64,64,64")
     ap.add_argument("--inner-tol", type=float, default=3e-3, help="FP32 PCG tolerance of the mixed path")
     ap.add_argument("--pcg-max", type=int, default=1000, help="FP32 PCG iteration cap of the mixed path")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--predictor", choices=["secant", "solve"], default="secant",
+                    help="load-step predictor of the mixed path: secant extrapolation of the two previous "
+                         "converged steps, or the linear solve with the prescribed increment")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=5.0, help="PCG iterations in the CPU sample")
     ap.add_argument("--micro", type=int, default=0,
@@ -275,11 +281,15 @@ def main():
     if args.micro:
         micro(S, cfg, info, args.micro)
         return
+    # the secant predictor extrapolates from the two previous converged steps
+    # (the zero state at load 0 and the warm-up step before the first timed one)
+    lift = 2 if mixed and args.predictor == "secant" else 1
     if args.warmup:
-        run_steps(S, cfg, prec, 1, args.warmup, args.inner_tol, False, args.pcg_max)
+        run_steps(S, cfg, prec, 1, args.warmup, args.inner_tol, False, args.pcg_max, lift)
     u0 = S.get_u() if args.warmup else np.zeros(cfg["X"].size)
     first = args.warmup + 1
-    wall, tot, worst, prof = run_steps(S, cfg, prec, first, args.steps, args.inner_tol, True, args.pcg_max)
+    wall, tot, worst, prof = run_steps(S, cfg, prec, first, args.steps, args.inner_tol, True, args.pcg_max, lift)
+    u_end = S.get_u()
     dofs = cfg["X"].size
     value = dofs * tot["converged"] / wall
     out = {
@@ -289,7 +299,8 @@ def main():
         "dtype": "fp32-mixed" if mixed else "f64", "data": "synthetic (seeded mesh jitter and lognormal fields)",
         "config": {"workload": f"config {args.config}: {cfg['dims'][0]}x{cfg['dims'][1]}x{cfg['dims'][2]} hexes x6 "
                                f"Kuhn tets, {dofs} DOFs, load steps {first}..{first + args.steps - 1} of "
-                               f"{cfg['n_steps']}", "dofs": dofs, "tets": info["ne"], "nnz": info["nnz"]},
+                               f"{cfg['n_steps']}", "dofs": dofs, "tets": info["ne"], "nnz": info["nnz"],
+                   "predictor": "secant" if lift == 2 else "solve"},
         "newton": {"converged_steps": tot["converged"], "newton_iters": tot["newton_iters"],
                    "pcg_iters": tot["pcg_iters"], "newton_it_per_s": tot["newton_iters"] / wall,
                    "worst_rel_res": worst},
@@ -323,10 +334,18 @@ def main():
                            "pcg_iters": tot64["pcg_iters"], "worst_rel_res": worst64}
         out["mixed_over_fp64"] = value / out["fp64_gpu"]["value"]
         u64 = S.get_u()
-        S.set_u(u0)
-        run_steps(S, cfg, prec, first, args.steps, args.inner_tol, False, args.pcg_max)
-        umix = S.get_u()
-        out["mixed_vs_fp64_u_rel_err"] = float(np.abs(umix - u64).max() / np.abs(u64).max())
+        out["mixed_vs_fp64_u_rel_err"] = float(np.abs(u_end - u64).max() / np.abs(u64).max())
+        if lift == 2:
+            # the mixed path with the predictor solve of the FP64 path, from the same
+            # state, on the first three timed steps (beside the secant run's times there)
+            k = min(3, args.steps)
+            S.set_u(u0)
+            wall1, tot1, worst1, _ = run_steps(S, cfg, prec, first, k, args.inner_tol, False, args.pcg_max)
+            out["mixed_predictor_solve"] = {"steps": f"{first}..{first + k - 1}",
+                                            "ms_per_step": 1e3 * wall1 / k,
+                                            "secant_ms_per_step": 1e3 * sum(tot["step_seconds"][:k]) / k,
+                                            "newton_iters": tot1["newton_iters"], "pcg_iters": tot1["pcg_iters"],
+                                            "worst_rel_res": worst1}
         counts = {k: tot64[k] / args.steps for k in ("assemblies", "residuals", "factorizations", "pcg_iters")}
     else:
         counts = {k: tot[k] / args.steps for k in ("assemblies", "residuals", "factorizations", "pcg_iters")}

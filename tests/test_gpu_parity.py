@@ -370,6 +370,31 @@ def test_newton_mixed_matches_fp64_oracle(fm, S, O):
     assert np.linalg.norm(f) <= 1e-6 * np.abs(f0).max()
 
 
+def test_newton_mixed_secant_predictor(fm, S, O):
+    """lift = 2: from the second step on, the load step starts from the secant
+    extrapolation of the two previous converged states (no predictor solve:
+    one factorization per Newton iteration); the solution still matches the
+    FP64 oracle, and any fm_set_* restarts the history."""
+    cfg = fm.configs.make(3, dims=(6, 6, 6))
+    cfg["n_steps"] = 5
+    fm.configs.load(S, cfg)
+    uo, _ = run_oracle(O, cfg, range(1, 6))
+    c = fm.configs.newton_cfg(cfg, inner_tol=1e-4, lift=2)
+    for s in range(1, 6):
+        rc, st = S.newton_step(s, c)
+        assert st["converged"] and st["rel_res"] <= 1e-8
+        # step 1 knows only the zero state: it takes the predictor solve
+        assert st["factorizations"] == st["newton_iters"] + (1 if s == 1 else 0), (s, st)
+        if s == 4:
+            u4 = S.get_u()
+    u5 = S.get_u()
+    assert rel(u5, uo) < 1e-7
+    S.set_u(u4)  # history restarts: step 5 again takes the predictor solve
+    rc, st = S.newton_step(5, c)
+    assert st["converged"] and st["factorizations"] == st["newton_iters"] + 1
+    assert rel(S.get_u(), u5) < 1e-7
+
+
 @pytest.mark.parametrize("ls_mode", [0, 1])
 def test_newton_traction_beam_fp64(fm, S, O, ls_mode):
     # traction-driven (no Dirichlet increment, so no lift): the first two of 40
