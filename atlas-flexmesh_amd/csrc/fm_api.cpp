@@ -1386,7 +1386,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         launch_fill_f64(s, ctx->u.p, n, 0.0);
         ctx->u_valid = true;
         ctx->sec_n = 1;
-        ctx->sec_lf1 = 0.0;
+        ctx->sec_lf[0] = 0.0;
     }
     // linear-solve mode and the precision the tangent is assembled in
     const int kprec = cfg.prec == FM_FP64 ? PREC64 : (cfg.prec == FM_MIXED ? PREC32 : PRECPC);
@@ -1401,20 +1401,34 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     launch_axpy_to(s, n, ctx->u.p, 1.0, ctx->dD.p, ctx->ut.p);
     FM_TRY(residual(ctx, ctx->ut.p, lf, &S.r0));
     S.residuals++;
-    // lift 2: secant predictor from the two previous converged states of this
-    // load path, u + (lf - lf1)/(lf1 - lf0) (u - u0), with the prescribed values
-    // set exactly (no linear solve); the predictor solve below where there is
-    // no such history
-    const bool secant = cfg.lift == 2 && ctx->sec_n == 2 && ctx->sec_lf1 != ctx->sec_lf0;
+    // lift 2: the step starts from the Lagrange extrapolation to lf of the
+    // previous converged states of this load path (quadratic through three,
+    // the secant through two), with the prescribed values set exactly (no
+    // linear solve); the predictor solve below where there is no such history
+    const double l0 = ctx->sec_lf[0], l1 = ctx->sec_lf[1], l2 = ctx->sec_lf[2];
+    const int order = cfg.lift != 2 ? 0
+                      : (ctx->sec_n >= 3 && l0 != l1 && l1 != l2 && l0 != l2) ? 2
+                      : (ctx->sec_n >= 2 && l0 != l1)                       ? 1
+                                                                             : 0;
+    const bool extrapolate = order > 0;
     if (cfg.lift == 2) {
         FM_HIP(ctx->sec_start.alloc(n));
-        FM_HIP(ctx->sec_u0.alloc(n));
+        FM_HIP(ctx->sec_u1.alloc(n));
+        FM_HIP(ctx->sec_u2.alloc(n));
         FM_HIP(hipMemcpyAsync(ctx->sec_start.p, ctx->u.p, n * 8ull, hipMemcpyDeviceToDevice, s));
     }
-    if (secant) {
-        const double theta = (lf - ctx->sec_lf1) / (ctx->sec_lf1 - ctx->sec_lf0);
-        launch_axpy_to(s, n, ctx->u.p, -1.0, ctx->sec_u0.p, ctx->du.p);
-        launch_axpy_to(s, n, ctx->u.p, theta, ctx->du.p, ctx->ut.p);
+    if (extrapolate) {
+        // weights of u (at l0), u1 (at l1), u2 (at l2)
+        double w0, w1, w2 = 0.0;
+        if (order == 2) {
+            w0 = (lf - l1) * (lf - l2) / ((l0 - l1) * (l0 - l2));
+            w1 = (lf - l0) * (lf - l2) / ((l1 - l0) * (l1 - l2));
+            w2 = (lf - l0) * (lf - l1) / ((l2 - l0) * (l2 - l1));
+        } else {
+            w0 = (lf - l1) / (l0 - l1);
+            w1 = (lf - l0) / (l1 - l0);
+        }
+        launch_lincomb3(s, n, w0, ctx->u.p, w1, ctx->sec_u1.p, w2, order == 2 ? ctx->sec_u2.p : nullptr, ctx->ut.p);
         launch_fill_f64(s, ctx->dD.p, n, 0.0);
         launch_dir_increment(s, ndir, ctx->d_dir_dofs.p, ctx->d_dir_vals.p, lf, ctx->ut.p, ctx->dD.p);
         launch_axpy_to(s, n, ctx->ut.p, 1.0, ctx->dD.p, ctx->u.p);
@@ -1489,12 +1503,15 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     S.newton_iters = it;
     S.converged = rn <= target;
     if (cfg.lift == 2 && S.converged) {
-        // the start state becomes u0, this step's converged state the current one
-        std::swap(ctx->sec_u0.p, ctx->sec_start.p);
-        std::swap(ctx->sec_u0.n, ctx->sec_start.n);
-        ctx->sec_n = ctx->sec_n >= 1 ? 2 : 1;
-        ctx->sec_lf0 = ctx->sec_lf1;
-        ctx->sec_lf1 = lf;
+        // the start state becomes u1, u1 becomes u2, this step's converged state the current one
+        std::swap(ctx->sec_u2.p, ctx->sec_u1.p);
+        std::swap(ctx->sec_u2.n, ctx->sec_u1.n);
+        std::swap(ctx->sec_u1.p, ctx->sec_start.p);
+        std::swap(ctx->sec_u1.n, ctx->sec_start.n);
+        ctx->sec_n = ctx->sec_n >= 1 ? std::min(ctx->sec_n + 1, 3) : 1;
+        ctx->sec_lf[2] = ctx->sec_lf[1];
+        ctx->sec_lf[1] = ctx->sec_lf[0];
+        ctx->sec_lf[0] = lf;
     } else {
         ctx->sec_n = 0;
     }

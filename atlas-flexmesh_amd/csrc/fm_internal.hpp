@@ -146,11 +146,12 @@ struct Ctx {
     DevBuf<uint32_t> n2e_ptr, n2e;
     bool node_asm = false;
     bool have_n2e = false;  // node -> incident element lists (colour order) on the device
-    // converged states for the secant predictor (lift = 2): the device state u
-    // at load factor sec_lf1 (sec_n >= 1) and the one before, sec_u0 at
-    // sec_lf0 (sec_n == 2); cleared by anything that changes the problem or u
-    DevBuf<double> sec_u0, sec_start;
-    double sec_lf0 = 0.0, sec_lf1 = 0.0;
+    // converged states for the extrapolating predictor (lift = 2): the device
+    // state u at load factor sec_lf[0] (sec_n >= 1), sec_u1 at sec_lf[1]
+    // (sec_n >= 2) and sec_u2 at sec_lf[2] (sec_n == 3); cleared by anything
+    // that changes the problem or u
+    DevBuf<double> sec_u1, sec_u2, sec_start;
+    double sec_lf[3] = {0.0, 0.0, 0.0};
     int sec_n = 0;
     bool have_material = false;
 
@@ -313,6 +314,9 @@ void launch_elem_rot(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, d
 // maximum stretch, F (9)
 constexpr int kFieldCount = 21;
 void launch_element_fields(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, double* out);
+// out = a x + b y (+ c z when z is given)
+void launch_lincomb3(hipStream_t s, uint32_t n, double a, const double* x, double b, const double* y, double c,
+                     const double* z, double* out);
 // ZZ estimate: zz is kZzRows x ne scratch (colour order), raw and rec 3 nn,
 // err ne (element order); returns the number of partials of sum V sq / 4
 constexpr int kZzRows = 7;

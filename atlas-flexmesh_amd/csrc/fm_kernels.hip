@@ -1962,6 +1962,16 @@ __global__ __launch_bounds__(256) void zz_err_kernel(uint32_t ne, const uint32_t
     if (threadIdx.x == 0) partials[blockIdx.x] = bs;
 }
 
+__global__ void lincomb3_kernel(uint32_t n, double a, const double* __restrict__ x, double b,
+                                const double* __restrict__ y, double c, const double* __restrict__ z,
+                                double* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = a * x[i] + b * y[i];
+    if (z) v += c * z[i];
+    out[i] = v;
+}
+
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 // waves of the persistent kernels: few for the sweeps (only the frontier
@@ -2399,6 +2409,10 @@ uint32_t launch_zz(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, con
     const uint32_t nb = cdiv(a.ne, 256);
     zz_err_kernel<<<nb, 256, 0, s>>>(a.ne, a.conn, eiperm, zz, raw, rec, err, partials);
     return nb;
+}
+void launch_lincomb3(hipStream_t s, uint32_t n, double a, const double* x, double b, const double* y, double c,
+                     const double* z, double* out) {
+    if (n) lincomb3_kernel<<<cdiv(n, 256), 256, 0, s>>>(n, a, x, b, y, c, z, out);
 }
 void launch_element_fields(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, double* out) {
     if (a.ne) element_fields_kernel<<<cdiv(a.ne, 128), 128, 0, s>>>(a, eiperm, out);

@@ -124,10 +124,11 @@ typedef struct {
     double inner_tol;     /* FP32 PCG tol of the mixed path (inexact-Newton forcing term; default 3e-3) */
     int ls_max;           /* Armijo backtracks (armijo_line_search_true, :933-985) */
     double armijo_c;
-    int lift;             /* 1: predictor solve with the prescribed increment; 2: secant predictor
-                             u + (lf - lf1)/(lf1 - lf0) (u - u0) from the two previous converged
-                             steps of the device-resident state (the predictor solve until two
-                             are known; any fm_set_* or a host u restarts the history) */
+    int lift;             /* 1: predictor solve with the prescribed increment; 2: start from the
+                             Lagrange extrapolation of the previous converged steps of the
+                             device-resident state (quadratic through three, linear through two;
+                             the predictor solve until two are known; any fm_set_* or a host u
+                             restarts the history) */
     int inner_max;        /* FP32 PCG iteration cap of the mixed path */
     int ls_mode;          /* 0: Armijo on ||R|| only (reference); 1: a trial is also accepted when the
                              potential energy sum V psi - lf f_ext.u decreases by c alpha R.du */

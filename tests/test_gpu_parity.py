@@ -370,11 +370,11 @@ def test_newton_mixed_matches_fp64_oracle(fm, S, O):
     assert np.linalg.norm(f) <= 1e-6 * np.abs(f0).max()
 
 
-def test_newton_mixed_secant_predictor(fm, S, O):
-    """lift = 2: from the second step on, the load step starts from the secant
-    extrapolation of the two previous converged states (no predictor solve:
-    one factorization per Newton iteration); the solution still matches the
-    FP64 oracle, and any fm_set_* restarts the history."""
+def test_newton_mixed_extrapolating_predictor(fm, S, O):
+    """lift = 2: from the second step on, the load step starts from the
+    extrapolation of the previous converged states (linear, then quadratic; no
+    predictor solve: one factorization per Newton iteration); the solution
+    still matches the FP64 oracle, and any fm_set_* restarts the history."""
     cfg = fm.configs.make(3, dims=(6, 6, 6))
     cfg["n_steps"] = 5
     fm.configs.load(S, cfg)
