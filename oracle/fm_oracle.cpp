@@ -1046,7 +1046,7 @@ void orc_element_fields(uint32_t nn, const double* X, uint32_t ne, const uint32_
 // element order (:30-99, :166-196), element indicators
 // sqrt(sum over the 4 nodes of |recovered - average|^2 / 4) cbrt(V_e) and the
 // global sqrt(sum V_e sum/4) (:198-214).  Corrections: F is the exact,
-// unprojected gradient; the fit uses coordinates relative to the
+// unprojected gradient (see orc_element_fields); the fit uses coordinates relative to the
 // node, so its constant term is the recovered value at the node (the
 // reference fits in global coordinates and takes the value at the origin,
 // contrary to its own comment at :94-96); a patch whose normal equations are

@@ -2,7 +2,8 @@
 
 Run in the container that holds the reference (it needs oracle/_ref/libref.so,
 built by oracle/build_ref.sh from the reference sources).  Each fixture stores
-inputs and the reference's outputs only; the GPU box never runs this script.
+inputs and the reference's outputs only; tests read the fixtures and never run
+this script.
 
     python tests/golden/make_golden.py
 """
