@@ -50,6 +50,25 @@
 #include <cstdlib>
 
 namespace fm {
+// LLVM's raw buffer intrinsics, declared directly: the clang builtins for
+// loads wider than one dword return the first dword splatted in this
+// toolchain.  Outside the anonymous namespace: they have no definition.
+typedef float fm_f2 __attribute__((ext_vector_type(2)));
+typedef float fm_f3 __attribute__((ext_vector_type(3)));
+typedef float fm_f4 __attribute__((ext_vector_type(4)));
+typedef double fm_d2 __attribute__((ext_vector_type(2)));
+typedef unsigned int fm_u2 __attribute__((ext_vector_type(2)));
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ fm_f2 buf_load_f2(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f32");
+__device__ fm_f3 buf_load_f3(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v3f32");
+__device__ fm_f4 buf_load_f4(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v4f32");
+__device__ fm_d2 buf_load_d2(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f64");
+__device__ double buf_load_d1(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.f64");
+__device__ fm_u2 buf_load_u2(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2i32");
+__device__ uint32_t buf_load_u1(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.i32");
+__device__ void buf_store_u32(uint32_t, Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.store.i32");
+__device__ void buf_store_u64(unsigned long long, Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.store.i64");
+
 namespace {
 
 constexpr int kLPN = 4;                    // lanes per node: one quad
@@ -93,24 +112,6 @@ __device__ __forceinline__ double quad(double v) {
 // ---------------------------------------------------------------------------
 // buffer accesses
 // ---------------------------------------------------------------------------
-// LLVM's raw buffer intrinsics, declared directly: the clang builtins for
-// loads wider than one dword return the first dword splatted in this
-// toolchain.
-typedef float fm_f2 __attribute__((ext_vector_type(2)));
-typedef float fm_f3 __attribute__((ext_vector_type(3)));
-typedef float fm_f4 __attribute__((ext_vector_type(4)));
-typedef double fm_d2 __attribute__((ext_vector_type(2)));
-typedef unsigned int fm_u2 __attribute__((ext_vector_type(2)));
-typedef __amdgpu_buffer_rsrc_t Rsrc;
-__device__ fm_f2 buf_load_f2(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f32");
-__device__ fm_f3 buf_load_f3(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v3f32");
-__device__ fm_f4 buf_load_f4(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v4f32");
-__device__ fm_d2 buf_load_d2(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f64");
-__device__ double buf_load_d1(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.f64");
-__device__ fm_u2 buf_load_u2(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2i32");
-__device__ uint32_t buf_load_u1(Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.load.i32");
-__device__ void buf_store_u32(uint32_t, Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.store.i32");
-__device__ void buf_store_u64(unsigned long long, Rsrc, int, int, int) __asm("llvm.amdgcn.raw.ptr.buffer.store.i64");
 
 // A lane whose offset is past the buffer (kOob, or kOffNone of the step
 // record) loads zeros and its store is dropped, without a memory access.
