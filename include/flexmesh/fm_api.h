@@ -129,7 +129,9 @@ typedef struct {
                              Lagrange extrapolation of the previous converged steps of the
                              device-resident state (quadratic through three, linear through two;
                              the predictor solve until two are known; any fm_set_* or a host u
-                             restarts the history) */
+                             restarts the history); from a quadratic start, later Newton
+                             iterations keep the first tangent and factor while each iteration
+                             cuts ||R|| by 100x or more */
     int inner_max;        /* FP32 PCG iteration cap of the mixed path */
     int ls_mode;          /* 0: Armijo on ||R|| only (reference); 1: a trial is also accepted when the
                              potential energy sum V psi - lf f_ext.u decreases by c alpha R.du */

@@ -373,8 +373,9 @@ def test_newton_mixed_matches_fp64_oracle(fm, S, O):
 def test_newton_mixed_extrapolating_predictor(fm, S, O):
     """lift = 2: from the second step on, the load step starts from the
     extrapolation of the previous converged states (linear, then quadratic; no
-    predictor solve: one factorization per Newton iteration); the solution
-    still matches the FP64 oracle, and any fm_set_* restarts the history."""
+    predictor solve), and from the quadratic start later Newton iterations may
+    keep the first tangent and factor; the solution still matches the FP64
+    oracle, and any fm_set_* restarts the history."""
     cfg = fm.configs.make(3, dims=(6, 6, 6))
     cfg["n_steps"] = 5
     fm.configs.load(S, cfg)
@@ -383,8 +384,12 @@ def test_newton_mixed_extrapolating_predictor(fm, S, O):
     for s in range(1, 6):
         rc, st = S.newton_step(s, c)
         assert st["converged"] and st["rel_res"] <= 1e-8
-        # step 1 knows only the zero state: it takes the predictor solve
-        assert st["factorizations"] == st["newton_iters"] + (1 if s == 1 else 0), (s, st)
+        # step 1 knows only the zero state: it takes the predictor solve;
+        # step 2 starts from the secant, later steps from the quadratic
+        if s <= 2:
+            assert st["factorizations"] == st["newton_iters"] + (1 if s == 1 else 0), (s, st)
+        else:
+            assert 1 <= st["factorizations"] == st["assemblies"] <= st["newton_iters"], (s, st)
         if s == 4:
             u4 = S.get_u()
     u5 = S.get_u()
