@@ -293,21 +293,39 @@ __global__ __launch_bounds__(64) void assemble_node_kernel(ElemArgs A, const uin
         t = w >> 2;
         q = (int)(w & 3u);
         elem_eval(A, t, E);
+        // the node's own vertex q of the element, selected without a run-time
+        // index into the element arrays (which would move them to scratch)
+        double gq[3], aq[3], fgq[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            gq[i] = E.g[0][i];
+            aq[i] = E.a[0][i];
+            fgq[i] = E.Fg[0][i];
+        }
+#pragma unroll
+        for (int qq = 1; qq < 4; ++qq)
+            if (q == qq)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    gq[i] = E.g[qq][i];
+                    aq[i] = E.a[qq][i];
+                    fgq[i] = E.Fg[qq][i];
+                }
         const double s = E.lam * E.lnJ - E.mu;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) sf[h][hl][i] = E.ok ? E.V * (E.mu * E.Fg[q][i] + s * E.a[q][i]) : 0.0;
+        for (int i = 0; i < 3; ++i) sf[h][hl][i] = E.ok ? E.V * (E.mu * fgq[i] + s * aq[i]) : 0.0;
         const double lp = E.clamped ? 0.0 : E.lam;
         const double c = E.mu - E.lam * E.lnJ;
         const T V = (T)E.V, mu = (T)E.mu, lpe = (T)lp, ce = (T)c;
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             srk[h][hl][b] = A.erank[(size_t)(q * 4 + b) * A.ne + t];
-            const T gg = (T)E.g[q][0] * (T)E.g[b][0] + (T)E.g[q][1] * (T)E.g[b][1] + (T)E.g[q][2] * (T)E.g[b][2];
+            const T gg = (T)gq[0] * (T)E.g[b][0] + (T)gq[1] * (T)E.g[b][1] + (T)gq[2] * (T)E.g[b][2];
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    T val = lpe * (T)E.a[q][i] * (T)E.a[b][k] + ce * (T)E.a[q][k] * (T)E.a[b][i];
+                    T val = lpe * (T)aq[i] * (T)E.a[b][k] + ce * (T)aq[k] * (T)E.a[b][i];
                     if (i == k) val += mu * gg;
                     sK[h][hl][(i * 4 + b) * 3 + k] = E.ok ? V * val : (T)0;
                 }
@@ -322,11 +340,11 @@ __global__ __launch_bounds__(64) void assemble_node_kernel(ElemArgs A, const uin
                 for (int i = 0; i < 3; ++i) dq[b][i] = dD[3 * (size_t)E.node[b] + i];
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const double gg = E.g[q][0] * E.g[b][0] + E.g[q][1] * E.g[b][1] + E.g[q][2] * E.g[b][2];
+                const double gg = gq[0] * E.g[b][0] + gq[1] * E.g[b][1] + gq[2] * E.g[b][2];
                 const double ad = E.a[b][0] * dq[b][0] + E.a[b][1] * dq[b][1] + E.a[b][2] * dq[b][2];
-                const double qd = E.a[q][0] * dq[b][0] + E.a[q][1] * dq[b][1] + E.a[q][2] * dq[b][2];
+                const double qd = aq[0] * dq[b][0] + aq[1] * dq[b][1] + aq[2] * dq[b][2];
 #pragma unroll
-                for (int i = 0; i < 3; ++i) out[i] += E.mu * gg * dq[b][i] + lp * E.a[q][i] * ad + c * E.a[b][i] * qd;
+                for (int i = 0; i < 3; ++i) out[i] += E.mu * gg * dq[b][i] + lp * aq[i] * ad + c * E.a[b][i] * qd;
             }
         }
 #pragma unroll
