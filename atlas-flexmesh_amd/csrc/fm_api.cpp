@@ -790,14 +790,17 @@ int pcg_run_pc(fm_ctx* ctx, Matrix& A, double tol, int max_it, PcgResult* res) {
     return FM_OK;
 }
 
-// Solve K du = rhs (FP64 device vector) with the tangent; du FP64.
-int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double tol, int max_it, int* iters) {
+// Solve K du = rhs (FP64 device vector) with the tangent; du FP64.  The mixed
+// solves, and the FP64 solve when `inexact`, work on the normalised right-hand
+// side with a forcing-term tolerance.
+int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double tol, int max_it, int* iters,
+                 bool inexact = false, int* capped = nullptr) {
     Matrix& A = ctx->tangent;
     ctx->cur = &A;
     const uint32_t n = A.n;
     PcgResult pr{};
     double nr = 1.0;
-    if (prec != PREC64) {
+    if (prec != PREC64 || inexact) {
         // the inner solves work on the normalised right-hand side: the
         // reference's absolute early exit (||r|| < tol 1e-3) would otherwise
         // end a loose forcing-term solve at once on a small load-step residual
@@ -820,10 +823,16 @@ int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double to
         FM_TRY(pcg_run_pc(ctx, A, tol, max_it, &pr));
         launch_scale(ctx->stream, n, ctx->x64.p, nr, du);
         *iters = pr.iters;
+        if (capped && pr.status != 1) ++*capped;
         return FM_OK;
     }
     FM_TRY(pcg_vectors(ctx, prec, n));
-    if (prec == PREC64) {
+    if (prec == PREC64 && inexact) {
+        launch_scale(ctx->stream, n, rhs, 1.0 / nr, ctx->b64.p);
+        launch_fill_f64(ctx->stream, ctx->x64.p, n, 0.0);
+        FM_TRY(pcg_run(ctx, A, PREC64, tol, max_it, &pr));
+        launch_scale(ctx->stream, n, ctx->x64.p, nr, du);
+    } else if (prec == PREC64) {
         FM_HIP(hipMemcpyAsync(ctx->b64.p, rhs, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));
         launch_fill_f64(ctx->stream, ctx->x64.p, n, 0.0);
         FM_TRY(pcg_run(ctx, A, PREC64, tol, max_it, &pr));
@@ -836,6 +845,7 @@ int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double to
         launch_f32_to_f64(ctx->stream, n, ctx->x32.p, nr, du);
     }
     *iters = pr.iters;
+    if (capped && pr.status != 1) ++*capped;
     return FM_OK;
 }
 
@@ -987,6 +997,21 @@ int fm_set_mesh(fm_ctx* ctx, uint32_t n_nodes, const double* X, uint32_t n_elems
     return FM_OK;
 }
 
+int fm_set_coords(fm_ctx* ctx, uint32_t n_nodes, const double* X) {
+    if (!ctx || !X) return fail(ctx, FM_EINVAL, "fm_set_coords: bad arguments");
+    if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
+    if (n_nodes != ctx->nn) return fail(ctx, FM_EINVAL, "fm_set_coords: node count differs from the mesh");
+    FM_HIP(hipSetDevice(ctx->device));
+    ctx->hX.assign(X, X + 3ull * n_nodes);
+    FM_HIP(hipMemcpy(ctx->X.p, X, 3ull * n_nodes * 8, hipMemcpyHostToDevice));
+    ctx->sec_n = 0;
+    Matrix& M = ctx->tangent;
+    M.val_valid[0] = M.val_valid[1] = false;
+    M.lu_valid[0] = M.lu_valid[1] = false;
+    M.pk_valid[0] = M.pk_valid[1] = false;
+    return FM_OK;
+}
+
 int fm_set_material(fm_ctx* ctx, const double* mu_e, const double* lam_e) {
     if (!ctx || !mu_e || !lam_e) return fail(ctx, FM_EINVAL, "fm_set_material: bad arguments");
     ctx->sec_n = 0;
@@ -1101,6 +1126,35 @@ int fm_set_matrix(fm_ctx* ctx, uint32_t n, const uint32_t* row_ptr, const uint32
     FM_HIP(hipSetDevice(ctx->device));
     Matrix& M = ctx->ext;
     const uint64_t nnz = row_ptr[n];
+    // the same pattern as the last call (same rows, same columns in the same
+    // storage order): keep the sort, the schedules and the device pattern,
+    // upload the values only
+    bool same = M.ready && M.n == n && M.nnz == nnz && M.perm.size() == nnz &&
+                std::equal(row_ptr, row_ptr + n + 1, M.h_row_ptr.begin());
+    if (same) {
+        int diff = 0;
+#pragma omp parallel for schedule(static) reduction(| : diff)
+        for (int64_t k = 0; k < (int64_t)nnz; ++k) diff |= col_idx[M.perm[k]] != M.h_col_idx[k];
+        same = !diff;
+    }
+    if (same) {
+        std::vector<double> sv(nnz);
+        std::vector<float> sf(nnz);
+#pragma omp parallel for schedule(static)
+        for (int64_t k = 0; k < (int64_t)nnz; ++k) {
+            sv[k] = vals[M.perm[k]];
+            sf[k] = (float)sv[k];
+        }
+        FM_HIP(hipMemcpy(M.val64.p, sv.data(), nnz * 8, hipMemcpyHostToDevice));
+        FM_HIP(hipMemcpy(M.val32.p, sf.data(), nnz * 4, hipMemcpyHostToDevice));
+        M.val_valid[0] = M.val_valid[1] = true;
+        M.lu_valid[0] = M.lu_valid[1] = false;
+        M.pk_valid[0] = M.pk_valid[1] = false;
+        ctx->cur = &M;
+        ++ctx->ext_value_uploads;
+        return FM_OK;
+    }
+    M.ready = false;
     M.n = n;
     M.nnz = nnz;
     M.h_row_ptr.assign(row_ptr, row_ptr + n + 1);
@@ -1328,14 +1382,44 @@ void fm_newton_cfg_default(fm_newton_cfg* c) {
     c->lift = 1;
     c->inner_max = 1000;
     c->ls_mode = 0;
+    c->tol_absolute = 0.0;
+    c->ls_alpha0 = 1.0;
+    c->ls_beta = 0.5;
+    c->reuse_tangent = 0;
+    c->energy_tol = 0.0;
+    c->record_energy = 0;
+    c->inexact = 0;
 }
 
 int fm_set_u(fm_ctx* ctx, const double* u) {
-    if (!ctx || !u) return FM_EINVAL;
-    ctx->sec_n = 0;
+    if (!ctx) return FM_EINVAL;
+    if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
     FM_TRY(ensure_vectors(ctx));
+    if (!u) {
+        // the unloaded configuration: u = 0 is the equilibrium at load factor 0
+        launch_fill_f64(ctx->stream, ctx->u.p, 3ull * ctx->nn, 0.0);
+        FM_TRY(sync(ctx));
+        ctx->u_valid = true;
+        ctx->sec_n = 1;
+        ctx->sec_lf[0] = 0.0;
+        return FM_OK;
+    }
+    ctx->sec_n = 0;
     FM_HIP(hipMemcpy(ctx->u.p, u, 3ull * ctx->nn * 8, hipMemcpyHostToDevice));
     ctx->u_valid = true;
+    return FM_OK;
+}
+
+int fm_newton_history(fm_ctx* ctx, uint32_t cap, double* residual, double* step_size, double* energy,
+                      uint32_t* len) {
+    if (!ctx) return FM_EINVAL;
+    const size_t n = ctx->hist_res.size();
+    for (size_t i = 0; i < n && i < cap; ++i) {
+        if (residual) residual[i] = ctx->hist_res[i];
+        if (energy) energy[i] = ctx->hist_energy[i];
+        if (step_size && i + 1 < n) step_size[i] = ctx->hist_alpha[i];
+    }
+    if (len) *len = (uint32_t)n;
     return FM_OK;
 }
 
@@ -1372,6 +1456,9 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     if (!(load_factor >= 0.0) || !std::isfinite(load_factor)) return fail(ctx, FM_EINVAL, "bad load factor");
     if (cfg.prec != FM_FP64 && cfg.prec != FM_MIXED && cfg.prec != FM_MIXED_PC)
         return fail(ctx, FM_EINVAL, "bad precision");
+    if (!(cfg.ls_beta > 0.0 && cfg.ls_beta < 1.0) || !(cfg.ls_alpha0 > 0.0) || !(cfg.tol_absolute >= 0.0) ||
+        !(cfg.energy_tol >= 0.0))
+        return fail(ctx, FM_EINVAL, "bad line-search or tolerance setting");
     FM_TRY(ensure_ready(ctx));
     const double t0 = now_s();
     hipStream_t s = ctx->stream;
@@ -1391,8 +1478,26 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     // linear-solve mode and the precision the tangent is assembled in
     const int kprec = cfg.prec == FM_FP64 ? PREC64 : (cfg.prec == FM_MIXED ? PREC32 : PRECPC);
     const int aprec = kprec == PREC32 ? PREC32 : PREC64;
+    // the FP64 path's inner solves: the reference's pcg_tol / pcg_max, or the
+    // mixed path's forcing term when inexact
+    const bool inexact = kprec == PREC64 && cfg.inexact;
+    const double lin_tol = kprec == PREC64 && !inexact ? cfg.pcg_tol : cfg.inner_tol;
+    const int lin_cap = kprec == PREC64 && !inexact ? cfg.pcg_max : cfg.inner_max;
     const double lf = load_factor;
     fm_newton_stats S{};
+    const bool want_energy = cfg.record_energy || cfg.energy_tol > 0.0;
+    ctx->hist_res.clear();
+    ctx->hist_alpha.clear();
+    ctx->hist_energy.clear();
+    // one history entry: ||R|| of `state`, the step that reached it, its energy
+    auto record = [&](double rnorm, double alpha, const double* state) -> int {
+        if (!ctx->hist_res.empty()) ctx->hist_alpha.push_back(alpha);
+        ctx->hist_res.push_back(rnorm);
+        double pi = std::nan("");
+        if (want_energy) FM_TRY(energy(ctx, state, lf, nullptr, &pi, nullptr));
+        ctx->hist_energy.push_back(pi);
+        return FM_OK;
+    };
     const uint32_t ndir = (uint32_t)ctx->dir_dofs.size();
     bool any_inc = false;
     for (double v : ctx->dir_vals) any_inc |= v != 0.0;
@@ -1401,6 +1506,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     launch_axpy_to(s, n, ctx->u.p, 1.0, ctx->dD.p, ctx->ut.p);
     FM_TRY(residual(ctx, ctx->ut.p, lf, &S.r0));
     S.residuals++;
+    FM_TRY(record(S.r0, 1.0, ctx->ut.p));
     // lift 2: the step starts from the Lagrange extrapolation to lf of the
     // previous converged states of this load path (quadratic through three,
     // the secant through two), with the prescribed values set exactly (no
@@ -1440,45 +1546,50 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
                                ctx->partials.p);
         launch_sub_masked(s, n, ctx->R.p, ctx->lift.p, ctx->isdir.p, ctx->hv.p);
         int its = 0;
-        const double tol = kprec == PREC64 ? cfg.pcg_tol : cfg.inner_tol;
-        const int cap = kprec == PREC64 ? cfg.pcg_max : cfg.inner_max;
-        FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, tol, cap, &its));
+        FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, lin_tol, lin_cap, &its, inexact, &S.pcg_cap_hits));
         S.pcg_iters += its;
         S.factorizations++;
         launch_apply_predictor(s, n, ctx->du.p, ctx->dD.p, ctx->isdir.p, ctx->u.p);
     } else {
         launch_axpy_to(s, n, ctx->u.p, 1.0, ctx->dD.p, ctx->u.p);
     }
+    const bool predicted = extrapolate || (cfg.lift && any_inc);
     double rn = 0.0;
     FM_TRY(residual(ctx, ctx->u.p, lf, &rn));
     S.residuals++;
+    if (predicted) FM_TRY(record(rn, 1.0, ctx->u.p));
     const double target = cfg.newton_rtol * S.r0;
+    // NonlinearNewtonSolver's tests (FEM/nonlinear_solver.hpp:127, :215-227):
+    // relative, absolute, and the energy change of the last iteration
+    auto done = [&](double r, bool energy_stop) {
+        return r <= target || r < cfg.tol_absolute || energy_stop;
+    };
     int it = 0;
     double rn_prev = 0.0;
-    while (!(rn <= target) && it < cfg.max_newton) {
+    bool energy_stop = false;
+    while (!done(rn, energy_stop) && it < cfg.max_newton) {
         // from the quadratic extrapolation, later iterations keep the step's
         // first tangent and its factor while each iteration still cuts ||R||
         // by 100x or more (modified Newton; the FP64 residual decides
         // convergence, and a mixed iteration's reduction is set by the FP32
         // tangent rather than by this lag)
-        const bool keep = order == 2 && it > 0 && rn <= 1e-2 * rn_prev;
+        // (reuse_tangent keeps it for every iteration of the step)
+        const bool keep = it > 0 && (cfg.reuse_tangent || (order == 2 && rn <= 1e-2 * rn_prev));
         if (!keep) {
             FM_TRY(assemble(ctx, ctx->u.p, true, aprec, true, nullptr, FM_K_ASSEMBLE));
             S.assemblies++;
         }
         int its = 0;
-        const double tol = kprec == PREC64 ? cfg.pcg_tol : cfg.inner_tol;
-        const int cap = kprec == PREC64 ? cfg.pcg_max : cfg.inner_max;
         // R of the current u is in ctx->R (last residual evaluation)
         FM_HIP(hipMemcpyAsync(ctx->hv.p, ctx->R.p, n * 8ull, hipMemcpyDeviceToDevice, s));
-        FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, tol, cap, &its));
+        FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, lin_tol, lin_cap, &its, inexact, &S.pcg_cap_hits));
         S.pcg_iters += its;
         if (!keep) S.factorizations++;
         // armijo_line_search_true (FEM/adaptive_fem_engine.hpp:933-985); with
         // ls_mode 1 a trial that fails the residual-norm test is also accepted
         // when the energy decreases by c alpha R.du (the residual norm may grow
         // on the way into the quadratic basin while the energy decreases)
-        double alpha = 1.0, rt = 0.0, pi0 = 0.0, slope = 0.0;
+        double alpha = cfg.ls_alpha0, rt = 0.0, pi0 = 0.0, slope = 0.0;
         bool accepted = false;
         if (cfg.ls_mode == 1) FM_TRY(energy(ctx, ctx->u.p, lf, ctx->du.p, &pi0, &slope));
         for (int bt = 0; bt < cfg.ls_max; ++bt) {
@@ -1497,13 +1608,16 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
                     break;
                 }
             }
-            alpha *= 0.5;
+            alpha *= cfg.ls_beta;
         }
         if (!accepted) {
             launch_axpy_to(s, n, ctx->u.p, alpha, ctx->du.p, ctx->ut.p);
             FM_TRY(residual(ctx, ctx->ut.p, lf, &rt));
             S.residuals++;
         }
+        const double e_prev = ctx->hist_energy.back();
+        FM_TRY(record(rt, alpha, ctx->ut.p));
+        if (cfg.energy_tol > 0.0) energy_stop = std::fabs(ctx->hist_energy.back() - e_prev) < cfg.energy_tol;
         std::swap(ctx->u.p, ctx->ut.p);
         rn_prev = rn;
         rn = rt;
@@ -1511,7 +1625,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     }
     FM_TRY(sync(ctx));
     S.newton_iters = it;
-    S.converged = rn <= target;
+    S.converged = done(rn, energy_stop);
     if (cfg.lift == 2 && S.converged) {
         // the start state becomes u1, u1 becomes u2, this step's converged state the current one
         std::swap(ctx->sec_u2.p, ctx->sec_u1.p);

@@ -160,6 +160,7 @@ struct Ctx {
     std::vector<uint32_t> nbr_ptr, nbr;
     Matrix tangent;
     Matrix ext;
+    uint64_t ext_value_uploads = 0;  // fm_set_matrix calls that kept the previous pattern and plan
     Matrix* cur = nullptr;
     bool tangent_elim = false;
 
@@ -175,6 +176,8 @@ struct Ctx {
     // FP64 state and work vectors
     DevBuf<double> u, f, R, du, ut, dD, lift, hv;
     bool u_valid = false;
+    // per-iteration history of the last Newton solve (host scalars)
+    std::vector<double> hist_res, hist_alpha, hist_energy;
     // PCG work vectors
     DevBuf<double> b64, x64, r64, p64, Ap64, z64, y64;
     DevBuf<float> b32, x32, r32, p32, Ap32, z32, y32;

@@ -27,7 +27,9 @@ class NewtonCfg(C.Structure):
     _fields_ = [("n_steps", C.c_uint32), ("prec", C.c_int), ("newton_rtol", C.c_double), ("max_newton", C.c_int),
                 ("pcg_tol", C.c_double), ("pcg_max", C.c_int), ("inner_tol", C.c_double), ("ls_max", C.c_int),
                 ("armijo_c", C.c_double), ("lift", C.c_int), ("inner_max", C.c_int),
-                ("ls_mode", C.c_int)]
+                ("ls_mode", C.c_int), ("tol_absolute", C.c_double), ("ls_alpha0", C.c_double),
+                ("ls_beta", C.c_double), ("reuse_tangent", C.c_int), ("energy_tol", C.c_double),
+                ("record_energy", C.c_int), ("inexact", C.c_int)]
 
 
 FIELD_NAMES = ("det_F", "cauchy", "von_mises", "pressure", "max_principal", "energy", "max_stretch", "F")
@@ -40,7 +42,7 @@ class ElementFieldsOut(C.Structure):
 class NewtonStats(C.Structure):
     _fields_ = [("converged", C.c_int), ("newton_iters", C.c_int), ("pcg_iters", C.c_int), ("assemblies", C.c_int),
                 ("residuals", C.c_int), ("factorizations", C.c_int), ("r0", C.c_double), ("rel_res", C.c_double),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("pcg_cap_hits", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -72,6 +74,7 @@ def lib():
     L.fm_mesh_jitter.argtypes = [C.c_uint32, _f64p, _u8p, C.c_double, C.c_uint64]
     L.fm_field_lognormal.argtypes = [C.c_uint32, C.c_double, C.c_double, C.c_uint64, _f64p]
     L.fm_set_mesh.argtypes = [_vp, C.c_uint32, _f64p, C.c_uint32, _u32p]
+    L.fm_set_coords.argtypes = [_vp, C.c_uint32, _f64p]
     L.fm_set_material.argtypes = [_vp, _f64p, _f64p]
     L.fm_set_bcs.argtypes = [_vp, C.c_uint32, _vp, _vp]
     L.fm_set_load.argtypes = [_vp, _vp]
@@ -89,8 +92,9 @@ def lib():
     L.fm_newton_cfg_default.restype = None
     L.fm_newton_step.argtypes = [_vp, C.c_uint32, _vp, C.POINTER(NewtonCfg), C.POINTER(NewtonStats)]
     L.fm_newton_solve_to.argtypes = [_vp, C.c_double, _vp, C.POINTER(NewtonCfg), C.POINTER(NewtonStats)]
+    L.fm_newton_history.argtypes = [_vp, C.c_uint32, _vp, _vp, _vp, C.POINTER(C.c_uint32)]
     L.fm_get_u.argtypes = [_vp, _f64p]
-    L.fm_set_u.argtypes = [_vp, _f64p]
+    L.fm_set_u.argtypes = [_vp, _vp]
     L.fm_adjoint.argtypes = [_vp, _f64p, C.c_double, C.c_int, _vp, _vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.fm_rot_logexp_batch.argtypes = [_vp, C.c_uint32, _f64p, C.c_double, _vp, _vp, _vp]
     L.fm_element_rotations.argtypes = [_vp, _f64p]
@@ -180,6 +184,11 @@ class Solver:
         self.conn = u32(conn)
         self.nn, self.ne = self.X.size // 3, self.conn.size // 4
         self._chk(self.L.fm_set_mesh(self.h, self.nn, self.X, self.ne, self.conn), "fm_set_mesh")
+
+    def set_coords(self, X):
+        """New node coordinates, same topology (pattern, colouring, schedules kept)."""
+        self.X = f64(X)
+        self._chk(self.L.fm_set_coords(self.h, self.X.size // 3, self.X), "fm_set_coords")
 
     def set_material(self, mu, lam):
         self._chk(self.L.fm_set_material(self.h, f64(mu), f64(lam)), "fm_set_material")
@@ -293,7 +302,19 @@ class Solver:
         return u
 
     def set_u(self, u):
-        self._chk(self.L.fm_set_u(self.h, f64(u)), "fm_set_u")
+        """Device state u (3 nn); None returns to the unloaded configuration
+        (u = 0 at load factor 0, predictor history restarted there)."""
+        uu = None if u is None else f64(u)
+        self._chk(self.L.fm_set_u(self.h, _ptr(uu)), "fm_set_u")
+
+    def newton_history(self):
+        """(residual norms, step sizes, energies) of the last Newton solve."""
+        n = C.c_uint32(0)
+        self._chk(self.L.fm_newton_history(self.h, 0, None, None, None, C.byref(n)), "fm_newton_history")
+        res, alpha, en = np.zeros(n.value), np.zeros(max(n.value - 1, 0)), np.zeros(n.value)
+        self._chk(self.L.fm_newton_history(self.h, n.value, res.ctypes.data, alpha.ctypes.data if alpha.size else None,
+                                           en.ctypes.data, C.byref(n)), "fm_newton_history")
+        return res, alpha, en
 
     # ---- adjoint / rotations / checkpoint --------------------------------------
     def adjoint(self, dLdu, tol=1e-12, max_it=5000):

@@ -1,18 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the quasi-static Newton path on one MI355X.
 
-Workload (default): configuration 3 -- a 128^3-hex cube as 12,582,912 Kuhn
-tets (2,146,689 nodes, 6,440,067 DOFs), interior jitter, lognormal mu_e,
-nu = 0.49, x=0 clamped, 25% uniaxial stretch of x=1 over 20 load steps.  A
-"step" is one load step solved by Newton to ||R|| <= 1e-8 ||R_0|| (FP64
-residual) with FP32 assembly, SpMV, ILU(0) and PCG (the mixed path).  Inputs
-are synthetic and already resident in HBM when the timed region starts.
+Workload (default): a 128^3-hex cube as 12,582,912 Kuhn tets (2,146,689
+nodes, 6,440,067 DOFs), interior jitter, lognormal mu_e, nu = 0.49, x=0
+clamped, 25% uniaxial stretch of x=1 over 20 load steps.  A "step" is one
+load step solved by Newton to ||R|| <= 1e-8 ||R_0|| (FP64 residual) with FP32
+assembly, SpMV, ILU(0) and PCG (the mixed path).  Inputs are synthetic and
+already resident in HBM when the timed region starts.
+
+Step plan: the first `--warmup` load steps run once as a throw-away pass,
+the state returns to the unloaded configuration (u = 0 at load factor 0), and
+load steps 1..K of the path are timed (K = --steps, at most the path's load
+steps).  Every library call is synchronous on return, so the wall clock
+around the K steps brackets exactly the device work of the timed region.
 
 value = DOFs x (load steps converged) / wall seconds of the timed steps.
-The same steps are then re-run from the same state on the plain FP64 GPU path,
-and the FP64 CPU oracle is timed on a bounded sample on the host cores
-(assembly, residual, ILU(0) factorization, PCG iterations) and extrapolated
-with the operation counts of the FP64 GPU run.
+After the timed region, on the same load steps from the unloaded state:
+  * the plain FP64 GPU path running the same algorithm (predictor, kept
+    tangent, forcing term) -- the precision-only ratio mixed_over_fp64;
+  * the plain FP64 GPU path with the reference's solver settings (predictor
+    solve, PCG tol 1e-10, 300-iteration cap), with its cap hits counted;
+  * the FP64 CPU oracle on a bounded sample on the host cores (assembly,
+    residual, ILU(0) factorization, PCG iterations), extrapolated with the
+    operation counts of the same-algorithm FP64 run, and the oracle's FP64
+    residual of the final mixed displacement.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -119,9 +130,12 @@ def physical_cores():
         return os.cpu_count() or 1
 
 
-def cpu_baseline(cfg, u_start, step, counts, S, budget_s):
-    """Time the FP64 CPU oracle on a bounded sample of the same load step and
-    extrapolate one load step with the FP64 GPU run's operation counts."""
+def cpu_baseline(cfg, u, counts, S, budget_its, counts_from):
+    """Time the FP64 CPU oracle on a bounded sample of the workload -- one
+    tangent assembly with the Dirichlet elimination, one residual, one
+    natural-order ILU(0) factorization and budget_its PCG iterations at the
+    state u -- and extrapolate one load step with the operation counts per
+    load step given in `counts`."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
@@ -130,8 +144,6 @@ def cpu_baseline(cfg, u_start, step, counts, S, budget_s):
     X, conn, mu, lam = cfg["X"], cfg["conn"], cfg["mu"], cfg["lam"]
     rp, ci = S.pattern()  # bit-exact with the oracle's pattern (tested)
     colors, nc = S.colors()
-    lf = step / cfg["n_steps"]
-    u = u_start.copy()
     isd = np.zeros(u.size, dtype=np.uint8)
     isd[cfg["dir_dofs"]] = 1
     t = time.perf_counter()
@@ -139,57 +151,93 @@ def cpu_baseline(cfg, u_start, step, counts, S, budget_s):
     O.eliminate(rp, ci, K, isd)
     t_asm = time.perf_counter() - t
     t = time.perf_counter()
-    _, f = O.assemble(X, conn, mu, lam, colors, nc, u, rp, ci, want_K=False)
+    O.assemble(X, conn, mu, lam, colors, nc, u, rp, ci, want_K=False)
     t_res = time.perf_counter() - t
-    R = -f
-    R[isd == 1] = 0.0
     t = time.perf_counter()
-    LU, diag = O.ilu0(rp, ci, K)
+    O.ilu0(rp, ci, K)
     t_fac = time.perf_counter() - t
-    # k iterations of the oracle's pcg_solve_ilu0 (which factorizes first)
-    k = max(1, int(budget_s))
+    # k iterations of the oracle's pcg_solve_ilu0 (which factorizes first) on a
+    # right-hand side with the constrained rows zeroed; tol 1e-30 is never met
+    k = max(1, int(budget_its))
+    b = np.random.default_rng(1).standard_normal(u.size)
+    b[isd == 1] = 0.0
     t = time.perf_counter()
-    O.pcg(rp, ci, K, R, tol=1e-30, max_iter=k)
+    O.pcg(rp, ci, K, b, tol=1e-30, max_iter=k)
     t_it = max(time.perf_counter() - t - t_fac, 1e-9) / k
-    n_it = k
     t_step = (counts["assemblies"] * t_asm + counts["residuals"] * t_res + counts["factorizations"] * t_fac +
               counts["pcg_iters"] * t_it)
     dofs = u.size
-    sample = (f"one FP64 assembly ({t_asm:.2f} s), one residual ({t_res:.2f} s), one natural-order ILU(0) "
-              f"factorization ({t_fac:.2f} s) and {n_it} PCG iterations ({t_it:.3f} s each) of load step {step} at "
-              f"{dofs} DOFs on {cores} cores ({cpu_model()}); one load step extrapolated with the FP64 GPU run's "
-              f"counts ({counts['assemblies']} assemblies, {counts['residuals']} residuals, "
-              f"{counts['factorizations']} factorizations, {counts['pcg_iters']} PCG iterations) = {t_step:.1f} s")
+    c = {k: round(v, 2) for k, v in counts.items()}
+    sample = (f"one FP64 tangent assembly with Dirichlet elimination ({t_asm:.2f} s), one residual ({t_res:.2f} s), "
+              f"one natural-order ILU(0) factorization ({t_fac:.2f} s) and {k} ILU(0)-PCG iterations ({t_it:.3f} s "
+              f"each) at {dofs} DOFs on {cores} cores ({cpu_model()}); one load step extrapolated with the "
+              f"per-step counts of {counts_from} ({c['assemblies']} assemblies, {c['residuals']} residuals, "
+              f"{c['factorizations']} factorizations, {c['pcg_iters']} PCG iterations) = {t_step:.1f} s")
     return {"value": dofs / t_step, "unit": "DOFs/s", "cores": cores, "kind": "port", "sample": sample,
             "seconds_per_load_step": t_step, "t_assemble": t_asm, "t_residual": t_res, "t_factor": t_fac,
             "t_pcg_iter": t_it}
 
 
-def run_steps(S, cfg, prec, first, count, inner_tol, profile, pcg_max=1000, lift=1):
+def step_plan(n_steps, steps, warmup):
+    """Load steps of the warm-up pass and of the timed pass.
+
+    The warm-up pass runs load steps 1..min(warmup, n_steps) and is thrown
+    away; the timed pass restarts from the unloaded state and runs load steps
+    1..min(steps, n_steps).  Returns (warm_last, timed_last); 0 = none."""
+    if n_steps < 1:
+        raise ValueError("a load path needs at least one load step")
+    return max(0, min(warmup, n_steps)), max(1, min(steps, n_steps))
+
+
+def run_steps(S, cfg, prec, first, last, inner_tol, profile, pcg_max=1000, lift=1, **kw):
+    """Load steps first..last on the device-resident state; totals of the
+    Newton statistics and the wall seconds around them."""
     import flexmesh as fm
 
-    c = fm.configs.newton_cfg(cfg, prec=prec, inner_tol=inner_tol, inner_max=pcg_max, lift=lift)
-    tot = dict(newton_iters=0, pcg_iters=0, assemblies=0, residuals=0, factorizations=0, converged=0)
-    tot["step_seconds"] = []
+    c = fm.configs.newton_cfg(cfg, prec=prec, inner_tol=inner_tol, inner_max=pcg_max, lift=lift, **kw)
+    tot = dict(newton_iters=0, pcg_iters=0, assemblies=0, residuals=0, factorizations=0, converged=0,
+               pcg_cap_hits=0)
+    tot["step_seconds"], tot["r0"], tot["rel_res"] = [], [], []
     worst = 0.0
     if profile:
         S.profile(True)
         S.profile_reset()
+    name = {fm.FM_FP64: "fp64", fm.FM_MIXED: "mixed", fm.FM_MIXED_PC: "mixed-pc"}[prec]
     t0 = time.perf_counter()
-    for s in range(first, first + count):
+    for s in range(first, last + 1):
         rc, st = S.newton_step(s, c, check=False)
-        for k in tot:
-            if k != "step_seconds":
-                tot[k] += st[k]
+        for k in ("newton_iters", "pcg_iters", "assemblies", "residuals", "factorizations", "converged",
+                  "pcg_cap_hits"):
+            tot[k] += st[k]
         tot["step_seconds"].append(st["seconds"])
+        tot["r0"].append(st["r0"])
+        tot["rel_res"].append(st["rel_res"])
         worst = max(worst, st["rel_res"])
-        log(f"  {'fp64' if prec == fm.FM_FP64 else 'mixed'} step {s}: newton {st['newton_iters']} pcg "
-            f"{st['pcg_iters']} rel_res {st['rel_res']:.2e} {st['seconds']:.2f} s")
+        log(f"  {name} step {s}: newton {st['newton_iters']} pcg {st['pcg_iters']} (capped solves "
+            f"{st['pcg_cap_hits']}) rel_res {st['rel_res']:.2e} {st['seconds']:.2f} s")
     wall = time.perf_counter() - t0
     prof = S.profile_get() if profile else None
     if profile:
         S.profile(False)
     return wall, tot, worst, prof
+
+
+def oracle_residual(cfg, S, u, load_factor):
+    """||R(u)||_2 of the FP64 CPU oracle's residual-only assembly
+    (load f_ext - f_int(u), constrained rows zeroed) on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    O.set_threads(physical_cores())
+    rp, ci = S.pattern()
+    colors, nc = S.colors()
+    t = time.perf_counter()
+    _, f = O.assemble(cfg["X"], cfg["conn"], cfg["mu"], cfg["lam"], colors, nc, u, rp, ci, want_K=False)
+    R = -f
+    if cfg["f_ext"] is not None:
+        R += load_factor * cfg["f_ext"]
+    R[cfg["dir_dofs"]] = 0.0
+    return float(np.linalg.norm(R)), time.perf_counter() - t
 
 
 def element_fields_timing(S, info, kind="fields", reps=5):
@@ -241,18 +289,20 @@ def micro(S, cfg, info, iters):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=19, help="timed load steps (after the warm-up steps)")
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="timed load steps 1..K (at most the path's load steps)")
+    ap.add_argument("--warmup", type=int, default=1, help="load steps of the throw-away warm-up pass")
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--dims", type=str, default=None, help="override hex counts, e.g. 64,64,64")
     ap.add_argument("--inner-tol", type=float, default=None,
-                    help="FP32 PCG tolerance of the mixed path (default 1e-2 with the extrapolating "
+                    help="PCG forcing term of the inexact Newton solves (default 1e-2 with the extrapolating "
                          "predictor, 3e-3 with the predictor solve)")
-    ap.add_argument("--pcg-max", type=int, default=1000, help="FP32 PCG iteration cap of the mixed path")
-    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--pcg-max", type=int, default=1000, help="PCG iteration cap of the inexact Newton solves")
+    ap.add_argument("--no-fp64", action="store_true", help="skip both FP64 GPU rows")
+    ap.add_argument("--fp64-ref-steps", type=int, default=None,
+                    help="load steps of the FP64 row with the reference's solver settings (default: all timed)")
     ap.add_argument("--predictor", choices=["extrapolate", "solve"], default="extrapolate",
-                    help="load-step predictor of the mixed path: extrapolation of the previous converged "
-                         "steps (quadratic through three), or the linear solve with the prescribed increment")
+                    help="load-step predictor: extrapolation of the previous converged steps (quadratic "
+                         "through three), or the linear solve with the prescribed increment")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=5.0, help="PCG iterations in the CPU sample")
     ap.add_argument("--micro", type=int, default=0,
@@ -269,9 +319,21 @@ def main():
     cfg = fm.configs.make(args.config, dims=dims)
     log(f"config {args.config} dims {cfg['dims']}: {cfg['X'].size} DOFs, {cfg['conn'].size // 4} tets "
         f"({time.perf_counter() - t:.1f} s)")
-    if args.warmup + args.steps > cfg["n_steps"]:
-        raise SystemExit("warmup + steps exceeds the load steps of the configuration")
+    warm_last, K = step_plan(cfg["n_steps"], args.steps, args.warmup)
+    if K < args.steps:
+        log(f"--steps {args.steps} exceeds the {cfg['n_steps']} load steps of the path: timing load steps 1..{K}")
     S = fm.Solver(0)
+    try:
+        out = bench(S, cfg, args, warm_last, K)
+    finally:
+        S.close()
+    if out is not None:
+        print(json.dumps(out), flush=True)
+
+
+def bench(S, cfg, args, warm_last, K):
+    import flexmesh as fm
+
     t = time.perf_counter()
     fm.configs.load(S, cfg)
     info = S.info()
@@ -282,34 +344,38 @@ def main():
     prec = fm.FM_MIXED if mixed else fm.FM_FP64
     if args.micro:
         micro(S, cfg, info, args.micro)
-        return
+        return None
     # the extrapolating predictor uses the previous converged steps of the
-    # path (the zero state at load 0 and the warm-up step count among them)
-    lift = 2 if mixed and args.predictor == "extrapolate" else 1
-    # the forcing term that minimises the PCG work: a step from the extrapolated
-    # start needs two Newton iterations up to 1e-2 (3e-3 from a predictor solve)
-    tol_solve = args.inner_tol if args.inner_tol is not None else 3e-3
+    # path (the unloaded state at load factor 0 among them)
+    lift = 2 if args.predictor == "extrapolate" else 1
+    # the forcing term that minimises the PCG work: a step from the
+    # extrapolated start needs two Newton iterations up to 1e-2 (3e-3 from a
+    # predictor solve)
     inner_tol = args.inner_tol if args.inner_tol is not None else (1e-2 if lift == 2 else 3e-3)
-    if args.warmup:
-        run_steps(S, cfg, prec, 1, args.warmup, inner_tol, False, args.pcg_max, lift)
-    u0 = S.get_u() if args.warmup else np.zeros(cfg["X"].size)
-    first = args.warmup + 1
-    wall, tot, worst, prof = run_steps(S, cfg, prec, first, args.steps, inner_tol, True, args.pcg_max, lift)
+    if warm_last:
+        log(f"warm-up pass: load steps 1..{warm_last}")
+        run_steps(S, cfg, prec, 1, warm_last, inner_tol, False, args.pcg_max, lift)
+    S.set_u(None)  # back to the unloaded state: the timed pass is the load path from its start
+    log(f"timed pass: load steps 1..{K} of {cfg['n_steps']}")
+    wall, tot, worst, prof = run_steps(S, cfg, prec, 1, K, inner_tol, True, args.pcg_max, lift)
     u_end = S.get_u()
     dofs = cfg["X"].size
     value = dofs * tot["converged"] / wall
+    nx, ny, nz = cfg["dims"]
     out = {
-        "metric": "Newton DOFs/s to 1e-8 rel. residual (FP32-mixed; FP64 GPU and CPU beside it)",
-        "value": value, "unit": "DOFs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32-mixed" if mixed else "f64", "data": "synthetic (seeded mesh jitter and lognormal fields)",
-        "config": {"workload": f"config {args.config}: {cfg['dims'][0]}x{cfg['dims'][1]}x{cfg['dims'][2]} hexes x6 "
-                               f"Kuhn tets, {dofs} DOFs, load steps {first}..{first + args.steps - 1} of "
-                               f"{cfg['n_steps']}", "dofs": dofs, "tets": info["ne"], "nnz": info["nnz"],
-                   "predictor": "extrapolate" if lift == 2 else "solve", "inner_tol": inner_tol},
+        "metric": "Newton DOFs/s to 1e-8 rel. residual, FP32-mixed vs FP64 GPU vs CPU; HBM GB/s",
+        "value": value, "unit": "DOFs/s", "n_gpus": 1, "steps": K, "warmup": warm_last,
+        "ms_per_step": 1e3 * wall / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32-mixed" if mixed else "f64",
+        "data": "synthetic (seeded mesh jitter and lognormal fields)",
+        "config": {"workload": f"config {args.config}: {nx}x{ny}x{nz} hexes x6 Kuhn tets, {dofs} DOFs, "
+                               f"load steps 1..{K} of {cfg['n_steps']}", "dofs": dofs, "tets": info["ne"],
+                   "nnz": info["nnz"], "predictor": "extrapolate" if lift == 2 else "solve",
+                   "inner_tol": inner_tol, "newton_rtol": cfg["newton_rtol"]},
         "newton": {"converged_steps": tot["converged"], "newton_iters": tot["newton_iters"],
                    "pcg_iters": tot["pcg_iters"], "newton_it_per_s": tot["newton_iters"] / wall,
-                   "worst_rel_res": worst},
+                   "pcg_it_per_s": tot["pcg_iters"] / wall, "worst_rel_res": worst,
+                   "step_seconds": [round(x, 4) for x in tot["step_seconds"]]},
     }
     # roofline of the dominant kernel class
     v = 4 if mixed else 8
@@ -325,44 +391,61 @@ def main():
     out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": d["GBps"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                        "frac": d["GBps"] / PEAK_HBM_GBS,
                        "traffic": measured_traffic(dom, "fp32" if mixed else "fp64", cfg["dims"]),
-                       "algorithmic_bytes": d["bytes_per_launch"]}
+                       "algorithmic_bytes": d["bytes_per_launch"], "avg_us": d["avg_us"]}
     out["kernels"] = kern
     if "spmv" in kern:
         out["spmv_frac_of_peak"] = kern["spmv"]["GBps"] / PEAK_HBM_GBS
     out["element_fields"] = element_fields_timing(S, info)
     out["zz_estimate"] = element_fields_timing(S, info, "zz")
-    # FP64 GPU path on the same steps from the same state
+    counts = {k: tot[k] / K for k in ("assemblies", "residuals", "factorizations", "pcg_iters")}
+    counts_from = "the mixed run"
+    u64 = None
     if mixed and not args.no_fp64:
-        S.set_u(u0)
-        wall64, tot64, worst64, _ = run_steps(S, cfg, fm.FM_FP64, first, args.steps, inner_tol, False)
-        out["fp64_gpu"] = {"value": dofs * tot64["converged"] / wall64, "unit": "DOFs/s",
-                           "ms_per_step": 1e3 * wall64 / args.steps, "newton_iters": tot64["newton_iters"],
-                           "pcg_iters": tot64["pcg_iters"], "worst_rel_res": worst64}
-        out["mixed_over_fp64"] = value / out["fp64_gpu"]["value"]
+        # precision alone: the FP64 path running the same algorithm on the same steps
+        S.set_u(None)
+        log(f"FP64 GPU, same algorithm: load steps 1..{K}")
+        wall64, tot64, worst64, _ = run_steps(S, cfg, fm.FM_FP64, 1, K, inner_tol, False, args.pcg_max, lift,
+                                              inexact=1)
         u64 = S.get_u()
+        out["fp64_gpu"] = {"value": dofs * tot64["converged"] / wall64, "unit": "DOFs/s",
+                           "ms_per_step": 1e3 * wall64 / K, "newton_iters": tot64["newton_iters"],
+                           "pcg_iters": tot64["pcg_iters"], "converged_steps": tot64["converged"],
+                           "worst_rel_res": worst64,
+                           "algorithm": "the mixed run's: predictor, kept tangent, forcing term "
+                                        f"{inner_tol:g}, FP64 tangent, SpMV, ILU(0) and PCG"}
+        out["mixed_over_fp64"] = value / out["fp64_gpu"]["value"]
         out["mixed_vs_fp64_u_rel_err"] = float(np.abs(u_end - u64).max() / np.abs(u64).max())
-        if lift == 2:
-            # the mixed path with the predictor solve of the FP64 path, from the same
-            # state, on the first three timed steps (beside the secant run's times there)
-            k = min(3, args.steps)
-            S.set_u(u0)
-            wall1, tot1, worst1, _ = run_steps(S, cfg, prec, first, k, tol_solve, False, args.pcg_max)
-            out["mixed_predictor_solve"] = {"steps": f"{first}..{first + k - 1}",
-                                            "ms_per_step": 1e3 * wall1 / k,
-                                            "extrapolate_ms_per_step": 1e3 * sum(tot["step_seconds"][:k]) / k,
-                                            "newton_iters": tot1["newton_iters"], "pcg_iters": tot1["pcg_iters"],
-                                            "worst_rel_res": worst1}
-            # precision alone: both paths with the predictor solve, same steps
-            out["mixed_over_fp64_same_predictor"] = sum(tot64["step_seconds"][:k]) / wall1
-        counts = {k: tot64[k] / args.steps for k in ("assemblies", "residuals", "factorizations", "pcg_iters")}
-    else:
-        counts = {k: tot[k] / args.steps for k in ("assemblies", "residuals", "factorizations", "pcg_iters")}
+        counts = {k: tot64[k] / K for k in ("assemblies", "residuals", "factorizations", "pcg_iters")}
+        counts_from = "the same-algorithm FP64 GPU run"
+        # the reference's solver settings: predictor solve, PCG tol 1e-10, 300 iterations
+        R = K if args.fp64_ref_steps is None else max(1, min(args.fp64_ref_steps, K))
+        S.set_u(None)
+        log(f"FP64 GPU, reference solver settings: load steps 1..{R}")
+        c_ref = fm.configs.newton_cfg(cfg, prec=fm.FM_FP64)
+        wallr, totr, worstr, _ = run_steps(S, cfg, fm.FM_FP64, 1, R, c_ref.inner_tol, False, c_ref.inner_max, 1)
+        out["fp64_gpu_reference_settings"] = {
+            "steps": f"1..{R}", "value": dofs * totr["converged"] / wallr, "unit": "DOFs/s",
+            "ms_per_step": 1e3 * wallr / R, "newton_iters": totr["newton_iters"], "pcg_iters": totr["pcg_iters"],
+            "linear_solves": totr["factorizations"], "pcg_cap_hits": totr["pcg_cap_hits"],
+            "converged_steps": totr["converged"],
+            "worst_rel_res": worstr, "pcg_tol": c_ref.pcg_tol, "pcg_cap": c_ref.pcg_max,
+            "mixed_over_this": value / (dofs * totr["converged"] / wallr) if totr["converged"] else None}
     if not args.no_cpu:
-        cb = cpu_baseline(cfg, u0, first, counts, S, args.cpu_budget)
+        cb = cpu_baseline(cfg, u_end, counts, S, args.cpu_budget, counts_from)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
         out["gpu_over_cpu"] = value / cb["value"]
-    S.close()
-    print(json.dumps(out), flush=True)
+        # the oracle's FP64 residual of the final mixed displacement (and of the FP64 one)
+        lf = K / cfg["n_steps"]
+        r_mixed, t_res = oracle_residual(cfg, S, u_end, lf)
+        chk = {"load_factor": lf, "r0_of_last_step_gpu": tot["r0"][-1], "oracle_norm_R_mixed_u": r_mixed,
+               "oracle_rel_res_mixed_u": r_mixed / tot["r0"][-1], "gpu_rel_res_last_step": tot["rel_res"][-1],
+               "seconds": t_res}
+        if u64 is not None:
+            r64, _ = oracle_residual(cfg, S, u64, lf)
+            chk["oracle_rel_res_fp64_u"] = r64 / tot["r0"][-1]
+        out["oracle_residual_check"] = chk
+        log(f"oracle FP64 residual of the final mixed u: {r_mixed:.3e} = {r_mixed / tot['r0'][-1]:.2e} R0")
+    return out
 
 
 if __name__ == "__main__":

@@ -64,6 +64,13 @@ int fm_field_lognormal(uint32_t n, double median, double sigma, uint64_t seed, d
  * material is per element (mu_e, lam_e) instead of one (E, nu) per problem
  * (ProblemDescriptor.E/.nu, used at FEM/adaptive_fem_engine.hpp:1034, 1187). */
 int fm_set_mesh(fm_ctx* ctx, uint32_t n_nodes, const double* X, uint32_t n_elems, const uint32_t* conn);
+/* New node coordinates for the mesh already set (3*n_nodes doubles SoA, the same
+   node count and connectivity): the pattern, colouring, schedules, material,
+   boundary conditions, load and u are kept; the assembled tangent and its
+   factor are dropped and the predictor history restarts.  The adapt loop
+   moves nodes in place this way (laplacian_smooth after edge_split,
+   FEM/adaptive_fem_engine.hpp:2662-2663). */
+int fm_set_coords(fm_ctx* ctx, uint32_t n_nodes, const double* X);
 int fm_set_material(fm_ctx* ctx, const double* mu_e, const double* lam_e);
 /* Dirichlet DOFs and their values at load factor 1 (BoundaryConditions::dirichlet_bcs,
    FEM/fem_types.hpp:258-264); vals may be NULL (all zero). */
@@ -135,6 +142,19 @@ typedef struct {
     int inner_max;        /* FP32 PCG iteration cap of the mixed path */
     int ls_mode;          /* 0: Armijo on ||R|| only (reference); 1: a trial is also accepted when the
                              potential energy sum V psi - lf f_ext.u decreases by c alpha R.du */
+    /* NewtonSolverConfig members (FEM/nonlinear_solver.hpp:23-34) */
+    double tol_absolute;  /* also converged when ||R|| < tol_absolute (:25, test at :127); 0 = off */
+    double ls_alpha0;     /* first trial step of the line search (line_search_alpha, :27); default 1 */
+    double ls_beta;       /* backtracking factor alpha <- beta alpha, 0 < beta < 1 (line_search_beta, :28) */
+    int reuse_tangent;    /* 1: every iteration of a step solves with the step's first tangent and
+                             factor (reuse_tangent, :30, :141-143) */
+    double energy_tol;    /* also converged when |Pi_k - Pi_(k-1)| < energy_tol after an iteration
+                             (energy_convergence_tol, :32, :215-227); 0 = off */
+    int record_energy;    /* 1: evaluate the potential energy at the start and after every iteration
+                             for fm_newton_history (it is evaluated anyway when energy_tol > 0) */
+    int inexact;          /* FM_FP64 only: 1 = the FP64 inner solves take the mixed path's forcing
+                             term (inner_tol, inner_max, normalised right-hand side) instead of the
+                             reference's pcg_tol / pcg_max, so both precisions run one algorithm */
 } fm_newton_cfg;
 
 typedef struct {
@@ -147,6 +167,7 @@ typedef struct {
     double r0;
     double rel_res;
     double seconds;
+    int pcg_cap_hits;     /* linear solves that stopped at their iteration cap */
 } fm_newton_stats;
 
 void fm_newton_cfg_default(fm_newton_cfg* cfg);
@@ -156,7 +177,19 @@ int fm_newton_step(fm_ctx* ctx, uint32_t step, double* u, const fm_newton_cfg* c
    FEM/fem_types.hpp:266-271): prescribed values load_factor*dir_vals, external
    force load_factor*f_ext */
 int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newton_cfg* cfg, fm_newton_stats* stats);
-/* device-resident displacement (3nn doubles) */
+/* Per-iteration history of the last fm_newton_step / fm_newton_solve_to
+ * (NewtonSolverResult::residual_history, step_size_history, energy_history,
+ * FEM/nonlinear_solver.hpp:42-44): residual[0] = ||R_0||, then ||R|| after the
+ * load-step predictor when one ran (step size 1), then after every Newton
+ * iteration; step_size[i] is the line-search step that led to residual[i+1]
+ * (len - 1 entries); energy[i] is the potential energy at the state of
+ * residual[i] (NaN unless record_energy or energy_tol was set).  At most cap
+ * entries are written to each non-NULL array; *len is the full length. */
+int fm_newton_history(fm_ctx* ctx, uint32_t cap, double* residual, double* step_size, double* energy,
+                      uint32_t* len);
+/* device-resident displacement (3nn doubles); fm_set_u(ctx, NULL) returns the
+   state to the unloaded configuration (u = 0 at load factor 0, the start of a
+   load path, with the predictor history restarted there) */
 int fm_get_u(fm_ctx* ctx, double* u);
 int fm_set_u(fm_ctx* ctx, const double* u);
 
