@@ -1105,6 +1105,21 @@ int fm_assemble(fm_ctx* ctx, const double* u, double* f_int_out, double* K_vals_
     return FM_OK;
 }
 
+int fm_assemble_residual_only(fm_ctx* ctx, const double* u, double* f_int_out) {
+    if (!ctx || !f_int_out) return fail(ctx, FM_EINVAL, "fm_assemble_residual_only: bad arguments");
+    FM_TRY(ensure_ready(ctx));
+    const size_t n = 3ull * ctx->nn;
+    const double* du = nullptr;
+    if (u) {
+        FM_HIP(hipMemcpy(ctx->hv.p, u, n * 8, hipMemcpyHostToDevice));
+        du = ctx->hv.p;
+    }
+    FM_TRY(assemble(ctx, du, false, PREC64, false, nullptr, FM_K_RESIDUAL));
+    FM_TRY(sync(ctx));
+    FM_HIP(hipMemcpy(f_int_out, ctx->f.p, n * 8, hipMemcpyDeviceToHost));
+    return FM_OK;
+}
+
 int fm_residual(fm_ctx* ctx, const double* u, double load, double* R_out, double* norm_out) {
     if (!ctx) return FM_EINVAL;
     FM_TRY(ensure_ready(ctx));
@@ -1511,8 +1526,10 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     // previous converged states of this load path (quadratic through three,
     // the secant through two), with the prescribed values set exactly (no
     // linear solve); the predictor solve below where there is no such history
+    // a start state that already passes the absolute test takes no predictor
+    const bool at_rest = S.r0 < cfg.tol_absolute;
     const double l0 = ctx->sec_lf[0], l1 = ctx->sec_lf[1], l2 = ctx->sec_lf[2];
-    const int order = cfg.lift != 2 ? 0
+    const int order = (cfg.lift != 2 || at_rest) ? 0
                       : (ctx->sec_n >= 3 && l0 != l1 && l1 != l2 && l0 != l2) ? 2
                       : (ctx->sec_n >= 2 && l0 != l1)                       ? 1
                                                                              : 0;
@@ -1538,7 +1555,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         launch_fill_f64(s, ctx->dD.p, n, 0.0);
         launch_dir_increment(s, ndir, ctx->d_dir_dofs.p, ctx->d_dir_vals.p, lf, ctx->ut.p, ctx->dD.p);
         launch_axpy_to(s, n, ctx->ut.p, 1.0, ctx->dD.p, ctx->u.p);
-    } else if (cfg.lift && any_inc) {
+    } else if (cfg.lift && any_inc && !at_rest) {
         FM_TRY(assemble(ctx, ctx->u.p, true, aprec, true, ctx->dD.p, FM_K_ASSEMBLE));
         S.assemblies++;
         FM_TRY(ensure_partials(ctx, n / 256 + 1));
@@ -1553,7 +1570,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     } else {
         launch_axpy_to(s, n, ctx->u.p, 1.0, ctx->dD.p, ctx->u.p);
     }
-    const bool predicted = extrapolate || (cfg.lift && any_inc);
+    const bool predicted = extrapolate || (cfg.lift && any_inc && !at_rest);
     double rn = 0.0;
     FM_TRY(residual(ctx, ctx->u.p, lf, &rn));
     S.residuals++;

@@ -52,6 +52,16 @@ struct Elem {
     bool clamped, ok;
 };
 
+// a + b with the product that forms b already rounded: no fused multiply-add
+// across the accumulation, so a sum of element terms has the same bits whether
+// the terms are added straight into memory (colour launches) or first stored
+// and then added (node-gather kernel)
+template <typename T>
+__device__ __forceinline__ T add_rounded(T a, T b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+
 __device__ __forceinline__ double det3(const double* A) {
     return A[0] * (A[4] * A[8] - A[7] * A[5]) - A[3] * (A[1] * A[8] - A[7] * A[2]) +
            A[6] * (A[1] * A[5] - A[4] * A[2]);
@@ -176,7 +186,7 @@ __global__ __launch_bounds__(64) void assemble_color_kernel(ElemArgs A, uint32_t
         for (int q = 0; q < 4; ++q) {
             double* fq = f + 3 * (size_t)E.node[q];
 #pragma unroll
-            for (int i = 0; i < 3; ++i) fq[i] += E.V * (E.mu * E.Fg[q][i] + s * E.a[q][i]);
+            for (int i = 0; i < 3; ++i) fq[i] = add_rounded(fq[i], E.V * (E.mu * E.Fg[q][i] + s * E.a[q][i]));
         }
     }
     if constexpr (WITH_K) {
@@ -231,7 +241,7 @@ __global__ __launch_bounds__(64) void assemble_color_kernel(ElemArgs A, uint32_t
                         }
                         double* lq = lift + 3 * (size_t)E.node[q];
 #pragma unroll
-                        for (int i = 0; i < 3; ++i) lq[i] += E.V * out[i];
+                        for (int i = 0; i < 3; ++i) lq[i] = add_rounded(lq[i], E.V * out[i]);
                     }
                 }
             }
@@ -251,7 +261,8 @@ __global__ __launch_bounds__(64) void assemble_color_kernel(ElemArgs A, uint32_t
                              sg[el][q * 3 + 2] * sg[el][b * 3 + 2];
                 T v = lpe * sa[el][q * 3 + i] * sa[el][b * 3 + k] + ce * sa[el][q * 3 + k] * sa[el][b * 3 + i];
                 if (i == k) v += mu * gg;
-                K[srp[el][qi] + 3u * srank[el][q * 4 + b] + k] += V * v;
+                T* Kp = K + srp[el][qi] + 3u * srank[el][q * 4 + b] + k;
+                *Kp = add_rounded(*Kp, V * v);
             }
         }
     }

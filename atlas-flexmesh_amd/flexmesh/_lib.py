@@ -82,6 +82,7 @@ def lib():
     L.fm_color.argtypes = [_vp, _vp, C.POINTER(C.c_uint32)]
     L.fm_assemble.argtypes = [_vp, _vp, _vp, _vp, C.c_int]
     L.fm_residual.argtypes = [_vp, _vp, C.c_double, _vp, C.POINTER(C.c_double)]
+    L.fm_assemble_residual_only.argtypes = [_vp, _vp, _f64p]
     L.fm_set_matrix.argtypes = [_vp, C.c_uint32, _u32p, _u32p, _f64p]
     L.fm_use_tangent.argtypes = [_vp, C.c_int]
     L.fm_spmv.argtypes = [_vp, C.c_int, _f64p, _f64p]
@@ -229,6 +230,13 @@ class Solver:
         uu = None if u is None else f64(u)
         self._chk(self.L.fm_assemble(self.h, _ptr(uu), f.ctypes.data, _ptr(K), prec), "fm_assemble")
         return f, K
+
+    def internal_force(self, u=None):
+        """f_int(u) alone (assemble_global_residual_only), no state change."""
+        f = np.zeros(3 * self.nn)
+        uu = None if u is None else f64(u)
+        self._chk(self.L.fm_assemble_residual_only(self.h, _ptr(uu), f), "fm_assemble_residual_only")
+        return f
 
     def residual(self, u=None, load=1.0):
         R = np.zeros(3 * self.nn)

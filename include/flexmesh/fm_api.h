@@ -88,10 +88,14 @@ int fm_color(fm_ctx* ctx, uint32_t* colors_out, uint32_t* n_colors_out);
 
 /* ---- assembly ------------------------------------------------------------------
  * assemble_global_threadlocal (FEM/adaptive_fem_engine.hpp:1126-1233):
- * f_int(u) and, when K_vals_out != NULL or keep != 0, the tangent K on the
- * pattern.  prec selects the K precision on the device (K_vals_out is FP64
- * either way).  No boundary conditions are applied.  u may be NULL (zero). */
+ * f_int(u) and the tangent K on the pattern, kept on the device (fm_use_tangent)
+ * and copied to K_vals_out when it is not NULL.  prec selects the K precision
+ * on the device (K_vals_out is FP64 either way).  No boundary conditions are
+ * applied.  u may be NULL (zero); f_int_out may be NULL. */
 int fm_assemble(fm_ctx* ctx, const double* u, double* f_int_out, double* K_vals_out, int prec);
+/* assemble_global_residual_only (:990-1059): f_int(u) alone (3nn doubles), FP64,
+   no tangent, no boundary conditions; the context's state is not changed. */
+int fm_assemble_residual_only(fm_ctx* ctx, const double* u, double* f_int_out);
 /* assemble_global_residual_only (:990-1059) and the Newton residual
    R = load*f_ext - f_int(u) with constrained rows zeroed. R_out may be NULL;
    norm_out (||R||_2) may be NULL. */

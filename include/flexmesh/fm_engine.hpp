@@ -26,7 +26,11 @@
 namespace atlas::fem::gpu {
 
 // One device context per thread (device 0 unless select_device() is called
-// first); the mesh is uploaded again only when its address or size changes.
+// first).  The mesh is recognised by content: a hash of the connectivity and
+// one of the coordinates.  A new topology is uploaded with fm_set_mesh (new
+// pattern, colouring and schedules); new coordinates on the same topology
+// (nodes moved in place, as the adapt loop's smoothing does) with
+// fm_set_coords, which keeps them.
 class Device {
 public:
     static Device& get() {
@@ -45,20 +49,29 @@ public:
 
     template <class Mesh>
     void upload(const Mesh& mesh) {
-        const void* key = static_cast<const void*>(&mesh);
         const size_t nn = mesh.nodes.size(), ne = mesh.elements.size();
-        if (key == mesh_key_ && nn == nn_ && ne == ne_) return;
+        std::vector<uint32_t> conn(4 * ne);
+        for (size_t e = 0; e < ne; ++e)
+            for (int a = 0; a < 4; ++a) conn[a * ne + e] = (uint32_t)mesh.elements[e].nodes[a];
         std::vector<double> X(3 * nn);
         for (size_t n = 0; n < nn; ++n) {
             X[n] = mesh.nodes[n].x;
             X[nn + n] = mesh.nodes[n].y;
             X[2 * nn + n] = mesh.nodes[n].z;
         }
-        std::vector<uint32_t> conn(4 * ne);
-        for (size_t e = 0; e < ne; ++e)
-            for (int a = 0; a < 4; ++a) conn[a * ne + e] = mesh.elements[e].nodes[a];
+        const uint64_t ht = hash(conn.data(), conn.size() * sizeof(uint32_t), nn * 0x9E3779B97F4A7C15ull + ne);
+        const uint64_t hx = hash(X.data(), X.size() * sizeof(double), 0x243F6A8885A308D3ull);
+        if (have_mesh_ && ht == topo_hash_ && nn == nn_ && ne == ne_) {
+            if (hx != coord_hash_) {
+                check(fm_set_coords(ctx(), (uint32_t)nn, X.data()), "fm_set_coords");
+                coord_hash_ = hx;
+            }
+            return;
+        }
         check(fm_set_mesh(ctx(), (uint32_t)nn, X.data(), (uint32_t)ne, conn.data()), "fm_set_mesh");
-        mesh_key_ = key;
+        have_mesh_ = true;
+        topo_hash_ = ht;
+        coord_hash_ = hx;
         nn_ = nn;
         ne_ = ne;
         E_ = nu_ = std::nan("");
@@ -87,9 +100,23 @@ public:
 
 private:
     Device() = default;
+    // 64-bit multiply-xorshift hash over 8-byte words (and the tail bytes)
+    static uint64_t hash(const void* data, size_t bytes, uint64_t h) {
+        const unsigned char* p = static_cast<const unsigned char*>(data);
+        size_t i = 0;
+        for (; i + 8 <= bytes; i += 8) {
+            uint64_t w;
+            std::memcpy(&w, p + i, 8);
+            h = (h ^ w) * 0xBF58476D1CE4E5B9ull;
+            h ^= h >> 31;
+        }
+        for (; i < bytes; ++i) h = (h ^ p[i]) * 0x94D049BB133111EBull;
+        return h ^ (h >> 29);
+    }
     fm_ctx* ctx_ = nullptr;
     int device_ = 0;
-    const void* mesh_key_ = nullptr;
+    bool have_mesh_ = false;
+    uint64_t topo_hash_ = 0, coord_hash_ = 0;
     size_t nn_ = 0, ne_ = 0;
     double E_ = std::nan(""), nu_ = std::nan("");
 };
@@ -126,7 +153,8 @@ void assemble_global_threadlocal(const Mesh& mesh, const Prob& prob, const doubl
     d.check(fm_assemble(d.ctx(), u, f_int.data(), K.val.data(), FM_FP64), "fm_assemble");
 }
 
-// assemble_global_residual_only (FEM/adaptive_fem_engine.hpp:990-1059)
+// assemble_global_residual_only (FEM/adaptive_fem_engine.hpp:990-1059): f_int
+// resized and overwritten; no boundary condition or other state is touched.
 template <class Mesh, class Prob>
 void assemble_global_residual_only(const Mesh& mesh, const Prob& prob, const double* u,
                                    std::vector<double>& f_int) noexcept {
@@ -134,12 +162,8 @@ void assemble_global_residual_only(const Mesh& mesh, const Prob& prob, const dou
         Device& d = Device::get();
         d.upload(mesh);
         d.material(prob.E, prob.nu);
-        std::vector<double> R(mesh.n_dofs);
-        // R = 0 * f_ext - f_int with no constraints set: f_int = -R
-        d.check(fm_set_bcs(d.ctx(), 0, nullptr, nullptr), "fm_set_bcs");
-        d.check(fm_residual(d.ctx(), u, 0.0, R.data(), nullptr), "fm_residual");
         f_int.assign(mesh.n_dofs, 0.0);
-        for (size_t i = 0; i < R.size(); ++i) f_int[i] = -R[i];
+        d.check(fm_assemble_residual_only(d.ctx(), u, f_int.data()), "fm_assemble_residual_only");
     } catch (...) {
         f_int.assign(mesh.n_dofs, std::nan(""));
     }
@@ -148,7 +172,9 @@ void assemble_global_residual_only(const Mesh& mesh, const Prob& prob, const dou
 // pcg_solve_ilu0 (FEM/adaptive_fem_engine.hpp:1283-1341): factorizes A, runs
 // PCG from x with the stopping rule sqrt(r.z / b.b) < tol; returns the
 // iteration count (max_iter = not converged, 0 = trivially converged), -1 on a
-// device error (the reference is noexcept).
+// device error (the reference is noexcept).  A call with the pattern of the
+// previous call uploads the values only (the sort and the sweep schedules are
+// kept by fm_set_matrix).
 template <class CSR>
 [[nodiscard]] int pcg_solve_ilu0(const CSR& A, const double* b, double* x, uint32_t N, double tol = 1e-10,
                                  int max_iter = 1000) noexcept {
@@ -265,10 +291,16 @@ private:
     const std::vector<double>& u_;
 };
 
-// NewtonSolverConfig / NewtonSolverResult (FEM/nonlinear_solver.hpp:23-64)
+// NewtonSolverConfig / NewtonSolverResult (FEM/nonlinear_solver.hpp:23-64).
+// Every member of the reference's configuration is honoured except verbose
+// (nothing is printed): the absolute and relative residual tests, the energy
+// test |E_k - E_(k-1)| < energy_convergence_tol after an iteration, the
+// backtracking line search from line_search_alpha with factor
+// line_search_beta, and reuse_tangent (the step's first tangent and its
+// ILU(0) factor for every iteration).
 struct NewtonSolverConfig {
     int max_iterations = 50;
-    double tol_absolute = 1e-8;  // not used: the device loop stops on the relative test
+    double tol_absolute = 1e-8;
     double tol_relative = 1e-6;
     int max_line_search = 10;
     double line_search_alpha = 1.0;
@@ -282,6 +314,10 @@ struct NewtonSolverConfig {
     int pcg_max = 300;
 };
 
+// residual_history[0] = ||r_0||, then one entry per iteration (and one after
+// the load-step predictor solve when the prescribed values changed);
+// step_size_history has one entry fewer, energy_history one per residual entry
+// (the potential energy sum_e V_e psi_e - f_ext . u).
 struct NewtonSolverResult {
     bool converged = false;
     int num_iterations = 0;
@@ -298,7 +334,10 @@ struct NewtonSolverResult {
 // the mesh, a material with params.E/params.nu and BoundaryConditions
 // (dirichlet_bcs (node, dof, value), neumann_bcs (node, fx, fy, fz)); solve()
 // brings u to equilibrium at load.load_factor, with the Dirichlet values and
-// the Neumann forces scaled by it.
+// the Neumann forces scaled by it.  A trial step is accepted when
+// ||r(u + alpha du)|| <= (1 - 0.1 alpha) ||r(u)|| (:163-165); when no trial is
+// accepted the last (smallest) step is taken and the iteration continues, as
+// the engine's armijo_line_search_true does (FEM/adaptive_fem_engine.hpp:933-985).
 template <class Mesh, class Material, class BCs>
 class NonlinearNewtonSolver {
 public:
@@ -311,6 +350,7 @@ public:
         Device& d = Device::get();
         d.upload(mesh_);
         d.material(material_.params.E, material_.params.nu);
+        if (u.size() != (size_t)mesh_.n_dofs) throw std::runtime_error("flexmesh: u is not n_dofs long");
         std::vector<uint32_t> dofs;
         std::vector<double> vals;
         for (const auto& bc : bcs_.dirichlet_bcs) {
@@ -330,8 +370,15 @@ public:
         fm_newton_cfg_default(&cfg);
         cfg.prec = config.precision;
         cfg.newton_rtol = config.tol_relative;
+        cfg.tol_absolute = config.tol_absolute;
         cfg.max_newton = config.max_iterations;
         cfg.ls_max = config.max_line_search;
+        cfg.ls_alpha0 = config.line_search_alpha;
+        cfg.ls_beta = config.line_search_beta;
+        cfg.armijo_c = 0.1;
+        cfg.reuse_tangent = config.reuse_tangent ? 1 : 0;
+        cfg.energy_tol = config.energy_convergence_tol;
+        cfg.record_energy = 1;
         cfg.pcg_tol = config.pcg_tol;
         cfg.pcg_max = config.pcg_max;
         fm_newton_stats st{};
@@ -342,8 +389,26 @@ public:
         r.num_iterations = st.newton_iters;
         r.final_residual_relative = st.rel_res;
         r.final_residual_norm = st.rel_res * st.r0;
-        r.residual_history = {st.r0, r.final_residual_norm};
         r.pcg_iterations = st.pcg_iters;
+        uint32_t len = 0;
+        d.check(fm_newton_history(d.ctx(), 0, nullptr, nullptr, nullptr, &len), "fm_newton_history");
+        r.residual_history.assign(len, 0.0);
+        r.energy_history.assign(len, 0.0);
+        r.step_size_history.assign(len > 0 ? len - 1 : 0, 0.0);
+        d.check(fm_newton_history(d.ctx(), len, r.residual_history.data(),
+                                  r.step_size_history.empty() ? nullptr : r.step_size_history.data(),
+                                  r.energy_history.data(), &len),
+                "fm_newton_history");
+        if (r.converged && !r.energy_history.empty()) {
+            const bool by_residual = r.final_residual_norm < config.tol_absolute ||
+                                     r.final_residual_relative <= config.tol_relative;
+            const size_t k = r.energy_history.size();
+            // the reference reports E_last - E_first after the residual tests and
+            // |E_k - E_(k-1)| after the energy test (:130-131, :219)
+            r.final_energy_change = by_residual || k < 2
+                                        ? r.energy_history.back() - r.energy_history.front()
+                                        : std::fabs(r.energy_history[k - 1] - r.energy_history[k - 2]);
+        }
         return r;
     }
 

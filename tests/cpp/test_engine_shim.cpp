@@ -134,6 +134,30 @@ int run() {
                              4000, &relo);
     check("pcg_solve_ilu0 iterations vs oracle (+-1)", std::abs(its - itso) <= 1, (double)its);
     check("pcg_solve_ilu0 solution vs oracle", rel(x, xo) < 1e-9, rel(x, xo));
+    // the same pattern again with new values: the kept plan must give the new solution
+    SparseCSR A2 = A;
+    for (uint32_t r = 0; r < A2.n_rows; ++r)
+        for (uint32_t k = A2.row_ptr[r]; k < A2.row_ptr[r + 1]; ++k)
+            if (A2.col_idx[k] == r) A2.val[k] = 6 * ih2;
+    for (uint32_t r = 0; r < S.n_rows; ++r)
+        for (uint32_t k = S.row_ptr[r]; k < S.row_ptr[r + 1]; ++k)
+            if (S.col_idx[k] == r) S.val[k] = 6 * ih2;
+    std::fill(x.begin(), x.end(), 0.0);
+    std::fill(xo.begin(), xo.end(), 0.0);
+    const int its2 = G::pcg_solve_ilu0(A2, b.data(), x.data(), A2.n_rows, 1e-10, 4000);
+    const int itso2 = orc_pcg(S.n_rows, S.row_ptr.data(), S.col_idx.data(), S.val.data(), b.data(), xo.data(), 1e-10,
+                              4000, &relo);
+    check("pcg_solve_ilu0 same pattern, new values vs oracle", std::abs(its2 - itso2) <= 1 && rel(x, xo) < 1e-9,
+          rel(x, xo));
+
+    // nodes moved in place (same mesh object, same topology): the device copy follows
+    for (uint32_t n = 0; n < nn; ++n) mesh.nodes[n].y += 0.004 * std::cos(3.0 * n);
+    for (uint32_t n = 0; n < nn; ++n) X[nn + n] = mesh.nodes[n].y;
+    G::assemble_global_threadlocal(mesh, prob, u.data(), K, f);
+    orc_assemble(nn, ne, X.data(), conn.data(), mv.data(), lv.data(), co.data(), nco, u.data(), rp.data(), ci.data(),
+                 Ko.data(), fo.data());
+    check("nodes moved in place: K, f follow the coordinates", rel(K.val, Ko) < 1e-12 && rel(f, fo) < 1e-12,
+          std::fmax(rel(K.val, Ko), rel(f, fo)));
 
     // NonlinearNewtonSolver: homogeneous uniaxial stretch with sliding faces has
     // the closed form F = diag(l1, l2, l2), mu (l2 - 1/l2) + lam ln(l1 l2^2)/l2 = 0
@@ -157,11 +181,75 @@ int run() {
     std::vector<double> uc(3 * nn, 0.0);
     G::NewtonSolverConfig cfg;
     cfg.tol_relative = 1e-12;
+    cfg.tol_absolute = 0.0;
+    cfg.energy_convergence_tol = 0.0;
     cfg.pcg_tol = 1e-13;
     cfg.pcg_max = 2000;
     LoadStep half{0.5}, full{1.0};
     auto r1 = solver.solve(uc, half, cfg);
     auto r2 = solver.solve(uc, full, cfg);
+    {
+        // histories in the reference's shapes (NewtonSolverResult::print indexes all three)
+        const size_t L = r2.residual_history.size();
+        bool ok = L >= 2 && r2.step_size_history.size() == L - 1 && r2.energy_history.size() == L &&
+                  std::fabs(r2.residual_history.back() - r2.final_residual_norm) <= 1e-12 * r2.residual_history[0] &&
+                  std::fabs(r2.final_residual_norm / r2.residual_history[0] - r2.final_residual_relative) <= 1e-12;
+        for (size_t i = 0; i < L; ++i) ok = ok && std::isfinite(r2.energy_history[i]);
+        for (double a : r2.step_size_history) ok = ok && a > 0.0 && a <= 1.0;
+        // no external force: the energy is the strain energy, positive at the stretched state
+        ok = ok && r2.energy_history.back() > 0.0;
+        check("NonlinearNewtonSolver histories (lengths, last = final)", ok, (double)L);
+    }
+    {
+        // tol_absolute: a solve from the converged state stops at once (:127)
+        G::NewtonSolverConfig c2 = cfg;
+        c2.tol_absolute = 1e-8;
+        std::vector<double> ucc = uc;
+        auto r3 = solver.solve(ucc, full, c2);
+        check("tol_absolute: converged state, 0 iterations", r3.converged && r3.num_iterations == 0 &&
+                                                              r3.residual_history.size() == 1,
+              r3.final_residual_norm);
+    }
+    {
+        // the energy test (:215-227) with the reference's default 1e-2: stops once |dE| < 1e-2
+        G::NewtonSolverConfig c3;
+        c3.tol_absolute = 0.0;
+        c3.tol_relative = 1e-14;
+        std::vector<double> u0(3 * nn, 0.0);
+        auto r4 = solver.solve(u0, half, c3);
+        const size_t k = r4.energy_history.size();
+        const double dE = k >= 2 ? std::fabs(r4.energy_history[k - 1] - r4.energy_history[k - 2]) : 1.0;
+        check("energy_convergence_tol stops Newton", r4.converged && k >= 2 && dE < 1e-2 &&
+                                                    r4.final_residual_relative > 1e-14 &&
+                                                    std::fabs(r4.final_energy_change - dE) <= 1e-15,
+              dE);
+    }
+    {
+        // reuse_tangent: the first tangent for every iteration still reaches the same state
+        G::NewtonSolverConfig c4 = cfg;
+        c4.reuse_tangent = true;
+        c4.tol_relative = 1e-10;
+        c4.max_iterations = 200;
+        std::vector<double> u0(3 * nn, 0.0);
+        auto r5 = solver.solve(u0, half, c4);
+        auto r5b = solver.solve(u0, full, c4);
+        check("reuse_tangent converges to the same state", r5.converged && r5b.converged &&
+                                                          r5b.num_iterations >= r2.num_iterations &&
+                                                          rel(u0, uc) < 1e-8,
+              rel(u0, uc));
+    }
+    {
+        bool threw = false;
+        G::NewtonSolverConfig c5 = cfg;
+        c5.line_search_beta = 1.5;
+        std::vector<double> u0(3 * nn, 0.0);
+        try {
+            (void)solver.solve(u0, half, c5);
+        } catch (const std::exception&) {
+            threw = true;
+        }
+        check("line_search_beta outside (0, 1) is rejected", threw, 0.0);
+    }
     double lo = 0.3, hi = 1.5;
     auto fl = [&](double l2) { return mu * (l2 - 1 / l2) + lam * std::log((1 + delta) * l2 * l2) / l2; };
     for (int it = 0; it < 200; ++it) {
