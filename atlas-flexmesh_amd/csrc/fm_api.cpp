@@ -1823,46 +1823,145 @@ int fm_zz_estimate(fm_ctx* ctx, const double* u, double* elem_errors, double* no
     return FM_OK;
 }
 
+// ---- checkpoint ------------------------------------------------------------------
+// File: "FMCKPT02", u32 version 2, u32 flags 0, u64 length L of the next part;
+// then L bytes laid out as CheckpointFile::save writes them
+// (FEM/adaptive_fem_engine.hpp:1347-1398): u32 load_step, adapt_iter, n_nodes,
+// n_elements, n_dofs; per node x, y, z, u, v, w; per element 4 x u32 nodes,
+// volume, jacobian_det, quality_metric; per element state F_data (9, F_ij at
+// i*3+j), stress (6, Mandel: M11, M22, M33, sqrt2 M12, sqrt2 M13, sqrt2 M23 of
+// M = C S), eps_plastic (6), kappa, gamma_p; u.  Then mu_e and lam_e per
+// element, and the FNV-1a hash of every byte before it.
+namespace {
+
+struct CkptWriter {
+    FILE* fp = nullptr;
+    uint64_t h = 1469598103934665603ull;
+    std::vector<uint8_t> buf;
+    bool ok = true;
+    void put(const void* p, size_t n) {
+        fnv1a_update(h, p, n);
+        const uint8_t* b = static_cast<const uint8_t*>(p);
+        buf.insert(buf.end(), b, b + n);
+        if (buf.size() >= (8u << 20)) flush();
+    }
+    void flush() {
+        if (!buf.empty()) ok = ok && std::fwrite(buf.data(), 1, buf.size(), fp) == buf.size();
+        buf.clear();
+    }
+};
+
+constexpr size_t kCkptNodeBytes = 48, kCkptElemBytes = 40, kCkptStateBytes = 23 * 8;
+
+// the reference's tet_signed_volume6 / tet_volume / tet_quality
+// (FEM/mesh_adaptation.hpp:110-146) on element e of the SoA mesh
+void elem_record(const std::vector<double>& X, uint32_t nn, const uint32_t* nd, double* vol, double* jac,
+                 double* qual) {
+    double P[4][3];
+    for (int a = 0; a < 4; ++a)
+        for (int d = 0; d < 3; ++d) P[a][d] = X[(size_t)d * nn + nd[a]];
+    const double e1[3] = {P[1][0] - P[0][0], P[1][1] - P[0][1], P[1][2] - P[0][2]};
+    const double e2[3] = {P[2][0] - P[0][0], P[2][1] - P[0][1], P[2][2] - P[0][2]};
+    const double e3[3] = {P[3][0] - P[0][0], P[3][1] - P[0][1], P[3][2] - P[0][2]};
+    const double v6 = std::fabs(e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+                                e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]));
+    *vol = v6 / 6.0;
+    *jac = *vol * 6.0;
+    double sq = 0.0;
+    for (int a = 0; a < 4; ++a)
+        for (int b = a + 1; b < 4; ++b)
+            for (int d = 0; d < 3; ++d) sq += (P[a][d] - P[b][d]) * (P[a][d] - P[b][d]);
+    *qual = sq < 1e-30 ? 0.0 : 12.0 * std::sqrt(2.0) * v6 / std::pow(sq, 1.5);
+}
+
+}  // namespace
+
 int fm_checkpoint_save(fm_ctx* ctx, const char* path, uint32_t load_step) {
     if (!ctx || !path) return FM_EINVAL;
     if (!ctx->u_valid) return fail(ctx, FM_ESTATE, "no device state");
+    if (!ctx->have_material) return fail(ctx, FM_ESTATE, "no material");
     const uint32_t nn = ctx->nn, ne = ctx->ne, nd = 3 * nn;
     std::vector<double> u(nd);
     FM_HIP(hipMemcpy(u.data(), ctx->u.p, nd * 8ull, hipMemcpyDeviceToHost));
+    // F of every element (element order) from the element-fields kernel
+    std::vector<double> F((size_t)9 * ne);
+    {
+        fm_element_fields_out o{};
+        o.F = F.data();
+        FM_TRY(fm_element_fields(ctx, nullptr, &o));
+    }
     FILE* fp = std::fopen(path, "wb");
     if (!fp) return fail(ctx, FM_EINVAL, std::string("cannot open ") + path);
-    uint64_t h = 1469598103934665603ull;
-    auto put = [&](const void* p, size_t bytes) {
-        fnv1a_update(h, p, bytes);
-        return std::fwrite(p, 1, bytes, fp) == bytes;
-    };
-    const char magic[8] = {'F', 'M', 'C', 'K', 'P', 'T', '0', '1'};
-    const uint32_t hdr[6] = {1u /*version*/, load_step, 0u /*adapt_iter*/, nn, ne, nd};
-    bool ok = put(magic, 8) && put(hdr, sizeof hdr);
-    // nodes: x, y, z, u, v, w (the reference's 6 x f64 node record)
-    for (uint32_t v = 0; v < nn && ok; ++v) {
+    CkptWriter w;
+    w.fp = fp;
+    const uint64_t image = 20 + (uint64_t)nn * kCkptNodeBytes + (uint64_t)ne * (kCkptElemBytes + kCkptStateBytes) +
+                           (uint64_t)nd * 8;
+    const uint32_t ver[2] = {2u, 0u};
+    w.put("FMCKPT02", 8);
+    w.put(ver, sizeof ver);
+    w.put(&image, 8);
+    const uint32_t hdr[5] = {load_step, 0u /*adapt_iter*/, nn, ne, nd};
+    w.put(hdr, sizeof hdr);
+    for (uint32_t v = 0; v < nn; ++v) {
         const double rec[6] = {ctx->hX[v], ctx->hX[nn + v], ctx->hX[2ull * nn + v], u[3ull * v], u[3ull * v + 1],
                                u[3ull * v + 2]};
-        ok = put(rec, sizeof rec);
+        w.put(rec, sizeof rec);
     }
-    // elements: 4 x u32 nodes, mu_e, lam_e
-    for (uint32_t e = 0; e < ne && ok; ++e) {
+    std::vector<double> geo((size_t)3 * ne);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < (int64_t)ne; ++e) {
         const uint32_t c[4] = {ctx->hconn[e], ctx->hconn[(size_t)ne + e], ctx->hconn[2ull * ne + e],
                                ctx->hconn[3ull * ne + e]};
-        const double m[2] = {ctx->hmu.empty() ? 0.0 : ctx->hmu[e], ctx->hlam.empty() ? 0.0 : ctx->hlam[e]};
-        ok = put(c, sizeof c) && put(m, sizeof m);
+        elem_record(ctx->hX, nn, c, &geo[3 * e], &geo[3 * e + 1], &geo[3 * e + 2]);
     }
-    ok = ok && put(u.data(), nd * 8ull);
-    const uint64_t hh = h;
-    ok = ok && std::fwrite(&hh, 1, 8, fp) == 8;
+    for (uint32_t e = 0; e < ne; ++e) {
+        const uint32_t c[4] = {ctx->hconn[e], ctx->hconn[(size_t)ne + e], ctx->hconn[2ull * ne + e],
+                               ctx->hconn[3ull * ne + e]};
+        w.put(c, sizeof c);
+        w.put(&geo[3ull * e], 24);
+    }
+    // states: F, the Mandel stress M = C S = mu (C - I) + lam ln(max(J, 0.01)) I
+    // (neo_hookean_response, FEM/adaptive_fem_engine.hpp:79-129, with J = |det F|),
+    // zero plastic fields; degenerate elements keep the initial state (F = I)
+    std::vector<double> rec((size_t)23 * ne, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < (int64_t)ne; ++e) {
+        double* r = &rec[23 * e];
+        double f[9];
+        for (int k = 0; k < 9; ++k) f[k] = F[(size_t)k * ne + e];
+        if (geo[3 * e] < 1e-30) {
+            for (int k = 0; k < 9; ++k) r[k] = (k % 4 == 0) ? 1.0 : 0.0;
+            continue;
+        }
+        double C[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) C[i * 3 + j] = f[i] * f[j] + f[3 + i] * f[3 + j] + f[6 + i] * f[6 + j];
+        const double J = std::fabs(f[0] * (f[4] * f[8] - f[5] * f[7]) - f[1] * (f[3] * f[8] - f[5] * f[6]) +
+                                   f[2] * (f[3] * f[7] - f[4] * f[6]));
+        const double mu = ctx->hmu[e], lam = ctx->hlam[e];
+        const double p = lam * std::log(std::max(J, 0.01)) - mu;
+        const double s2 = std::sqrt(2.0);
+        for (int k = 0; k < 9; ++k) r[k] = f[k];
+        r[9] = mu * C[0] + p;
+        r[10] = mu * C[4] + p;
+        r[11] = mu * C[8] + p;
+        r[12] = s2 * mu * C[1];
+        r[13] = s2 * mu * C[2];
+        r[14] = s2 * mu * C[5];
+    }
+    w.put(rec.data(), rec.size() * 8);
+    w.put(u.data(), nd * 8ull);
+    w.put(ctx->hmu.data(), (size_t)ne * 8);
+    w.put(ctx->hlam.data(), (size_t)ne * 8);
+    w.flush();
+    const uint64_t hh = w.h;
+    bool ok = w.ok && std::fwrite(&hh, 1, 8, fp) == 8;
     ok = (std::fclose(fp) == 0) && ok;
     return ok ? FM_OK : fail(ctx, FM_EINVAL, "checkpoint write failed");
 }
 
 int fm_checkpoint_load(fm_ctx* ctx, const char* path, uint32_t* load_step) {
     if (!ctx || !path) return FM_EINVAL;
-    ctx->sec_n = 0;
-    if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
     FILE* fp = std::fopen(path, "rb");
     if (!fp) return fail(ctx, FM_EINVAL, std::string("cannot open ") + path);
     std::fseek(fp, 0, SEEK_END);
@@ -1871,22 +1970,64 @@ int fm_checkpoint_load(fm_ctx* ctx, const char* path, uint32_t* load_step) {
     std::vector<uint8_t> buf(size > 0 ? (size_t)size : 0);
     const bool rd = size > 0 && std::fread(buf.data(), 1, buf.size(), fp) == buf.size();
     std::fclose(fp);
-    const uint32_t nn = ctx->nn, ne = ctx->ne, nd = 3 * nn;
-    const size_t expect = 8 + 24 + (size_t)nn * 48 + (size_t)ne * 32 + (size_t)nd * 8 + 8;
-    if (!rd || buf.size() != expect) return fail(ctx, FM_EINVAL, "checkpoint truncated or of another mesh size");
-    if (std::memcmp(buf.data(), "FMCKPT01", 8) != 0) return fail(ctx, FM_EINVAL, "not a checkpoint file");
+    if (!rd || buf.size() < 24 + 20 + 8) return fail(ctx, FM_EINVAL, "checkpoint truncated");
+    if (std::memcmp(buf.data(), "FMCKPT02", 8) != 0) return fail(ctx, FM_EINVAL, "not a version-2 checkpoint file");
     uint64_t h = 1469598103934665603ull;
     fnv1a_update(h, buf.data(), buf.size() - 8);
     uint64_t stored;
     std::memcpy(&stored, buf.data() + buf.size() - 8, 8);
     if (h != stored) return fail(ctx, FM_EINVAL, "checkpoint checksum mismatch");
-    uint32_t hdr[6];
-    std::memcpy(hdr, buf.data() + 8, sizeof hdr);
-    if (hdr[0] != 1 || hdr[3] != nn || hdr[4] != ne || hdr[5] != nd)
-        return fail(ctx, FM_EINVAL, "checkpoint version or sizes differ from the mesh");
-    std::vector<double> u(nd);
-    std::memcpy(u.data(), buf.data() + buf.size() - 8 - (size_t)nd * 8, (size_t)nd * 8);
-    if (load_step) *load_step = hdr[1];
+    uint64_t image;
+    std::memcpy(&image, buf.data() + 16, 8);
+    const uint8_t* img = buf.data() + 24;
+    uint32_t hdr[5];
+    std::memcpy(hdr, img, sizeof hdr);
+    const uint32_t nn = hdr[2], ne = hdr[3], nd = hdr[4];
+    const uint64_t expect_img = 20 + (uint64_t)nn * kCkptNodeBytes + (uint64_t)ne * (kCkptElemBytes + kCkptStateBytes) +
+                                (uint64_t)nd * 8;
+    if (nd != 3ull * nn || image != expect_img || buf.size() != 24 + image + 16ull * ne + 8)
+        return fail(ctx, FM_EINVAL, "checkpoint sizes are inconsistent");
+    // mesh and material of the file
+    std::vector<double> X((size_t)3 * nn), u(nd), mu(ne), lam(ne);
+    std::vector<uint32_t> conn((size_t)4 * ne);
+    const uint8_t* pn = img + 20;
+    for (uint32_t v = 0; v < nn; ++v) {
+        double r[6];
+        std::memcpy(r, pn + (size_t)v * kCkptNodeBytes, sizeof r);
+        X[v] = r[0];
+        X[nn + v] = r[1];
+        X[2ull * nn + v] = r[2];
+    }
+    const uint8_t* pe = pn + (size_t)nn * kCkptNodeBytes;
+    for (uint32_t e = 0; e < ne; ++e) {
+        uint32_t c[4];
+        std::memcpy(c, pe + (size_t)e * kCkptElemBytes, sizeof c);
+        for (int a = 0; a < 4; ++a) {
+            if (c[a] >= nn) return fail(ctx, FM_EINVAL, "checkpoint connectivity out of range");
+            conn[(size_t)a * ne + e] = c[a];
+        }
+    }
+    const uint8_t* pu = pe + (size_t)ne * (kCkptElemBytes + kCkptStateBytes);
+    std::memcpy(u.data(), pu, (size_t)nd * 8);
+    std::memcpy(mu.data(), pu + (size_t)nd * 8, (size_t)ne * 8);
+    std::memcpy(lam.data(), pu + (size_t)nd * 8 + (size_t)ne * 8, (size_t)ne * 8);
+    if (ctx->nn) {
+        // a context with a mesh restores u only, and only onto the same mesh and material
+        if (nn != ctx->nn || ne != ctx->ne) return fail(ctx, FM_EINVAL, "checkpoint is of a mesh of another size");
+        if (std::memcmp(X.data(), ctx->hX.data(), X.size() * 8) != 0)
+            return fail(ctx, FM_EINVAL, "checkpoint node coordinates differ from the mesh");
+        if (std::memcmp(conn.data(), ctx->hconn.data(), conn.size() * 4) != 0)
+            return fail(ctx, FM_EINVAL, "checkpoint connectivity differs from the mesh");
+        if (ctx->have_material && (std::memcmp(mu.data(), ctx->hmu.data(), (size_t)ne * 8) != 0 ||
+                                   std::memcmp(lam.data(), ctx->hlam.data(), (size_t)ne * 8) != 0))
+            return fail(ctx, FM_EINVAL, "checkpoint material differs from the context's");
+        if (!ctx->have_material) FM_TRY(fm_set_material(ctx, mu.data(), lam.data()));
+    } else {
+        // an empty context takes the mesh and the material from the file
+        FM_TRY(fm_set_mesh(ctx, nn, X.data(), ne, conn.data()));
+        FM_TRY(fm_set_material(ctx, mu.data(), lam.data()));
+    }
+    if (load_step) *load_step = hdr[0];
     return fm_set_u(ctx, u.data());
 }
 

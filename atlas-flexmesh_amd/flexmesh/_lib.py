@@ -174,6 +174,35 @@ class Solver:
         except Exception:
             pass
 
+    def _sizes(self):
+        """Node and element counts of the context (the library's, which a
+        checkpoint load into an empty context sets)."""
+        o = np.zeros(5, dtype=np.uint64)
+        self._chk(self.L.fm_info(self.h, o), "fm_info")
+        self.nn, self.ne = int(o[3]), int(o[2])
+        return self.nn, self.ne
+
+    def _nodal(self, a, what="u"):
+        """a as a contiguous FP64 array of 3 nn values (None stays None); the
+        library reads exactly that many."""
+        if a is None:
+            return None
+        a = f64(a)
+        nn, _ = self._sizes()
+        if a.size != 3 * nn:
+            raise ValueError(f"{what} has {a.size} values, the mesh has 3 x {nn}")
+        return a
+
+    def _nodal_inout(self, u):
+        """u passed in and written back: it must already be a contiguous FP64
+        array of 3 nn values."""
+        if u is None:
+            return None
+        nn, _ = self._sizes()
+        if not (isinstance(u, np.ndarray) and u.dtype == np.float64 and u.flags.c_contiguous and u.size == 3 * nn):
+            raise ValueError(f"u must be a contiguous float64 array of 3 x {nn} values")
+        return u
+
     def _chk(self, rc, what):
         if rc != FM_OK:
             msg = self.L.fm_last_error(self.h).decode()
@@ -208,12 +237,14 @@ class Solver:
     def pattern(self):
         nnz = C.c_uint64(0)
         self._chk(self.L.fm_build_pattern(self.h, None, None, C.byref(nnz)), "fm_build_pattern")
+        self._sizes()
         rp = np.zeros(3 * self.nn + 1, dtype=np.uint32)
         ci = np.zeros(nnz.value, dtype=np.uint32)
         self._chk(self.L.fm_build_pattern(self.h, rp.ctypes.data, ci.ctypes.data, C.byref(nnz)), "fm_build_pattern")
         return rp, ci
 
     def colors(self):
+        self._sizes()
         c = np.zeros(self.ne, dtype=np.uint32)
         nc = C.c_uint32(0)
         self._chk(self.L.fm_color(self.h, c.ctypes.data, C.byref(nc)), "fm_color")
@@ -221,27 +252,27 @@ class Solver:
 
     # ---- assembly ---------------------------------------------------------------
     def assemble(self, u=None, prec=FM_FP64, want_K=True):
+        uu = self._nodal(u)
         f = np.zeros(3 * self.nn)
         K = None
         if want_K:
             nnz = C.c_uint64(0)
             self._chk(self.L.fm_build_pattern(self.h, None, None, C.byref(nnz)), "fm_build_pattern")
             K = np.zeros(nnz.value)
-        uu = None if u is None else f64(u)
         self._chk(self.L.fm_assemble(self.h, _ptr(uu), f.ctypes.data, _ptr(K), prec), "fm_assemble")
         return f, K
 
     def internal_force(self, u=None):
         """f_int(u) alone (assemble_global_residual_only), no state change."""
+        uu = self._nodal(u)
         f = np.zeros(3 * self.nn)
-        uu = None if u is None else f64(u)
         self._chk(self.L.fm_assemble_residual_only(self.h, _ptr(uu), f), "fm_assemble_residual_only")
         return f
 
     def residual(self, u=None, load=1.0):
+        uu = self._nodal(u)
         R = np.zeros(3 * self.nn)
         nr = C.c_double(0.0)
-        uu = None if u is None else f64(u)
         self._chk(self.L.fm_residual(self.h, _ptr(uu), load, R.ctypes.data, C.byref(nr)), "fm_residual")
         return R, nr.value
 
@@ -291,7 +322,7 @@ class Solver:
 
     def newton_step(self, step, cfg, u=None, check=True):
         st = NewtonStats()
-        uu = None if u is None else u
+        uu = self._nodal_inout(u)
         rc = self.L.fm_newton_step(self.h, step, _ptr(uu), C.byref(cfg), C.byref(st))
         if check:
             self._chk(rc, "fm_newton_step")
@@ -299,20 +330,21 @@ class Solver:
 
     def newton_solve_to(self, load_factor, cfg, u=None, check=True):
         st = NewtonStats()
-        rc = self.L.fm_newton_solve_to(self.h, load_factor, _ptr(u), C.byref(cfg), C.byref(st))
+        rc = self.L.fm_newton_solve_to(self.h, load_factor, _ptr(self._nodal_inout(u)), C.byref(cfg), C.byref(st))
         if check:
             self._chk(rc, "fm_newton_solve_to")
         return rc, st.as_dict()
 
     def get_u(self):
-        u = np.zeros(3 * self.nn)
+        nn, _ = self._sizes()
+        u = np.zeros(3 * nn)
         self._chk(self.L.fm_get_u(self.h, u), "fm_get_u")
         return u
 
     def set_u(self, u):
         """Device state u (3 nn); None returns to the unloaded configuration
         (u = 0 at load factor 0, predictor history restarted there)."""
-        uu = None if u is None else f64(u)
+        uu = self._nodal(u)
         self._chk(self.L.fm_set_u(self.h, _ptr(uu)), "fm_set_u")
 
     def newton_history(self):
@@ -326,11 +358,12 @@ class Solver:
 
     # ---- adjoint / rotations / checkpoint --------------------------------------
     def adjoint(self, dLdu, tol=1e-12, max_it=5000):
+        g = self._nodal(dLdu, "dL/du")
         dmu = np.zeros(self.ne)
         dlam = np.zeros(self.ne)
         dl = C.c_double(0.0)
         its = C.c_int(0)
-        self._chk(self.L.fm_adjoint(self.h, f64(dLdu), tol, max_it, dmu.ctypes.data, dlam.ctypes.data, C.byref(dl),
+        self._chk(self.L.fm_adjoint(self.h, g, tol, max_it, dmu.ctypes.data, dlam.ctypes.data, C.byref(dl),
                                     C.byref(its)), "fm_adjoint")
         return dmu, dlam, dl.value, its.value
 
@@ -345,6 +378,7 @@ class Solver:
         return R.reshape(n, 3, 3), w, Q.reshape(n, 3, 3)
 
     def element_rotations(self):
+        self._sizes()
         w = np.zeros(3 * self.ne)
         self._chk(self.L.fm_element_rotations(self.h, w), "fm_element_rotations")
         return w.reshape(self.ne, 3)
@@ -352,7 +386,7 @@ class Solver:
     def element_fields(self, u=None, fields=FIELD_NAMES):
         """Per-element fields of u (None: the device state), element order:
         cauchy is 6 x ne (xx, yy, zz, yz, xz, xy), F is 9 x ne (F_ij row-major)."""
-        uu = None if u is None else f64(u)
+        uu = self._nodal(u)
         out = {}
         o = ElementFieldsOut()
         for k in fields:
@@ -364,7 +398,7 @@ class Solver:
     def zz_estimate(self, u=None):
         """ZZ estimate of u (None: the device state): (global, element indicators,
         node averages 3*nn, recovered values 3*nn)."""
-        uu = None if u is None else f64(u)
+        uu = self._nodal(u)
         err, raw, rec = np.zeros(self.ne), np.zeros(3 * self.nn), np.zeros(3 * self.nn)
         g = C.c_double(0.0)
         self._chk(self.L.fm_zz_estimate(self.h, _ptr(uu), err.ctypes.data, raw.ctypes.data, rec.ctypes.data,
@@ -377,6 +411,7 @@ class Solver:
     def checkpoint_load(self, path):
         s = C.c_uint32(0)
         self._chk(self.L.fm_checkpoint_load(self.h, path.encode(), C.byref(s)), "fm_checkpoint_load")
+        self._sizes()  # an empty context has taken the file's mesh
         return s.value
 
     # ---- measurement ------------------------------------------------------------

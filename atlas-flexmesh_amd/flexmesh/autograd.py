@@ -37,18 +37,24 @@ class Problem:
 
     def solve(self, mu, lam, scale):
         cfg, S = self.cfg, self.S
+        n = cfg["n_steps"]
+        restart = self.restart and self.have_ckpt
+        if restart:
+            # the checkpoint holds the step-(n-1) state of the fields it was
+            # written with (a load checks them); the new fields are set on it
+            S.set_material(*self.ckpt_fields)
+            S.checkpoint_load(self.ckpt)
         S.set_material(mu, lam)
         S.set_bcs(cfg["dir_dofs"], scale * cfg["dir_vals"])
         S.set_load(None if cfg["f_ext"] is None else scale * cfg["f_ext"])
-        n = cfg["n_steps"]
-        if self.restart and self.have_ckpt:
-            S.checkpoint_load(self.ckpt)
+        if restart:
             S.newton_step(n, self.newton)
         else:
             S.set_u(np.zeros(cfg["X"].size))
             for s in range(1, n + 1):
                 if s == n:
                     S.checkpoint_save(self.ckpt, s - 1)
+                    self.ckpt_fields = (np.array(mu, dtype=np.float64), np.array(lam, dtype=np.float64))
                     self.have_ckpt = True
                 S.newton_step(s, self.newton)
         return S.get_u()

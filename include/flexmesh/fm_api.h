@@ -252,9 +252,21 @@ int fm_zz_estimate(fm_ctx* ctx, const double* u, double* elem_errors, double* no
                    double* global_estimate);
 
 /* ---- checkpoint ----------------------------------------------------------------------
- * Versioned, checksummed load-step checkpoint (superset of CheckpointFile,
- * FEM/adaptive_fem_engine.hpp:1347-1452): magic, version, counts, load step,
- * u, FNV-1a checksum. */
+ * Versioned, checksummed load-step checkpoint, a superset of CheckpointFile
+ * (FEM/adaptive_fem_engine.hpp:1347-1452): "FMCKPT02", u32 version 2, u32
+ * flags, u64 length L; then L bytes exactly as CheckpointFile::save writes them
+ * (u32 load_step, adapt_iter, n_nodes, n_elements, n_dofs; per node x, y, z, u,
+ * v, w; per element 4 x u32 nodes, volume, jacobian_det, quality_metric; per
+ * element state F_data[9] (F_ij at i*3+j), the Mandel stress of
+ * neo_hookean_response (M = C S, M11, M22, M33, sqrt2 M12, sqrt2 M13, sqrt2 M23)
+ * at J = |det F|, eps_plastic[6] = 0, kappa = 0, gamma_p = 0; u); then mu_e
+ * and lam_e per element; then the FNV-1a hash of all preceding bytes.  F is
+ * the exact P1 gradient (not projected onto det F = 1).
+ * fm_checkpoint_save writes the device-resident state.  fm_checkpoint_load
+ * checks the hash and, on a context with a mesh, that the file's node
+ * coordinates, connectivity and material are the context's, bit for bit
+ * (FM_EINVAL otherwise), then restores u; on an empty context it sets the
+ * mesh, the material and u from the file. */
 int fm_checkpoint_save(fm_ctx* ctx, const char* path, uint32_t load_step);
 int fm_checkpoint_load(fm_ctx* ctx, const char* path, uint32_t* load_step);
 

@@ -130,6 +130,55 @@ int ref_pcg(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const 
     return pcg_solve_ilu0(A, b, x, n, tol, max_iter);
 }
 
+// CheckpointFile::save (FEM/adaptive_fem_engine.hpp:1347-1398) of a mesh with
+// the displacement u (3 nn), which also goes into the node records' u, v, w.
+// Element records: the reference's tet_volume and tet_quality
+// (FEM/mesh_adaptation.hpp:110-146) and jacobian_det = 6 volume, as
+// generate_unit_cube_mesh sets them (:2266-2268).  States: F of each element
+// (the P1 gradient, F_data[i*3+j] = F_ij as update_element_states_rigorous
+// fills it, :2755-2758, without its projection onto det F = 1) and the Mandel
+// stress of neo_hookean_response (:79-129) with J = |det F|; the plastic fields
+// stay zero.  Returns 0 on success.
+int ref_checkpoint_save(const char* path, uint32_t nn, const double* X, const double* u, uint32_t ne,
+                        const uint32_t* conn, double E, double nu, uint32_t load_step, uint32_t adapt_iter) {
+    MeshTopology m = mesh_from(nn, X, ne, conn);
+    for (uint32_t n = 0; n < nn; ++n) {
+        m.nodes[n].u = u[3 * n];
+        m.nodes[n].v = u[3 * n + 1];
+        m.nodes[n].w = u[3 * n + 2];
+    }
+    for (uint32_t e = 0; e < ne; ++e) {
+        TetraElement& el = m.elements[e];
+        const MeshNode &n0 = m.nodes[el.nodes[0]], &n1 = m.nodes[el.nodes[1]], &n2 = m.nodes[el.nodes[2]],
+                       &n3 = m.nodes[el.nodes[3]];
+        el.volume = tet_volume(n0, n1, n2, n3);
+        el.quality_metric = tet_quality(n0, n1, n2, n3);
+        el.jacobian_det = el.volume * 6.0;
+        double J[9];
+        const MeshNode* nd[4] = {&n0, &n1, &n2, &n3};
+        for (int d = 0; d < 3; ++d)
+            for (int a = 1; a < 4; ++a) J[d * 3 + a - 1] = nd[a]->coord(d) - nd[0]->coord(d);
+        double Ji[9];
+        matrix_3x3_inv(J, Ji);
+        const double dxi[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        double F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        for (int a = 0; a < 4; ++a) {
+            double g[3];
+            for (int i = 0; i < 3; ++i) g[i] = Ji[0 * 3 + i] * dxi[a][0] + Ji[1 * 3 + i] * dxi[a][1] + Ji[2 * 3 + i] * dxi[a][2];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) F[i * 3 + j] += u[3 * el.nodes[a] + i] * g[j];
+        }
+        const double detF = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                            F[2] * (F[3] * F[7] - F[4] * F[6]);
+        ElementState& st = m.states[e];
+        st.initialize();
+        for (int k = 0; k < 9; ++k) st.F_data[k] = F[k];
+        neo_hookean_response(st.F_data, E, nu, st.stress, std::fabs(detF));
+    }
+    std::vector<double> uu(u, u + 3 * (size_t)nn);
+    return CheckpointFile::save(path, m, uu, load_step, adapt_iter) ? 0 : 1;
+}
+
 // exponential_map / matrix_logarithm (CORE/lie_operator.cpp:94-275), column-major 3x3
 void ref_expm(const double A[9], double out[9]) {
     atlas::Matrix3x3 M;

@@ -5,7 +5,8 @@ built by oracle/build_ref.sh from the reference sources).  Each fixture stores
 inputs and the reference's outputs only; tests read the fixtures and never run
 this script.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py              # every fixture
+    python tests/golden/make_golden.py checkpoint   # the checkpoint fixture only
 """
 import ctypes as C
 import os
@@ -131,7 +132,35 @@ def stress_fixture(L):
              sigma=np.array(Sa), von_mises=np.array(Va))
 
 
+def checkpoint_fixture(L):
+    """CheckpointFile::save bytes for a jittered 2x3x2 Kuhn box with a smooth
+    displacement, E = 1, nu = 0.3, load step 7, adapt iteration 0."""
+    import tempfile
+
+    L.ref_checkpoint_save.argtypes = [C.c_char_p, C.c_uint32, f64p, f64p, C.c_uint32, u32p, C.c_double,
+                                      C.c_double, C.c_uint32, C.c_uint32]
+    L.ref_checkpoint_save.restype = C.c_int
+    X, conn = O.box_mesh(2, 3, 2)
+    nn, ne = X.size // 3, conn.size // 4
+    P = X.reshape(3, nn).T
+    rng = np.random.default_rng(21)
+    P = P + rng.uniform(-0.05, 0.05, P.shape)
+    X = np.ascontiguousarray(P.T).ravel()
+    u = np.ascontiguousarray(np.stack([0.12 * P[:, 0] + 0.03 * np.sin(3 * P[:, 1]),
+                                       -0.04 * P[:, 1] + 0.02 * P[:, 0] * P[:, 2],
+                                       -0.05 * P[:, 2] + 0.01 * np.cos(2 * P[:, 0])], axis=1).ravel())
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "ref.ckpt")
+        assert L.ref_checkpoint_save(path.encode(), nn, X, u, ne, conn, 1.0, 0.3, 7, 0) == 0
+        data = np.frombuffer(open(path, "rb").read(), dtype=np.uint8).copy()
+    np.savez_compressed(os.path.join(HERE, "checkpoint.npz"), X=X, conn=conn, u=u, E=np.array([1.0]),
+                        nu=np.array([0.3]), load_step=np.array([7], np.uint32), ref_bytes=data)
+
+
 def main():
+    if sys.argv[1:] == ["checkpoint"]:
+        checkpoint_fixture(load_ref())
+        return
     L = load_ref()
     out = {}
 
@@ -240,6 +269,7 @@ def main():
     np.savez(os.path.join(HERE, "lie.npz"), w=np.array(ws), expm=np.array(Es), logm=np.array(Ls),
              xi_ref=xi_ref.ravel(), expm_xi_ref=E.reshape(3, 3).T.ravel())
     stress_fixture(L)
+    checkpoint_fixture(L)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

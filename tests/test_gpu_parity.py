@@ -534,23 +534,6 @@ def test_zz_estimate_matches_oracle(fm, S, O):
         T.close()
 
 
-def test_checkpoint_roundtrip(fm, S, tmp_path):
-    cfg = fm.configs.make(1, dims=(3, 3, 3))
-    fm.configs.load(S, cfg)
-    u = np.random.default_rng(6).standard_normal(cfg["X"].size)
-    S.set_u(u)
-    p = str(tmp_path / "ck.bin")
-    S.checkpoint_save(p, 7)
-    S.set_u(np.zeros_like(u))
-    assert S.checkpoint_load(p) == 7
-    assert np.array_equal(S.get_u(), u)
-    data = bytearray(open(p, "rb").read())
-    data[100] ^= 1
-    open(p, "wb").write(bytes(data))
-    with pytest.raises(fm.FlexMeshError):
-        S.checkpoint_load(p)
-
-
 def test_one_context_across_meshes_and_matrices(fm, O):
     """One context: mesh A (pattern, colours, assembly), an external CSR
     matrix through PCG, then mesh B through Newton -- the sequence a host

@@ -158,10 +158,14 @@ def run_cfg4(args):
     fd_rel = []
 
     def loss_at(mu, lam, scale=1.0):
+        # the checkpoint restores the step-(n-1) state of the nominal fields
+        # (a load checks that the context holds the file's mesh and material);
+        # then the perturbed fields are set on it
+        S.set_material(cfg["mu"], cfg["lam"])
+        S.checkpoint_load(ck)
         S.set_material(mu, lam)
         if scale != 1.0:
             S.set_bcs(cfg["dir_dofs"], scale * cfg["dir_vals"])
-        S.checkpoint_load(ck)
         _, st = S.newton_step(cfg["n_steps"], c, check=False)
         fd_rel.append(st["rel_res"])
         if scale != 1.0:
@@ -257,10 +261,11 @@ def run_cfg5(args):
         lf = n / cfg["n_steps"]
 
         def defl_at(scale=1.0, mu=None):
+            # the step-(n-1) state of the nominal parameters, then the perturbed ones
+            S.checkpoint_load(ck)
             S.set_load(scale * cfg["f_ext"])
             if mu is not None:
                 S.set_material(mu, cfg["lam"])
-            S.checkpoint_load(ck)
             S.newton_solve_to(lf, cfd)
             _, st = S.newton_solve_to(lf, cpol, check=False)
             fd_rel.append(st["rel_res"])
