@@ -67,7 +67,7 @@ struct ProfScope {
     int k;
     hipEvent_t a = nullptr, b = nullptr;
     ProfScope(fm_ctx* c, int kind) : ctx(c), k(kind) {
-        if (!ctx->prof) return;
+        if (!ctx->prof || kind < 0) return;  // kind < 0: never timed (creates no events)
         if (ctx->ev_pool.size() < 2) {
             if (ctx->ev_pending.size() >= 1024) prof_flush(ctx);
             if (ctx->ev_pool.size() < 2) {
@@ -895,6 +895,12 @@ int fm_create(int device, fm_ctx** out) {
     }
     int zero = 0;
     (void)hipMemcpy(ctx->d_err.p, &zero, sizeof(int), hipMemcpyHostToDevice);
+    // the allocation-free residual's scratch address on this device
+    const hipError_t se = noalloc_scratch_init(device);
+    if (se != hipSuccess) {
+        fm_destroy(ctx);
+        return FM_EHIP;
+    }
     *out = ctx;
     return FM_OK;
 }
@@ -1133,6 +1139,41 @@ int fm_residual(fm_ctx* ctx, const double* u, double load, double* R_out, double
     FM_TRY(residual(ctx, du, load, &nr));
     if (R_out) FM_HIP(hipMemcpy(R_out, ctx->R.p, n * 8, hipMemcpyDeviceToHost));
     if (norm_out) *norm_out = nr;
+    return FM_OK;
+}
+
+int fm_residual_noalloc(fm_ctx* ctx, const double* u, double load, double* R_out, double* norm_out) {
+    if (!ctx) return FM_EINVAL;
+    const uint64_t n = 3ull * ctx->nn;
+    if (n > 256ull * kNoAllocPartials)
+        return fail(ctx, FM_EINVAL,
+                    "fm_residual_noalloc: " + std::to_string(n) + " DOFs exceed its scratch limit of " +
+                        std::to_string(256ull * kNoAllocPartials) + " (use fm_residual)");
+    // only buffers the set-up already made: fm_residual or fm_assemble makes them
+    if (!ctx->nn || !ctx->have_pattern || !ctx->have_colors || !ctx->have_material || ctx->f.n != n ||
+        ctx->R.n != n || ctx->hv.n != n)
+        return fail(ctx, FM_ESTATE, "fm_residual_noalloc: the context is not set up (call fm_residual once first)");
+    const NoAllocScratch sc = noalloc_scratch(ctx->device);
+    if (!sc.partials) return fail(ctx, FM_EHIP, "fm_residual_noalloc: no scratch address for this device");
+    FM_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const double* du = nullptr;
+    if (u) {
+        FM_HIP(hipMemcpyAsync(ctx->hv.p, u, n * 8, hipMemcpyHostToDevice, s));
+        du = ctx->hv.p;
+    }
+    // the reduce state starts from zero on every call (the array keeps the last call's contents)
+    launch_fill_u8(s, reinterpret_cast<uint8_t*>(sc.state), sizeof(PcgState), 0);
+    FM_TRY(assemble(ctx, du, false, PREC64, false, nullptr, -1));
+    const uint32_t np = launch_residual_finish(s, (uint32_t)n, load, ctx->have_fext ? ctx->f_ext.p : nullptr,
+                                               ctx->f.p, ctx->dir_dofs.empty() ? nullptr : ctx->isdir.p, ctx->R.p,
+                                               sc.partials);
+    launch_reduce(s, sc.partials, np, sc.state, 0, 0);
+    FM_HIP(hipMemcpyAsync(ctx->h_st, sc.state, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    FM_TRY(check_launch(ctx, "residual_noalloc"));
+    FM_HIP(hipStreamSynchronize(s));
+    if (norm_out) *norm_out = std::sqrt(ctx->h_st->slot[0]);
+    if (R_out) FM_HIP(hipMemcpy(R_out, ctx->R.p, n * 8, hipMemcpyDeviceToHost));
     return FM_OK;
 }
 

@@ -228,6 +228,19 @@ struct ElemArgs {
     const double* u;       // 3nn or nullptr
 };
 
+// scratch of fm_residual_noalloc (a static device array, fm_kernels.hip):
+// kNoAllocPartials block partials (one per 256 rows, so at most
+// 256 * kNoAllocPartials rows) and one PcgState
+constexpr uint32_t kNoAllocPartials = 32768;
+struct NoAllocScratch {
+    double* partials;
+    PcgState* state;
+};
+// looks the array's address up for `device`, which must be current; once per device
+hipError_t noalloc_scratch_init(int device);
+// the address looked up for `device` (null members before noalloc_scratch_init)
+NoAllocScratch noalloc_scratch(int device);
+
 // error of the last failed persistent-kernel launch (cleared when read)
 hipError_t take_launch_error();
 void record_launch_error(hipError_t e);

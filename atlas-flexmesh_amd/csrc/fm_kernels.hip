@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 #include <string>
 
 namespace fm {
@@ -2042,6 +2043,45 @@ uint32_t factor_waves() {
 // ===========================================================================
 // launchers
 // ===========================================================================
+// ---- scratch of the allocation-free residual (fm_residual_noalloc) ------------
+// Block partials of R.R and the reduce state, in one file-scope device array:
+// each GPU holds its own copy for the life of the process, shared by every
+// call on that GPU.  Host code uses the address hipGetSymbolAddress returns
+// for the current device, never the array's name.
+constexpr size_t kNoAllocStateWords = (sizeof(PcgState) + 255) / 256 * 32;
+static __device__ __align__(256) double scratch[kNoAllocPartials + kNoAllocStateWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return hipSuccess;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = static_cast<double*>(p);
+    return hipSuccess;
+}
+
+NoAllocScratch noalloc_scratch(int device) {
+    NoAllocScratch s{};
+    if (device < 0 || device >= kMaxDevices) return s;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) return s;
+    s.partials = g_scratch[device];
+    s.state = reinterpret_cast<PcgState*>(g_scratch[device] + kNoAllocPartials);
+    return s;
+}
+
 hipError_t take_launch_error() {
     const hipError_t e = g_launch_err;
     g_launch_err = hipSuccess;

@@ -83,6 +83,7 @@ def lib():
     L.fm_assemble.argtypes = [_vp, _vp, _vp, _vp, C.c_int]
     L.fm_residual.argtypes = [_vp, _vp, C.c_double, _vp, C.POINTER(C.c_double)]
     L.fm_assemble_residual_only.argtypes = [_vp, _vp, _f64p]
+    L.fm_residual_noalloc.argtypes = [_vp, _vp, C.c_double, _vp, C.POINTER(C.c_double)]
     L.fm_set_matrix.argtypes = [_vp, C.c_uint32, _u32p, _u32p, _f64p]
     L.fm_use_tangent.argtypes = [_vp, C.c_int]
     L.fm_spmv.argtypes = [_vp, C.c_int, _f64p, _f64p]
@@ -274,6 +275,15 @@ class Solver:
         R = np.zeros(3 * self.nn)
         nr = C.c_double(0.0)
         self._chk(self.L.fm_residual(self.h, _ptr(uu), load, R.ctypes.data, C.byref(nr)), "fm_residual")
+        return R, nr.value
+
+    def residual_noalloc(self, u=None, load=1.0):
+        """residual() through the allocation-free entry point (static scratch)."""
+        uu = self._nodal(u)
+        R = np.zeros(3 * self.nn)
+        nr = C.c_double(0.0)
+        self._chk(self.L.fm_residual_noalloc(self.h, _ptr(uu), load, R.ctypes.data, C.byref(nr)),
+                  "fm_residual_noalloc")
         return R, nr.value
 
     # ---- sparse linear algebra -------------------------------------------------

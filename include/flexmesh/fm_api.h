@@ -101,6 +101,17 @@ int fm_assemble_residual_only(fm_ctx* ctx, const double* u, double* f_int_out);
    norm_out (||R||_2) may be NULL. */
 int fm_residual(fm_ctx* ctx, const double* u, double load, double* R_out, double* norm_out);
 
+/* No counterpart in the reference.  fm_residual without any allocation, for
+   callers that cannot allocate (code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc): the same kernels and results as
+   fm_residual, with the reduction's scratch (block partials of R.R and the
+   reduce state, 256 KiB) in a static device array that each GPU holds until
+   the process ends.  Limit: n_dofs <= 8,388,608 (FM_EINVAL beyond).  The
+   context must be set up (FM_ESTATE otherwise: call fm_residual once first).
+   Every call on one GPU shares that GPU's array: calls that overlap on one GPU,
+   from any contexts or threads, race. */
+int fm_residual_noalloc(fm_ctx* ctx, const double* u, double load, double* R_out, double* norm_out);
+
 /* ---- sparse linear algebra -----------------------------------------------------
  * A general CSR matrix (sorted or unsorted columns) for the solver entries
  * below; replaces SparseCSR (FEM/fem_types.hpp:304-320).  Rows whose groups of
