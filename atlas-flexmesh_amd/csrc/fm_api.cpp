@@ -895,6 +895,12 @@ int fm_create(int device, fm_ctx** out) {
     }
     int zero = 0;
     (void)hipMemcpy(ctx->d_err.p, &zero, sizeof(int), hipMemcpyHostToDevice);
+    // the PCG state starts zeroed
+    launch_fill_u8(ctx->stream, reinterpret_cast<uint8_t*>(ctx->st.p), sizeof(PcgState), 0);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        fm_destroy(ctx);
+        return FM_EHIP;
+    }
     // the allocation-free residual's scratch address on this device
     const hipError_t se = noalloc_scratch_init(device);
     if (se != hipSuccess) {

@@ -410,8 +410,13 @@ __device__ __forceinline__ void process(const SweepArgs<T>& A, const Near<T>& N,
 #endif
 
 // Persistent: block b of the grid takes bundles b, b + P, ... in processing
-// order, which is a topological order of the bundle graph, so the earliest
-// unfinished bundle only ever waits on finished ones.
+// order, which is a topological order of the bundle graph, so a block only
+// ever waits on bundles of lower index.  The grid is one block per bundle
+// whenever the device holds that many at once (every chain sweep at the box
+// configurations): progress then needs only that blocks start in index order.
+// (Bundles taken from an atomic ticket counter instead, which needs neither,
+// cost ~100 us per sweep at 1,089 bundles: three serialised atomics per block
+// on one address, and bundles placed on CUs at random.)
 template <typename T, int D, bool BWD>
 __global__ __launch_bounds__(64, FM_SWEEP_MINW) void chain_sweep_kernel(SweepArgs<T> A) {
     if (A.check_done && A.st->done) return;
@@ -419,8 +424,8 @@ __global__ __launch_bounds__(64, FM_SWEEP_MINW) void chain_sweep_kernel(SweepArg
     __shared__ T ring[(kZeroSlot + 1) * kRingStride + 64];
     const int lane = threadIdx.x;
     if (lane < kRingStride) ring[kZeroSlot * kRingStride + lane] = (T)0;
-    double dot = 0.0;
     for (uint32_t b = blockIdx.x; b < A.nb; b += gridDim.x) {
+        double dot = 0.0;  // r.y of this bundle: one partial per bundle, a fixed order whichever block runs it
         // wave-uniform: the descriptors built from them must live in scalar registers
         const uint32_t q0 = __builtin_amdgcn_readfirstlane(A.bstart[b]);
         const uint32_t ns = __builtin_amdgcn_readfirstlane(A.bsteps[b]), sl = ns - 1;
@@ -467,11 +472,11 @@ __global__ __launch_bounds__(64, FM_SWEEP_MINW) void chain_sweep_kernel(SweepArg
                 far_load(R, min(s + 2 * D, sl), lane, F[u]);
             }
         }
-    }
-    if (BWD && A.partials) {
+        if (BWD && A.partials) {
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o);
-        if (lane == 0) A.partials[blockIdx.x] = dot;
+            for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o);
+            if (lane == 0) A.partials[b] = dot;
+        }
     }
 }
 
@@ -566,7 +571,7 @@ uint32_t launch_sweep(hipStream_t s, const Matrix& A, const ChainSweep& W, const
     const uint32_t grid = persistent_grid(reinterpret_cast<const void*>(kern), W.nb);
     kern<<<grid, 64, 0, s>>>(a);
     record_launch_error(hipGetLastError());
-    return grid;
+    return W.nb;  // partials: one per bundle
 }
 
 template <typename T>
