@@ -351,10 +351,10 @@ int assemble(fm_ctx* ctx, const double* u, bool with_K, int prec, bool mask, con
         // node-gather tangent assembly: every entry of f, lift and K is written once
         void* K;
         if (prec == PREC64) {
-            FM_HIP(M.val64.alloc(M.nnz));
+            FM_HIP(M.val64.alloc(M.nnz + kValPad));
             K = M.val64.p;
         } else {
-            FM_HIP(M.val32.alloc(M.nnz));
+            FM_HIP(M.val32.alloc(M.nnz + kValPad));
             K = M.val32.p;
         }
         {
@@ -376,11 +376,11 @@ int assemble(fm_ctx* ctx, const double* u, bool with_K, int prec, bool mask, con
     void* K = nullptr;
     if (with_K) {
         if (prec == PREC64) {
-            FM_HIP(M.val64.alloc(M.nnz));
+            FM_HIP(M.val64.alloc(M.nnz + kValPad));
             launch_fill_f64(s, M.val64.p, M.nnz, 0.0);
             K = M.val64.p;
         } else {
-            FM_HIP(M.val32.alloc(M.nnz));
+            FM_HIP(M.val32.alloc(M.nnz + kValPad));
             launch_fill_f32(s, M.val32.p, M.nnz, 0.f);
             K = M.val32.p;
         }
@@ -685,7 +685,7 @@ int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult*
 
 // FP32 values derived on the device from the FP64 ones
 int values_to_fp32(fm_ctx* ctx, Matrix& M) {
-    FM_HIP(M.val32.alloc(M.nnz));
+    FM_HIP(M.val32.alloc(M.nnz + kValPad));
     launch_f64_to_f32(ctx->stream, (uint32_t)M.nnz, M.val64.p, 1.0, M.val32.p);
     FM_TRY(check_launch(ctx, "f64_to_f32"));
     M.val_valid[PREC32] = true;
@@ -1242,8 +1242,8 @@ int fm_set_matrix(fm_ctx* ctx, uint32_t n, const uint32_t* row_ptr, const uint32
     FM_TRY(node_cols_setup(ctx, M, M.h_row_ptr, M.h_col_idx));
     FM_HIP(M.row_ptr.alloc(n + 1));
     FM_HIP(M.col_idx.alloc(nnz));
-    FM_HIP(M.val64.alloc(nnz));
-    FM_HIP(M.val32.alloc(nnz));
+    FM_HIP(M.val64.alloc(nnz + kValPad));
+    FM_HIP(M.val32.alloc(nnz + kValPad));
     FM_HIP(hipMemcpy(M.row_ptr.p, M.h_row_ptr.data(), (n + 1) * 4ull, hipMemcpyHostToDevice));
     FM_HIP(hipMemcpy(M.col_idx.p, M.h_col_idx.data(), nnz * 4, hipMemcpyHostToDevice));
     FM_HIP(hipMemcpy(M.val64.p, sv.data(), nnz * 8, hipMemcpyHostToDevice));

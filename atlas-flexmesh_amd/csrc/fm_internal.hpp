@@ -118,6 +118,10 @@ struct Matrix {
     bool ready = false;
 };
 
+// matrix value arrays hold this many elements past nnz: the LDS-staged SpMV
+// reads whole 16-byte chunks, the last of which may end past the last value
+constexpr size_t kValPad = 4;
+
 enum { PREC64 = 0, PREC32 = 1, PRECPC = 2 };  // PRECPC: FP64 Krylov, FP32 ILU(0) preconditioner
 
 struct Ctx {

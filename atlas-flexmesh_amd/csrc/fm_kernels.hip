@@ -735,6 +735,140 @@ __global__ __launch_bounds__(256) void spmv_node_kernel(uint32_t nn, const uint3
     }
 }
 
+// LDS-staged node SpMV.  A wave takes 4 G consecutive nodes, whose values
+// are one contiguous run of the value array (node v's three rows start at
+// row_ptr[0] + 9 nptr[v], node after node).  The wave streams the run into
+// LDS with 16-byte loads of consecutive lanes, so every load instruction
+// covers 1 KB of consecutive addresses and every cache line is read whole,
+// once; lanes past the run reload its last chunk (the same line, no extra
+// traffic).  The 12-byte row segments of the direct kernel touch several
+// partial lines per instruction, and the L1 spends its cycles on pending
+// misses.  Each 16-lane group then takes its node's blocks from LDS with the
+// arithmetic of the direct kernel (the same bits).  A run longer than the LDS
+// slice (more than ~17 blocks per node on average; a Kuhn box has at most 15)
+// reads its values directly.  Five 16-byte rounds per lane leave 20 KB of LDS
+// per block, so eight waves per SIMD stay resident: 285 us per FP32 product
+// at 128^3 hexes against 295 us with six rounds and 301 us unstaged.
+#ifndef FM_SPMV_LDS_ROUNDS
+#define FM_SPMV_LDS_ROUNDS 5
+#endif
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Vec16<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+
+template <typename T, bool DOT>
+__global__ __launch_bounds__(256) void spmv_node_lds_kernel(uint32_t nn, const uint32_t* __restrict__ rp,
+                                                            const uint32_t* __restrict__ nptr,
+                                                            const uint32_t* __restrict__ ncol,
+                                                            const T* __restrict__ val, uint32_t nval,
+                                                            const T* __restrict__ x, T* __restrict__ y,
+                                                            double* __restrict__ partials, T* __restrict__ sent0,
+                                                            T* __restrict__ sent1, const PcgState* __restrict__ st) {
+    if (st && st->done) return;
+    using V = typename Vec16<T>::type;
+    constexpr int G = nodes_per_group<T>(), NPW = 4 * G, W = 16 / (int)sizeof(T);
+    constexpr int ROUNDS = FM_SPMV_LDS_ROUNDS, CAP = ROUNDS * 64 * W;
+    __shared__ V sv[4][ROUNDS * 64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, k = lane & 15;
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t v0 = (blk * 4 + wave) * NPW;  // first node of the wave
+    const uint32_t rbase = rp[0];
+    // the wave's value run [r0, r1)
+    const uint32_t pl = nptr[min(v0 + min((uint32_t)lane, (uint32_t)NPW), nn)];
+    const uint32_t r0 = rbase + 9 * __shfl(pl, 0), r1 = rbase + 9 * __shfl(pl, NPW);
+    const uint32_t base = r0 & ~(uint32_t)(W - 1);
+    const bool staged = r1 > r0 && r1 - base <= (uint32_t)CAP;  // wave-uniform
+    if (staged) {
+        const uint32_t last = (r1 - 1) & ~(uint32_t)(W - 1);  // chunk holding the run's last value
+        V t[ROUNDS];
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) {
+            const uint32_t c = min(base + (uint32_t)(r * 64 + lane) * W, last);
+            t[r] = __builtin_nontemporal_load(reinterpret_cast<const V*>(val + c));
+        }
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) sv[wave][r * 64 + lane] = t[r];
+    }
+    (void)nval;
+    __syncthreads();
+    const T* lv = reinterpret_cast<const T*>(&sv[wave][0]);
+    auto av = [&](uint32_t off) { return staged ? lv[off - base] : val[off]; };
+    // group q takes nodes v0 + 4 i + q
+    uint32_t node[G], b0[G], nb[G];
+    T xown[G];
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+        node[i] = v0 + 4 * i + q;
+        const uint32_t v = min(node[i], nn - 1);
+        b0[i] = nptr[v];
+        nb[i] = nptr[v + 1];
+        if (DOT) xown[i] = x[3 * v + min(k, 2)];
+    }
+#pragma unroll
+    for (int i = 0; i < G; ++i) nb[i] = node[i] < nn ? nb[i] - b0[i] : 0;
+    uint32_t kk[G], cn[G];
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+        kk[i] = min((uint32_t)k, max(nb[i], 1u) - 1);
+        cn[i] = spmv_stream_load(ncol + b0[i] + kk[i]);
+    }
+    T s[G][3];
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+        const uint32_t o0 = rbase + 9 * b0[i] + 3 * kk[i], o1 = o0 + 3 * nb[i], o2 = o1 + 3 * nb[i];
+        T a[9];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            a[c] = av(o0 + c);
+            a[3 + c] = av(o1 + c);
+            a[6 + c] = av(o2 + c);
+        }
+        const T* xm = x + 3 * (size_t)cn[i];
+        const T w = (uint32_t)k < nb[i] ? (T)1 : (T)0;
+        const T x0 = xm[0], x1 = xm[1], x2 = xm[2];
+        s[i][0] = (a[0] * x0 + a[1] * x1 + a[2] * x2) * w;
+        s[i][1] = (a[3] * x0 + a[4] * x1 + a[5] * x2) * w;
+        s[i][2] = (a[6] * x0 + a[7] * x1 + a[8] * x2) * w;
+    }
+    // nodes with more than 16 blocks
+#pragma unroll
+    for (int i = 0; i < G; ++i)
+        for (uint32_t kb = k + kNodeLanes; kb < nb[i]; kb += kNodeLanes) {
+            const uint32_t o0 = rbase + 9 * b0[i] + 3 * kb, o1 = o0 + 3 * nb[i], o2 = o1 + 3 * nb[i];
+            const T* xm = x + 3 * (size_t)ncol[b0[i] + kb];
+            const T x0 = xm[0], x1 = xm[1], x2 = xm[2];
+            s[i][0] += av(o0) * x0 + av(o0 + 1) * x1 + av(o0 + 2) * x2;
+            s[i][1] += av(o1) * x0 + av(o1 + 1) * x1 + av(o1 + 2) * x2;
+            s[i][2] += av(o2) * x0 + av(o2 + 1) * x1 + av(o2 + 2) * x2;
+        }
+    double part = 0.0;
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+        const T t0 = row16_sum(s[i][0]), t1 = row16_sum(s[i][1]), t2 = row16_sum(s[i][2]);
+        if (node[i] < nn && k < 3) {
+            const uint32_t row = 3 * node[i] + k;
+            const T sum = k == 0 ? t0 : (k == 1 ? t1 : t2);
+            y[row] = sum;
+            if (sent0) {
+                sent0[row] = sentinel_value<T>();
+                sent1[row] = sentinel_value<T>();
+            }
+            if (DOT) part += (double)sum * (double)xown[i];
+        }
+    }
+    if (DOT) {
+        const double bs = block_sum(part);
+        if (threadIdx.x == 0) partials[blk] = bs;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // natural-order ILU(0), node-blocked (B rows per node) and sync-free:
 // nodes are taken from a level-ordered queue through a ticket counter, so a
@@ -2198,6 +2332,15 @@ static bool spmv_use_stream(const Matrix& A) {
     return A.n && (double)A.nnz / A.n <= 48.0;
 }
 
+// FM_SPMV_LDS=0 keeps the direct node kernel (default: LDS-staged)
+static bool spmv_use_lds() {
+    static const bool on = [] {
+        const char* e = std::getenv("FM_SPMV_LDS");
+        return !(e && e[0] == '0' && e[1] == 0);
+    }();
+    return on;
+}
+
 static bool spmv_use_node(const Matrix& A) {
     static const bool csr_only = [] {
         const char* e = std::getenv("FM_SPMV");
@@ -2217,6 +2360,16 @@ static void spmv_t(hipStream_t s, const Matrix& A, const T* val, const T* x, T* 
                    T* s0, T* s1, const PcgState* st) {
     if (spmv_use_node(A)) {
         const uint32_t nn = A.n / 3, nb = cdiv(nn, 256 / kNodeLanes * nodes_per_group<T>());
+        if (spmv_use_lds()) {
+            const uint32_t nval = (uint32_t)A.nnz;
+            if (dot)
+                spmv_node_lds_kernel<T, true><<<nb, 256, 0, s>>>(nn, A.row_ptr.p, A.nptr.p, A.ncol.p, val, nval, x, y,
+                                                                 partials, s0, s1, st);
+            else
+                spmv_node_lds_kernel<T, false><<<nb, 256, 0, s>>>(nn, A.row_ptr.p, A.nptr.p, A.ncol.p, val, nval, x, y,
+                                                                  partials, s0, s1, st);
+            return;
+        }
         if (dot)
             spmv_node_kernel<T, true><<<nb, 256, 0, s>>>(nn, A.row_ptr.p, A.nptr.p, A.ncol.p, val, x, y, partials, s0,
                                                          s1, st);
