@@ -9,6 +9,7 @@
 #include "fem/constitutive_models.hpp"
 #include "fem/stress_extraction.hpp"
 
+#include <chrono>
 #include <cstring>
 
 using namespace atlas::fem;
@@ -177,6 +178,46 @@ int ref_checkpoint_save(const char* path, uint32_t nn, const double* X, const do
     }
     std::vector<double> uu(u, u + 3 * (size_t)nn);
     return CheckpointFile::save(path, m, uu, load_step, adapt_iter) ? 0 : 1;
+}
+
+// Wall seconds (best of reps) of assemble_global_threadlocal
+// (FEM/adaptive_fem_engine.hpp:1126-1233), or of assemble_global_residual_only
+// (:990-1059) when residual_only, on the mesh with one (E, nu); the pattern
+// comes from build_csr_sparsity_pattern_rigorous (:1069-1120), built once.
+double ref_time_assemble(uint32_t nn, const double* X, uint32_t ne, const uint32_t* conn, double E, double nu,
+                         const double* u, int reps, int residual_only) {
+    MeshTopology m = mesh_from(nn, X, ne, conn);
+    ProblemDescriptor prob;
+    prob.E = E;
+    prob.nu = nu;
+    SparseCSR K;
+    build_csr_sparsity_pattern_rigorous(m, K);
+    std::vector<double> f;
+    double best = 1e300;
+    for (int r = 0; r < reps; ++r) {
+        const auto t0 = std::chrono::steady_clock::now();
+        if (residual_only)
+            assemble_global_residual_only(m, prob, u, f);
+        else
+            assemble_global_threadlocal(m, prob, u, K, f);
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        best = dt < best ? dt : best;
+    }
+    return best;
+}
+
+// Wall seconds (best of reps) of SparseCSR::matvec (FEM/fem_types.hpp:310-319)
+double ref_time_spmv(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const double* val,
+                     const double* x, double* y, int reps) {
+    SparseCSR A = csr_from(n, row_ptr, col_idx, val);
+    double best = 1e300;
+    for (int r = 0; r < reps; ++r) {
+        const auto t0 = std::chrono::steady_clock::now();
+        A.matvec(x, y);
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        best = dt < best ? dt : best;
+    }
+    return best;
 }
 
 // exponential_map / matrix_logarithm (CORE/lie_operator.cpp:94-275), column-major 3x3

@@ -191,6 +191,41 @@ def test_element_tangent_is_force_derivative(O):
         assert np.abs(f.reshape(4, 3).sum(axis=0)).max() < 1e-14 * max(1, np.abs(f).max())
 
 
+def test_element_tangent_at_the_j_clamp_is_force_derivative(O):
+    """J < 0.01 (compressed and inverted): ln J is clamped at ln 0.01
+    (FEM/constitutive_models.hpp:96), so the force has no lambda ln J
+    derivative and the tangent drops the lam' term.  Central FD of f pins it:
+    a tangent that kept lam' = lam would differ by lam V (F^-T g)(F^-T g)^T,
+    ~1e4 times the tolerance at det F = 0.005."""
+    rng = np.random.default_rng(12)
+    for target in (0.005, -0.004, 0.0095):
+        Xe = _rand_tet(rng)
+        # move node 3 toward the plane of nodes 0..2: det F = target
+        n = np.cross(Xe[1] - Xe[0], Xe[2] - Xe[0])
+        n /= np.linalg.norm(n)
+        d = np.dot(Xe[3] - Xe[0], n)
+        ue = np.zeros(12)
+        ue[9:12] = -(1.0 - target) * d * n
+        ue += rng.uniform(-1e-4, 1e-4, 12) * (np.arange(12) < 9)
+        J0 = np.stack([Xe[1] - Xe[0], Xe[2] - Xe[0], Xe[3] - Xe[0]], axis=1)
+        x = Xe + ue.reshape(4, 3)
+        J1 = np.stack([x[1] - x[0], x[2] - x[0], x[3] - x[0]], axis=1)
+        detF = np.linalg.det(J1) / np.linalg.det(J0)
+        assert abs(detF) < 0.01 and abs(abs(detF) - abs(target)) < 1e-3
+        mu, lam = 0.7, 34.0
+        f, K, skipped = O.element(Xe, ue, mu, lam)
+        assert skipped == 0
+        h = 1e-7
+        Kfd = np.zeros((12, 12))
+        for j in range(12):
+            up, um = ue.copy(), ue.copy()
+            up[j] += h
+            um[j] -= h
+            Kfd[:, j] = (O.element(Xe, up, mu, lam)[0] - O.element(Xe, um, mu, lam)[0]) / (2 * h)
+        assert np.abs(K - Kfd).max() < 1e-6 * np.abs(K).max(), (target, np.abs(K - Kfd).max() / np.abs(K).max())
+        assert np.abs(K - K.T).max() < 1e-13 * np.abs(K).max()
+
+
 def test_element_linear_limit_is_BtDB(O):
     rng = np.random.default_rng(3)
     Xe = _rand_tet(rng)
