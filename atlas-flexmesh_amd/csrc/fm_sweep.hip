@@ -215,17 +215,27 @@ struct Near {
     T vec1[3];   // backward: of in1
 };
 
-// external dependency values of a step, polled kPollAhead steps early (may
+// external dependency values of a step, polled poll_ahead<T>() steps early (may
 // still be the sentinel)
 template <typename T>
 struct Ext {
     T v[kBPL][3];
 };
 // The payload is prefetched D steps ahead (enough loads in flight to cover
-// the memory latency), the external dependencies are polled only kPollAhead
-// steps ahead: a poll issued earlier finds the producer's value missing more
-// often, and each miss costs a re-poll round trip.
-constexpr int kPollAhead = 2;
+// the memory latency), the external dependencies are polled one step ahead:
+// a poll issued earlier finds the producer's value missing more often, and
+// each miss costs a re-poll round trip (at the 128^3-hex box, FP32: 350/380 us
+// forward/backward polling one step ahead, 416/425 us polling three ahead).
+#ifndef FM_CHAIN_POLL32
+#define FM_CHAIN_POLL32 1
+#endif
+#ifndef FM_CHAIN_POLL64
+#define FM_CHAIN_POLL64 1
+#endif
+template <typename T>
+constexpr int poll_ahead() {
+    return sizeof(T) == 4 ? FM_CHAIN_POLL32 : FM_CHAIN_POLL64;
+}
 
 // descriptors of one bundle's step records and payload
 struct BundleRsrc {
@@ -330,9 +340,17 @@ __device__ __forceinline__ void await_late(const SweepArgs<T>& A, const Far& f, 
 
 // One step of one quad: its node's row block against the dependency values,
 // the 3x3 diagonal-block solve, publication of the three results.
+// the publication of one step: lanes 0..2 of a quad store component gl of
+// its node (an idle quad's offset is past the buffer)
+template <typename T>
+struct Pub {
+    uint32_t off;
+    T v;
+};
+
 template <typename T, bool BWD>
-__device__ __forceinline__ void process(const SweepArgs<T>& A, const Near<T>& N, const Ext<T>& X, uint32_t s,
-                                        int lane, T* ring, double& dot) {
+__device__ __forceinline__ Pub<T> process(const SweepArgs<T>& A, const Near<T>& N, const Ext<T>& X, uint32_t s,
+                                          int lane, T* ring, double& dot) {
     using S = Shape<T>;
     const int g = lane / kLPN, gl = lane & (kLPN - 1);
     T e[kBPL][3];
@@ -396,11 +414,11 @@ __device__ __forceinline__ void process(const SweepArgs<T>& A, const Near<T>& N,
     // buffer); lane 3 stores into its spare LDS slot
     const bool wr = gl < 3;
     const T vd = gl == 0 ? v0 : (gl == 1 ? v1 : v2);
-    publish_at(make_rsrc(A.out, A.out_bytes), wr ? scaled<T>(N.f.pub) + gl * (uint32_t)sizeof(T) : kOob, vd);
     ring[wr ? ((s % kChainRing) * kNPW + g) * kRingStride + gl : (kZeroSlot + 1) * kRingStride + lane] = vd;
     // r.y partials (idle quads contribute zeros)
     if (BWD && gl == 1)
         dot += (double)N.vec1[0] * (double)v0 + (double)N.vec1[1] * (double)v1 + (double)N.vec1[2] * (double)v2;
+    return Pub<T>{wr ? scaled<T>(N.f.pub) + gl * (uint32_t)sizeof(T) : kOob, vd};
 }
 
 // minimum waves per SIMD asked of the register allocator (1: no limit; the
@@ -431,7 +449,7 @@ __global__ __launch_bounds__(64, FM_SWEEP_MINW) void chain_sweep_kernel(SweepArg
         const uint32_t ns = __builtin_amdgcn_readfirstlane(A.bsteps[b]), sl = ns - 1;
         const BundleRsrc R = bundle_rsrc(A, q0, ns);
         // steps past the end re-read the last step (never processed)
-        constexpr int DE = kPollAhead;
+        constexpr int DE = poll_ahead<T>();
         static_assert(DE <= D && (2 * D) % DE == 0, "poll distance must divide the unrolled block");
         Far F[2 * D];
         Near<T> N[D];
@@ -446,15 +464,19 @@ __global__ __launch_bounds__(64, FM_SWEEP_MINW) void chain_sweep_kernel(SweepArg
         // operations, so the waits the compiler derives inside the loop see
         // loads issued a full block earlier.  The first block is peeled (its
         // data was issued just before it), the partial last block follows
-        // the loop with guards.
+        // the loop with guards.  A step's value is stored after that step's
+        // loads are issued: the memory counter the waits use drains in issue
+        // order and counts stores too, so a store issued before a poll would
+        // hold that poll's wait until the store is acknowledged.
         auto block = [&](uint32_t s0) {
 #pragma unroll
             for (int u = 0; u < 2 * D; ++u) {
                 const uint32_t s = s0 + u;
-                process<T, BWD>(A, N[u % D], X[u % DE], s, lane, ring, dot);
+                const Pub<T> pb = process<T, BWD>(A, N[u % D], X[u % DE], s, lane, ring, dot);
                 poll_ext(A, F[(u + DE) % (2 * D)], X[u % DE]);
                 near_load<T, BWD>(A, R, min(s + D, sl), lane, F[(u + D) % (2 * D)], N[u % D]);
                 far_load(R, min(s + 2 * D, sl), lane, F[u]);
+                publish_at(make_rsrc(A.out, A.out_bytes), pb.off, pb.v);
             }
         };
         uint32_t s0 = 0;
@@ -466,10 +488,11 @@ __global__ __launch_bounds__(64, FM_SWEEP_MINW) void chain_sweep_kernel(SweepArg
         for (int u = 0; u < 2 * D; ++u) {
             const uint32_t s = s0 + u;
             if (s < ns) {
-                process<T, BWD>(A, N[u % D], X[u % DE], s, lane, ring, dot);
+                const Pub<T> pb = process<T, BWD>(A, N[u % D], X[u % DE], s, lane, ring, dot);
                 poll_ext(A, F[(u + DE) % (2 * D)], X[u % DE]);
                 near_load<T, BWD>(A, R, min(s + D, sl), lane, F[(u + D) % (2 * D)], N[u % D]);
                 far_load(R, min(s + 2 * D, sl), lane, F[u]);
+                publish_at(make_rsrc(A.out, A.out_bytes), pb.off, pb.v);
             }
         }
         if (BWD && A.partials) {
