@@ -410,13 +410,19 @@ __global__ void set_dir_diag_kernel(uint32_t n_dir, const uint32_t* __restrict__
     K[row_ptr[r] + 3u * nlow[node] + d] = (T)1;
 }
 
+// Dirichlet rows and columns of an assembled matrix: kEliminateLanes
+// consecutive lanes walk one row, so a wave reads whole runs of col_idx
+// (one lane per row read each row's own cache lines: ~16x the bytes)
+constexpr int kEliminateLanes = 32;
 template <typename T>
-__global__ void eliminate_kernel(uint32_t n, const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col_idx,
-                                 const uint8_t* __restrict__ isdir, T* __restrict__ K) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void eliminate_kernel(uint32_t n, const uint32_t* __restrict__ row_ptr,
+                                                        const uint32_t* __restrict__ col_idx,
+                                                        const uint8_t* __restrict__ isdir, T* __restrict__ K) {
+    const uint32_t r = blockIdx.x * (256 / kEliminateLanes) + threadIdx.x / kEliminateLanes;
     if (r >= n) return;
     const bool dr = isdir[r];
-    for (uint32_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k) {
+    const uint32_t e = row_ptr[r + 1];
+    for (uint32_t k = row_ptr[r] + threadIdx.x % kEliminateLanes; k < e; k += kEliminateLanes) {
         const uint32_t c = col_idx[k];
         if (dr || isdir[c]) K[k] = (c == r) ? (T)1 : (T)0;
     }
@@ -2289,10 +2295,11 @@ void launch_set_dir_diag(hipStream_t s, int prec, uint32_t n_dir, const uint32_t
 
 void launch_eliminate(hipStream_t s, int prec, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
                       const uint8_t* isdir, void* K) {
+    const uint32_t nb = cdiv(n, 256 / kEliminateLanes);
     if (prec == PREC64)
-        eliminate_kernel<double><<<cdiv(n, 256), 256, 0, s>>>(n, row_ptr, col_idx, isdir, (double*)K);
+        eliminate_kernel<double><<<nb, 256, 0, s>>>(n, row_ptr, col_idx, isdir, (double*)K);
     else
-        eliminate_kernel<float><<<cdiv(n, 256), 256, 0, s>>>(n, row_ptr, col_idx, isdir, (float*)K);
+        eliminate_kernel<float><<<nb, 256, 0, s>>>(n, row_ptr, col_idx, isdir, (float*)K);
 }
 
 uint32_t launch_residual_finish(hipStream_t s, uint32_t n, double load, const double* f_ext, const double* f,

@@ -103,6 +103,33 @@ def test_assembly_is_bitwise_repeatable(fm, S):
     assert np.array_equal(f1, f2) and np.array_equal(K1, K2)
 
 
+def test_use_tangent_eliminates_constrained_rows_and_columns(fm, S, O):
+    """fm_use_tangent(eliminate=1) on the assembled tangent: zero rows and
+    columns with a unit diagonal at the constrained DOFs
+    (FEM/nonlinear_solver.hpp:388-396), in both precisions, seen through the
+    SpMV of the matrix it leaves current."""
+    cfg = small_problem(fm, 2, (5, 4, 6))
+    fm.configs.load(S, cfg)
+    nn = cfg["X"].size // 3
+    rng = np.random.default_rng(7)
+    u = rng.uniform(-0.02, 0.02, 3 * nn)
+    rp, ci = O.pattern(nn, cfg["conn"])
+    colors, nc = O.color(nn, cfg["conn"])
+    Ko, _ = O.assemble(cfg["X"], cfg["conn"], cfg["mu"], cfg["lam"], colors, nc, u, rp, ci)
+    isd = np.zeros(3 * nn, dtype=np.uint8)
+    isd[cfg["dir_dofs"]] = 1
+    O.eliminate(rp, ci, Ko, isd)
+    x = rng.standard_normal(3 * nn)
+    yo = O.spmv(rp, ci, Ko, x)
+    d = cfg["dir_dofs"]
+    for prec, tol, xd in ((fm.FM_FP64, 1e-12, x[d]), (fm.FM_MIXED, 2e-6, x[d].astype(np.float32).astype(np.float64))):
+        S.assemble(u, prec)
+        S.use_tangent(True)
+        y = S.spmv(x, prec)
+        assert rel(y, yo) < tol
+        assert np.array_equal(y[d], xd)  # unit diagonal, nothing else in the row
+
+
 # ---------------------------------------------------------------------------- SpMV / ILU(0) / PCG
 def poisson(g):
     sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
