@@ -40,6 +40,19 @@ __device__ __forceinline__ double block_sum(double v) {
         for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += s_part[i];
     return r;
 }
+// the same sum, with wave w's partial in the caller's LDS at part[w * stride]
+// (a slot the wave no longer reads), so the kernel needs no LDS beyond its own
+__device__ __forceinline__ double block_sum(double v, double* part, int stride) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) part[w * stride] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += part[i * stride];
+    return r;
+}
 
 // ---------------------------------------------------------------------------
 // element evaluation (FP64): geometry, kinematics, force and tangent factors
@@ -786,49 +799,64 @@ __global__ __launch_bounds__(256) void spmv_node_lds_kernel(uint32_t nn, const u
     const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t v0 = (blk * 4 + wave) * NPW;  // first node of the wave
     const uint32_t rbase = rp[0];
-    // the wave's value run [r0, r1)
-    const uint32_t pl = nptr[min(v0 + min((uint32_t)lane, (uint32_t)NPW), nn)];
-    const uint32_t r0 = rbase + 9 * __shfl(pl, 0), r1 = rbase + 9 * __shfl(pl, NPW);
-    const uint32_t base = r0 & ~(uint32_t)(W - 1);
-    const bool staged = r1 > r0 && r1 - base <= (uint32_t)CAP;  // wave-uniform
-    if (staged) {
-        const uint32_t last = (r1 - 1) & ~(uint32_t)(W - 1);  // chunk holding the run's last value
-        V t[ROUNDS];
-#pragma unroll
-        for (int r = 0; r < ROUNDS; ++r) {
-            const uint32_t c = min(base + (uint32_t)(r * 64 + lane) * W, last);
-            t[r] = __builtin_nontemporal_load(reinterpret_cast<const V*>(val + c));
-        }
-#pragma unroll
-        for (int r = 0; r < ROUNDS; ++r) sv[wave][r * 64 + lane] = t[r];
-    }
     (void)nval;
-    __syncthreads();
-    const T* lv = reinterpret_cast<const T*>(&sv[wave][0]);
-    auto av = [&](uint32_t off) { return staged ? lv[off - base] : val[off]; };
-    // group q takes nodes v0 + 4 i + q
-    uint32_t node[G], b0[G], nb[G];
+    // Three dependent rounds: (1) the block ranges of the wave's nodes, lane j
+    // holding nptr[v0 + j] (j <= NPW); (2) the column nodes, then the value
+    // run; (3) the x gathers, which wait only for the column nodes (loads
+    // complete in issue order) while the value run is still in flight.
+    const uint32_t pl = nptr[min(v0 + min((uint32_t)lane, (uint32_t)NPW), nn)];
+    uint32_t node[G];
     T xown[G];
 #pragma unroll
     for (int i = 0; i < G; ++i) {
         node[i] = v0 + 4 * i + q;
-        const uint32_t v = min(node[i], nn - 1);
-        b0[i] = nptr[v];
-        nb[i] = nptr[v + 1];
-        if (DOT) xown[i] = x[3 * v + min(k, 2)];
+        if (DOT) xown[i] = x[3 * min(node[i], nn - 1) + min(k, 2)];
     }
-#pragma unroll
-    for (int i = 0; i < G; ++i) nb[i] = node[i] < nn ? nb[i] - b0[i] : 0;
-    uint32_t kk[G], cn[G];
+    // the wave's value run [r0, r1), 16-byte chunks from base
+    const uint32_t r0 = rbase + 9 * (uint32_t)__builtin_amdgcn_readlane((int)pl, 0);
+    const uint32_t r1 = rbase + 9 * (uint32_t)__builtin_amdgcn_readlane((int)pl, NPW);
+    const uint32_t base = r0 & ~(uint32_t)(W - 1);
+    const bool staged = r1 - base <= (uint32_t)CAP;  // wave-uniform
+    const uint32_t last = r1 > base ? (r1 - 1) & ~(uint32_t)(W - 1) : base;  // chunk holding the run's last value
+    // group q takes nodes v0 + 4 i + q; a node past the end gets no blocks
+    // and reads block 0 of the column list and the run's first values
+    uint32_t b0[G], off[G], nb[G], cn[G];
 #pragma unroll
     for (int i = 0; i < G; ++i) {
-        kk[i] = min((uint32_t)k, max(nb[i], 1u) - 1);
-        cn[i] = spmv_stream_load(ncol + b0[i] + kk[i]);
+        const uint32_t bb = __shfl(pl, 4 * i + q), e = __shfl(pl, 4 * i + q + 1);
+        const bool ok = node[i] < nn;
+        nb[i] = ok ? e - bb : 0;
+        b0[i] = ok ? bb : 0;
+        const uint32_t kk = min((uint32_t)k, max(nb[i], 1u) - 1);
+        off[i] = ok ? rbase + 9 * bb + 3 * kk : base;
+        cn[i] = spmv_stream_load(ncol + b0[i] + kk);
     }
+    __builtin_amdgcn_sched_barrier(0);  // column nodes issue before the value run
+    V t[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const uint32_t c = min(base + (uint32_t)(r * 64 + lane) * W, last);
+        t[r] = __builtin_nontemporal_load(reinterpret_cast<const V*>(val + c));
+    }
+    __builtin_amdgcn_sched_barrier(0);  // x gathers issue after the value run
+    T xg[G][3];
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+        const T* xm = x + 3 * (size_t)cn[i];
+        xg[i][0] = xm[0];
+        xg[i][1] = xm[1];
+        xg[i][2] = xm[2];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) sv[wave][r * 64 + lane] = t[r];
+    __syncthreads();
+    const T* lv = reinterpret_cast<const T*>(&sv[wave][0]);
+    auto av = [&](uint32_t o) { return staged ? lv[o - base] : val[o]; };
     T s[G][3];
 #pragma unroll
     for (int i = 0; i < G; ++i) {
-        const uint32_t o0 = rbase + 9 * b0[i] + 3 * kk[i], o1 = o0 + 3 * nb[i], o2 = o1 + 3 * nb[i];
+        const uint32_t o0 = off[i], o1 = o0 + 3 * nb[i], o2 = o1 + 3 * nb[i];
         T a[9];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -836,9 +864,8 @@ __global__ __launch_bounds__(256) void spmv_node_lds_kernel(uint32_t nn, const u
             a[3 + c] = av(o1 + c);
             a[6 + c] = av(o2 + c);
         }
-        const T* xm = x + 3 * (size_t)cn[i];
         const T w = (uint32_t)k < nb[i] ? (T)1 : (T)0;
-        const T x0 = xm[0], x1 = xm[1], x2 = xm[2];
+        const T x0 = xg[i][0], x1 = xg[i][1], x2 = xg[i][2];
         s[i][0] = (a[0] * x0 + a[1] * x1 + a[2] * x2) * w;
         s[i][1] = (a[3] * x0 + a[4] * x1 + a[5] * x2) * w;
         s[i][2] = (a[6] * x0 + a[7] * x1 + a[8] * x2) * w;
@@ -870,7 +897,9 @@ __global__ __launch_bounds__(256) void spmv_node_lds_kernel(uint32_t nn, const u
         }
     }
     if (DOT) {
-        const double bs = block_sum(part);
+        // each wave's partial goes to the start of its own slice, which it has
+        // finished reading: 20 KB of LDS per block keeps 8 blocks per CU
+        const double bs = block_sum(part, reinterpret_cast<double*>(&sv[0][0]), ROUNDS * 64 * 16 / 8);
         if (threadIdx.x == 0) partials[blk] = bs;
     }
 }
