@@ -248,9 +248,10 @@ NoAllocScratch noalloc_scratch(int device);
 // error of the last failed persistent-kernel launch (cleared when read)
 hipError_t take_launch_error();
 void record_launch_error(hipError_t e);
-// grid of a persistent one-wave-block kernel: min(want, half of what the
-// device keeps resident at once), so every block runs from the start
-uint32_t persistent_grid(const void* kern, uint32_t want);
+// grid of a persistent kernel with `block` threads per block: min(want, half
+// of what the device keeps resident at once), so every block runs from the
+// start
+uint32_t persistent_grid(const void* kern, uint32_t want, int block = 64);
 void launch_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v);
 void launch_fill_f64(hipStream_t s, double* p, size_t n, double v);
 void launch_fill_f32(hipStream_t s, float* p, size_t n, float v);

@@ -2259,7 +2259,7 @@ hipError_t take_launch_error() {
 void record_launch_error(hipError_t e) {
     if (e != hipSuccess) g_launch_err = e;
 }
-uint32_t persistent_grid(const void* kern, uint32_t want) {
+uint32_t persistent_grid(const void* kern, uint32_t want, int block) {
     static thread_local int dev = -1, cus = 0;
     int d = 0;
     (void)hipGetDevice(&d);
@@ -2268,7 +2268,7 @@ uint32_t persistent_grid(const void* kern, uint32_t want) {
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d);
     }
     int occ = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, 0);
     const uint32_t cap = (uint32_t)std::max(1, occ * cus / 2);
     return std::max(1u, std::min(want, cap));
 }

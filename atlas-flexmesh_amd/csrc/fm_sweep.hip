@@ -415,9 +415,11 @@ __device__ __forceinline__ Pub<T> process(const SweepArgs<T>& A, const Near<T>& 
     const bool wr = gl < 3;
     const T vd = gl == 0 ? v0 : (gl == 1 ? v1 : v2);
     ring[wr ? ((s % kChainRing) * kNPW + g) * kRingStride + gl : (kZeroSlot + 1) * kRingStride + lane] = vd;
-    // r.y partials (idle quads contribute zeros)
+    // r.y partials (idle quads contribute zeros), with explicit fused
+    // multiply-adds: the same bits in every kernel that inlines this step
     if (BWD && gl == 1)
-        dot += (double)N.vec1[0] * (double)v0 + (double)N.vec1[1] * (double)v1 + (double)N.vec1[2] * (double)v2;
+        dot = __builtin_fma((double)N.vec1[2], (double)v2,
+                            __builtin_fma((double)N.vec1[1], (double)v1, __builtin_fma((double)N.vec1[0], (double)v0, dot)));
     return Pub<T>{wr ? scaled<T>(N.f.pub) + gl * (uint32_t)sizeof(T) : kOob, vd};
 }
 
@@ -503,6 +505,249 @@ __global__ __launch_bounds__(64, FM_SWEEP_MINW) void chain_sweep_kernel(SweepArg
     }
 }
 
+// ---------------------------------------------------------------------------
+// Two-wave variant: a loader wave streams the step records, the payload and
+// the nodes' right-hand sides into an LDS ring of R steps; the solving wave
+// polls, solves and publishes.  A wave's vector memory counter completes in
+// issue order, so in the one-wave kernel a polled value is usable only once
+// every payload load issued before the poll has landed: the step waits for
+// the loaded latency of the payload stream.  Here the solving wave's counter
+// holds only its polls and publications.  The waves hand over steps through
+// two LDS counters (steps staged, steps consumed); LDS instructions complete
+// in order per wave and a wave waits for its LDS writes before it bumps its
+// counter, so a counter value read by the other wave covers the data.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct LdSlot {
+    static constexpr uint32_t DEP = Shape<T>::STEP;            // payload first (global layout), then
+    static constexpr uint32_t PUB = DEP + 64 * 8;               // two dependency words per lane,
+    static constexpr uint32_t VEC = PUB + 64 * 4;               // the node word of each lane,
+    static constexpr uint32_t VEC1 = VEC + kNPW * 4 * sizeof(T);  // in0 of each quad's node (4 values),
+    static constexpr uint32_t BYTES = VEC1 + kNPW * 4 * sizeof(T);  // in1 (backward)
+};
+
+#ifndef FM_LD_SLOTS32
+#define FM_LD_SLOTS32 4
+#endif
+#ifndef FM_LD_SLOTS64
+#define FM_LD_SLOTS64 3
+#endif
+#ifndef FM_LD_DEPTH32
+#define FM_LD_DEPTH32 3
+#endif
+#ifndef FM_LD_DEPTH64
+#define FM_LD_DEPTH64 3
+#endif
+template <typename T>
+constexpr int ld_slots() {
+    return sizeof(T) == 4 ? FM_LD_SLOTS32 : FM_LD_SLOTS64;
+}
+template <typename T>
+constexpr int ld_depth() {
+    return sizeof(T) == 4 ? FM_LD_DEPTH32 : FM_LD_DEPTH64;
+}
+// minimum waves per SIMD asked of the register allocator: at most 128 VGPRs
+// in FP32, 256 in FP64
+#ifndef FM_LD_MINW32
+#define FM_LD_MINW32 4
+#endif
+#ifndef FM_LD_MINW64
+#define FM_LD_MINW64 2
+#endif
+
+__device__ __forceinline__ void lds_wait() { __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+// the hand-over counters, addressed as LDS (a volatile generic pointer would
+// become a flat access, which also waits for the wave's global loads)
+typedef __attribute__((address_space(3))) volatile uint32_t LdsCounter;
+// spin on an LDS counter of the other wave (bounded like the global waits;
+// the other wave always advances, so the bound is never reached)
+__device__ __forceinline__ void lds_await(const LdsCounter* c, uint32_t at_least, int* err) {
+    uint32_t spins = 0;
+    while (*c < at_least) {
+        if (++spins > kSpinLimit) {
+            atomicOr(err, 1);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __asm__ volatile("" ::: "memory");
+}
+
+template <typename T, bool BWD>
+__device__ __forceinline__ void slot_write(char* slot, int lane, const Near<T>& N) {
+    using S = Shape<T>;
+    using L = LdSlot<T>;
+    using V4 = T __attribute__((ext_vector_type(4)));
+#pragma unroll
+    for (int c = 0; c < S::NC; ++c) {
+        if constexpr (sizeof(T) == 4)
+            *reinterpret_cast<fm_f4*>(slot + c * 1024 + lane * 16) =
+                fm_f4{N.v[c * 4], N.v[c * 4 + 1], N.v[c * 4 + 2], N.v[c * 4 + 3]};
+        else
+            *reinterpret_cast<fm_d2*>(slot + c * 1024 + lane * 16) = fm_d2{N.v[c * 2], N.v[c * 2 + 1]};
+    }
+    if constexpr (S::TAIL == 2 && sizeof(T) == 4)
+        *reinterpret_cast<fm_f2*>(slot + S::NC * 1024 + lane * 8) = fm_f2{N.v[S::NC * 4], N.v[S::NC * 4 + 1]};
+    *reinterpret_cast<fm_u2*>(slot + L::DEP + lane * 8) = fm_u2{N.f.dep[0], N.f.dep[1]};
+    *reinterpret_cast<uint32_t*>(slot + L::PUB + lane * 4) = N.f.pub;
+    // the four lanes of a quad loaded the same node triple: lane 0 stores it
+    if ((lane & (kLPN - 1)) == 0) {
+        *reinterpret_cast<V4*>(slot + L::VEC + (lane / kLPN) * 4 * sizeof(T)) = V4{N.vec[0], N.vec[1], N.vec[2], (T)0};
+        if (BWD)
+            *reinterpret_cast<V4*>(slot + L::VEC1 + (lane / kLPN) * 4 * sizeof(T)) =
+                V4{N.vec1[0], N.vec1[1], N.vec1[2], (T)0};
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void slot_read_far(const char* slot, int lane, Far& F) {
+    using L = LdSlot<T>;
+    const fm_u2 d = *reinterpret_cast<const fm_u2*>(slot + L::DEP + lane * 8);
+    F.dep[0] = d.x;
+    F.dep[1] = d.y;
+    F.pub = *reinterpret_cast<const uint32_t*>(slot + L::PUB + lane * 4);
+}
+
+template <typename T, bool BWD>
+__device__ __forceinline__ void slot_read(const char* slot, int lane, const Far& F, Near<T>& N) {
+    using S = Shape<T>;
+    using L = LdSlot<T>;
+    using V4 = T __attribute__((ext_vector_type(4)));
+    N.f = F;
+#pragma unroll
+    for (int c = 0; c < S::NC; ++c) {
+        if constexpr (sizeof(T) == 4) {
+            const fm_f4 w = *reinterpret_cast<const fm_f4*>(slot + c * 1024 + lane * 16);
+            N.v[c * 4 + 0] = w.x;
+            N.v[c * 4 + 1] = w.y;
+            N.v[c * 4 + 2] = w.z;
+            N.v[c * 4 + 3] = w.w;
+        } else {
+            const fm_d2 w = *reinterpret_cast<const fm_d2*>(slot + c * 1024 + lane * 16);
+            N.v[c * 2 + 0] = w.x;
+            N.v[c * 2 + 1] = w.y;
+        }
+    }
+    if constexpr (S::TAIL == 2 && sizeof(T) == 4) {
+        const fm_f2 w = *reinterpret_cast<const fm_f2*>(slot + S::NC * 1024 + lane * 8);
+        N.v[S::NC * 4 + 0] = w.x;
+        N.v[S::NC * 4 + 1] = w.y;
+    }
+    const V4 a = *reinterpret_cast<const V4*>(slot + L::VEC + (lane / kLPN) * 4 * sizeof(T));
+    N.vec[0] = a.x;
+    N.vec[1] = a.y;
+    N.vec[2] = a.z;
+    if (BWD) {
+        const V4 c = *reinterpret_cast<const V4*>(slot + L::VEC1 + (lane / kLPN) * 4 * sizeof(T));
+        N.vec1[0] = c.x;
+        N.vec1[1] = c.y;
+        N.vec1[2] = c.z;
+    }
+}
+
+// loader wave: the one-wave kernel's prefetch pipeline (records 2P steps
+// ahead, payload and right-hand side P steps ahead), each step then written
+// to its LDS slot once the solving wave has consumed the slot's previous step
+template <typename T, int P, int R, bool BWD>
+__device__ __forceinline__ void sweep_loader(const SweepArgs<T>& A, uint32_t q0, uint32_t ns, int lane, char* slots,
+                                             LdsCounter* ready, LdsCounter* done) {
+    using L = LdSlot<T>;
+    const uint32_t sl = ns - 1;
+    const BundleRsrc Rs = bundle_rsrc(A, q0, ns);
+    Far F[2 * P];
+    Near<T> N[P];
+#pragma unroll
+    for (int u = 0; u < 2 * P; ++u) far_load(Rs, min((uint32_t)u, sl), lane, F[u]);
+#pragma unroll
+    for (int u = 0; u < P; ++u) near_load<T, BWD>(A, Rs, min((uint32_t)u, sl), lane, F[u], N[u]);
+    auto step = [&](uint32_t j, int u) {
+        if (j >= (uint32_t)R) lds_await(done, j - R + 1, A.err);
+        slot_write<T, BWD>(slots + (j % R) * L::BYTES, lane, N[u % P]);
+        lds_wait();
+        if (lane == 0) *ready = j + 1;
+        near_load<T, BWD>(A, Rs, min(j + P, sl), lane, F[(u + P) % (2 * P)], N[u % P]);
+        far_load(Rs, min(j + 2 * P, sl), lane, F[u]);
+    };
+    uint32_t j0 = 0;
+    if (ns >= 2 * P) {
+#pragma unroll
+        for (int u = 0; u < 2 * P; ++u) step(u, u);
+        for (j0 = 2 * P; j0 + 2 * P <= ns; j0 += 2 * P)
+#pragma unroll
+            for (int u = 0; u < 2 * P; ++u) step(j0 + u, u);
+    }
+#pragma unroll
+    for (int u = 0; u < 2 * P; ++u)
+        if (j0 + u < ns) step(j0 + u, u);
+}
+
+// solving wave: the records of step s + 1 are read from LDS and polled while
+// step s is solved
+template <typename T, int R, bool BWD>
+__device__ __forceinline__ void sweep_solver(const SweepArgs<T>& A, uint32_t ns, int lane, const char* slots,
+                                             T* ring, LdsCounter* ready, LdsCounter* done, double& dot) {
+    using L = LdSlot<T>;
+    const Rsrc ro = make_rsrc(A.out, A.out_bytes);
+    Far F[2];
+    Ext<T> X[2];
+    lds_await(ready, 1, A.err);
+    slot_read_far<T>(slots, lane, F[0]);
+    poll_ext(A, F[0], X[0]);
+    auto step = [&](uint32_t s, int u) {
+        const uint32_t s1 = min(s + 1, ns - 1);
+        lds_await(ready, s1 + 1, A.err);
+        slot_read_far<T>(slots + (s1 % R) * L::BYTES, lane, F[u ^ 1]);
+        poll_ext(A, F[u ^ 1], X[u ^ 1]);
+        Near<T> N;
+        slot_read<T, BWD>(slots + (s % R) * L::BYTES, lane, F[u], N);
+        const Pub<T> pb = process<T, BWD>(A, N, X[u], s, lane, ring, dot);
+        publish_at(ro, pb.off, pb.v);
+        lds_wait();
+        if (lane == 0) *done = s + 1;
+    };
+    uint32_t s0 = 0;
+    if (ns >= 2) {
+        step(0, 0);
+        step(1, 1);
+        for (s0 = 2; s0 + 2 <= ns; s0 += 2) {
+            step(s0, 0);
+            step(s0 + 1, 1);
+        }
+    }
+    if (s0 < ns) step(s0, 0);
+}
+
+template <typename T, int P, int R, bool BWD>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? FM_LD_MINW32
+                                                                                      : FM_LD_MINW64))) void
+chain_sweep2_kernel(SweepArgs<T> A) {
+    if (A.check_done && A.st->done) return;
+    __shared__ T ring[(kZeroSlot + 1) * kRingStride + 64];
+    __shared__ __align__(16) char slots[R * LdSlot<T>::BYTES];
+    __shared__ uint32_t counters[2];  // steps staged by the loader, steps consumed by the solver
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (threadIdx.x < kRingStride) ring[kZeroSlot * kRingStride + threadIdx.x] = (T)0;
+    for (uint32_t b = blockIdx.x; b < A.nb; b += gridDim.x) {
+        if (threadIdx.x == 0) counters[0] = counters[1] = 0;
+        __syncthreads();
+        const uint32_t q0 = __builtin_amdgcn_readfirstlane(A.bstart[b]);
+        const uint32_t ns = __builtin_amdgcn_readfirstlane(A.bsteps[b]);
+        if (wave == 1) {
+            sweep_loader<T, P, R, BWD>(A, q0, ns, lane, slots, (LdsCounter*)&counters[0], (LdsCounter*)&counters[1]);
+        } else {
+            double dot = 0.0;
+            sweep_solver<T, R, BWD>(A, ns, lane, slots, ring, (LdsCounter*)&counters[0], (LdsCounter*)&counters[1],
+                                    dot);
+            if (BWD && A.partials) {
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o);
+                if (lane == 0) A.partials[b] = dot;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Repack of the CSR factor into the step-major payload of one sweep: one
 // thread per (wave-step, lane).  Forward: lower blocks k (node-block position
 // k of the row), unit lower diagonal block (l10, l20, l21).  Backward: upper
@@ -581,6 +826,19 @@ constexpr int depth() {
     return sizeof(T) == 4 ? FM_CHAIN_D32 : FM_CHAIN_D64;
 }
 
+// The two-wave kernel serves the FP64 sweeps, the one-wave kernel the FP32
+// sweeps (the faster of the two for each at the 128^3-hex box);
+// FM_SWEEP_LOADER=0 or 1 selects one of them for both (read once per process)
+template <typename T>
+static bool sweep_loader_on() {
+    static const int mode = [] {
+        const char* e = std::getenv("FM_SWEEP_LOADER");
+        if (e && (e[0] == '0' || e[0] == '1') && e[1] == 0) return e[0] - '0';
+        return -1;
+    }();
+    return mode < 0 ? sizeof(T) == 8 : mode == 1;
+}
+
 template <typename T, bool BWD>
 uint32_t launch_sweep(hipStream_t s, const Matrix& A, const ChainSweep& W, const T* pk, SweepArgs<T> a) {
     if (!a.in1) a.in1 = a.in0;  // always read (value unused)
@@ -590,9 +848,15 @@ uint32_t launch_sweep(hipStream_t s, const Matrix& A, const ChainSweep& W, const
     a.bsteps = W.bsteps.p;
     a.far = W.far.p;
     a.pk = pk;
-    auto kern = chain_sweep_kernel<T, depth<T>(), BWD>;
-    const uint32_t grid = persistent_grid(reinterpret_cast<const void*>(kern), W.nb);
-    kern<<<grid, 64, 0, s>>>(a);
+    if (sweep_loader_on<T>()) {
+        // one block per bundle: a bundle only waits on bundles of lower
+        // index, whose blocks started earlier, so progress needs no co-residency
+        chain_sweep2_kernel<T, ld_depth<T>(), ld_slots<T>(), BWD><<<W.nb, 128, 0, s>>>(a);
+    } else {
+        auto kern = chain_sweep_kernel<T, depth<T>(), BWD>;
+        const uint32_t grid = persistent_grid(reinterpret_cast<const void*>(kern), W.nb);
+        kern<<<grid, 64, 0, s>>>(a);
+    }
     record_launch_error(hipGetLastError());
     return W.nb;  // partials: one per bundle
 }

@@ -1,8 +1,11 @@
 """Run in a child process by the GPU tests: ILU(0) apply of an elastic tangent
 through the C ABI against the oracle, with the sweep kind the environment
-selects (FM_SWEEP=level forces the level-scheduled sweeps).
+selects (FM_SWEEP=level forces the level-scheduled sweeps; FM_SWEEP_LOADER=0
+or 1 selects the one-wave or the two-wave chain-sweep kernel).  With an output
+path it also writes the sweep results of three meshes (ILU(0) applies and PCG
+solves in both precisions) for the parent test to compare bit for bit.
 
-  python tests/sweep_check.py level|chain
+  python tests/sweep_check.py level|chain [OUT.npz]
 """
 import os
 import sys
@@ -37,6 +40,23 @@ def main():
     xg, its, _ = S.pcg(x, tol=1e-10, max_it=500)
     print(f"{kind}: ilu fp64 {e64:.2e} fp32 {e32:.2e} pcg its {its} vs {itso} x {rel(xg, xo):.2e}")
     assert e64 < 1e-12 and e32 < 1e-4 and abs(its - itso) <= 1 and rel(xg, xo) < 1e-9
+    if len(sys.argv) > 2:
+        out = {}
+        # a cube, a beam (one bundle, long chains) and a mesh whose bundles
+        # have fewer steps than the kernels' unrolled blocks
+        for dims in ((7, 6, 5), (20, 3, 3), (2, 2, 1)):
+            rp, ci, va = elastic_matrix(O, fm, dims=dims)
+            S.set_matrix(rp, ci, va)
+            tag = "x".join(map(str, dims))
+            out[f"{tag}_kind"] = np.array([S.sweep_info()["sweep"] == "chain"])
+            x = np.random.default_rng(2).standard_normal(rp.size - 1)
+            out[f"{tag}_z64"] = S.ilu0_apply(x)
+            S.ilu0_factor(fm.FM_MIXED)
+            out[f"{tag}_z32"] = S.ilu0_apply(x, fm.FM_MIXED)
+            for name, prec, tol in (("pcg64", fm.FM_FP64, 1e-10), ("pcg32", fm.FM_MIXED, 1e-5)):
+                xs, its, relres = S.pcg(x, tol=tol, max_it=500, prec=prec)
+                out[f"{tag}_{name}"] = np.append(xs, [its, relres])
+        np.savez(sys.argv[2], **out)
     S.close()
     print(f"SWEEP OK {kind}")
 

@@ -281,6 +281,27 @@ def test_level_sweeps_still_match_oracle(fm):
     assert p.returncode == 0 and "SWEEP OK level" in p.stdout
 
 
+def test_chain_sweep_kernels_same_bits(tmp_path):
+    """The one-wave and the two-wave (loader + solver) chain-sweep kernels do
+    the same arithmetic in the same order: identical ILU(0) applies and PCG
+    solves (whose backward sweeps carry the fused r.y partials), FP64 and FP32."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    res = {}
+    for v in ("0", "1"):
+        out = tmp_path / f"loader{v}.npz"
+        env = dict(os.environ, FM_SWEEP_LOADER=v)
+        env.pop("FM_SWEEP", None)
+        p = subprocess.run([sys.executable, os.path.join(here, "sweep_check.py"), "chain", str(out)], env=env,
+                           capture_output=True, text=True, timeout=300)
+        print(p.stdout, p.stderr)
+        assert p.returncode == 0 and "SWEEP OK chain" in p.stdout
+        res[v] = dict(np.load(out))
+    assert res["0"].keys() == res["1"].keys() and len(res["0"]) == 15
+    assert res["0"]["7x6x5_kind"][0] and res["0"]["20x3x3_kind"][0]
+    for k in res["0"]:
+        assert np.array_equal(res["0"][k], res["1"][k]), k
+
+
 @pytest.mark.parametrize("gn", [16, 32, 48])
 def test_pcg_matches_reference_golden(fm, S, O, gn):
     g = golden("pcg.npz")
