@@ -195,6 +195,8 @@ struct Ctx {
 
     // profiling
     bool prof = false;
+    int prof_every = 1;               // time one launch in prof_every of each class
+    uint64_t prof_seen[FM_K_COUNT] = {0};  // launches of each class while profiling
     double ms[FM_K_COUNT] = {0};
     uint64_t launches[FM_K_COUNT] = {0};
     std::vector<hipEvent_t> ev_pool;

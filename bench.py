@@ -41,6 +41,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "atlas-flexmesh_amd"))
 
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
+# The timed pass brackets one launch in PROF_EVERY of the kernels launched once
+# per PCG iteration (SpMV, the two sweeps, the other PCG kernels) with HIP
+# events, and every launch of the other classes: each pair of events puts a
+# bubble of a few microseconds into the stream (about 4% of a load step when
+# every launch is bracketed).
+PROF_EVERY = 16
 
 
 def log(msg):
@@ -200,7 +206,7 @@ def run_steps(S, cfg, prec, first, last, inner_tol, profile, pcg_max=1000, lift=
     tot["step_seconds"], tot["r0"], tot["rel_res"] = [], [], []
     worst = 0.0
     if profile:
-        S.profile(True)
+        S.profile(PROF_EVERY)
         S.profile_reset()
     name = {fm.FM_FP64: "fp64", fm.FM_MIXED: "mixed", fm.FM_MIXED_PC: "mixed-pc"}[prec]
     t0 = time.perf_counter()
@@ -393,6 +399,9 @@ def bench(S, cfg, args, warm_last, K):
                        "traffic": measured_traffic(dom, "fp32" if mixed else "fp64", cfg["dims"]),
                        "algorithmic_bytes": d["bytes_per_launch"], "avg_us": d["avg_us"]}
     out["kernels"] = kern
+    out["kernel_timing"] = (f"HIP events on the library's stream around one launch in {PROF_EVERY} of spmv, ilu_fwd, "
+                            "ilu_bwd and pcg_other and around every other launch, over the timed pass; ms_total = "
+                            "mean of the timed launches x launches")
     if "spmv" in kern:
         out["spmv_frac_of_peak"] = kern["spmv"]["GBps"] / PEAK_HBM_GBS
     out["element_fields"] = element_fields_timing(S, info)

@@ -292,9 +292,15 @@ int fm_checkpoint_load(fm_ctx* ctx, const char* path, uint32_t* load_step);
 #define FM_K_PCG_OTHER 6
 #define FM_K_FIELDS 7
 #define FM_K_COUNT 8
+/* on = 0: off; 1: every launch of the timed kernel classes is bracketed by
+   HIP events on the context's stream; k > 1: of the classes launched once per
+   PCG iteration (SpMV, the two sweeps, the other PCG kernels) one launch in
+   every k (launches 0, k, 2k, ... counted from fm_profile_reset), since each
+   pair of events puts a bubble of a few microseconds into the stream */
 int fm_profile_enable(fm_ctx* ctx, int on);
 int fm_profile_reset(fm_ctx* ctx);
-/* ms[k] total milliseconds and launches[k] count for kernel class k */
+/* launches[k]: launches of kernel class k while profiling was on; ms[k]: their
+   total milliseconds (the mean of the timed launches times launches[k]) */
 int fm_profile_get(fm_ctx* ctx, double* ms, uint64_t* launches);
 /* sizes for roofline bookkeeping: n (rows), nnz, n_elems, n_nodes, n_colors */
 int fm_info(fm_ctx* ctx, uint64_t* out5);

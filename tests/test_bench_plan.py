@@ -58,3 +58,18 @@ def test_algorithmic_bytes_of_the_node_spmv():
     info = {"n": 6_440_067, "nnz": 286_222_473, "ne": 12_582_912, "nn": 2_146_689, "ncolors": 24, "spmv": "node"}
     b = m.kernel_bytes(info, "spmv", 4)
     assert b == 286_222_473 * 4 + (286_222_473 // 9) * 4 + 2_146_689 * 16 + 6_440_067 * 8
+
+
+def test_committed_traffic_summary_feeds_every_roofline_class():
+    """bench.py's roofline.traffic comes from the machine-readable lines of the
+    committed PMC summary: every class in both precisions must be there, and
+    each figure must be at least the class's algorithmic bytes at 128^3 hexes."""
+    import bench
+
+    info = {"n": 6440067, "nnz": 286222473, "ne": 12582912, "nn": 2146689, "spmv": "node", "sweep": "chain"}
+    for prec, v in (("fp32", 4), ("fp64", 8)):
+        for kind in ("spmv", "ilu_fwd", "ilu_bwd", "ilu_factor"):
+            t = bench.measured_traffic(kind, prec, (128, 128, 128))
+            assert t is not None, (kind, prec)
+            assert t >= 0.95 * bench.kernel_bytes(info, kind, v), (kind, prec, t)
+    assert bench.measured_traffic("spmv", "fp32", (64, 64, 64)) is None
