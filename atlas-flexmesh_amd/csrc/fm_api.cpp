@@ -799,8 +799,106 @@ int pcg_run_pc(fm_ctx* ctx, Matrix& A, double tol, int max_it, PcgResult* res) {
 // Solve K du = rhs (FP64 device vector) with the tangent; du FP64.  The mixed
 // solves, and the FP64 solve when `inexact`, work on the normalised right-hand
 // side with a forcing-term tolerance.
+// Galerkin start of a PCG solve A x = b (b, x: the PCG vectors of precision
+// p): x0 = V c with (V^T A V) c = V^T b over the m most recent basis vectors,
+// i.e. the A-norm-best start in span(V).  A vector whose pivot in the
+// Cholesky factor of V^T A V is not above 1e-10 of its diagonal adds nothing
+// new and is left out.
+constexpr int kGalMax = 3;
+int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int m) {
+    hipStream_t s = ctx->stream;
+    const uint32_t n = A.n;
+    const int pi = p == PREC64 ? 0 : 1;
+    m = std::min({m, ctx->gal_n[pi], kGalMax});
+    void* b = vb(ctx, p, VB);
+    void* x = vb(ctx, p, VX);
+    void* t = vb(ctx, p, VAP);
+    const void* V[kGalMax];
+    for (int j = 0; j < m; ++j) {
+        const int slot = (ctx->gal_next[pi] - 1 - j + 2 * kGalMax) % kGalMax;
+        V[j] = pi == 0 ? (const void*)ctx->gal64[slot].p : (const void*)ctx->gal32[slot].p;
+    }
+    FM_TRY(ensure_partials(ctx, n / 256 + 1));
+    for (int j = 0; j < m; ++j) {
+        launch_spmv(s, p, A, V[j], t, false, nullptr, nullptr, nullptr, nullptr);
+        for (int i = 0; i <= j; ++i) {
+            const uint32_t np = launch_dot(s, p, n, V[i], t, ctx->partials.p);
+            launch_reduce(s, ctx->partials.p, np, ctx->st.p, 5, i * kGalMax + j);
+        }
+        const uint32_t np = launch_dot(s, p, n, V[j], b, ctx->partials.p);
+        launch_reduce(s, ctx->partials.p, np, ctx->st.p, 5, kGalMax * kGalMax + j);
+    }
+    FM_HIP(hipMemcpyAsync(ctx->h_st, ctx->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    FM_TRY(sync(ctx));
+    const double* g = ctx->h_st->gal;
+    // Cholesky of the Gram matrix over the kept vectors, then the two triangular solves
+    double L[kGalMax][kGalMax] = {}, y[kGalMax] = {}, c[kGalMax] = {};
+    int keep[kGalMax], k = 0;
+    for (int j = 0; j < m; ++j) {
+        double row[kGalMax];
+        for (int a = 0; a < k; ++a) {
+            const int i = keep[a];
+            double v = g[std::min(i, j) * kGalMax + std::max(i, j)];
+            for (int q = 0; q < a; ++q) v -= L[a][q] * row[q];
+            row[a] = v / L[a][a];
+        }
+        double d = g[j * kGalMax + j];
+        for (int a = 0; a < k; ++a) d -= row[a] * row[a];
+        if (!(d > 1e-10 * g[j * kGalMax + j])) continue;
+        for (int a = 0; a < k; ++a) L[k][a] = row[a];
+        L[k][k] = std::sqrt(d);
+        keep[k++] = j;
+    }
+    for (int a = 0; a < k; ++a) {
+        double v = g[kGalMax * kGalMax + keep[a]];
+        for (int q = 0; q < a; ++q) v -= L[a][q] * y[q];
+        y[a] = v / L[a][a];
+    }
+    for (int a = k - 1; a >= 0; --a) {
+        double v = y[a];
+        for (int q = a + 1; q < k; ++q) v -= L[q][a] * c[q];
+        c[a] = v / L[a][a];
+    }
+    const void* W[kGalMax];
+    for (int a = 0; a < k; ++a) W[a] = V[keep[a]];
+    if (k)
+        launch_combine(s, p, n, k, c, W, x);
+    else if (p == PREC64)
+        launch_fill_f64(s, (double*)x, n, 0.0);
+    else
+        launch_fill_f32(s, (float*)x, n, 0.f);
+    return FM_OK;
+}
+
+// the solution of the last solve (the PCG vector x of precision p), normalised,
+// becomes the newest basis vector
+int galerkin_push(fm_ctx* ctx, uint32_t n, int p) {
+    hipStream_t s = ctx->stream;
+    const int pi = p == PREC64 ? 0 : 1;
+    const int slot = ctx->gal_next[pi];
+    void* dst;
+    if (pi == 0) {
+        FM_HIP(ctx->gal64[slot].alloc(n));
+        dst = ctx->gal64[slot].p;
+    } else {
+        FM_HIP(ctx->gal32[slot].alloc(n));
+        dst = ctx->gal32[slot].p;
+    }
+    const void* x = vb(ctx, p, VX);
+    FM_TRY(ensure_partials(ctx, n / 256 + 1));
+    const uint32_t np = launch_dot(s, p, n, x, x, ctx->partials.p);
+    launch_reduce(s, ctx->partials.p, np, ctx->st.p, 5, 15);
+    launch_normalize(s, p, n, &ctx->st.p->gal[15], x, dst);
+    ctx->gal_next[pi] = (slot + 1) % kGalMax;
+    ctx->gal_n[pi] = std::min(ctx->gal_n[pi] + 1, kGalMax);
+    return FM_OK;
+}
+
+// recycle > 0: the solve starts from the Galerkin start over that many of the
+// latest basis vectors (FP32 and inexact FP64 solves); push: its solution
+// becomes the newest basis vector
 int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double tol, int max_it, int* iters,
-                 bool inexact = false, int* capped = nullptr) {
+                 bool inexact = false, int* capped = nullptr, int recycle = 0, bool push = false) {
     Matrix& A = ctx->tangent;
     ctx->cur = &A;
     const uint32_t n = A.n;
@@ -835,8 +933,12 @@ int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double to
     FM_TRY(pcg_vectors(ctx, prec, n));
     if (prec == PREC64 && inexact) {
         launch_scale(ctx->stream, n, rhs, 1.0 / nr, ctx->b64.p);
-        launch_fill_f64(ctx->stream, ctx->x64.p, n, 0.0);
+        if (recycle > 0 && ctx->gal_n[0] > 0)
+            FM_TRY(galerkin_start(ctx, A, PREC64, recycle));
+        else
+            launch_fill_f64(ctx->stream, ctx->x64.p, n, 0.0);
         FM_TRY(pcg_run(ctx, A, PREC64, tol, max_it, &pr));
+        if (push) FM_TRY(galerkin_push(ctx, n, PREC64));
         launch_scale(ctx->stream, n, ctx->x64.p, nr, du);
     } else if (prec == PREC64) {
         FM_HIP(hipMemcpyAsync(ctx->b64.p, rhs, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));
@@ -846,8 +948,12 @@ int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double to
     } else {
         // FP32 inner solve on the normalised right-hand side
         launch_f64_to_f32(ctx->stream, n, rhs, 1.0 / nr, ctx->b32.p);
-        launch_fill_f32(ctx->stream, ctx->x32.p, n, 0.f);
+        if (recycle > 0 && ctx->gal_n[1] > 0)
+            FM_TRY(galerkin_start(ctx, A, PREC32, recycle));
+        else
+            launch_fill_f32(ctx->stream, ctx->x32.p, n, 0.f);
         FM_TRY(pcg_run(ctx, A, PREC32, tol, max_it, &pr));
+        if (push) FM_TRY(galerkin_push(ctx, n, PREC32));
         launch_f32_to_f64(ctx->stream, n, ctx->x32.p, nr, du);
     }
     *iters = pr.iters;
@@ -992,6 +1098,7 @@ int fm_field_lognormal(uint32_t n, double median, double sigma, uint64_t seed, d
 int fm_set_mesh(fm_ctx* ctx, uint32_t n_nodes, const double* X, uint32_t n_elems, const uint32_t* conn) {
     if (!ctx || !X || !conn || !n_nodes || !n_elems) return fail(ctx, FM_EINVAL, "fm_set_mesh: bad arguments");
     ctx->sec_n = 0;
+    ctx->gal_n[0] = ctx->gal_n[1] = 0;
     if (3ull * n_nodes > 0xFFFFFFFFull) return fail(ctx, FM_EINVAL, "too many nodes");
     for (size_t k = 0; k < 4ull * n_elems; ++k)
         if (conn[k] >= n_nodes) return fail(ctx, FM_EINVAL, "connectivity refers to a node out of range");
@@ -1023,6 +1130,7 @@ int fm_set_coords(fm_ctx* ctx, uint32_t n_nodes, const double* X) {
     ctx->hX.assign(X, X + 3ull * n_nodes);
     FM_HIP(hipMemcpy(ctx->X.p, X, 3ull * n_nodes * 8, hipMemcpyHostToDevice));
     ctx->sec_n = 0;
+    ctx->gal_n[0] = ctx->gal_n[1] = 0;
     Matrix& M = ctx->tangent;
     M.val_valid[0] = M.val_valid[1] = false;
     M.lu_valid[0] = M.lu_valid[1] = false;
@@ -1033,6 +1141,7 @@ int fm_set_coords(fm_ctx* ctx, uint32_t n_nodes, const double* X) {
 int fm_set_material(fm_ctx* ctx, const double* mu_e, const double* lam_e) {
     if (!ctx || !mu_e || !lam_e) return fail(ctx, FM_EINVAL, "fm_set_material: bad arguments");
     ctx->sec_n = 0;
+    ctx->gal_n[0] = ctx->gal_n[1] = 0;
     if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
     ctx->hmu.assign(mu_e, mu_e + ctx->ne);
     ctx->hlam.assign(lam_e, lam_e + ctx->ne);
@@ -1043,6 +1152,7 @@ int fm_set_material(fm_ctx* ctx, const double* mu_e, const double* lam_e) {
 int fm_set_bcs(fm_ctx* ctx, uint32_t n_dir, const uint32_t* dofs, const double* vals) {
     if (!ctx || (n_dir && !dofs)) return fail(ctx, FM_EINVAL, "fm_set_bcs: bad arguments");
     ctx->sec_n = 0;
+    ctx->gal_n[0] = ctx->gal_n[1] = 0;
     if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
     const uint32_t n = 3 * ctx->nn;
     std::vector<uint8_t> mask(n, 0);
@@ -1067,6 +1177,7 @@ int fm_set_bcs(fm_ctx* ctx, uint32_t n_dir, const uint32_t* dofs, const double* 
 int fm_set_load(fm_ctx* ctx, const double* f_ext) {
     if (!ctx) return FM_EINVAL;
     ctx->sec_n = 0;
+    ctx->gal_n[0] = ctx->gal_n[1] = 0;
     if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
     if (!f_ext) {
         ctx->have_fext = false;
@@ -1457,6 +1568,8 @@ void fm_newton_cfg_default(fm_newton_cfg* c) {
     c->energy_tol = 0.0;
     c->record_energy = 0;
     c->inexact = 0;
+    c->recycle = 0;
+    c->inner_tol_max = 0.0;
 }
 
 int fm_set_u(fm_ctx* ctx, const double* u) {
@@ -1469,10 +1582,12 @@ int fm_set_u(fm_ctx* ctx, const double* u) {
         FM_TRY(sync(ctx));
         ctx->u_valid = true;
         ctx->sec_n = 1;
+        ctx->gal_n[0] = ctx->gal_n[1] = 0;
         ctx->sec_lf[0] = 0.0;
         return FM_OK;
     }
     ctx->sec_n = 0;
+    ctx->gal_n[0] = ctx->gal_n[1] = 0;
     FM_HIP(hipMemcpy(ctx->u.p, u, 3ull * ctx->nn * 8, hipMemcpyHostToDevice));
     ctx->u_valid = true;
     return FM_OK;
@@ -1535,12 +1650,14 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         FM_HIP(hipMemcpyAsync(ctx->u.p, u, n * 8ull, hipMemcpyHostToDevice, s));
         ctx->u_valid = true;
         ctx->sec_n = 0;  // a state from outside: no known history
+        ctx->gal_n[0] = ctx->gal_n[1] = 0;
     }
     if (!ctx->u_valid) {
         // u = 0 is the equilibrium at load factor 0 (no load, no prescribed values)
         launch_fill_f64(s, ctx->u.p, n, 0.0);
         ctx->u_valid = true;
         ctx->sec_n = 1;
+        ctx->gal_n[0] = ctx->gal_n[1] = 0;
         ctx->sec_lf[0] = 0.0;
     }
     // linear-solve mode and the precision the tangent is assembled in
@@ -1550,6 +1667,9 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     // mixed path's forcing term when inexact
     const bool inexact = kprec == PREC64 && cfg.inexact;
     const double lin_tol = kprec == PREC64 && !inexact ? cfg.pcg_tol : cfg.inner_tol;
+    // the options of the inner solves: target-aware forcing, Galerkin start
+    const bool adapt = (kprec == PREC32 || inexact) && cfg.inner_tol_max > lin_tol;
+    const bool recycle = (kprec == PREC32 || inexact) && cfg.lift == 2 && cfg.recycle > 0;
     const int lin_cap = kprec == PREC64 && !inexact ? cfg.pcg_max : cfg.inner_max;
     const double lf = load_factor;
     fm_newton_stats S{};
@@ -1644,7 +1764,10 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         // convergence, and a mixed iteration's reduction is set by the FP32
         // tangent rather than by this lag)
         // (reuse_tangent keeps it for every iteration of the step)
-        const bool keep = it > 0 && (cfg.reuse_tangent || (order == 2 && rn <= 1e-2 * rn_prev));
+        // (not with the target-aware forcing: its second solve is the
+        // expensive one, and on the first tangent it takes more PCG
+        // iterations than the assembly and factorization it would save)
+        const bool keep = it > 0 && (cfg.reuse_tangent || (order == 2 && !adapt && rn <= 1e-2 * rn_prev));
         if (!keep) {
             FM_TRY(assemble(ctx, ctx->u.p, true, aprec, true, nullptr, FM_K_ASSEMBLE));
             S.assemblies++;
@@ -1652,7 +1775,16 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         int its = 0;
         // R of the current u is in ctx->R (last residual evaluation)
         FM_HIP(hipMemcpyAsync(ctx->hv.p, ctx->R.p, n * 8ull, hipMemcpyDeviceToDevice, s));
-        FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, lin_tol, lin_cap, &its, inexact, &S.pcg_cap_hits));
+        // target-aware forcing term: a solve that needs only a small reduction
+        // of ||R|| to reach the target stops early (it achieves about half the
+        // forcing term's reduction, so 0.5 target / ||R|| leaves a margin of ~4)
+        double eta = lin_tol;
+        if (adapt) eta = std::min(cfg.inner_tol_max, std::max(lin_tol, 0.5 * target / rn));
+        // the first solve of a step starts from the Galerkin projection onto
+        // the previous steps' first corrections, and its solution joins them
+        const bool first = recycle && it == 0;
+        FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, eta, lin_cap, &its, inexact, &S.pcg_cap_hits,
+                            first ? cfg.recycle : 0, first));
         S.pcg_iters += its;
         if (!keep) S.factorizations++;
         // armijo_line_search_true (FEM/adaptive_fem_engine.hpp:933-985); with
@@ -1708,6 +1840,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         ctx->sec_lf[0] = lf;
     } else {
         ctx->sec_n = 0;
+        ctx->gal_n[0] = ctx->gal_n[1] = 0;
     }
     S.rel_res = S.r0 > 0 ? rn / S.r0 : 0.0;
     S.seconds = now_s() - t0;

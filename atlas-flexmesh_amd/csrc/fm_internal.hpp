@@ -46,6 +46,7 @@ constexpr int kReduceBlocks = 64;  // most blocks of one reduce (each sums a con
 struct PcgState {
     double rz, pAp, alpha, beta, bb, rz_new, relres, tol;
     double slot[4];
+    double gal[16];  // Gram matrix and right-hand side of a Galerkin start (reduce op 5)
     int done;     // 1 once converged, broken down or out of iterations
     int status;   // 0 running, 1 converged, 2 breakdown (|pAp| < 1e-60), 3 max_it
     int iters;
@@ -154,6 +155,13 @@ struct Ctx {
     // state u at load factor sec_lf[0] (sec_n >= 1), sec_u1 at sec_lf[1]
     // (sec_n >= 2) and sec_u2 at sec_lf[2] (sec_n == 3); cleared by anything
     // that changes the problem or u
+    // normalised first Newton corrections of the latest load steps of this
+    // load path, per PCG precision ([0] FP64, [1] FP32): a ring of kGalMax,
+    // gal_n valid, gal_next the slot written next; they span the Galerkin
+    // start of a step's first linear solve (cleared with the history below)
+    DevBuf<double> gal64[3];
+    DevBuf<float> gal32[3];
+    int gal_n[2] = {0, 0}, gal_next[2] = {0, 0};
     DevBuf<double> sec_u1, sec_u2, sec_start;
     double sec_lf[3] = {0.0, 0.0, 0.0};
     int sec_n = 0;
@@ -277,7 +285,7 @@ uint32_t launch_residual_finish(hipStream_t s, uint32_t n, double load, const do
 // partials of a.b; returns #partials
 uint32_t launch_dot64(hipStream_t s, uint32_t n, const double* a, const double* b, double* partials);
 // reduce partials into st: op 0 = store in slot[k]; 1 = pAp/alpha; 2 = rz_new/convergence; 3 = init rz;
-// 4 = as 2 with the flexible beta (the r.z_old term stored in slot[k] by op 0)
+// 4 = as 2 with the flexible beta (the r.z_old term stored in slot[k] by op 0); 5 = store in gal[k]
 void launch_reduce(hipStream_t s, const double* partials, uint32_t np, PcgState* st, int op, int k);
 void launch_spmv(hipStream_t s, int prec, const Matrix& A, const void* x, void* y, bool dot, double* partials,
                  void* sent0, void* sent1, const PcgState* st);
@@ -340,6 +348,13 @@ void launch_element_fields(hipStream_t s, const ElemArgs& a, const uint32_t* eip
 // out = a x + b y (+ c z when z is given)
 void launch_lincomb3(hipStream_t s, uint32_t n, double a, const double* x, double b, const double* y, double c,
                      const double* z, double* out);
+// Galerkin start of a linear solve (precision prec: PREC32 or PREC64 vectors)
+// block partials of a.b with FP64 accumulation; returns #partials
+uint32_t launch_dot(hipStream_t s, int prec, uint32_t n, const void* a, const void* b, double* partials);
+// out = sum_{i < m} c[i] v[i] (m <= 4)
+void launch_combine(hipStream_t s, int prec, uint32_t n, int m, const double* c, const void* const* v, void* out);
+// out = in / sqrt(*nrm2) (*nrm2 read on the device)
+void launch_normalize(hipStream_t s, int prec, uint32_t n, const double* nrm2, const void* in, void* out);
 // ZZ estimate: zz is kZzRows x ne scratch (colour order), raw and rec 3 nn,
 // err ne (element order); returns the number of partials of sum V sq / 4
 constexpr int kZzRows = 7;

@@ -1562,6 +1562,9 @@ __global__ __launch_bounds__(1024) void reduce_kernel(const double* __restrict__
             st->rz = s;
             st->relres = sqrt(s / fmax(st->bb, 1e-300));
             break;
+        case 5:
+            st->gal[k] = s;
+            break;
         case 4: {  // as 2, with the flexible beta r_new.(z_new - z_old) / r_old.z_old (z_old term in slot[k])
             if (st->done) return;
             st->rz_new = s;
@@ -1679,6 +1682,38 @@ __global__ __launch_bounds__(256) void dot64_kernel(uint32_t n, const double* __
     const double part = (i < n) ? a[i] * b[i] : 0.0;
     const double s = block_sum(part);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void dot_kernel(uint32_t n, const T* __restrict__ a, const T* __restrict__ b,
+                                                  double* __restrict__ partials) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double part = (i < n) ? (double)a[i] * (double)b[i] : 0.0;
+    const double s = block_sum(part);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// out = c0 v0 + c1 v1 + c2 v2 + c3 v3 (the first m terms), in FP64, rounded once
+struct Combine {
+    double c[4];
+    const void* v[4];
+};
+template <typename T>
+__global__ void combine_kernel(uint32_t n, int m, Combine cb, T* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int j = 0; j < m; ++j) acc += cb.c[j] * (double)static_cast<const T*>(cb.v[j])[i];
+    out[i] = (T)acc;
+}
+
+template <typename T>
+__global__ void normalize_kernel(uint32_t n, const double* __restrict__ nrm2, const T* __restrict__ in,
+                                 T* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double s = *nrm2 > 0.0 ? 1.0 / sqrt(*nrm2) : 0.0;
+    out[i] = (T)((double)in[i] * s);
 }
 
 __global__ void f64_to_f32_kernel(uint32_t n, const double* __restrict__ s, double scale, float* __restrict__ d) {
@@ -2348,6 +2383,35 @@ uint32_t launch_residual_finish(hipStream_t s, uint32_t n, double load, const do
     const uint32_t nb = cdiv(n, 256);
     residual_finish_kernel<<<nb, 256, 0, s>>>(n, load, f_ext, f, isdir, R, partials);
     return nb;
+}
+
+uint32_t launch_dot(hipStream_t s, int prec, uint32_t n, const void* a, const void* b, double* partials) {
+    const uint32_t nb = cdiv(n, 256);
+    if (prec == PREC64)
+        dot_kernel<double><<<nb, 256, 0, s>>>(n, (const double*)a, (const double*)b, partials);
+    else
+        dot_kernel<float><<<nb, 256, 0, s>>>(n, (const float*)a, (const float*)b, partials);
+    return nb;
+}
+
+void launch_combine(hipStream_t s, int prec, uint32_t n, int m, const double* c, const void* const* v, void* out) {
+    Combine cb{};
+    for (int j = 0; j < m && j < 4; ++j) {
+        cb.c[j] = c[j];
+        cb.v[j] = v[j];
+    }
+    m = std::min(m, 4);
+    if (prec == PREC64)
+        combine_kernel<double><<<cdiv(n, 256), 256, 0, s>>>(n, m, cb, (double*)out);
+    else
+        combine_kernel<float><<<cdiv(n, 256), 256, 0, s>>>(n, m, cb, (float*)out);
+}
+
+void launch_normalize(hipStream_t s, int prec, uint32_t n, const double* nrm2, const void* in, void* out) {
+    if (prec == PREC64)
+        normalize_kernel<double><<<cdiv(n, 256), 256, 0, s>>>(n, nrm2, (const double*)in, (double*)out);
+    else
+        normalize_kernel<float><<<cdiv(n, 256), 256, 0, s>>>(n, nrm2, (const float*)in, (float*)out);
 }
 
 uint32_t launch_dot64(hipStream_t s, uint32_t n, const double* a, const double* b, double* partials) {

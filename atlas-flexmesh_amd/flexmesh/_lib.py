@@ -29,7 +29,8 @@ class NewtonCfg(C.Structure):
                 ("armijo_c", C.c_double), ("lift", C.c_int), ("inner_max", C.c_int),
                 ("ls_mode", C.c_int), ("tol_absolute", C.c_double), ("ls_alpha0", C.c_double),
                 ("ls_beta", C.c_double), ("reuse_tangent", C.c_int), ("energy_tol", C.c_double),
-                ("record_energy", C.c_int), ("inexact", C.c_int)]
+                ("record_energy", C.c_int), ("inexact", C.c_int), ("recycle", C.c_int),
+                ("inner_tol_max", C.c_double)]
 
 
 FIELD_NAMES = ("det_F", "cauchy", "von_mises", "pressure", "max_principal", "energy", "max_stretch", "F")

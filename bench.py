@@ -16,8 +16,9 @@ around the K steps brackets exactly the device work of the timed region.
 
 value = DOFs x (load steps converged) / wall seconds of the timed steps.
 After the timed region, on the same load steps from the unloaded state:
-  * the plain FP64 GPU path running the same algorithm (predictor, kept
-    tangent, forcing term) -- the precision-only ratio mixed_over_fp64;
+  * the plain FP64 GPU path running the same algorithm (extrapolating
+    predictor, Galerkin start of each step's first solve, target-aware
+    forcing term) -- the precision-only ratio mixed_over_fp64;
   * the plain FP64 GPU path with the reference's solver settings (predictor
     solve, PCG tol 1e-10, 300-iteration cap), with its cap hits counted;
   * the FP64 CPU oracle on a bounded sample on the host cores (assembly,
@@ -300,8 +301,15 @@ def main():
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--dims", type=str, default=None, help="override hex counts, e.g. 64,64,64")
     ap.add_argument("--inner-tol", type=float, default=None,
-                    help="PCG forcing term of the inexact Newton solves (default 1e-2 with the extrapolating "
-                         "predictor, 3e-3 with the predictor solve)")
+                    help="PCG forcing term of the inexact Newton solves, the smallest one with --inner-tol-max "
+                         "(default 1e-3 with the extrapolating predictor and the Galerkin start, 1e-2 with the "
+                         "extrapolating predictor alone, 3e-3 with the predictor solve)")
+    ap.add_argument("--inner-tol-max", type=float, default=None,
+                    help="largest forcing term of the target-aware rule clamp(0.5 target / ||R||, inner_tol, "
+                         "inner_tol_max) (default 0.1 with the extrapolating predictor; 0: a fixed forcing term)")
+    ap.add_argument("--recycle", type=int, default=None,
+                    help="previous load steps whose first Newton corrections span the Galerkin start of a "
+                         "step's first solve (default 2 with the extrapolating predictor; 0: x0 = 0)")
     ap.add_argument("--pcg-max", type=int, default=1000, help="PCG iteration cap of the inexact Newton solves")
     ap.add_argument("--no-fp64", action="store_true", help="skip both FP64 GPU rows")
     ap.add_argument("--fp64-ref-steps", type=int, default=None,
@@ -354,16 +362,27 @@ def bench(S, cfg, args, warm_last, K):
     # the extrapolating predictor uses the previous converged steps of the
     # path (the unloaded state at load factor 0 among them)
     lift = 2 if args.predictor == "extrapolate" else 1
-    # the forcing term that minimises the PCG work: a step from the
-    # extrapolated start needs two Newton iterations up to 1e-2 (3e-3 from a
-    # predictor solve)
-    inner_tol = args.inner_tol if args.inner_tol is not None else (1e-2 if lift == 2 else 3e-3)
+    # the inner solves: with the extrapolated start, the first solve of a
+    # step starts from the Galerkin projection onto the previous two steps'
+    # first corrections and goes to 1e-3 (that start leaves it a few tens of
+    # iterations), and the second one stops at the target-aware forcing term
+    # (about 0.1: it needs only a few-fold reduction); without them the
+    # extrapolated start takes 1e-2 throughout, the predictor solve 3e-3
+    recycle = args.recycle if args.recycle is not None else (2 if lift == 2 else 0)
+    inner_tol_max = args.inner_tol_max if args.inner_tol_max is not None else (0.1 if lift == 2 else 0.0)
+    if args.inner_tol is not None:
+        inner_tol = args.inner_tol
+    elif lift == 2:
+        inner_tol = 1e-3 if (recycle or inner_tol_max) else 1e-2
+    else:
+        inner_tol = 3e-3
+    solver_kw = dict(recycle=recycle, inner_tol_max=inner_tol_max)
     if warm_last:
         log(f"warm-up pass: load steps 1..{warm_last}")
-        run_steps(S, cfg, prec, 1, warm_last, inner_tol, False, args.pcg_max, lift)
+        run_steps(S, cfg, prec, 1, warm_last, inner_tol, False, args.pcg_max, lift, **solver_kw)
     S.set_u(None)  # back to the unloaded state: the timed pass is the load path from its start
     log(f"timed pass: load steps 1..{K} of {cfg['n_steps']}")
-    wall, tot, worst, prof = run_steps(S, cfg, prec, 1, K, inner_tol, True, args.pcg_max, lift)
+    wall, tot, worst, prof = run_steps(S, cfg, prec, 1, K, inner_tol, True, args.pcg_max, lift, **solver_kw)
     u_end = S.get_u()
     dofs = cfg["X"].size
     value = dofs * tot["converged"] / wall
@@ -377,7 +396,8 @@ def bench(S, cfg, args, warm_last, K):
         "config": {"workload": f"config {args.config}: {nx}x{ny}x{nz} hexes x6 Kuhn tets, {dofs} DOFs, "
                                f"load steps 1..{K} of {cfg['n_steps']}", "dofs": dofs, "tets": info["ne"],
                    "nnz": info["nnz"], "predictor": "extrapolate" if lift == 2 else "solve",
-                   "inner_tol": inner_tol, "newton_rtol": cfg["newton_rtol"]},
+                   "inner_tol": inner_tol, "inner_tol_max": inner_tol_max, "recycle": recycle,
+                   "newton_rtol": cfg["newton_rtol"]},
         "newton": {"converged_steps": tot["converged"], "newton_iters": tot["newton_iters"],
                    "pcg_iters": tot["pcg_iters"], "newton_it_per_s": tot["newton_iters"] / wall,
                    "pcg_it_per_s": tot["pcg_iters"] / wall, "worst_rel_res": worst,
@@ -414,14 +434,17 @@ def bench(S, cfg, args, warm_last, K):
         S.set_u(None)
         log(f"FP64 GPU, same algorithm: load steps 1..{K}")
         wall64, tot64, worst64, _ = run_steps(S, cfg, fm.FM_FP64, 1, K, inner_tol, False, args.pcg_max, lift,
-                                              inexact=1)
+                                              inexact=1, **solver_kw)
         u64 = S.get_u()
         out["fp64_gpu"] = {"value": dofs * tot64["converged"] / wall64, "unit": "DOFs/s",
                            "ms_per_step": 1e3 * wall64 / K, "newton_iters": tot64["newton_iters"],
                            "pcg_iters": tot64["pcg_iters"], "converged_steps": tot64["converged"],
                            "worst_rel_res": worst64,
-                           "algorithm": "the mixed run's: predictor, kept tangent, forcing term "
-                                        f"{inner_tol:g}, FP64 tangent, SpMV, ILU(0) and PCG"}
+                           "algorithm": "the mixed run's: extrapolating predictor, "
+                                        + (f"Galerkin start over {recycle} previous corrections, " if recycle else "")
+                                        + (f"forcing term clamp(0.5 target/||R||, {inner_tol:g}, {inner_tol_max:g})"
+                                           if inner_tol_max > inner_tol else f"forcing term {inner_tol:g}")
+                                        + ", with the FP64 tangent, SpMV, ILU(0) and PCG"}
         out["mixed_over_fp64"] = value / out["fp64_gpu"]["value"]
         out["mixed_vs_fp64_u_rel_err"] = float(np.abs(u_end - u64).max() / np.abs(u64).max())
         counts = {k: tot64[k] / K for k in ("assemblies", "residuals", "factorizations", "pcg_iters")}

@@ -170,6 +170,15 @@ typedef struct {
     int inexact;          /* FM_FP64 only: 1 = the FP64 inner solves take the mixed path's forcing
                              term (inner_tol, inner_max, normalised right-hand side) instead of the
                              reference's pcg_tol / pcg_max, so both precisions run one algorithm */
+    /* inner solves of FM_MIXED and inexact FM_FP64 (no counterpart in the reference, whose
+       pcg_solve_ilu0 takes its start x from the caller, :1283-1288) */
+    int recycle;          /* lift = 2: the first Newton solve of a load step starts from the
+                             Galerkin projection onto the first corrections of up to `recycle`
+                             (<= 3) previous load steps of the load path (the A-norm-best start in
+                             their span); 0 = x0 = 0 */
+    double inner_tol_max; /* > inner_tol: target-aware forcing term, each inner solve to
+                             clamp(0.5 newton_rtol R_0 / ||R_k||, inner_tol, inner_tol_max) (a solve
+                             that needs only a small reduction stops early); 0 = inner_tol throughout */
 } fm_newton_cfg;
 
 typedef struct {

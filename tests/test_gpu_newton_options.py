@@ -234,3 +234,37 @@ def test_repeated_matrix_pattern_uploads_values(fm, S, O):
         finally:
             F.close()
         assert itsf == its and np.array_equal(xf, x)
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_galerkin_start_and_target_forcing(fm, S, O, prec):
+    """recycle = 2 with the target-aware forcing term (inner_tol 1e-3, at most
+    0.1): every step still reaches ||R|| <= 1e-8 ||R_0||, the final state
+    matches the FP64 oracle's, and from the third step on the steps take
+    fewer PCG iterations than with the same forcing and x0 = 0.
+    A reset to the unloaded state clears the start basis with the rest of the
+    load-path history: the path then repeats bit for bit."""
+    cfg = cube(fm, n_steps=8, dims=(8, 8, 8))
+    fm.configs.load(S, cfg)
+    P = oracle(O, cfg, pcg_tol=1e-12, pcg_max=4000)
+    uo = np.zeros(cfg["X"].size)
+    for s in range(1, 9):
+        P.step(s, uo)
+    kw = dict(prec=fm.FM_MIXED) if prec == "mixed" else dict(prec=fm.FM_FP64, inexact=1)
+    plain = fm.configs.newton_cfg(cfg, lift=2, inner_tol=1e-3, inner_tol_max=0.1, recycle=0, **kw)
+    rec = fm.configs.newton_cfg(cfg, lift=2, inner_tol=1e-3, inner_tol_max=0.1, recycle=2, **kw)
+    runs = {}
+    for name, c in (("plain", plain), ("rec", rec), ("rec again", rec)):
+        S.set_u(None)
+        stats = []
+        for s in range(1, 9):
+            rc, st = S.newton_step(s, c)
+            assert st["converged"] and st["rel_res"] <= 1e-8 and st["pcg_cap_hits"] == 0, (name, s, st)
+            stats.append((st["newton_iters"], st["pcg_iters"], st["rel_res"]))
+        runs[name] = (S.get_u(), stats)
+        # both stop at ||R|| <= 1e-8 ||R_0|| (nu = 0.49): up to the tangent's
+        # condition number times that apart
+        assert rel(runs[name][0], uo) < 1e-6, name
+    assert np.array_equal(runs["rec"][0], runs["rec again"][0]) and runs["rec"][1] == runs["rec again"][1]
+    later = lambda name: sum(t[1] for t in runs[name][1][2:])
+    assert later("rec") < later("plain"), (runs["rec"][1], runs["plain"][1])
