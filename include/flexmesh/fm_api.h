@@ -144,7 +144,8 @@ typedef struct {
     double pcg_tol;       /* pcg_solve_ilu0 tol of the FP64 path */
     int pcg_max;
     double inner_tol;     /* FP32 PCG tol of the mixed path (inexact-Newton forcing term; default 3e-3,
-                             which suits lift = 1; 1e-2 suits lift = 2) */
+                             which suits lift = 1; 1e-2 suits lift = 2 alone, 1e-3 with recycle and
+                             inner_tol_max, where it is the smallest forcing term) */
     int ls_max;           /* Armijo backtracks (armijo_line_search_true, :933-985) */
     double armijo_c;
     int lift;             /* 1: predictor solve with the prescribed increment; 2: start from the
