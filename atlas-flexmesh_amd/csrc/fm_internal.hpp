@@ -46,7 +46,7 @@ constexpr int kReduceBlocks = 64;  // most blocks of one reduce (each sums a con
 struct PcgState {
     double rz, pAp, alpha, beta, bb, rz_new, relres, tol;
     double slot[4];
-    double gal[16];  // Gram matrix and right-hand side of a Galerkin start (reduce op 5)
+    double gal[48];  // Gram matrix, right-hand side and a norm of a Galerkin start (reduce op 5)
     int done;     // 1 once converged, broken down or out of iterations
     int status;   // 0 running, 1 converged, 2 breakdown (|pAp| < 1e-60), 3 max_it
     int iters;
@@ -155,12 +155,12 @@ struct Ctx {
     // state u at load factor sec_lf[0] (sec_n >= 1), sec_u1 at sec_lf[1]
     // (sec_n >= 2) and sec_u2 at sec_lf[2] (sec_n == 3); cleared by anything
     // that changes the problem or u
-    // normalised first Newton corrections of the latest load steps of this
-    // load path, per PCG precision ([0] FP64, [1] FP32): a ring of kGalMax,
-    // gal_n valid, gal_next the slot written next; they span the Galerkin
-    // start of a step's first linear solve (cleared with the history below)
-    DevBuf<double> gal64[3];
-    DevBuf<float> gal32[3];
+    // normalised Newton corrections of the latest load steps of this load
+    // path, per PCG precision ([0] FP64, [1] FP32): a ring of six, gal_n
+    // valid, gal_next the slot written next; they span the Galerkin start of
+    // a step's first linear solve (cleared with the history below)
+    DevBuf<double> gal64[6];
+    DevBuf<float> gal32[6];
     int gal_n[2] = {0, 0}, gal_next[2] = {0, 0};
     DevBuf<double> sec_u1, sec_u2, sec_start;
     double sec_lf[3] = {0.0, 0.0, 0.0};

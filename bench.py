@@ -308,8 +308,9 @@ def main():
                     help="largest forcing term of the target-aware rule clamp(0.5 target / ||R||, inner_tol, "
                          "inner_tol_max) (default 0.1 with the extrapolating predictor; 0: a fixed forcing term)")
     ap.add_argument("--recycle", type=int, default=None,
-                    help="previous load steps whose first Newton corrections span the Galerkin start of a "
-                         "step's first solve (default 2 with the extrapolating predictor; 0: x0 = 0)")
+                    help="previous load steps whose Newton corrections (the latest 2x this many) span the "
+                         "Galerkin start of a step's first solve (default 2 with the extrapolating "
+                         "predictor; 0: x0 = 0)")
     ap.add_argument("--pcg-max", type=int, default=1000, help="PCG iteration cap of the inexact Newton solves")
     ap.add_argument("--no-fp64", action="store_true", help="skip both FP64 GPU rows")
     ap.add_argument("--fp64-ref-steps", type=int, default=None,
@@ -364,7 +365,7 @@ def bench(S, cfg, args, warm_last, K):
     lift = 2 if args.predictor == "extrapolate" else 1
     # the inner solves: with the extrapolated start, the first solve of a
     # step starts from the Galerkin projection onto the previous two steps'
-    # first corrections and goes to 1e-3 (that start leaves it a few tens of
+    # Newton corrections and goes to 1e-3 (that start leaves it a few
     # iterations), and the second one stops at the target-aware forcing term
     # (about 0.1: it needs only a few-fold reduction); without them the
     # extrapolated start takes 1e-2 throughout, the predictor solve 3e-3
@@ -441,7 +442,8 @@ def bench(S, cfg, args, warm_last, K):
                            "pcg_iters": tot64["pcg_iters"], "converged_steps": tot64["converged"],
                            "worst_rel_res": worst64,
                            "algorithm": "the mixed run's: extrapolating predictor, "
-                                        + (f"Galerkin start over {recycle} previous corrections, " if recycle else "")
+                                        + (f"Galerkin start over the corrections of {recycle} previous steps, "
+                                           if recycle else "")
                                         + (f"forcing term clamp(0.5 target/||R||, {inner_tol:g}, {inner_tol_max:g})"
                                            if inner_tol_max > inner_tol else f"forcing term {inner_tol:g}")
                                         + ", with the FP64 tangent, SpMV, ILU(0) and PCG"}

@@ -174,8 +174,9 @@ typedef struct {
     /* inner solves of FM_MIXED and inexact FM_FP64 (no counterpart in the reference, whose
        pcg_solve_ilu0 takes its start x from the caller, :1283-1288) */
     int recycle;          /* lift = 2: the first Newton solve of a load step starts from the
-                             Galerkin projection onto the first corrections of up to `recycle`
-                             (<= 3) previous load steps of the load path (the A-norm-best start in
+                             Galerkin projection onto the latest 2 recycle (<= 6) Newton
+                             corrections of the load path, i.e. both corrections of each of the
+                             previous `recycle` steps in a steady run (the A-norm-best start in
                              their span); 0 = x0 = 0 */
     double inner_tol_max; /* > inner_tol: target-aware forcing term, each inner solve to
                              clamp(0.5 newton_rtol R_0 / ||R_k||, inner_tol, inner_tol_max) (a solve
