@@ -805,117 +805,160 @@ int pcg_run_pc(fm_ctx* ctx, Matrix& A, double tol, int max_it, PcgResult* res) {
 constexpr double kForcingShare = 0.5;
 
 // Galerkin start of a PCG solve A x = b (b, x: the PCG vectors of precision
-// p): x0 = V c with (V^T A V) c = V^T b over the m most recent basis vectors,
-// i.e. the A-norm-best start in span(V).  A vector whose pivot in the
-// Cholesky factor of V^T A V is not above 1e-10 of its diagonal adds nothing
-// new and is left out.
-constexpr int kGalMax = 6;
-static_assert(kGalMax * kGalMax + kGalMax + 1 <= 48, "PcgState::gal holds the Gram matrix, rhs and a norm");
-int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int m) {
+// p): x0 = V c with (V^T A V) c = V^T b over the chosen basis columns, i.e.
+// the A-norm-best start in their span.  Columns are taken in order (the ring
+// of recycled corrections, newest first, then the polynomial fields); one
+// whose pivot in the Cholesky factor of V^T A V is not above 1e-10 of its
+// diagonal adds nothing new and is left out.
+constexpr int kRing = 6;          // recycled corrections kept (basis columns 0..5)
+constexpr int kMonos[4] = {0, 4, 10, 20};  // monomials of total degree <= 0, 1, 2, 3
+
+// the basis matrices of precision p with room for the polynomial fields of
+// degree deg (their columns rebuilt when missing)
+int coarse_space(fm_ctx* ctx, int p, uint32_t n, int deg) {
+    const int pi = p == PREC64 ? 0 : 1;
+    const int nmono = kMonos[std::max(0, std::min(deg, 3))];
+    const int cols = kRing + 3 * nmono;
+    if (cols > ctx->cs_cols[pi] || (pi == 0 ? ctx->cs_V64.n : ctx->cs_V32.n) < (size_t)n * cols) {
+        const int c = std::max(cols, ctx->cs_cols[pi]);
+        if (pi == 0) {
+            FM_HIP(ctx->cs_V64.alloc((size_t)n * c));
+            FM_HIP(ctx->cs_W64.alloc((size_t)n * (c + 1)));
+        } else {
+            FM_HIP(ctx->cs_V32.alloc((size_t)n * c));
+            FM_HIP(ctx->cs_W32.alloc((size_t)n * (c + 1)));
+        }
+        ctx->cs_cols[pi] = c;
+        ctx->cs_poly_ok[pi] = false;
+        ctx->gal_n[pi] = 0;  // the ring's contents are gone
+    }
+    if (nmono && (!ctx->cs_poly_ok[pi] || ctx->cs_nmono[pi] < nmono)) {
+        // coordinates mapped to [-1, 1] over the mesh's bounding box
+        double center[3], scale[3];
+        for (int d = 0; d < 3; ++d) {
+            const double* x = ctx->hX.data() + (size_t)d * ctx->nn;
+            const auto mm = std::minmax_element(x, x + ctx->nn);
+            center[d] = 0.5 * (*mm.first + *mm.second);
+            scale[d] = *mm.second > *mm.first ? 2.0 / (*mm.second - *mm.first) : 1.0;
+        }
+        const int build = std::max(nmono, ctx->cs_nmono[pi]);
+        void* V = pi == 0 ? (void*)ctx->cs_V64.p : (void*)ctx->cs_V32.p;
+        launch_poly_basis(ctx->stream, p, ctx->nn, ctx->X.p, ctx->isdir.p, std::min(build, (ctx->cs_cols[pi] - kRing) / 3),
+                          center, scale, V, n, kRing);
+        ctx->cs_nmono[pi] = std::min(build, (ctx->cs_cols[pi] - kRing) / 3);
+        ctx->cs_poly_ok[pi] = true;
+    }
+    return FM_OK;
+}
+
+int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg) {
     hipStream_t s = ctx->stream;
     const uint32_t n = A.n;
     const int pi = p == PREC64 ? 0 : 1;
-    m = std::min({m, ctx->gal_n[pi], kGalMax});
+    FM_TRY(coarse_space(ctx, p, n, deg));
+    const int nmono = std::min(kMonos[std::max(0, std::min(deg, 3))], ctx->cs_nmono[pi]);
+    // the candidate columns, in order
+    int cand[kRing + 3 * 20], nc = 0;
+    ring_use = std::min({ring_use, ctx->gal_n[pi], kRing});
+    for (int j = 0; j < ring_use; ++j) cand[nc++] = (ctx->gal_next[pi] - 1 - j + 2 * kRing) % kRing;
+    for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < nmono; ++k) cand[nc++] = kRing + c * ctx->cs_nmono[pi] + k;
     void* b = vb(ctx, p, VB);
     void* x = vb(ctx, p, VX);
-    void* t = vb(ctx, p, VAP);
-    const void* V[kGalMax];
-    for (int j = 0; j < m; ++j) {
-        const int slot = (ctx->gal_next[pi] - 1 - j + 2 * kGalMax) % kGalMax;
-        V[j] = pi == 0 ? (const void*)ctx->gal64[slot].p : (const void*)ctx->gal32[slot].p;
+    const size_t es = p == PREC64 ? 8 : 4;
+    char* V = pi == 0 ? (char*)ctx->cs_V64.p : (char*)ctx->cs_V32.p;
+    char* W = pi == 0 ? (char*)ctx->cs_W64.p : (char*)ctx->cs_W32.p;
+    if (nc == 0) {
+        if (p == PREC64)
+            launch_fill_f64(s, (double*)x, n, 0.0);
+        else
+            launch_fill_f32(s, (float*)x, n, 0.f);
+        return FM_OK;
     }
-    FM_TRY(ensure_partials(ctx, n / 256 + 1));
-    for (int j = 0; j < m; ++j) {
-        launch_spmv(s, p, A, V[j], t, false, nullptr, nullptr, nullptr, nullptr);
-        for (int i = 0; i <= j; ++i) {
-            const uint32_t np = launch_dot(s, p, n, V[i], t, ctx->partials.p);
-            launch_reduce(s, ctx->partials.p, np, ctx->st.p, 5, i * kGalMax + j);
-        }
-        const uint32_t np = launch_dot(s, p, n, V[j], b, ctx->partials.p);
-        launch_reduce(s, ctx->partials.p, np, ctx->st.p, 5, kGalMax * kGalMax + j);
+    // the contiguous column range that holds the candidates; W column c1 is b
+    int c0 = cand[0], c1 = cand[0] + 1;
+    for (int q = 0; q < nc; ++q) {
+        c0 = std::min(c0, cand[q]);
+        c1 = std::max(c1, cand[q] + 1);
     }
-    FM_HIP(hipMemcpyAsync(ctx->h_st, ctx->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    const int m = c1 - c0;
+    for (int q = 0; q < nc; ++q)
+        launch_spmv(s, p, A, V + (size_t)cand[q] * n * es, W + (size_t)cand[q] * n * es, false, nullptr, nullptr,
+                    nullptr, nullptr);
+    FM_HIP(hipMemcpyAsync(W + (size_t)c1 * n * es, b, n * es, hipMemcpyDeviceToDevice, s));
+    FM_HIP(ctx->cs_partials.alloc(std::max<size_t>(ctx->cs_partials.n, (size_t)gram_blocks() * m * (m + 1))));
+    FM_HIP(ctx->cs_G.alloc(std::max<size_t>(ctx->cs_G.n, (size_t)m * (m + 1))));
+    launch_gram(s, p, n, m, m + 1, V + (size_t)c0 * n * es, W + (size_t)c0 * n * es, ctx->cs_partials.p, ctx->cs_G.p);
+    std::vector<double> G((size_t)m * (m + 1));
+    FM_HIP(hipMemcpyAsync(G.data(), ctx->cs_G.p, G.size() * 8, hipMemcpyDeviceToHost, s));
     FM_TRY(sync(ctx));
-    const double* g = ctx->h_st->gal;
-    // Cholesky of the Gram matrix over the kept vectors, then the two triangular solves
-    double L[kGalMax][kGalMax] = {}, y[kGalMax] = {}, c[kGalMax] = {};
-    int keep[kGalMax], k = 0;
-    for (int j = 0; j < m; ++j) {
-        double row[kGalMax];
+    auto g = [&](int i, int j) {  // symmetric part of V^T A V over V-column indices
+        const int a = i - c0, c = j - c0;
+        return 0.5 * (G[(size_t)a * (m + 1) + c] + G[(size_t)c * (m + 1) + a]);
+    };
+    // Cholesky over the kept candidates, then the two triangular solves
+    std::vector<double> L((size_t)nc * nc, 0.0), row(nc), y(nc), coef(nc);
+    std::vector<int> keep;
+    for (int q = 0; q < nc; ++q) {
+        const int j = cand[q];
+        const int k = (int)keep.size();
         for (int a = 0; a < k; ++a) {
-            const int i = keep[a];
-            double v = g[std::min(i, j) * kGalMax + std::max(i, j)];
-            for (int q = 0; q < a; ++q) v -= L[a][q] * row[q];
-            row[a] = v / L[a][a];
+            double v = g(keep[a], j);
+            for (int t = 0; t < a; ++t) v -= L[(size_t)a * nc + t] * row[t];
+            row[a] = v / L[(size_t)a * nc + a];
         }
-        double d = g[j * kGalMax + j];
+        double d = g(j, j);
         for (int a = 0; a < k; ++a) d -= row[a] * row[a];
-        if (!(d > 1e-10 * g[j * kGalMax + j])) continue;
-        for (int a = 0; a < k; ++a) L[k][a] = row[a];
-        L[k][k] = std::sqrt(d);
-        keep[k++] = j;
+        if (!(d > 1e-10 * g(j, j))) continue;
+        for (int a = 0; a < k; ++a) L[(size_t)k * nc + a] = row[a];
+        L[(size_t)k * nc + k] = std::sqrt(d);
+        keep.push_back(j);
     }
+    const int k = (int)keep.size();
     for (int a = 0; a < k; ++a) {
-        double v = g[kGalMax * kGalMax + keep[a]];
-        for (int q = 0; q < a; ++q) v -= L[a][q] * y[q];
-        y[a] = v / L[a][a];
+        double v = G[(size_t)(keep[a] - c0) * (m + 1) + m];
+        for (int t = 0; t < a; ++t) v -= L[(size_t)a * nc + t] * y[t];
+        y[a] = v / L[(size_t)a * nc + a];
     }
     for (int a = k - 1; a >= 0; --a) {
         double v = y[a];
-        for (int q = a + 1; q < k; ++q) v -= L[q][a] * c[q];
-        c[a] = v / L[a][a];
+        for (int t = a + 1; t < k; ++t) v -= L[(size_t)t * nc + a] * coef[t];
+        coef[a] = v / L[(size_t)a * nc + a];
     }
-    const void* W[kGalMax];
-    for (int a = 0; a < k; ++a) W[a] = V[keep[a]];
-    if (k) {
-        // x = sum c_a W_a, four terms per pass (later passes add x itself)
-        launch_combine(s, p, n, std::min(k, 4), c, W, x);
-        for (int a0 = 4; a0 < k; a0 += 3) {
-            double cc[4] = {1.0, 0.0, 0.0, 0.0};
-            const void* ww[4] = {x, nullptr, nullptr, nullptr};
-            const int m2 = std::min(3, k - a0);
-            for (int q = 0; q < m2; ++q) {
-                cc[1 + q] = c[a0 + q];
-                ww[1 + q] = W[a0 + q];
-            }
-            launch_combine(s, p, n, 1 + m2, cc, ww, x);
-        }
-    } else if (p == PREC64)
-        launch_fill_f64(s, (double*)x, n, 0.0);
-    else
-        launch_fill_f32(s, (float*)x, n, 0.f);
+    FM_HIP(ctx->cs_col.alloc(std::max<size_t>(ctx->cs_col.n, kRing + 60)));
+    FM_HIP(ctx->cs_c.alloc(std::max<size_t>(ctx->cs_c.n, kRing + 60)));
+    FM_HIP(hipMemcpyAsync(ctx->cs_col.p, keep.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
+    FM_HIP(hipMemcpyAsync(ctx->cs_c.p, coef.data(), k * 8, hipMemcpyHostToDevice, s));
+    launch_basis_combine(s, p, n, k, ctx->cs_col.p, ctx->cs_c.p, V, x);
+    FM_TRY(sync(ctx));  // the host arrays above are copied before they go out of scope
     return FM_OK;
 }
 
 // the solution of the last solve (the PCG vector x of precision p), normalised,
-// becomes the newest basis vector
-int galerkin_push(fm_ctx* ctx, uint32_t n, int p) {
+// becomes the newest ring column
+int galerkin_push(fm_ctx* ctx, uint32_t n, int p, int deg) {
     hipStream_t s = ctx->stream;
     const int pi = p == PREC64 ? 0 : 1;
+    FM_TRY(coarse_space(ctx, p, n, deg));
     const int slot = ctx->gal_next[pi];
-    void* dst;
-    if (pi == 0) {
-        FM_HIP(ctx->gal64[slot].alloc(n));
-        dst = ctx->gal64[slot].p;
-    } else {
-        FM_HIP(ctx->gal32[slot].alloc(n));
-        dst = ctx->gal32[slot].p;
-    }
+    const size_t es = p == PREC64 ? 8 : 4;
+    char* V = pi == 0 ? (char*)ctx->cs_V64.p : (char*)ctx->cs_V32.p;
     const void* x = vb(ctx, p, VX);
     FM_TRY(ensure_partials(ctx, n / 256 + 1));
     const uint32_t np = launch_dot(s, p, n, x, x, ctx->partials.p);
-    launch_reduce(s, ctx->partials.p, np, ctx->st.p, 5, kGalMax * kGalMax + kGalMax);
-    launch_normalize(s, p, n, &ctx->st.p->gal[kGalMax * kGalMax + kGalMax], x, dst);
-    ctx->gal_next[pi] = (slot + 1) % kGalMax;
-    ctx->gal_n[pi] = std::min(ctx->gal_n[pi] + 1, kGalMax);
+    launch_reduce(s, ctx->partials.p, np, ctx->st.p, 5, 47);
+    launch_normalize(s, p, n, &ctx->st.p->gal[47], x, V + (size_t)slot * n * es);
+    ctx->gal_next[pi] = (slot + 1) % kRing;
+    ctx->gal_n[pi] = std::min(ctx->gal_n[pi] + 1, kRing);
     return FM_OK;
 }
 
-// recycle > 0: the solve starts from the Galerkin start over that many of the
-// latest basis vectors (FP32 and inexact FP64 solves); push: its solution
-// becomes the newest basis vector
+// FP32 and inexact FP64 solves: ring_use > 0 or deg > 0: the solve starts
+// from the Galerkin start over that many of the latest recycled corrections
+// and the polynomial fields of degree <= deg; push: its solution joins the
+// recycled corrections
 int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double tol, int max_it, int* iters,
-                 bool inexact = false, int* capped = nullptr, int recycle = 0, bool push = false) {
+                 bool inexact = false, int* capped = nullptr, int ring_use = 0, int deg = 0, bool push = false) {
     Matrix& A = ctx->tangent;
     ctx->cur = &A;
     const uint32_t n = A.n;
@@ -950,12 +993,12 @@ int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double to
     FM_TRY(pcg_vectors(ctx, prec, n));
     if (prec == PREC64 && inexact) {
         launch_scale(ctx->stream, n, rhs, 1.0 / nr, ctx->b64.p);
-        if (recycle > 0 && ctx->gal_n[0] > 0)
-            FM_TRY(galerkin_start(ctx, A, PREC64, recycle));
+        if ((ring_use > 0 && ctx->gal_n[0] > 0) || deg > 0)
+            FM_TRY(galerkin_start(ctx, A, PREC64, ring_use, deg));
         else
             launch_fill_f64(ctx->stream, ctx->x64.p, n, 0.0);
         FM_TRY(pcg_run(ctx, A, PREC64, tol, max_it, &pr));
-        if (push) FM_TRY(galerkin_push(ctx, n, PREC64));
+        if (push) FM_TRY(galerkin_push(ctx, n, PREC64, deg));
         launch_scale(ctx->stream, n, ctx->x64.p, nr, du);
     } else if (prec == PREC64) {
         FM_HIP(hipMemcpyAsync(ctx->b64.p, rhs, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));
@@ -965,12 +1008,12 @@ int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double to
     } else {
         // FP32 inner solve on the normalised right-hand side
         launch_f64_to_f32(ctx->stream, n, rhs, 1.0 / nr, ctx->b32.p);
-        if (recycle > 0 && ctx->gal_n[1] > 0)
-            FM_TRY(galerkin_start(ctx, A, PREC32, recycle));
+        if ((ring_use > 0 && ctx->gal_n[1] > 0) || deg > 0)
+            FM_TRY(galerkin_start(ctx, A, PREC32, ring_use, deg));
         else
             launch_fill_f32(ctx->stream, ctx->x32.p, n, 0.f);
         FM_TRY(pcg_run(ctx, A, PREC32, tol, max_it, &pr));
-        if (push) FM_TRY(galerkin_push(ctx, n, PREC32));
+        if (push) FM_TRY(galerkin_push(ctx, n, PREC32, deg));
         launch_f32_to_f64(ctx->stream, n, ctx->x32.p, nr, du);
     }
     *iters = pr.iters;
@@ -1116,6 +1159,7 @@ int fm_set_mesh(fm_ctx* ctx, uint32_t n_nodes, const double* X, uint32_t n_elems
     if (!ctx || !X || !conn || !n_nodes || !n_elems) return fail(ctx, FM_EINVAL, "fm_set_mesh: bad arguments");
     ctx->sec_n = 0;
     ctx->gal_n[0] = ctx->gal_n[1] = 0;
+    ctx->cs_poly_ok[0] = ctx->cs_poly_ok[1] = false;  // the polynomial fields depend on X and the constraints
     if (3ull * n_nodes > 0xFFFFFFFFull) return fail(ctx, FM_EINVAL, "too many nodes");
     for (size_t k = 0; k < 4ull * n_elems; ++k)
         if (conn[k] >= n_nodes) return fail(ctx, FM_EINVAL, "connectivity refers to a node out of range");
@@ -1148,6 +1192,7 @@ int fm_set_coords(fm_ctx* ctx, uint32_t n_nodes, const double* X) {
     FM_HIP(hipMemcpy(ctx->X.p, X, 3ull * n_nodes * 8, hipMemcpyHostToDevice));
     ctx->sec_n = 0;
     ctx->gal_n[0] = ctx->gal_n[1] = 0;
+    ctx->cs_poly_ok[0] = ctx->cs_poly_ok[1] = false;  // the polynomial fields depend on X and the constraints
     Matrix& M = ctx->tangent;
     M.val_valid[0] = M.val_valid[1] = false;
     M.lu_valid[0] = M.lu_valid[1] = false;
@@ -1170,6 +1215,7 @@ int fm_set_bcs(fm_ctx* ctx, uint32_t n_dir, const uint32_t* dofs, const double* 
     if (!ctx || (n_dir && !dofs)) return fail(ctx, FM_EINVAL, "fm_set_bcs: bad arguments");
     ctx->sec_n = 0;
     ctx->gal_n[0] = ctx->gal_n[1] = 0;
+    ctx->cs_poly_ok[0] = ctx->cs_poly_ok[1] = false;  // the polynomial fields depend on X and the constraints
     if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
     const uint32_t n = 3 * ctx->nn;
     std::vector<uint8_t> mask(n, 0);
@@ -1587,6 +1633,7 @@ void fm_newton_cfg_default(fm_newton_cfg* c) {
     c->inexact = 0;
     c->recycle = 0;
     c->inner_tol_max = 0.0;
+    c->coarse_degree = 0;
 }
 
 int fm_set_u(fm_ctx* ctx, const double* u) {
@@ -1687,6 +1734,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     // the options of the inner solves: target-aware forcing, Galerkin start
     const bool adapt = (kprec == PREC32 || inexact) && cfg.inner_tol_max > lin_tol;
     const bool recycle = (kprec == PREC32 || inexact) && cfg.lift == 2 && cfg.recycle > 0;
+    const int coarse = (kprec == PREC32 || inexact) ? std::max(0, std::min(cfg.coarse_degree, 3)) : 0;
     const int lin_cap = kprec == PREC64 && !inexact ? cfg.pcg_max : cfg.inner_max;
     const double lf = load_factor;
     fm_newton_stats S{};
@@ -1753,7 +1801,8 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
                                ctx->partials.p);
         launch_sub_masked(s, n, ctx->R.p, ctx->lift.p, ctx->isdir.p, ctx->hv.p);
         int its = 0;
-        FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, lin_tol, lin_cap, &its, inexact, &S.pcg_cap_hits));
+        FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, lin_tol, lin_cap, &its, inexact, &S.pcg_cap_hits, 0,
+                            coarse, false));
         S.pcg_iters += its;
         S.factorizations++;
         launch_apply_predictor(s, n, ctx->du.p, ctx->dD.p, ctx->isdir.p, ctx->u.p);
@@ -1801,9 +1850,9 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         // the latest 2 recycle Newton corrections (both corrections of each of
         // the previous `recycle` steps in a steady run), and every correction
         // joins them
-        const int start_from = recycle && it == 0 ? std::min(2 * cfg.recycle, kGalMax) : 0;
+        const int start_from = recycle && it == 0 ? std::min(2 * cfg.recycle, kRing) : 0;
         FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, eta, lin_cap, &its, inexact, &S.pcg_cap_hits,
-                            start_from, recycle));
+                            start_from, coarse, recycle));
         S.pcg_iters += its;
         if (!keep) S.factorizations++;
         // armijo_line_search_true (FEM/adaptive_fem_engine.hpp:933-985); with

@@ -155,12 +155,20 @@ struct Ctx {
     // state u at load factor sec_lf[0] (sec_n >= 1), sec_u1 at sec_lf[1]
     // (sec_n >= 2) and sec_u2 at sec_lf[2] (sec_n == 3); cleared by anything
     // that changes the problem or u
-    // normalised Newton corrections of the latest load steps of this load
-    // path, per PCG precision ([0] FP64, [1] FP32): a ring of six, gal_n
-    // valid, gal_next the slot written next; they span the Galerkin start of
-    // a step's first linear solve (cleared with the history below)
-    DevBuf<double> gal64[6];
-    DevBuf<float> gal32[6];
+    // Galerkin-start basis per PCG precision ([0] FP64, [1] FP32): V is
+    // n x cs_cols (leading dimension n) with a ring of the latest six
+    // normalised Newton corrections of this load path in columns 0..5
+    // (gal_n valid, gal_next the slot written next; cleared with the history
+    // below) and the polynomial fields of degree <= cs_deg from column 6 on
+    // (cs_nmono monomials per component; rebuilt after a change of the mesh,
+    // the coordinates or the constrained DOFs); W holds K V and the
+    // right-hand side
+    DevBuf<double> cs_V64, cs_W64;
+    DevBuf<float> cs_V32, cs_W32;
+    int cs_cols[2] = {0, 0}, cs_nmono[2] = {0, 0};
+    bool cs_poly_ok[2] = {false, false};
+    DevBuf<double> cs_partials, cs_G, cs_c;
+    DevBuf<int> cs_col;
     int gal_n[2] = {0, 0}, gal_next[2] = {0, 0};
     DevBuf<double> sec_u1, sec_u2, sec_start;
     double sec_lf[3] = {0.0, 0.0, 0.0};
@@ -355,6 +363,20 @@ uint32_t launch_dot(hipStream_t s, int prec, uint32_t n, const void* a, const vo
 void launch_combine(hipStream_t s, int prec, uint32_t n, int m, const double* c, const void* const* v, void* out);
 // out = in / sqrt(*nrm2) (*nrm2 read on the device)
 void launch_normalize(hipStream_t s, int prec, uint32_t n, const double* nrm2, const void* in, void* out);
+// columns col0 .. col0 + 3 nmono - 1 of out (leading dimension ld = 3 nn): the
+// first nmono monomials (1, x, y, z, x^2, y^2, z^2, xy, yz, xz, cubics) of the
+// node coordinates mapped by (X - center) * scale, one displacement component
+// per group of nmono, zero on constrained DOFs
+void launch_poly_basis(hipStream_t s, int prec, uint32_t nn, const double* X, const uint8_t* isdir, int nmono,
+                       const double* center, const double* scale, void* out, uint32_t ld, int col0);
+// G (m x mw, row-major, FP64) = V^T W for m and mw columns of leading
+// dimension n, in a fixed order; partials: gram_blocks() * m * mw doubles
+uint32_t gram_blocks();
+void launch_gram(hipStream_t s, int prec, uint32_t n, int m, int mw, const void* V, const void* W, double* partials,
+                 double* G);
+// x = sum_q c[q] V[:, col[q]] (device arrays col, c of k entries)
+void launch_basis_combine(hipStream_t s, int prec, uint32_t n, int k, const int* col, const double* c, const void* V,
+                          void* x);
 // ZZ estimate: zz is kZzRows x ne scratch (colour order), raw and rec 3 nn,
 // err ne (element order); returns the number of partials of sum V sq / 4
 constexpr int kZzRows = 7;

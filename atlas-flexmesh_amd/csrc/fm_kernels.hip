@@ -1716,6 +1716,120 @@ __global__ void normalize_kernel(uint32_t n, const double* __restrict__ nrm2, co
     out[i] = (T)((double)in[i] * s);
 }
 
+// ---------------------------------------------------------------------------
+// coarse space of the Galerkin start: polynomial displacement fields, the
+// Gram matrix of a tall-skinny basis, and the start x0 = V c
+// ---------------------------------------------------------------------------
+constexpr int kMaxMono = 20;  // monomials of total degree <= 3 in three variables
+
+// column (col0 + c nmono + k) holds monomial k of the scaled node coordinates
+// in displacement component c, zero in the other components and on
+// constrained DOFs; one thread per node
+template <typename T>
+__global__ void poly_basis_kernel(uint32_t nn, const double* __restrict__ X, const uint8_t* __restrict__ isdir,
+                                  int nmono, double cx, double cy, double cz, double sx, double sy, double sz,
+                                  T* __restrict__ out, uint32_t ld, int col0) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nn) return;
+    const double x = (X[v] - cx) * sx, y = (X[nn + v] - cy) * sy, z = (X[2 * nn + v] - cz) * sz;
+    const double mono[kMaxMono] = {1.0,       x,         y,         z,         x * x,     y * y,     z * z,
+                                   x * y,     y * z,     x * z,     x * x * x, y * y * y, z * z * z, x * x * y,
+                                   x * x * z, y * y * x, y * y * z, z * z * x, z * z * y, x * y * z};
+    for (int c = 0; c < 3; ++c) {
+        const bool free = !isdir || !isdir[3 * v + c];
+        for (int k = 0; k < nmono; ++k) {
+            T* col = out + (size_t)(col0 + c * nmono + k) * ld + 3 * (size_t)v;
+            for (int d = 0; d < 3; ++d) col[d] = (d == c && free) ? (T)mono[k] : (T)0;
+        }
+    }
+}
+
+// block partials of G = V^T W (V: m columns, W: mw columns, both with leading
+// dimension n), FP64 accumulation.  A block takes a contiguous row range in
+// chunks of kGramRows rows staged in LDS; a thread owns 4x4 tiles of G.
+constexpr int kGramRows = 32;
+template <typename T>
+__global__ __launch_bounds__(256) void gram_kernel(uint32_t n, int m, int mw, const T* __restrict__ V,
+                                                   const T* __restrict__ W, double* __restrict__ partials) {
+    extern __shared__ __align__(16) unsigned char gram_smem[];
+    T* sV = reinterpret_cast<T*>(gram_smem);
+    T* sW = sV + (size_t)m * kGramRows;
+    const int ti_n = (m + 3) / 4, tj_n = (mw + 3) / 4, ntile = ti_n * tj_n;
+    const uint32_t per = ((n + gridDim.x - 1) / gridDim.x + kGramRows - 1) / kGramRows * kGramRows;
+    const uint32_t r0 = blockIdx.x * per, r1 = min(n, r0 + per);
+    constexpr int TPT = 2;  // tiles per thread (ntile <= 512)
+    double acc[TPT][4][4];
+#pragma unroll
+    for (int q = 0; q < TPT; ++q)
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[q][a][b] = 0.0;
+    for (uint32_t rc = r0; rc < r1; rc += kGramRows) {
+        for (int e = threadIdx.x; e < m * kGramRows; e += blockDim.x) {
+            const int j = e / kGramRows, r = e % kGramRows;
+            sV[e] = rc + r < r1 ? V[(size_t)j * n + rc + r] : (T)0;
+        }
+        for (int e = threadIdx.x; e < mw * kGramRows; e += blockDim.x) {
+            const int j = e / kGramRows, r = e % kGramRows;
+            sW[e] = rc + r < r1 ? W[(size_t)j * n + rc + r] : (T)0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            const int t = threadIdx.x + q * blockDim.x;
+            if (t < ntile) {
+                const int i0 = (t / tj_n) * 4, j0 = (t % tj_n) * 4;
+                for (int r = 0; r < kGramRows; ++r) {
+                    double a[4], b[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        a[u] = i0 + u < m ? (double)sV[(i0 + u) * kGramRows + r] : 0.0;
+                        b[u] = j0 + u < mw ? (double)sW[(j0 + u) * kGramRows + r] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int w = 0; w < 4; ++w) acc[q][u][w] += a[u] * b[w];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double* out = partials + (size_t)blockIdx.x * m * mw;
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+        const int t = threadIdx.x + q * blockDim.x;
+        if (t < ntile) {
+            const int i0 = (t / tj_n) * 4, j0 = (t % tj_n) * 4;
+            for (int u = 0; u < 4; ++u)
+                for (int w = 0; w < 4; ++w)
+                    if (i0 + u < m && j0 + w < mw) out[(i0 + u) * mw + j0 + w] = acc[q][u][w];
+        }
+    }
+}
+
+// G[p] = sum over the blocks of partials[b][p], in block order
+__global__ void gram_reduce_kernel(uint32_t nb, uint32_t np, const double* __restrict__ partials,
+                                   double* __restrict__ G) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    double s = 0.0;
+    for (uint32_t b = 0; b < nb; ++b) s += partials[(size_t)b * np + p];
+    G[p] = s;
+}
+
+// x = sum_k c[k] V[:, col[k]] (FP64 accumulation)
+template <typename T>
+__global__ void basis_combine_kernel(uint32_t n, int k, const int* __restrict__ col, const double* __restrict__ c,
+                                     const T* __restrict__ V, T* __restrict__ x) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int q = 0; q < k; ++q) acc += c[q] * (double)V[(size_t)col[q] * n + i];
+    x[i] = (T)acc;
+}
+
 __global__ void f64_to_f32_kernel(uint32_t n, const double* __restrict__ s, double scale, float* __restrict__ d) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) d[i] = (float)(s[i] * scale);
@@ -2405,6 +2519,39 @@ void launch_combine(hipStream_t s, int prec, uint32_t n, int m, const double* c,
         combine_kernel<double><<<cdiv(n, 256), 256, 0, s>>>(n, m, cb, (double*)out);
     else
         combine_kernel<float><<<cdiv(n, 256), 256, 0, s>>>(n, m, cb, (float*)out);
+}
+
+void launch_poly_basis(hipStream_t s, int prec, uint32_t nn, const double* X, const uint8_t* isdir, int nmono,
+                       const double* center, const double* scale, void* out, uint32_t ld, int col0) {
+    if (prec == PREC64)
+        poly_basis_kernel<double><<<cdiv(nn, 256), 256, 0, s>>>(nn, X, isdir, nmono, center[0], center[1], center[2],
+                                                               scale[0], scale[1], scale[2], (double*)out, ld, col0);
+    else
+        poly_basis_kernel<float><<<cdiv(nn, 256), 256, 0, s>>>(nn, X, isdir, nmono, center[0], center[1], center[2],
+                                                              scale[0], scale[1], scale[2], (float*)out, ld, col0);
+}
+
+uint32_t gram_blocks() { return 512; }
+
+void launch_gram(hipStream_t s, int prec, uint32_t n, int m, int mw, const void* V, const void* W, double* partials,
+                 double* G) {
+    const uint32_t nb = gram_blocks();
+    const size_t es = prec == PREC64 ? 8 : 4;
+    const size_t lds = (size_t)(m + mw) * kGramRows * es;
+    if (prec == PREC64)
+        gram_kernel<double><<<nb, 256, lds, s>>>(n, m, mw, (const double*)V, (const double*)W, partials);
+    else
+        gram_kernel<float><<<nb, 256, lds, s>>>(n, m, mw, (const float*)V, (const float*)W, partials);
+    const uint32_t np = (uint32_t)(m * mw);
+    gram_reduce_kernel<<<cdiv(np, 256), 256, 0, s>>>(nb, np, partials, G);
+}
+
+void launch_basis_combine(hipStream_t s, int prec, uint32_t n, int k, const int* col, const double* c, const void* V,
+                          void* x) {
+    if (prec == PREC64)
+        basis_combine_kernel<double><<<cdiv(n, 256), 256, 0, s>>>(n, k, col, c, (const double*)V, (double*)x);
+    else
+        basis_combine_kernel<float><<<cdiv(n, 256), 256, 0, s>>>(n, k, col, c, (const float*)V, (float*)x);
 }
 
 void launch_normalize(hipStream_t s, int prec, uint32_t n, const double* nrm2, const void* in, void* out) {

@@ -17,8 +17,9 @@ around the K steps brackets exactly the device work of the timed region.
 value = DOFs x (load steps converged) / wall seconds of the timed steps.
 After the timed region, on the same load steps from the unloaded state:
   * the plain FP64 GPU path running the same algorithm (extrapolating
-    predictor, Galerkin start of each step's first solve, target-aware
-    forcing term) -- the precision-only ratio mixed_over_fp64;
+    predictor, Galerkin start over recycled corrections and polynomial
+    fields, target-aware forcing term) -- the precision-only ratio
+    mixed_over_fp64;
   * the plain FP64 GPU path with the reference's solver settings (predictor
     solve, PCG tol 1e-10, 300-iteration cap), with its cap hits counted;
   * the FP64 CPU oracle on a bounded sample on the host cores (assembly,
@@ -307,6 +308,9 @@ def main():
     ap.add_argument("--inner-tol-max", type=float, default=None,
                     help="largest forcing term of the target-aware rule clamp(0.5 target / ||R||, inner_tol, "
                          "inner_tol_max) (default 0.1 with the extrapolating predictor; 0: a fixed forcing term)")
+    ap.add_argument("--coarse", type=int, default=None,
+                    help="degree of the polynomial displacement fields in the Galerkin start of every inner "
+                         "solve (default 3 with the extrapolating predictor; 0: none)")
     ap.add_argument("--recycle", type=int, default=None,
                     help="previous load steps whose Newton corrections (the latest 2x this many) span the "
                          "Galerkin start of a step's first solve (default 2 with the extrapolating "
@@ -363,21 +367,23 @@ def bench(S, cfg, args, warm_last, K):
     # the extrapolating predictor uses the previous converged steps of the
     # path (the unloaded state at load factor 0 among them)
     lift = 2 if args.predictor == "extrapolate" else 1
-    # the inner solves: with the extrapolated start, the first solve of a
-    # step starts from the Galerkin projection onto the previous two steps'
-    # Newton corrections and goes to 1e-3 (that start leaves it a few
-    # iterations), and the second one stops at the target-aware forcing term
+    # the inner solves: with the extrapolated start, every solve starts from
+    # the Galerkin projection onto the polynomial displacement fields of
+    # degree <= 3 (the smooth modes ILU(0) resolves slowly), the first solve
+    # of a step also onto the previous two steps' Newton corrections; the
+    # first goes to 1e-3, the second stops at the target-aware forcing term
     # (about 0.1: it needs only a few-fold reduction); without them the
     # extrapolated start takes 1e-2 throughout, the predictor solve 3e-3
     recycle = args.recycle if args.recycle is not None else (2 if lift == 2 else 0)
+    coarse = args.coarse if args.coarse is not None else (3 if lift == 2 else 0)
     inner_tol_max = args.inner_tol_max if args.inner_tol_max is not None else (0.1 if lift == 2 else 0.0)
     if args.inner_tol is not None:
         inner_tol = args.inner_tol
     elif lift == 2:
-        inner_tol = 1e-3 if (recycle or inner_tol_max) else 1e-2
+        inner_tol = 1e-3 if (recycle or inner_tol_max or coarse) else 1e-2
     else:
         inner_tol = 3e-3
-    solver_kw = dict(recycle=recycle, inner_tol_max=inner_tol_max)
+    solver_kw = dict(recycle=recycle, inner_tol_max=inner_tol_max, coarse_degree=coarse)
     if warm_last:
         log(f"warm-up pass: load steps 1..{warm_last}")
         run_steps(S, cfg, prec, 1, warm_last, inner_tol, False, args.pcg_max, lift, **solver_kw)
@@ -398,6 +404,7 @@ def bench(S, cfg, args, warm_last, K):
                                f"load steps 1..{K} of {cfg['n_steps']}", "dofs": dofs, "tets": info["ne"],
                    "nnz": info["nnz"], "predictor": "extrapolate" if lift == 2 else "solve",
                    "inner_tol": inner_tol, "inner_tol_max": inner_tol_max, "recycle": recycle,
+                   "coarse_degree": coarse,
                    "newton_rtol": cfg["newton_rtol"]},
         "newton": {"converged_steps": tot["converged"], "newton_iters": tot["newton_iters"],
                    "pcg_iters": tot["pcg_iters"], "newton_it_per_s": tot["newton_iters"] / wall,
@@ -442,8 +449,9 @@ def bench(S, cfg, args, warm_last, K):
                            "pcg_iters": tot64["pcg_iters"], "converged_steps": tot64["converged"],
                            "worst_rel_res": worst64,
                            "algorithm": "the mixed run's: extrapolating predictor, "
-                                        + (f"Galerkin start over the corrections of {recycle} previous steps, "
-                                           if recycle else "")
+                                        + (f"Galerkin start over the corrections of {recycle} previous steps"
+                                           + (f" and the polynomial fields of degree <= {coarse}" if coarse else "")
+                                           + ", " if (recycle or coarse) else "")
                                         + (f"forcing term clamp(0.5 target/||R||, {inner_tol:g}, {inner_tol_max:g})"
                                            if inner_tol_max > inner_tol else f"forcing term {inner_tol:g}")
                                         + ", with the FP64 tangent, SpMV, ILU(0) and PCG"}

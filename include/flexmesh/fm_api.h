@@ -181,6 +181,12 @@ typedef struct {
     double inner_tol_max; /* > inner_tol: target-aware forcing term, each inner solve to
                              clamp(0.5 newton_rtol R_0 / ||R_k||, inner_tol, inner_tol_max) (a solve
                              that needs only a small reduction stops early); 0 = inner_tol throughout */
+    int coarse_degree;    /* 1..3: every inner solve (and the predictor solve) starts from the
+                             Galerkin projection onto the polynomial displacement fields of total
+                             degree <= coarse_degree in the node coordinates (3, 12, 30 or 60
+                             fields, zero on constrained DOFs), together with the recycled
+                             corrections in a step's first solve: the smooth modes ILU(0) resolves
+                             slowly; 0 = off */
 } fm_newton_cfg;
 
 typedef struct {
