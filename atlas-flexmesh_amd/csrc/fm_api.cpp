@@ -71,8 +71,8 @@ struct ProfScope {
         // sampled profiling: the events put a bubble of a few microseconds
         // into the stream, so only every prof_every-th launch of the classes
         // launched once per PCG iteration is timed
-        const bool per_iteration =
-            kind == FM_K_SPMV || kind == FM_K_ILU_FWD || kind == FM_K_ILU_BWD || kind == FM_K_PCG_OTHER;
+        const bool per_iteration = kind == FM_K_SPMV || kind == FM_K_ILU_FWD || kind == FM_K_ILU_BWD ||
+                                   kind == FM_K_PCG_OTHER || kind == FM_K_COARSE;
         if (ctx->prof_seen[kind]++ % (uint64_t)(per_iteration ? ctx->prof_every : 1) != 0) return;
         if (ctx->ev_pool.size() < 2) {
             if (ctx->ev_pending.size() >= 1024) prof_flush(ctx);
@@ -644,10 +644,15 @@ int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult*
     ctx->h_st->tol = tol;
     ctx->h_st->max_it = max_it;
     FM_HIP(hipMemcpyAsync(ctx->st.p, ctx->h_st, sizeof(PcgState), hipMemcpyHostToDevice, s));
-    // z = M^{-1} r; rz; p = z
+    // z = M^{-1} r (+ Z E^{-1} Z^T r); rz; p = z
+    const int ad = ctx->ad_nmono;
+    if (ad) launch_coarse_correct(s, prec, n, ad, ctx->cs_xn.p, r, ctx->ad_Einv.p, ctx->ad_part.p, ctx->ad_coef.p, ctx->st.p);
     np = (uint32_t)precond_apply(ctx, A, prec, r, z, y, ctx->partials.p);
     launch_reduce(s, ctx->partials.p, np, ctx->st.p, 3, 0);
-    launch_copy(s, prec, n, y, p);
+    if (ad)
+        launch_pcg_p_update_coarse(s, prec, n, ad, y, p, ctx->cs_xn.p, ctx->ad_coef.p, true, ctx->st.p);
+    else
+        launch_copy(s, prec, n, y, p);
     FM_TRY(check_launch(ctx, "pcg init"));
     const void* lu = prec == PREC64 ? (const void*)A.lu64.p : (const void*)A.lu32.p;
     FM_TRY(pcg_batches(ctx, max_it, [&] {
@@ -678,8 +683,16 @@ int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult*
                           ? launch_chain_bwd(s, prec, A, z, r, y, ctx->partials.p, ctx->st.p, true, ctx->d_err.p)
                           : launch_ilu_bwd(s, prec, A, lu, z, r, y, ctx->partials.p, ctx->st.p, true, ctx->d_err.p);
             }
+            if (ad) {
+                ProfScope pc(ctx, FM_K_COARSE);
+                launch_coarse_correct(s, prec, n, ad, ctx->cs_xn.p, r, ctx->ad_Einv.p, ctx->ad_part.p,
+                                      ctx->ad_coef.p, ctx->st.p);
+            }
             launch_reduce(s, ctx->partials.p, nbb, ctx->st.p, 2, 0);
-            launch_pcg_p_update(s, prec, n, y, p, ctx->st.p);
+            if (ad)
+                launch_pcg_p_update_coarse(s, prec, n, ad, y, p, ctx->cs_xn.p, ctx->ad_coef.p, false, ctx->st.p);
+            else
+                launch_pcg_p_update(s, prec, n, y, p, ctx->st.p);
     }));
     FM_TRY(check_device_error(ctx));
     const PcgState& h = *ctx->h_st;
@@ -811,13 +824,15 @@ constexpr double kForcingShare = 0.5;
 // whose pivot in the Cholesky factor of V^T A V is not above 1e-10 of its
 // diagonal adds nothing new and is left out.
 constexpr int kRing = 6;          // recycled corrections kept (basis columns 0..5)
-constexpr int kMonos[4] = {0, 4, 10, 20};  // monomials of total degree <= 0, 1, 2, 3
+constexpr int kMaxDegree = 4;
+constexpr int kMonos[kMaxDegree + 1] = {0, 4, 10, 20, 35};  // monomials of total degree <= 0 (none), 1, 2, 3, 4
+static_assert(kRing + 3 * kMonos[kMaxDegree] + 1 <= 128, "the Gram kernel takes at most 128 columns");
 
 // the basis matrices of precision p with room for the polynomial fields of
 // degree deg (their columns rebuilt when missing)
 int coarse_space(fm_ctx* ctx, int p, uint32_t n, int deg) {
     const int pi = p == PREC64 ? 0 : 1;
-    const int nmono = kMonos[std::max(0, std::min(deg, 3))];
+    const int nmono = kMonos[std::max(0, std::min(deg, kMaxDegree))];
     const int cols = kRing + 3 * nmono;
     if (cols > ctx->cs_cols[pi] || (pi == 0 ? ctx->cs_V64.n : ctx->cs_V32.n) < (size_t)n * cols) {
         const int c = std::max(cols, ctx->cs_cols[pi]);
@@ -841,24 +856,52 @@ int coarse_space(fm_ctx* ctx, int p, uint32_t n, int deg) {
             center[d] = 0.5 * (*mm.first + *mm.second);
             scale[d] = *mm.second > *mm.first ? 2.0 / (*mm.second - *mm.first) : 1.0;
         }
-        const int build = std::max(nmono, ctx->cs_nmono[pi]);
+        FM_HIP(ctx->cs_xn.alloc(ctx->nn));
+        launch_coarse_coords(ctx->stream, ctx->nn, ctx->X.p, ctx->isdir.p, center, scale, ctx->cs_xn.p);
+        const int build = std::min(std::max(nmono, ctx->cs_nmono[pi]), (ctx->cs_cols[pi] - kRing) / 3);
         void* V = pi == 0 ? (void*)ctx->cs_V64.p : (void*)ctx->cs_V32.p;
-        launch_poly_basis(ctx->stream, p, ctx->nn, ctx->X.p, ctx->isdir.p, std::min(build, (ctx->cs_cols[pi] - kRing) / 3),
-                          center, scale, V, n, kRing);
-        ctx->cs_nmono[pi] = std::min(build, (ctx->cs_cols[pi] - kRing) / 3);
+        launch_poly_basis(ctx->stream, p, ctx->nn, ctx->cs_xn.p, build, build, V, n, kRing);
+        ctx->cs_nmono[pi] = build;
         ctx->cs_poly_ok[pi] = true;
     }
     return FM_OK;
 }
 
-int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg) {
+// Cholesky factor of the symmetric matrix g over the columns cols (in
+// order), leaving out a column whose pivot is not above 1e-10 of its
+// diagonal; L (nc x nc, row-major) holds the kept columns' rows
+template <typename G>
+void pivoted_cholesky(const G& g, const int* cols, int nc, std::vector<double>& L, std::vector<int>& keep) {
+    L.assign((size_t)nc * nc, 0.0);
+    keep.clear();
+    std::vector<double> row(nc);
+    for (int q = 0; q < nc; ++q) {
+        const int j = cols[q];
+        const int k = (int)keep.size();
+        for (int a = 0; a < k; ++a) {
+            double v = g(keep[a], j);
+            for (int t = 0; t < a; ++t) v -= L[(size_t)a * nc + t] * row[t];
+            row[a] = v / L[(size_t)a * nc + a];
+        }
+        double d = g(j, j);
+        for (int a = 0; a < k; ++a) d -= row[a] * row[a];
+        if (!(d > 1e-10 * g(j, j))) continue;
+        for (int a = 0; a < k; ++a) L[(size_t)k * nc + a] = row[a];
+        L[(size_t)k * nc + k] = std::sqrt(d);
+        keep.push_back(j);
+    }
+}
+
+// additive: also E^{-1} over the polynomial fields for the additive coarse
+// correction of the PCG that follows (ctx->ad_nmono)
+int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool additive) {
     hipStream_t s = ctx->stream;
     const uint32_t n = A.n;
     const int pi = p == PREC64 ? 0 : 1;
     FM_TRY(coarse_space(ctx, p, n, deg));
-    const int nmono = std::min(kMonos[std::max(0, std::min(deg, 3))], ctx->cs_nmono[pi]);
+    const int nmono = std::min(kMonos[std::max(0, std::min(deg, kMaxDegree))], ctx->cs_nmono[pi]);
     // the candidate columns, in order
-    int cand[kRing + 3 * 20], nc = 0;
+    int cand[kRing + 3 * kMonos[kMaxDegree]], nc = 0;
     ring_use = std::min({ring_use, ctx->gal_n[pi], kRing});
     for (int j = 0; j < ring_use; ++j) cand[nc++] = (ctx->gal_next[pi] - 1 - j + 2 * kRing) % kRing;
     for (int c = 0; c < 3; ++c)
@@ -882,13 +925,22 @@ int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg) {
         c1 = std::max(c1, cand[q] + 1);
     }
     const int m = c1 - c0;
-    for (int q = 0; q < nc; ++q)
-        launch_spmv(s, p, A, V + (size_t)cand[q] * n * es, W + (size_t)cand[q] * n * es, false, nullptr, nullptr,
-                    nullptr, nullptr);
-    FM_HIP(hipMemcpyAsync(W + (size_t)c1 * n * es, b, n * es, hipMemcpyDeviceToDevice, s));
+    // the products: the polynomial fields in one pass over a node-blocked
+    // matrix, the recycled corrections one by one
+    const bool spmm = A.node_spmv && nmono > 0;
     FM_HIP(ctx->cs_partials.alloc(std::max<size_t>(ctx->cs_partials.n, (size_t)gram_blocks() * m * (m + 1))));
     FM_HIP(ctx->cs_G.alloc(std::max<size_t>(ctx->cs_G.n, (size_t)m * (m + 1))));
-    launch_gram(s, p, n, m, m + 1, V + (size_t)c0 * n * es, W + (size_t)c0 * n * es, ctx->cs_partials.p, ctx->cs_G.p);
+    {
+        ProfScope ps(ctx, FM_K_GALERKIN);
+        if (spmm) launch_poly_spmm(s, p, A, ctx->cs_xn.p, nmono, ctx->cs_nmono[pi], W, kRing);
+        for (int q = 0; q < nc; ++q)
+            if (!spmm || cand[q] < kRing)
+                launch_spmv(s, p, A, V + (size_t)cand[q] * n * es, W + (size_t)cand[q] * n * es, false, nullptr,
+                            nullptr, nullptr, nullptr);
+        FM_HIP(hipMemcpyAsync(W + (size_t)c1 * n * es, b, n * es, hipMemcpyDeviceToDevice, s));
+        launch_gram(s, p, n, m, m + 1, V + (size_t)c0 * n * es, W + (size_t)c0 * n * es, ctx->cs_partials.p,
+                    ctx->cs_G.p);
+    }
     std::vector<double> G((size_t)m * (m + 1));
     FM_HIP(hipMemcpyAsync(G.data(), ctx->cs_G.p, G.size() * 8, hipMemcpyDeviceToHost, s));
     FM_TRY(sync(ctx));
@@ -897,23 +949,9 @@ int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg) {
         return 0.5 * (G[(size_t)a * (m + 1) + c] + G[(size_t)c * (m + 1) + a]);
     };
     // Cholesky over the kept candidates, then the two triangular solves
-    std::vector<double> L((size_t)nc * nc, 0.0), row(nc), y(nc), coef(nc);
+    std::vector<double> L, y(nc), coef(nc);
     std::vector<int> keep;
-    for (int q = 0; q < nc; ++q) {
-        const int j = cand[q];
-        const int k = (int)keep.size();
-        for (int a = 0; a < k; ++a) {
-            double v = g(keep[a], j);
-            for (int t = 0; t < a; ++t) v -= L[(size_t)a * nc + t] * row[t];
-            row[a] = v / L[(size_t)a * nc + a];
-        }
-        double d = g(j, j);
-        for (int a = 0; a < k; ++a) d -= row[a] * row[a];
-        if (!(d > 1e-10 * g(j, j))) continue;
-        for (int a = 0; a < k; ++a) L[(size_t)k * nc + a] = row[a];
-        L[(size_t)k * nc + k] = std::sqrt(d);
-        keep.push_back(j);
-    }
+    pivoted_cholesky(g, cand, nc, L, keep);
     const int k = (int)keep.size();
     for (int a = 0; a < k; ++a) {
         double v = G[(size_t)(keep[a] - c0) * (m + 1) + m];
@@ -925,11 +963,47 @@ int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg) {
         for (int t = a + 1; t < k; ++t) v -= L[(size_t)t * nc + a] * coef[t];
         coef[a] = v / L[(size_t)a * nc + a];
     }
-    FM_HIP(ctx->cs_col.alloc(std::max<size_t>(ctx->cs_col.n, kRing + 60)));
-    FM_HIP(ctx->cs_c.alloc(std::max<size_t>(ctx->cs_c.n, kRing + 60)));
+    FM_HIP(ctx->cs_col.alloc(std::max<size_t>(ctx->cs_col.n, kRing + 3 * kMonos[kMaxDegree])));
+    FM_HIP(ctx->cs_c.alloc(std::max<size_t>(ctx->cs_c.n, kRing + 3 * kMonos[kMaxDegree])));
     FM_HIP(hipMemcpyAsync(ctx->cs_col.p, keep.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
     FM_HIP(hipMemcpyAsync(ctx->cs_c.p, coef.data(), k * 8, hipMemcpyHostToDevice, s));
     launch_basis_combine(s, p, n, k, ctx->cs_col.p, ctx->cs_c.p, V, x);
+    // E^{-1} = (Z^T A Z)^{-1} over the polynomial fields (index c nmono + k),
+    // from the same Gram entries; a left-out field gets a zero row and column
+    const int ma = 3 * nmono;
+    std::vector<double> Einv((size_t)ma * ma, 0.0);
+    ctx->ad_nmono = 0;
+    if (additive && ma > 0) {
+        int pc[3 * kMonos[kMaxDegree]];
+        for (int c = 0; c < 3; ++c)
+            for (int q = 0; q < nmono; ++q) pc[c * nmono + q] = kRing + c * ctx->cs_nmono[pi] + q;
+        std::vector<double> LE;
+        std::vector<int> kept;
+        pivoted_cholesky(g, pc, ma, LE, kept);
+        const int ke = (int)kept.size();
+        // inverse of the kept block: columns of L^{-T} L^{-1}, one unit vector at a time
+        std::vector<double> e(ke), w(ke);
+        std::vector<int> pos(ke);
+        for (int a = 0; a < ke; ++a) pos[a] = (int)(std::find(pc, pc + ma, kept[a]) - pc);
+        for (int col = 0; col < ke; ++col) {
+            for (int a = 0; a < ke; ++a) {
+                double v = a == col ? 1.0 : 0.0;
+                for (int t = 0; t < a; ++t) v -= LE[(size_t)a * ma + t] * e[t];
+                e[a] = v / LE[(size_t)a * ma + a];
+            }
+            for (int a = ke - 1; a >= 0; --a) {
+                double v = e[a];
+                for (int t = a + 1; t < ke; ++t) v -= LE[(size_t)t * ma + a] * w[t];
+                w[a] = v / LE[(size_t)a * ma + a];
+            }
+            for (int a = 0; a < ke; ++a) Einv[(size_t)pos[a] * ma + pos[col]] = w[a];
+        }
+        FM_HIP(ctx->ad_Einv.alloc(std::max<size_t>(ctx->ad_Einv.n, (size_t)ma * ma)));
+        FM_HIP(ctx->ad_coef.alloc(std::max<size_t>(ctx->ad_coef.n, (size_t)ma)));
+        FM_HIP(ctx->ad_part.alloc(std::max<size_t>(ctx->ad_part.n, (size_t)coarse_moment_blocks() * ma)));
+        FM_HIP(hipMemcpyAsync(ctx->ad_Einv.p, Einv.data(), Einv.size() * 8, hipMemcpyHostToDevice, s));
+        ctx->ad_nmono = nmono;
+    }
     FM_TRY(sync(ctx));  // the host arrays above are copied before they go out of scope
     return FM_OK;
 }
@@ -958,7 +1032,9 @@ int galerkin_push(fm_ctx* ctx, uint32_t n, int p, int deg) {
 // and the polynomial fields of degree <= deg; push: its solution joins the
 // recycled corrections
 int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double tol, int max_it, int* iters,
-                 bool inexact = false, int* capped = nullptr, int ring_use = 0, int deg = 0, bool push = false) {
+                 bool inexact = false, int* capped = nullptr, int ring_use = 0, int deg = 0, bool push = false,
+                 bool additive = false) {
+    ctx->ad_nmono = 0;
     Matrix& A = ctx->tangent;
     ctx->cur = &A;
     const uint32_t n = A.n;
@@ -994,10 +1070,12 @@ int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double to
     if (prec == PREC64 && inexact) {
         launch_scale(ctx->stream, n, rhs, 1.0 / nr, ctx->b64.p);
         if ((ring_use > 0 && ctx->gal_n[0] > 0) || deg > 0)
-            FM_TRY(galerkin_start(ctx, A, PREC64, ring_use, deg));
+            FM_TRY(galerkin_start(ctx, A, PREC64, ring_use, deg, additive));
         else
             launch_fill_f64(ctx->stream, ctx->x64.p, n, 0.0);
-        FM_TRY(pcg_run(ctx, A, PREC64, tol, max_it, &pr));
+        const int prc = pcg_run(ctx, A, PREC64, tol, max_it, &pr);
+        ctx->ad_nmono = 0;
+        FM_TRY(prc);
         if (push) FM_TRY(galerkin_push(ctx, n, PREC64, deg));
         launch_scale(ctx->stream, n, ctx->x64.p, nr, du);
     } else if (prec == PREC64) {
@@ -1009,10 +1087,12 @@ int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double to
         // FP32 inner solve on the normalised right-hand side
         launch_f64_to_f32(ctx->stream, n, rhs, 1.0 / nr, ctx->b32.p);
         if ((ring_use > 0 && ctx->gal_n[1] > 0) || deg > 0)
-            FM_TRY(galerkin_start(ctx, A, PREC32, ring_use, deg));
+            FM_TRY(galerkin_start(ctx, A, PREC32, ring_use, deg, additive));
         else
             launch_fill_f32(ctx->stream, ctx->x32.p, n, 0.f);
-        FM_TRY(pcg_run(ctx, A, PREC32, tol, max_it, &pr));
+        const int prc = pcg_run(ctx, A, PREC32, tol, max_it, &pr);
+        ctx->ad_nmono = 0;
+        FM_TRY(prc);
         if (push) FM_TRY(galerkin_push(ctx, n, PREC32, deg));
         launch_f32_to_f64(ctx->stream, n, ctx->x32.p, nr, du);
     }
@@ -1634,6 +1714,7 @@ void fm_newton_cfg_default(fm_newton_cfg* c) {
     c->recycle = 0;
     c->inner_tol_max = 0.0;
     c->coarse_degree = 0;
+    c->coarse_additive = 0;
 }
 
 int fm_set_u(fm_ctx* ctx, const double* u) {
@@ -1734,7 +1815,8 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     // the options of the inner solves: target-aware forcing, Galerkin start
     const bool adapt = (kprec == PREC32 || inexact) && cfg.inner_tol_max > lin_tol;
     const bool recycle = (kprec == PREC32 || inexact) && cfg.lift == 2 && cfg.recycle > 0;
-    const int coarse = (kprec == PREC32 || inexact) ? std::max(0, std::min(cfg.coarse_degree, 3)) : 0;
+    const int coarse = (kprec == PREC32 || inexact) ? std::max(0, std::min(cfg.coarse_degree, kMaxDegree)) : 0;
+    const bool additive = coarse > 0 && cfg.coarse_additive != 0;
     const int lin_cap = kprec == PREC64 && !inexact ? cfg.pcg_max : cfg.inner_max;
     const double lf = load_factor;
     fm_newton_stats S{};
@@ -1802,7 +1884,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         launch_sub_masked(s, n, ctx->R.p, ctx->lift.p, ctx->isdir.p, ctx->hv.p);
         int its = 0;
         FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, lin_tol, lin_cap, &its, inexact, &S.pcg_cap_hits, 0,
-                            coarse, false));
+                            coarse, false, additive));
         S.pcg_iters += its;
         S.factorizations++;
         launch_apply_predictor(s, n, ctx->du.p, ctx->dD.p, ctx->isdir.p, ctx->u.p);
@@ -1852,7 +1934,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         // joins them
         const int start_from = recycle && it == 0 ? std::min(2 * cfg.recycle, kRing) : 0;
         FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, eta, lin_cap, &its, inexact, &S.pcg_cap_hits,
-                            start_from, coarse, recycle));
+                            start_from, coarse, recycle, additive));
         S.pcg_iters += its;
         if (!keep) S.factorizations++;
         // armijo_line_search_true (FEM/adaptive_fem_engine.hpp:933-985); with

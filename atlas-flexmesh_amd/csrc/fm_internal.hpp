@@ -47,6 +47,7 @@ struct PcgState {
     double rz, pAp, alpha, beta, bb, rz_new, relres, tol;
     double slot[4];
     double gal[48];  // Gram matrix, right-hand side and a norm of a Galerkin start (reduce op 5)
+    double rz_coarse;  // r.(Z E^{-1} Z^T r) of the additive coarse correction, added to r.z (0 without one)
     int done;     // 1 once converged, broken down or out of iterations
     int status;   // 0 running, 1 converged, 2 breakdown (|pAp| < 1e-60), 3 max_it
     int iters;
@@ -169,6 +170,13 @@ struct Ctx {
     bool cs_poly_ok[2] = {false, false};
     DevBuf<double> cs_partials, cs_G, cs_c;
     DevBuf<int> cs_col;
+    // the scaled node coordinates of the polynomial fields (built with them)
+    // and the additive coarse correction of the running PCG: E^{-1} over the
+    // 3 ad_nmono polynomial fields (ad_nmono = 0: none), its moment partials
+    // and coefficients
+    DevBuf<float4> cs_xn;
+    DevBuf<double> ad_Einv, ad_part, ad_coef;
+    int ad_nmono = 0;
     int gal_n[2] = {0, 0}, gal_next[2] = {0, 0};
     DevBuf<double> sec_u1, sec_u2, sec_start;
     double sec_lf[3] = {0.0, 0.0, 0.0};
@@ -363,14 +371,31 @@ uint32_t launch_dot(hipStream_t s, int prec, uint32_t n, const void* a, const vo
 void launch_combine(hipStream_t s, int prec, uint32_t n, int m, const double* c, const void* const* v, void* out);
 // out = in / sqrt(*nrm2) (*nrm2 read on the device)
 void launch_normalize(hipStream_t s, int prec, uint32_t n, const double* nrm2, const void* in, void* out);
-// columns col0 .. col0 + 3 nmono - 1 of out (leading dimension ld = 3 nn): the
-// first nmono monomials (1, x, y, z, x^2, y^2, z^2, xy, yz, xz, cubics) of the
-// node coordinates mapped by (X - center) * scale, one displacement component
-// per group of nmono, zero on constrained DOFs
-void launch_poly_basis(hipStream_t s, int prec, uint32_t nn, const double* X, const uint8_t* isdir, int nmono,
-                       const double* center, const double* scale, void* out, uint32_t ld, int col0);
-// G (m x mw, row-major, FP64) = V^T W for m and mw columns of leading
-// dimension n, in a fixed order; partials: gram_blocks() * m * mw doubles
+// xn[v]: node coordinates mapped by (X - center) * scale (FP32) and, in .w,
+// the bit mask of the node's constrained components
+void launch_coarse_coords(hipStream_t s, uint32_t nn, const double* X, const uint8_t* isdir, const double* center,
+                          const double* scale, float4* xn);
+// columns col0 + c ldm + k (k < nmono, c < 3) of out (leading dimension
+// ld = 3 nn): monomial k (1, x, y, z, x^2, y^2, z^2, xy, yz, xz, cubics,
+// quartics) of the scaled coordinates xn in displacement component c, zero
+// on constrained DOFs
+void launch_poly_basis(hipStream_t s, int prec, uint32_t nn, const float4* xn, int nmono, int ldm, void* out,
+                       uint32_t ld, int col0);
+// the same columns of W = A Z for the node-blocked matrix A, in one pass
+void launch_poly_spmm(hipStream_t s, int prec, const Matrix& A, const float4* xn, int nmono, int ldm, void* W,
+                      int col0);
+// additive coarse correction: moments g = Z^T r (partials:
+// coarse_moment_blocks() * 3 nmono doubles), coef = Einv g (Einv 3 nmono
+// square, row-major) and st->rz_coarse = g.coef; a no-op once st->done
+uint32_t coarse_moment_blocks();
+void launch_coarse_correct(hipStream_t s, int prec, uint32_t n, int nmono, const float4* xn, const void* r,
+                           const double* Einv, double* partials, double* coef, PcgState* st);
+// p = y + Z coef + beta p (first: p = y + Z coef)
+void launch_pcg_p_update_coarse(hipStream_t s, int prec, uint32_t n, int nmono, const void* y, void* p,
+                                const float4* xn, const double* coef, bool first, const PcgState* st);
+// G (m x mw, row-major, FP64) = V^T W for m and mw (both <= 128) columns of
+// leading dimension n, in a fixed order; partials: gram_blocks() * m * mw
+// doubles
 uint32_t gram_blocks();
 void launch_gram(hipStream_t s, int prec, uint32_t n, int m, int mw, const void* V, const void* W, double* partials,
                  double* G);

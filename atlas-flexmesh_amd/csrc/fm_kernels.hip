@@ -1542,6 +1542,7 @@ __global__ __launch_bounds__(1024) void reduce_kernel(const double* __restrict__
             break;
         case 2: {  // r.z -> stopping rule sqrt(rz/bb) < tol, beta
             if (st->done) return;
+            s += st->rz_coarse;  // r.(Z E^{-1} Z^T r) of the additive coarse correction (0 without)
             st->rz_new = s;
             st->iters += 1;
             const double rel = sqrt(s / fmax(st->bb, 1e-300));
@@ -1559,6 +1560,7 @@ __global__ __launch_bounds__(1024) void reduce_kernel(const double* __restrict__
             break;
         }
         case 3:
+            s += st->rz_coarse;
             st->rz = s;
             st->relres = sqrt(s / fmax(st->bb, 1e-300));
             break;
@@ -1720,35 +1722,288 @@ __global__ void normalize_kernel(uint32_t n, const double* __restrict__ nrm2, co
 // coarse space of the Galerkin start: polynomial displacement fields, the
 // Gram matrix of a tall-skinny basis, and the start x0 = V c
 // ---------------------------------------------------------------------------
-constexpr int kMaxMono = 20;  // monomials of total degree <= 3 in three variables
+constexpr int kMaxMono = 35;  // monomials of total degree <= 4 in three variables
 
-// column (col0 + c nmono + k) holds monomial k of the scaled node coordinates
-// in displacement component c, zero in the other components and on
-// constrained DOFs; one thread per node
+// Monomial k of total degree <= 4 in (x, y, z): exponents packed as
+// ex | ey << 3 | ez << 6, by degree (1; x, y, z; x^2, y^2, z^2, xy, yz, xz;
+// the cubics; the quartics)
+__host__ __device__ constexpr int mono_code(int k) {
+    switch (k) {
+        case 0: return 0;
+        case 1: return 1;
+        case 2: return 1 << 3;
+        case 3: return 1 << 6;
+        case 4: return 2;
+        case 5: return 2 << 3;
+        case 6: return 2 << 6;
+        case 7: return 1 | 1 << 3;
+        case 8: return 1 << 3 | 1 << 6;
+        case 9: return 1 | 1 << 6;
+        case 10: return 3;
+        case 11: return 3 << 3;
+        case 12: return 3 << 6;
+        case 13: return 2 | 1 << 3;
+        case 14: return 2 | 1 << 6;
+        case 15: return 1 | 2 << 3;
+        case 16: return 2 << 3 | 1 << 6;
+        case 17: return 1 | 2 << 6;
+        case 18: return 1 << 3 | 2 << 6;
+        case 19: return 1 | 1 << 3 | 1 << 6;
+        case 20: return 4;
+        case 21: return 4 << 3;
+        case 22: return 4 << 6;
+        case 23: return 3 | 1 << 3;
+        case 24: return 3 | 1 << 6;
+        case 25: return 1 | 3 << 3;
+        case 26: return 3 << 3 | 1 << 6;
+        case 27: return 1 | 3 << 6;
+        case 28: return 1 << 3 | 3 << 6;
+        case 29: return 2 | 2 << 3;
+        case 30: return 2 << 3 | 2 << 6;
+        case 31: return 2 | 2 << 6;
+        case 32: return 2 | 1 << 3 | 1 << 6;
+        case 33: return 1 | 2 << 3 | 1 << 6;
+        case 34: return 1 | 1 << 3 | 2 << 6;
+        default: return 0;
+    }
+}
+__constant__ int kMonoCodeDev[kMaxMono] = {
+    mono_code(0),  mono_code(1),  mono_code(2),  mono_code(3),  mono_code(4),  mono_code(5),  mono_code(6),
+    mono_code(7),  mono_code(8),  mono_code(9),  mono_code(10), mono_code(11), mono_code(12), mono_code(13),
+    mono_code(14), mono_code(15), mono_code(16), mono_code(17), mono_code(18), mono_code(19), mono_code(20),
+    mono_code(21), mono_code(22), mono_code(23), mono_code(24), mono_code(25), mono_code(26), mono_code(27),
+    mono_code(28), mono_code(29), mono_code(30), mono_code(31), mono_code(32), mono_code(33), mono_code(34)};
+
+// powers 1, x, .., x^4 and their product for one monomial: every kernel
+// evaluates a monomial as (x^ex y^ey) z^ez from these, so the basis columns,
+// the products with the matrix and the coarse correction see the same bits
 template <typename T>
-__global__ void poly_basis_kernel(uint32_t nn, const double* __restrict__ X, const uint8_t* __restrict__ isdir,
-                                  int nmono, double cx, double cy, double cz, double sx, double sy, double sz,
+__device__ __forceinline__ void powers5(T a, T* q) {
+    q[0] = (T)1;
+    q[1] = a;
+    q[2] = a * a;
+    q[3] = q[2] * a;
+    q[4] = q[2] * q[2];
+}
+template <typename T>
+struct Powers {
+    T x[5], y[5], z[5];
+    __device__ __forceinline__ explicit Powers(float4 c) {
+        powers5((T)c.x, x);
+        powers5((T)c.y, y);
+        powers5((T)c.z, z);
+    }
+    // compile-time exponents (unrolled loops)
+    __device__ __forceinline__ T mono(int code) const { return (x[code & 7] * y[(code >> 3) & 7]) * z[code >> 6]; }
+    // run-time exponents, selected without indexing the register arrays
+    __device__ __forceinline__ static T pick(const T* q, int e) {
+        return e == 0 ? q[0] : e == 1 ? q[1] : e == 2 ? q[2] : e == 3 ? q[3] : q[4];
+    }
+    __device__ __forceinline__ T mono_rt(int code) const {
+        return (pick(x, code & 7) * pick(y, (code >> 3) & 7)) * pick(z, code >> 6);
+    }
+};
+
+// xn[v] = ((X_v - center) * scale, mask): the node coordinates mapped to
+// [-1, 1] over the bounding box, in FP32, and in .w the bit mask of the
+// node's constrained components (bit c: component c)
+__global__ void coarse_coords_kernel(uint32_t nn, const double* __restrict__ X, const uint8_t* __restrict__ isdir,
+                                     double cx, double cy, double cz, double sx, double sy, double sz,
+                                     float4* __restrict__ xn) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nn) return;
+    int mask = 0;
+    if (isdir)
+        for (int c = 0; c < 3; ++c) mask |= isdir[3 * v + c] ? 1 << c : 0;
+    xn[v] = make_float4((float)((X[v] - cx) * sx), (float)((X[nn + v] - cy) * sy), (float)((X[2 * nn + v] - cz) * sz),
+                        (float)mask);
+}
+
+// column (col0 + c ldm + k), k < nmono, holds monomial k of the scaled node
+// coordinates in displacement component c, zero in the other components and
+// on constrained DOFs; one thread per node
+template <typename T>
+__global__ void poly_basis_kernel(uint32_t nn, const float4* __restrict__ xn, int nmono, int ldm,
                                   T* __restrict__ out, uint32_t ld, int col0) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nn) return;
-    const double x = (X[v] - cx) * sx, y = (X[nn + v] - cy) * sy, z = (X[2 * nn + v] - cz) * sz;
-    const double mono[kMaxMono] = {1.0,       x,         y,         z,         x * x,     y * y,     z * z,
-                                   x * y,     y * z,     x * z,     x * x * x, y * y * y, z * z * z, x * x * y,
-                                   x * x * z, y * y * x, y * y * z, z * z * x, z * z * y, x * y * z};
-    for (int c = 0; c < 3; ++c) {
-        const bool free = !isdir || !isdir[3 * v + c];
-        for (int k = 0; k < nmono; ++k) {
-            T* col = out + (size_t)(col0 + c * nmono + k) * ld + 3 * (size_t)v;
-            for (int d = 0; d < 3; ++d) col[d] = (d == c && free) ? (T)mono[k] : (T)0;
+    const float4 c4 = xn[v];
+    const Powers<T> pw(c4);
+    const int mask = (int)c4.w;
+    for (int k = 0; k < nmono; ++k) {
+        const T m = pw.mono_rt(kMonoCodeDev[k]);
+        for (int c = 0; c < 3; ++c) {
+            T* col = out + (size_t)(col0 + c * ldm + k) * ld + 3 * (size_t)v;
+            for (int d = 0; d < 3; ++d) col[d] = (d == c && !(mask >> c & 1)) ? m : (T)0;
         }
+    }
+}
+
+// W columns (col0 + c ldm + k), k < nmono: the products of the node-blocked
+// matrix with those polynomial fields, in one pass over the matrix (one
+// product per field would read the matrix 3 nmono times).  One thread per
+// (row node, monomial); the nmono threads of a node read its value rows
+// together, so the rows come from HBM once.
+template <typename T>
+__global__ __launch_bounds__(256) void poly_spmm_kernel(uint32_t nn, const uint32_t* __restrict__ rp,
+                                                        const uint32_t* __restrict__ nptr,
+                                                        const uint32_t* __restrict__ ncol, const T* __restrict__ val,
+                                                        const float4* __restrict__ xn, int nmono, int ldm,
+                                                        T* __restrict__ W, uint32_t ld, int col0) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t v = t / (uint32_t)nmono, k = t % (uint32_t)nmono;
+    if (v >= nn) return;
+    const int code = kMonoCodeDev[k];
+    const uint32_t b0 = nptr[v], nb = nptr[v + 1] - b0;
+    const T* a = val + rp[0] + 9 * (size_t)b0;  // row 0 of the node; rows 1 and 2 follow at 3 nb
+    T acc[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[r][c] = (T)0;
+    for (uint32_t q = 0; q < nb; ++q) {
+        const float4 c4 = xn[ncol[b0 + q]];
+        const T m = Powers<T>(c4).mono_rt(code);
+        const int mask = (int)c4.w;
+        T f[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) f[c] = (mask >> c & 1) ? (T)0 : m;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[r][c] += a[3 * nb * r + 3 * q + c] * f[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) W[(size_t)(col0 + c * ldm + k) * ld + 3 * (size_t)v + r] = acc[r][c];
+}
+
+// ---------------------------------------------------------------------------
+// additive coarse correction of the preconditioner: z = M^{-1} r + Z E^{-1}
+// Z^T r with Z the polynomial fields (3 NM columns) and E = Z^T A Z
+// ---------------------------------------------------------------------------
+constexpr int kMomentBlocks = 512;
+
+// block partials of Z^T r (3 NM moments, FP64), grid-stride over the nodes
+template <typename T, int NM>
+__global__ __launch_bounds__(256) void coarse_moments_kernel(uint32_t nn, const float4* __restrict__ xn,
+                                                             const T* __restrict__ r, double* __restrict__ partials,
+                                                             const PcgState* __restrict__ st) {
+    if (st->done) return;
+    __shared__ double s_w[4][3 * NM];
+    double acc[3][NM];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < NM; ++k) acc[c][k] = 0.0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nn; v += gridDim.x * blockDim.x) {
+        const float4 c4 = xn[v];
+        const Powers<T> pw(c4);
+        const int mask = (int)c4.w;
+        T rv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) rv[c] = (mask >> c & 1) ? (T)0 : r[3 * (size_t)v + c];
+#pragma unroll
+        for (int k = 0; k < NM; ++k) {
+            const T m = pw.mono(mono_code(k));
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[c][k] += (double)(m * rv[c]);
+        }
+    }
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < NM; ++k) {
+            double x = acc[c][k];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+            if (lane == 0) s_w[w][c * NM + k] = x;
+        }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * NM; i += blockDim.x)
+        partials[(size_t)blockIdx.x * 3 * NM + i] = (s_w[0][i] + s_w[1][i]) + (s_w[2][i] + s_w[3][i]);
+}
+
+// one block of 1024 threads: moments g = sum of the nb block partials,
+// coefficients c = E^{-1} g (E^{-1} symmetric) and g.c (the coarse part of
+// r.z) into the PCG state.  Thread (s, i) takes moment i of every S-th
+// partial, then entries i of every S-th row of E^{-1}; the S slices are
+// added in slice order.
+__global__ __launch_bounds__(1024) void coarse_solve_kernel(uint32_t nb, int m, const double* __restrict__ partials,
+                                                            const double* __restrict__ Einv,
+                                                            double* __restrict__ coef, PcgState* __restrict__ st) {
+    if (st->done) return;
+    __shared__ double s_part[1024];
+    __shared__ double s_g[3 * kMaxMono];
+    int mp = 16;
+    while (mp < m) mp <<= 1;
+    const int S = (int)blockDim.x / mp, i = threadIdx.x % mp, sl = threadIdx.x / mp;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < m) {
+        uint32_t b = sl;
+        for (; b + 3 * S < nb; b += 4 * S)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] += partials[(size_t)(b + u * S) * m + i];
+        for (; b < nb; b += S) a[0] += partials[(size_t)b * m + i];
+    }
+    s_part[threadIdx.x] = (a[0] + a[1]) + (a[2] + a[3]);
+    __syncthreads();
+    if (threadIdx.x < m) {
+        double g = 0.0;
+        for (int q = 0; q < S; ++q) g += s_part[q * mp + threadIdx.x];
+        s_g[threadIdx.x] = g;
+    }
+    __syncthreads();
+    double c = 0.0;
+    if (i < m)
+        for (int j = sl; j < m; j += S) c += Einv[(size_t)j * m + i] * s_g[j];
+    s_part[threadIdx.x] = c;
+    __syncthreads();
+    double gc = 0.0;
+    if (threadIdx.x < m) {
+        double ci = 0.0;
+        for (int q = 0; q < S; ++q) ci += s_part[q * mp + threadIdx.x];
+        coef[threadIdx.x] = ci;
+        gc = ci * s_g[threadIdx.x];
+    }
+    __syncthreads();  // block_sum's slots are its own; the barrier orders the reads above
+    const double t = block_sum(gc);
+    if (threadIdx.x == 0) st->rz_coarse = t;
+}
+
+// p = (y + Z c) + beta p, or p = y + Z c on the first (beta ignored); one
+// thread per node, so the coefficient loads are the same for every lane
+template <typename T, int NM>
+__global__ void pcg_p_update_coarse_kernel(uint32_t nn, const T* __restrict__ y, T* __restrict__ p,
+                                           const float4* __restrict__ xn, const double* __restrict__ coef, int first,
+                                           const PcgState* __restrict__ st) {
+    if (st->done) return;
+    const uint32_t v = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (v >= nn) return;
+    const float4 c4 = xn[v];
+    const Powers<T> pw(c4);
+    const int mask = (int)c4.w;
+    const T beta = first ? (T)0 : (T)st->beta;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T zc = (T)0;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) zc += (T)coef[c * NM + k] * pw.mono(mono_code(k));
+        if (mask >> c & 1) zc = (T)0;
+        const size_t i = 3 * (size_t)v + c;
+        const T z = y[i] + zc;
+        p[i] = first ? z : z + beta * p[i];
     }
 }
 
 // block partials of G = V^T W (V: m columns, W: mw columns, both with leading
 // dimension n), FP64 accumulation.  A block takes a contiguous row range in
-// chunks of kGramRows rows staged in LDS; a thread owns 4x4 tiles of G.
+// chunks of kGramRows rows staged in LDS; a thread owns TPT 4x4 tiles of G
+// (ntile <= 256 TPT).
 constexpr int kGramRows = 32;
-template <typename T>
+template <typename T, int TPT>
 __global__ __launch_bounds__(256) void gram_kernel(uint32_t n, int m, int mw, const T* __restrict__ V,
                                                    const T* __restrict__ W, double* __restrict__ partials) {
     extern __shared__ __align__(16) unsigned char gram_smem[];
@@ -1757,7 +2012,6 @@ __global__ __launch_bounds__(256) void gram_kernel(uint32_t n, int m, int mw, co
     const int ti_n = (m + 3) / 4, tj_n = (mw + 3) / 4, ntile = ti_n * tj_n;
     const uint32_t per = ((n + gridDim.x - 1) / gridDim.x + kGramRows - 1) / kGramRows * kGramRows;
     const uint32_t r0 = blockIdx.x * per, r1 = min(n, r0 + per);
-    constexpr int TPT = 2;  // tiles per thread (ntile <= 512)
     double acc[TPT][4][4];
 #pragma unroll
     for (int q = 0; q < TPT; ++q)
@@ -2521,14 +2775,72 @@ void launch_combine(hipStream_t s, int prec, uint32_t n, int m, const double* c,
         combine_kernel<float><<<cdiv(n, 256), 256, 0, s>>>(n, m, cb, (float*)out);
 }
 
-void launch_poly_basis(hipStream_t s, int prec, uint32_t nn, const double* X, const uint8_t* isdir, int nmono,
-                       const double* center, const double* scale, void* out, uint32_t ld, int col0) {
+void launch_coarse_coords(hipStream_t s, uint32_t nn, const double* X, const uint8_t* isdir, const double* center,
+                          const double* scale, float4* xn) {
+    coarse_coords_kernel<<<cdiv(nn, 256), 256, 0, s>>>(nn, X, isdir, center[0], center[1], center[2], scale[0],
+                                                        scale[1], scale[2], xn);
+}
+
+void launch_poly_basis(hipStream_t s, int prec, uint32_t nn, const float4* xn, int nmono, int ldm, void* out,
+                       uint32_t ld, int col0) {
     if (prec == PREC64)
-        poly_basis_kernel<double><<<cdiv(nn, 256), 256, 0, s>>>(nn, X, isdir, nmono, center[0], center[1], center[2],
-                                                               scale[0], scale[1], scale[2], (double*)out, ld, col0);
+        poly_basis_kernel<double><<<cdiv(nn, 256), 256, 0, s>>>(nn, xn, nmono, ldm, (double*)out, ld, col0);
     else
-        poly_basis_kernel<float><<<cdiv(nn, 256), 256, 0, s>>>(nn, X, isdir, nmono, center[0], center[1], center[2],
-                                                              scale[0], scale[1], scale[2], (float*)out, ld, col0);
+        poly_basis_kernel<float><<<cdiv(nn, 256), 256, 0, s>>>(nn, xn, nmono, ldm, (float*)out, ld, col0);
+}
+
+void launch_poly_spmm(hipStream_t s, int prec, const Matrix& A, const float4* xn, int nmono, int ldm, void* W,
+                      int col0) {
+    const uint32_t nn = A.n / 3, nt = nn * (uint32_t)nmono;
+    if (prec == PREC64)
+        poly_spmm_kernel<double><<<cdiv(nt, 256), 256, 0, s>>>(nn, A.row_ptr.p, A.nptr.p, A.ncol.p, A.val64.p, xn,
+                                                                nmono, ldm, (double*)W, A.n, col0);
+    else
+        poly_spmm_kernel<float><<<cdiv(nt, 256), 256, 0, s>>>(nn, A.row_ptr.p, A.nptr.p, A.ncol.p, A.val32.p, xn,
+                                                               nmono, ldm, (float*)W, A.n, col0);
+}
+
+uint32_t coarse_moment_blocks() { return kMomentBlocks; }
+
+template <typename T>
+static void coarse_correct_t(hipStream_t s, uint32_t n, int nmono, const float4* xn, const T* r, const double* Einv,
+                             double* partials, double* coef, PcgState* st) {
+    const uint32_t nn = n / 3;
+    switch (nmono) {
+        case 4: coarse_moments_kernel<T, 4><<<kMomentBlocks, 256, 0, s>>>(nn, xn, r, partials, st); break;
+        case 10: coarse_moments_kernel<T, 10><<<kMomentBlocks, 256, 0, s>>>(nn, xn, r, partials, st); break;
+        case 20: coarse_moments_kernel<T, 20><<<kMomentBlocks, 256, 0, s>>>(nn, xn, r, partials, st); break;
+        default: coarse_moments_kernel<T, 35><<<kMomentBlocks, 256, 0, s>>>(nn, xn, r, partials, st); break;
+    }
+    coarse_solve_kernel<<<1, 1024, 0, s>>>(kMomentBlocks, 3 * nmono, partials, Einv, coef, st);
+}
+
+void launch_coarse_correct(hipStream_t s, int prec, uint32_t n, int nmono, const float4* xn, const void* r,
+                           const double* Einv, double* partials, double* coef, PcgState* st) {
+    if (prec == PREC64)
+        coarse_correct_t<double>(s, n, nmono, xn, (const double*)r, Einv, partials, coef, st);
+    else
+        coarse_correct_t<float>(s, n, nmono, xn, (const float*)r, Einv, partials, coef, st);
+}
+
+template <typename T>
+static void p_update_coarse_t(hipStream_t s, uint32_t n, int nmono, const T* y, T* p, const float4* xn,
+                              const double* coef, int first, const PcgState* st) {
+    const uint32_t nn = n / 3, nb = cdiv(nn, 256);
+    switch (nmono) {
+        case 4: pcg_p_update_coarse_kernel<T, 4><<<nb, 256, 0, s>>>(nn, y, p, xn, coef, first, st); break;
+        case 10: pcg_p_update_coarse_kernel<T, 10><<<nb, 256, 0, s>>>(nn, y, p, xn, coef, first, st); break;
+        case 20: pcg_p_update_coarse_kernel<T, 20><<<nb, 256, 0, s>>>(nn, y, p, xn, coef, first, st); break;
+        default: pcg_p_update_coarse_kernel<T, 35><<<nb, 256, 0, s>>>(nn, y, p, xn, coef, first, st); break;
+    }
+}
+
+void launch_pcg_p_update_coarse(hipStream_t s, int prec, uint32_t n, int nmono, const void* y, void* p,
+                                const float4* xn, const double* coef, bool first, const PcgState* st) {
+    if (prec == PREC64)
+        p_update_coarse_t<double>(s, n, nmono, (const double*)y, (double*)p, xn, coef, first ? 1 : 0, st);
+    else
+        p_update_coarse_t<float>(s, n, nmono, (const float*)y, (float*)p, xn, coef, first ? 1 : 0, st);
 }
 
 uint32_t gram_blocks() { return 512; }
@@ -2538,10 +2850,18 @@ void launch_gram(hipStream_t s, int prec, uint32_t n, int m, int mw, const void*
     const uint32_t nb = gram_blocks();
     const size_t es = prec == PREC64 ? 8 : 4;
     const size_t lds = (size_t)(m + mw) * kGramRows * es;
-    if (prec == PREC64)
-        gram_kernel<double><<<nb, 256, lds, s>>>(n, m, mw, (const double*)V, (const double*)W, partials);
-    else
-        gram_kernel<float><<<nb, 256, lds, s>>>(n, m, mw, (const float*)V, (const float*)W, partials);
+    const int ntile = ((m + 3) / 4) * ((mw + 3) / 4);  // at most 1024 (m, mw <= 128)
+    if (prec == PREC64) {
+        if (ntile <= 512)
+            gram_kernel<double, 2><<<nb, 256, lds, s>>>(n, m, mw, (const double*)V, (const double*)W, partials);
+        else
+            gram_kernel<double, 4><<<nb, 256, lds, s>>>(n, m, mw, (const double*)V, (const double*)W, partials);
+    } else {
+        if (ntile <= 512)
+            gram_kernel<float, 2><<<nb, 256, lds, s>>>(n, m, mw, (const float*)V, (const float*)W, partials);
+        else
+            gram_kernel<float, 4><<<nb, 256, lds, s>>>(n, m, mw, (const float*)V, (const float*)W, partials);
+    }
     const uint32_t np = (uint32_t)(m * mw);
     gram_reduce_kernel<<<cdiv(np, 256), 256, 0, s>>>(nb, np, partials, G);
 }

@@ -15,7 +15,8 @@ LIB_PATH = os.environ.get("FM_LIB_PATH") or os.path.join(HERE, "libflexmesh.so")
 
 FM_OK, FM_EINVAL, FM_ESTATE, FM_EHIP, FM_ENOCONV, FM_EDEVICE = 0, -1, -2, -3, -4, -5
 FM_FP64, FM_MIXED, FM_MIXED_PC = 0, 1, 2
-KERNEL_CLASSES = ["spmv", "ilu_fwd", "ilu_bwd", "ilu_factor", "assemble", "residual", "pcg_other", "fields"]
+KERNEL_CLASSES = ["spmv", "ilu_fwd", "ilu_bwd", "ilu_factor", "assemble", "residual", "pcg_other", "fields",
+                  "galerkin", "coarse"]
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -30,7 +31,8 @@ class NewtonCfg(C.Structure):
                 ("ls_mode", C.c_int), ("tol_absolute", C.c_double), ("ls_alpha0", C.c_double),
                 ("ls_beta", C.c_double), ("reuse_tangent", C.c_int), ("energy_tol", C.c_double),
                 ("record_energy", C.c_int), ("inexact", C.c_int), ("recycle", C.c_int),
-                ("inner_tol_max", C.c_double), ("coarse_degree", C.c_int)]
+                ("inner_tol_max", C.c_double), ("coarse_degree", C.c_int),
+                ("coarse_additive", C.c_int)]
 
 
 FIELD_NAMES = ("det_F", "cauchy", "von_mises", "pressure", "max_principal", "energy", "max_stretch", "F")

@@ -18,8 +18,8 @@ value = DOFs x (load steps converged) / wall seconds of the timed steps.
 After the timed region, on the same load steps from the unloaded state:
   * the plain FP64 GPU path running the same algorithm (extrapolating
     predictor, Galerkin start over recycled corrections and polynomial
-    fields, target-aware forcing term) -- the precision-only ratio
-    mixed_over_fp64;
+    fields, the same fields as an additive coarse level, target-aware forcing
+    term) -- the precision-only ratio mixed_over_fp64;
   * the plain FP64 GPU path with the reference's solver settings (predictor
     solve, PCG tol 1e-10, 300-iteration cap), with its cap hits counted;
   * the FP64 CPU oracle on a bounded sample on the host cores (assembly,
@@ -96,6 +96,9 @@ def kernel_bytes(info, kind, v, launches_per_unit=None):
         # back, the node -> element lists; node averages and fits written and read
         # back; indicators written
         return ne * (16 + 16 + 2 * 7 * 8 + 16 + 4 + 8) + nn * (24 + 24 + 2 * 48)
+    if kind == "coarse":
+        # the residual in the PCG precision and the scaled node coordinates
+        return n * v + nn * 16
     if kind == "fields":
         # connectivity and the two material values per element, X and u once per
         # node, 21 FP64 outputs per element
@@ -311,6 +314,9 @@ def main():
     ap.add_argument("--coarse", type=int, default=None,
                     help="degree of the polynomial displacement fields in the Galerkin start of every inner "
                          "solve (default 3 with the extrapolating predictor; 0: none)")
+    ap.add_argument("--additive", type=int, default=None,
+                    help="1: those polynomial fields also form the coarse level of a two-level preconditioner "
+                         "in every PCG iteration (default 1 with the extrapolating predictor)")
     ap.add_argument("--recycle", type=int, default=None,
                     help="previous load steps whose Newton corrections (the latest 2x this many) span the "
                          "Galerkin start of a step's first solve (default 2 with the extrapolating "
@@ -370,12 +376,15 @@ def bench(S, cfg, args, warm_last, K):
     # the inner solves: with the extrapolated start, every solve starts from
     # the Galerkin projection onto the polynomial displacement fields of
     # degree <= 3 (the smooth modes ILU(0) resolves slowly), the first solve
-    # of a step also onto the previous two steps' Newton corrections; the
-    # first goes to 1e-3, the second stops at the target-aware forcing term
-    # (about 0.1: it needs only a few-fold reduction); without them the
+    # of a step also onto the previous two steps' Newton corrections, and the
+    # same fields stay in every PCG iteration as the coarse level of a
+    # two-level preconditioner z = M^{-1} r + Z (Z^T K Z)^{-1} Z^T r; the
+    # first solve goes to 1e-3, the second stops at the target-aware forcing
+    # term (about 0.1: it needs only a few-fold reduction); without them the
     # extrapolated start takes 1e-2 throughout, the predictor solve 3e-3
     recycle = args.recycle if args.recycle is not None else (2 if lift == 2 else 0)
     coarse = args.coarse if args.coarse is not None else (3 if lift == 2 else 0)
+    additive = args.additive if args.additive is not None else (1 if lift == 2 and coarse else 0)
     inner_tol_max = args.inner_tol_max if args.inner_tol_max is not None else (0.1 if lift == 2 else 0.0)
     if args.inner_tol is not None:
         inner_tol = args.inner_tol
@@ -383,7 +392,7 @@ def bench(S, cfg, args, warm_last, K):
         inner_tol = 1e-3 if (recycle or inner_tol_max or coarse) else 1e-2
     else:
         inner_tol = 3e-3
-    solver_kw = dict(recycle=recycle, inner_tol_max=inner_tol_max, coarse_degree=coarse)
+    solver_kw = dict(recycle=recycle, inner_tol_max=inner_tol_max, coarse_degree=coarse, coarse_additive=additive)
     if warm_last:
         log(f"warm-up pass: load steps 1..{warm_last}")
         run_steps(S, cfg, prec, 1, warm_last, inner_tol, False, args.pcg_max, lift, **solver_kw)
@@ -404,7 +413,7 @@ def bench(S, cfg, args, warm_last, K):
                                f"load steps 1..{K} of {cfg['n_steps']}", "dofs": dofs, "tets": info["ne"],
                    "nnz": info["nnz"], "predictor": "extrapolate" if lift == 2 else "solve",
                    "inner_tol": inner_tol, "inner_tol_max": inner_tol_max, "recycle": recycle,
-                   "coarse_degree": coarse,
+                   "coarse_degree": coarse, "coarse_additive": additive,
                    "newton_rtol": cfg["newton_rtol"]},
         "newton": {"converged_steps": tot["converged"], "newton_iters": tot["newton_iters"],
                    "pcg_iters": tot["pcg_iters"], "newton_it_per_s": tot["newton_iters"] / wall,
@@ -428,7 +437,8 @@ def bench(S, cfg, args, warm_last, K):
                        "algorithmic_bytes": d["bytes_per_launch"], "avg_us": d["avg_us"]}
     out["kernels"] = kern
     out["kernel_timing"] = (f"HIP events on the library's stream around one launch in {PROF_EVERY} of spmv, ilu_fwd, "
-                            "ilu_bwd and pcg_other and around every other launch, over the timed pass; ms_total = "
+                            "ilu_bwd, pcg_other and coarse and around every other launch, over the timed pass; "
+                            "ms_total = "
                             "mean of the timed launches x launches")
     if "spmv" in kern:
         out["spmv_frac_of_peak"] = kern["spmv"]["GBps"] / PEAK_HBM_GBS
@@ -452,6 +462,8 @@ def bench(S, cfg, args, warm_last, K):
                                         + (f"Galerkin start over the corrections of {recycle} previous steps"
                                            + (f" and the polynomial fields of degree <= {coarse}" if coarse else "")
                                            + ", " if (recycle or coarse) else "")
+                                        + ("the same fields as the additive coarse level of the preconditioner, "
+                                           if coarse and additive else "")
                                         + (f"forcing term clamp(0.5 target/||R||, {inner_tol:g}, {inner_tol_max:g})"
                                            if inner_tol_max > inner_tol else f"forcing term {inner_tol:g}")
                                         + ", with the FP64 tangent, SpMV, ILU(0) and PCG"}

@@ -181,12 +181,17 @@ typedef struct {
     double inner_tol_max; /* > inner_tol: target-aware forcing term, each inner solve to
                              clamp(0.5 newton_rtol R_0 / ||R_k||, inner_tol, inner_tol_max) (a solve
                              that needs only a small reduction stops early); 0 = inner_tol throughout */
-    int coarse_degree;    /* 1..3: every inner solve (and the predictor solve) starts from the
+    int coarse_degree;    /* 1..4: every inner solve (and the predictor solve) starts from the
                              Galerkin projection onto the polynomial displacement fields of total
-                             degree <= coarse_degree in the node coordinates (3, 12, 30 or 60
+                             degree <= coarse_degree in the node coordinates (12, 30, 60 or 105
                              fields, zero on constrained DOFs), together with the recycled
                              corrections in a step's first solve: the smooth modes ILU(0) resolves
                              slowly; 0 = off */
+    int coarse_additive;  /* with coarse_degree > 0: those fields Z also enter every PCG iteration
+                             as an additive coarse correction, z = M^{-1} r + Z (Z^T K Z)^{-1} Z^T r
+                             (a two-level preconditioner with the ILU(0) solve M^{-1}); the PCG
+                             stopping rule sqrt(r.z / b.b) then measures r in that
+                             preconditioner's norm; 0 = off */
 } fm_newton_cfg;
 
 typedef struct {
@@ -308,10 +313,13 @@ int fm_checkpoint_load(fm_ctx* ctx, const char* path, uint32_t* load_step);
 #define FM_K_RESIDUAL 5
 #define FM_K_PCG_OTHER 6
 #define FM_K_FIELDS 7
-#define FM_K_COUNT 8
+#define FM_K_GALERKIN 8  /* Galerkin start: products with the basis, Gram matrix, combination */
+#define FM_K_COARSE 9    /* additive coarse correction: moments Z^T r and the coarse solve */
+#define FM_K_COUNT 10
 /* on = 0: off; 1: every launch of the timed kernel classes is bracketed by
    HIP events on the context's stream; k > 1: of the classes launched once per
-   PCG iteration (SpMV, the two sweeps, the other PCG kernels) one launch in
+   PCG iteration (SpMV, the two sweeps, the other PCG kernels, the coarse
+   correction) one launch in
    every k (launches 0, k, 2k, ... counted from fm_profile_reset), since each
    pair of events puts a bubble of a few microseconds into the stream */
 int fm_profile_enable(fm_ctx* ctx, int on);
