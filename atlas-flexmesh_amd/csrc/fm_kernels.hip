@@ -1999,78 +1999,134 @@ __global__ void pcg_p_update_coarse_kernel(uint32_t nn, const T* __restrict__ y,
 }
 
 // block partials of G = V^T W (V: m columns, W: mw columns, both with leading
-// dimension n), FP64 accumulation.  A block takes a contiguous row range in
-// chunks of kGramRows rows staged in LDS; a thread owns TPT 4x4 tiles of G
-// (ntile <= 256 TPT).
+// dimension n, m and mw <= 128), FP64 accumulation.  A block takes a
+// contiguous row range in chunks of kGramRows rows, staged in LDS row by row
+// (row r of V at sV[r mp ..], of W at sW[r mwp ..], padded to multiples of 4
+// columns), so that the lanes of a wave read their four consecutive columns
+// of W from consecutive 16-byte slots (conflict-free) and share V's (a
+// broadcast).  Thread t owns the 4x4 tile (t / (mwp / 4), t % (mwp / 4)) of
+// G; the next chunk's loads are in flight while the current one is summed.
+// blockDim: gram_threads(m, mw), so that every thread stages at most
+// kGramStage values per chunk.
 constexpr int kGramRows = 32;
-template <typename T, int TPT>
-__global__ __launch_bounds__(256) void gram_kernel(uint32_t n, int m, int mw, const T* __restrict__ V,
-                                                   const T* __restrict__ W, double* __restrict__ partials) {
+constexpr int kGramStage = 16;
+template <typename T>
+__global__ __launch_bounds__(1024) void gram_kernel(uint32_t n, int m, int mw, const T* __restrict__ V,
+                                                    const T* __restrict__ W, double* __restrict__ partials) {
     extern __shared__ __align__(16) unsigned char gram_smem[];
+    using V4 = typename Vec16<T>::type;  // 16 bytes: 4 floats or 2 doubles
+    constexpr int PV = 16 / (int)sizeof(T);
+    const int mp = (m + 3) & ~3, mwp = (mw + 3) & ~3;
     T* sV = reinterpret_cast<T*>(gram_smem);
-    T* sW = sV + (size_t)m * kGramRows;
-    const int ti_n = (m + 3) / 4, tj_n = (mw + 3) / 4, ntile = ti_n * tj_n;
+    T* sW = sV + kGramRows * mp;
+    const int tj_n = mwp / 4, ntile = (mp / 4) * tj_n;
+    const int t = threadIdx.x;
+    const bool active = t < ntile;
+    const int i0 = active ? (t / tj_n) * 4 : 0, j0 = active ? (t % tj_n) * 4 : 0;
     const uint32_t per = ((n + gridDim.x - 1) / gridDim.x + kGramRows - 1) / kGramRows * kGramRows;
     const uint32_t r0 = blockIdx.x * per, r1 = min(n, r0 + per);
-    double acc[TPT][4][4];
+    const int nv = mp * kGramRows, total = (mp + mwp) * kGramRows;
+    double acc[4][4];
 #pragma unroll
-    for (int q = 0; q < TPT; ++q)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    T buf[kGramStage];
+    auto load = [&](uint32_t rc) {
 #pragma unroll
-            for (int b = 0; b < 4; ++b) acc[q][a][b] = 0.0;
+        for (int q = 0; q < kGramStage; ++q) {
+            const int e = t + q * (int)blockDim.x;
+            T v = (T)0;
+            if (e < total) {
+                const bool inv = e < nv;
+                const int f = inv ? e : e - nv;
+                const int j = f / kGramRows, r = f % kGramRows;
+                if (rc + r < r1 && j < (inv ? m : mw)) v = (inv ? V : W)[(size_t)j * n + rc + r];
+            }
+            buf[q] = v;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int q = 0; q < kGramStage; ++q) {
+            const int e = t + q * (int)blockDim.x;
+            if (e < total) {
+                const bool inv = e < nv;
+                const int f = inv ? e : e - nv;
+                const int j = f / kGramRows, r = f % kGramRows;
+                if (inv)
+                    sV[r * mp + j] = buf[q];
+                else
+                    sW[r * mwp + j] = buf[q];
+            }
+        }
+    };
+    if (r0 < r1) load(r0);
     for (uint32_t rc = r0; rc < r1; rc += kGramRows) {
-        for (int e = threadIdx.x; e < m * kGramRows; e += blockDim.x) {
-            const int j = e / kGramRows, r = e % kGramRows;
-            sV[e] = rc + r < r1 ? V[(size_t)j * n + rc + r] : (T)0;
-        }
-        for (int e = threadIdx.x; e < mw * kGramRows; e += blockDim.x) {
-            const int j = e / kGramRows, r = e % kGramRows;
-            sW[e] = rc + r < r1 ? W[(size_t)j * n + rc + r] : (T)0;
-        }
+        store();
         __syncthreads();
+        if (rc + kGramRows < r1) load(rc + kGramRows);
+        if (active) {
+            for (int r = 0; r < kGramRows; ++r) {
+                double a[4], b[4];
 #pragma unroll
-        for (int q = 0; q < TPT; ++q) {
-            const int t = threadIdx.x + q * blockDim.x;
-            if (t < ntile) {
-                const int i0 = (t / tj_n) * 4, j0 = (t % tj_n) * 4;
-                for (int r = 0; r < kGramRows; ++r) {
-                    double a[4], b[4];
+                for (int h = 0; h < 4 / PV; ++h) {
+                    const V4 va = *reinterpret_cast<const V4*>(sV + r * mp + i0 + h * PV);
+                    const V4 vb = *reinterpret_cast<const V4*>(sW + r * mwp + j0 + h * PV);
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        a[u] = i0 + u < m ? (double)sV[(i0 + u) * kGramRows + r] : 0.0;
-                        b[u] = j0 + u < mw ? (double)sW[(j0 + u) * kGramRows + r] : 0.0;
+                    for (int u = 0; u < PV; ++u) {
+                        a[h * PV + u] = (double)va[u];
+                        b[h * PV + u] = (double)vb[u];
                     }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u)
-#pragma unroll
-                        for (int w = 0; w < 4; ++w) acc[q][u][w] += a[u] * b[w];
                 }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int w = 0; w < 4; ++w) acc[u][w] += a[u] * b[w];
             }
         }
         __syncthreads();
     }
+    if (!active) return;
     double* out = partials + (size_t)blockIdx.x * m * mw;
-#pragma unroll
-    for (int q = 0; q < TPT; ++q) {
-        const int t = threadIdx.x + q * blockDim.x;
-        if (t < ntile) {
-            const int i0 = (t / tj_n) * 4, j0 = (t % tj_n) * 4;
-            for (int u = 0; u < 4; ++u)
-                for (int w = 0; w < 4; ++w)
-                    if (i0 + u < m && j0 + w < mw) out[(i0 + u) * mw + j0 + w] = acc[q][u][w];
-        }
-    }
+    for (int u = 0; u < 4; ++u)
+        for (int w = 0; w < 4; ++w)
+            if (i0 + u < m && j0 + w < mw) out[(i0 + u) * mw + j0 + w] = acc[u][w];
 }
 
-// G[p] = sum over the blocks of partials[b][p], in block order
-__global__ void gram_reduce_kernel(uint32_t nb, uint32_t np, const double* __restrict__ partials,
-                                   double* __restrict__ G) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    double s = 0.0;
-    for (uint32_t b = 0; b < nb; ++b) s += partials[(size_t)b * np + p];
-    G[p] = s;
+// threads of a Gram block: one per 4x4 tile, and enough that each stages at
+// most kGramStage values of a chunk
+static int gram_threads(int m, int mw) {
+    const int mp = (m + 3) & ~3, mwp = (mw + 3) & ~3;
+    const int ntile = (mp / 4) * (mwp / 4);
+    const int stage = ((mp + mwp) * kGramRows + kGramStage - 1) / kGramStage;
+    const int t = std::max(ntile, stage);
+    return std::min(1024, (t + 63) / 64 * 64);
+}
+
+// G[p] = sum over the blocks of partials[b][p]: eight slices of every
+// eighth block each, added in slice order
+__global__ __launch_bounds__(256) void gram_reduce_kernel(uint32_t nb, uint32_t np,
+                                                          const double* __restrict__ partials,
+                                                          double* __restrict__ G) {
+    __shared__ double s_sl[8][32];
+    const int e = threadIdx.x & 31, sl = threadIdx.x >> 5;
+    const uint32_t p = blockIdx.x * 32 + e;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    if (p < np) {
+        uint32_t b = sl;
+        for (; b + 24 < nb; b += 32)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] += partials[(size_t)(b + 8 * u) * np + p];
+        for (; b < nb; b += 8) a[0] += partials[(size_t)b * np + p];
+    }
+    s_sl[sl][e] = (a[0] + a[1]) + (a[2] + a[3]);
+    __syncthreads();
+    if (sl == 0 && p < np) {
+        double g = 0.0;
+        for (int q = 0; q < 8; ++q) g += s_sl[q][e];
+        G[p] = g;
+    }
 }
 
 // x = sum_k c[k] V[:, col[k]] (FP64 accumulation)
@@ -2849,21 +2905,14 @@ void launch_gram(hipStream_t s, int prec, uint32_t n, int m, int mw, const void*
                  double* G) {
     const uint32_t nb = gram_blocks();
     const size_t es = prec == PREC64 ? 8 : 4;
-    const size_t lds = (size_t)(m + mw) * kGramRows * es;
-    const int ntile = ((m + 3) / 4) * ((mw + 3) / 4);  // at most 1024 (m, mw <= 128)
-    if (prec == PREC64) {
-        if (ntile <= 512)
-            gram_kernel<double, 2><<<nb, 256, lds, s>>>(n, m, mw, (const double*)V, (const double*)W, partials);
-        else
-            gram_kernel<double, 4><<<nb, 256, lds, s>>>(n, m, mw, (const double*)V, (const double*)W, partials);
-    } else {
-        if (ntile <= 512)
-            gram_kernel<float, 2><<<nb, 256, lds, s>>>(n, m, mw, (const float*)V, (const float*)W, partials);
-        else
-            gram_kernel<float, 4><<<nb, 256, lds, s>>>(n, m, mw, (const float*)V, (const float*)W, partials);
-    }
+    const size_t lds = (size_t)(((m + 3) & ~3) + ((mw + 3) & ~3)) * kGramRows * es;
+    const int nt = gram_threads(m, mw);
+    if (prec == PREC64)
+        gram_kernel<double><<<nb, nt, lds, s>>>(n, m, mw, (const double*)V, (const double*)W, partials);
+    else
+        gram_kernel<float><<<nb, nt, lds, s>>>(n, m, mw, (const float*)V, (const float*)W, partials);
     const uint32_t np = (uint32_t)(m * mw);
-    gram_reduce_kernel<<<cdiv(np, 256), 256, 0, s>>>(nb, np, partials, G);
+    gram_reduce_kernel<<<cdiv(np, 32), 256, 0, s>>>(nb, np, partials, G);
 }
 
 void launch_basis_combine(hipStream_t s, int prec, uint32_t n, int k, const int* col, const double* c, const void* V,
