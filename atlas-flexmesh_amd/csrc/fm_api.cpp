@@ -1004,7 +1004,39 @@ int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool ad
         FM_HIP(hipMemcpyAsync(ctx->ad_Einv.p, Einv.data(), Einv.size() * 8, hipMemcpyHostToDevice, s));
         ctx->ad_nmono = nmono;
     }
+    ctx->lag_nmono = ctx->ad_nmono;
+    ctx->lag_prec = p;
     FM_TRY(sync(ctx));  // the host arrays above are copied before they go out of scope
+    return FM_OK;
+}
+
+// the start of a solve with a lagged preconditioner: x0 = Z E^{-1} Z^T b with
+// the coarse operator E of the step's first tangent (the Galerkin start over
+// the polynomial fields for that tangent), and the same E^{-1} as the additive
+// coarse level of the PCG that follows
+int lagged_start(fm_ctx* ctx, Matrix& A, int p) {
+    hipStream_t s = ctx->stream;
+    const uint32_t n = A.n;
+    const int nm = ctx->lag_nmono;
+    // the coarse kernels return at once on a finished PCG state
+    PcgState init{};
+    *ctx->h_st = init;
+    FM_HIP(hipMemcpyAsync(ctx->st.p, ctx->h_st, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    void* b = vb(ctx, p, VB);
+    void* x = vb(ctx, p, VX);
+    void* z = vb(ctx, p, VZ);
+    {
+        ProfScope ps(ctx, FM_K_GALERKIN);
+        launch_coarse_correct(s, p, n, nm, ctx->cs_xn.p, b, ctx->ad_Einv.p, ctx->ad_part.p, ctx->ad_coef.p,
+                              ctx->st.p);
+        if (p == PREC64)
+            launch_fill_f64(s, (double*)z, n, 0.0);
+        else
+            launch_fill_f32(s, (float*)z, n, 0.f);
+        launch_pcg_p_update_coarse(s, p, n, nm, z, x, ctx->cs_xn.p, ctx->ad_coef.p, true, ctx->st.p);
+    }
+    FM_TRY(check_launch(ctx, "lagged start"));
+    ctx->ad_nmono = nm;
     return FM_OK;
 }
 
@@ -1033,7 +1065,7 @@ int galerkin_push(fm_ctx* ctx, uint32_t n, int p, int deg) {
 // recycled corrections
 int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double tol, int max_it, int* iters,
                  bool inexact = false, int* capped = nullptr, int ring_use = 0, int deg = 0, bool push = false,
-                 bool additive = false) {
+                 bool additive = false, bool lag = false) {
     ctx->ad_nmono = 0;
     Matrix& A = ctx->tangent;
     ctx->cur = &A;
@@ -1069,7 +1101,9 @@ int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double to
     FM_TRY(pcg_vectors(ctx, prec, n));
     if (prec == PREC64 && inexact) {
         launch_scale(ctx->stream, n, rhs, 1.0 / nr, ctx->b64.p);
-        if ((ring_use > 0 && ctx->gal_n[0] > 0) || deg > 0)
+        if (lag && additive && ring_use == 0 && ctx->lag_nmono > 0 && ctx->lag_prec == PREC64)
+            FM_TRY(lagged_start(ctx, A, PREC64));
+        else if ((ring_use > 0 && ctx->gal_n[0] > 0) || deg > 0)
             FM_TRY(galerkin_start(ctx, A, PREC64, ring_use, deg, additive));
         else
             launch_fill_f64(ctx->stream, ctx->x64.p, n, 0.0);
@@ -1086,7 +1120,9 @@ int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double to
     } else {
         // FP32 inner solve on the normalised right-hand side
         launch_f64_to_f32(ctx->stream, n, rhs, 1.0 / nr, ctx->b32.p);
-        if ((ring_use > 0 && ctx->gal_n[1] > 0) || deg > 0)
+        if (lag && additive && ring_use == 0 && ctx->lag_nmono > 0 && ctx->lag_prec == PREC32)
+            FM_TRY(lagged_start(ctx, A, PREC32));
+        else if ((ring_use > 0 && ctx->gal_n[1] > 0) || deg > 0)
             FM_TRY(galerkin_start(ctx, A, PREC32, ring_use, deg, additive));
         else
             launch_fill_f32(ctx->stream, ctx->x32.p, n, 0.f);
@@ -1715,6 +1751,7 @@ void fm_newton_cfg_default(fm_newton_cfg* c) {
     c->inner_tol_max = 0.0;
     c->coarse_degree = 0;
     c->coarse_additive = 0;
+    c->lag_precond = 0;
 }
 
 int fm_set_u(fm_ctx* ctx, const double* u) {
@@ -1916,9 +1953,16 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         // expensive one, and on the first tangent it takes more PCG
         // iterations than the assembly and factorization it would save)
         const bool keep = it > 0 && (cfg.reuse_tangent || (order == 2 && !adapt && rn <= 1e-2 * rn_prev));
+        // a lagged preconditioner: the new tangent with the first iteration's
+        // factor (and coarse operator), which the assembly leaves in place
+        const bool lag_on = it > 0 && !keep && cfg.lag_precond && (kprec == PREC32 || inexact);
+        bool lag = false;
         if (!keep) {
+            const int fp = kprec == PREC32 ? PREC32 : PREC64;
+            lag = lag_on && ctx->tangent.lu_valid[fp];
             FM_TRY(assemble(ctx, ctx->u.p, true, aprec, true, nullptr, FM_K_ASSEMBLE));
             S.assemblies++;
+            if (lag) ctx->tangent.lu_valid[fp] = true;
         }
         int its = 0;
         // R of the current u is in ctx->R (last residual evaluation)
@@ -1934,9 +1978,9 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         // joins them
         const int start_from = recycle && it == 0 ? std::min(2 * cfg.recycle, kRing) : 0;
         FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, eta, lin_cap, &its, inexact, &S.pcg_cap_hits,
-                            start_from, coarse, recycle, additive));
+                            start_from, coarse, recycle, additive, lag));
         S.pcg_iters += its;
-        if (!keep) S.factorizations++;
+        if (!keep && !lag) S.factorizations++;
         // armijo_line_search_true (FEM/adaptive_fem_engine.hpp:933-985); with
         // ls_mode 1 a trial that fails the residual-norm test is also accepted
         // when the energy decreases by c alpha R.du (the residual norm may grow

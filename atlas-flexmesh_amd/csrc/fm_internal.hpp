@@ -177,6 +177,10 @@ struct Ctx {
     DevBuf<float4> cs_xn;
     DevBuf<double> ad_Einv, ad_part, ad_coef;
     int ad_nmono = 0;
+    // E^{-1} in ad_Einv belongs to the last Galerkin start's tangent, over
+    // lag_nmono monomials per component in precision lag_prec (0: none); a
+    // lagged preconditioner reuses it within the load step
+    int lag_nmono = 0, lag_prec = -1;
     int gal_n[2] = {0, 0}, gal_next[2] = {0, 0};
     DevBuf<double> sec_u1, sec_u2, sec_start;
     double sec_lf[3] = {0.0, 0.0, 0.0};

@@ -18,8 +18,9 @@ value = DOFs x (load steps converged) / wall seconds of the timed steps.
 After the timed region, on the same load steps from the unloaded state:
   * the plain FP64 GPU path running the same algorithm (extrapolating
     predictor, Galerkin start over recycled corrections and polynomial
-    fields, the same fields as an additive coarse level, target-aware forcing
-    term) -- the precision-only ratio mixed_over_fp64;
+    fields, the same fields as an additive coarse level, the first Newton
+    iteration's preconditioner kept for the later ones of a step,
+    target-aware forcing term) -- the precision-only ratio mixed_over_fp64;
   * the plain FP64 GPU path with the reference's solver settings (predictor
     solve, PCG tol 1e-10, 300-iteration cap), with its cap hits counted;
   * the FP64 CPU oracle on a bounded sample on the host cores (assembly,
@@ -317,6 +318,10 @@ def main():
     ap.add_argument("--additive", type=int, default=None,
                     help="1: those polynomial fields also form the coarse level of a two-level preconditioner "
                          "in every PCG iteration (default 1 with the extrapolating predictor)")
+    ap.add_argument("--lag", type=int, default=None,
+                    help="1: the later Newton iterations of a load step keep the first iteration's ILU(0) factor "
+                         "and coarse operator (a lagged preconditioner; the tangent is re-assembled) (default 1 "
+                         "with the coarse level)")
     ap.add_argument("--recycle", type=int, default=None,
                     help="previous load steps whose Newton corrections (the latest 2x this many) span the "
                          "Galerkin start of a step's first solve (default 2 with the extrapolating "
@@ -392,7 +397,9 @@ def bench(S, cfg, args, warm_last, K):
         inner_tol = 1e-3 if (recycle or inner_tol_max or coarse) else 1e-2
     else:
         inner_tol = 3e-3
-    solver_kw = dict(recycle=recycle, inner_tol_max=inner_tol_max, coarse_degree=coarse, coarse_additive=additive)
+    lag = args.lag if args.lag is not None else (1 if additive else 0)
+    solver_kw = dict(recycle=recycle, inner_tol_max=inner_tol_max, coarse_degree=coarse, coarse_additive=additive,
+                     lag_precond=lag)
     if warm_last:
         log(f"warm-up pass: load steps 1..{warm_last}")
         run_steps(S, cfg, prec, 1, warm_last, inner_tol, False, args.pcg_max, lift, **solver_kw)
@@ -413,7 +420,7 @@ def bench(S, cfg, args, warm_last, K):
                                f"load steps 1..{K} of {cfg['n_steps']}", "dofs": dofs, "tets": info["ne"],
                    "nnz": info["nnz"], "predictor": "extrapolate" if lift == 2 else "solve",
                    "inner_tol": inner_tol, "inner_tol_max": inner_tol_max, "recycle": recycle,
-                   "coarse_degree": coarse, "coarse_additive": additive,
+                   "coarse_degree": coarse, "coarse_additive": additive, "lag_precond": lag,
                    "newton_rtol": cfg["newton_rtol"]},
         "newton": {"converged_steps": tot["converged"], "newton_iters": tot["newton_iters"],
                    "pcg_iters": tot["pcg_iters"], "newton_it_per_s": tot["newton_iters"] / wall,
@@ -464,6 +471,8 @@ def bench(S, cfg, args, warm_last, K):
                                            + ", " if (recycle or coarse) else "")
                                         + ("the same fields as the additive coarse level of the preconditioner, "
                                            if coarse and additive else "")
+                                        + ("the first iteration's factor and coarse operator kept for the later "
+                                           "iterations of a step, " if lag else "")
                                         + (f"forcing term clamp(0.5 target/||R||, {inner_tol:g}, {inner_tol_max:g})"
                                            if inner_tol_max > inner_tol else f"forcing term {inner_tol:g}")
                                         + ", with the FP64 tangent, SpMV, ILU(0) and PCG"}

@@ -192,6 +192,11 @@ typedef struct {
                              (a two-level preconditioner with the ILU(0) solve M^{-1}); the PCG
                              stopping rule sqrt(r.z / b.b) then measures r in that
                              preconditioner's norm; 0 = off */
+    int lag_precond;      /* 1: the later Newton iterations of a load step re-assemble the tangent
+                             but keep the preconditioner of the step's first iteration: its ILU(0)
+                             factor and, with coarse_additive, its coarse operator Z^T K Z, whose
+                             inverse also gives those solves' Galerkin start (a lagged
+                             preconditioner: the Krylov operator is the current tangent); 0 = off */
 } fm_newton_cfg;
 
 typedef struct {

@@ -236,13 +236,17 @@ def test_repeated_matrix_pattern_uploads_values(fm, S, O):
         assert itsf == its and np.array_equal(xf, x)
 
 
-@pytest.mark.parametrize("prec,coarse,additive", [("mixed", 0, 0), ("fp64", 0, 0), ("mixed", 3, 0), ("fp64", 2, 0),
-                                                  ("mixed", 3, 1), ("fp64", 2, 1), ("mixed", 4, 1), ("fp64", 1, 1)])
-def test_galerkin_start_and_target_forcing(fm, S, O, prec, coarse, additive):
+@pytest.mark.parametrize("prec,coarse,additive,lag", [("mixed", 0, 0, 0), ("fp64", 0, 0, 0), ("mixed", 3, 0, 0),
+                                                      ("fp64", 2, 0, 0), ("mixed", 3, 1, 0), ("fp64", 2, 1, 0),
+                                                      ("mixed", 4, 1, 0), ("fp64", 1, 1, 0), ("mixed", 3, 1, 1),
+                                                      ("fp64", 3, 1, 1), ("mixed", 0, 0, 1)])
+def test_galerkin_start_and_target_forcing(fm, S, O, prec, coarse, additive, lag):
     """recycle = 2 with the target-aware forcing term (inner_tol 1e-3, at most
     0.1), and with the polynomial fields of degree <= coarse in the Galerkin
     start (and, with additive, in every PCG iteration as the coarse level of
-    a two-level preconditioner): every step still reaches ||R|| <= 1e-8 ||R_0||, the final state
+    a two-level preconditioner; with lag, the later Newton iterations of a
+    step keep the first one's ILU(0) factor and coarse operator): every step
+    still reaches ||R|| <= 1e-8 ||R_0||, the final state
     matches the FP64 oracle's, and from the third step on the steps take
     fewer PCG iterations than with the same forcing and x0 = 0.
     A reset to the unloaded state clears the start basis with the rest of the
@@ -256,7 +260,7 @@ def test_galerkin_start_and_target_forcing(fm, S, O, prec, coarse, additive):
     kw = dict(prec=fm.FM_MIXED) if prec == "mixed" else dict(prec=fm.FM_FP64, inexact=1)
     plain = fm.configs.newton_cfg(cfg, lift=2, inner_tol=1e-3, inner_tol_max=0.1, recycle=0, **kw)
     rec = fm.configs.newton_cfg(cfg, lift=2, inner_tol=1e-3, inner_tol_max=0.1, recycle=2, coarse_degree=coarse,
-                                coarse_additive=additive, **kw)
+                                coarse_additive=additive, lag_precond=lag, **kw)
     runs = {}
     for name, c in (("plain", plain), ("rec", rec), ("rec again", rec)):
         S.set_u(None)
