@@ -823,7 +823,7 @@ constexpr double kForcingShare = 0.5;
 // of recycled corrections, newest first, then the polynomial fields); one
 // whose pivot in the Cholesky factor of V^T A V is not above 1e-10 of its
 // diagonal adds nothing new and is left out.
-constexpr int kRing = 6;          // recycled corrections kept (basis columns 0..5)
+constexpr int kRing = 12;         // recycled corrections kept (basis columns 0..11)
 constexpr int kMaxDegree = 4;
 constexpr int kMonos[kMaxDegree + 1] = {0, 4, 10, 20, 35};  // monomials of total degree <= 0 (none), 1, 2, 3, 4
 static_assert(kRing + 3 * kMonos[kMaxDegree] + 1 <= 128, "the Gram kernel takes at most 128 columns");
@@ -1752,6 +1752,7 @@ void fm_newton_cfg_default(fm_newton_cfg* c) {
     c->coarse_degree = 0;
     c->coarse_additive = 0;
     c->lag_precond = 0;
+    c->extrap_order = 0;
 }
 
 int fm_set_u(fm_ctx* ctx, const double* u) {
@@ -1808,6 +1809,17 @@ int fm_newton_step(fm_ctx* ctx, uint32_t step, double* u, const fm_newton_cfg* c
         fm_newton_cfg_default(&cfg);
     if (cfg.n_steps == 0 || step == 0 || step > cfg.n_steps) return fail(ctx, FM_EINVAL, "bad load step");
     return fm_newton_solve_to(ctx, (double)step / (double)cfg.n_steps, u, &cfg, stats);
+}
+
+// FM_TRACE_SOLVES=1: one line per linear solve of the Newton loop on stderr
+// (load factor, Newton iteration or -1 for the predictor solve, forcing
+// term, PCG iterations, lagged preconditioner)
+static void trace_solve(double lf, int it, double eta, int its, bool lag) {
+    static const bool on = [] {
+        const char* e = std::getenv("FM_TRACE_SOLVES");
+        return e && *e == '1';
+    }();
+    if (on) std::fprintf(stderr, "[fm] lf %.4f newton %d eta %.3g pcg %d%s\n", lf, it, eta, its, lag ? " lagged" : "");
 }
 
 int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newton_cfg* cfg_in,
@@ -1880,35 +1892,43 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     S.residuals++;
     FM_TRY(record(S.r0, 1.0, ctx->ut.p));
     // lift 2: the step starts from the Lagrange extrapolation to lf of the
-    // previous converged states of this load path (quadratic through three,
-    // the secant through two), with the prescribed values set exactly (no
-    // linear solve); the predictor solve below where there is no such history
-    // a start state that already passes the absolute test takes no predictor
+    // previous converged states of this load path (of degree extrap_order
+    // through that many + 1 states, default quadratic through three; lower
+    // degrees while fewer are known), with the prescribed values set exactly
+    // (no linear solve); the predictor solve below where there is no such
+    // history; a start state that already passes the absolute test takes no
+    // predictor
     const bool at_rest = S.r0 < cfg.tol_absolute;
-    const double l0 = ctx->sec_lf[0], l1 = ctx->sec_lf[1], l2 = ctx->sec_lf[2];
-    const int order = (cfg.lift != 2 || at_rest) ? 0
-                      : (ctx->sec_n >= 3 && l0 != l1 && l1 != l2 && l0 != l2) ? 2
-                      : (ctx->sec_n >= 2 && l0 != l1)                       ? 1
-                                                                             : 0;
+    const int max_order = cfg.extrap_order > 0 ? std::min(cfg.extrap_order, 3) : 2;
+    int order = 0;
+    if (cfg.lift == 2 && !at_rest) {
+        // the highest degree whose states are known at distinct load factors
+        for (int o = std::min(max_order, ctx->sec_n - 1); o >= 1 && order == 0; --o) {
+            bool distinct = true;
+            for (int a = 0; a <= o; ++a)
+                for (int b = a + 1; b <= o; ++b) distinct &= ctx->sec_lf[a] != ctx->sec_lf[b];
+            if (distinct) order = o;
+        }
+    }
     const bool extrapolate = order > 0;
     if (cfg.lift == 2) {
         FM_HIP(ctx->sec_start.alloc(n));
         FM_HIP(ctx->sec_u1.alloc(n));
         FM_HIP(ctx->sec_u2.alloc(n));
+        if (max_order >= 3) FM_HIP(ctx->sec_u3.alloc(n));
         FM_HIP(hipMemcpyAsync(ctx->sec_start.p, ctx->u.p, n * 8ull, hipMemcpyDeviceToDevice, s));
     }
     if (extrapolate) {
-        // weights of u (at l0), u1 (at l1), u2 (at l2)
-        double w0, w1, w2 = 0.0;
-        if (order == 2) {
-            w0 = (lf - l1) * (lf - l2) / ((l0 - l1) * (l0 - l2));
-            w1 = (lf - l0) * (lf - l2) / ((l1 - l0) * (l1 - l2));
-            w2 = (lf - l0) * (lf - l1) / ((l2 - l0) * (l2 - l1));
-        } else {
-            w0 = (lf - l1) / (l0 - l1);
-            w1 = (lf - l0) / (l1 - l0);
+        // Lagrange weights of u (at sec_lf[0]), u1, u2, u3 at lf
+        double w[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int a = 0; a <= order; ++a) {
+            double v = 1.0;
+            for (int b = 0; b <= order; ++b)
+                if (b != a) v *= (lf - ctx->sec_lf[b]) / (ctx->sec_lf[a] - ctx->sec_lf[b]);
+            w[a] = v;
         }
-        launch_lincomb3(s, n, w0, ctx->u.p, w1, ctx->sec_u1.p, w2, order == 2 ? ctx->sec_u2.p : nullptr, ctx->ut.p);
+        launch_lincomb4(s, n, w[0], ctx->u.p, w[1], ctx->sec_u1.p, w[2], order >= 2 ? ctx->sec_u2.p : nullptr, w[3],
+                        order >= 3 ? ctx->sec_u3.p : nullptr, ctx->ut.p);
         launch_fill_f64(s, ctx->dD.p, n, 0.0);
         launch_dir_increment(s, ndir, ctx->d_dir_dofs.p, ctx->d_dir_vals.p, lf, ctx->ut.p, ctx->dD.p);
         launch_axpy_to(s, n, ctx->ut.p, 1.0, ctx->dD.p, ctx->u.p);
@@ -1924,6 +1944,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
                             coarse, false, additive));
         S.pcg_iters += its;
         S.factorizations++;
+        trace_solve(lf, -1, lin_tol, its, false);
         launch_apply_predictor(s, n, ctx->du.p, ctx->dD.p, ctx->isdir.p, ctx->u.p);
     } else {
         launch_axpy_to(s, n, ctx->u.p, 1.0, ctx->dD.p, ctx->u.p);
@@ -1981,6 +2002,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
                             start_from, coarse, recycle, additive, lag));
         S.pcg_iters += its;
         if (!keep && !lag) S.factorizations++;
+        trace_solve(lf, it, eta, its, lag);
         // armijo_line_search_true (FEM/adaptive_fem_engine.hpp:933-985); with
         // ls_mode 1 a trial that fails the residual-norm test is also accepted
         // when the energy decreases by c alpha R.du (the residual norm may grow
@@ -2024,11 +2046,18 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     S.converged = done(rn, energy_stop);
     if (cfg.lift == 2 && S.converged) {
         // the start state becomes u1, u1 becomes u2, this step's converged state the current one
+        // (u2 becomes u3 when the cubic is in use)
+        const int keep_n = max_order >= 3 ? 4 : 3;
+        if (keep_n == 4) {
+            std::swap(ctx->sec_u3.p, ctx->sec_u2.p);
+            std::swap(ctx->sec_u3.n, ctx->sec_u2.n);
+        }
         std::swap(ctx->sec_u2.p, ctx->sec_u1.p);
         std::swap(ctx->sec_u2.n, ctx->sec_u1.n);
         std::swap(ctx->sec_u1.p, ctx->sec_start.p);
         std::swap(ctx->sec_u1.n, ctx->sec_start.n);
-        ctx->sec_n = ctx->sec_n >= 1 ? std::min(ctx->sec_n + 1, 3) : 1;
+        ctx->sec_n = ctx->sec_n >= 1 ? std::min(ctx->sec_n + 1, keep_n) : 1;
+        ctx->sec_lf[3] = ctx->sec_lf[2];
         ctx->sec_lf[2] = ctx->sec_lf[1];
         ctx->sec_lf[1] = ctx->sec_lf[0];
         ctx->sec_lf[0] = lf;

@@ -154,13 +154,14 @@ struct Ctx {
     bool have_n2e = false;  // node -> incident element lists (colour order) on the device
     // converged states for the extrapolating predictor (lift = 2): the device
     // state u at load factor sec_lf[0] (sec_n >= 1), sec_u1 at sec_lf[1]
-    // (sec_n >= 2) and sec_u2 at sec_lf[2] (sec_n == 3); cleared by anything
-    // that changes the problem or u
+    // (sec_n >= 2), sec_u2 at sec_lf[2] (sec_n >= 3) and, for the cubic,
+    // sec_u3 at sec_lf[3] (sec_n == 4); cleared by anything that changes the
+    // problem or u
     // Galerkin-start basis per PCG precision ([0] FP64, [1] FP32): V is
-    // n x cs_cols (leading dimension n) with a ring of the latest six
-    // normalised Newton corrections of this load path in columns 0..5
+    // n x cs_cols (leading dimension n) with a ring of the latest twelve
+    // normalised Newton corrections of this load path in columns 0..11
     // (gal_n valid, gal_next the slot written next; cleared with the history
-    // below) and the polynomial fields of degree <= cs_deg from column 6 on
+    // below) and the polynomial fields of degree <= cs_deg from column 12 on
     // (cs_nmono monomials per component; rebuilt after a change of the mesh,
     // the coordinates or the constrained DOFs); W holds K V and the
     // right-hand side
@@ -182,8 +183,8 @@ struct Ctx {
     // lagged preconditioner reuses it within the load step
     int lag_nmono = 0, lag_prec = -1;
     int gal_n[2] = {0, 0}, gal_next[2] = {0, 0};
-    DevBuf<double> sec_u1, sec_u2, sec_start;
-    double sec_lf[3] = {0.0, 0.0, 0.0};
+    DevBuf<double> sec_u1, sec_u2, sec_u3, sec_start;
+    double sec_lf[4] = {0.0, 0.0, 0.0, 0.0};
     int sec_n = 0;
     bool have_material = false;
 
@@ -365,9 +366,9 @@ void launch_elem_rot(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, d
 // maximum stretch, F (9)
 constexpr int kFieldCount = 21;
 void launch_element_fields(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, double* out);
-// out = a x + b y (+ c z when z is given)
-void launch_lincomb3(hipStream_t s, uint32_t n, double a, const double* x, double b, const double* y, double c,
-                     const double* z, double* out);
+// out = a x + b y (+ c z when z is given) (+ d w when w is given)
+void launch_lincomb4(hipStream_t s, uint32_t n, double a, const double* x, double b, const double* y, double c,
+                     const double* z, double d, const double* w, double* out);
 // Galerkin start of a linear solve (precision prec: PREC32 or PREC64 vectors)
 // block partials of a.b with FP64 accumulation; returns #partials
 uint32_t launch_dot(hipStream_t s, int prec, uint32_t n, const void* a, const void* b, double* partials);

@@ -314,18 +314,21 @@ def main():
                          "inner_tol_max) (default 0.1 with the extrapolating predictor; 0: a fixed forcing term)")
     ap.add_argument("--coarse", type=int, default=None,
                     help="degree of the polynomial displacement fields in the Galerkin start of every inner "
-                         "solve (default 3 with the extrapolating predictor; 0: none)")
+                         "solve (default 4 with the extrapolating predictor; 0: none)")
     ap.add_argument("--additive", type=int, default=None,
                     help="1: those polynomial fields also form the coarse level of a two-level preconditioner "
                          "in every PCG iteration (default 1 with the extrapolating predictor)")
+    ap.add_argument("--order", type=int, default=None,
+                    help="degree of the load-step extrapolation through the previous converged states (1..3; "
+                         "default 3, cubic through four)")
     ap.add_argument("--lag", type=int, default=None,
                     help="1: the later Newton iterations of a load step keep the first iteration's ILU(0) factor "
                          "and coarse operator (a lagged preconditioner; the tangent is re-assembled) (default 1 "
                          "with the coarse level)")
     ap.add_argument("--recycle", type=int, default=None,
-                    help="previous load steps whose Newton corrections (the latest 2x this many) span the "
-                         "Galerkin start of a step's first solve (default 2 with the extrapolating "
-                         "predictor; 0: x0 = 0)")
+                    help="the latest 2x this many Newton corrections of the load path span, with the "
+                         "polynomial fields, the Galerkin start of a step's first solve (default 6 with the "
+                         "extrapolating predictor; 0: x0 = 0)")
     ap.add_argument("--pcg-max", type=int, default=1000, help="PCG iteration cap of the inexact Newton solves")
     ap.add_argument("--no-fp64", action="store_true", help="skip both FP64 GPU rows")
     ap.add_argument("--fp64-ref-steps", type=int, default=None,
@@ -378,17 +381,20 @@ def bench(S, cfg, args, warm_last, K):
     # the extrapolating predictor uses the previous converged steps of the
     # path (the unloaded state at load factor 0 among them)
     lift = 2 if args.predictor == "extrapolate" else 1
-    # the inner solves: with the extrapolated start, every solve starts from
-    # the Galerkin projection onto the polynomial displacement fields of
-    # degree <= 3 (the smooth modes ILU(0) resolves slowly), the first solve
-    # of a step also onto the previous two steps' Newton corrections, and the
-    # same fields stay in every PCG iteration as the coarse level of a
-    # two-level preconditioner z = M^{-1} r + Z (Z^T K Z)^{-1} Z^T r; the
-    # first solve goes to 1e-3, the second stops at the target-aware forcing
-    # term (about 0.1: it needs only a few-fold reduction); without them the
-    # extrapolated start takes 1e-2 throughout, the predictor solve 3e-3
-    recycle = args.recycle if args.recycle is not None else (2 if lift == 2 else 0)
-    coarse = args.coarse if args.coarse is not None else (3 if lift == 2 else 0)
+    # the inner solves: with the extrapolated start (cubic through the last
+    # four converged states), every solve starts from the Galerkin projection
+    # onto the polynomial displacement fields of degree <= 4 (the smooth
+    # modes ILU(0) resolves slowly), the first solve of a step also onto the
+    # latest twelve Newton corrections of the path, and the same fields stay
+    # in every PCG iteration as the coarse level of a two-level
+    # preconditioner z = M^{-1} r + Z (Z^T K Z)^{-1} Z^T r; a solve stops at
+    # the target-aware forcing term clamp(0.5 target/||R||, 1e-3, 0.1), so
+    # that from the cubic start one Newton iteration reaches the target; a
+    # later iteration of a step keeps the first one's factor and coarse
+    # operator; without them the extrapolated start takes 1e-2 throughout,
+    # the predictor solve 3e-3
+    recycle = args.recycle if args.recycle is not None else (6 if lift == 2 else 0)
+    coarse = args.coarse if args.coarse is not None else (4 if lift == 2 else 0)
     additive = args.additive if args.additive is not None else (1 if lift == 2 and coarse else 0)
     inner_tol_max = args.inner_tol_max if args.inner_tol_max is not None else (0.1 if lift == 2 else 0.0)
     if args.inner_tol is not None:
@@ -398,8 +404,9 @@ def bench(S, cfg, args, warm_last, K):
     else:
         inner_tol = 3e-3
     lag = args.lag if args.lag is not None else (1 if additive else 0)
+    order = args.order if args.order is not None else 3
     solver_kw = dict(recycle=recycle, inner_tol_max=inner_tol_max, coarse_degree=coarse, coarse_additive=additive,
-                     lag_precond=lag)
+                     lag_precond=lag, extrap_order=order)
     if warm_last:
         log(f"warm-up pass: load steps 1..{warm_last}")
         run_steps(S, cfg, prec, 1, warm_last, inner_tol, False, args.pcg_max, lift, **solver_kw)
@@ -420,7 +427,7 @@ def bench(S, cfg, args, warm_last, K):
                                f"load steps 1..{K} of {cfg['n_steps']}", "dofs": dofs, "tets": info["ne"],
                    "nnz": info["nnz"], "predictor": "extrapolate" if lift == 2 else "solve",
                    "inner_tol": inner_tol, "inner_tol_max": inner_tol_max, "recycle": recycle,
-                   "coarse_degree": coarse, "coarse_additive": additive, "lag_precond": lag,
+                   "coarse_degree": coarse, "coarse_additive": additive, "lag_precond": lag, "extrap_order": order,
                    "newton_rtol": cfg["newton_rtol"]},
         "newton": {"converged_steps": tot["converged"], "newton_iters": tot["newton_iters"],
                    "pcg_iters": tot["pcg_iters"], "newton_it_per_s": tot["newton_iters"] / wall,

@@ -174,10 +174,9 @@ typedef struct {
     /* inner solves of FM_MIXED and inexact FM_FP64 (no counterpart in the reference, whose
        pcg_solve_ilu0 takes its start x from the caller, :1283-1288) */
     int recycle;          /* lift = 2: the first Newton solve of a load step starts from the
-                             Galerkin projection onto the latest 2 recycle (<= 6) Newton
-                             corrections of the load path, i.e. both corrections of each of the
-                             previous `recycle` steps in a steady run (the A-norm-best start in
-                             their span); 0 = x0 = 0 */
+                             Galerkin projection onto the latest 2 recycle (<= 12) Newton
+                             corrections of the load path (the A-norm-best start in their span;
+                             a step that takes k Newton iterations adds k of them); 0 = x0 = 0 */
     double inner_tol_max; /* > inner_tol: target-aware forcing term, each inner solve to
                              clamp(0.5 newton_rtol R_0 / ||R_k||, inner_tol, inner_tol_max) (a solve
                              that needs only a small reduction stops early); 0 = inner_tol throughout */
@@ -197,6 +196,8 @@ typedef struct {
                              factor and, with coarse_additive, its coarse operator Z^T K Z, whose
                              inverse also gives those solves' Galerkin start (a lagged
                              preconditioner: the Krylov operator is the current tangent); 0 = off */
+    int extrap_order;     /* lift = 2: degree of the extrapolation through the previous converged
+                             states, 1..3 (3: cubic through four); 0 = the default, quadratic */
 } fm_newton_cfg;
 
 typedef struct {
