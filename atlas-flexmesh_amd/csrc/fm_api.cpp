@@ -664,8 +664,14 @@ int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult*
             {
                 if (A.chain.ok) {
                     // the update runs as its own streaming kernel: fused, its
-                    // stores would sit on the sweep's serial step path
-                    {
+                    // stores would sit on the sweep's serial step path; with
+                    // the coarse level it also forms the moments of the new r
+                    if (ad) {
+                        ProfScope pu(ctx, FM_K_COARSE);
+                        launch_pcg_xr_update_moments(s, prec, n, ad, p, Ap, x, r, ctx->cs_xn.p, ctx->ad_part.p,
+                                                     ctx->st.p);
+                        launch_coarse_solve(s, ad, ctx->ad_part.p, ctx->ad_Einv.p, ctx->ad_coef.p, ctx->st.p);
+                    } else {
                         ProfScope pu(ctx, FM_K_PCG_OTHER);
                         launch_pcg_xr_update(s, prec, n, p, Ap, x, r, ctx->st.p);
                     }
@@ -683,7 +689,7 @@ int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult*
                           ? launch_chain_bwd(s, prec, A, z, r, y, ctx->partials.p, ctx->st.p, true, ctx->d_err.p)
                           : launch_ilu_bwd(s, prec, A, lu, z, r, y, ctx->partials.p, ctx->st.p, true, ctx->d_err.p);
             }
-            if (ad) {
+            if (ad && !A.chain.ok) {
                 ProfScope pc(ctx, FM_K_COARSE);
                 launch_coarse_correct(s, prec, n, ad, ctx->cs_xn.p, r, ctx->ad_Einv.p, ctx->ad_part.p,
                                       ctx->ad_coef.p, ctx->st.p);
@@ -828,6 +834,21 @@ constexpr int kMaxDegree = 4;
 constexpr int kMonos[kMaxDegree + 1] = {0, 4, 10, 20, 35};  // monomials of total degree <= 0 (none), 1, 2, 3, 4
 static_assert(kRing + 3 * kMonos[kMaxDegree] + 1 <= 128, "the Gram kernel takes at most 128 columns");
 
+// the node coordinates mapped to [-1, 1] over the mesh's bounding box (FP32)
+// and the constrained-component masks of the polynomial fields
+int coarse_coords(fm_ctx* ctx) {
+    double center[3], scale[3];
+    for (int d = 0; d < 3; ++d) {
+        const double* x = ctx->hX.data() + (size_t)d * ctx->nn;
+        const auto mm = std::minmax_element(x, x + ctx->nn);
+        center[d] = 0.5 * (*mm.first + *mm.second);
+        scale[d] = *mm.second > *mm.first ? 2.0 / (*mm.second - *mm.first) : 1.0;
+    }
+    FM_HIP(ctx->cs_xn.alloc(ctx->nn));
+    launch_coarse_coords(ctx->stream, ctx->nn, ctx->X.p, ctx->isdir.p, center, scale, ctx->cs_xn.p);
+    return FM_OK;
+}
+
 // the basis matrices of precision p with room for the polynomial fields of
 // degree deg (their columns rebuilt when missing)
 int coarse_space(fm_ctx* ctx, int p, uint32_t n, int deg) {
@@ -848,16 +869,7 @@ int coarse_space(fm_ctx* ctx, int p, uint32_t n, int deg) {
         ctx->gal_n[pi] = 0;  // the ring's contents are gone
     }
     if (nmono && (!ctx->cs_poly_ok[pi] || ctx->cs_nmono[pi] < nmono)) {
-        // coordinates mapped to [-1, 1] over the mesh's bounding box
-        double center[3], scale[3];
-        for (int d = 0; d < 3; ++d) {
-            const double* x = ctx->hX.data() + (size_t)d * ctx->nn;
-            const auto mm = std::minmax_element(x, x + ctx->nn);
-            center[d] = 0.5 * (*mm.first + *mm.second);
-            scale[d] = *mm.second > *mm.first ? 2.0 / (*mm.second - *mm.first) : 1.0;
-        }
-        FM_HIP(ctx->cs_xn.alloc(ctx->nn));
-        launch_coarse_coords(ctx->stream, ctx->nn, ctx->X.p, ctx->isdir.p, center, scale, ctx->cs_xn.p);
+        FM_TRY(coarse_coords(ctx));
         const int build = std::min(std::max(nmono, ctx->cs_nmono[pi]), (ctx->cs_cols[pi] - kRing) / 3);
         void* V = pi == 0 ? (void*)ctx->cs_V64.p : (void*)ctx->cs_V32.p;
         launch_poly_basis(ctx->stream, p, ctx->nn, ctx->cs_xn.p, build, build, V, n, kRing);
@@ -963,11 +975,37 @@ int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool ad
         for (int t = a + 1; t < k; ++t) v -= L[(size_t)t * nc + a] * coef[t];
         coef[a] = v / L[(size_t)a * nc + a];
     }
+    // x0: the kept ring columns combined from memory, the polynomial fields'
+    // part evaluated from the node coordinates (coefficient c nmono + k, FP64
+    // then FP32, the coarse kernels' layout)
+    std::vector<int> rcol;
+    std::vector<double> rcoef, pcoef(nmono ? 6 * nmono : 0, 0.0);
+    for (int a = 0; a < k; ++a) {
+        if (keep[a] < kRing) {
+            rcol.push_back(keep[a]);
+            rcoef.push_back(coef[a]);
+        } else {
+            const int col = keep[a] - kRing, c = col / ctx->cs_nmono[pi], q = col % ctx->cs_nmono[pi];
+            pcoef[(size_t)c * nmono + q] = coef[a];
+        }
+    }
+    for (int i = 0; i < 3 * nmono; ++i) reinterpret_cast<float*>(pcoef.data() + 3 * nmono)[i] = (float)pcoef[i];
     FM_HIP(ctx->cs_col.alloc(std::max<size_t>(ctx->cs_col.n, kRing + 3 * kMonos[kMaxDegree])));
-    FM_HIP(ctx->cs_c.alloc(std::max<size_t>(ctx->cs_c.n, kRing + 3 * kMonos[kMaxDegree])));
-    FM_HIP(hipMemcpyAsync(ctx->cs_col.p, keep.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
-    FM_HIP(hipMemcpyAsync(ctx->cs_c.p, coef.data(), k * 8, hipMemcpyHostToDevice, s));
-    launch_basis_combine(s, p, n, k, ctx->cs_col.p, ctx->cs_c.p, V, x);
+    FM_HIP(ctx->cs_c.alloc(std::max<size_t>(ctx->cs_c.n, kRing + 6 * kMonos[kMaxDegree])));
+    FM_HIP(hipMemcpyAsync(ctx->cs_col.p, rcol.data(), rcol.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    FM_HIP(hipMemcpyAsync(ctx->cs_c.p, rcoef.data(), rcoef.size() * 8, hipMemcpyHostToDevice, s));
+    if (nmono) {
+        FM_HIP(hipMemcpyAsync(ctx->cs_c.p + kRing, pcoef.data(), pcoef.size() * 8, hipMemcpyHostToDevice, s));
+        void* z = vb(ctx, p, VZ);
+        launch_basis_combine(s, p, n, (int)rcol.size(), ctx->cs_col.p, ctx->cs_c.p, V, z);
+        // the coarse kernels return at once on a finished PCG state
+        PcgState init{};
+        *ctx->h_st = init;
+        FM_HIP(hipMemcpyAsync(ctx->st.p, ctx->h_st, sizeof(PcgState), hipMemcpyHostToDevice, s));
+        launch_pcg_p_update_coarse(s, p, n, nmono, z, x, ctx->cs_xn.p, ctx->cs_c.p + kRing, true, ctx->st.p);
+    } else {
+        launch_basis_combine(s, p, n, (int)rcol.size(), ctx->cs_col.p, ctx->cs_c.p, V, x);
+    }
     // E^{-1} = (Z^T A Z)^{-1} over the polynomial fields (index c nmono + k),
     // from the same Gram entries; a left-out field gets a zero row and column
     const int ma = 3 * nmono;
@@ -999,7 +1037,7 @@ int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool ad
             for (int a = 0; a < ke; ++a) Einv[(size_t)pos[a] * ma + pos[col]] = w[a];
         }
         FM_HIP(ctx->ad_Einv.alloc(std::max<size_t>(ctx->ad_Einv.n, (size_t)ma * ma)));
-        FM_HIP(ctx->ad_coef.alloc(std::max<size_t>(ctx->ad_coef.n, (size_t)ma)));
+        FM_HIP(ctx->ad_coef.alloc(std::max<size_t>(ctx->ad_coef.n, (size_t)2 * ma)));
         FM_HIP(ctx->ad_part.alloc(std::max<size_t>(ctx->ad_part.n, (size_t)coarse_moment_blocks() * ma)));
         FM_HIP(hipMemcpyAsync(ctx->ad_Einv.p, Einv.data(), Einv.size() * 8, hipMemcpyHostToDevice, s));
         ctx->ad_nmono = nmono;
@@ -1786,6 +1824,42 @@ int fm_newton_history(fm_ctx* ctx, uint32_t cap, double* residual, double* step_
         if (step_size && i + 1 < n) step_size[i] = ctx->hist_alpha[i];
     }
     if (len) *len = (uint32_t)n;
+    return FM_OK;
+}
+
+int fm_coarse_moments(fm_ctx* ctx, int prec, int degree, const double* r, double* out) {
+    if (!ctx || !r || !out) return FM_EINVAL;
+    if (prec != FM_FP64 && prec != FM_MIXED) return fail(ctx, FM_EINVAL, "bad precision");
+    if (degree < 1 || degree > kMaxDegree) return fail(ctx, FM_EINVAL, "degree must be 1..4");
+    FM_TRY(ensure_ready(ctx));
+    FM_TRY(ensure_vectors(ctx));
+    hipStream_t s = ctx->stream;
+    const int p = prec == FM_FP64 ? PREC64 : PREC32;
+    const uint32_t n = 3 * ctx->nn;
+    const int nmono = kMonos[degree], m = 3 * nmono;
+    FM_TRY(coarse_coords(ctx));
+    FM_TRY(pcg_vectors(ctx, p, n));
+    FM_HIP(hipMemcpyAsync(ctx->hv.p, r, n * 8ull, hipMemcpyHostToDevice, s));
+    void* rv = vb(ctx, p, VB);
+    if (p == PREC64)
+        FM_HIP(hipMemcpyAsync(rv, ctx->hv.p, n * 8ull, hipMemcpyDeviceToDevice, s));
+    else
+        launch_f64_to_f32(s, n, ctx->hv.p, 1.0, (float*)rv);
+    const uint32_t nb = coarse_moment_blocks();
+    FM_HIP(ctx->ad_part.alloc(std::max<size_t>(ctx->ad_part.n, (size_t)nb * m)));
+    PcgState init{};
+    *ctx->h_st = init;
+    FM_HIP(hipMemcpyAsync(ctx->st.p, ctx->h_st, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    launch_coarse_moments(s, p, n, nmono, ctx->cs_xn.p, rv, ctx->ad_part.p, ctx->st.p);
+    FM_TRY(check_launch(ctx, "coarse moments"));
+    std::vector<double> part((size_t)nb * m);
+    FM_HIP(hipMemcpyAsync(part.data(), ctx->ad_part.p, part.size() * 8, hipMemcpyDeviceToHost, s));
+    FM_TRY(sync(ctx));
+    for (int i = 0; i < m; ++i) {
+        double v = 0.0;
+        for (uint32_t b = 0; b < nb; ++b) v += part[(size_t)b * m + i];
+        out[i] = v;
+    }
     return FM_OK;
 }
 

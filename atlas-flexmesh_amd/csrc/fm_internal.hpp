@@ -395,6 +395,16 @@ void launch_poly_spmm(hipStream_t s, int prec, const Matrix& A, const float4* xn
 uint32_t coarse_moment_blocks();
 void launch_coarse_correct(hipStream_t s, int prec, uint32_t n, int nmono, const float4* xn, const void* r,
                            const double* Einv, double* partials, double* coef, PcgState* st);
+// x += alpha p, r -= alpha Ap with the moment partials of the new r in the
+// same pass; launch_coarse_solve then turns them into the coefficients (coef:
+// 3 nmono FP64 values followed by the same in FP32, room for 6 nmono doubles)
+void launch_pcg_xr_update_moments(hipStream_t s, int prec, uint32_t n, int nmono, const void* p, const void* Ap,
+                                  void* x, void* r, const float4* xn, double* partials, const PcgState* st);
+void launch_coarse_solve(hipStream_t s, int nmono, const double* partials, const double* Einv, double* coef,
+                         PcgState* st);
+// block partials (coarse_moment_blocks() x 3 nmono) of the moments Z^T r alone
+void launch_coarse_moments(hipStream_t s, int prec, uint32_t n, int nmono, const float4* xn, const void* r,
+                           double* partials, const PcgState* st);
 // p = y + Z coef + beta p (first: p = y + Z coef)
 void launch_pcg_p_update_coarse(hipStream_t s, int prec, uint32_t n, int nmono, const void* y, void* p,
                                 const float4* xn, const double* coef, bool first, const PcgState* st);

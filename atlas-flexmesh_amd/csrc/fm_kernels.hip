@@ -1885,18 +1885,83 @@ __global__ __launch_bounds__(256) void poly_spmm_kernel(uint32_t nn, const uint3
 // ---------------------------------------------------------------------------
 constexpr int kMomentBlocks = 512;
 
+// DPP lane exchanges of a wave (GFX9 controls): the row-masked source value,
+// zero in the rows the mask leaves out
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_src(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROWS, 0xF, false));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_src(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, ROWS, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, ROWS, 0xF, false);
+    return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
+}
+// the sum over the wave's 64 lanes, complete in lane 63: pairs and quads
+// (quad permutes), rows of 16 (half-row and row mirrors), then row 0 into
+// rows 1 and 3 (broadcast of lane 15) and rows 0 + 1 into rows 2 and 3
+// (broadcast of lane 31); VALU only, no LDS round trip
+template <typename T>
+__device__ __forceinline__ T wave_sum_to_63(T v) {
+    v += dpp_src<0xB1, 0xF>(v);   // quad_perm(1, 0, 3, 2)
+    v += dpp_src<0x4E, 0xF>(v);   // quad_perm(2, 3, 0, 1)
+    v += dpp_src<0x141, 0xF>(v);  // row_half_mirror
+    v += dpp_src<0x140, 0xF>(v);  // row_mirror
+    v += dpp_src<0x142, 0xA>(v);  // row_bcast:15
+    v += dpp_src<0x143, 0xC>(v);  // row_bcast:31
+    return v;
+}
+
+// The moments Z^T r (3 NM values) of one thread's nodes accumulate in T
+// (FP32 products and sums over the ~16 nodes of a thread, then over the
+// wave, on the FP32 path: a preconditioner component, as exact as the FP32
+// sweeps beside it); the block sums are FP64, written as the block's
+// partials.  The wave sums are DPP reductions (a butterfly of LDS lane
+// exchanges waits for the exchange at every step).
+template <typename T, int NM>
+struct MomentAcc {
+    T acc[3][NM];
+    __device__ __forceinline__ MomentAcc() {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int k = 0; k < NM; ++k) acc[c][k] = (T)0;
+    }
+    // r: the node's three components, masked by the caller
+    __device__ __forceinline__ void add(const Powers<T>& pw, const T (&rv)[3]) {
+#pragma unroll
+        for (int k = 0; k < NM; ++k) {
+            const T m = pw.mono(mono_code(k));
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[c][k] = fma(m, rv[c], acc[c][k]);
+        }
+    }
+    __device__ __forceinline__ void store(double* __restrict__ partials) {
+        __shared__ double s_w[4][3 * NM];
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int k = 0; k < NM; ++k) acc[c][k] = wave_sum_to_63(acc[c][k]);
+        if (lane == 63)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int k = 0; k < NM; ++k) s_w[w][c * NM + k] = (double)acc[c][k];
+        __syncthreads();
+        for (int i = threadIdx.x; i < 3 * NM; i += blockDim.x)
+            partials[(size_t)blockIdx.x * 3 * NM + i] = (s_w[0][i] + s_w[1][i]) + (s_w[2][i] + s_w[3][i]);
+    }
+};
+
 // block partials of Z^T r (3 NM moments, FP64), grid-stride over the nodes
 template <typename T, int NM>
 __global__ __launch_bounds__(256) void coarse_moments_kernel(uint32_t nn, const float4* __restrict__ xn,
                                                              const T* __restrict__ r, double* __restrict__ partials,
                                                              const PcgState* __restrict__ st) {
     if (st->done) return;
-    __shared__ double s_w[4][3 * NM];
-    double acc[3][NM];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int k = 0; k < NM; ++k) acc[c][k] = 0.0;
+    MomentAcc<T, NM> M;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nn; v += gridDim.x * blockDim.x) {
         const float4 c4 = xn[v];
         const Powers<T> pw(c4);
@@ -1904,33 +1969,74 @@ __global__ __launch_bounds__(256) void coarse_moments_kernel(uint32_t nn, const 
         T rv[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) rv[c] = (mask >> c & 1) ? (T)0 : r[3 * (size_t)v + c];
-#pragma unroll
-        for (int k = 0; k < NM; ++k) {
-            const T m = pw.mono(mono_code(k));
-#pragma unroll
-            for (int c = 0; c < 3; ++c) acc[c][k] += (double)(m * rv[c]);
-        }
+        M.add(pw, rv);
     }
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    M.store(partials);
+}
+
+// the PCG update x += alpha p, r -= alpha Ap over whole nodes, with the block
+// partials of the moments Z^T r of the updated r (the additive coarse
+// correction's input) in the same pass.  A thread takes every G-th node
+// (G = the grid's threads, ~16 nodes each); the next node's five loads are
+// issued before the current node's moments are summed, so that two waves
+// per SIMD keep the memory busy.
+template <typename T, int NM>
+__global__ __launch_bounds__(256) void pcg_xr_update_moments_kernel(uint32_t nn, const T* __restrict__ p,
+                                                                    const T* __restrict__ Ap, T* __restrict__ x,
+                                                                    T* __restrict__ r, const float4* __restrict__ xn,
+                                                                    double* __restrict__ partials,
+                                                                    const PcgState* __restrict__ st) {
+    if (st->done) return;
+    const T alpha = (T)st->alpha;
+    MomentAcc<T, NM> M;
+    const uint32_t G = gridDim.x * blockDim.x;
+    struct In {
+        float4 c4;
+        T p[3], ap[3], x[3], r[3];
+    };
+    auto load = [&](uint32_t v, In& d) {
+        const uint32_t w = min(v, nn - 1);  // past the end: a valid node, never stored
+        const size_t i = 3 * (size_t)w;
+        d.c4 = xn[w];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int k = 0; k < NM; ++k) {
-            double x = acc[c][k];
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-            if (lane == 0) s_w[w][c * NM + k] = x;
+        for (int c = 0; c < 3; ++c) {
+            d.p[c] = p[i + c];
+            d.ap[c] = Ap[i + c];
+            d.x[c] = x[i + c];
+            d.r[c] = r[i + c];
         }
-    __syncthreads();
-    for (int i = threadIdx.x; i < 3 * NM; i += blockDim.x)
-        partials[(size_t)blockIdx.x * 3 * NM + i] = (s_w[0][i] + s_w[1][i]) + (s_w[2][i] + s_w[3][i]);
+    };
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    In cur;
+    load(v, cur);
+    for (; v < nn; v += G) {
+        In nxt;
+        load(v + G, nxt);
+        const size_t i = 3 * (size_t)v;
+        T rv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            x[i + c] = cur.x[c] + alpha * cur.p[c];
+            const T rn = cur.r[c] - alpha * cur.ap[c];
+            r[i + c] = rn;
+            rv[c] = rn;
+        }
+        const Powers<T> pw(cur.c4);
+        const int mask = (int)cur.c4.w;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) rv[c] = (mask >> c & 1) ? (T)0 : rv[c];
+        M.add(pw, rv);
+        cur = nxt;
+    }
+    M.store(partials);
 }
 
 // one block of 1024 threads: moments g = sum of the nb block partials,
 // coefficients c = E^{-1} g (E^{-1} symmetric) and g.c (the coarse part of
-// r.z) into the PCG state.  Thread (s, i) takes moment i of every S-th
-// partial, then entries i of every S-th row of E^{-1}; the S slices are
-// added in slice order.
+// r.z) into the PCG state; coef holds the m FP64 coefficients followed by
+// the same rounded to FP32 (the FP32 p update's).  Thread (s, i) takes
+// moment i of every S-th partial, sixteen loads in flight, then entries i of
+// every S-th row of E^{-1}; the S slices are added in slice order.
 __global__ __launch_bounds__(1024) void coarse_solve_kernel(uint32_t nb, int m, const double* __restrict__ partials,
                                                             const double* __restrict__ Einv,
                                                             double* __restrict__ coef, PcgState* __restrict__ st) {
@@ -1943,9 +2049,13 @@ __global__ __launch_bounds__(1024) void coarse_solve_kernel(uint32_t nb, int m, 
     double a[4] = {0.0, 0.0, 0.0, 0.0};
     if (i < m) {
         uint32_t b = sl;
-        for (; b + 3 * S < nb; b += 4 * S)
+        for (; b + 15 * S < nb; b += 16 * S) {
+            double t[16];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) a[u] += partials[(size_t)(b + u * S) * m + i];
+            for (int u = 0; u < 16; ++u) t[u] = partials[(size_t)(b + u * S) * m + i];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) a[u & 3] += t[u];
+        }
         for (; b < nb; b += S) a[0] += partials[(size_t)b * m + i];
     }
     s_part[threadIdx.x] = (a[0] + a[1]) + (a[2] + a[3]);
@@ -1966,6 +2076,7 @@ __global__ __launch_bounds__(1024) void coarse_solve_kernel(uint32_t nb, int m, 
         double ci = 0.0;
         for (int q = 0; q < S; ++q) ci += s_part[q * mp + threadIdx.x];
         coef[threadIdx.x] = ci;
+        reinterpret_cast<float*>(coef + m)[threadIdx.x] = (float)ci;
         gc = ci * s_g[threadIdx.x];
     }
     __syncthreads();  // block_sum's slots are its own; the barrier orders the reads above
@@ -1975,9 +2086,10 @@ __global__ __launch_bounds__(1024) void coarse_solve_kernel(uint32_t nb, int m, 
 
 // p = (y + Z c) + beta p, or p = y + Z c on the first (beta ignored); one
 // thread per node, so the coefficient loads are the same for every lane
+// (coefT: the coefficients in T, see coarse_solve_kernel)
 template <typename T, int NM>
 __global__ void pcg_p_update_coarse_kernel(uint32_t nn, const T* __restrict__ y, T* __restrict__ p,
-                                           const float4* __restrict__ xn, const double* __restrict__ coef, int first,
+                                           const float4* __restrict__ xn, const T* __restrict__ coefT, int first,
                                            const PcgState* __restrict__ st) {
     if (st->done) return;
     const uint32_t v = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -1990,7 +2102,7 @@ __global__ void pcg_p_update_coarse_kernel(uint32_t nn, const T* __restrict__ y,
     for (int c = 0; c < 3; ++c) {
         T zc = (T)0;
 #pragma unroll
-        for (int k = 0; k < NM; ++k) zc += (T)coef[c * NM + k] * pw.mono(mono_code(k));
+        for (int k = 0; k < NM; ++k) zc += coefT[c * NM + k] * pw.mono(mono_code(k));
         if (mask >> c & 1) zc = (T)0;
         const size_t i = 3 * (size_t)v + c;
         const T z = y[i] + zc;
@@ -2872,6 +2984,53 @@ static void coarse_correct_t(hipStream_t s, uint32_t n, int nmono, const float4*
     coarse_solve_kernel<<<1, 1024, 0, s>>>(kMomentBlocks, 3 * nmono, partials, Einv, coef, st);
 }
 
+template <typename T>
+static void coarse_moments_t(hipStream_t s, uint32_t n, int nmono, const float4* xn, const T* r, double* partials,
+                             const PcgState* st) {
+    const uint32_t nn = n / 3;
+    switch (nmono) {
+        case 4: coarse_moments_kernel<T, 4><<<kMomentBlocks, 256, 0, s>>>(nn, xn, r, partials, st); break;
+        case 10: coarse_moments_kernel<T, 10><<<kMomentBlocks, 256, 0, s>>>(nn, xn, r, partials, st); break;
+        case 20: coarse_moments_kernel<T, 20><<<kMomentBlocks, 256, 0, s>>>(nn, xn, r, partials, st); break;
+        default: coarse_moments_kernel<T, 35><<<kMomentBlocks, 256, 0, s>>>(nn, xn, r, partials, st); break;
+    }
+}
+
+void launch_coarse_moments(hipStream_t s, int prec, uint32_t n, int nmono, const float4* xn, const void* r,
+                           double* partials, const PcgState* st) {
+    if (prec == PREC64)
+        coarse_moments_t<double>(s, n, nmono, xn, (const double*)r, partials, st);
+    else
+        coarse_moments_t<float>(s, n, nmono, xn, (const float*)r, partials, st);
+}
+
+template <typename T>
+static void xr_update_moments_t(hipStream_t s, uint32_t n, int nmono, const T* p, const T* Ap, T* x, T* r,
+                                const float4* xn, double* partials, const PcgState* st) {
+    const uint32_t nn = n / 3;
+    switch (nmono) {
+        case 4: pcg_xr_update_moments_kernel<T, 4><<<kMomentBlocks, 256, 0, s>>>(nn, p, Ap, x, r, xn, partials, st); break;
+        case 10: pcg_xr_update_moments_kernel<T, 10><<<kMomentBlocks, 256, 0, s>>>(nn, p, Ap, x, r, xn, partials, st); break;
+        case 20: pcg_xr_update_moments_kernel<T, 20><<<kMomentBlocks, 256, 0, s>>>(nn, p, Ap, x, r, xn, partials, st); break;
+        default: pcg_xr_update_moments_kernel<T, 35><<<kMomentBlocks, 256, 0, s>>>(nn, p, Ap, x, r, xn, partials, st); break;
+    }
+}
+
+void launch_pcg_xr_update_moments(hipStream_t s, int prec, uint32_t n, int nmono, const void* p, const void* Ap,
+                                  void* x, void* r, const float4* xn, double* partials, const PcgState* st) {
+    if (prec == PREC64)
+        xr_update_moments_t<double>(s, n, nmono, (const double*)p, (const double*)Ap, (double*)x, (double*)r, xn,
+                                    partials, st);
+    else
+        xr_update_moments_t<float>(s, n, nmono, (const float*)p, (const float*)Ap, (float*)x, (float*)r, xn, partials,
+                                   st);
+}
+
+void launch_coarse_solve(hipStream_t s, int nmono, const double* partials, const double* Einv, double* coef,
+                         PcgState* st) {
+    coarse_solve_kernel<<<1, 1024, 0, s>>>(kMomentBlocks, 3 * nmono, partials, Einv, coef, st);
+}
+
 void launch_coarse_correct(hipStream_t s, int prec, uint32_t n, int nmono, const float4* xn, const void* r,
                            const double* Einv, double* partials, double* coef, PcgState* st) {
     if (prec == PREC64)
@@ -2882,8 +3041,10 @@ void launch_coarse_correct(hipStream_t s, int prec, uint32_t n, int nmono, const
 
 template <typename T>
 static void p_update_coarse_t(hipStream_t s, uint32_t n, int nmono, const T* y, T* p, const float4* xn,
-                              const double* coef, int first, const PcgState* st) {
+                              const double* coef64, int first, const PcgState* st) {
     const uint32_t nn = n / 3, nb = cdiv(nn, 256);
+    const T* coef = sizeof(T) == 8 ? reinterpret_cast<const T*>(coef64)
+                                   : reinterpret_cast<const T*>(coef64 + 3 * nmono);
     switch (nmono) {
         case 4: pcg_p_update_coarse_kernel<T, 4><<<nb, 256, 0, s>>>(nn, y, p, xn, coef, first, st); break;
         case 10: pcg_p_update_coarse_kernel<T, 10><<<nb, 256, 0, s>>>(nn, y, p, xn, coef, first, st); break;

@@ -99,6 +99,7 @@ def lib():
     L.fm_newton_step.argtypes = [_vp, C.c_uint32, _vp, C.POINTER(NewtonCfg), C.POINTER(NewtonStats)]
     L.fm_newton_solve_to.argtypes = [_vp, C.c_double, _vp, C.POINTER(NewtonCfg), C.POINTER(NewtonStats)]
     L.fm_newton_history.argtypes = [_vp, C.c_uint32, _vp, _vp, _vp, C.POINTER(C.c_uint32)]
+    L.fm_coarse_moments.argtypes = [_vp, C.c_int, C.c_int, _f64p, _f64p]
     L.fm_get_u.argtypes = [_vp, _f64p]
     L.fm_set_u.argtypes = [_vp, _vp]
     L.fm_adjoint.argtypes = [_vp, _f64p, C.c_double, C.c_int, _vp, _vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
@@ -369,6 +370,14 @@ class Solver:
         self._chk(self.L.fm_newton_history(self.h, n.value, res.ctypes.data, alpha.ctypes.data if alpha.size else None,
                                            en.ctypes.data, C.byref(n)), "fm_newton_history")
         return res, alpha, en
+
+    def coarse_moments(self, r, degree, prec):
+        """Z^T r over the polynomial fields of total degree <= degree (the
+        coarse level's moment kernel in precision prec), 3 nmono values."""
+        rr = self._nodal(r, "r")
+        out = np.zeros(3 * (4, 10, 20, 35)[degree - 1])
+        self._chk(self.L.fm_coarse_moments(self.h, prec, degree, rr, out), "fm_coarse_moments")
+        return out
 
     # ---- adjoint / rotations / checkpoint --------------------------------------
     def adjoint(self, dLdu, tol=1e-12, max_it=5000):

@@ -333,6 +333,17 @@ int fm_profile_reset(fm_ctx* ctx);
 /* launches[k]: launches of kernel class k while profiling was on; ms[k]: their
    total milliseconds (the mean of the timed launches times launches[k]) */
 int fm_profile_get(fm_ctx* ctx, double* ms, uint64_t* launches);
+/* No counterpart in the reference.  The moments Z^T r of r (3 nn doubles) over
+   the polynomial displacement fields of total degree <= degree (1..4) of the
+   Galerkin start and the additive coarse level, computed by the PCG's own
+   moment kernel in the precision prec (FM_FP64 or FM_MIXED): out[c nmono + k]
+   = sum over nodes v of mono_k(xs_v) r[3v + c], zero on constrained DOFs,
+   with xs the node coordinates mapped to [-1, 1] over the bounding box and
+   rounded to FP32, and the monomials mono_k by degree (1; x, y, z; x^2, y^2,
+   z^2, xy, yz, xz; x^3, y^3, z^3, x^2 y, x^2 z, x y^2, y^2 z, x z^2, y z^2,
+   xyz; x^4, y^4, z^4, x^3 y, x^3 z, x y^3, y^3 z, x z^3, y z^3, x^2 y^2,
+   y^2 z^2, x^2 z^2, x^2 y z, x y^2 z, x y z^2); nmono = 4, 10, 20, 35 */
+int fm_coarse_moments(fm_ctx* ctx, int prec, int degree, const double* r, double* out);
 /* sizes for roofline bookkeeping: n (rows), nnz, n_elems, n_nodes, n_colors */
 int fm_info(fm_ctx* ctx, uint64_t* out5);
 /* kernels chosen for the current matrix: ILU(0) sweep kind (0 level-scheduled,

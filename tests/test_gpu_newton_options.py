@@ -281,3 +281,44 @@ def test_galerkin_start_and_target_forcing(fm, S, O, prec, coarse, additive, lag
     assert np.array_equal(runs["rec"][0], runs["rec again"][0]) and runs["rec"][1] == runs["rec again"][1]
     later = lambda name: sum(t[1] for t in runs[name][1][2:])
     assert later("rec") < later("plain"), (runs["rec"][1], runs["plain"][1])
+
+
+# monomials of the polynomial fields (exponents of x, y, z), in the kernels' order
+MONOS = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 1, 0), (0, 1, 1), (1, 0, 1),
+         (3, 0, 0), (0, 3, 0), (0, 0, 3), (2, 1, 0), (2, 0, 1), (1, 2, 0), (0, 2, 1), (1, 0, 2), (0, 1, 2), (1, 1, 1),
+         (4, 0, 0), (0, 4, 0), (0, 0, 4), (3, 1, 0), (3, 0, 1), (1, 3, 0), (0, 3, 1), (1, 0, 3), (0, 1, 3), (2, 2, 0),
+         (0, 2, 2), (2, 0, 2), (2, 1, 1), (1, 2, 1), (1, 1, 2)]
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3, 4])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_coarse_moments(fm, S, degree, prec):
+    """The moment kernel of the coarse level (per-thread sums in the PCG's
+    precision, DPP wave sums, FP64 block sums) against a float64 sum over the
+    same FP32-rounded scaled coordinates, on a jittered box of odd size; the
+    constrained components contribute nothing.  A wrong lane pattern in the
+    wave sums drops or doubles lanes' partials, which this bound catches."""
+    cfg = cube(fm, dims=(9, 7, 11))
+    fm.configs.load(S, cfg)
+    nn = cfg["X"].size // 3
+    X = cfg["X"].reshape(3, nn)
+    xs = np.empty((3, nn))
+    for d in range(3):
+        lo, hi = X[d].min(), X[d].max()
+        xs[d] = ((X[d] - 0.5 * (lo + hi)) * (2.0 / (hi - lo))).astype(np.float32)
+    r = np.random.default_rng(degree).standard_normal(3 * nn)
+    mask = np.zeros(3 * nn, bool)
+    mask[cfg["dir_dofs"]] = True
+    rm = np.where(mask, 0.0, r).reshape(nn, 3)
+    nm = (4, 10, 20, 35)[degree - 1]
+    phi = np.stack([xs[0] ** a * xs[1] ** b * xs[2] ** c for a, b, c in MONOS[:nm]])  # nm x nn
+    want = (phi @ rm).T.reshape(-1)  # index c nm + k
+    scale = np.abs(phi) @ np.abs(rm)
+    p = fm.FM_FP64 if prec == "fp64" else fm.FM_MIXED
+    got = S.coarse_moments(r, degree, p)
+    tol = 1e-13 if prec == "fp64" else 2e-6
+    assert np.all(np.abs(got - want) <= tol * scale.T.reshape(-1) + 1e-300), (prec, degree)
+    # the constrained components: r there changes nothing
+    r2 = r.copy()
+    r2[cfg["dir_dofs"]] += 1.0
+    assert np.array_equal(S.coarse_moments(r2, degree, p), got)
