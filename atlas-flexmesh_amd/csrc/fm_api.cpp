@@ -930,35 +930,72 @@ int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool ad
             launch_fill_f32(s, (float*)x, n, 0.f);
         return FM_OK;
     }
-    // the contiguous column range that holds the candidates; W column c1 is b
+    // On a node-blocked matrix the polynomial fields' rows of the Gram matrix
+    // come from one fused pass over the matrix (Z^T A Z, Z^T A V_ring and
+    // Z^T b, without forming A Z), and the ring's rows from the products of
+    // the recycled corrections, one by one: G[ring] over the twelve ring
+    // slots with W column kRing = b.  Otherwise every candidate column's
+    // product is formed, and one Gram matrix covers the contiguous column
+    // range [c0, c1) with W column c1 = b (FM_GALERKIN=generic selects this
+    // path for every matrix; read once per process).
+    static const bool generic = [] {
+        const char* e = std::getenv("FM_GALERKIN");
+        return e && std::string(e) == "generic";
+    }();
+    const bool fused = !generic && A.node_spmv && nmono > 0;
     int c0 = cand[0], c1 = cand[0] + 1;
     for (int q = 0; q < nc; ++q) {
         c0 = std::min(c0, cand[q]);
         c1 = std::max(c1, cand[q] + 1);
     }
-    const int m = c1 - c0;
-    // the products: the polynomial fields in one pass over a node-blocked
-    // matrix, the recycled corrections one by one
-    const bool spmm = A.node_spmv && nmono > 0;
-    FM_HIP(ctx->cs_partials.alloc(std::max<size_t>(ctx->cs_partials.n, (size_t)gram_blocks() * m * (m + 1))));
-    FM_HIP(ctx->cs_G.alloc(std::max<size_t>(ctx->cs_G.n, (size_t)m * (m + 1))));
+    const int m = fused ? kRing : c1 - c0;
+    const int mp = 3 * nmono, np = fused ? mp * (mp + kPolyGramStored) : 0;
+    FM_HIP(ctx->cs_partials.alloc(std::max<size_t>(
+        ctx->cs_partials.n, std::max((size_t)gram_blocks(m, m + 1) * m * (m + 1), (size_t)poly_gram_blocks() * np))));
+    FM_HIP(ctx->cs_G.alloc(std::max<size_t>(ctx->cs_G.n, (size_t)m * (m + 1) + np)));
+    static_assert(kPolyGramStored == kRing + 1, "the fused pass reads the ring products and b");
     {
         ProfScope ps(ctx, FM_K_GALERKIN);
-        if (spmm) launch_poly_spmm(s, p, A, ctx->cs_xn.p, nmono, ctx->cs_nmono[pi], W, kRing);
         for (int q = 0; q < nc; ++q)
-            if (!spmm || cand[q] < kRing)
+            if (!fused || cand[q] < kRing)
                 launch_spmv(s, p, A, V + (size_t)cand[q] * n * es, W + (size_t)cand[q] * n * es, false, nullptr,
                             nullptr, nullptr, nullptr);
-        FM_HIP(hipMemcpyAsync(W + (size_t)c1 * n * es, b, n * es, hipMemcpyDeviceToDevice, s));
-        launch_gram(s, p, n, m, m + 1, V + (size_t)c0 * n * es, W + (size_t)c0 * n * es, ctx->cs_partials.p,
-                    ctx->cs_G.p);
+        const int bcol = fused ? kRing : c1;
+        FM_HIP(hipMemcpyAsync(W + (size_t)bcol * n * es, b, n * es, hipMemcpyDeviceToDevice, s));
+        if (fused) {
+            // ring slots that are not candidates hold stale columns: their
+            // entries are never read
+            if (ring_use > 0) launch_gram(s, p, n, m, m + 1, V, W, ctx->cs_partials.p, ctx->cs_G.p);
+            launch_poly_gram(s, p, A, ctx->cs_xn.p, nmono, W, ctx->cs_partials.p, ctx->cs_G.p + (size_t)m * (m + 1));
+        } else {
+            launch_gram(s, p, n, m, m + 1, V + (size_t)c0 * n * es, W + (size_t)c0 * n * es, ctx->cs_partials.p,
+                        ctx->cs_G.p);
+        }
     }
-    std::vector<double> G((size_t)m * (m + 1));
+    std::vector<double> G((size_t)m * (m + 1) + np);
     FM_HIP(hipMemcpyAsync(G.data(), ctx->cs_G.p, G.size() * 8, hipMemcpyDeviceToHost, s));
     FM_TRY(sync(ctx));
-    auto g = [&](int i, int j) {  // symmetric part of V^T A V over V-column indices
-        const int a = i - c0, c = j - c0;
-        return 0.5 * (G[(size_t)a * (m + 1) + c] + G[(size_t)c * (m + 1) + a]);
+    const double* Gp = G.data() + (size_t)m * (m + 1);  // fused: mp x (mp + kPolyGramStored)
+    const int ldp = mp + kPolyGramStored;
+    auto prow = [&](int col) {  // row of polynomial V-column col in Gp
+        const int o = col - kRing;
+        return (o / ctx->cs_nmono[pi]) * nmono + o % ctx->cs_nmono[pi];
+    };
+    // symmetric part of V^T A V over V-column indices, and V^T b
+    auto g = [&](int i, int j) -> double {
+        if (!fused) {
+            const int a = i - c0, c = j - c0;
+            return 0.5 * (G[(size_t)a * (m + 1) + c] + G[(size_t)c * (m + 1) + a]);
+        }
+        if (i < kRing && j < kRing) return 0.5 * (G[(size_t)i * (m + 1) + j] + G[(size_t)j * (m + 1) + i]);
+        if (i < kRing) return Gp[(size_t)prow(j) * ldp + mp + i];
+        if (j < kRing) return Gp[(size_t)prow(i) * ldp + mp + j];
+        const int a = prow(i), c = prow(j);
+        return 0.5 * (Gp[(size_t)a * ldp + c] + Gp[(size_t)c * ldp + a]);
+    };
+    auto vtb = [&](int i) -> double {
+        if (!fused) return G[(size_t)(i - c0) * (m + 1) + m];
+        return i < kRing ? G[(size_t)i * (m + 1) + m] : Gp[(size_t)prow(i) * ldp + mp + kRing];
     };
     // Cholesky over the kept candidates, then the two triangular solves
     std::vector<double> L, y(nc), coef(nc);
@@ -966,7 +1003,7 @@ int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool ad
     pivoted_cholesky(g, cand, nc, L, keep);
     const int k = (int)keep.size();
     for (int a = 0; a < k; ++a) {
-        double v = G[(size_t)(keep[a] - c0) * (m + 1) + m];
+        double v = vtb(keep[a]);
         for (int t = 0; t < a; ++t) v -= L[(size_t)a * nc + t] * y[t];
         y[a] = v / L[(size_t)a * nc + a];
     }

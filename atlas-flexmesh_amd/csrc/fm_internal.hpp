@@ -386,9 +386,6 @@ void launch_coarse_coords(hipStream_t s, uint32_t nn, const double* X, const uin
 // on constrained DOFs
 void launch_poly_basis(hipStream_t s, int prec, uint32_t nn, const float4* xn, int nmono, int ldm, void* out,
                        uint32_t ld, int col0);
-// the same columns of W = A Z for the node-blocked matrix A, in one pass
-void launch_poly_spmm(hipStream_t s, int prec, const Matrix& A, const float4* xn, int nmono, int ldm, void* W,
-                      int col0);
 // additive coarse correction: moments g = Z^T r (partials:
 // coarse_moment_blocks() * 3 nmono doubles), coef = Einv g (Einv 3 nmono
 // square, row-major) and st->rz_coarse = g.coef; a no-op once st->done
@@ -409,9 +406,17 @@ void launch_coarse_moments(hipStream_t s, int prec, uint32_t n, int nmono, const
 void launch_pcg_p_update_coarse(hipStream_t s, int prec, uint32_t n, int nmono, const void* y, void* p,
                                 const float4* xn, const double* coef, bool first, const PcgState* st);
 // G (m x mw, row-major, FP64) = V^T W for m and mw (both <= 128) columns of
-// leading dimension n, in a fixed order; partials: gram_blocks() * m * mw
+// leading dimension n, in a fixed order; partials: gram_blocks(m, mw) * m * mw
 // doubles
-uint32_t gram_blocks();
+uint32_t gram_blocks(int m, int mw);
+// G (3 nmono x (3 nmono + 13), row-major, FP64): the Gram rows of the
+// polynomial fields Z over the node-blocked matrix A without forming A Z:
+// Z^T A Z, then Z^T S_j for the 13 stored columns S (leading dimension A.n);
+// partials: poly_gram_blocks() * 3 nmono * (3 nmono + 13)
+uint32_t poly_gram_blocks();
+constexpr int kPolyGramStored = 13;
+void launch_poly_gram(hipStream_t s, int prec, const Matrix& A, const float4* xn, int nmono, const void* S,
+                      double* partials, double* G);
 void launch_gram(hipStream_t s, int prec, uint32_t n, int m, int mw, const void* V, const void* W, double* partials,
                  double* G);
 // x = sum_q c[q] V[:, col[q]] (device arrays col, c of k entries)

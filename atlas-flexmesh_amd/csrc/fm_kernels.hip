@@ -1839,44 +1839,183 @@ __global__ void poly_basis_kernel(uint32_t nn, const float4* __restrict__ xn, in
     }
 }
 
-// W columns (col0 + c ldm + k), k < nmono: the products of the node-blocked
-// matrix with those polynomial fields, in one pass over the matrix (one
-// product per field would read the matrix 3 nmono times).  One thread per
-// (row node, monomial); the nmono threads of a node read its value rows
-// together, so the rows come from HBM once.
+// Rows of the Galerkin start's Gram matrix that belong to the polynomial
+// fields Z (3 NM columns, (k, c) at c NM + k), without forming K Z:
+//   G[(k,c), (k',c')] = sum_v zeta_k,c(v) H_v[c][(k',c')],
+//   H_v[c][(k',c')]   = sum_q K_vq[c][c'] zeta_k',c'(u_q)
+// (zeta_k,c(v): monomial k at node v, zero where component c is constrained;
+// K_vq: the 3x3 block of node row v and column node u_q), and the moments
+// G[(k,c), 3 NM + j] = sum_v zeta_k,c(v) S_j[3v + c] of kPgStored stored
+// vectors S_j (the ring's K V columns and the right-hand side).  A block
+// takes tiles of TILE consecutive nodes, whose value runs are one contiguous
+// range: the range and the powers of the column nodes' coordinates are
+// staged in LDS with independent loads (a tile whose range exceeds the LDS
+// slice reads them directly), then the tile's monomials and H rows (the
+// row's products with every field, then the stored values), and each update
+// thread adds the tile's contributions to its 5 x 7 tile of G (sums in T over
+// the tile, FP64 over the tiles).  partials: one 3 NM x (3 NM + kPgStored)
+// block per workgroup, row-major.
+constexpr int kPgStored = 13;
+constexpr int kPgBlocks = 512;
+constexpr int kPgThreads = 384;
 template <typename T>
-__global__ __launch_bounds__(256) void poly_spmm_kernel(uint32_t nn, const uint32_t* __restrict__ rp,
-                                                        const uint32_t* __restrict__ nptr,
-                                                        const uint32_t* __restrict__ ncol, const T* __restrict__ val,
-                                                        const float4* __restrict__ xn, int nmono, int ldm,
-                                                        T* __restrict__ W, uint32_t ld, int col0) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t v = t / (uint32_t)nmono, k = t % (uint32_t)nmono;
-    if (v >= nn) return;
-    const int code = kMonoCodeDev[k];
-    const uint32_t b0 = nptr[v], nb = nptr[v + 1] - b0;
-    const T* a = val + rp[0] + 9 * (size_t)b0;  // row 0 of the node; rows 1 and 2 follow at 3 nb
-    T acc[3][3];
+constexpr int pg_tile() {
+    return sizeof(T) == 4 ? 16 : 8;
+}
+template <typename T, int NM>
+__global__ __launch_bounds__(kPgThreads) void poly_gram_kernel(uint32_t nn, const uint32_t* __restrict__ rp,
+                                                               const uint32_t* __restrict__ nptr,
+                                                               const uint32_t* __restrict__ ncol,
+                                                               const T* __restrict__ val,
+                                                               const float4* __restrict__ xn,
+                                                               const T* __restrict__ S, uint32_t ld,
+                                                               double* __restrict__ partials) {
+    constexpr int TILE = pg_tile<T>();
+    constexpr int CAPB = 20 * TILE;           // staged column blocks per tile
+    constexpr int MP = 3 * NM;                // polynomial columns
+    constexpr int NCOL = MP + kPgStored;      // columns of a G row
+    constexpr int RG = (NM + 4) / 5;          // row groups of five per component
+    constexpr int CG = (NCOL + 6) / 7;        // column groups of seven
+    constexpr int NU = 3 * RG * CG;           // update threads
+    static_assert(NU <= kPgThreads && TILE * NM <= 2 * kPgThreads, "tile work exceeds the block");
+    __shared__ T s_phi[3][TILE][RG * 5];      // zeta_k,c at the tile's nodes (zero padded)
+    __shared__ T s_h[TILE][3][CG * 7];        // H rows, then the stored values (zero padded)
+    __shared__ T s_k[CAPB * 9];               // the tile's value range
+    __shared__ T s_pw[CAPB][16];              // per column block: x^0..4, y^0..4, z^0..4, mask
+    const int t = threadIdx.x;
+    const uint32_t rbase = rp[0];
+    // the update thread's tile of G: component c, rows 5 rg.., columns 7 cg..
+    const int uc = t / (RG * CG), urg = (t % (RG * CG)) / CG, ucg = t % CG;
+    double accd[5][7];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+    for (int i = 0; i < 5; ++i)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) acc[r][c] = (T)0;
-    for (uint32_t q = 0; q < nb; ++q) {
-        const float4 c4 = xn[ncol[b0 + q]];
-        const T m = Powers<T>(c4).mono_rt(code);
-        const int mask = (int)c4.w;
-        T f[3];
+        for (int j = 0; j < 7; ++j) accd[i][j] = 0.0;
+    // the padding of both arrays stays zero
+    for (int i = t; i < 3 * TILE * RG * 5; i += kPgThreads) (&s_phi[0][0][0])[i] = (T)0;
+    for (int i = t; i < TILE * 3 * CG * 7; i += kPgThreads) (&s_h[0][0][0])[i] = (T)0;
+    const uint32_t ntile = (nn + TILE - 1) / TILE;
+    for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const uint32_t v0 = tile * TILE, v1 = min(v0 + TILE, nn);
+        const uint32_t bs = nptr[v0], be = nptr[v1], nbt = be - bs;
+        const bool staged = nbt <= (uint32_t)CAPB;  // block-uniform
+        if (staged) {
+            const T* src = val + rbase + 9 * (size_t)bs;
+            for (uint32_t i = t; i < 9 * nbt; i += kPgThreads) s_k[i] = src[i];
+            for (uint32_t q = t; q < nbt; q += kPgThreads) {
+                const float4 c4 = xn[ncol[bs + q]];
+                T p[5];
+                powers5((T)c4.x, p);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) f[c] = (mask >> c & 1) ? (T)0 : m;
+                for (int e = 0; e < 5; ++e) s_pw[q][e] = p[e];
+                powers5((T)c4.y, p);
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
+                for (int e = 0; e < 5; ++e) s_pw[q][5 + e] = p[e];
+                powers5((T)c4.z, p);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) acc[r][c] += a[3 * nb * r + 3 * q + c] * f[c];
+                for (int e = 0; e < 5; ++e) s_pw[q][10 + e] = p[e];
+                s_pw[q][15] = (T)c4.w;
+            }
+        }
+        // (2) the stored values of the tile's rows (zero past the last node)
+        for (int w = t; w < kPgStored * TILE * 3; w += kPgThreads) {
+            const int j = w / (TILE * 3), vr = w % (TILE * 3);
+            const uint32_t row = 3 * v0 + vr;
+            s_h[vr / 3][vr % 3][MP + j] = row < 3 * nn ? S[(size_t)j * ld + row] : (T)0;
+        }
+        __syncthreads();
+        // (1) monomials and H rows: work item (v, k), two rounds
+        for (int w = t; w < TILE * NM; w += kPgThreads) {
+            const int v = w / NM, k = w % NM;
+            const uint32_t node = v0 + v;
+            T h[3][3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) h[r][c] = (T)0;
+            T ph = (T)0;
+            int pmask = 7;
+            if (node < nn) {
+                const int code = kMonoCodeDev[k];
+                const int ex = code & 7, ey = (code >> 3) & 7, ez = code >> 6;
+                const float4 c4 = xn[node];
+                ph = Powers<T>(c4).mono_rt(code);
+                pmask = (int)c4.w;
+                const uint32_t b0 = nptr[node], nb = nptr[node + 1] - b0;
+                if (staged) {
+                    const T* a = s_k + 9 * (b0 - bs);
+                    for (uint32_t q = 0; q < nb; ++q) {
+                        const T* pw = s_pw[b0 - bs + q];
+                        const T m = (pw[ex] * pw[5 + ey]) * pw[10 + ez];
+                        const int um = (int)pw[15];
+                        T f[3];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) f[c] = (um >> c & 1) ? (T)0 : m;
+#pragma unroll
+                        for (int r = 0; r < 3; ++r)
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) h[r][c] += a[3 * nb * r + 3 * q + c] * f[c];
+                    }
+                } else {
+                    const T* a = val + rbase + 9 * (size_t)b0;
+                    for (uint32_t q = 0; q < nb; ++q) {
+                        const float4 u4 = xn[ncol[b0 + q]];
+                        const T m = Powers<T>(u4).mono_rt(code);
+                        const int um = (int)u4.w;
+                        T f[3];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) f[c] = (um >> c & 1) ? (T)0 : m;
+#pragma unroll
+                        for (int r = 0; r < 3; ++r)
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) h[r][c] += a[3 * nb * r + 3 * q + c] * f[c];
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s_phi[c][v][k] = (pmask >> c & 1) ? (T)0 : ph;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) s_h[v][r][c * NM + k] = h[r][c];
+        }
+        __syncthreads();
+        // (3) the update threads add the tile to their 5 x 7 tiles of G
+        if (t < NU) {
+            T acc[5][7];
+#pragma unroll
+            for (int i = 0; i < 5; ++i)
+#pragma unroll
+                for (int j = 0; j < 7; ++j) acc[i][j] = (T)0;
+#pragma unroll 4
+            for (int v = 0; v < TILE; ++v) {
+                T p[5], hv[7];
+#pragma unroll
+                for (int i = 0; i < 5; ++i) p[i] = s_phi[uc][v][5 * urg + i];
+#pragma unroll
+                for (int j = 0; j < 7; ++j) hv[j] = s_h[v][uc][7 * ucg + j];
+#pragma unroll
+                for (int i = 0; i < 5; ++i)
+#pragma unroll
+                    for (int j = 0; j < 7; ++j) acc[i][j] = fma(p[i], hv[j], acc[i][j]);
+            }
+#pragma unroll
+            for (int i = 0; i < 5; ++i)
+#pragma unroll
+                for (int j = 0; j < 7; ++j) accd[i][j] += (double)acc[i][j];
+        }
+        __syncthreads();  // the next tile overwrites the staged arrays
     }
+    if (t < NU) {
+        double* out = partials + (size_t)blockIdx.x * MP * NCOL;
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+        for (int i = 0; i < 5; ++i)
 #pragma unroll
-        for (int r = 0; r < 3; ++r) W[(size_t)(col0 + c * ldm + k) * ld + 3 * (size_t)v + r] = acc[r][c];
+            for (int j = 0; j < 7; ++j) {
+                const int k = 5 * urg + i, col = 7 * ucg + j;
+                if (k < NM && col < NCOL) out[(size_t)(uc * NM + k) * NCOL + col] = accd[i][j];
+            }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2227,9 +2366,13 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(uint32_t nb, uint32_t 
     double a[4] = {0.0, 0.0, 0.0, 0.0};
     if (p < np) {
         uint32_t b = sl;
-        for (; b + 24 < nb; b += 32)
+        for (; b + 8 * 15 < nb; b += 8 * 16) {
+            double t[16];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) a[u] += partials[(size_t)(b + 8 * u) * np + p];
+            for (int u = 0; u < 16; ++u) t[u] = partials[(size_t)(b + 8 * u) * np + p];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) a[u & 3] += t[u];
+        }
         for (; b < nb; b += 8) a[0] += partials[(size_t)b * np + p];
     }
     s_sl[sl][e] = (a[0] + a[1]) + (a[2] + a[3]);
@@ -2958,16 +3101,6 @@ void launch_poly_basis(hipStream_t s, int prec, uint32_t nn, const float4* xn, i
         poly_basis_kernel<float><<<cdiv(nn, 256), 256, 0, s>>>(nn, xn, nmono, ldm, (float*)out, ld, col0);
 }
 
-void launch_poly_spmm(hipStream_t s, int prec, const Matrix& A, const float4* xn, int nmono, int ldm, void* W,
-                      int col0) {
-    const uint32_t nn = A.n / 3, nt = nn * (uint32_t)nmono;
-    if (prec == PREC64)
-        poly_spmm_kernel<double><<<cdiv(nt, 256), 256, 0, s>>>(nn, A.row_ptr.p, A.nptr.p, A.ncol.p, A.val64.p, xn,
-                                                                nmono, ldm, (double*)W, A.n, col0);
-    else
-        poly_spmm_kernel<float><<<cdiv(nt, 256), 256, 0, s>>>(nn, A.row_ptr.p, A.nptr.p, A.ncol.p, A.val32.p, xn,
-                                                               nmono, ldm, (float*)W, A.n, col0);
-}
 
 uint32_t coarse_moment_blocks() { return kMomentBlocks; }
 
@@ -3061,11 +3194,48 @@ void launch_pcg_p_update_coarse(hipStream_t s, int prec, uint32_t n, int nmono, 
         p_update_coarse_t<float>(s, n, nmono, (const float*)y, (float*)p, xn, coef, first ? 1 : 0, st);
 }
 
-uint32_t gram_blocks() { return 512; }
+// a Gram matrix of few columns has few update threads per block (one 4x4
+// tile each): more blocks keep the chip's memory pipes busy
+uint32_t gram_blocks(int m, int mw) { return m * mw <= 32 * 32 ? 2048 : 512; }
+uint32_t poly_gram_blocks() { return kPgBlocks; }
+
+template <typename T>
+static void poly_gram_t(hipStream_t s, const Matrix& A, const float4* xn, int nmono, const T* val, const T* S,
+                        double* partials) {
+    const uint32_t nn = A.n / 3;
+    switch (nmono) {
+        case 4:
+            poly_gram_kernel<T, 4><<<kPgBlocks, kPgThreads, 0, s>>>(nn, A.row_ptr.p, A.nptr.p, A.ncol.p, val, xn, S,
+                                                                     A.n, partials);
+            break;
+        case 10:
+            poly_gram_kernel<T, 10><<<kPgBlocks, kPgThreads, 0, s>>>(nn, A.row_ptr.p, A.nptr.p, A.ncol.p, val, xn, S,
+                                                                      A.n, partials);
+            break;
+        case 20:
+            poly_gram_kernel<T, 20><<<kPgBlocks, kPgThreads, 0, s>>>(nn, A.row_ptr.p, A.nptr.p, A.ncol.p, val, xn, S,
+                                                                      A.n, partials);
+            break;
+        default:
+            poly_gram_kernel<T, 35><<<kPgBlocks, kPgThreads, 0, s>>>(nn, A.row_ptr.p, A.nptr.p, A.ncol.p, val, xn, S,
+                                                                      A.n, partials);
+            break;
+    }
+}
+
+void launch_poly_gram(hipStream_t s, int prec, const Matrix& A, const float4* xn, int nmono, const void* S,
+                      double* partials, double* G) {
+    if (prec == PREC64)
+        poly_gram_t<double>(s, A, xn, nmono, A.val64.p, (const double*)S, partials);
+    else
+        poly_gram_t<float>(s, A, xn, nmono, A.val32.p, (const float*)S, partials);
+    const uint32_t np = (uint32_t)(3 * nmono * (3 * nmono + kPgStored));
+    gram_reduce_kernel<<<cdiv(np, 32), 256, 0, s>>>(kPgBlocks, np, partials, G);
+}
 
 void launch_gram(hipStream_t s, int prec, uint32_t n, int m, int mw, const void* V, const void* W, double* partials,
                  double* G) {
-    const uint32_t nb = gram_blocks();
+    const uint32_t nb = gram_blocks(m, mw);
     const size_t es = prec == PREC64 ? 8 : 4;
     const size_t lds = (size_t)(((m + 3) & ~3) + ((mw + 3) & ~3)) * kGramRows * es;
     const int nt = gram_threads(m, mw);

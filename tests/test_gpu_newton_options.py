@@ -4,6 +4,8 @@ boundary's state handling: per-iteration histories, the unloaded-state reset,
 residual-only assembly without side effects, new coordinates on a kept
 topology, and repeated external matrices with one pattern.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -322,3 +324,33 @@ def test_coarse_moments(fm, S, degree, prec):
     r2 = r.copy()
     r2[cfg["dir_dofs"]] += 1.0
     assert np.array_equal(S.coarse_moments(r2, degree, p), got)
+
+
+def test_fused_polynomial_gram_matches_generic(tmp_path):
+    """The fused pass for the polynomial fields' Gram rows (no A Z formed)
+    against the generic path (every basis column's product, one dense Gram
+    matrix) over eight load steps with the bench's inner-solve options: the
+    same Newton and PCG iteration counts in FP64 (within two PCG iterations
+    per step in FP32, whose Gram entries round differently) and the same
+    final states to the solves' accuracy."""
+    import subprocess
+    import sys
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    outs = {}
+    for mode in ("fused", "generic"):
+        env = dict(os.environ)
+        env.pop("FM_GALERKIN", None)
+        if mode == "generic":
+            env["FM_GALERKIN"] = "generic"
+        f = str(tmp_path / f"{mode}.npz")
+        p = subprocess.run([sys.executable, os.path.join(here, "galerkin_check.py"), f], env=env,
+                           capture_output=True, text=True, timeout=300)
+        assert p.returncode == 0, p.stdout + p.stderr
+        outs[mode] = np.load(f)
+    a, b = outs["fused"], outs["generic"]
+    assert np.array_equal(a["fp64_its"], b["fp64_its"]), (a["fp64_its"], b["fp64_its"])
+    assert np.array_equal(a["mixed_its"][:, 0], b["mixed_its"][:, 0])
+    assert np.abs(a["mixed_its"][:, 1] - b["mixed_its"][:, 1]).max() <= 2, (a["mixed_its"], b["mixed_its"])
+    for name in ("mixed", "fp64"):
+        assert rel(a[name + "_u"], b[name + "_u"]) < 1e-6, name
