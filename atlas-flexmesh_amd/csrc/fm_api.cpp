@@ -2010,7 +2010,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     // history; a start state that already passes the absolute test takes no
     // predictor
     const bool at_rest = S.r0 < cfg.tol_absolute;
-    const int max_order = cfg.extrap_order > 0 ? std::min(cfg.extrap_order, 3) : 2;
+    const int max_order = cfg.extrap_order > 0 ? std::min(cfg.extrap_order, fm_ctx::kExtrapMax) : 2;
     int order = 0;
     if (cfg.lift == 2 && !at_rest) {
         // the highest degree whose states are known at distinct load factors
@@ -2024,22 +2024,21 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     const bool extrapolate = order > 0;
     if (cfg.lift == 2) {
         FM_HIP(ctx->sec_start.alloc(n));
-        FM_HIP(ctx->sec_u1.alloc(n));
-        FM_HIP(ctx->sec_u2.alloc(n));
-        if (max_order >= 3) FM_HIP(ctx->sec_u3.alloc(n));
+        for (int i = 0; i < std::max(max_order, 2); ++i) FM_HIP(ctx->sec_old[i].alloc(n));
         FM_HIP(hipMemcpyAsync(ctx->sec_start.p, ctx->u.p, n * 8ull, hipMemcpyDeviceToDevice, s));
     }
     if (extrapolate) {
-        // Lagrange weights of u (at sec_lf[0]), u1, u2, u3 at lf
-        double w[4] = {0.0, 0.0, 0.0, 0.0};
+        // Lagrange weights of u (at sec_lf[0]) and sec_old[a - 1] (at sec_lf[a]) at lf
+        Lincomb lc{};
+        lc.k = order + 1;
         for (int a = 0; a <= order; ++a) {
             double v = 1.0;
             for (int b = 0; b <= order; ++b)
                 if (b != a) v *= (lf - ctx->sec_lf[b]) / (ctx->sec_lf[a] - ctx->sec_lf[b]);
-            w[a] = v;
+            lc.w[a] = v;
+            lc.x[a] = a == 0 ? ctx->u.p : ctx->sec_old[a - 1].p;
         }
-        launch_lincomb4(s, n, w[0], ctx->u.p, w[1], ctx->sec_u1.p, w[2], order >= 2 ? ctx->sec_u2.p : nullptr, w[3],
-                        order >= 3 ? ctx->sec_u3.p : nullptr, ctx->ut.p);
+        launch_lincomb(s, n, lc, ctx->ut.p);
         launch_fill_f64(s, ctx->dD.p, n, 0.0);
         launch_dir_increment(s, ndir, ctx->d_dir_dofs.p, ctx->d_dir_vals.p, lf, ctx->ut.p, ctx->dD.p);
         launch_axpy_to(s, n, ctx->ut.p, 1.0, ctx->dD.p, ctx->u.p);
@@ -2156,21 +2155,18 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     S.newton_iters = it;
     S.converged = done(rn, energy_stop);
     if (cfg.lift == 2 && S.converged) {
-        // the start state becomes u1, u1 becomes u2, this step's converged state the current one
-        // (u2 becomes u3 when the cubic is in use)
-        const int keep_n = max_order >= 3 ? 4 : 3;
-        if (keep_n == 4) {
-            std::swap(ctx->sec_u3.p, ctx->sec_u2.p);
-            std::swap(ctx->sec_u3.n, ctx->sec_u2.n);
+        // the start state (the previous converged one) becomes sec_old[0], each
+        // older one moves down, this step's converged state is the current
+        // one; the states kept are those the degree uses (at least three)
+        const int keep_n = std::max(max_order, 2) + 1;
+        for (int i = keep_n - 2; i >= 1; --i) {
+            std::swap(ctx->sec_old[i].p, ctx->sec_old[i - 1].p);
+            std::swap(ctx->sec_old[i].n, ctx->sec_old[i - 1].n);
         }
-        std::swap(ctx->sec_u2.p, ctx->sec_u1.p);
-        std::swap(ctx->sec_u2.n, ctx->sec_u1.n);
-        std::swap(ctx->sec_u1.p, ctx->sec_start.p);
-        std::swap(ctx->sec_u1.n, ctx->sec_start.n);
+        std::swap(ctx->sec_old[0].p, ctx->sec_start.p);
+        std::swap(ctx->sec_old[0].n, ctx->sec_start.n);
         ctx->sec_n = ctx->sec_n >= 1 ? std::min(ctx->sec_n + 1, keep_n) : 1;
-        ctx->sec_lf[3] = ctx->sec_lf[2];
-        ctx->sec_lf[2] = ctx->sec_lf[1];
-        ctx->sec_lf[1] = ctx->sec_lf[0];
+        for (int i = keep_n - 1; i >= 1; --i) ctx->sec_lf[i] = ctx->sec_lf[i - 1];
         ctx->sec_lf[0] = lf;
     } else {
         ctx->sec_n = 0;

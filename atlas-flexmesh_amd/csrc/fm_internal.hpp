@@ -153,9 +153,9 @@ struct Ctx {
     bool node_asm = false;
     bool have_n2e = false;  // node -> incident element lists (colour order) on the device
     // converged states for the extrapolating predictor (lift = 2): the device
-    // state u at load factor sec_lf[0] (sec_n >= 1), sec_u1 at sec_lf[1]
-    // (sec_n >= 2), sec_u2 at sec_lf[2] (sec_n >= 3) and, for the cubic,
-    // sec_u3 at sec_lf[3] (sec_n == 4); cleared by anything that changes the
+    // state u at load factor sec_lf[0] (sec_n >= 1) and the older ones,
+    // sec_old[i] at sec_lf[i + 1] (sec_n >= i + 2), newest first, as many as
+    // the extrapolation's degree uses; cleared by anything that changes the
     // problem or u
     // Galerkin-start basis per PCG precision ([0] FP64, [1] FP32): V is
     // n x cs_cols (leading dimension n) with a ring of the latest twelve
@@ -183,8 +183,9 @@ struct Ctx {
     // lagged preconditioner reuses it within the load step
     int lag_nmono = 0, lag_prec = -1;
     int gal_n[2] = {0, 0}, gal_next[2] = {0, 0};
-    DevBuf<double> sec_u1, sec_u2, sec_u3, sec_start;
-    double sec_lf[4] = {0.0, 0.0, 0.0, 0.0};
+    static constexpr int kExtrapMax = 4;  // highest degree of the extrapolation
+    DevBuf<double> sec_old[kExtrapMax], sec_start;
+    double sec_lf[kExtrapMax + 1] = {0.0, 0.0, 0.0, 0.0, 0.0};
     int sec_n = 0;
     bool have_material = false;
 
@@ -366,9 +367,13 @@ void launch_elem_rot(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, d
 // maximum stretch, F (9)
 constexpr int kFieldCount = 21;
 void launch_element_fields(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, double* out);
-// out = a x + b y (+ c z when z is given) (+ d w when w is given)
-void launch_lincomb4(hipStream_t s, uint32_t n, double a, const double* x, double b, const double* y, double c,
-                     const double* z, double d, const double* w, double* out);
+// out = sum over i < k of w[i] x[i] (k <= 5)
+struct Lincomb {
+    const double* x[5];
+    double w[5];
+    int k;
+};
+void launch_lincomb(hipStream_t s, uint32_t n, const Lincomb& c, double* out);
 // Galerkin start of a linear solve (precision prec: PREC32 or PREC64 vectors)
 // block partials of a.b with FP64 accumulation; returns #partials
 uint32_t launch_dot(hipStream_t s, int prec, uint32_t n, const void* a, const void* b, double* partials);

@@ -2887,14 +2887,11 @@ __global__ __launch_bounds__(256) void zz_err_kernel(uint32_t ne, const uint32_t
     if (threadIdx.x == 0) partials[blockIdx.x] = bs;
 }
 
-__global__ void lincomb4_kernel(uint32_t n, double a, const double* __restrict__ x, double b,
-                                const double* __restrict__ y, double c, const double* __restrict__ z, double d,
-                                const double* __restrict__ w, double* __restrict__ out) {
+__global__ void lincomb_kernel(uint32_t n, Lincomb c, double* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double v = a * x[i] + b * y[i];
-    if (z) v += c * z[i];
-    if (w) v += d * w[i];
+    double v = c.w[0] * c.x[0][i];
+    for (int q = 1; q < c.k; ++q) v += c.w[q] * c.x[q][i];
     out[i] = v;
 }
 
@@ -3592,9 +3589,8 @@ uint32_t launch_zz(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, con
     zz_err_kernel<<<nb, 256, 0, s>>>(a.ne, a.conn, eiperm, zz, raw, rec, err, partials);
     return nb;
 }
-void launch_lincomb4(hipStream_t s, uint32_t n, double a, const double* x, double b, const double* y, double c,
-                     const double* z, double d, const double* w, double* out) {
-    if (n) lincomb4_kernel<<<cdiv(n, 256), 256, 0, s>>>(n, a, x, b, y, c, z, d, w, out);
+void launch_lincomb(hipStream_t s, uint32_t n, const Lincomb& c, double* out) {
+    if (n) lincomb_kernel<<<cdiv(n, 256), 256, 0, s>>>(n, c, out);
 }
 void launch_element_fields(hipStream_t s, const ElemArgs& a, const uint32_t* eiperm, double* out) {
     if (a.ne) element_fields_kernel<<<cdiv(a.ne, 128), 128, 0, s>>>(a, eiperm, out);

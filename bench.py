@@ -319,7 +319,7 @@ def main():
                     help="1: those polynomial fields also form the coarse level of a two-level preconditioner "
                          "in every PCG iteration (default 1 with the extrapolating predictor)")
     ap.add_argument("--order", type=int, default=None,
-                    help="degree of the load-step extrapolation through the previous converged states (1..3; "
+                    help="degree of the load-step extrapolation through the previous converged states (1..4; "
                          "default 3, cubic through four)")
     ap.add_argument("--lag", type=int, default=None,
                     help="1: the later Newton iterations of a load step keep the first iteration's ILU(0) factor "

@@ -197,7 +197,7 @@ typedef struct {
                              inverse also gives those solves' Galerkin start (a lagged
                              preconditioner: the Krylov operator is the current tangent); 0 = off */
     int extrap_order;     /* lift = 2: degree of the extrapolation through the previous converged
-                             states, 1..3 (3: cubic through four); 0 = the default, quadratic */
+                             states, 1..4 (3: cubic through four); 0 = the default, quadratic */
 } fm_newton_cfg;
 
 typedef struct {

@@ -243,7 +243,7 @@ def test_repeated_matrix_pattern_uploads_values(fm, S, O):
                           ("fp64", 2, 0, 0, 2, 2), ("mixed", 3, 1, 0, 2, 2), ("fp64", 2, 1, 0, 2, 2),
                           ("mixed", 4, 1, 0, 2, 2), ("fp64", 1, 1, 0, 2, 2), ("mixed", 3, 1, 1, 2, 2),
                           ("fp64", 3, 1, 1, 2, 2), ("mixed", 0, 0, 1, 2, 2), ("mixed", 4, 1, 1, 3, 6),
-                          ("fp64", 4, 1, 1, 3, 6), ("mixed", 2, 1, 1, 1, 4)])
+                          ("fp64", 4, 1, 1, 3, 6), ("mixed", 2, 1, 1, 1, 4), ("mixed", 4, 1, 1, 4, 6)])
 def test_galerkin_start_and_target_forcing(fm, S, O, prec, coarse, additive, lag, order, recycle):
     """recycle = 2 with the target-aware forcing term (inner_tol 1e-3, at most
     0.1), and with the polynomial fields of degree <= coarse in the Galerkin
