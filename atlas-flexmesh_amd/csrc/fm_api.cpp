@@ -1349,6 +1349,7 @@ int fm_field_lognormal(uint32_t n, double median, double sigma, uint64_t seed, d
 int fm_set_mesh(fm_ctx* ctx, uint32_t n_nodes, const double* X, uint32_t n_elems, const uint32_t* conn) {
     if (!ctx || !X || !conn || !n_nodes || !n_elems) return fail(ctx, FM_EINVAL, "fm_set_mesh: bad arguments");
     ctx->sec_n = 0;
+    ctx->slope_ok = false;
     ctx->gal_n[0] = ctx->gal_n[1] = 0;
     ctx->cs_poly_ok[0] = ctx->cs_poly_ok[1] = false;  // the polynomial fields depend on X and the constraints
     if (3ull * n_nodes > 0xFFFFFFFFull) return fail(ctx, FM_EINVAL, "too many nodes");
@@ -1382,6 +1383,7 @@ int fm_set_coords(fm_ctx* ctx, uint32_t n_nodes, const double* X) {
     ctx->hX.assign(X, X + 3ull * n_nodes);
     FM_HIP(hipMemcpy(ctx->X.p, X, 3ull * n_nodes * 8, hipMemcpyHostToDevice));
     ctx->sec_n = 0;
+    ctx->slope_ok = false;
     ctx->gal_n[0] = ctx->gal_n[1] = 0;
     ctx->cs_poly_ok[0] = ctx->cs_poly_ok[1] = false;  // the polynomial fields depend on X and the constraints
     Matrix& M = ctx->tangent;
@@ -1394,6 +1396,7 @@ int fm_set_coords(fm_ctx* ctx, uint32_t n_nodes, const double* X) {
 int fm_set_material(fm_ctx* ctx, const double* mu_e, const double* lam_e) {
     if (!ctx || !mu_e || !lam_e) return fail(ctx, FM_EINVAL, "fm_set_material: bad arguments");
     ctx->sec_n = 0;
+    ctx->slope_ok = false;
     ctx->gal_n[0] = ctx->gal_n[1] = 0;
     if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
     ctx->hmu.assign(mu_e, mu_e + ctx->ne);
@@ -1405,6 +1408,7 @@ int fm_set_material(fm_ctx* ctx, const double* mu_e, const double* lam_e) {
 int fm_set_bcs(fm_ctx* ctx, uint32_t n_dir, const uint32_t* dofs, const double* vals) {
     if (!ctx || (n_dir && !dofs)) return fail(ctx, FM_EINVAL, "fm_set_bcs: bad arguments");
     ctx->sec_n = 0;
+    ctx->slope_ok = false;
     ctx->gal_n[0] = ctx->gal_n[1] = 0;
     ctx->cs_poly_ok[0] = ctx->cs_poly_ok[1] = false;  // the polynomial fields depend on X and the constraints
     if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
@@ -1431,6 +1435,7 @@ int fm_set_bcs(fm_ctx* ctx, uint32_t n_dir, const uint32_t* dofs, const double* 
 int fm_set_load(fm_ctx* ctx, const double* f_ext) {
     if (!ctx) return FM_EINVAL;
     ctx->sec_n = 0;
+    ctx->slope_ok = false;
     ctx->gal_n[0] = ctx->gal_n[1] = 0;
     if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
     if (!f_ext) {
@@ -1828,6 +1833,7 @@ void fm_newton_cfg_default(fm_newton_cfg* c) {
     c->coarse_additive = 0;
     c->lag_precond = 0;
     c->extrap_order = 0;
+    c->slope_start = 0;
 }
 
 int fm_set_u(fm_ctx* ctx, const double* u) {
@@ -1840,11 +1846,13 @@ int fm_set_u(fm_ctx* ctx, const double* u) {
         FM_TRY(sync(ctx));
         ctx->u_valid = true;
         ctx->sec_n = 1;
+        ctx->slope_ok = false;
         ctx->gal_n[0] = ctx->gal_n[1] = 0;
         ctx->sec_lf[0] = 0.0;
         return FM_OK;
     }
     ctx->sec_n = 0;
+    ctx->slope_ok = false;
     ctx->gal_n[0] = ctx->gal_n[1] = 0;
     FM_HIP(hipMemcpy(ctx->u.p, u, 3ull * ctx->nn * 8, hipMemcpyHostToDevice));
     ctx->u_valid = true;
@@ -1922,6 +1930,9 @@ int fm_newton_step(fm_ctx* ctx, uint32_t step, double* u, const fm_newton_cfg* c
     return fm_newton_solve_to(ctx, (double)step / (double)cfg.n_steps, u, &cfg, stats);
 }
 
+// the predictor solve that gives the path's slope at the unloaded state
+constexpr double kSlopeTol = 1e-5;
+
 // FM_TRACE_SOLVES=1: one line per linear solve of the Newton loop on stderr
 // (load factor, Newton iteration or -1 for the predictor solve, forcing
 // term, PCG iterations, lagged preconditioner)
@@ -1955,6 +1966,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         FM_HIP(hipMemcpyAsync(ctx->u.p, u, n * 8ull, hipMemcpyHostToDevice, s));
         ctx->u_valid = true;
         ctx->sec_n = 0;  // a state from outside: no known history
+        ctx->slope_ok = false;
         ctx->gal_n[0] = ctx->gal_n[1] = 0;
     }
     if (!ctx->u_valid) {
@@ -1962,6 +1974,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         launch_fill_f64(s, ctx->u.p, n, 0.0);
         ctx->u_valid = true;
         ctx->sec_n = 1;
+        ctx->slope_ok = false;
         ctx->gal_n[0] = ctx->gal_n[1] = 0;
         ctx->sec_lf[0] = 0.0;
     }
@@ -2012,6 +2025,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     const bool at_rest = S.r0 < cfg.tol_absolute;
     const int max_order = cfg.extrap_order > 0 ? std::min(cfg.extrap_order, fm_ctx::kExtrapMax) : 2;
     int order = 0;
+    bool hermite = false;
     if (cfg.lift == 2 && !at_rest) {
         // the highest degree whose states are known at distinct load factors
         for (int o = std::min(max_order, ctx->sec_n - 1); o >= 1 && order == 0; --o) {
@@ -2019,6 +2033,20 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
             for (int a = 0; a <= o; ++a)
                 for (int b = a + 1; b <= o; ++b) distinct &= ctx->sec_lf[a] != ctx->sec_lf[b];
             if (distinct) order = o;
+        }
+        // slope_start: while the states alone fall short of the degree, all
+        // m known states and the slope at one of them give degree m
+        const int m = ctx->sec_n;
+        if (cfg.slope_start && ctx->slope_ok && order < max_order && m >= 1 && m <= max_order) {
+            bool has = false, distinct = true;
+            for (int a = 0; a < m; ++a) {
+                has |= ctx->sec_lf[a] == ctx->slope_lf;
+                for (int b = a + 1; b < m; ++b) distinct &= ctx->sec_lf[a] != ctx->sec_lf[b];
+            }
+            if (has && distinct && !(m == 1 && ctx->sec_lf[0] == lf)) {
+                order = m;
+                hermite = true;
+            }
         }
     }
     const bool extrapolate = order > 0;
@@ -2028,15 +2056,43 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         FM_HIP(hipMemcpyAsync(ctx->sec_start.p, ctx->u.p, n * 8ull, hipMemcpyDeviceToDevice, s));
     }
     if (extrapolate) {
-        // Lagrange weights of u (at sec_lf[0]) and sec_old[a - 1] (at sec_lf[a]) at lf
         Lincomb lc{};
         lc.k = order + 1;
-        for (int a = 0; a <= order; ++a) {
-            double v = 1.0;
-            for (int b = 0; b <= order; ++b)
-                if (b != a) v *= (lf - ctx->sec_lf[b]) / (ctx->sec_lf[a] - ctx->sec_lf[b]);
-            lc.w[a] = v;
-            lc.x[a] = a == 0 ? ctx->u.p : ctx->sec_old[a - 1].p;
+        if (!hermite) {
+            // Lagrange weights of u (at sec_lf[0]) and sec_old[a - 1] (at sec_lf[a]) at lf
+            for (int a = 0; a <= order; ++a) {
+                double v = 1.0;
+                for (int b = 0; b <= order; ++b)
+                    if (b != a) v *= (lf - ctx->sec_lf[b]) / (ctx->sec_lf[a] - ctx->sec_lf[b]);
+                lc.w[a] = v;
+                lc.x[a] = a == 0 ? ctx->u.p : ctx->sec_old[a - 1].p;
+            }
+        } else {
+            // Hermite weights: the m = order states and the slope at slope_lf
+            // reproduce every polynomial of degree <= m, sum_a w_a l_a^k +
+            // w_d k slope_lf^(k-1) = lf^k for k = 0..m (Gaussian elimination
+            // with partial pivoting; the last unknown is the slope's weight)
+            const int m = order, N = m + 1;
+            double M[5][6];
+            for (int k = 0; k < N; ++k) {
+                for (int a = 0; a < m; ++a) M[k][a] = std::pow(ctx->sec_lf[a], k);
+                M[k][m] = k == 0 ? 0.0 : k * std::pow(ctx->slope_lf, k - 1);
+                M[k][N] = std::pow(lf, k);
+            }
+            for (int c = 0; c < N; ++c) {
+                int piv = c;
+                for (int r = c + 1; r < N; ++r)
+                    if (std::fabs(M[r][c]) > std::fabs(M[piv][c])) piv = r;
+                for (int j = 0; j <= N; ++j) std::swap(M[c][j], M[piv][j]);
+                for (int r = 0; r < N; ++r)
+                    if (r != c) {
+                        const double f = M[r][c] / M[c][c];
+                        for (int j = c; j <= N; ++j) M[r][j] -= f * M[c][j];
+                    }
+            }
+            for (int a = 0; a < N; ++a) lc.w[a] = M[a][N] / M[a][a];
+            for (int a = 0; a < m; ++a) lc.x[a] = a == 0 ? ctx->u.p : ctx->sec_old[a - 1].p;
+            lc.x[m] = ctx->slope.p;
         }
         launch_lincomb(s, n, lc, ctx->ut.p);
         launch_fill_f64(s, ctx->dD.p, n, 0.0);
@@ -2050,12 +2106,29 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
                                ctx->partials.p);
         launch_sub_masked(s, n, ctx->R.p, ctx->lift.p, ctx->isdir.p, ctx->hv.p);
         int its = 0;
-        FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, lin_tol, lin_cap, &its, inexact, &S.pcg_cap_hits, 0,
+        // from the unloaded state the predictor solve's increment over lf is
+        // the path's slope there (the linearised problem is linear in the
+        // load factor): with slope_start it is solved to kSlopeTol and kept
+        const bool want_slope = cfg.lift == 2 && cfg.slope_start && ctx->sec_n == 1 && ctx->sec_lf[0] == 0.0 && lf > 0.0;
+        const double ptol = want_slope ? std::min(lin_tol, kSlopeTol) : lin_tol;
+        FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, ptol, lin_cap, &its, inexact, &S.pcg_cap_hits, 0,
                             coarse, false, additive));
         S.pcg_iters += its;
         S.factorizations++;
-        trace_solve(lf, -1, lin_tol, its, false);
+        trace_solve(lf, -1, ptol, its, false);
         launch_apply_predictor(s, n, ctx->du.p, ctx->dD.p, ctx->isdir.p, ctx->u.p);
+        if (want_slope) {
+            FM_HIP(ctx->slope.alloc(n));
+            Lincomb lc{};
+            lc.k = 2;
+            lc.x[0] = ctx->u.p;
+            lc.w[0] = 1.0 / lf;
+            lc.x[1] = ctx->sec_start.p;
+            lc.w[1] = -1.0 / lf;
+            launch_lincomb(s, n, lc, ctx->slope.p);
+            ctx->slope_lf = 0.0;
+            ctx->slope_ok = true;
+        }
     } else {
         launch_axpy_to(s, n, ctx->u.p, 1.0, ctx->dD.p, ctx->u.p);
     }
@@ -2170,6 +2243,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         ctx->sec_lf[0] = lf;
     } else {
         ctx->sec_n = 0;
+        ctx->slope_ok = false;
         ctx->gal_n[0] = ctx->gal_n[1] = 0;
     }
     S.rel_res = S.r0 > 0 ? rn / S.r0 : 0.0;

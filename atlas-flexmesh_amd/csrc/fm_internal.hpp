@@ -186,6 +186,12 @@ struct Ctx {
     static constexpr int kExtrapMax = 4;  // highest degree of the extrapolation
     DevBuf<double> sec_old[kExtrapMax], sec_start;
     double sec_lf[kExtrapMax + 1] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    // du/dlf at load factor slope_lf (slope_ok): the predictor solve's
+    // increment from the unloaded state over its load factor, Hermite data
+    // of the extrapolation while fewer states are known than its degree needs
+    DevBuf<double> slope;
+    double slope_lf = 0.0;
+    bool slope_ok = false;
     int sec_n = 0;
     bool have_material = false;
 

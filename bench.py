@@ -321,6 +321,9 @@ def main():
     ap.add_argument("--order", type=int, default=None,
                     help="degree of the load-step extrapolation through the previous converged states (1..4; "
                          "default 3, cubic through four)")
+    ap.add_argument("--slope", type=int, default=None,
+                    help="1: the first step's predictor solve (to 1e-5) gives the path's slope at the unloaded "
+                         "state, Hermite data of the second and third steps' extrapolation (default 0)")
     ap.add_argument("--lag", type=int, default=None,
                     help="1: the later Newton iterations of a load step keep the first iteration's ILU(0) factor "
                          "and coarse operator (a lagged preconditioner; the tangent is re-assembled) (default 1 "
@@ -405,8 +408,9 @@ def bench(S, cfg, args, warm_last, K):
         inner_tol = 3e-3
     lag = args.lag if args.lag is not None else (1 if additive else 0)
     order = args.order if args.order is not None else 3
+    slope = args.slope if args.slope is not None else 0
     solver_kw = dict(recycle=recycle, inner_tol_max=inner_tol_max, coarse_degree=coarse, coarse_additive=additive,
-                     lag_precond=lag, extrap_order=order)
+                     lag_precond=lag, extrap_order=order, slope_start=slope)
     if warm_last:
         log(f"warm-up pass: load steps 1..{warm_last}")
         run_steps(S, cfg, prec, 1, warm_last, inner_tol, False, args.pcg_max, lift, **solver_kw)
@@ -427,7 +431,7 @@ def bench(S, cfg, args, warm_last, K):
                                f"load steps 1..{K} of {cfg['n_steps']}", "dofs": dofs, "tets": info["ne"],
                    "nnz": info["nnz"], "predictor": "extrapolate" if lift == 2 else "solve",
                    "inner_tol": inner_tol, "inner_tol_max": inner_tol_max, "recycle": recycle,
-                   "coarse_degree": coarse, "coarse_additive": additive, "lag_precond": lag, "extrap_order": order,
+                   "coarse_degree": coarse, "coarse_additive": additive, "lag_precond": lag, "extrap_order": order, "slope_start": slope,
                    "newton_rtol": cfg["newton_rtol"]},
         "newton": {"converged_steps": tot["converged"], "newton_iters": tot["newton_iters"],
                    "pcg_iters": tot["pcg_iters"], "newton_it_per_s": tot["newton_iters"] / wall,

@@ -198,6 +198,12 @@ typedef struct {
                              preconditioner: the Krylov operator is the current tangent); 0 = off */
     int extrap_order;     /* lift = 2: degree of the extrapolation through the previous converged
                              states, 1..4 (3: cubic through four); 0 = the default, quadratic */
+    int slope_start;      /* lift = 2: the predictor solve from the unloaded state (the first step of
+                             a load path) is solved to 1e-5 and its increment over the load factor
+                             kept as the path's slope there; while fewer states are known than
+                             extrap_order needs, the extrapolation takes the known states and
+                             that slope (Hermite data: quadratic for the second step, cubic for
+                             the third); 0 = off */
 } fm_newton_cfg;
 
 typedef struct {
