@@ -1834,6 +1834,7 @@ void fm_newton_cfg_default(fm_newton_cfg* c) {
     c->lag_precond = 0;
     c->extrap_order = 0;
     c->slope_start = 0;
+    c->factor_steps = 0;
 }
 
 int fm_set_u(fm_ctx* ctx, const double* u) {
@@ -2164,9 +2165,17 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         if (!keep) {
             const int fp = kprec == PREC32 ? PREC32 : PREC64;
             lag = lag_on && ctx->tangent.lu_valid[fp];
+            // factor_steps: a step that starts from an extrapolation of degree
+            // >= 2 keeps the factor of an earlier step while it has served
+            // fewer than factor_steps steps
+            const bool keep_factor = it == 0 && cfg.lag_precond && cfg.factor_steps > 1 && order >= 2 &&
+                                     (kprec == PREC32 || inexact) && ctx->tangent.lu_valid[fp] &&
+                                     ctx->fact_age < cfg.factor_steps;
             FM_TRY(assemble(ctx, ctx->u.p, true, aprec, true, nullptr, FM_K_ASSEMBLE));
             S.assemblies++;
-            if (lag) ctx->tangent.lu_valid[fp] = true;
+            if (lag || keep_factor) ctx->tangent.lu_valid[fp] = true;
+            if (it == 0) ctx->fact_age = keep_factor ? ctx->fact_age + 1 : 1;
+            lag |= keep_factor;
         }
         int its = 0;
         // R of the current u is in ctx->R (last residual evaluation)

@@ -192,6 +192,9 @@ struct Ctx {
     DevBuf<double> slope;
     double slope_lf = 0.0;
     bool slope_ok = false;
+    // load steps whose first Newton iteration used the current tangent factor
+    // (the factor_steps option)
+    int fact_age = 0;
     int sec_n = 0;
     bool have_material = false;
 

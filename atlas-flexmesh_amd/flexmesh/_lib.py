@@ -33,7 +33,8 @@ class NewtonCfg(C.Structure):
                 ("record_energy", C.c_int), ("inexact", C.c_int), ("recycle", C.c_int),
                 ("inner_tol_max", C.c_double), ("coarse_degree", C.c_int),
                 ("coarse_additive", C.c_int), ("lag_precond", C.c_int),
-                ("extrap_order", C.c_int), ("slope_start", C.c_int)]
+                ("extrap_order", C.c_int), ("slope_start", C.c_int),
+                ("factor_steps", C.c_int)]
 
 
 FIELD_NAMES = ("det_F", "cauchy", "von_mises", "pressure", "max_principal", "energy", "max_stretch", "F")

@@ -324,6 +324,9 @@ def main():
     ap.add_argument("--slope", type=int, default=None,
                     help="1: the first step's predictor solve (to 1e-5) gives the path's slope at the unloaded "
                          "state, Hermite data of the second and third steps' extrapolation (default 0)")
+    ap.add_argument("--factor-steps", type=int, default=None,
+                    help="with the lagged preconditioner: load steps one ILU(0) factor serves (default 6 with the "
+                         "coarse level; 1: a new factor every step)")
     ap.add_argument("--lag", type=int, default=None,
                     help="1: the later Newton iterations of a load step keep the first iteration's ILU(0) factor "
                          "and coarse operator (a lagged preconditioner; the tangent is re-assembled) (default 1 "
@@ -409,8 +412,9 @@ def bench(S, cfg, args, warm_last, K):
     lag = args.lag if args.lag is not None else (1 if additive else 0)
     order = args.order if args.order is not None else 3
     slope = args.slope if args.slope is not None else 0
+    fsteps = args.factor_steps if args.factor_steps is not None else (6 if lag else 1)
     solver_kw = dict(recycle=recycle, inner_tol_max=inner_tol_max, coarse_degree=coarse, coarse_additive=additive,
-                     lag_precond=lag, extrap_order=order, slope_start=slope)
+                     lag_precond=lag, extrap_order=order, slope_start=slope, factor_steps=fsteps)
     if warm_last:
         log(f"warm-up pass: load steps 1..{warm_last}")
         run_steps(S, cfg, prec, 1, warm_last, inner_tol, False, args.pcg_max, lift, **solver_kw)
@@ -431,7 +435,7 @@ def bench(S, cfg, args, warm_last, K):
                                f"load steps 1..{K} of {cfg['n_steps']}", "dofs": dofs, "tets": info["ne"],
                    "nnz": info["nnz"], "predictor": "extrapolate" if lift == 2 else "solve",
                    "inner_tol": inner_tol, "inner_tol_max": inner_tol_max, "recycle": recycle,
-                   "coarse_degree": coarse, "coarse_additive": additive, "lag_precond": lag, "extrap_order": order, "slope_start": slope,
+                   "coarse_degree": coarse, "coarse_additive": additive, "lag_precond": lag, "extrap_order": order, "slope_start": slope, "factor_steps": fsteps,
                    "newton_rtol": cfg["newton_rtol"]},
         "newton": {"converged_steps": tot["converged"], "newton_iters": tot["newton_iters"],
                    "pcg_iters": tot["pcg_iters"], "newton_it_per_s": tot["newton_iters"] / wall,
@@ -484,6 +488,8 @@ def bench(S, cfg, args, warm_last, K):
                                            if coarse and additive else "")
                                         + ("the first iteration's factor and coarse operator kept for the later "
                                            "iterations of a step, " if lag else "")
+                                        + (f"one ILU(0) factor for up to {fsteps} load steps, " if lag and fsteps > 1
+                                           else "")
                                         + (f"forcing term clamp(0.5 target/||R||, {inner_tol:g}, {inner_tol_max:g})"
                                            if inner_tol_max > inner_tol else f"forcing term {inner_tol:g}")
                                         + ", with the FP64 tangent, SpMV, ILU(0) and PCG"}

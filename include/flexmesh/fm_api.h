@@ -204,6 +204,12 @@ typedef struct {
                              extrap_order needs, the extrapolation takes the known states and
                              that slope (Hermite data: quadratic for the second step, cubic for
                              the third); 0 = off */
+    int factor_steps;     /* with lag_precond: a load step that starts from an extrapolation of
+                             degree >= 2 re-assembles the tangent but keeps the ILU(0) factor of
+                             an earlier step while that factor has served fewer than factor_steps
+                             steps (a lagged preconditioner across steps; the Galerkin start and
+                             the coarse operator use the current tangent); 0 or 1 = a new factor
+                             every step */
 } fm_newton_cfg;
 
 typedef struct {

@@ -63,7 +63,9 @@ def test_algorithmic_bytes_of_the_node_spmv():
 def test_committed_traffic_summary_feeds_every_roofline_class():
     """bench.py's roofline.traffic comes from the machine-readable lines of the
     committed PMC summary: every class in both precisions must be there, and
-    each figure must be at least the class's algorithmic bytes at 128^3 hexes."""
+    each figure must be at least 0.85 of the class's algorithmic bytes at
+    128^3 hexes (the matrix values alone, which no cache holds, are 0.84 of
+    the SpMV's; its vectors may partly come from the 256 MB last-level cache)."""
     import bench
 
     info = {"n": 6440067, "nnz": 286222473, "ne": 12582912, "nn": 2146689, "spmv": "node", "sweep": "chain"}
@@ -71,5 +73,5 @@ def test_committed_traffic_summary_feeds_every_roofline_class():
         for kind in ("spmv", "ilu_fwd", "ilu_bwd", "ilu_factor"):
             t = bench.measured_traffic(kind, prec, (128, 128, 128))
             assert t is not None, (kind, prec)
-            assert t >= 0.95 * bench.kernel_bytes(info, kind, v), (kind, prec, t)
+            assert t >= 0.85 * bench.kernel_bytes(info, kind, v), (kind, prec, t)
     assert bench.measured_traffic("spmv", "fp32", (64, 64, 64)) is None
