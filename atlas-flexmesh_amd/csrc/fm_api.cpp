@@ -503,6 +503,7 @@ int factorize(fm_ctx* ctx, Matrix& A, int prec, bool pack = true) {
             launch_ilu_factor(ctx->stream, prec, A, val, lu, ctx->st.p, ctx->d_err.p);
         }
         FM_TRY(check_launch(ctx, "ilu_factor"));
+        if (&A == &ctx->tangent) ++ctx->fact_gen;
         A.pk_valid[prec] = false;
         if (pack) FM_TRY(pack_factor(ctx, A, prec));
         FM_TRY(check_device_error(ctx));  // a pivot wait that hit its bound
@@ -906,7 +907,7 @@ void pivoted_cholesky(const G& g, const int* cols, int nc, std::vector<double>& 
 
 // additive: also E^{-1} over the polynomial fields for the additive coarse
 // correction of the PCG that follows (ctx->ad_nmono)
-int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool additive) {
+int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool additive, bool reuse_poly = false) {
     hipStream_t s = ctx->stream;
     const uint32_t n = A.n;
     const int pi = p == PREC64 ? 0 : 1;
@@ -954,6 +955,16 @@ int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool ad
         ctx->cs_partials.n, std::max((size_t)gram_blocks(m, m + 1) * m * (m + 1), (size_t)poly_gram_blocks() * np))));
     FM_HIP(ctx->cs_G.alloc(std::max<size_t>(ctx->cs_G.n, (size_t)m * (m + 1) + np)));
     static_assert(kPolyGramStored == kRing + 1, "the fused pass reads the ring products and b");
+    // reuse_poly (a step that keeps an earlier step's factor): Z^T A Z of that
+    // step's start stays; the moments of the stored columns (Z^T A v_ring,
+    // Z^T b) come from the coarse level's moment kernel
+    const bool reuse = fused && reuse_poly && ctx->gpp_ok && ctx->gpp_gen == ctx->fact_gen &&
+                       ctx->gpp_nmono == nmono && ctx->gpp_prec == p;
+    const uint32_t nmb = coarse_moment_blocks();
+    if (reuse) {
+        FM_HIP(ctx->cs_partials.alloc(std::max<size_t>(ctx->cs_partials.n, (size_t)nmb * mp)));
+        FM_HIP(ctx->cs_G.alloc(std::max<size_t>(ctx->cs_G.n, (size_t)m * (m + 1) + (size_t)kPolyGramStored * mp)));
+    }
     {
         ProfScope ps(ctx, FM_K_GALERKIN);
         for (int q = 0; q < nc; ++q)
@@ -966,17 +977,57 @@ int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool ad
             // ring slots that are not candidates hold stale columns: their
             // entries are never read
             if (ring_use > 0) launch_gram(s, p, n, m, m + 1, V, W, ctx->cs_partials.p, ctx->cs_G.p);
-            launch_poly_gram(s, p, A, ctx->cs_xn.p, nmono, W, ctx->cs_partials.p, ctx->cs_G.p + (size_t)m * (m + 1));
+            if (reuse) {
+                // the moment kernels return at once on a finished PCG state
+                PcgState init{};
+                *ctx->h_st = init;
+                FM_HIP(hipMemcpyAsync(ctx->st.p, ctx->h_st, sizeof(PcgState), hipMemcpyHostToDevice, s));
+                for (int j = 0; j < kPolyGramStored; ++j) {
+                    bool used = j == kRing;
+                    for (int q = 0; q < nc && !used; ++q) used = cand[q] == j;
+                    if (!used) continue;
+                    launch_coarse_moments(s, p, n, nmono, ctx->cs_xn.p, W + (size_t)j * n * es, ctx->cs_partials.p,
+                                          ctx->st.p);
+                    launch_partial_sums(s, nmb, mp, ctx->cs_partials.p, ctx->cs_G.p + (size_t)m * (m + 1) + j * mp);
+                }
+            } else {
+                launch_poly_gram(s, p, A, ctx->cs_xn.p, nmono, W, ctx->cs_partials.p,
+                                 ctx->cs_G.p + (size_t)m * (m + 1));
+            }
         } else {
             launch_gram(s, p, n, m, m + 1, V + (size_t)c0 * n * es, W + (size_t)c0 * n * es, ctx->cs_partials.p,
                         ctx->cs_G.p);
         }
     }
     std::vector<double> G((size_t)m * (m + 1) + np);
-    FM_HIP(hipMemcpyAsync(G.data(), ctx->cs_G.p, G.size() * 8, hipMemcpyDeviceToHost, s));
-    FM_TRY(sync(ctx));
-    const double* Gp = G.data() + (size_t)m * (m + 1);  // fused: mp x (mp + kPolyGramStored)
     const int ldp = mp + kPolyGramStored;
+    if (reuse) {
+        // Gp in the fused pass's layout: the kept block, then the moments
+        std::vector<double> Gm((size_t)kPolyGramStored * mp);
+        FM_HIP(hipMemcpyAsync(G.data(), ctx->cs_G.p, (size_t)m * (m + 1) * 8, hipMemcpyDeviceToHost, s));
+        FM_HIP(hipMemcpyAsync(Gm.data(), ctx->cs_G.p + (size_t)m * (m + 1), Gm.size() * 8, hipMemcpyDeviceToHost, s));
+        FM_TRY(sync(ctx));
+        double* gp = G.data() + (size_t)m * (m + 1);
+        for (int r = 0; r < mp; ++r) {
+            for (int c = 0; c < mp; ++c) gp[(size_t)r * ldp + c] = ctx->gpp[(size_t)r * mp + c];
+            for (int j = 0; j < kPolyGramStored; ++j) gp[(size_t)r * ldp + mp + j] = Gm[(size_t)j * mp + r];
+        }
+    } else {
+        FM_HIP(hipMemcpyAsync(G.data(), ctx->cs_G.p, G.size() * 8, hipMemcpyDeviceToHost, s));
+        FM_TRY(sync(ctx));
+        if (fused) {
+            // keep Z^T A Z for the steps that keep the factor this solve forms
+            ctx->gpp.resize((size_t)mp * mp);
+            const double* gp = G.data() + (size_t)m * (m + 1);
+            for (int r = 0; r < mp; ++r)
+                for (int c = 0; c < mp; ++c) ctx->gpp[(size_t)r * mp + c] = gp[(size_t)r * ldp + c];
+            ctx->gpp_nmono = nmono;
+            ctx->gpp_prec = p;
+            ctx->gpp_gen = ctx->fact_gen + (A.lu_valid[p] ? 0 : 1);
+            ctx->gpp_ok = true;
+        }
+    }
+    const double* Gp = G.data() + (size_t)m * (m + 1);  // fused: mp x (mp + kPolyGramStored)
     auto prow = [&](int col) {  // row of polynomial V-column col in Gp
         const int o = col - kRing;
         return (o / ctx->cs_nmono[pi]) * nmono + o % ctx->cs_nmono[pi];
@@ -1048,7 +1099,9 @@ int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool ad
     const int ma = 3 * nmono;
     std::vector<double> Einv((size_t)ma * ma, 0.0);
     ctx->ad_nmono = 0;
-    if (additive && ma > 0) {
+    if (additive && ma > 0 && reuse && ctx->einv_ok && ctx->einv_gen == ctx->gpp_gen) {
+        ctx->ad_nmono = nmono;  // the kept block's inverse is on the device already
+    } else if (additive && ma > 0) {
         int pc[3 * kMonos[kMaxDegree]];
         for (int c = 0; c < 3; ++c)
             for (int q = 0; q < nmono; ++q) pc[c * nmono + q] = kRing + c * ctx->cs_nmono[pi] + q;
@@ -1078,6 +1131,8 @@ int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool ad
         FM_HIP(ctx->ad_part.alloc(std::max<size_t>(ctx->ad_part.n, (size_t)coarse_moment_blocks() * ma)));
         FM_HIP(hipMemcpyAsync(ctx->ad_Einv.p, Einv.data(), Einv.size() * 8, hipMemcpyHostToDevice, s));
         ctx->ad_nmono = nmono;
+        ctx->einv_ok = fused && !reuse;
+        ctx->einv_gen = ctx->gpp_gen;
     }
     ctx->lag_nmono = ctx->ad_nmono;
     ctx->lag_prec = p;
@@ -1140,7 +1195,7 @@ int galerkin_push(fm_ctx* ctx, uint32_t n, int p, int deg) {
 // recycled corrections
 int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double tol, int max_it, int* iters,
                  bool inexact = false, int* capped = nullptr, int ring_use = 0, int deg = 0, bool push = false,
-                 bool additive = false, bool lag = false) {
+                 bool additive = false, bool lag = false, bool reuse_poly = false) {
     ctx->ad_nmono = 0;
     Matrix& A = ctx->tangent;
     ctx->cur = &A;
@@ -1179,7 +1234,7 @@ int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double to
         if (lag && additive && ring_use == 0 && ctx->lag_nmono > 0 && ctx->lag_prec == PREC64)
             FM_TRY(lagged_start(ctx, A, PREC64));
         else if ((ring_use > 0 && ctx->gal_n[0] > 0) || deg > 0)
-            FM_TRY(galerkin_start(ctx, A, PREC64, ring_use, deg, additive));
+            FM_TRY(galerkin_start(ctx, A, PREC64, ring_use, deg, additive, reuse_poly));
         else
             launch_fill_f64(ctx->stream, ctx->x64.p, n, 0.0);
         const int prc = pcg_run(ctx, A, PREC64, tol, max_it, &pr);
@@ -1198,7 +1253,7 @@ int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double to
         if (lag && additive && ring_use == 0 && ctx->lag_nmono > 0 && ctx->lag_prec == PREC32)
             FM_TRY(lagged_start(ctx, A, PREC32));
         else if ((ring_use > 0 && ctx->gal_n[1] > 0) || deg > 0)
-            FM_TRY(galerkin_start(ctx, A, PREC32, ring_use, deg, additive));
+            FM_TRY(galerkin_start(ctx, A, PREC32, ring_use, deg, additive, reuse_poly));
         else
             launch_fill_f32(ctx->stream, ctx->x32.p, n, 0.f);
         const int prc = pcg_run(ctx, A, PREC32, tol, max_it, &pr);
@@ -1835,6 +1890,7 @@ void fm_newton_cfg_default(fm_newton_cfg* c) {
     c->extrap_order = 0;
     c->slope_start = 0;
     c->factor_steps = 0;
+    c->keep_coarse = 0;
 }
 
 int fm_set_u(fm_ctx* ctx, const double* u) {
@@ -2161,7 +2217,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         // a lagged preconditioner: the new tangent with the first iteration's
         // factor (and coarse operator), which the assembly leaves in place
         const bool lag_on = it > 0 && !keep && cfg.lag_precond && (kprec == PREC32 || inexact);
-        bool lag = false;
+        bool lag = false, reuse_poly = false;
         if (!keep) {
             const int fp = kprec == PREC32 ? PREC32 : PREC64;
             lag = lag_on && ctx->tangent.lu_valid[fp];
@@ -2176,6 +2232,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
             if (lag || keep_factor) ctx->tangent.lu_valid[fp] = true;
             if (it == 0) ctx->fact_age = keep_factor ? ctx->fact_age + 1 : 1;
             lag |= keep_factor;
+            reuse_poly = keep_factor;
         }
         int its = 0;
         // R of the current u is in ctx->R (last residual evaluation)
@@ -2191,7 +2248,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         // joins them
         const int start_from = recycle && it == 0 ? std::min(2 * cfg.recycle, kRing) : 0;
         FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, eta, lin_cap, &its, inexact, &S.pcg_cap_hits,
-                            start_from, coarse, recycle, additive, lag));
+                            start_from, coarse, recycle, additive, lag, reuse_poly && cfg.keep_coarse));
         S.pcg_iters += its;
         if (!keep && !lag) S.factorizations++;
         trace_solve(lf, it, eta, its, lag);

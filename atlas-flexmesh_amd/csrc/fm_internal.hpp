@@ -193,8 +193,20 @@ struct Ctx {
     double slope_lf = 0.0;
     bool slope_ok = false;
     // load steps whose first Newton iteration used the current tangent factor
-    // (the factor_steps option)
+    // (the factor_steps option); fact_gen counts the tangent's factorizations
     int fact_age = 0;
+    uint64_t fact_gen = 0;
+    // the polynomial fields' block Z^T K Z of the last fused Galerkin start
+    // (host, row-major, 3 gpp_nmono square, precision gpp_prec) and the factor
+    // generation whose tangent it was formed with: a step that keeps that
+    // factor keeps this block too
+    std::vector<double> gpp;
+    int gpp_nmono = 0, gpp_prec = -1;
+    uint64_t gpp_gen = 0;
+    bool gpp_ok = false;
+    // ad_Einv is the inverse of that block (generation einv_gen)
+    uint64_t einv_gen = 0;
+    bool einv_ok = false;
     int sec_n = 0;
     bool have_material = false;
 
@@ -428,6 +440,8 @@ uint32_t gram_blocks(int m, int mw);
 // Z^T A Z, then Z^T S_j for the 13 stored columns S (leading dimension A.n);
 // partials: poly_gram_blocks() * 3 nmono * (3 nmono + 13)
 uint32_t poly_gram_blocks();
+// out[p] = sum over b < nb of partials[b np + p], in a fixed order
+void launch_partial_sums(hipStream_t s, uint32_t nb, uint32_t np, const double* partials, double* out);
 constexpr int kPolyGramStored = 13;
 void launch_poly_gram(hipStream_t s, int prec, const Matrix& A, const float4* xn, int nmono, const void* S,
                       double* partials, double* G);

@@ -3220,6 +3220,10 @@ static void poly_gram_t(hipStream_t s, const Matrix& A, const float4* xn, int nm
     }
 }
 
+void launch_partial_sums(hipStream_t s, uint32_t nb, uint32_t np, const double* partials, double* out) {
+    gram_reduce_kernel<<<cdiv(np, 32), 256, 0, s>>>(nb, np, partials, out);
+}
+
 void launch_poly_gram(hipStream_t s, int prec, const Matrix& A, const float4* xn, int nmono, const void* S,
                       double* partials, double* G) {
     if (prec == PREC64)

@@ -34,7 +34,7 @@ class NewtonCfg(C.Structure):
                 ("inner_tol_max", C.c_double), ("coarse_degree", C.c_int),
                 ("coarse_additive", C.c_int), ("lag_precond", C.c_int),
                 ("extrap_order", C.c_int), ("slope_start", C.c_int),
-                ("factor_steps", C.c_int)]
+                ("factor_steps", C.c_int), ("keep_coarse", C.c_int)]
 
 
 FIELD_NAMES = ("det_F", "cauchy", "von_mises", "pressure", "max_principal", "energy", "max_stretch", "F")

@@ -327,6 +327,8 @@ def main():
     ap.add_argument("--factor-steps", type=int, default=None,
                     help="with the lagged preconditioner: load steps one ILU(0) factor serves (default 6 with the "
                          "coarse level; 1: a new factor every step)")
+    ap.add_argument("--keep-coarse", type=int, default=None,
+                    help="1: the steps that keep an earlier factor also keep its coarse operator Z^T K Z (default 0)")
     ap.add_argument("--lag", type=int, default=None,
                     help="1: the later Newton iterations of a load step keep the first iteration's ILU(0) factor "
                          "and coarse operator (a lagged preconditioner; the tangent is re-assembled) (default 1 "
@@ -413,8 +415,10 @@ def bench(S, cfg, args, warm_last, K):
     order = args.order if args.order is not None else 3
     slope = args.slope if args.slope is not None else 0
     fsteps = args.factor_steps if args.factor_steps is not None else (6 if lag else 1)
+    kcoarse = args.keep_coarse if args.keep_coarse is not None else 0
     solver_kw = dict(recycle=recycle, inner_tol_max=inner_tol_max, coarse_degree=coarse, coarse_additive=additive,
-                     lag_precond=lag, extrap_order=order, slope_start=slope, factor_steps=fsteps)
+                     lag_precond=lag, extrap_order=order, slope_start=slope, factor_steps=fsteps,
+                     keep_coarse=kcoarse)
     if warm_last:
         log(f"warm-up pass: load steps 1..{warm_last}")
         run_steps(S, cfg, prec, 1, warm_last, inner_tol, False, args.pcg_max, lift, **solver_kw)
@@ -435,7 +439,7 @@ def bench(S, cfg, args, warm_last, K):
                                f"load steps 1..{K} of {cfg['n_steps']}", "dofs": dofs, "tets": info["ne"],
                    "nnz": info["nnz"], "predictor": "extrapolate" if lift == 2 else "solve",
                    "inner_tol": inner_tol, "inner_tol_max": inner_tol_max, "recycle": recycle,
-                   "coarse_degree": coarse, "coarse_additive": additive, "lag_precond": lag, "extrap_order": order, "slope_start": slope, "factor_steps": fsteps,
+                   "coarse_degree": coarse, "coarse_additive": additive, "lag_precond": lag, "extrap_order": order, "slope_start": slope, "factor_steps": fsteps, "keep_coarse": kcoarse,
                    "newton_rtol": cfg["newton_rtol"]},
         "newton": {"converged_steps": tot["converged"], "newton_iters": tot["newton_iters"],
                    "pcg_iters": tot["pcg_iters"], "newton_it_per_s": tot["newton_iters"] / wall,

@@ -210,6 +210,9 @@ typedef struct {
                              steps (a lagged preconditioner across steps; the Galerkin start and
                              the coarse operator use the current tangent); 0 or 1 = a new factor
                              every step */
+    int keep_coarse;      /* with factor_steps: such a step also keeps the coarse operator
+                             Z^T K Z of the step that formed the factor (its Galerkin start
+                             forms only the recycled corrections' rows); 0 = off */
 } fm_newton_cfg;
 
 typedef struct {

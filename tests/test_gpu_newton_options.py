@@ -244,15 +244,16 @@ def test_repeated_matrix_pattern_uploads_values(fm, S, O):
                           ("mixed", 4, 1, 0, 2, 2, 0), ("fp64", 1, 1, 0, 2, 2, 0), ("mixed", 3, 1, 1, 2, 2, 0),
                           ("fp64", 3, 1, 1, 2, 2, 0), ("mixed", 0, 0, 1, 2, 2, 0), ("mixed", 4, 1, 1, 3, 6, 0),
                           ("fp64", 4, 1, 1, 3, 6, 0), ("mixed", 2, 1, 1, 1, 4, 0), ("mixed", 4, 1, 1, 4, 6, 0),
-                          ("mixed", 4, 1, 1, 3, 6, 1), ("fp64", 4, 1, 1, 3, 6, 1), ("mixed", 4, 1, 3, 3, 6, 0),
-                          ("fp64", 3, 1, 4, 3, 6, 0)])
+                          ("mixed", 4, 1, 1, 3, 6, 1), ("fp64", 4, 1, 1, 3, 6, 1), ("mixed", 4, 1, 2, 3, 6, 0),
+                          ("mixed", 4, 1, 3, 3, 6, 0), ("fp64", 3, 1, 4, 3, 6, 0)])
 def test_galerkin_start_and_target_forcing(fm, S, O, prec, coarse, additive, lag, order, recycle, slope):
     """recycle = 2 with the target-aware forcing term (inner_tol 1e-3, at most
     0.1), and with the polynomial fields of degree <= coarse in the Galerkin
     start (and, with additive, in every PCG iteration as the coarse level of
     a two-level preconditioner; with lag, the later Newton iterations of a
     step keep the first one's ILU(0) factor and coarse operator, and with
-    lag > 1 one factor serves up to lag load steps; order: the
+    lag > 1 one factor serves up to lag load steps, with lag > 2 together
+    with its coarse operator; order: the
     degree of the load-step extrapolation, recycle: half the corrections in
     the first solve's start, slope: the predictor solve's slope as Hermite
     data of the second and third steps' extrapolation): every step
@@ -271,7 +272,8 @@ def test_galerkin_start_and_target_forcing(fm, S, O, prec, coarse, additive, lag
     plain = fm.configs.newton_cfg(cfg, lift=2, inner_tol=1e-3, inner_tol_max=0.1, recycle=0, **kw)
     rec = fm.configs.newton_cfg(cfg, lift=2, inner_tol=1e-3, inner_tol_max=0.1, recycle=recycle,
                                 coarse_degree=coarse, coarse_additive=additive, lag_precond=min(lag, 1),
-                                factor_steps=lag, extrap_order=order, slope_start=slope, **kw)
+                                factor_steps=lag, keep_coarse=int(lag >= 3), extrap_order=order,
+                                slope_start=slope, **kw)
     runs = {}
     for name, c in (("plain", plain), ("rec", rec), ("rec again", rec)):
         S.set_u(None)
