@@ -967,8 +967,21 @@ int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool ad
     }
     {
         ProfScope ps(ctx, FM_K_GALERKIN);
+        // the ring candidates' products: one pass over the matrix for the
+        // contiguous slot range that holds them where the multi-vector SpMV
+        // applies (slots in the range that are not candidates get products
+        // that are never read), one product per column otherwise
+        int rlo = kRing, rhi = -1;
         for (int q = 0; q < nc; ++q)
-            if (!fused || cand[q] < kRing)
+            if (cand[q] < kRing) {
+                rlo = std::min(rlo, cand[q]);
+                rhi = std::max(rhi, cand[q]);
+            }
+        const bool multi = fused && rhi >= rlo &&
+                           launch_spmv_multi(s, p, A, V + (size_t)rlo * n * es, W + (size_t)rlo * n * es, n,
+                                             rhi - rlo + 1);
+        for (int q = 0; q < nc; ++q)
+            if (!fused || (cand[q] < kRing && !multi))
                 launch_spmv(s, p, A, V + (size_t)cand[q] * n * es, W + (size_t)cand[q] * n * es, false, nullptr,
                             nullptr, nullptr, nullptr);
         const int bcol = fused ? kRing : c1;

@@ -332,6 +332,10 @@ uint32_t launch_dot64(hipStream_t s, uint32_t n, const double* a, const double* 
 void launch_reduce(hipStream_t s, const double* partials, uint32_t np, PcgState* st, int op, int k);
 void launch_spmv(hipStream_t s, int prec, const Matrix& A, const void* x, void* y, bool dot, double* partials,
                  void* sent0, void* sent1, const PcgState* st);
+// Y[:, j] = A X[:, j], j < nv (leading dimension ld), with the matrix read
+// once for all vectors; false (nothing launched) where the LDS-staged
+// node-blocked SpMV does not apply
+bool launch_spmv_multi(hipStream_t s, int prec, const Matrix& A, const void* X, void* Y, uint32_t ld, int nv);
 uint32_t spmv_blocks(const Matrix& A, int prec);  // partials written by launch_spmv with dot
 void launch_ilu_factor(hipStream_t s, int prec, const Matrix& A, const void* val, void* lu, PcgState* st,
                        int* err);

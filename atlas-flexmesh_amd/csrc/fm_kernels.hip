@@ -916,6 +916,99 @@ __global__ __launch_bounds__(256) void spmv_node_lds_kernel(uint32_t nn, const u
     }
 }
 
+// Y[:, j] = A X[:, j] for nv vectors of leading dimension ld: the LDS-staged
+// node SpMV with the wave's value run staged once and used for every vector
+// (the matrix is read once for all of them; each vector costs its x gathers
+// and its stores).  Per vector the arithmetic is spmv_node_lds_kernel's, in
+// the same order.
+template <typename T>
+__global__ __launch_bounds__(256) void spmv_node_multi_kernel(uint32_t nn, const uint32_t* __restrict__ rp,
+                                                              const uint32_t* __restrict__ nptr,
+                                                              const uint32_t* __restrict__ ncol,
+                                                              const T* __restrict__ val, const T* __restrict__ X,
+                                                              T* __restrict__ Y, uint32_t ld, int nv) {
+    using V = typename Vec16<T>::type;
+    constexpr int G = nodes_per_group<T>(), NPW = 4 * G, W = 16 / (int)sizeof(T);
+    constexpr int ROUNDS = FM_SPMV_LDS_ROUNDS, CAP = ROUNDS * 64 * W;
+    __shared__ V sv[4][ROUNDS * 64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, k = lane & 15;
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t v0 = (blk * 4 + wave) * NPW;
+    const uint32_t rbase = rp[0];
+    const uint32_t pl = nptr[min(v0 + min((uint32_t)lane, (uint32_t)NPW), nn)];
+    uint32_t node[G];
+#pragma unroll
+    for (int i = 0; i < G; ++i) node[i] = v0 + 4 * i + q;
+    const uint32_t r0 = rbase + 9 * (uint32_t)__builtin_amdgcn_readlane((int)pl, 0);
+    const uint32_t r1 = rbase + 9 * (uint32_t)__builtin_amdgcn_readlane((int)pl, NPW);
+    const uint32_t base = r0 & ~(uint32_t)(W - 1);
+    const bool staged = r1 - base <= (uint32_t)CAP;  // wave-uniform
+    const uint32_t last = r1 > base ? (r1 - 1) & ~(uint32_t)(W - 1) : base;
+    uint32_t b0[G], off[G], nb[G], cn[G];
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+        const uint32_t bb = __shfl(pl, 4 * i + q), e = __shfl(pl, 4 * i + q + 1);
+        const bool ok = node[i] < nn;
+        nb[i] = ok ? e - bb : 0;
+        b0[i] = ok ? bb : 0;
+        const uint32_t kk = min((uint32_t)k, max(nb[i], 1u) - 1);
+        off[i] = ok ? rbase + 9 * bb + 3 * kk : base;
+        cn[i] = ncol[b0[i] + kk];
+    }
+    if (staged) {
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) {
+            const uint32_t c = min(base + (uint32_t)(r * 64 + lane) * W, last);
+            sv[wave][r * 64 + lane] = __builtin_nontemporal_load(reinterpret_cast<const V*>(val + c));
+        }
+    }
+    __syncthreads();
+    const T* lv = reinterpret_cast<const T*>(&sv[wave][0]);
+    for (int j = 0; j < nv; ++j) {
+        const T* x = X + (size_t)j * ld;
+        T s[G][3];
+        auto products = [&](auto av) {
+#pragma unroll
+            for (int i = 0; i < G; ++i) {
+                const uint32_t o0 = off[i], o1 = o0 + 3 * nb[i], o2 = o1 + 3 * nb[i];
+                T a[9];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    a[c] = av(o0 + c);
+                    a[3 + c] = av(o1 + c);
+                    a[6 + c] = av(o2 + c);
+                }
+                const T w = (uint32_t)k < nb[i] ? (T)1 : (T)0;
+                const T* xm = x + 3 * (size_t)cn[i];
+                const T x0 = xm[0], x1 = xm[1], x2 = xm[2];
+                s[i][0] = (a[0] * x0 + a[1] * x1 + a[2] * x2) * w;
+                s[i][1] = (a[3] * x0 + a[4] * x1 + a[5] * x2) * w;
+                s[i][2] = (a[6] * x0 + a[7] * x1 + a[8] * x2) * w;
+            }
+#pragma unroll
+            for (int i = 0; i < G; ++i)
+                for (uint32_t kb = k + kNodeLanes; kb < nb[i]; kb += kNodeLanes) {
+                    const uint32_t o0 = rbase + 9 * b0[i] + 3 * kb, o1 = o0 + 3 * nb[i], o2 = o1 + 3 * nb[i];
+                    const T* xm = x + 3 * (size_t)ncol[b0[i] + kb];
+                    const T x0 = xm[0], x1 = xm[1], x2 = xm[2];
+                    s[i][0] += av(o0) * x0 + av(o0 + 1) * x1 + av(o0 + 2) * x2;
+                    s[i][1] += av(o1) * x0 + av(o1 + 1) * x1 + av(o1 + 2) * x2;
+                    s[i][2] += av(o2) * x0 + av(o2 + 1) * x1 + av(o2 + 2) * x2;
+                }
+        };
+        if (staged)
+            products([&](uint32_t o) { return lv[o - base]; });
+        else
+            products([&](uint32_t o) { return val[o]; });
+        T* y = Y + (size_t)j * ld;
+#pragma unroll
+        for (int i = 0; i < G; ++i) {
+            const T t0 = row16_sum(s[i][0]), t1 = row16_sum(s[i][1]), t2 = row16_sum(s[i][2]);
+            if (node[i] < nn && k < 3) y[3 * node[i] + k] = k == 0 ? t0 : (k == 1 ? t1 : t2);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // natural-order ILU(0), node-blocked (B rows per node) and sync-free:
 // nodes are taken from a level-ordered queue through a ticket counter, so a
@@ -3372,6 +3465,21 @@ static void spmv_t(hipStream_t s, const Matrix& A, const T* val, const T* x, T* 
             break;
     }
 #undef FM_SPMV
+}
+
+bool launch_spmv_multi(hipStream_t s, int prec, const Matrix& A, const void* X, void* Y, uint32_t ld, int nv) {
+    if (!spmv_use_node(A) || !spmv_use_lds() || nv <= 0) return false;
+    const uint32_t nn = A.n / 3;
+    if (prec == PREC64) {
+        const uint32_t nb = cdiv(nn, 256 / kNodeLanes * nodes_per_group<double>());
+        spmv_node_multi_kernel<double><<<nb, 256, 0, s>>>(nn, A.row_ptr.p, A.nptr.p, A.ncol.p, A.val64.p,
+                                                          (const double*)X, (double*)Y, ld, nv);
+    } else {
+        const uint32_t nb = cdiv(nn, 256 / kNodeLanes * nodes_per_group<float>());
+        spmv_node_multi_kernel<float><<<nb, 256, 0, s>>>(nn, A.row_ptr.p, A.nptr.p, A.ncol.p, A.val32.p,
+                                                         (const float*)X, (float*)Y, ld, nv);
+    }
+    return true;
 }
 
 void launch_spmv(hipStream_t s, int prec, const Matrix& A, const void* x, void* y, bool dot, double* partials,
