@@ -1951,9 +1951,15 @@ __global__ void poly_basis_kernel(uint32_t nn, const float4* __restrict__ xn, in
 constexpr int kPgStored = 13;
 constexpr int kPgBlocks = 512;
 constexpr int kPgThreads = 384;
+#ifndef FM_PG_TILE32
+#define FM_PG_TILE32 16
+#endif
+#ifndef FM_PG_UNROLL
+#define FM_PG_UNROLL 4
+#endif
 template <typename T>
 constexpr int pg_tile() {
-    return sizeof(T) == 4 ? 16 : 8;
+    return sizeof(T) == 4 ? FM_PG_TILE32 : 8;
 }
 template <typename T, int NM>
 __global__ __launch_bounds__(kPgThreads) void poly_gram_kernel(uint32_t nn, const uint32_t* __restrict__ rp,
@@ -2037,6 +2043,7 @@ __global__ __launch_bounds__(kPgThreads) void poly_gram_kernel(uint32_t nn, cons
                 const uint32_t b0 = nptr[node], nb = nptr[node + 1] - b0;
                 if (staged) {
                     const T* a = s_k + 9 * (b0 - bs);
+#pragma unroll FM_PG_UNROLL
                     for (uint32_t q = 0; q < nb; ++q) {
                         const T* pw = s_pw[b0 - bs + q];
                         const T m = (pw[ex] * pw[5 + ey]) * pw[10 + ez];
