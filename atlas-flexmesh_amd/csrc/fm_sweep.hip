@@ -95,7 +95,7 @@ struct Shape {
 constexpr int qp(int a, int b, int c, int d) { return a | (b << 2) | (c << 4) | (d << 6); }
 constexpr int kXor1 = qp(1, 0, 3, 2);
 constexpr int kXor2 = qp(2, 3, 0, 1);
-constexpr int kBc0 = qp(0, 0, 0, 0), kBc3 = qp(3, 3, 3, 3);
+constexpr int kBc3 = qp(3, 3, 3, 3);
 
 template <int CTRL>
 __device__ __forceinline__ float quad(float v) {
@@ -383,10 +383,11 @@ __device__ __forceinline__ Pub<T> process(const SweepArgs<T>& A, const Near<T>& 
         part[d] += quad<kXor1>(part[d]);
         part[d] += quad<kXor2>(part[d]);
     }
-    // right-hand side: quad lane 0's view of in0
+    // right-hand side: every lane of a quad holds its node's triple of in0
+    // (loaded from the quad's node offset, or staged per quad by the loader)
     T b[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) b[d] = quad<kBc0>(N.vec[d]);
+    for (int d = 0; d < 3; ++d) b[d] = N.vec[d];
     T v0, v1, v2;
     if (!BWD) {
         // unit lower diagonal block (l10, l20, l21 in quad lane 3)
