@@ -310,7 +310,8 @@ def main():
     ap.add_argument("--prec", choices=["mixed", "fp64", "mixed_pc"], default=None,
                     help="precision of the linear solves (default: the configuration's)")
     ap.add_argument("--inner-tol", type=float, default=None, help="FP32 PCG forcing term of the mixed path")
-    ap.add_argument("--out", default=os.path.join(ROOT, "gpurun_out"))
+    ap.add_argument("--out", default=os.path.join(ROOT, "validation_results"),
+                    help="directory of the JSON results (default: validation_results at the repository root)")
     ap.add_argument("--save-gpu", default=None, help="cfg2: run the GPU half only and save u here (.npz)")
     ap.add_argument("--gpu-result", default=None, help="cfg2: compare a saved GPU result with the oracle")
     args = ap.parse_args()
