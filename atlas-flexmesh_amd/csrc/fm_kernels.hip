@@ -2122,7 +2122,14 @@ __global__ __launch_bounds__(kPgThreads) void poly_gram_kernel(uint32_t nn, cons
 // additive coarse correction of the preconditioner: z = M^{-1} r + Z E^{-1}
 // Z^T r with Z the polynomial fields (3 NM columns) and E = Z^T A Z
 // ---------------------------------------------------------------------------
-constexpr int kMomentBlocks = 512;
+// blocks of the moment passes (and partials of the coarse solve): 256 cut
+// the coarse kernels from 46 to 42 us per iteration at 128^3 hexes, but the
+// changed rounding of the moments took 2,225 PCG iterations over the 20 load
+// steps against 2,209
+#ifndef FM_MOMENT_BLOCKS
+#define FM_MOMENT_BLOCKS 512
+#endif
+constexpr int kMomentBlocks = FM_MOMENT_BLOCKS;
 
 // DPP lane exchanges of a wave (GFX9 controls): the row-masked source value,
 // zero in the rows the mask leaves out
