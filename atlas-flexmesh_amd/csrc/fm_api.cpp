@@ -581,8 +581,19 @@ struct PcgResult {
 // the device does not idle at a host check.  Iterations enqueued past
 // convergence return at once (every kernel of the loop tests st->done).  On
 // return the final state is in h_st[0].
+// FM_TRACE_SOLVES=1 (read once per process): the Newton loop's linear
+// solves and the PCG's relative residual at every batch on stderr
+static bool trace_solves() {
+    static const bool on = [] {
+        const char* e = std::getenv("FM_TRACE_SOLVES");
+        return e && *e == '1';
+    }();
+    return on;
+}
+
 template <typename F>
 int pcg_batches(fm_ctx* ctx, int max_it, F&& enqueue_one) {
+    const bool trace = trace_solves();
     hipStream_t s = ctx->stream;
     int enq = 0;
     for (int b = 0;; ++b) {
@@ -592,7 +603,9 @@ int pcg_batches(fm_ctx* ctx, int max_it, F&& enqueue_one) {
         FM_HIP(hipEventRecord(ctx->batch_ev[b & 1], s));
         if (b > 0) {
             FM_HIP(hipEventSynchronize(ctx->batch_ev[(b - 1) & 1]));
-            if (ctx->h_st[1 + ((b - 1) & 1)].done) break;
+            const PcgState& h = ctx->h_st[1 + ((b - 1) & 1)];
+            if (trace) std::fprintf(stderr, "[fm]   pcg %d relres %.3e\n", h.iters, h.relres);
+            if (h.done) break;
         }
         if (enq >= max_it) break;
     }
@@ -616,6 +629,11 @@ int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult*
     PcgState init{};
     init.tol = tol;
     init.max_it = max_it;
+    // alpha and beta of every iteration, for fm_pcg_history
+    FM_HIP(ctx->pcg_hist.alloc(2 * (size_t)kPcgHistCap));
+    init.hist = ctx->pcg_hist.p;
+    init.hist_cap = kPcgHistCap;
+    ctx->pcg_hist_len = 0;
     *ctx->h_st = init;
     FM_HIP(hipMemcpyAsync(ctx->st.p, ctx->h_st, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
     // r = b - A x; ||r||^2 and b.b
@@ -647,11 +665,24 @@ int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult*
     FM_HIP(hipMemcpyAsync(ctx->st.p, ctx->h_st, sizeof(PcgState), hipMemcpyHostToDevice, s));
     // z = M^{-1} r (+ Z E^{-1} Z^T r); rz; p = z
     const int ad = ctx->ad_nmono;
+    // the Lanczos vectors of this solve for its Ritz vectors (with the coarse level)
+    void* lz = nullptr;
+    uint32_t lz_cap = 0;
+    if (ctx->harvest > 0 && ad) {
+        if (prec == PREC64) {
+            FM_HIP(ctx->lz64.alloc((size_t)n * kLanczosCap));
+            lz = ctx->lz64.p;
+        } else {
+            FM_HIP(ctx->lz32.alloc((size_t)n * kLanczosCap));
+            lz = ctx->lz32.p;
+        }
+        lz_cap = kLanczosCap;
+    }
     if (ad) launch_coarse_correct(s, prec, n, ad, ctx->cs_xn.p, r, ctx->ad_Einv.p, ctx->ad_part.p, ctx->ad_coef.p, ctx->st.p);
     np = (uint32_t)precond_apply(ctx, A, prec, r, z, y, ctx->partials.p);
     launch_reduce(s, ctx->partials.p, np, ctx->st.p, 3, 0);
     if (ad)
-        launch_pcg_p_update_coarse(s, prec, n, ad, y, p, ctx->cs_xn.p, ctx->ad_coef.p, true, ctx->st.p);
+        launch_pcg_p_update_coarse(s, prec, n, ad, y, p, ctx->cs_xn.p, ctx->ad_coef.p, true, ctx->st.p, lz, lz_cap);
     else
         launch_copy(s, prec, n, y, p);
     FM_TRY(check_launch(ctx, "pcg init"));
@@ -697,7 +728,8 @@ int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult*
             }
             launch_reduce(s, ctx->partials.p, nbb, ctx->st.p, 2, 0);
             if (ad)
-                launch_pcg_p_update_coarse(s, prec, n, ad, y, p, ctx->cs_xn.p, ctx->ad_coef.p, false, ctx->st.p);
+                launch_pcg_p_update_coarse(s, prec, n, ad, y, p, ctx->cs_xn.p, ctx->ad_coef.p, false, ctx->st.p, lz,
+                                           lz_cap);
             else
                 launch_pcg_p_update(s, prec, n, y, p, ctx->st.p);
     }));
@@ -706,6 +738,7 @@ int pcg_run(fm_ctx* ctx, Matrix& A, int prec, double tol, int max_it, PcgResult*
     res->status = h.status;
     res->relres = h.relres;
     res->iters = (h.status == 1) ? h.iters : max_it;
+    ctx->pcg_hist_len = (uint32_t)std::min<int>(std::max(h.iters, 0), kPcgHistCap);
     return FM_OK;
 }
 
@@ -827,10 +860,13 @@ constexpr double kForcingShare = 0.5;
 // Galerkin start of a PCG solve A x = b (b, x: the PCG vectors of precision
 // p): x0 = V c with (V^T A V) c = V^T b over the chosen basis columns, i.e.
 // the A-norm-best start in their span.  Columns are taken in order (the ring
-// of recycled corrections, newest first, then the polynomial fields); one
-// whose pivot in the Cholesky factor of V^T A V is not above 1e-10 of its
-// diagonal adds nothing new and is left out.
-constexpr int kRing = 12;         // recycled corrections kept (basis columns 0..11)
+// of recycled corrections, newest first, the Ritz vectors of the last solve,
+// then the polynomial fields); one whose pivot in the Cholesky factor of
+// V^T A V is not above 1e-10 of its diagonal adds nothing new and is left out.
+constexpr int kCorr = 12;              // recycled corrections kept (basis columns 0..11)
+constexpr int kRitz = 8;               // Ritz vectors of the last solve (columns 12..19)
+constexpr int kRing = kCorr + kRitz;   // stored basis columns before the polynomial fields
+static_assert(kRitz <= 8, "the combination kernel forms at most eight Ritz vectors");
 constexpr int kMaxDegree = 4;
 constexpr int kMonos[kMaxDegree + 1] = {0, 4, 10, 20, 35};  // monomials of total degree <= 0 (none), 1, 2, 3, 4
 static_assert(kRing + 3 * kMonos[kMaxDegree] + 1 <= 128, "the Gram kernel takes at most 128 columns");
@@ -868,6 +904,7 @@ int coarse_space(fm_ctx* ctx, int p, uint32_t n, int deg) {
         ctx->cs_cols[pi] = c;
         ctx->cs_poly_ok[pi] = false;
         ctx->gal_n[pi] = 0;  // the ring's contents are gone
+        ctx->nritz[pi] = 0;
     }
     if (nmono && (!ctx->cs_poly_ok[pi] || ctx->cs_nmono[pi] < nmono)) {
         FM_TRY(coarse_coords(ctx));
@@ -907,7 +944,8 @@ void pivoted_cholesky(const G& g, const int* cols, int nc, std::vector<double>& 
 
 // additive: also E^{-1} over the polynomial fields for the additive coarse
 // correction of the PCG that follows (ctx->ad_nmono)
-int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool additive, bool reuse_poly = false) {
+int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool additive, bool reuse_poly = false,
+                   int ritz_use = 0) {
     hipStream_t s = ctx->stream;
     const uint32_t n = A.n;
     const int pi = p == PREC64 ? 0 : 1;
@@ -915,8 +953,10 @@ int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool ad
     const int nmono = std::min(kMonos[std::max(0, std::min(deg, kMaxDegree))], ctx->cs_nmono[pi]);
     // the candidate columns, in order
     int cand[kRing + 3 * kMonos[kMaxDegree]], nc = 0;
-    ring_use = std::min({ring_use, ctx->gal_n[pi], kRing});
-    for (int j = 0; j < ring_use; ++j) cand[nc++] = (ctx->gal_next[pi] - 1 - j + 2 * kRing) % kRing;
+    ring_use = std::min({ring_use, ctx->gal_n[pi], kCorr});
+    ritz_use = std::min(ritz_use, ctx->nritz[pi]);
+    for (int j = 0; j < ring_use; ++j) cand[nc++] = (ctx->gal_next[pi] - 1 - j + 2 * kCorr) % kCorr;
+    for (int j = 0; j < ritz_use; ++j) cand[nc++] = kCorr + j;
     for (int c = 0; c < 3; ++c)
         for (int k = 0; k < nmono; ++k) cand[nc++] = kRing + c * ctx->cs_nmono[pi] + k;
     void* b = vb(ctx, p, VB);
@@ -933,9 +973,9 @@ int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool ad
     }
     // On a node-blocked matrix the polynomial fields' rows of the Gram matrix
     // come from one fused pass over the matrix (Z^T A Z, Z^T A V_ring and
-    // Z^T b, without forming A Z), and the ring's rows from the products of
-    // the recycled corrections, one by one: G[ring] over the twelve ring
-    // slots with W column kRing = b.  Otherwise every candidate column's
+    // Z^T b, without forming A Z), and the rows of the stored columns (the
+    // recycled corrections and the Ritz vectors) from their products: G[ring]
+    // over the kRing stored slots with W column kRing = b.  Otherwise every candidate column's
     // product is formed, and one Gram matrix covers the contiguous column
     // range [c0, c1) with W column c1 = b (FM_GALERKIN=generic selects this
     // path for every matrix; read once per process).
@@ -1183,6 +1223,95 @@ int lagged_start(fm_ctx* ctx, Matrix& A, int p) {
     return FM_OK;
 }
 
+// eigenpairs of the symmetric m x m matrix T (row-major, overwritten) by
+// cyclic Jacobi rotations (Golub & Van Loan, Alg. 8.5.3): w the eigenvalues,
+// Q (row-major) the eigenvectors in its columns
+void jacobi_eigen(int m, std::vector<double>& T, std::vector<double>& w, std::vector<double>& Q) {
+    Q.assign((size_t)m * m, 0.0);
+    for (int i = 0; i < m; ++i) Q[(size_t)i * m + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, nrm = 0.0;
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) {
+                const double v = T[(size_t)i * m + j] * T[(size_t)i * m + j];
+                nrm += v;
+                if (i != j) off += v;
+            }
+        if (off <= 1e-30 * nrm) break;
+        for (int p = 0; p < m; ++p)
+            for (int q = p + 1; q < m; ++q) {
+                const double apq = T[(size_t)p * m + q];
+                if (apq == 0.0) continue;
+                const double app = T[(size_t)p * m + p], aqq = T[(size_t)q * m + q];
+                const double theta = (aqq - app) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < m; ++k) {  // columns p, q
+                    const double akp = T[(size_t)k * m + p], akq = T[(size_t)k * m + q];
+                    T[(size_t)k * m + p] = c * akp - sn * akq;
+                    T[(size_t)k * m + q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < m; ++k) {  // rows p, q
+                    const double apk = T[(size_t)p * m + k], aqk = T[(size_t)q * m + k];
+                    T[(size_t)p * m + k] = c * apk - sn * aqk;
+                    T[(size_t)q * m + k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < m; ++k) {
+                    const double qkp = Q[(size_t)k * m + p], qkq = Q[(size_t)k * m + q];
+                    Q[(size_t)k * m + p] = c * qkp - sn * qkq;
+                    Q[(size_t)k * m + q] = sn * qkp + c * qkq;
+                }
+            }
+    }
+    w.resize(m);
+    for (int i = 0; i < m; ++i) w[i] = T[(size_t)i * m + i];
+}
+
+// The Ritz vectors of the last PCG solve (run with ctx->harvest set): with
+// z_j the preconditioned residuals, the vectors z_j / sqrt(r_j.z_j) kept by
+// the solve and the Lanczos matrix of its alpha, beta (T_jj = 1/alpha_j +
+// beta_(j-1)/alpha_(j-1), T_j,j+1 = -sqrt(beta_j)/alpha_j) give P A V = V T,
+// P the preconditioner; the k eigenvectors of T with the smallest eigenvalues
+// combine V into approximations of the slowest modes of the preconditioned
+// operator, stored in the basis columns after the recycled corrections.  A
+// solve with fewer than 2k iterations keeps the previous ones.
+int ritz_harvest(fm_ctx* ctx, uint32_t n, int p, int k) {
+    const int pi = p == PREC64 ? 0 : 1;
+    const int m = std::min<int>((int)ctx->pcg_hist_len, kLanczosCap);
+    k = std::min(k, kRitz);
+    if (k <= 0 || m < 2 * k) return FM_OK;
+    std::vector<double> h(2 * (size_t)m);
+    FM_HIP(hipMemcpyAsync(h.data(), ctx->pcg_hist.p, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+    FM_TRY(sync(ctx));
+    std::vector<double> T((size_t)m * m, 0.0), w, Q;
+    for (int j = 0; j < m; ++j) {
+        const double a = h[2 * j];
+        T[(size_t)j * m + j] = 1.0 / a + (j > 0 ? h[2 * (j - 1) + 1] / h[2 * (j - 1)] : 0.0);
+        if (j + 1 < m) {
+            const double e = -std::sqrt(std::max(h[2 * j + 1], 0.0)) / a;
+            T[(size_t)j * m + j + 1] = e;
+            T[(size_t)(j + 1) * m + j] = e;
+        }
+    }
+    jacobi_eigen(m, T, w, Q);
+    std::vector<int> idx(m);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](int a, int b) { return w[a] < w[b]; });
+    std::vector<double> S((size_t)m * k);
+    for (int j = 0; j < m; ++j)
+        for (int q = 0; q < k; ++q) S[(size_t)j * k + q] = Q[(size_t)j * m + idx[q]];
+    FM_HIP(ctx->ritz_S.alloc(std::max<size_t>(ctx->ritz_S.n, (size_t)kLanczosCap * kRitz)));
+    FM_HIP(hipMemcpyAsync(ctx->ritz_S.p, S.data(), S.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    const size_t es = p == PREC64 ? 8 : 4;
+    char* V = pi == 0 ? (char*)ctx->cs_V64.p : (char*)ctx->cs_V32.p;
+    const void* Z = pi == 0 ? (const void*)ctx->lz64.p : (const void*)ctx->lz32.p;
+    launch_basis_combine_multi(ctx->stream, p, n, m, k, ctx->ritz_S.p, Z, V + (size_t)kCorr * n * es);
+    FM_TRY(check_launch(ctx, "ritz vectors"));
+    FM_TRY(sync(ctx));  // S is copied before it goes out of scope
+    ctx->nritz[pi] = k;
+    return FM_OK;
+}
+
 // the solution of the last solve (the PCG vector x of precision p), normalised,
 // becomes the newest ring column
 int galerkin_push(fm_ctx* ctx, uint32_t n, int p, int deg) {
@@ -1197,8 +1326,8 @@ int galerkin_push(fm_ctx* ctx, uint32_t n, int p, int deg) {
     const uint32_t np = launch_dot(s, p, n, x, x, ctx->partials.p);
     launch_reduce(s, ctx->partials.p, np, ctx->st.p, 5, 47);
     launch_normalize(s, p, n, &ctx->st.p->gal[47], x, V + (size_t)slot * n * es);
-    ctx->gal_next[pi] = (slot + 1) % kRing;
-    ctx->gal_n[pi] = std::min(ctx->gal_n[pi] + 1, kRing);
+    ctx->gal_next[pi] = (slot + 1) % kCorr;
+    ctx->gal_n[pi] = std::min(ctx->gal_n[pi] + 1, kCorr);
     return FM_OK;
 }
 
@@ -1208,7 +1337,8 @@ int galerkin_push(fm_ctx* ctx, uint32_t n, int p, int deg) {
 // recycled corrections
 int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double tol, int max_it, int* iters,
                  bool inexact = false, int* capped = nullptr, int ring_use = 0, int deg = 0, bool push = false,
-                 bool additive = false, bool lag = false, bool reuse_poly = false) {
+                 bool additive = false, bool lag = false, bool reuse_poly = false, int ritz = 0,
+                 int ritz_use = 0) {
     ctx->ad_nmono = 0;
     Matrix& A = ctx->tangent;
     ctx->cur = &A;
@@ -1247,12 +1377,16 @@ int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double to
         if (lag && additive && ring_use == 0 && ctx->lag_nmono > 0 && ctx->lag_prec == PREC64)
             FM_TRY(lagged_start(ctx, A, PREC64));
         else if ((ring_use > 0 && ctx->gal_n[0] > 0) || deg > 0)
-            FM_TRY(galerkin_start(ctx, A, PREC64, ring_use, deg, additive, reuse_poly));
+            FM_TRY(galerkin_start(ctx, A, PREC64, ring_use, deg, additive, reuse_poly, ritz_use));
         else
             launch_fill_f64(ctx->stream, ctx->x64.p, n, 0.0);
+        ctx->harvest = additive ? ritz : 0;
         const int prc = pcg_run(ctx, A, PREC64, tol, max_it, &pr);
+        const bool harvested = ctx->harvest > 0 && ctx->ad_nmono > 0;
+        ctx->harvest = 0;
         ctx->ad_nmono = 0;
         FM_TRY(prc);
+        if (harvested) FM_TRY(ritz_harvest(ctx, n, PREC64, ritz));
         if (push) FM_TRY(galerkin_push(ctx, n, PREC64, deg));
         launch_scale(ctx->stream, n, ctx->x64.p, nr, du);
     } else if (prec == PREC64) {
@@ -1266,12 +1400,16 @@ int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double to
         if (lag && additive && ring_use == 0 && ctx->lag_nmono > 0 && ctx->lag_prec == PREC32)
             FM_TRY(lagged_start(ctx, A, PREC32));
         else if ((ring_use > 0 && ctx->gal_n[1] > 0) || deg > 0)
-            FM_TRY(galerkin_start(ctx, A, PREC32, ring_use, deg, additive, reuse_poly));
+            FM_TRY(galerkin_start(ctx, A, PREC32, ring_use, deg, additive, reuse_poly, ritz_use));
         else
             launch_fill_f32(ctx->stream, ctx->x32.p, n, 0.f);
+        ctx->harvest = additive ? ritz : 0;
         const int prc = pcg_run(ctx, A, PREC32, tol, max_it, &pr);
+        const bool harvested = ctx->harvest > 0 && ctx->ad_nmono > 0;
+        ctx->harvest = 0;
         ctx->ad_nmono = 0;
         FM_TRY(prc);
+        if (harvested) FM_TRY(ritz_harvest(ctx, n, PREC32, ritz));
         if (push) FM_TRY(galerkin_push(ctx, n, PREC32, deg));
         launch_f32_to_f64(ctx->stream, n, ctx->x32.p, nr, du);
     }
@@ -1419,6 +1557,7 @@ int fm_set_mesh(fm_ctx* ctx, uint32_t n_nodes, const double* X, uint32_t n_elems
     ctx->sec_n = 0;
     ctx->slope_ok = false;
     ctx->gal_n[0] = ctx->gal_n[1] = 0;
+    ctx->nritz[0] = ctx->nritz[1] = 0;
     ctx->cs_poly_ok[0] = ctx->cs_poly_ok[1] = false;  // the polynomial fields depend on X and the constraints
     if (3ull * n_nodes > 0xFFFFFFFFull) return fail(ctx, FM_EINVAL, "too many nodes");
     for (size_t k = 0; k < 4ull * n_elems; ++k)
@@ -1453,6 +1592,7 @@ int fm_set_coords(fm_ctx* ctx, uint32_t n_nodes, const double* X) {
     ctx->sec_n = 0;
     ctx->slope_ok = false;
     ctx->gal_n[0] = ctx->gal_n[1] = 0;
+    ctx->nritz[0] = ctx->nritz[1] = 0;
     ctx->cs_poly_ok[0] = ctx->cs_poly_ok[1] = false;  // the polynomial fields depend on X and the constraints
     Matrix& M = ctx->tangent;
     M.val_valid[0] = M.val_valid[1] = false;
@@ -1466,6 +1606,7 @@ int fm_set_material(fm_ctx* ctx, const double* mu_e, const double* lam_e) {
     ctx->sec_n = 0;
     ctx->slope_ok = false;
     ctx->gal_n[0] = ctx->gal_n[1] = 0;
+    ctx->nritz[0] = ctx->nritz[1] = 0;
     if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
     ctx->hmu.assign(mu_e, mu_e + ctx->ne);
     ctx->hlam.assign(lam_e, lam_e + ctx->ne);
@@ -1478,6 +1619,7 @@ int fm_set_bcs(fm_ctx* ctx, uint32_t n_dir, const uint32_t* dofs, const double* 
     ctx->sec_n = 0;
     ctx->slope_ok = false;
     ctx->gal_n[0] = ctx->gal_n[1] = 0;
+    ctx->nritz[0] = ctx->nritz[1] = 0;
     ctx->cs_poly_ok[0] = ctx->cs_poly_ok[1] = false;  // the polynomial fields depend on X and the constraints
     if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
     const uint32_t n = 3 * ctx->nn;
@@ -1505,6 +1647,7 @@ int fm_set_load(fm_ctx* ctx, const double* f_ext) {
     ctx->sec_n = 0;
     ctx->slope_ok = false;
     ctx->gal_n[0] = ctx->gal_n[1] = 0;
+    ctx->nritz[0] = ctx->nritz[1] = 0;
     if (!ctx->nn) return fail(ctx, FM_ESTATE, "no mesh");
     if (!f_ext) {
         ctx->have_fext = false;
@@ -1904,6 +2047,7 @@ void fm_newton_cfg_default(fm_newton_cfg* c) {
     c->slope_start = 0;
     c->factor_steps = 0;
     c->keep_coarse = 0;
+    c->ritz = 0;
 }
 
 int fm_set_u(fm_ctx* ctx, const double* u) {
@@ -1918,12 +2062,14 @@ int fm_set_u(fm_ctx* ctx, const double* u) {
         ctx->sec_n = 1;
         ctx->slope_ok = false;
         ctx->gal_n[0] = ctx->gal_n[1] = 0;
+        ctx->nritz[0] = ctx->nritz[1] = 0;
         ctx->sec_lf[0] = 0.0;
         return FM_OK;
     }
     ctx->sec_n = 0;
     ctx->slope_ok = false;
     ctx->gal_n[0] = ctx->gal_n[1] = 0;
+    ctx->nritz[0] = ctx->nritz[1] = 0;
     FM_HIP(hipMemcpy(ctx->u.p, u, 3ull * ctx->nn * 8, hipMemcpyHostToDevice));
     ctx->u_valid = true;
     return FM_OK;
@@ -1939,6 +2085,22 @@ int fm_newton_history(fm_ctx* ctx, uint32_t cap, double* residual, double* step_
         if (step_size && i + 1 < n) step_size[i] = ctx->hist_alpha[i];
     }
     if (len) *len = (uint32_t)n;
+    return FM_OK;
+}
+
+int fm_pcg_history(fm_ctx* ctx, uint32_t cap, double* alpha, double* beta, uint32_t* len) {
+    if (!ctx) return FM_EINVAL;
+    const uint32_t n = std::min(cap, ctx->pcg_hist_len);
+    if (n && (!alpha || !beta)) return FM_EINVAL;
+    if (n) {
+        std::vector<double> h(2 * (size_t)n);
+        FM_HIP(hipMemcpy(h.data(), ctx->pcg_hist.p, h.size() * 8, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) {
+            alpha[i] = h[2 * i];
+            beta[i] = h[2 * i + 1];
+        }
+    }
+    if (len) *len = ctx->pcg_hist_len;
     return FM_OK;
 }
 
@@ -2007,11 +2169,7 @@ constexpr double kSlopeTol = 1e-5;
 // (load factor, Newton iteration or -1 for the predictor solve, forcing
 // term, PCG iterations, lagged preconditioner)
 static void trace_solve(double lf, int it, double eta, int its, bool lag) {
-    static const bool on = [] {
-        const char* e = std::getenv("FM_TRACE_SOLVES");
-        return e && *e == '1';
-    }();
-    if (on) std::fprintf(stderr, "[fm] lf %.4f newton %d eta %.3g pcg %d%s\n", lf, it, eta, its, lag ? " lagged" : "");
+    if (trace_solves()) std::fprintf(stderr, "[fm] lf %.4f newton %d eta %.3g pcg %d%s\n", lf, it, eta, its, lag ? " lagged" : "");
 }
 
 int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newton_cfg* cfg_in,
@@ -2038,6 +2196,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         ctx->sec_n = 0;  // a state from outside: no known history
         ctx->slope_ok = false;
         ctx->gal_n[0] = ctx->gal_n[1] = 0;
+        ctx->nritz[0] = ctx->nritz[1] = 0;
     }
     if (!ctx->u_valid) {
         // u = 0 is the equilibrium at load factor 0 (no load, no prescribed values)
@@ -2046,6 +2205,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         ctx->sec_n = 1;
         ctx->slope_ok = false;
         ctx->gal_n[0] = ctx->gal_n[1] = 0;
+        ctx->nritz[0] = ctx->nritz[1] = 0;
         ctx->sec_lf[0] = 0.0;
     }
     // linear-solve mode and the precision the tangent is assembled in
@@ -2060,6 +2220,8 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     const bool recycle = (kprec == PREC32 || inexact) && cfg.lift == 2 && cfg.recycle > 0;
     const int coarse = (kprec == PREC32 || inexact) ? std::max(0, std::min(cfg.coarse_degree, kMaxDegree)) : 0;
     const bool additive = coarse > 0 && cfg.coarse_additive != 0;
+    // Ritz vectors of each solve for the next step's first solve (with the coarse level)
+    const int ritz = additive && (kprec == PREC32 || inexact) ? std::max(0, std::min(cfg.ritz, kRitz)) : 0;
     const int lin_cap = kprec == PREC64 && !inexact ? cfg.pcg_max : cfg.inner_max;
     const double lf = load_factor;
     fm_newton_stats S{};
@@ -2182,7 +2344,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         const bool want_slope = cfg.lift == 2 && cfg.slope_start && ctx->sec_n == 1 && ctx->sec_lf[0] == 0.0 && lf > 0.0;
         const double ptol = want_slope ? std::min(lin_tol, kSlopeTol) : lin_tol;
         FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, ptol, lin_cap, &its, inexact, &S.pcg_cap_hits, 0,
-                            coarse, false, additive));
+                            coarse, false, additive, false, false, ritz, 0));
         S.pcg_iters += its;
         S.factorizations++;
         trace_solve(lf, -1, ptol, its, false);
@@ -2259,9 +2421,10 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         // the latest 2 recycle Newton corrections (both corrections of each of
         // the previous `recycle` steps in a steady run), and every correction
         // joins them
-        const int start_from = recycle && it == 0 ? std::min(2 * cfg.recycle, kRing) : 0;
+        const int start_from = recycle && it == 0 ? std::min(2 * cfg.recycle, kCorr) : 0;
         FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, eta, lin_cap, &its, inexact, &S.pcg_cap_hits,
-                            start_from, coarse, recycle, additive, lag, reuse_poly && cfg.keep_coarse));
+                            start_from, coarse, recycle, additive, lag, reuse_poly && cfg.keep_coarse, ritz,
+                            it == 0 ? ritz : 0));
         S.pcg_iters += its;
         if (!keep && !lag) S.factorizations++;
         trace_solve(lf, it, eta, its, lag);
@@ -2324,6 +2487,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         ctx->sec_n = 0;
         ctx->slope_ok = false;
         ctx->gal_n[0] = ctx->gal_n[1] = 0;
+        ctx->nritz[0] = ctx->nritz[1] = 0;
     }
     S.rel_res = S.r0 > 0 ? rn / S.r0 : 0.0;
     S.seconds = now_s() - t0;

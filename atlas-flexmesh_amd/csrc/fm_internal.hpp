@@ -41,7 +41,9 @@ struct DevBuf {
 
 // Scalars of one PCG solve, kept on the device so that no host round trip is
 // needed between iterations.  Kernels read `done` and return at once when set.
-constexpr int kReduceBlocks = 64;  // most blocks of one reduce (each sums a contiguous chunk of partials)
+constexpr int kReduceBlocks = 64;
+constexpr int kPcgHistCap = 4096;  // iterations whose alpha and beta a PCG solve records
+constexpr int kLanczosCap = 48;    // Lanczos vectors a harvesting PCG solve keeps  // most blocks of one reduce (each sums a contiguous chunk of partials)
 
 struct PcgState {
     double rz, pAp, alpha, beta, bb, rz_new, relres, tol;
@@ -55,6 +57,8 @@ struct PcgState {
     uint32_t ticket[4];  // work-queue counters of the triangular-solve kernels
     uint32_t red_count;  // blocks of a multi-block reduce that have stored their sum (0 between reduces)
     double red_stage[kReduceBlocks];
+    double* hist;        // alpha and beta of every iteration (hist_cap iterations at most), or null
+    uint32_t hist_cap;
 };
 
 // Level schedule of the natural-order ILU(0) dependence graph on groups of B
@@ -183,6 +187,14 @@ struct Ctx {
     // lagged preconditioner reuses it within the load step
     int lag_nmono = 0, lag_prec = -1;
     int gal_n[2] = {0, 0}, gal_next[2] = {0, 0};
+    // Ritz vectors of the last harvested solve in the basis columns after the
+    // corrections (nritz valid per precision), and the Lanczos vectors
+    // z_j / sqrt(r_j.z_j) of the running solve (n x lz_cap, PCG precision)
+    int nritz[2] = {0, 0};
+    DevBuf<float> lz32;
+    DevBuf<double> lz64;
+    int harvest = 0;  // Ritz vectors the next pcg_run's Lanczos vectors are kept for (0: none)
+    DevBuf<double> ritz_S;  // the Ritz combination coefficients (kLanczosCap x kRitz)
     static constexpr int kExtrapMax = 4;  // highest degree of the extrapolation
     DevBuf<double> sec_old[kExtrapMax], sec_start;
     double sec_lf[kExtrapMax + 1] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -192,6 +204,9 @@ struct Ctx {
     DevBuf<double> slope;
     double slope_lf = 0.0;
     bool slope_ok = false;
+    // alpha, beta of the last pcg_run (fm_pcg_history)
+    DevBuf<double> pcg_hist;
+    uint32_t pcg_hist_len = 0;
     // load steps whose first Newton iteration used the current tangent factor
     // (the factor_steps option); fact_gen counts the tangent's factorizations
     int fact_age = 0;
@@ -432,21 +447,27 @@ void launch_coarse_solve(hipStream_t s, int nmono, const double* partials, const
 // block partials (coarse_moment_blocks() x 3 nmono) of the moments Z^T r alone
 void launch_coarse_moments(hipStream_t s, int prec, uint32_t n, int nmono, const float4* xn, const void* r,
                            double* partials, const PcgState* st);
-// p = y + Z coef + beta p (first: p = y + Z coef)
+// p = y + Z coef + beta p (first: p = y + Z coef); lz (optional, lz_cap
+// columns of n): the iteration's Lanczos vector z / sqrt(r.z), z = y + Z coef
 void launch_pcg_p_update_coarse(hipStream_t s, int prec, uint32_t n, int nmono, const void* y, void* p,
-                                const float4* xn, const double* coef, bool first, const PcgState* st);
+                                const float4* xn, const double* coef, bool first, const PcgState* st,
+                                void* lz = nullptr, uint32_t lz_cap = 0);
+// Y[:, q] = sum_j S[j k + q] Z[:, j], q < k <= 8, j < m (S row-major, FP64)
+void launch_basis_combine_multi(hipStream_t s, int prec, uint32_t n, int m, int k, const double* S, const void* Z,
+                                void* Y);
 // G (m x mw, row-major, FP64) = V^T W for m and mw (both <= 128) columns of
 // leading dimension n, in a fixed order; partials: gram_blocks(m, mw) * m * mw
 // doubles
 uint32_t gram_blocks(int m, int mw);
-// G (3 nmono x (3 nmono + 13), row-major, FP64): the Gram rows of the
-// polynomial fields Z over the node-blocked matrix A without forming A Z:
-// Z^T A Z, then Z^T S_j for the 13 stored columns S (leading dimension A.n);
-// partials: poly_gram_blocks() * 3 nmono * (3 nmono + 13)
+// G (3 nmono x (3 nmono + kPolyGramStored), row-major, FP64): the Gram rows
+// of the polynomial fields Z over the node-blocked matrix A without forming
+// A Z: Z^T A Z, then Z^T S_j for the kPolyGramStored stored columns S
+// (leading dimension A.n; the basis products and the right-hand side);
+// partials: poly_gram_blocks() * 3 nmono * (3 nmono + kPolyGramStored)
 uint32_t poly_gram_blocks();
 // out[p] = sum over b < nb of partials[b np + p], in a fixed order
 void launch_partial_sums(hipStream_t s, uint32_t nb, uint32_t np, const double* partials, double* out);
-constexpr int kPolyGramStored = 13;
+constexpr int kPolyGramStored = 21;
 void launch_poly_gram(hipStream_t s, int prec, const Matrix& A, const float4* xn, int nmono, const void* S,
                       double* partials, double* G);
 void launch_gram(hipStream_t s, int prec, uint32_t n, int m, int mw, const void* V, const void* W, double* partials,

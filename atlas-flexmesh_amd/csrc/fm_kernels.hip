@@ -1650,6 +1650,10 @@ __global__ __launch_bounds__(1024) void reduce_kernel(const double* __restrict__
                 st->beta = s / st->rz;
                 st->rz = s;
             }
+            if (st->hist && (uint32_t)(st->iters - 1) < st->hist_cap) {
+                st->hist[2 * (st->iters - 1)] = st->alpha;
+                st->hist[2 * (st->iters - 1) + 1] = st->done ? 0.0 : st->beta;
+            }
             break;
         }
         case 3:
@@ -1948,7 +1952,7 @@ __global__ void poly_basis_kernel(uint32_t nn, const float4* __restrict__ xn, in
 // thread adds the tile's contributions to its 5 x 7 tile of G (sums in T over
 // the tile, FP64 over the tiles).  partials: one 3 NM x (3 NM + kPgStored)
 // block per workgroup, row-major.
-constexpr int kPgStored = 13;
+constexpr int kPgStored = 21;
 constexpr int kPgBlocks = 512;
 constexpr int kPgThreads = 384;
 #ifndef FM_PG_TILE32
@@ -2332,11 +2336,13 @@ __global__ __launch_bounds__(1024) void coarse_solve_kernel(uint32_t nb, int m, 
 
 // p = (y + Z c) + beta p, or p = y + Z c on the first (beta ignored); one
 // thread per node, so the coefficient loads are the same for every lane
-// (coefT: the coefficients in T, see coarse_solve_kernel)
+// (coefT: the coefficients in T, see coarse_solve_kernel).  lz (optional):
+// the Lanczos vector z_j / sqrt(r_j.z_j) of this iteration j = st->iters
+// (z = y + Z c, the preconditioned residual) goes to column j while j < lz_cap.
 template <typename T, int NM>
 __global__ void pcg_p_update_coarse_kernel(uint32_t nn, const T* __restrict__ y, T* __restrict__ p,
                                            const float4* __restrict__ xn, const T* __restrict__ coefT, int first,
-                                           const PcgState* __restrict__ st) {
+                                           const PcgState* __restrict__ st, T* __restrict__ lz, uint32_t lz_cap) {
     if (st->done) return;
     const uint32_t v = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (v >= nn) return;
@@ -2344,6 +2350,9 @@ __global__ void pcg_p_update_coarse_kernel(uint32_t nn, const T* __restrict__ y,
     const Powers<T> pw(c4);
     const int mask = (int)c4.w;
     const T beta = first ? (T)0 : (T)st->beta;
+    const uint32_t j = (uint32_t)st->iters;
+    T* lzj = lz && j < lz_cap ? lz + (size_t)j * 3 * nn : nullptr;
+    const T zs = lzj ? (T)(1.0 / sqrt(st->rz)) : (T)0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         T zc = (T)0;
@@ -2353,6 +2362,7 @@ __global__ void pcg_p_update_coarse_kernel(uint32_t nn, const T* __restrict__ y,
         const size_t i = 3 * (size_t)v + c;
         const T z = y[i] + zc;
         p[i] = first ? z : z + beta * p[i];
+        if (lzj) lzj[i] = z * zs;
     }
 }
 
@@ -3278,24 +3288,55 @@ void launch_coarse_correct(hipStream_t s, int prec, uint32_t n, int nmono, const
 
 template <typename T>
 static void p_update_coarse_t(hipStream_t s, uint32_t n, int nmono, const T* y, T* p, const float4* xn,
-                              const double* coef64, int first, const PcgState* st) {
+                              const double* coef64, int first, const PcgState* st, T* lz, uint32_t lz_cap) {
     const uint32_t nn = n / 3, nb = cdiv(nn, 256);
     const T* coef = sizeof(T) == 8 ? reinterpret_cast<const T*>(coef64)
                                    : reinterpret_cast<const T*>(coef64 + 3 * nmono);
     switch (nmono) {
-        case 4: pcg_p_update_coarse_kernel<T, 4><<<nb, 256, 0, s>>>(nn, y, p, xn, coef, first, st); break;
-        case 10: pcg_p_update_coarse_kernel<T, 10><<<nb, 256, 0, s>>>(nn, y, p, xn, coef, first, st); break;
-        case 20: pcg_p_update_coarse_kernel<T, 20><<<nb, 256, 0, s>>>(nn, y, p, xn, coef, first, st); break;
-        default: pcg_p_update_coarse_kernel<T, 35><<<nb, 256, 0, s>>>(nn, y, p, xn, coef, first, st); break;
+        case 4: pcg_p_update_coarse_kernel<T, 4><<<nb, 256, 0, s>>>(nn, y, p, xn, coef, first, st, lz, lz_cap); break;
+        case 10: pcg_p_update_coarse_kernel<T, 10><<<nb, 256, 0, s>>>(nn, y, p, xn, coef, first, st, lz, lz_cap); break;
+        case 20: pcg_p_update_coarse_kernel<T, 20><<<nb, 256, 0, s>>>(nn, y, p, xn, coef, first, st, lz, lz_cap); break;
+        default: pcg_p_update_coarse_kernel<T, 35><<<nb, 256, 0, s>>>(nn, y, p, xn, coef, first, st, lz, lz_cap); break;
     }
 }
 
 void launch_pcg_p_update_coarse(hipStream_t s, int prec, uint32_t n, int nmono, const void* y, void* p,
-                                const float4* xn, const double* coef, bool first, const PcgState* st) {
+                                const float4* xn, const double* coef, bool first, const PcgState* st, void* lz,
+                                uint32_t lz_cap) {
     if (prec == PREC64)
-        p_update_coarse_t<double>(s, n, nmono, (const double*)y, (double*)p, xn, coef, first ? 1 : 0, st);
+        p_update_coarse_t<double>(s, n, nmono, (const double*)y, (double*)p, xn, coef, first ? 1 : 0, st,
+                                  (double*)lz, lz_cap);
     else
-        p_update_coarse_t<float>(s, n, nmono, (const float*)y, (float*)p, xn, coef, first ? 1 : 0, st);
+        p_update_coarse_t<float>(s, n, nmono, (const float*)y, (float*)p, xn, coef, first ? 1 : 0, st,
+                                 (float*)lz, lz_cap);
+}
+
+// Y[:, q] = sum over j < m of S[j k + q] Z[:, j] for q < k (k <= 8; Z, Y of
+// leading dimension n; S row-major m x k, FP64): one thread per row, every
+// column of Z read once
+template <typename T>
+__global__ void basis_combine_multi_kernel(uint32_t n, int m, int k, const double* __restrict__ S,
+                                           const T* __restrict__ Z, T* __restrict__ Y) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < m; ++j) {
+        const double z = (double)Z[(size_t)j * n + i];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (q < k) acc[q] += S[j * k + q] * z;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        if (q < k) Y[(size_t)q * n + i] = (T)acc[q];
+}
+
+void launch_basis_combine_multi(hipStream_t s, int prec, uint32_t n, int m, int k, const double* S, const void* Z,
+                                void* Y) {
+    if (prec == PREC64)
+        basis_combine_multi_kernel<double><<<cdiv(n, 256), 256, 0, s>>>(n, m, k, S, (const double*)Z, (double*)Y);
+    else
+        basis_combine_multi_kernel<float><<<cdiv(n, 256), 256, 0, s>>>(n, m, k, S, (const float*)Z, (float*)Y);
 }
 
 // a Gram matrix of few columns has few update threads per block (one 4x4

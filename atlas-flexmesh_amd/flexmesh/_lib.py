@@ -34,7 +34,7 @@ class NewtonCfg(C.Structure):
                 ("inner_tol_max", C.c_double), ("coarse_degree", C.c_int),
                 ("coarse_additive", C.c_int), ("lag_precond", C.c_int),
                 ("extrap_order", C.c_int), ("slope_start", C.c_int),
-                ("factor_steps", C.c_int), ("keep_coarse", C.c_int)]
+                ("factor_steps", C.c_int), ("keep_coarse", C.c_int), ("ritz", C.c_int)]
 
 
 FIELD_NAMES = ("det_F", "cauchy", "von_mises", "pressure", "max_principal", "energy", "max_stretch", "F")
@@ -101,6 +101,7 @@ def lib():
     L.fm_newton_solve_to.argtypes = [_vp, C.c_double, _vp, C.POINTER(NewtonCfg), C.POINTER(NewtonStats)]
     L.fm_newton_history.argtypes = [_vp, C.c_uint32, _vp, _vp, _vp, C.POINTER(C.c_uint32)]
     L.fm_coarse_moments.argtypes = [_vp, C.c_int, C.c_int, _f64p, _f64p]
+    L.fm_pcg_history.argtypes = [_vp, C.c_uint32, _vp, _vp, C.POINTER(C.c_uint32)]
     L.fm_get_u.argtypes = [_vp, _f64p]
     L.fm_set_u.argtypes = [_vp, _vp]
     L.fm_adjoint.argtypes = [_vp, _f64p, C.c_double, C.c_int, _vp, _vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
@@ -371,6 +372,17 @@ class Solver:
         self._chk(self.L.fm_newton_history(self.h, n.value, res.ctypes.data, alpha.ctypes.data if alpha.size else None,
                                            en.ctypes.data, C.byref(n)), "fm_newton_history")
         return res, alpha, en
+
+    def pcg_history(self):
+        """(alpha, beta) of every iteration of the last PCG solve (Lanczos
+        coefficients of the preconditioned operator)."""
+        n = C.c_uint32(0)
+        self._chk(self.L.fm_pcg_history(self.h, 0, None, None, C.byref(n)), "fm_pcg_history")
+        a, b = np.zeros(n.value), np.zeros(n.value)
+        if n.value:
+            self._chk(self.L.fm_pcg_history(self.h, n.value, a.ctypes.data, b.ctypes.data, C.byref(n)),
+                      "fm_pcg_history")
+        return a, b
 
     def coarse_moments(self, r, degree, prec):
         """Z^T r over the polynomial fields of total degree <= degree (the

@@ -327,6 +327,9 @@ def main():
     ap.add_argument("--factor-steps", type=int, default=None,
                     help="with the lagged preconditioner: load steps one ILU(0) factor serves (default 6 with the "
                          "coarse level; 1: a new factor every step)")
+    ap.add_argument("--ritz", type=int, default=None,
+                    help="Ritz vectors of each solve (<= 8) in the next step's Galerkin start (default 4 with the "
+                         "coarse level)")
     ap.add_argument("--keep-coarse", type=int, default=None,
                     help="1: the steps that keep an earlier factor also keep its coarse operator Z^T K Z (default 0)")
     ap.add_argument("--lag", type=int, default=None,
@@ -399,8 +402,9 @@ def bench(S, cfg, args, warm_last, K):
     # the target-aware forcing term clamp(0.5 target/||R||, 1e-3, 0.1), so
     # that from the cubic start one Newton iteration reaches the target; a
     # later iteration of a step keeps the first one's factor and coarse
-    # operator; without them the extrapolated start takes 1e-2 throughout,
-    # the predictor solve 3e-3
+    # operator, and the four Ritz vectors of the slowest modes that the last
+    # solve's Lanczos process found join the next step's start; without them
+    # the extrapolated start takes 1e-2 throughout, the predictor solve 3e-3
     recycle = args.recycle if args.recycle is not None else (6 if lift == 2 else 0)
     coarse = args.coarse if args.coarse is not None else (4 if lift == 2 else 0)
     additive = args.additive if args.additive is not None else (1 if lift == 2 and coarse else 0)
@@ -416,9 +420,10 @@ def bench(S, cfg, args, warm_last, K):
     slope = args.slope if args.slope is not None else 0
     fsteps = args.factor_steps if args.factor_steps is not None else (6 if lag else 1)
     kcoarse = args.keep_coarse if args.keep_coarse is not None else 0
+    ritz = args.ritz if args.ritz is not None else (4 if additive else 0)
     solver_kw = dict(recycle=recycle, inner_tol_max=inner_tol_max, coarse_degree=coarse, coarse_additive=additive,
                      lag_precond=lag, extrap_order=order, slope_start=slope, factor_steps=fsteps,
-                     keep_coarse=kcoarse)
+                     keep_coarse=kcoarse, ritz=ritz)
     if warm_last:
         log(f"warm-up pass: load steps 1..{warm_last}")
         run_steps(S, cfg, prec, 1, warm_last, inner_tol, False, args.pcg_max, lift, **solver_kw)
@@ -439,7 +444,7 @@ def bench(S, cfg, args, warm_last, K):
                                f"load steps 1..{K} of {cfg['n_steps']}", "dofs": dofs, "tets": info["ne"],
                    "nnz": info["nnz"], "predictor": "extrapolate" if lift == 2 else "solve",
                    "inner_tol": inner_tol, "inner_tol_max": inner_tol_max, "recycle": recycle,
-                   "coarse_degree": coarse, "coarse_additive": additive, "lag_precond": lag, "extrap_order": order, "slope_start": slope, "factor_steps": fsteps, "keep_coarse": kcoarse,
+                   "coarse_degree": coarse, "coarse_additive": additive, "lag_precond": lag, "extrap_order": order, "slope_start": slope, "factor_steps": fsteps, "keep_coarse": kcoarse, "ritz": ritz,
                    "newton_rtol": cfg["newton_rtol"]},
         "newton": {"converged_steps": tot["converged"], "newton_iters": tot["newton_iters"],
                    "pcg_iters": tot["pcg_iters"], "newton_it_per_s": tot["newton_iters"] / wall,
@@ -493,6 +498,8 @@ def bench(S, cfg, args, warm_last, K):
                                         + ("the first iteration's factor and coarse operator kept for the later "
                                            "iterations of a step, " if lag else "")
                                         + (f"one ILU(0) factor for up to {fsteps} load steps, " if lag and fsteps > 1
+                                           else "")
+                                        + (f"the {ritz} Ritz vectors of the last solve in the start, " if ritz
                                            else "")
                                         + (f"forcing term clamp(0.5 target/||R||, {inner_tol:g}, {inner_tol_max:g})"
                                            if inner_tol_max > inner_tol else f"forcing term {inner_tol:g}")

@@ -213,6 +213,11 @@ typedef struct {
     int keep_coarse;      /* with factor_steps: such a step also keeps the coarse operator
                              Z^T K Z of the step that formed the factor (its Galerkin start
                              forms only the recycled corrections' rows); 0 = off */
+    int ritz;             /* with coarse_additive: each inner solve keeps its first 48 Lanczos
+                             vectors, and the eigenvectors of its Lanczos matrix with the ritz
+                             (<= 8) smallest eigenvalues combine them into approximations of the
+                             preconditioned operator's slowest modes; the next load step's first
+                             solve starts from the Galerkin projection onto them too; 0 = off */
 } fm_newton_cfg;
 
 typedef struct {
@@ -359,6 +364,13 @@ int fm_profile_get(fm_ctx* ctx, double* ms, uint64_t* launches);
    xyz; x^4, y^4, z^4, x^3 y, x^3 z, x y^3, y^3 z, x z^3, y z^3, x^2 y^2,
    y^2 z^2, x^2 z^2, x^2 y z, x y^2 z, x y z^2); nmono = 4, 10, 20, 35 */
 int fm_coarse_moments(fm_ctx* ctx, int prec, int degree, const double* r, double* out);
+/* No counterpart in the reference.  alpha_j and beta_j of every iteration of
+   the last PCG solve of the FP32 or FP64 Krylov path (the first min(cap, len)
+   of len; beta of the last iteration 0): the Lanczos coefficients from which
+   the preconditioned operator's Ritz values follow (T_jj = 1/alpha_j +
+   beta_(j-1)/alpha_(j-1), T_j,j+1 = sqrt(beta_j)/alpha_j).  At most 4096
+   iterations are recorded. */
+int fm_pcg_history(fm_ctx* ctx, uint32_t cap, double* alpha, double* beta, uint32_t* len);
 /* sizes for roofline bookkeeping: n (rows), nnz, n_elems, n_nodes, n_colors */
 int fm_info(fm_ctx* ctx, uint64_t* out5);
 /* kernels chosen for the current matrix: ILU(0) sweep kind (0 level-scheduled,

@@ -360,3 +360,40 @@ def test_fused_polynomial_gram_matches_generic(tmp_path):
     assert np.abs(a["mixed_its"][:, 1] - b["mixed_its"][:, 1]).max() <= 2, (a["mixed_its"], b["mixed_its"])
     for name in ("mixed", "fp64"):
         assert rel(a[name + "_u"], b[name + "_u"]) < 1e-6, name
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_ritz_vectors_in_the_start(fm, S, O, prec):
+    """With ritz = 6, every solve keeps its Lanczos vectors and the next
+    step's first solve also projects onto the Ritz vectors of the slowest
+    modes: every step still reaches ||R|| <= 1e-8 ||R_0||, the final state
+    matches the FP64 oracle's, the path repeats bit for bit after a reset,
+    and the Lanczos coefficients of a solve give a Lanczos matrix whose
+    eigenvalues lie in (0, ~6] (the preconditioned tangent is SPD)."""
+    cfg = cube(fm, n_steps=8, dims=(8, 8, 8))
+    fm.configs.load(S, cfg)
+    P = oracle(O, cfg, pcg_tol=1e-12, pcg_max=4000)
+    uo = np.zeros(cfg["X"].size)
+    for s in range(1, 9):
+        P.step(s, uo)
+    kw = dict(prec=fm.FM_MIXED) if prec == "mixed" else dict(prec=fm.FM_FP64, inexact=1)
+    c = fm.configs.newton_cfg(cfg, lift=2, inner_tol=1e-3, inner_tol_max=0.1, recycle=6, coarse_degree=4,
+                              coarse_additive=1, lag_precond=1, factor_steps=6, extrap_order=3, ritz=6, **kw)
+    runs = []
+    for _ in range(2):
+        S.set_u(None)
+        stats = []
+        for s in range(1, 9):
+            rc, st = S.newton_step(s, c)
+            assert st["converged"] and st["rel_res"] <= 1e-8 and st["pcg_cap_hits"] == 0, (s, st)
+            stats.append((st["newton_iters"], st["pcg_iters"]))
+            a, b = S.pcg_history()
+            if a.size >= 3:
+                d = 1.0 / a
+                d[1:] += b[:-1] / a[:-1]
+                e = -np.sqrt(np.maximum(b[:-1], 0)) / a[:-1]
+                w = np.linalg.eigvalsh(np.diag(d) + np.diag(e, 1) + np.diag(e, -1))
+                assert w.min() > 0 and w.max() < 10, (s, w.min(), w.max())
+        runs.append((S.get_u(), stats))
+        assert rel(runs[-1][0], uo) < 1e-6
+    assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1] == runs[1][1]
