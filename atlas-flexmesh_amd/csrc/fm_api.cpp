@@ -1028,8 +1028,10 @@ int galerkin_start(fm_ctx* ctx, Matrix& A, int p, int ring_use, int deg, bool ad
         FM_HIP(hipMemcpyAsync(W + (size_t)bcol * n * es, b, n * es, hipMemcpyDeviceToDevice, s));
         if (fused) {
             // ring slots that are not candidates hold stale columns: their
-            // entries are never read
-            if (ring_use > 0) launch_gram(s, p, n, m, m + 1, V, W, ctx->cs_partials.p, ctx->cs_G.p);
+            // entries are never read (the Ritz vectors are stored columns
+            // too: the first solve after a reset has them without any
+            // recycled correction)
+            if (ring_use > 0 || ritz_use > 0) launch_gram(s, p, n, m, m + 1, V, W, ctx->cs_partials.p, ctx->cs_G.p);
             if (reuse) {
                 // the moment kernels return at once on a finished PCG state
                 PcgState init{};
@@ -2422,9 +2424,15 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         // the previous `recycle` steps in a steady run), and every correction
         // joins them
         const int start_from = recycle && it == 0 ? std::min(2 * cfg.recycle, kCorr) : 0;
+        // the Ritz vectors of the previous solve join that start only beside
+        // recycled corrections: at a path's first step they come from the
+        // predictor solve at the unloaded state, and there they cost the
+        // 6.4M-DOF cube a Newton iteration (3 solves, 502 PCG iterations
+        // against 2 and 391)
+        const int ritz_use = it == 0 && ctx->gal_n[kprec == PREC64 ? 0 : 1] > 0 ? ritz : 0;
         FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, eta, lin_cap, &its, inexact, &S.pcg_cap_hits,
                             start_from, coarse, recycle, additive, lag, reuse_poly && cfg.keep_coarse, ritz,
-                            it == 0 ? ritz : 0));
+                            ritz_use));
         S.pcg_iters += its;
         if (!keep && !lag) S.factorizations++;
         trace_solve(lf, it, eta, its, lag);

@@ -217,7 +217,8 @@ typedef struct {
                              vectors, and the eigenvectors of its Lanczos matrix with the ritz
                              (<= 8) smallest eigenvalues combine them into approximations of the
                              preconditioned operator's slowest modes; the next load step's first
-                             solve starts from the Galerkin projection onto them too; 0 = off */
+                             solve starts from the Galerkin projection onto them too, when it
+                             also has recycled corrections (not at a path's first step); 0 = off */
 } fm_newton_cfg;
 
 typedef struct {

@@ -369,8 +369,14 @@ def test_ritz_vectors_in_the_start(fm, S, O, prec):
     modes: every step still reaches ||R|| <= 1e-8 ||R_0||, the final state
     matches the FP64 oracle's, the path repeats bit for bit after a reset,
     and the Lanczos coefficients of a solve give a Lanczos matrix whose
-    eigenvalues lie in (0, ~6] (the preconditioned tangent is SPD)."""
-    cfg = cube(fm, n_steps=8, dims=(8, 8, 8))
+    eigenvalues lie in (0, ~6] (the preconditioned tangent is SPD).
+
+    The path runs three times from the unloaded state: its first 3 steps,
+    then twice in full.  The first Newton solve after a reset projects onto
+    the predictor solve's Ritz vectors with no recycled correction beside
+    them, and each pass leaves the device's Gram buffers in another state,
+    so the passes agree only if that start forms every entry it reads."""
+    cfg = cube(fm, n_steps=8, dims=(12, 12, 12))
     fm.configs.load(S, cfg)
     P = oracle(O, cfg, pcg_tol=1e-12, pcg_max=4000)
     uo = np.zeros(cfg["X"].size)
@@ -378,12 +384,12 @@ def test_ritz_vectors_in_the_start(fm, S, O, prec):
         P.step(s, uo)
     kw = dict(prec=fm.FM_MIXED) if prec == "mixed" else dict(prec=fm.FM_FP64, inexact=1)
     c = fm.configs.newton_cfg(cfg, lift=2, inner_tol=1e-3, inner_tol_max=0.1, recycle=6, coarse_degree=4,
-                              coarse_additive=1, lag_precond=1, factor_steps=6, extrap_order=3, ritz=6, **kw)
+                              coarse_additive=1, lag_precond=1, factor_steps=6, extrap_order=3, ritz=4, **kw)
     runs = []
-    for _ in range(2):
+    for last in (3, 8, 8):
         S.set_u(None)
         stats = []
-        for s in range(1, 9):
+        for s in range(1, last + 1):
             rc, st = S.newton_step(s, c)
             assert st["converged"] and st["rel_res"] <= 1e-8 and st["pcg_cap_hits"] == 0, (s, st)
             stats.append((st["newton_iters"], st["pcg_iters"]))
@@ -395,5 +401,6 @@ def test_ritz_vectors_in_the_start(fm, S, O, prec):
                 w = np.linalg.eigvalsh(np.diag(d) + np.diag(e, 1) + np.diag(e, -1))
                 assert w.min() > 0 and w.max() < 10, (s, w.min(), w.max())
         runs.append((S.get_u(), stats))
-        assert rel(runs[-1][0], uo) < 1e-6
-    assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1] == runs[1][1]
+    assert rel(runs[1][0], uo) < 1e-6
+    assert runs[1][1][:3] == runs[0][1], (runs[0][1], runs[1][1])
+    assert np.array_equal(runs[1][0], runs[2][0]) and runs[1][1] == runs[2][1], (runs[1][1], runs[2][1])
