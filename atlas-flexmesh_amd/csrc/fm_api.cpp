@@ -775,6 +775,7 @@ int pcg_run_pc(fm_ctx* ctx, Matrix& A, double tol, int max_it, PcgResult* res) {
     PcgState init{};
     init.tol = tol;
     init.max_it = max_it;
+    ctx->pcg_hist_len = 0;  // the flexible beta has no Lanczos relation: no coefficients recorded
     *ctx->h_st = init;
     FM_HIP(hipMemcpyAsync(ctx->st.p, ctx->h_st, sizeof(PcgState), hipMemcpyHostToDevice, s));
     {
@@ -822,9 +823,16 @@ int pcg_run_pc(fm_ctx* ctx, Matrix& A, double tol, int max_it, PcgResult* res) {
         return launch_f32_to_f64_dot(s, n, y32, r, y, ctx->partials.p, check_done ? old_part : nullptr,
                                      check_done ? ctx->st.p : nullptr);
     };
+    // with the additive coarse level (ctx->ad_nmono fields) z = y + Z c,
+    // c = E^{-1} Z^T r, whose r.(Z c) the coarse solve adds to r.z
+    const int ad = ctx->ad_nmono;
+    if (ad) launch_coarse_correct(s, PREC64, n, ad, ctx->cs_xn.p, r, ctx->ad_Einv.p, ctx->ad_part.p, ctx->ad_coef.p, ctx->st.p);
     np = precond(false);
     launch_reduce(s, ctx->partials.p, np, ctx->st.p, 3, 0);
-    launch_copy(s, PREC64, n, y, p);
+    if (ad)
+        launch_pcg_p_update_coarse(s, PREC64, n, ad, y, p, ctx->cs_xn.p, ctx->ad_coef.p, true, ctx->st.p);
+    else
+        launch_copy(s, PREC64, n, y, p);
     FM_TRY(check_launch(ctx, "pcg init"));
     FM_TRY(pcg_batches(ctx, max_it, [&] {
             {
@@ -832,14 +840,23 @@ int pcg_run_pc(fm_ctx* ctx, Matrix& A, double tol, int max_it, PcgResult* res) {
                 launch_spmv(s, PREC64, A, p, Ap, true, ctx->partials.p, nullptr, nullptr, ctx->st.p);
             }
             launch_reduce(s, ctx->partials.p, spmv_blocks(A, PREC64), ctx->st.p, 1, 0);
-            {
+            if (ad) {
+                // the moments of the new r, then c and the coarse terms of r.z
+                // and of the flexible beta (r_new.(Z c_old))
+                ProfScope pu(ctx, FM_K_COARSE);
+                launch_pcg_xr_update_moments(s, PREC64, n, ad, p, Ap, x, r, ctx->cs_xn.p, ctx->ad_part.p, ctx->st.p);
+                launch_coarse_solve(s, ad, ctx->ad_part.p, ctx->ad_Einv.p, ctx->ad_coef.p, ctx->st.p);
+            } else {
                 ProfScope pu(ctx, FM_K_PCG_OTHER);
                 launch_pcg_xr_update(s, PREC64, n, p, Ap, x, r, ctx->st.p);
             }
             np = precond(true);
             launch_reduce(s, old_part, np, ctx->st.p, 0, 3);
             launch_reduce(s, ctx->partials.p, np, ctx->st.p, 4, 3);
-            launch_pcg_p_update(s, PREC64, n, y, p, ctx->st.p);
+            if (ad)
+                launch_pcg_p_update_coarse(s, PREC64, n, ad, y, p, ctx->cs_xn.p, ctx->ad_coef.p, false, ctx->st.p);
+            else
+                launch_pcg_p_update(s, PREC64, n, y, p, ctx->st.p);
     }));
     FM_TRY(check_device_error(ctx));
     const PcgState& h = *ctx->h_st;
@@ -1364,10 +1381,20 @@ int linear_solve(fm_ctx* ctx, int prec, const double* rhs, double* du, double to
         }
     }
     if (prec == PRECPC) {
+        // FP64 Krylov vectors: the FP64 path's Galerkin start, recycled
+        // corrections and coarse level (no Ritz vectors)
         FM_TRY(pcg_vectors(ctx, PREC64, n));
         launch_scale(ctx->stream, n, rhs, 1.0 / nr, ctx->b64.p);
-        launch_fill_f64(ctx->stream, ctx->x64.p, n, 0.0);
-        FM_TRY(pcg_run_pc(ctx, A, tol, max_it, &pr));
+        if (lag && additive && ring_use == 0 && ctx->lag_nmono > 0 && ctx->lag_prec == PREC64)
+            FM_TRY(lagged_start(ctx, A, PREC64));
+        else if ((ring_use > 0 && ctx->gal_n[0] > 0) || deg > 0)
+            FM_TRY(galerkin_start(ctx, A, PREC64, ring_use, deg, additive, reuse_poly, 0));
+        else
+            launch_fill_f64(ctx->stream, ctx->x64.p, n, 0.0);
+        const int prc = pcg_run_pc(ctx, A, tol, max_it, &pr);
+        ctx->ad_nmono = 0;
+        FM_TRY(prc);
+        if (push) FM_TRY(galerkin_push(ctx, n, PREC64, deg));
         launch_scale(ctx->stream, n, ctx->x64.p, nr, du);
         *iters = pr.iters;
         if (capped && pr.status != 1) ++*capped;
@@ -2218,9 +2245,11 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
     const bool inexact = kprec == PREC64 && cfg.inexact;
     const double lin_tol = kprec == PREC64 && !inexact ? cfg.pcg_tol : cfg.inner_tol;
     // the options of the inner solves: target-aware forcing, Galerkin start
-    const bool adapt = (kprec == PREC32 || inexact) && cfg.inner_tol_max > lin_tol;
-    const bool recycle = (kprec == PREC32 || inexact) && cfg.lift == 2 && cfg.recycle > 0;
-    const int coarse = (kprec == PREC32 || inexact) ? std::max(0, std::min(cfg.coarse_degree, kMaxDegree)) : 0;
+    // (the forcing-term solves: mixed, FP32-preconditioned, inexact FP64)
+    const bool inner = kprec != PREC64 || inexact;
+    const bool adapt = inner && cfg.inner_tol_max > lin_tol;
+    const bool recycle = inner && cfg.lift == 2 && cfg.recycle > 0;
+    const int coarse = inner ? std::max(0, std::min(cfg.coarse_degree, kMaxDegree)) : 0;
     const bool additive = coarse > 0 && cfg.coarse_additive != 0;
     // Ritz vectors of each solve for the next step's first solve (with the coarse level)
     const int ritz = additive && (kprec == PREC32 || inexact) ? std::max(0, std::min(cfg.ritz, kRitz)) : 0;
@@ -2393,7 +2422,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         const bool keep = it > 0 && (cfg.reuse_tangent || (order == 2 && !adapt && rn <= 1e-2 * rn_prev));
         // a lagged preconditioner: the new tangent with the first iteration's
         // factor (and coarse operator), which the assembly leaves in place
-        const bool lag_on = it > 0 && !keep && cfg.lag_precond && (kprec == PREC32 || inexact);
+        const bool lag_on = it > 0 && !keep && cfg.lag_precond && inner;
         bool lag = false, reuse_poly = false;
         if (!keep) {
             const int fp = kprec == PREC32 ? PREC32 : PREC64;
@@ -2402,7 +2431,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
             // >= 2 keeps the factor of an earlier step while it has served
             // fewer than factor_steps steps
             const bool keep_factor = it == 0 && cfg.lag_precond && cfg.factor_steps > 1 && order >= 2 &&
-                                     (kprec == PREC32 || inexact) && ctx->tangent.lu_valid[fp] &&
+                                     inner && ctx->tangent.lu_valid[fp] &&
                                      ctx->fact_age < cfg.factor_steps;
             FM_TRY(assemble(ctx, ctx->u.p, true, aprec, true, nullptr, FM_K_ASSEMBLE));
             S.assemblies++;
@@ -2429,7 +2458,7 @@ int fm_newton_solve_to(fm_ctx* ctx, double load_factor, double* u, const fm_newt
         // predictor solve at the unloaded state, and there they cost the
         // 6.4M-DOF cube a Newton iteration (3 solves, 502 PCG iterations
         // against 2 and 391)
-        const int ritz_use = it == 0 && ctx->gal_n[kprec == PREC64 ? 0 : 1] > 0 ? ritz : 0;
+        const int ritz_use = it == 0 && ctx->gal_n[kprec == PREC32 ? 1 : 0] > 0 ? ritz : 0;
         FM_TRY(linear_solve(ctx, kprec, ctx->hv.p, ctx->du.p, eta, lin_cap, &its, inexact, &S.pcg_cap_hits,
                             start_from, coarse, recycle, additive, lag, reuse_poly && cfg.keep_coarse, ritz,
                             ritz_use));

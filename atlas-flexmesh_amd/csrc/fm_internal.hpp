@@ -41,15 +41,16 @@ struct DevBuf {
 
 // Scalars of one PCG solve, kept on the device so that no host round trip is
 // needed between iterations.  Kernels read `done` and return at once when set.
-constexpr int kReduceBlocks = 64;
+constexpr int kReduceBlocks = 64;  // most blocks of one reduce (each sums a contiguous chunk of partials)
 constexpr int kPcgHistCap = 4096;  // iterations whose alpha and beta a PCG solve records
-constexpr int kLanczosCap = 48;    // Lanczos vectors a harvesting PCG solve keeps  // most blocks of one reduce (each sums a contiguous chunk of partials)
+constexpr int kLanczosCap = 48;    // Lanczos vectors a harvesting PCG solve keeps
 
 struct PcgState {
     double rz, pAp, alpha, beta, bb, rz_new, relres, tol;
     double slot[4];
     double gal[48];  // Gram matrix, right-hand side and a norm of a Galerkin start (reduce op 5)
     double rz_coarse;  // r.(Z E^{-1} Z^T r) of the additive coarse correction, added to r.z (0 without one)
+    double rz_cross;   // r.(Z c_old) with the previous coefficients: the coarse part of the flexible beta's z_old term
     int done;     // 1 once converged, broken down or out of iterations
     int status;   // 0 running, 1 converged, 2 breakdown (|pAp| < 1e-60), 3 max_it
     int iters;

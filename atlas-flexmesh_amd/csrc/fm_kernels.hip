@@ -1666,6 +1666,10 @@ __global__ __launch_bounds__(1024) void reduce_kernel(const double* __restrict__
             break;
         case 4: {  // as 2, with the flexible beta r_new.(z_new - z_old) / r_old.z_old (z_old term in slot[k])
             if (st->done) return;
+            // with the additive coarse correction z = y + Z c: r.(Z c) and
+            // r_new.(Z c_old) from the coarse solve (0 without one)
+            s += st->rz_coarse;
+            st->slot[k] += st->rz_cross;
             st->rz_new = s;
             st->iters += 1;
             const double rel = sqrt(s / fmax(st->bb, 1e-300));
@@ -2321,17 +2325,25 @@ __global__ __launch_bounds__(1024) void coarse_solve_kernel(uint32_t nb, int m, 
         for (int j = sl; j < m; j += S) c += Einv[(size_t)j * m + i] * s_g[j];
     s_part[threadIdx.x] = c;
     __syncthreads();
-    double gc = 0.0;
+    double gc = 0.0, go = 0.0;
     if (threadIdx.x < m) {
         double ci = 0.0;
         for (int q = 0; q < S; ++q) ci += s_part[q * mp + threadIdx.x];
+        // the previous coefficients against the new moments (the flexible
+        // beta's r_new.(Z c_old)); read before they are overwritten
+        go = coef[threadIdx.x] * s_g[threadIdx.x];
         coef[threadIdx.x] = ci;
         reinterpret_cast<float*>(coef + m)[threadIdx.x] = (float)ci;
         gc = ci * s_g[threadIdx.x];
     }
     __syncthreads();  // block_sum's slots are its own; the barrier orders the reads above
     const double t = block_sum(gc);
-    if (threadIdx.x == 0) st->rz_coarse = t;
+    __syncthreads();  // block_sum's shared scratch is reused
+    const double to = block_sum(go);
+    if (threadIdx.x == 0) {
+        st->rz_coarse = t;
+        st->rz_cross = to;
+    }
 }
 
 // p = (y + Z c) + beta p, or p = y + Z c on the first (beta ignored); one

@@ -102,9 +102,14 @@ def make(k, dims=None, L_box=None):
         # ~4 n_x iterations before it converges, at any tolerance; its tangent
         # (condition number ~1e6-1e7) is too ill-conditioned for FP32 search
         # directions, so the Krylov iteration runs in FP64 with the FP32
-        # ILU(0) factor as preconditioner
+        # ILU(0) factor as preconditioner.  The bending modes are smooth: the
+        # polynomial coarse level and Galerkin start of the inexact Newton
+        # solves remove the plateau (inner_opts, applied to the forcing-term
+        # solves of FM_MIXED_PC and FM_MIXED)
         cfg.update(newton_rtol=1e-8, pcg_tol=1e-10, prec=L.FM_MIXED_PC, ls_mode=1, max_newton=60,
-                   pcg_max=8 * nx + 400, inner_max=8 * nx + 400, inner_tol=1e-2)
+                   pcg_max=8 * nx + 400, inner_max=8 * nx + 400, inner_tol=1e-3,
+                   inner_opts=dict(lift=2, inner_tol_max=0.1, recycle=6, coarse_degree=4, coarse_additive=1,
+                                   lag_precond=1, factor_steps=6, extrap_order=3))
     else:
         raise ValueError(k)
     if stretch != 0.0:
@@ -127,11 +132,18 @@ def load(solver, cfg):
 
 
 def newton_cfg(cfg, **kw):
+    """The configuration's Newton settings, then the keyword overrides.  A
+    configuration's inner_opts (options of the forcing-term inner solves)
+    apply when the resulting precision is FM_MIXED or FM_MIXED_PC."""
     c = L.Solver.newton_cfg(n_steps=cfg["n_steps"], prec=cfg["prec"], newton_rtol=cfg["newton_rtol"],
                             pcg_tol=cfg["pcg_tol"])
     for k in ("ls_mode", "max_newton", "pcg_max", "inner_max", "inner_tol"):
         if k in cfg:
             setattr(c, k, cfg[k])
+    prec = kw.get("prec", cfg["prec"])
+    if prec in (L.FM_MIXED, L.FM_MIXED_PC):
+        for k, v in cfg.get("inner_opts", {}).items():
+            setattr(c, k, v)
     for k, v in kw.items():
         setattr(c, k, v)
     return c

@@ -138,7 +138,7 @@ int fm_pcg(fm_ctx* ctx, int prec, const double* b, double* x, double tol, int ma
  * interface (FEM/nonlinear_solver.hpp:23-64, 87-238). */
 typedef struct {
     uint32_t n_steps;     /* load steps to reach load factor 1 */
-    int prec;             /* FM_FP64 or FM_MIXED */
+    int prec;             /* FM_FP64, FM_MIXED or FM_MIXED_PC */
     double newton_rtol;   /* ||R_k|| <= rtol ||R_0|| */
     int max_newton;
     double pcg_tol;       /* pcg_solve_ilu0 tol of the FP64 path */
@@ -171,8 +171,9 @@ typedef struct {
     int inexact;          /* FM_FP64 only: 1 = the FP64 inner solves take the mixed path's forcing
                              term (inner_tol, inner_max, normalised right-hand side) instead of the
                              reference's pcg_tol / pcg_max, so both precisions run one algorithm */
-    /* inner solves of FM_MIXED and inexact FM_FP64 (no counterpart in the reference, whose
-       pcg_solve_ilu0 takes its start x from the caller, :1283-1288) */
+    /* inner solves of FM_MIXED, FM_MIXED_PC and inexact FM_FP64 (no counterpart in the
+       reference, whose pcg_solve_ilu0 takes its start x from the caller, :1283-1288);
+       ritz applies to FM_MIXED and inexact FM_FP64 only */
     int recycle;          /* lift = 2: the first Newton solve of a load step starts from the
                              Galerkin projection onto the latest 2 recycle (<= 12) Newton
                              corrections of the load path (the A-norm-best start in their span;
@@ -370,7 +371,8 @@ int fm_coarse_moments(fm_ctx* ctx, int prec, int degree, const double* r, double
    of len; beta of the last iteration 0): the Lanczos coefficients from which
    the preconditioned operator's Ritz values follow (T_jj = 1/alpha_j +
    beta_(j-1)/alpha_(j-1), T_j,j+1 = sqrt(beta_j)/alpha_j).  At most 4096
-   iterations are recorded. */
+   iterations are recorded; len is 0 after an FM_MIXED_PC solve (its flexible
+   beta has no Lanczos relation). */
 int fm_pcg_history(fm_ctx* ctx, uint32_t cap, double* alpha, double* beta, uint32_t* len);
 /* sizes for roofline bookkeeping: n (rows), nnz, n_elems, n_nodes, n_colors */
 int fm_info(fm_ctx* ctx, uint64_t* out5);

@@ -404,3 +404,35 @@ def test_ritz_vectors_in_the_start(fm, S, O, prec):
     assert rel(runs[1][0], uo) < 1e-6
     assert runs[1][1][:3] == runs[0][1], (runs[0][1], runs[1][1])
     assert np.array_equal(runs[1][0], runs[2][0]) and runs[1][1] == runs[2][1], (runs[1][1], runs[2][1])
+
+
+def test_fp32_preconditioned_beam_with_coarse_level(fm, S, O):
+    """Configuration 5 form at 48x6x6 (FP64 Krylov vectors and SpMV, the FP64
+    ILU(0) factor rounded to FP32 as preconditioner, energy rule): with the
+    configuration's inner-solve options (extrapolated start, Galerkin start
+    over recycled corrections and the degree-4 polynomial fields, the same
+    fields as additive coarse level, whose r.(Z c) terms enter the flexible
+    beta) every step reaches ||R|| <= 1e-8 ||R_0|| with fewer PCG iterations
+    than the plain FP32-preconditioned solves, and both end at the FP64
+    oracle's state."""
+    cfg = fm.configs.make(5, dims=(48, 6, 6))
+    cfg["n_steps"] = 40
+    assert cfg["prec"] == fm.FM_MIXED_PC and cfg["inner_opts"]["coarse_additive"] == 1
+    fm.configs.load(S, cfg)
+    steps = range(1, 4)
+    P = oracle(O, cfg, pcg_tol=1e-12, pcg_max=6000, ls_mode=1, max_newton=60)
+    uo = np.zeros(cfg["X"].size)
+    for s in steps:
+        P.step(s, uo)
+    # the plain form: the configuration without its inner-solve options
+    plain = fm.configs.newton_cfg({**cfg, "inner_opts": {}}, inner_tol=1e-2)
+    pcg = {}
+    for name, c in (("coarse", fm.configs.newton_cfg(cfg)), ("plain", plain)):
+        S.set_u(None)
+        pcg[name] = 0
+        for s in steps:
+            rc, st = S.newton_step(s, c)
+            assert st["converged"] and st["rel_res"] <= 1e-8 and st["pcg_cap_hits"] == 0, (name, s, st)
+            pcg[name] += st["pcg_iters"]
+        assert rel(S.get_u(), uo) < 1e-6, name
+    assert pcg["coarse"] < 0.5 * pcg["plain"], pcg
