@@ -78,3 +78,26 @@ def test_config_sizes_and_boundary_data():
     beam = configs.make(5, dims=(16, 4, 4))
     f = beam["f_ext"].reshape(-1, 3)
     assert np.isclose(f[:, 2].sum(), -beam["traction"] * 1.0) and not f[:, :2].any()
+
+
+def test_configuration_inner_solve_options():
+    """newton_cfg applies a configuration's inner-solve options (config 5:
+    extrapolated start, Galerkin start, polynomial coarse level, lagged
+    factor) to its FP32-preconditioned and mixed forms only; an FP64 run
+    keeps the reference's settings."""
+    from flexmesh import configs
+    import flexmesh
+
+    beam = configs.make(5, dims=(16, 4, 4))
+    c = configs.newton_cfg(beam)
+    assert c.prec == flexmesh.FM_MIXED_PC and c.lift == 2 and c.coarse_degree == 4 and c.coarse_additive == 1
+    assert c.recycle == 6 and c.lag_precond == 1 and c.inner_tol == 1e-3 and c.inner_tol_max == 0.1
+    d = configs.newton_cfg(beam, prec=flexmesh.FM_FP64)
+    dflt = flexmesh.Solver.newton_cfg()
+    assert d.coarse_degree == 0 and d.recycle == 0 and d.lift == dflt.lift and d.inner_tol_max == dflt.inner_tol_max
+    # keyword overrides win over the options
+    e = configs.newton_cfg(beam, coarse_degree=2)
+    assert e.coarse_degree == 2 and e.coarse_additive == 1
+    # configurations without options are unchanged
+    cube = configs.make(3, dims=(4, 4, 4))
+    assert "inner_opts" not in cube and configs.newton_cfg(cube).coarse_degree == 0
