@@ -845,7 +845,7 @@ int pcg_run_pc(fm_ctx* ctx, Matrix& A, double tol, int max_it, PcgResult* res) {
                 // and of the flexible beta (r_new.(Z c_old))
                 ProfScope pu(ctx, FM_K_COARSE);
                 launch_pcg_xr_update_moments(s, PREC64, n, ad, p, Ap, x, r, ctx->cs_xn.p, ctx->ad_part.p, ctx->st.p);
-                launch_coarse_solve(s, ad, ctx->ad_part.p, ctx->ad_Einv.p, ctx->ad_coef.p, ctx->st.p);
+                launch_coarse_solve(s, ad, ctx->ad_part.p, ctx->ad_Einv.p, ctx->ad_coef.p, ctx->st.p, true);
             } else {
                 ProfScope pu(ctx, FM_K_PCG_OTHER);
                 launch_pcg_xr_update(s, PREC64, n, p, Ap, x, r, ctx->st.p);

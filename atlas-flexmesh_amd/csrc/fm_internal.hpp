@@ -444,7 +444,7 @@ void launch_coarse_correct(hipStream_t s, int prec, uint32_t n, int nmono, const
 void launch_pcg_xr_update_moments(hipStream_t s, int prec, uint32_t n, int nmono, const void* p, const void* Ap,
                                   void* x, void* r, const float4* xn, double* partials, const PcgState* st);
 void launch_coarse_solve(hipStream_t s, int nmono, const double* partials, const double* Einv, double* coef,
-                         PcgState* st);
+                         PcgState* st, bool cross = false);  // cross: also st->rz_cross (the flexible beta of pcg_run_pc)
 // block partials (coarse_moment_blocks() x 3 nmono) of the moments Z^T r alone
 void launch_coarse_moments(hipStream_t s, int prec, uint32_t n, int nmono, const float4* xn, const void* r,
                            double* partials, const PcgState* st);

@@ -2291,6 +2291,7 @@ __global__ __launch_bounds__(256) void pcg_xr_update_moments_kernel(uint32_t nn,
 // the same rounded to FP32 (the FP32 p update's).  Thread (s, i) takes
 // moment i of every S-th partial, sixteen loads in flight, then entries i of
 // every S-th row of E^{-1}; the S slices are added in slice order.
+template <bool CROSS>
 __global__ __launch_bounds__(1024) void coarse_solve_kernel(uint32_t nb, int m, const double* __restrict__ partials,
                                                             const double* __restrict__ Einv,
                                                             double* __restrict__ coef, PcgState* __restrict__ st) {
@@ -2329,20 +2330,20 @@ __global__ __launch_bounds__(1024) void coarse_solve_kernel(uint32_t nb, int m, 
     if (threadIdx.x < m) {
         double ci = 0.0;
         for (int q = 0; q < S; ++q) ci += s_part[q * mp + threadIdx.x];
-        // the previous coefficients against the new moments (the flexible
-        // beta's r_new.(Z c_old)); read before they are overwritten
-        go = coef[threadIdx.x] * s_g[threadIdx.x];
+        // CROSS: the previous coefficients against the new moments (the
+        // flexible beta's r_new.(Z c_old)); read before they are overwritten
+        if (CROSS) go = coef[threadIdx.x] * s_g[threadIdx.x];
         coef[threadIdx.x] = ci;
         reinterpret_cast<float*>(coef + m)[threadIdx.x] = (float)ci;
         gc = ci * s_g[threadIdx.x];
     }
     __syncthreads();  // block_sum's slots are its own; the barrier orders the reads above
     const double t = block_sum(gc);
-    __syncthreads();  // block_sum's shared scratch is reused
-    const double to = block_sum(go);
-    if (threadIdx.x == 0) {
-        st->rz_coarse = t;
-        st->rz_cross = to;
+    if (threadIdx.x == 0) st->rz_coarse = t;
+    if (CROSS) {
+        __syncthreads();  // block_sum's shared scratch is reused
+        const double to = block_sum(go);
+        if (threadIdx.x == 0) st->rz_cross = to;
     }
 }
 
@@ -3240,7 +3241,7 @@ static void coarse_correct_t(hipStream_t s, uint32_t n, int nmono, const float4*
         case 20: coarse_moments_kernel<T, 20><<<kMomentBlocks, 256, 0, s>>>(nn, xn, r, partials, st); break;
         default: coarse_moments_kernel<T, 35><<<kMomentBlocks, 256, 0, s>>>(nn, xn, r, partials, st); break;
     }
-    coarse_solve_kernel<<<1, 1024, 0, s>>>(kMomentBlocks, 3 * nmono, partials, Einv, coef, st);
+    coarse_solve_kernel<false><<<1, 1024, 0, s>>>(kMomentBlocks, 3 * nmono, partials, Einv, coef, st);
 }
 
 template <typename T>
@@ -3286,8 +3287,11 @@ void launch_pcg_xr_update_moments(hipStream_t s, int prec, uint32_t n, int nmono
 }
 
 void launch_coarse_solve(hipStream_t s, int nmono, const double* partials, const double* Einv, double* coef,
-                         PcgState* st) {
-    coarse_solve_kernel<<<1, 1024, 0, s>>>(kMomentBlocks, 3 * nmono, partials, Einv, coef, st);
+                         PcgState* st, bool cross) {
+    if (cross)
+        coarse_solve_kernel<true><<<1, 1024, 0, s>>>(kMomentBlocks, 3 * nmono, partials, Einv, coef, st);
+    else
+        coarse_solve_kernel<false><<<1, 1024, 0, s>>>(kMomentBlocks, 3 * nmono, partials, Einv, coef, st);
 }
 
 void launch_coarse_correct(hipStream_t s, int prec, uint32_t n, int nmono, const float4* xn, const void* r,
